@@ -1,0 +1,512 @@
+"""Parity of the hip variant with the oracle (the CPU restatement of the reference's scalar
+entries), through the C ABI, on seeded inputs.
+
+Modelled on the reference's tests/isa_variants.rs: row counts around vector and tile widths
+(0, 1, 3, 4, 7, 8, 15, 16, 17, 31, 32, 63, 64, 65, 300, 1000, 4500), views at offsets 1..7,
+float inputs mixing finite values, subnormals, +-0, +-inf and NaN, tiles cut at 64/384/992
+for reductions and 37/100 for selections and scans, one forced out-of-range index or key in
+some cases, and repeated keys for group_id. The bar is bit-exact results (a NaN result only
+needs to be NaN) and identical error strings.
+"""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [0, 1, 3, 4, 7, 8, 15, 16, 17, 31, 32, 63, 64, 65, 300, 1000, 4500]
+CMPS = ["eq", "ne", "lt", "le", "gt", "ge"]
+
+
+@pytest.fixture(scope="module")
+def D():
+    from verkehrt_amd import Registry
+    return Registry().resolve(0)
+
+
+def S(t, v):
+    from verkehrt_amd import Scalar
+    return Scalar(t, v)
+
+
+def fbits(v):
+    return struct.unpack("<Q", struct.pack("<d", float(v)))[0]
+
+
+def same_float(want, got):
+    if math.isnan(want):
+        return math.isnan(got)
+    return fbits(want) == fbits(got)
+
+
+def same_array(want, got):
+    want, got = np.asarray(want), np.asarray(got)
+    if want.shape != got.shape:
+        return False
+    if want.dtype.kind == "f":
+        wn, gn = np.isnan(want), np.isnan(got)
+        return bool(np.array_equal(wn, gn) and np.array_equal(want[~wn].view(np.uint64) if want.dtype == np.float64 else want[~wn],
+                                                               got[~gn].view(np.uint64) if got.dtype == np.float64 else got[~gn]))
+    return bool(np.array_equal(want, got))
+
+
+def floats(rng, n, special):
+    """isa_variants.rs:55-81: finite values in -1e3..1e3, wide normals/subnormals, specials."""
+    finite = rng.uniform(-1e3, 1e3, n)
+    wide = np.ldexp(rng.uniform(-1, 1, n), rng.integers(-1074, 1000, n))
+    if special:
+        specials = np.array([0.0, -0.0, np.inf, -np.inf, np.nan])
+        pick = rng.choice(3, n, p=[12 / 21, 4 / 21, 5 / 21])
+        sp = specials[rng.integers(0, 5, n)]
+    else:
+        specials = np.array([0.0, -0.0])
+        pick = rng.choice(3, n, p=[30 / 35, 4 / 35, 1 / 35])
+        sp = specials[rng.integers(0, 2, n)]
+    return np.where(pick == 0, finite, np.where(pick == 1, wide, sp))
+
+
+def view(D, host, offset, type=None):
+    """A device view of `host` starting `offset` rows into a larger buffer."""
+    host = np.asarray(host)
+    pad = np.concatenate([np.zeros(offset, host.dtype), host]) if offset else host
+    a = D.array(pad, type)
+    return a.slice(offset, host.size) if offset else a
+
+
+def err_of(f):
+    from verkehrt_amd import KernelError
+    try:
+        return ("ok", f())
+    except (KernelError, O.OracleError) as e:
+        return ("err", str(e))
+
+
+# ------------------------------------------------------------------ example
+
+def test_readme_example_plan(D):
+    # README.md:55-69: rand_f64 x2, dot_f64, count_gt_f64, length(filter_gt_f64)
+    x = D.call("rand_f64", [S("i64", 1_000_000), S("i64", 1)])
+    y = D.call("rand_f64", [S("i64", 1_000_000), S("i64", 2)])
+    assert np.array_equal(x.to_numpy().view(np.uint64), O.rand_f64(1_000_000, 1).view(np.uint64))
+    dot = D.call("dot_f64", [x, y])
+    assert repr(dot.value) == "249724.39002046228"
+    assert D.call("count_gt_f64", [x, S("f64", 0.5)]).value == 499521
+    big = D.call("filter_gt_f64", [x, S("f64", 0.99)])
+    assert len(big) == 9851 and big.nbytes == 78808
+    assert D.call("length_f64", [big]).value == 9851
+
+
+def test_ten_million_goldens(D):
+    n = 10_000_000
+    x = D.call("rand_f64", [S("i64", n), S("i64", 1)])
+    y = D.call("rand_f64", [S("i64", n), S("i64", 2)])
+    assert repr(D.call("reduce_add_f64", [x]).value) == "4998770.953740556"
+    assert repr(D.call("dot_f64", [x, y]).value) == "2499047.034215136"
+    assert D.call("argmin_f64", [x]).value == 5618431
+    assert repr(D.call("reduce_min_f64", [x]).value) == "2.5056049013372217e-08"
+    for c, want in [(0.99, 99695), (0.5, 4998931), (0.01, 9899130)]:
+        assert D.call("count_gt_f64", [x, S("f64", c)]).value == want
+        assert len(D.call("filter_gt_f64", [x, S("f64", c)])) == want
+
+
+def test_rand_i64(D):
+    r = D.call("rand_i64", [S("i64", 2), S("i64", 0)]).to_numpy()
+    assert list(r) == [-2152535657050944081, -7995527694508729151]
+    r = D.call("rand_i64", [S("i64", 5000), S("i64", -3)]).to_numpy()
+    assert np.array_equal(r, O.rand_i64(5000, -3))
+    e = err_of(lambda: D.call("rand_f64", [S("i64", -1), S("i64", 0)]))
+    assert e == ("err", "negative length -1")
+
+
+# ------------------------------------------------------- canonical reductions
+
+@pytest.mark.parametrize("special", [False, True])
+def test_canonical_float_reductions(D, special):
+    rng = np.random.default_rng(11 + special)
+    for n in LENGTHS:
+        for off in (0, 3):
+            x = floats(rng, n, special)
+            y = floats(rng, n, special)
+            dx, dy = view(D, x, off), view(D, y, (off + 2) % 8)
+            assert same_float(O.canonical_sum_f64(x), D.call("reduce_add_f64", [dx]).value), n
+            assert same_float(O.dot_f64(x, y), D.call("dot_f64", [dx, dy]).value), n
+            assert same_float(O.sumsq_f64(x), D.call("sumsq_f64", [dx]).value), n
+            c = float(x[n // 2]) if n else 0.5
+            for cmp in ("gt", "le", "ne"):
+                want = O.condsum_f64(x, cmp, c, -7.5)
+                got = D.call(f"condsum_{cmp}_f64", [dx, S("f64", c), S("f64", -7.5)]).value
+                assert same_float(want, got), (n, cmp)
+            z = 1.0 + rng.uniform(-1e-3, 1e-3, n)
+            assert same_float(O.canonical_product_f64(z), D.call("reduce_mul_f64", [view(D, z, off)]).value)
+
+
+def test_canonical_reducer_tiles(D):
+    # reduce.rs contract: tiles of multiples of 16 reproduce the one-shot bits
+    rng = np.random.default_rng(5)
+    x = floats(rng, 1000, False)
+    y = floats(rng, 1000, False)
+    dx, dy = D.array(x), D.array(y)
+    for name, args, want in [("reduce_add_f64", lambda a, b: [dx.slice(a, b - a)], O.canonical_sum_f64(x)),
+                             ("dot_f64", lambda a, b: [dx.slice(a, b - a), dy.slice(a, b - a)], O.dot_f64(x, y))]:
+        r = D.reducer(name)
+        for a, b in [(0, 64), (64, 384), (384, 992), (992, 1000)]:
+            r.update(args(a, b))
+        assert same_float(want, r.finish().value)
+    r = D.reducer("reduce_add_f64")
+    r.update([dx.slice(0, 17)])
+    e = err_of(lambda: r.update([dx.slice(17, 47)]))
+    assert e == ("err", "canonical update after a partial block")
+
+
+def test_golden_canonical_vector(D):
+    xs = [1e16, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.1, 1.2, 1.3, 1.4, 1.5, 1.6,
+          -1e16, 1.7, 1.8, 1.9, 2.1, 2.2, 2.3, 2.4, 2.5, 2.6, 2.7, 2.8, 2.9, 3.1, 3.2, 3.3,
+          0.01, 0.02, 0.03]
+    assert fbits(D.call("reduce_add_f64", [D.array(np.array(xs))]).value) == 0x4049147AE147AE14
+    assert fbits(D.call("reduce_add_f64", [D.array(np.zeros(0))]).value) == fbits(0.0)
+    assert D.call("reduce_mul_f64", [D.array(np.array([2.0, 3.0, 4.0]))]).value == 24.0
+
+
+# ------------------------------------------------------ streaming reductions
+
+def test_integer_sums_widen_and_wrap(D):
+    assert D.call("reduce_add_i32", [D.array(np.array([2**31 - 1, 1], np.int32))]) == S("i64", 2147483648)
+    assert D.call("reduce_add_u8", [D.array(np.array([200, 100], np.uint8))]) == S("u64", 300)
+    assert D.call("reduce_add_i64", [D.array(np.array([2**63 - 1, 1], np.int64))]) == S("i64", -(2**63))
+    rng = np.random.default_rng(3)
+    for n in LENGTHS + [1 << 20]:
+        for dt, t in [(np.int64, "i64"), (np.uint64, "u64"), (np.int32, "i32"), (np.uint8, "u8")]:
+            info = np.iinfo(dt)
+            x = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+            got = D.call(f"reduce_add_{t}", [view(D, x, n % 7)]).value
+            want = O.sum_int(x)
+            assert got == want, (t, n)
+
+
+def test_counts(D):
+    rng = np.random.default_rng(4)
+    for n in LENGTHS + [123457]:
+        x = floats(rng, n, True)
+        dx = view(D, x, n % 5)
+        for cmp in CMPS:
+            for c in [0.5, 0.0, float("nan"), float(x[0]) if n else 1.0]:
+                assert D.call(f"count_{cmp}_f64", [dx, S("f64", c)]).value == O.count_cmp(x, cmp, c)
+        xi = rng.integers(-5, 5, n).astype(np.int64)
+        for cmp in CMPS:
+            assert D.call(f"count_{cmp}_i64", [view(D, xi, 1), S("i64", 1)]).value == O.count_cmp(xi, cmp, 1)
+        m = (rng.random(n) < 0.3).astype(np.uint8)
+        assert D.call("count", [view(D, m, 2, "bool")]).value == O.count_bool(m)
+
+
+def test_extremes(D):
+    rng = np.random.default_rng(6)
+    for n in LENGTHS:
+        for special in (False, True):
+            x = floats(rng, n, special)
+            dx = view(D, x, n % 7)
+            for mx in (False, True):
+                name = "max" if mx else "min"
+                want = err_of(lambda: float(O.reduce_extreme(x, mx)))
+                got = err_of(lambda: D.call(f"reduce_{name}_f64", [dx]).value)
+                assert want[0] == got[0] and (want[0] == "err" and want == got or same_float(want[1], got[1])), (n, name)
+                want = err_of(lambda: O.arg_extreme(x, mx))
+                got = err_of(lambda: D.call(f"arg{name}_f64", [dx]).value)
+                assert want == got, (n, name, special)
+        xi = rng.integers(-3, 3, n).astype(np.int64)
+        for mx in (False, True):
+            name = "max" if mx else "min"
+            assert err_of(lambda: O.arg_extreme(xi, mx)) == err_of(lambda: D.call(f"arg{name}_i64", [view(D, xi, 1)]).value)
+    # tie and NaN rules, src/kernels/tests.rs:874-891
+    a = lambda name, v: D.call(name, [D.array(np.array(v, np.float64))]).value
+    assert a("argmin_f64", [3.0, 1.0, 1.0]) == 1
+    assert a("argmin_f64", [3.0, np.nan, np.nan]) == 1
+    assert a("argmin_f64", [0.0, -0.0]) == 1
+    assert a("argmax_f64", [-0.0, 0.0]) == 1
+    assert a("argmax_f64", [1.0, np.nan, 5.0]) == 1
+    assert fbits(a("reduce_min_f64", [0.0, -0.0])) == fbits(-0.0)
+    assert fbits(a("reduce_max_f64", [-0.0, 0.0])) == fbits(0.0)
+
+
+def test_arg_extreme_tiles_count_rows(D):
+    rng = np.random.default_rng(7)
+    x = floats(rng, 1000, False)
+    dx = D.array(x)
+    for mx in (False, True):
+        r = D.reducer("argmax_f64" if mx else "argmin_f64")
+        for a, b in [(0, 64), (64, 384), (384, 992), (992, 1000)]:
+            r.update([dx.slice(a, b - a)])
+        assert r.finish().value == O.arg_extreme(x, mx)
+
+
+def test_bool_and_bit_reductions(D):
+    b = lambda name, v: D.call(name, [D.array(np.array(v, np.uint8), "bool")]).value
+    assert b("any", []) is False and b("all", []) is True
+    assert b("any", [0, 1]) is True and b("all", [1, 0]) is False
+    assert b("reduce_xor_bool", [1, 1]) is False and b("reduce_xor_bool", [1, 1, 1]) is True
+    rng = np.random.default_rng(8)
+    x = rng.integers(0, 2**64, 3000, dtype=np.uint64)
+    assert D.call("reduce_xor_u64", [D.array(x)]).value == int(np.bitwise_xor.reduce(x))
+    assert D.call("reduce_and_u64", [D.array(np.zeros(0, np.uint64))]).value == 2**64 - 1
+
+
+# ----------------------------------------------------------------- selections
+
+@pytest.mark.parametrize("t,dt", [("f64", np.float64), ("i64", np.int64), ("u64", np.uint64),
+                                  ("i32", np.int32), ("f32", np.float32), ("u8", np.uint8)])
+def test_compress(D, t, dt):
+    rng = np.random.default_rng(9)
+    for n in LENGTHS + [100_000]:
+        x = rng.integers(0, 200, n).astype(dt)
+        for density in (0.0, 0.3, 1.0):
+            m = (rng.random(n) < density).astype(np.uint8)
+            got = D.call(f"compress_{t}", [view(D, x, n % 7), view(D, m, (n + 3) % 7, "bool")]).to_numpy()
+            assert np.array_equal(got, O.compress(x, m)), (n, density)
+    e = err_of(lambda: D.call(f"compress_{t}", [D.array(np.zeros(2, dt)), D.array(np.ones(1, np.uint8), "bool")]))
+    assert e == ("err", "length mismatch: 2 vs 1")
+
+
+def test_compress_bool_and_where(D):
+    got = D.call("compress_bool", [D.array(np.array([1, 0, 1], np.uint8), "bool"),
+                                   D.array(np.array([1, 1, 0], np.uint8), "bool")])
+    assert got.type == "bool" and list(got.to_numpy()) == [1, 0]
+    assert list(D.call("where", [D.array(np.array([0, 1, 1], np.uint8), "bool")]).to_numpy()) == [1, 2]
+    rng = np.random.default_rng(10)
+    for n in LENGTHS + [70_000]:
+        m = (rng.random(n) < 0.4).astype(np.uint8)
+        assert np.array_equal(D.call("where", [view(D, m, 5, "bool")]).to_numpy(), O.where(m))
+
+
+def test_filters(D):
+    rng = np.random.default_rng(12)
+    for n in LENGTHS + [200_003]:
+        x = floats(rng, n, True)
+        dx = view(D, x, n % 7)
+        for cmp in CMPS:
+            for c in [0.5, float("nan"), float(x[n // 3]) if n else 0.0]:
+                got = D.call(f"filter_{cmp}_f64", [dx, S("f64", c)]).to_numpy()
+                assert same_array(O.filter_cmp(x, cmp, c), got), (n, cmp, c)
+        xi = rng.integers(-10, 10, n).astype(np.int64)
+        got = D.call("filter_le_i64", [view(D, xi, 2), S("i64", 3)]).to_numpy()
+        assert np.array_equal(got, O.filter_cmp(xi, "le", 3))
+
+
+def test_selection_tiles(D):
+    # the output of a selection does not depend on the tile split, and `where` keeps global rows
+    rng = np.random.default_rng(13)
+    n = 4500
+    x = rng.random(n)
+    m = (rng.random(n) < 0.5).astype(np.uint8)
+    dx, dm = D.array(x), D.array(m, "bool")
+    for name, args, want in [("filter_gt_f64", lambda a, b: [dx.slice(a, b - a), S("f64", 0.3)], O.filter_cmp(x, "gt", 0.3)),
+                             ("compress_f64", lambda a, b: [dx.slice(a, b - a), dm.slice(a, b - a)], O.compress(x, m)),
+                             ("where", lambda a, b: [dm.slice(a, b - a)], O.where(m))]:
+        out = D.builder("i64" if name == "where" else "f64")
+        for a, b in [(0, 37), (37, 100), (100, n)]:
+            D.select_tile(name, args(a, b), a, out)
+        assert np.array_equal(out.finish().to_numpy(), want), name
+
+
+# ----------------------------------------------------------------------- scans
+
+@pytest.mark.parametrize("t,dt", [("i64", np.int64), ("u64", np.uint64), ("i32", np.int32), ("u8", np.uint8)])
+def test_scan_add(D, t, dt):
+    rng = np.random.default_rng(14)
+    info = np.iinfo(dt)
+    for n in LENGTHS + [1_000_003]:
+        x = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+        got = D.call(f"scan_add_{t}", [view(D, x, n % 7)]).to_numpy()
+        want = np.cumsum(x.astype(np.int64 if dt in (np.int64, np.int32) else np.uint64))
+        assert np.array_equal(got, want), (t, n)
+        if t == "i64":
+            assert np.array_equal(got, O.scan_add_i64(x))
+            s = (rng.random(n) < 1 / 16).astype(np.uint8)
+            got = D.call("scan_by_add_i64", [view(D, x, 1), view(D, s, 2, "bool")]).to_numpy()
+            assert np.array_equal(got, O.scan_by_add_i64(x, s)), n
+
+
+def test_scan_goldens_and_tiles(D):
+    # src/kernels/tests.rs:1416-1490
+    sc = lambda name, *a: D.call(name, list(a)).to_numpy()
+    assert list(sc("scan_add_i64", D.array(np.array([1, 2, 3, 4], np.int64)))) == [1, 3, 6, 10]
+    assert list(sc("scan_by_add_i64", D.array(np.array([1, 2, 3, 4, 5], np.int64)),
+                   D.array(np.array([0, 0, 1, 0, 1], np.uint8), "bool"))) == [1, 3, 3, 7, 5]
+    assert list(sc("scan_by_add_i64", D.array(np.array([1, 2], np.int64)),
+                   D.array(np.array([1, 0], np.uint8), "bool"))) == [1, 3]
+    nan = sc("scan_max_f64", D.array(np.array([1.0, np.nan, 3.0])))
+    assert nan[0] == 1.0 and np.isnan(nan[1]) and np.isnan(nan[2])
+    z = sc("scan_max_f64", D.array(np.array([-0.0, 0.0])))
+    assert [fbits(v) for v in z] == [fbits(-0.0), fbits(0.0)]
+    rng = np.random.default_rng(15)
+    n = 20_000
+    x = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    s = (rng.random(n) < 0.05).astype(np.uint8)
+    dx, ds = D.array(x), D.array(s, "bool")
+    from verkehrt_amd import Array
+    for name, want, by in [("scan_add_i64", O.scan_add_i64(x), False), ("scan_by_add_i64", O.scan_by_add_i64(x, s), True)]:
+        out = Array.empty(D.ctx, "i64", n)
+        sc_ = D.scanner(name)
+        for a, b in [(0, 37), (37, 100), (100, 4500), (4500, n)]:
+            args = [dx.slice(a, b - a)] + ([ds.slice(a, b - a)] if by else [])
+            sc_.update(args, out.slice(a, b - a))
+        assert np.array_equal(out.to_numpy(), want), name
+    f = floats(rng, 50_000, True)
+    for mx in (False, True):
+        got = D.call("scan_max_f64" if mx else "scan_min_f64", [D.array(f)]).to_numpy()
+        assert same_array(O.scan_extreme(f, mx), got)
+
+
+def test_scan_add_f64_defined_order(D):
+    rng = np.random.default_rng(16)
+    for n in LENGTHS:
+        x = floats(rng, n, False)
+        assert same_array(O.scan_add_f64(x), D.call("scan_add_f64", [view(D, x, 1)]).to_numpy()), n
+
+
+# ---------------------------------------------------------------------- gather
+
+@pytest.mark.parametrize("t,dt", [("f64", np.float64), ("i64", np.int64), ("i32", np.int32), ("u8", np.uint8)])
+def test_gather(D, t, dt):
+    rng = np.random.default_rng(17)
+    for n in LENGTHS + [300_000]:
+        xs = rng.integers(0, 100, max(n, 1)).astype(dt)
+        idx = rng.integers(0, xs.size, n).astype(np.int64)
+        if n and n % 3 == 0:  # one forced bad index, isa_variants.rs:173-190
+            idx[rng.integers(0, n)] = xs.size if n % 2 else -1
+        want = err_of(lambda: O.gather(xs, idx))
+        got = err_of(lambda: D.call(f"gather_{t}", [view(D, xs, 3), view(D, idx, 1)]).to_numpy())
+        assert want[0] == got[0], n
+        if want[0] == "err":
+            assert want == got
+        else:
+            assert np.array_equal(want[1], got[1])
+    e = err_of(lambda: D.call(f"gather_{t}", [D.array(np.array([10, 20, 30], dt)), D.array(np.array([0, 7, -1, 9], np.int64))]))
+    assert e == ("err", "gather index 7 out of range for length 3")
+    e = err_of(lambda: D.call(f"gather_{t}", [D.array(np.zeros(0, dt)), D.array(np.array([0], np.int64))]))
+    assert e == ("err", "gather index 0 out of range for length 0")
+
+
+# --------------------------------------------------------------------- buckets
+
+def test_histogram_and_reduce_into(D):
+    I = lambda v: D.array(np.array(v, np.int64))
+    assert list(D.call("histogram", [I([0, 2, 2, 3]), S("i64", 5)]).to_numpy()) == [1, 0, 2, 1, 0]
+    assert len(D.call("histogram", [I([]), S("i64", 0)])) == 0
+    for key, nb in [(3, 3), (-1, 3), (0, 0)]:
+        first = 0 if nb == 0 else key
+        want = f"bucket key {first} out of range for {nb} buckets"
+        assert err_of(lambda: D.call("histogram", [I([0, key]), S("i64", nb)])) == ("err", want)
+        assert err_of(lambda: D.call("reduce_into_add_i64", [I([0, key]), I([1, 1]), S("i64", nb)])) == ("err", want)
+    assert err_of(lambda: D.call("histogram", [I([]), S("i64", -1)])) == ("err", "negative bucket count -1")
+    assert err_of(lambda: D.call("reduce_into_add_i64", [I([0, 0]), I([1]), S("i64", 1)])) == \
+        ("err", "length mismatch: 2 keys for 1 values")
+    keys = [1, 0, 1, 3, 1]
+    assert list(D.call("reduce_into_add_i64", [I(keys), I([5, 7, 9, 2, 1]), S("i64", 4)]).to_numpy()) == [7, 15, 0, 2]
+    mn = D.call("reduce_into_min_f64", [I(keys), D.array(np.array([1.5, 2.5, 0.5, 9.0, 4.0])), S("i64", 4)]).to_numpy()
+    assert list(mn) == [2.5, 0.5, math.inf, 9.0]
+    rng = np.random.default_rng(18)
+    for n in LENGTHS + [500_000]:
+        for nb in (1, 7, 64, 5000):
+            k = rng.integers(0, nb, n).astype(np.int64)
+            v = rng.integers(-2**40, 2**40, n).astype(np.int64)
+            assert np.array_equal(D.call("histogram", [view(D, k, 1), S("i64", nb)]).to_numpy(), O.histogram(k, nb))
+            assert np.array_equal(D.call("reduce_into_add_i64", [view(D, k, 2), view(D, v, 3), S("i64", nb)]).to_numpy(),
+                                  O.reduce_into(k, v, nb))
+            f = floats(rng, n, True)
+            for op in ("min", "max"):
+                got = D.call(f"reduce_into_{op}_f64", [view(D, k, 1), D.array(f), S("i64", nb)]).to_numpy()
+                assert same_array(O.reduce_into(k, f, nb, op), got)
+
+
+def test_reduce_into_add_f64_defined_order(D):
+    rng = np.random.default_rng(19)
+    for n in LENGTHS + [100_000]:
+        for nb in (1, 5, 300):
+            k = rng.integers(0, nb, n).astype(np.int64)
+            v = floats(rng, n, False)
+            got = D.call("reduce_into_add_f64", [view(D, k, 1), view(D, v, 2), S("i64", nb)]).to_numpy()
+            assert same_array(O.reduce_into(k, v, nb), got), (n, nb)
+    # every key equal gives sum's bits
+    v = floats(rng, 10_007, False)
+    got = D.call("reduce_into_add_f64", [D.array(np.zeros(v.size, np.int64)), D.array(v), S("i64", 1)]).to_numpy()
+    assert fbits(got[0]) == fbits(O.canonical_sum_f64(v))
+
+
+# --------------------------------------------------------------------- argsort
+
+@pytest.mark.parametrize("t,dt", [("u64", np.uint64), ("i64", np.int64), ("i32", np.int32), ("u8", np.uint8)])
+def test_argsort(D, t, dt):
+    rng = np.random.default_rng(20)
+    info = np.iinfo(dt)
+    for n in LENGTHS + [5000, 123_457]:
+        for x in [rng.integers(info.min, info.max, n, dtype=dt, endpoint=True),
+                  rng.integers(0, 3, n).astype(dt)]:
+            got = D.call(f"argsort_{t}", [view(D, x, n % 7)]).to_numpy()
+            assert np.array_equal(got, O.argsort(x)), (t, n)
+    assert list(D.call("argsort_i64", [D.array(np.array([3, 1, 3, 1], np.int64))]).to_numpy()) == [1, 3, 0, 2]
+    assert list(D.call("argsort_u8", [D.array(np.array([200, 5, 200, 5, 0], np.uint8))]).to_numpy()) == [4, 1, 3, 0, 2]
+
+
+def test_argsort_f64_defined_order(D):
+    rng = np.random.default_rng(21)
+    for n in LENGTHS + [50_000]:
+        x = floats(rng, n, True)
+        got = D.call("argsort_f64", [view(D, x, 1)]).to_numpy()
+        assert np.array_equal(got, O.argsort(x)), n
+
+
+# -------------------------------------------------------------------- group_id
+
+def test_group_id(D):
+    assert list(D.call("group_id_i64", [D.array(np.array([7, -1, 7, 3, -1, 7], np.int64))]).to_numpy()) == [0, 1, 0, 2, 1, 0]
+    assert list(D.call("group_id2_i64_i64", [D.array(np.array([1, 2, 1, 2], np.int64)),
+                                             D.array(np.array([2, 1, 2, 2], np.int64))]).to_numpy()) == [0, 1, 0, 2]
+    assert list(D.call("group_id2_u8_i32", [D.array(np.array([5, 5, 9], np.uint8)),
+                                            D.array(np.array([-1, -1, -1], np.int32))]).to_numpy()) == [0, 0, 1]
+    rng = np.random.default_rng(22)
+    for n in LENGTHS + [200_000]:
+        for d in (1, 3, 64, max(n, 1)):
+            base = rng.integers(-2**63, 2**63 - 1, d, dtype=np.int64)
+            x = base[(np.arange(n) * 0x9E3779B9) % d]
+            assert np.array_equal(D.call("group_id_i64", [view(D, x, n % 7)]).to_numpy(), O.group_id(x)), (n, d)
+            y = rng.integers(0, 2, n).astype(np.int32)
+            assert np.array_equal(D.call("group_id2_i64_i32", [view(D, x, 1), view(D, y, 2)]).to_numpy(),
+                                  O.group_id(x, y)), (n, d)
+        xu = rng.integers(0, 256, n).astype(np.uint8)
+        assert np.array_equal(D.call("group_id_u8", [view(D, xu, 3)]).to_numpy(), O.group_id(xu))
+
+
+# ----------------------------------------------------------------- elementwise
+
+def test_elementwise(D):
+    rng = np.random.default_rng(23)
+    x, y = floats(rng, 1000, True), floats(rng, 1000, True)
+    dx, dy = D.array(x), D.array(y)
+    with np.errstate(invalid="ignore"):
+        for cmp, f in [("gt", np.greater), ("lt", np.less), ("eq", np.equal), ("ne", np.not_equal),
+                       ("ge", np.greater_equal), ("le", np.less_equal)]:
+            assert np.array_equal(D.call(f"{cmp}_f64", [dx, dy]).to_numpy(), f(x, y).astype(np.uint8))
+            assert np.array_equal(D.call(f"{cmp}_f64", [dx, S("f64", 0.5)]).to_numpy(), f(x, 0.5).astype(np.uint8))
+        assert same_array(x * y, D.call("mul_f64", [dx, dy]).to_numpy())
+        assert same_array(x + 2.0, D.call("add_f64", [dx, S("f64", 2.0)]).to_numpy())
+    xi = rng.integers(-2**63, 2**63 - 1, 1000, dtype=np.int64)
+    assert np.array_equal(D.call("add_i64", [D.array(xi), D.array(xi)]).to_numpy(), xi + xi)
+    b = (rng.random(1000) < 0.5).astype(np.uint8)
+    assert np.array_equal(D.call("cast_bool_i64", [D.array(b, "bool")]).to_numpy(), b.astype(np.int64))
+    assert np.array_equal(D.call("not_bool", [D.array(b, "bool")]).to_numpy(), 1 - b)
+
+
+def test_errors_as_strings(D):
+    e = err_of(lambda: D.call("dot_f64", [D.array(np.zeros(3)), D.array(np.zeros(2))]))
+    assert e == ("err", "length mismatch: 3 vs 2")
+    e = err_of(lambda: D.call("reduce_add_f64", [D.array(np.zeros(3, np.int64))]))
+    assert e == ("err", "expected f64[], got i64[]")
+    e = err_of(lambda: D.call("count_gt_f64", [D.array(np.zeros(3)), S("i64", 1)]))
+    assert e == ("err", "expected f64, got i64")
+    e = err_of(lambda: D.call("argmin_f64", [D.array(np.zeros(0))]))
+    assert e == ("err", "argmin of an empty array")
+    e = err_of(lambda: D.call("reduce_max_i64", [D.array(np.zeros(0, np.int64))]))
+    assert e == ("err", "reduce_max of an empty array")

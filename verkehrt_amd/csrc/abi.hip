@@ -1,0 +1,971 @@
+// The C ABI of the hip variant (include/vk_hip.h): the kernel table by reference name, the
+// reference's argument checks and error texts, reducer / scanner / builder states on the
+// device, and the trace lines of src/exec.rs:390-420.
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vk_internal.h"
+
+namespace vk {
+Ctx *create_ctx(int device);
+void destroy_ctx(Ctx *c);
+
+namespace {
+
+// ------------------------------------------------------------------ the table
+
+enum InKind { IN_ARRAY, IN_SCALAR, IN_EITHER };
+struct In {
+  InKind k;
+  int32_t t;
+};
+
+enum Impl {
+  R_CANON, R_SUM_INT, R_COUNT_CMP, R_COUNT_BOOL, R_EXTREME, R_ARG, R_BITS,
+  S_COMPRESS, S_FILTER, S_WHERE,
+  SC_SCAN, SC_FSCAN,
+  W_GATHER, W_ARGSORT, W_GROUP_ID, W_GROUP_ID2, W_HISTOGRAM, W_REDUCE_INTO, W_RAND_F64, W_RAND_I64,
+  W_IOTA, W_LENGTH, W_FILL,
+  E_BINARY, E_UNARY
+};
+
+struct Def {
+  std::string name;
+  int cls;
+  std::vector<In> in;
+  bool out_array;
+  int32_t out_t;
+  int impl;
+  int p0 = 0, p1 = 0;  // family parameters (cmp, op, kind, max flag, ...)
+  int32_t t0 = 0, t1 = 0;  // element types the family works on
+};
+
+std::vector<Def> g_defs;
+std::unordered_map<std::string, int> g_by_name;
+
+const char *kCmp[6] = {"eq", "ne", "lt", "le", "gt", "ge"};
+
+void add(Def d) {
+  g_by_name[d.name] = static_cast<int>(g_defs.size());
+  g_defs.push_back(std::move(d));
+}
+
+std::string tn(int32_t t) { return type_name(t); }
+
+void build_table() {
+  if (!g_defs.empty()) return;
+  const int32_t ints[] = {VK_I64, VK_U64, VK_I32, VK_U8};
+  const int32_t nums[] = {VK_F64, VK_F32, VK_I64, VK_U64, VK_I32, VK_U8};
+  const int32_t all[] = {VK_F64, VK_F32, VK_I64, VK_U64, VK_I32, VK_U8, VK_BOOL};
+  auto A = [](int32_t t) { return In{IN_ARRAY, t}; };
+  auto S = [](int32_t t) { return In{IN_SCALAR, t}; };
+  auto E = [](int32_t t) { return In{IN_EITHER, t}; };
+  auto sum_t = [](int32_t t) { return t == VK_I32 ? VK_I64 : t == VK_U8 ? VK_U64 : t; };
+
+  // reductions (src/kernels/scalar/reduce.rs:680-726)
+  add({"reduce_add_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 0});
+  add({"reduce_mul_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 4});
+  for (int32_t t : ints)
+    add({"reduce_add_" + tn(t), VK_REDUCTION, {A(t)}, false, sum_t(t), R_SUM_INT, 0, 0, t});
+  for (int32_t t : nums) {
+    add({"reduce_min_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_EXTREME, 0, 0, t});
+    add({"reduce_max_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_EXTREME, 1, 0, t});
+    add({"argmin_" + tn(t), VK_REDUCTION, {A(t)}, false, VK_I64, R_ARG, 0, 0, t});
+    add({"argmax_" + tn(t), VK_REDUCTION, {A(t)}, false, VK_I64, R_ARG, 1, 0, t});
+  }
+  const char *bitn[3] = {"and", "or", "xor"};
+  for (int32_t t : ints)
+    for (int b = 0; b < 3; ++b)
+      add({std::string("reduce_") + bitn[b] + "_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_BITS, b, 0, t});
+  for (int b = 0; b < 3; ++b)
+    add({std::string("reduce_") + bitn[b] + "_bool", VK_REDUCTION, {A(VK_BOOL)}, false, VK_BOOL, R_BITS, b, 0, VK_BOOL});
+  add({"count", VK_REDUCTION, {A(VK_BOOL)}, false, VK_I64, R_COUNT_BOOL});
+  add({"any", VK_REDUCTION, {A(VK_BOOL)}, false, VK_BOOL, R_BITS, 1, 0, VK_BOOL});
+  add({"all", VK_REDUCTION, {A(VK_BOOL)}, false, VK_BOOL, R_BITS, 0, 0, VK_BOOL});
+  // fused kernels (src/kernels/scalar/fused.rs:248-340)
+  add({"dot_f64", VK_REDUCTION, {A(VK_F64), A(VK_F64)}, false, VK_F64, R_CANON, 1});
+  add({"sumsq_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 2});
+  for (int c = 0; c < 6; ++c) {
+    add({std::string("condsum_") + kCmp[c] + "_f64", VK_REDUCTION, {A(VK_F64), S(VK_F64), S(VK_F64)},
+         false, VK_F64, R_CANON, 3, c});
+    for (int32_t t : {VK_F64, VK_F32, VK_I64, VK_U64, VK_I32}) {
+      add({std::string("count_") + kCmp[c] + "_" + tn(t), VK_REDUCTION, {A(t), S(t)}, false, VK_I64,
+           R_COUNT_CMP, c, 0, t});
+      add({std::string("filter_") + kCmp[c] + "_" + tn(t), VK_SELECTION, {A(t), S(t)}, true, t,
+           S_FILTER, c, 0, t});
+    }
+  }
+  // selections (select.rs:229-257)
+  for (int32_t t : all)
+    add({"compress_" + tn(t), VK_SELECTION, {A(t), A(VK_BOOL)}, true, t, S_COMPRESS, 0, 0, t});
+  add({"where", VK_SELECTION, {A(VK_BOOL)}, true, VK_I64, S_WHERE});
+  // scans (scan.rs:386-439)
+  for (int32_t t : ints) {
+    add({"scan_add_" + tn(t), VK_SCAN, {A(t)}, true, sum_t(t), SC_SCAN, 0, 0, t});
+    add({"scan_by_add_" + tn(t), VK_SCAN, {A(t), A(VK_BOOL)}, true, sum_t(t), SC_SCAN, 0, 1, t});
+  }
+  for (int32_t t : {VK_F64, VK_I64, VK_U64}) {
+    add({"scan_min_" + tn(t), VK_SCAN, {A(t)}, true, t, SC_SCAN, 1, 0, t});
+    add({"scan_max_" + tn(t), VK_SCAN, {A(t)}, true, t, SC_SCAN, 2, 0, t});
+    add({"scan_by_min_" + tn(t), VK_SCAN, {A(t), A(VK_BOOL)}, true, t, SC_SCAN, 1, 1, t});
+    add({"scan_by_max_" + tn(t), VK_SCAN, {A(t), A(VK_BOOL)}, true, t, SC_SCAN, 2, 1, t});
+  }
+  // Not a reference kernel (builtins.dd:50-54): the order defined by this backend.
+  add({"scan_add_f64", VK_SCAN, {A(VK_F64)}, true, VK_F64, SC_FSCAN});
+  // whole kernels (index.rs:277-326, group.rs:330-365, construct.rs:200-272)
+  for (int32_t t : all)
+    add({"gather_" + tn(t), VK_WHOLE, {A(t), A(VK_I64)}, true, t, W_GATHER, 0, 0, t});
+  for (int32_t t : {VK_I64, VK_U64, VK_I32, VK_U8, VK_F64})
+    add({"argsort_" + tn(t), VK_WHOLE, {A(t)}, true, VK_I64, W_ARGSORT, 0, 0, t});
+  for (int32_t a : ints) {
+    add({"group_id_" + tn(a), VK_WHOLE, {A(a)}, true, VK_I64, W_GROUP_ID, 0, 0, a});
+    for (int32_t b : ints)
+      add({"group_id2_" + tn(a) + "_" + tn(b), VK_WHOLE, {A(a), A(b)}, true, VK_I64, W_GROUP_ID2, 0, 0, a, b});
+  }
+  add({"histogram", VK_WHOLE, {A(VK_I64), S(VK_I64)}, true, VK_I64, W_HISTOGRAM});
+  add({"reduce_into_add_i64", VK_WHOLE, {A(VK_I64), A(VK_I64), S(VK_I64)}, true, VK_I64, W_REDUCE_INTO, 1, 0, VK_I64});
+  // Not a reference kernel (builtins.dd:67-71): the order defined by this backend.
+  add({"reduce_into_add_f64", VK_WHOLE, {A(VK_I64), A(VK_F64), S(VK_I64)}, true, VK_F64, W_REDUCE_INTO, 4, 0, VK_F64});
+  for (int32_t t : {VK_F64, VK_I64}) {
+    add({"reduce_into_min_" + tn(t), VK_WHOLE, {A(VK_I64), A(t), S(VK_I64)}, true, t, W_REDUCE_INTO, 2, 0, t});
+    add({"reduce_into_max_" + tn(t), VK_WHOLE, {A(VK_I64), A(t), S(VK_I64)}, true, t, W_REDUCE_INTO, 3, 0, t});
+  }
+  add({"rand_f64", VK_WHOLE, {S(VK_I64), S(VK_I64)}, true, VK_F64, W_RAND_F64});
+  add({"rand_i64", VK_WHOLE, {S(VK_I64), S(VK_I64)}, true, VK_I64, W_RAND_I64});
+  add({"iota", VK_WHOLE, {S(VK_I64)}, true, VK_I64, W_IOTA});
+  for (int32_t t : all) {
+    add({"length_" + tn(t), VK_WHOLE, {A(t)}, false, VK_I64, W_LENGTH, 0, 0, t});
+    add({"fill_" + tn(t), VK_WHOLE, {S(VK_I64), S(t)}, true, t, W_FILL, 0, 0, t});
+  }
+  // elementwise (elementwise.rs:740-912): the callers' glue of the hot path
+  const char *arith[5] = {"add", "sub", "mul", "min", "max"};
+  const int arith_op[5] = {0, 1, 2, 14, 15};
+  for (int32_t t : {VK_F64, VK_I64}) {
+    for (int k = 0; k < 5; ++k)
+      add({std::string(arith[k]) + "_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, t, E_BINARY, arith_op[k], 0, t});
+    for (int c = 0; c < 6; ++c)
+      add({std::string(kCmp[c]) + "_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, VK_BOOL, E_BINARY, 3 + c, 0, t});
+  }
+  add({"and_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 9, 0, VK_BOOL});
+  add({"or_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 10, 0, VK_BOOL});
+  add({"xor_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 11, 0, VK_BOOL});
+  add({"eq_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 3, 0, VK_BOOL});
+  add({"ne_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 4, 0, VK_BOOL});
+  add({"not_bool", VK_ELEMENTWISE, {E(VK_BOOL)}, true, VK_BOOL, E_UNARY, 12, 0, VK_BOOL});
+  add({"cast_bool_i64", VK_ELEMENTWISE, {E(VK_BOOL)}, true, VK_I64, E_UNARY, 13, 0, VK_BOOL});
+  add({"cast_i32_i64", VK_ELEMENTWISE, {E(VK_I32)}, true, VK_I64, E_UNARY, 13, 0, VK_I32});
+  add({"cast_u8_i64", VK_ELEMENTWISE, {E(VK_U8)}, true, VK_I64, E_UNARY, 13, 0, VK_U8});
+  add({"cast_i64_f64", VK_ELEMENTWISE, {E(VK_I64)}, true, VK_F64, E_UNARY, 16, 0, VK_I64});
+}
+
+const Def &def_of(const char *name) {
+  build_table();
+  auto it = g_by_name.find(name ? name : "");
+  if (it == g_by_name.end()) usage_error(std::string("no hip kernel named ") + (name ? name : "(null)"));
+  return g_defs[it->second];
+}
+
+// ------------------------------------------------------------ argument checks
+
+std::string arg_ty(const vk_arg &a) {
+  if (a.kind == VK_ROWS) return a.rows ? tn(a.rows->type) + "[]" : std::string("?[]");
+  return tn(a.splat.type);
+}
+
+// arity_error, elementwise.rs:215-217
+[[noreturn]] void arity(size_t want, size_t got) {
+  usage_error("expected " + std::to_string(want) + " arguments, got " + std::to_string(got));
+}
+
+// kind_error, dispatch.rs:323-330
+[[noreturn]] void kind_error(const In &s, const vk_arg &a) {
+  const std::string t = tn(s.t), got = arg_ty(a);
+  if (s.k == IN_ARRAY) usage_error("expected " + t + "[], got " + got);
+  if (s.k == IN_SCALAR) usage_error("expected " + t + ", got " + got);
+  usage_error("expected " + t + " or " + t + "[], got " + got);
+}
+
+bool accepts(const In &s, const vk_arg &a) {
+  if (a.kind == VK_ROWS) return a.rows && (s.k == IN_ARRAY || s.k == IN_EITHER) && a.rows->type == s.t;
+  return (s.k == IN_SCALAR || s.k == IN_EITHER) && a.splat.type == s.t;
+}
+
+// prepare / check_tile_args (dispatch.rs:268-321): arity, kinds, one row count.
+// Returns the common row count, or -1 when every argument is a scalar.
+int64_t check_args(const Def &d, const vk_arg *args, size_t nargs) {
+  if (nargs != d.in.size()) arity(d.in.size(), nargs);
+  int64_t rows = -1;
+  for (size_t i = 0; i < nargs; ++i) {
+    if (!accepts(d.in[i], args[i])) kind_error(d.in[i], args[i]);
+    if (args[i].kind == VK_ROWS) {
+      const int64_t n = static_cast<int64_t>(args[i].rows->len);
+      if (rows >= 0 && rows != n)
+        usage_error("length mismatch: " + std::to_string(rows) + " vs " + std::to_string(n));
+      rows = n;
+    }
+  }
+  return rows;
+}
+
+// rows_arg / splat_arg (elementwise.rs:178-213), for entries that check their own arguments.
+const vk_array *rows_arg(const vk_arg &a, int32_t ty) {
+  if (a.kind == VK_ROWS && a.rows && a.rows->type == ty) return a.rows;
+  if (a.kind == VK_ROWS) usage_error("expected " + tn(ty) + "[], got " + arg_ty(a));
+  usage_error("expected " + tn(ty) + "[], got " + tn(a.splat.type));
+}
+uint64_t splat_arg(const vk_arg &a, int32_t ty) {
+  if (a.kind == VK_SPLAT && a.splat.type == ty) return a.splat.bits;
+  usage_error("expected " + tn(ty) + ", got " + arg_ty(a));
+}
+// i64_arg (construct.rs:23-28) for whole kernels
+int64_t i64_arg(const vk_arg &a) {
+  if (a.kind == VK_SPLAT && a.splat.type == VK_I64) return static_cast<int64_t>(a.splat.bits);
+  usage_error("expected i64, got " + arg_ty(a));
+}
+const vk_array *array_arg(const vk_arg &a, int32_t ty) {
+  if (a.kind == VK_ROWS && a.rows && a.rows->type == ty) return a.rows;
+  usage_error("expected " + tn(ty) + "[], got " + arg_ty(a));
+}
+
+vk_scalar scalar(int32_t t, uint64_t bits) { return vk_scalar{t, bits}; }
+
+uint64_t dbits(double v) {
+  uint64_t b;
+  std::memcpy(&b, &v, 8);
+  return b;
+}
+
+// Allocation checks of Array::try_build (value.rs:283-307) for `len` rows of `t`.
+vk_array *alloc_result(Ctx &c, int32_t t, uint64_t len) { return new_array(c, t, len); }
+
+// ---------------------------------------------------------------- trace
+
+struct TraceScope {
+  Ctx &c;
+  const char *name;
+  uint64_t n = 0, read = 0, write = 0;
+  bool on;
+  TraceScope(Ctx &cc, const char *nm, const vk_arg *args, size_t nargs) : c(cc), name(nm), on(cc.trace) {
+    if (!on) return;
+    for (size_t i = 0; i < nargs; ++i)
+      if (args[i].kind == VK_ROWS && args[i].rows) {
+        if (args[i].rows->len > n) n = args[i].rows->len;
+        read += abytes(args[i].rows);
+      }
+    (void)hipEventRecord(c.ev0, c.stream);
+  }
+  void done(uint64_t write_bytes) {
+    if (!on) return;
+    write = write_bytes;
+    (void)hipEventRecord(c.ev1, c.stream);
+    (void)hipEventSynchronize(c.ev1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c.ev0, c.ev1);
+    c.last_us = ms * 1000.0;
+    const double secs = ms / 1000.0;
+    const double gbps = secs > 0 ? double(read + write) / secs / 1e9 : 0.0;
+    char line[512];
+    std::snprintf(line, sizeof line, "trace %s isa=hip n=%" PRIu64 " read=%" PRIu64 " write=%" PRIu64
+                  " time=%.1f gbps=%.2f\n", name, n, read, write, c.last_us, gbps);
+    c.trace_buf += line;
+  }
+};
+
+}  // namespace
+}  // namespace vk
+
+// ------------------------------------------------------------ device states
+
+struct vk_reducer {
+  vk::Ctx *c;
+  int def;
+  void *state;  // device: canon acc[16] + tail[16]; others up to 3 words
+  uint64_t consumed;
+  int tail_len;
+};
+
+struct vk_scanner {
+  vk::Ctx *c;
+  int def;
+  uint64_t *carry;  // device {acc bits, has}; f64 scan: 16 accumulators
+  uint64_t consumed;
+};
+
+struct vk_builder {
+  vk::Ctx *c;
+  int32_t type;
+  void *ptr;
+  uint64_t cap, len;
+};
+
+namespace vk {
+namespace {
+
+ReduceSpec reduce_spec(const Def &d, uint64_t base) {
+  ReduceSpec s{};
+  s.type = d.t0;
+  s.base_row = base;
+  switch (d.impl) {
+    case R_SUM_INT: s.kind = ReduceSpec::SUM_INT; break;
+    case R_COUNT_CMP: s.kind = ReduceSpec::COUNT_CMP; s.cmp = d.p0; break;
+    case R_COUNT_BOOL: s.kind = ReduceSpec::COUNT_BOOL; s.type = VK_U8; break;
+    case R_EXTREME: s.kind = ReduceSpec::EXTREME; s.want_max = d.p0 != 0; break;
+    case R_ARG: s.kind = ReduceSpec::ARG_EXTREME; s.want_max = d.p0 != 0; break;
+    default: s.kind = ReduceSpec::BITS; s.bitop = d.p0; break;
+  }
+  return s;
+}
+
+vk_reducer *reducer_new(Ctx &c, int def) {
+  auto *r = new vk_reducer{&c, def, nullptr, 0, 0};
+  r->state = tmp_alloc(c, 256);
+  const Def &d = g_defs[def];
+  if (d.impl == R_CANON) canon_init(c, d.p0 == 4, static_cast<double *>(r->state));
+  else reduce_state_init(c, reduce_spec(d, 0), r->state);
+  return r;
+}
+
+// Reducer::update with the entry's own argument checks (fused.rs, reduce.rs Fold::update).
+void reducer_update(vk_reducer *r, const vk_arg *args, size_t nargs) {
+  Ctx &c = *r->c;
+  const Def &d = g_defs[r->def];
+  if (d.impl == R_CANON) {
+    const int kind = d.p0;
+    const vk_array *x = nullptr, *y = nullptr;
+    CanonSpec cs{kind, d.p1, 0, 0};
+    if (kind == 1) {
+      if (nargs != 2) arity(2, nargs);
+      x = rows_arg(args[0], VK_F64);
+      y = rows_arg(args[1], VK_F64);
+      if (x->len != y->len)
+        usage_error("length mismatch: " + std::to_string(x->len) + " vs " + std::to_string(y->len));
+    } else if (kind == 3) {
+      if (nargs != 3) arity(3, nargs);
+      x = rows_arg(args[0], VK_F64);
+      uint64_t cb = splat_arg(args[1], VK_F64), eb = splat_arg(args[2], VK_F64);
+      std::memcpy(&cs.c, &cb, 8);
+      std::memcpy(&cs.e, &eb, 8);
+    } else {
+      if (nargs != 1) arity(1, nargs);
+      x = rows_arg(args[0], VK_F64);
+    }
+    // CanonicalAcc::feed_blocks (reduce.rs:284-308)
+    if (r->tail_len != 0) runtime_error("canonical update after a partial block");
+    const uint64_t n = x->len, blocks = n / 16, rest = n % 16;
+    const double *xp = static_cast<const double *>(dptr(x));
+    const double *yp = y ? static_cast<const double *>(dptr(y)) : nullptr;
+    double *acc = static_cast<double *>(r->state);
+    canon_blocks(c, cs, xp, yp, blocks, acc);
+    canon_rest(c, cs, xp + blocks * 16, yp ? yp + blocks * 16 : nullptr, rest, acc + 16);
+    r->tail_len = static_cast<int>(rest);
+    r->consumed += n;
+    return;
+  }
+  const vk_array *x;
+  if (d.impl == R_COUNT_CMP) {
+    if (nargs != 2) arity(2, nargs);
+    x = rows_arg(args[0], d.t0);
+    ReduceSpec s = reduce_spec(d, r->consumed);
+    s.cbits = splat_arg(args[1], d.t0);
+    reduce_rows(c, s, dptr(x), x->len, r->state);
+  } else {
+    if (nargs != 1) arity(1, nargs);
+    x = rows_arg(args[0], d.in[0].t);
+    reduce_rows(c, reduce_spec(d, r->consumed), dptr(x), x->len, r->state);
+  }
+  r->consumed += x->len;
+}
+
+vk_scalar reducer_finish(vk_reducer *r) {
+  Ctx &c = *r->c;
+  const Def &d = g_defs[r->def];
+  if (d.impl == R_CANON) {
+    double *acc = static_cast<double *>(r->state);
+    canon_finish(c, d.p0 == 4, acc, acc + 16, r->tail_len, reinterpret_cast<double *>(c.scratch.slots));
+    uint64_t b;
+    read_back(c, c.scratch.slots, 8, &b);
+    return scalar(VK_F64, b);
+  }
+  uint64_t w[3];
+  read_back(c, r->state, 24, w);
+  switch (d.impl) {
+    case R_EXTREME:
+      if (!w[1]) runtime_error(std::string("reduce_") + (d.p0 ? "max" : "min") + " of an empty array");
+      return scalar(d.out_t, w[0]);
+    case R_ARG: {
+      const int64_t nan = static_cast<int64_t>(w[2]), idx = static_cast<int64_t>(w[1]);
+      if (nan != INT64_MAX) return scalar(VK_I64, static_cast<uint64_t>(nan));
+      if (idx == INT64_MAX) runtime_error(std::string(d.p0 ? "argmax" : "argmin") + " of an empty array");
+      return scalar(VK_I64, static_cast<uint64_t>(idx));
+    }
+    case R_BITS: {
+      uint64_t v = w[0];
+      if (d.out_t == VK_BOOL) v = v != 0;
+      else if (type_width(d.out_t) < 8) v &= (uint64_t(1) << (8 * type_width(d.out_t))) - 1;
+      return scalar(d.out_t, v);
+    }
+    default: return scalar(d.out_t, w[0]);
+  }
+}
+
+void reducer_free(vk_reducer *r) {
+  if (!r) return;
+  tmp_free(*r->c, r->state);
+  delete r;
+}
+
+// Scanner::update (scan.rs:251-326) with tile_rows' checks (scan.rs:226-248).
+void scanner_update(vk_scanner *s, const vk_arg *args, size_t nargs, vk_array *out) {
+  Ctx &c = *s->c;
+  const Def &d = g_defs[s->def];
+  const bool by = d.impl == SC_SCAN && d.p1;
+  if (nargs != (by ? 2u : 1u)) arity(by ? 2 : 1, nargs);
+  const vk_array *x = rows_arg(args[0], d.in[0].t);
+  const vk_array *st = by ? rows_arg(args[1], VK_BOOL) : nullptr;
+  if (!out || out->type != d.out_t) usage_error("bad destination: the output rows are not " + tn(d.out_t));
+  if (out->len != x->len || (st && st->len != x->len))
+    usage_error("length mismatch: " + std::to_string(x->len) + " rows for a destination of " +
+                std::to_string(out->len) + " rows");
+  if (x->len == 0) return;
+  if (d.impl == SC_FSCAN) {
+    if (s->consumed % 16 != 0)
+      usage_error("scan_add_f64 takes tiles of multiples of 16 rows before the last");
+    scan_add_f64(c, static_cast<const double *>(dptr(x)), x->len, static_cast<double *>(dptr(out)),
+                 reinterpret_cast<double *>(s->carry));
+  } else {
+    ScanSpec sp{d.p0, d.t0, by};
+    scan_rows(c, sp, dptr(x), st ? static_cast<const uint8_t *>(dptr(st)) : nullptr, x->len, dptr(out),
+              s->carry);
+  }
+  s->consumed += x->len;
+}
+
+vk_scanner *scanner_new(Ctx &c, int def) {
+  auto *s = new vk_scanner{&c, def, nullptr, 0};
+  s->carry = static_cast<uint64_t *>(tmp_alloc(c, 256));
+  fill_u64(c, s->carry, 32, 0);
+  return s;
+}
+
+void scanner_free(vk_scanner *s) {
+  if (!s) return;
+  tmp_free(*s->c, s->carry);
+  delete s;
+}
+
+// ------------------------------------------------------------------ builder
+
+void builder_reserve(vk_builder *b, uint64_t extra) {
+  const uint64_t need = b->len + extra;
+  if (need <= b->cap) return;
+  uint64_t cap = b->cap * 2 > need ? b->cap * 2 : need;
+  const uint64_t w = type_width(b->type);
+  void *p = b->c->pool->alloc(cap * w);
+  if (!p) runtime_error("cannot allocate an array of " + std::to_string(cap) + " elements");
+  if (b->ptr) {
+    copy_d2d(*b->c, p, b->ptr, b->len * w);
+    b->c->pool->free(b->ptr);
+  }
+  b->ptr = p;
+  b->cap = cap;
+}
+
+// Appends the survivors of one tile (SelectionFn, kernels.rs:87-96).
+void select_tile(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, uint64_t base, vk_builder *b) {
+  const int64_t rows = check_args(d, args, nargs);
+  if (b->type != d.out_t)
+    usage_error(d.name + " produces " + tn(d.out_t) + " rows, got a " + tn(b->type) + " builder");
+  const uint64_t n = rows < 0 ? 0 : static_cast<uint64_t>(rows);
+  if (n == 0) return;
+  builder_reserve(b, n);
+  void *out = static_cast<char *>(b->ptr) + b->len * type_width(b->type);
+  uint64_t *d_cnt = c.scratch.slots + 16;
+  SelectSpec s{};
+  if (d.impl == S_COMPRESS) {
+    s.kind = 0;
+    s.type = d.t0;
+    select_rows(c, s, dptr(args[0].rows), static_cast<const uint8_t *>(dptr(args[1].rows)), n, out, d_cnt);
+  } else if (d.impl == S_FILTER) {
+    s.kind = 1;
+    s.type = d.t0;
+    s.cmp = d.p0;
+    s.cbits = args[1].splat.bits;
+    select_rows(c, s, dptr(args[0].rows), nullptr, n, out, d_cnt);
+  } else {
+    s.kind = 2;
+    s.type = VK_I64;
+    s.base = base;
+    select_rows(c, s, nullptr, static_cast<const uint8_t *>(dptr(args[0].rows)), n, out, d_cnt);
+  }
+  uint64_t got = 0;
+  read_back(c, d_cnt, 8, &got);
+  b->len += got;
+}
+
+vk_array *builder_take(vk_builder *b) {
+  Ctx &c = *b->c;
+  const uint64_t w = type_width(b->type);
+  // A survivor count far below the reservation gets an exact buffer, so a 1% filter does
+  // not hold the input's size in HBM.
+  if (b->ptr && b->cap - b->len >= (uint64_t(16) << 20) / w && b->len <= b->cap / 4) {
+    void *p = c.pool->alloc(b->len * w);
+    if (p) {
+      copy_d2d(c, p, b->ptr, b->len * w);
+      c.pool->free(b->ptr);
+      b->ptr = p;
+      b->cap = b->len;
+    }
+  }
+  if (!b->ptr) {
+    b->ptr = c.pool->alloc(256);
+    if (!b->ptr) runtime_error("cannot allocate an array of 0 elements");
+  }
+  auto *buf = new Buffer{&c, b->ptr, b->cap * w, true, 1};
+  auto *a = new vk_array{1, buf, 0, b->len, b->type};
+  b->ptr = nullptr;
+  b->cap = b->len = 0;
+  return a;
+}
+
+// ------------------------------------------------------------ whole kernels
+
+uint64_t count_arg(const vk_arg &a) {
+  const int64_t n = i64_arg(a);
+  if (n < 0) runtime_error("negative length " + std::to_string(n));
+  return static_cast<uint64_t>(n);
+}
+
+// checked_keys (index.rs:113-126): bucket count, then the first key out of range.
+uint64_t checked_keys(Ctx &c, const vk_array *keys, const vk_arg &nb) {
+  const int64_t buckets = i64_arg(nb);
+  if (buckets < 0) runtime_error("negative bucket count " + std::to_string(buckets));
+  return static_cast<uint64_t>(buckets);
+}
+
+void bucket_error(Ctx &c, const vk_array *keys, uint64_t bad_row, int64_t buckets) {
+  int64_t k;
+  read_back(c, static_cast<const int64_t *>(dptr(keys)) + bad_row, 8, &k);
+  runtime_error("bucket key " + std::to_string(k) + " out of range for " + std::to_string(buckets) +
+                " buckets");
+}
+
+vk_value run_whole(Ctx &c, const Def &d, const vk_arg *args, size_t nargs) {
+  vk_value v{VK_VALUE_ARRAY, nullptr, {0, 0}};
+  switch (d.impl) {
+    case W_GATHER: {
+      if (nargs != 2) arity(2, nargs);
+      const vk_array *xs = array_arg(args[0], d.t0);
+      const vk_array *idx = array_arg(args[1], VK_I64);
+      vk_array *out = alloc_result(c, d.t0, idx->len);
+      uint64_t *d_bad = c.scratch.slots + 17;
+      gather_rows(c, type_width(d.t0), dptr(xs), xs->len, static_cast<const int64_t *>(dptr(idx)),
+                  idx->len, dptr(out), d_bad);
+      uint64_t bad;
+      read_back(c, d_bad, 8, &bad);
+      if (bad != ~uint64_t(0)) {
+        release(out);
+        int64_t i;
+        read_back(c, static_cast<const int64_t *>(dptr(idx)) + bad, 8, &i);
+        runtime_error("gather index " + std::to_string(i) + " out of range for length " +
+                      std::to_string(xs->len));
+      }
+      v.array = out;
+      return v;
+    }
+    case W_ARGSORT: {
+      if (nargs != 1) arity(1, nargs);
+      const vk_array *xs = array_arg(args[0], d.t0);
+      vk_array *out = alloc_result(c, VK_I64, xs->len);
+      argsort(c, d.t0, dptr(xs), xs->len, static_cast<int64_t *>(dptr(out)));
+      v.array = out;
+      return v;
+    }
+    case W_GROUP_ID: case W_GROUP_ID2: {
+      const bool pair = d.impl == W_GROUP_ID2;
+      if (nargs != (pair ? 2u : 1u)) arity(pair ? 2 : 1, nargs);
+      const vk_array *a = array_arg(args[0], d.t0);
+      const vk_array *b = pair ? array_arg(args[1], d.t1) : nullptr;
+      if (b && a->len != b->len)
+        runtime_error("length mismatch: " + std::to_string(a->len) + " vs " + std::to_string(b->len));
+      vk_array *out = alloc_result(c, VK_I64, a->len);
+      group_id(c, d.t0, dptr(a), pair ? d.t1 : 0, b ? dptr(b) : nullptr, a->len,
+               static_cast<int64_t *>(dptr(out)));
+      v.array = out;
+      return v;
+    }
+    case W_HISTOGRAM: case W_REDUCE_INTO: {
+      const bool hist = d.impl == W_HISTOGRAM;
+      if (nargs != (hist ? 2u : 3u)) arity(hist ? 2 : 3, nargs);
+      const vk_array *vals = hist ? nullptr : array_arg(args[1], d.t0);  // vals first, index.rs:199
+      const vk_array *keys = array_arg(args[0], VK_I64);
+      const int64_t buckets = static_cast<int64_t>(checked_keys(c, keys, args[hist ? 1 : 2]));
+      uint64_t *d_bad = c.scratch.slots + 18;
+      uint64_t bad;
+      const int op = hist ? 0 : d.p0;
+      if (op == 4 || (!hist && keys->len != vals->len)) {
+        check_keys(c, static_cast<const int64_t *>(dptr(keys)), keys->len, uint64_t(buckets), d_bad);
+        read_back(c, d_bad, 8, &bad);
+        if (bad != ~uint64_t(0)) bucket_error(c, keys, bad, buckets);
+        if (!hist && keys->len != vals->len)
+          runtime_error("length mismatch: " + std::to_string(keys->len) + " keys for " +
+                        std::to_string(vals->len) + " values");
+      }
+      vk_array *out = alloc_result(c, d.out_t, uint64_t(buckets));
+      if (op == 4) {
+        reduce_into_add_f64(c, static_cast<const int64_t *>(dptr(keys)),
+                            static_cast<const double *>(dptr(vals)), keys->len, uint64_t(buckets),
+                            static_cast<double *>(dptr(out)));
+      } else {
+        bucket_init(c, op, d.out_t, dptr(out), uint64_t(buckets));
+        bucket_rows(c, op, d.out_t, static_cast<const int64_t *>(dptr(keys)), vals ? dptr(vals) : nullptr,
+                    keys->len, uint64_t(buckets), dptr(out), d_bad);
+        read_back(c, d_bad, 8, &bad);
+        if (bad != ~uint64_t(0)) {
+          release(out);
+          bucket_error(c, keys, bad, buckets);
+        }
+      }
+      v.array = out;
+      return v;
+    }
+    case W_RAND_F64: case W_RAND_I64: {
+      if (nargs != 2) arity(2, nargs);
+      const uint64_t n = count_arg(args[0]);
+      const uint64_t seed = static_cast<uint64_t>(i64_arg(args[1]));
+      vk_array *out = alloc_result(c, d.out_t, n);
+      if (d.impl == W_RAND_F64) rand_f64(c, n, seed, static_cast<double *>(dptr(out)));
+      else rand_i64(c, n, seed, static_cast<int64_t *>(dptr(out)));
+      v.array = out;
+      return v;
+    }
+    case W_IOTA: {
+      if (nargs != 1) arity(1, nargs);
+      const uint64_t n = count_arg(args[0]);
+      vk_array *out = alloc_result(c, VK_I64, n);
+      iota(c, n, static_cast<int64_t *>(dptr(out)));
+      v.array = out;
+      return v;
+    }
+    case W_LENGTH: {
+      if (nargs != 1) arity(1, nargs);
+      const vk_array *a = array_arg(args[0], d.t0);
+      v.kind = VK_VALUE_SCALAR;
+      v.scalar = scalar(VK_I64, a->len);
+      return v;
+    }
+    case W_FILL: {
+      if (nargs != 2) arity(2, nargs);
+      const uint64_t n = count_arg(args[0]);
+      if (args[1].kind != VK_SPLAT || args[1].splat.type != d.t0)
+        usage_error("expected " + tn(d.t0) + ", got " + arg_ty(args[1]));
+      vk_array *out = alloc_result(c, d.t0, n);
+      fill_bits(c, type_width(d.t0), n, args[1].splat.bits, dptr(out));
+      v.array = out;
+      return v;
+    }
+    default: usage_error("not a whole kernel: " + d.name);
+  }
+}
+
+void run_elementwise(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, vk_array *out) {
+  EwSpec s{};
+  s.op = d.p0;
+  s.in_type = d.t0;
+  const void *a = nullptr, *b = nullptr;
+  if (args[0].kind == VK_SPLAT) {
+    s.a_splat = true;
+    s.a_bits = args[0].splat.bits;
+  } else {
+    a = dptr(args[0].rows);
+  }
+  if (nargs > 1) {
+    if (args[1].kind == VK_SPLAT) {
+      s.b_splat = true;
+      s.b_bits = args[1].splat.bits;
+    } else {
+      b = dptr(args[1].rows);
+    }
+  }
+  elementwise(c, s, a, b, out->len, dptr(out));
+}
+
+// Dispatch::call (dispatch.rs:56-141)
+vk_value call(Ctx &c, const Def &d, const vk_arg *args, size_t nargs) {
+  vk_value v{VK_VALUE_ARRAY, nullptr, {0, 0}};
+  if (d.cls == VK_WHOLE) {
+    // whole kernels check their arguments themselves; kinds are checked like accepts()
+    if (nargs == d.in.size())
+      for (size_t i = 0; i < nargs; ++i)
+        if (!accepts(d.in[i], args[i])) kind_error(d.in[i], args[i]);
+    return run_whole(c, d, args, nargs);
+  }
+  const int64_t rows = check_args(d, args, nargs);
+  switch (d.cls) {
+    case VK_ELEMENTWISE: {
+      if (rows < 0) usage_error("an array kernel needs an array operand");
+      vk_array *out = alloc_result(c, d.out_t, uint64_t(rows));
+      run_elementwise(c, d, args, nargs, out);
+      v.array = out;
+      return v;
+    }
+    case VK_REDUCTION: {
+      if (rows < 0) usage_error("a reduction needs an array operand");
+      vk_reducer *r = reducer_new(c, g_by_name[d.name]);
+      try {
+        if (rows > 0) reducer_update(r, args, nargs);
+        v.kind = VK_VALUE_SCALAR;
+        v.scalar = reducer_finish(r);
+      } catch (...) {
+        reducer_free(r);
+        throw;
+      }
+      reducer_free(r);
+      return v;
+    }
+    case VK_SCAN: {
+      if (rows < 0) usage_error("a scan needs an array operand");
+      vk_array *out = alloc_result(c, d.out_t, uint64_t(rows));
+      vk_scanner *s = scanner_new(c, g_by_name[d.name]);
+      try {
+        if (rows > 0) scanner_update(s, args, nargs, out);
+      } catch (...) {
+        scanner_free(s);
+        release(out);
+        throw;
+      }
+      scanner_free(s);
+      v.array = out;
+      return v;
+    }
+    default: {  // selection, one tile with base 0
+      vk_builder b{&c, d.out_t, nullptr, 0, 0};
+      try {
+        select_tile(c, d, args, nargs, 0, &b);
+      } catch (...) {
+        if (b.ptr) c.pool->free(b.ptr);
+        throw;
+      }
+      v.array = builder_take(&b);
+      return v;
+    }
+  }
+}
+
+int set_status(vk_status *st, int code, const std::string &msg) {
+  if (st) {
+    st->code = code;
+    std::snprintf(st->msg, sizeof st->msg, "%s", msg.c_str());
+  }
+  return code;
+}
+
+template <typename F>
+int guarded(vk_status *st, F &&f) {
+  try {
+    f();
+    return set_status(st, VK_OK, "");
+  } catch (const Error &e) {
+    return set_status(st, e.code, e.msg);
+  } catch (const std::exception &e) {
+    return set_status(st, VK_ERR_DEVICE, e.what());
+  }
+}
+
+}  // namespace
+}  // namespace vk
+
+using namespace vk;
+
+extern "C" {
+
+int vk_ctx_create(int device, vk_ctx **out, vk_status *st) {
+  return guarded(st, [&] { *out = create_ctx(device); });
+}
+void vk_ctx_destroy(vk_ctx *ctx) { destroy_ctx(ctx); }
+int vk_ctx_sync(vk_ctx *ctx, vk_status *st) {
+  return guarded(st, [&] { check_hip(hipStreamSynchronize(ctx->stream), "sync"); });
+}
+void *vk_ctx_stream(vk_ctx *ctx) { return ctx->stream; }
+void vk_ctx_trim(vk_ctx *ctx) {
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->pool->trim();
+}
+uint64_t vk_ctx_pool_bytes(const vk_ctx *ctx, uint64_t *in_use) {
+  if (in_use) *in_use = ctx->pool->in_use();
+  return ctx->pool->held();
+}
+void vk_ctx_set_trace(vk_ctx *ctx, int on) { ctx->trace = on != 0; }
+size_t vk_ctx_trace_take(vk_ctx *ctx, char *buf, size_t cap) {
+  const size_t n = ctx->trace_buf.size();
+  if (buf && cap) {
+    const size_t k = n < cap - 1 ? n : cap - 1;
+    std::memcpy(buf, ctx->trace_buf.data(), k);
+    buf[k] = 0;
+  }
+  ctx->trace_buf.clear();
+  return n;
+}
+double vk_ctx_last_us(const vk_ctx *ctx) { return ctx->last_us; }
+
+int vk_array_new(vk_ctx *ctx, int32_t type, uint64_t len, vk_array **out, vk_status *st) {
+  return guarded(st, [&] {
+    if (!valid_type(type)) usage_error("unknown element type");
+    *out = new_array(*ctx, type, len);
+  });
+}
+int vk_array_from_host(vk_ctx *ctx, int32_t type, const void *host, uint64_t len, vk_array **out,
+                       vk_status *st) {
+  return guarded(st, [&] {
+    if (!valid_type(type)) usage_error("unknown element type");
+    vk_array *a = new_array(*ctx, type, len);
+    if (len)
+      check_hip(hipMemcpyAsync(dptr(a), host, abytes(a), hipMemcpyHostToDevice, ctx->stream), "upload");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+    *out = a;
+  });
+}
+int vk_array_to_host(vk_ctx *ctx, const vk_array *a, void *host, vk_status *st) {
+  return guarded(st, [&] {
+    if (a->len)
+      check_hip(hipMemcpyAsync(host, dptr(a), abytes(a), hipMemcpyDeviceToHost, ctx->stream), "download");
+    check_hip(hipStreamSynchronize(ctx->stream), "sync");
+  });
+}
+int vk_array_wrap(vk_ctx *ctx, int32_t type, void *device_ptr, uint64_t len, vk_array **out,
+                  vk_status *st) {
+  return guarded(st, [&] {
+    if (!valid_type(type)) usage_error("unknown element type");
+    if (reinterpret_cast<uintptr_t>(device_ptr) % type_width(type))
+      usage_error("device pointer is not aligned for " + tn(type));
+    auto *b = new Buffer{ctx, device_ptr, len * type_width(type), false, 1};
+    *out = new vk_array{1, b, 0, len, type};
+  });
+}
+int vk_array_slice(const vk_array *a, uint64_t start, uint64_t len, vk_array **out, vk_status *st) {
+  return guarded(st, [&] {
+    if (start > a->len || len > a->len - start)
+      usage_error("slice " + std::to_string(start) + "+" + std::to_string(len) + " out of range for length " +
+                  std::to_string(a->len));
+    a->buf->refs++;
+    *out = new vk_array{1, a->buf, a->offset + start * type_width(a->type), len, a->type};
+  });
+}
+void vk_array_retain(vk_array *a) {
+  if (a) a->refs++;
+}
+void vk_array_release(vk_array *a) { release(a); }
+uint64_t vk_array_len(const vk_array *a) { return a->len; }
+int32_t vk_array_type(const vk_array *a) { return a->type; }
+uint64_t vk_array_bytes(const vk_array *a) { return abytes(a); }
+void *vk_array_device_ptr(const vk_array *a) { return dptr(a); }
+
+int32_t vk_kernel_count(void) {
+  build_table();
+  return static_cast<int32_t>(g_defs.size());
+}
+int32_t vk_kernel_lookup(const char *name) {
+  build_table();
+  auto it = g_by_name.find(name ? name : "");
+  return it == g_by_name.end() ? -1 : it->second;
+}
+const char *vk_kernel_name(int32_t id) {
+  build_table();
+  return id >= 0 && id < int32_t(g_defs.size()) ? g_defs[id].name.c_str() : nullptr;
+}
+int32_t vk_kernel_class(int32_t id) {
+  build_table();
+  return id >= 0 && id < int32_t(g_defs.size()) ? g_defs[id].cls : -1;
+}
+const char *vk_variant_name(void) { return "isa=hip"; }
+
+int vk_call(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, vk_value *out, vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    TraceScope ts(*ctx, d.name.c_str(), args, nargs);
+    *out = call(*ctx, d, args, nargs);
+    ts.done(out->kind == VK_VALUE_ARRAY && out->array ? abytes(out->array) : 0);
+  });
+}
+
+int vk_reducer_new(vk_ctx *ctx, const char *name, vk_reducer **out, vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    if (d.cls != VK_REDUCTION) usage_error(d.name + " is not a reduction");
+    *out = reducer_new(*ctx, g_by_name[d.name]);
+  });
+}
+int vk_reducer_update(vk_reducer *r, const vk_arg *args, size_t nargs, vk_status *st) {
+  return guarded(st, [&] { reducer_update(r, args, nargs); });
+}
+int vk_reducer_finish(vk_reducer *r, vk_scalar *out, vk_status *st) {
+  return guarded(st, [&] { *out = reducer_finish(r); });
+}
+void vk_reducer_free(vk_reducer *r) { reducer_free(r); }
+
+int vk_scanner_new(vk_ctx *ctx, const char *name, vk_scanner **out, vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    if (d.cls != VK_SCAN) usage_error(d.name + " is not a scan");
+    *out = scanner_new(*ctx, g_by_name[d.name]);
+  });
+}
+int vk_scanner_update(vk_scanner *s, const vk_arg *args, size_t nargs, vk_array *out, vk_status *st) {
+  return guarded(st, [&] { scanner_update(s, args, nargs, out); });
+}
+void vk_scanner_free(vk_scanner *s) { scanner_free(s); }
+
+int vk_builder_new(vk_ctx *ctx, int32_t type, vk_builder **out, vk_status *st) {
+  return guarded(st, [&] {
+    if (!valid_type(type)) usage_error("unknown element type");
+    *out = new vk_builder{ctx, type, nullptr, 0, 0};
+  });
+}
+uint64_t vk_builder_len(const vk_builder *b) { return b->len; }
+int vk_builder_finish(vk_builder *b, vk_array **out, vk_status *st) {
+  int rc = guarded(st, [&] { *out = builder_take(b); });
+  delete b;
+  return rc;
+}
+void vk_builder_free(vk_builder *b) {
+  if (!b) return;
+  if (b->ptr) b->c->pool->free(b->ptr);
+  delete b;
+}
+
+int vk_select_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, uint64_t base,
+                   vk_builder *out, vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    if (d.cls != VK_SELECTION) usage_error(d.name + " is not a selection");
+    select_tile(*ctx, d, args, nargs, base, out);
+  });
+}
+
+int vk_elementwise_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, vk_array *out,
+                        vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    if (d.cls != VK_ELEMENTWISE) usage_error(d.name + " is not an elementwise kernel");
+    if (!out || out->type != d.out_t) usage_error("destination is not aligned for " + tn(d.out_t));
+    const int64_t rows = check_args(d, args, nargs);
+    if (rows < 0) usage_error("an array kernel needs an array operand");
+    if (uint64_t(rows) != out->len)
+      usage_error("length mismatch: " + std::to_string(rows) + " rows for a destination of " +
+                  std::to_string(out->len) + " rows");
+    run_elementwise(*ctx, d, args, nargs, out);
+  });
+}
+
+void vk_value_release(vk_value *v) {
+  if (v && v->kind == VK_VALUE_ARRAY && v->array) {
+    release(v->array);
+    v->array = nullptr;
+  }
+}
+
+}  // extern "C"

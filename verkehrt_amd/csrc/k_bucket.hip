@@ -1,0 +1,287 @@
+// Bucketed folds: histogram (index.rs:137-155) and reduce_into_<op>_<t> (ReduceInto,
+// index.rs:192-222): the fold of the values per key in 0..n, an empty bucket holding the
+// fold's identity. Keys outside 0..n are an error naming the first bad key in row order
+// (checked_keys, index.rs:113-126).
+//
+// Integer add and count commute, so rows fold with atomics: into a per-block LDS copy of the
+// buckets when they fit (one global atomic per bucket per block), else straight into HBM.
+// min/max use compare-and-swap loops with the IEEE minimum/maximum rules.
+//
+// f64 add has no reference kernel (builtins.dd:67-71); its order here is the tier 2
+// candidate (tier2-design.md:171): row i < b = n - n%16 folds into stripe i%16 of its bucket
+// in index order, the stripes of a bucket combine pairwise, then rows >= b add in index
+// order. Rows are stably sorted by (bucket, stripe), each stripe folds sequentially, and
+// stripes combine per bucket.
+#include "vk_device.h"
+#include "vk_internal.h"
+
+namespace vk {
+namespace {
+
+constexpr uint64_t kLdsBuckets = 4096;  // 32 KB of u64 buckets per block
+
+__device__ __forceinline__ void note_bad(uint64_t row, uint64_t *my_bad) {
+  if (row < *my_bad) *my_bad = row;
+}
+
+__device__ __forceinline__ void flush_bad(uint64_t my_bad, uint64_t *bad) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint64_t o = __shfl_xor(my_bad, m);
+    my_bad = o < my_bad ? o : my_bad;
+  }
+  if (lane_id() == 0 && my_bad != ~uint64_t(0))
+    atomicMin(reinterpret_cast<unsigned long long *>(bad), my_bad);
+}
+
+// op 0 count, 1 add. Values are i64 (add) or unused (count).
+template <int OP, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_bucket_add(const int64_t *__restrict__ keys,
+                                                       const int64_t *__restrict__ vals, uint64_t n,
+                                                       uint64_t buckets, unsigned long long *out,
+                                                       uint64_t *bad) {
+  __shared__ unsigned long long s_b[LDS ? kLdsBuckets : 1];
+  if (LDS) {
+    for (uint64_t k = threadIdx.x; k < buckets; k += blockDim.x) s_b[k] = 0;
+    __syncthreads();
+  }
+  uint64_t my_bad = ~uint64_t(0);
+  grid_rows(keys, n, [&](int64_t k, uint64_t row) {
+    if (static_cast<uint64_t>(k) >= buckets) {
+      note_bad(row, &my_bad);
+      return;
+    }
+    const unsigned long long v = OP == 0 ? 1ull : static_cast<unsigned long long>(vals[row]);
+    if (LDS) atomicAdd(&s_b[k], v);
+    else atomicAdd(&out[k], v);
+  });
+  flush_bad(my_bad, bad);
+  if (LDS) {
+    __syncthreads();
+    for (uint64_t k = threadIdx.x; k < buckets; k += blockDim.x)
+      if (s_b[k]) atomicAdd(&out[k], s_b[k]);
+  }
+}
+
+template <typename T, bool MAX>
+__global__ __launch_bounds__(kBlock) void k_bucket_ext(const int64_t *__restrict__ keys,
+                                                       const T *__restrict__ vals, uint64_t n,
+                                                       uint64_t buckets, T *out, uint64_t *bad) {
+  uint64_t my_bad = ~uint64_t(0);
+  grid_rows(keys, n, [&](int64_t k, uint64_t row) {
+    if (static_cast<uint64_t>(k) >= buckets) {
+      note_bad(row, &my_bad);
+      return;
+    }
+    const T v = vals[row];
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(&out[k]);
+    unsigned long long old = *p;
+    while (true) {
+      const T cur = from_bits<T>(old);
+      const T nv = MAX ? max2(cur, v) : min2(cur, v);
+      const unsigned long long nb = to_bits(nv);
+      if (nb == old) break;
+      const unsigned long long got = atomicCAS(p, old, nb);
+      if (got == old) break;
+      old = got;
+    }
+  });
+  flush_bad(my_bad, bad);
+}
+
+__global__ void k_check_keys(const int64_t *__restrict__ keys, uint64_t n, uint64_t buckets,
+                             uint64_t *bad) {
+  uint64_t my_bad = ~uint64_t(0);
+  grid_rows(keys, n, [&](int64_t k, uint64_t row) {
+    if (static_cast<uint64_t>(k) >= buckets) note_bad(row, &my_bad);
+  });
+  flush_bad(my_bad, bad);
+}
+
+// ----------------------------------------------------------- f64 reduce_into add
+
+// chain key of row i < b: bucket * 16 + i % 16
+__global__ void k_chain_keys(const int64_t *__restrict__ keys, uint64_t b, uint64_t *ck) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < b;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    ck[i] = static_cast<uint64_t>(keys[i]) * 16 + (i & 15);
+}
+
+__global__ void k_gather_vals(const double *__restrict__ vals, const int64_t *__restrict__ perm,
+                              uint64_t b, double *sv) {
+  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
+       p += uint64_t(gridDim.x) * blockDim.x)
+    sv[p] = vals[perm[p]];
+}
+
+// One thread per chain start: folds the chain's values (sorted positions, so index order).
+__global__ void k_chain_fold_thread(const uint64_t *__restrict__ sk, const double *__restrict__ sv,
+                                    uint64_t b, double *stripes) {
+  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
+       p += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = sk[p];
+    if (p > 0 && sk[p - 1] == k) continue;
+    double a = 0.0;
+    uint64_t q = p;
+    while (q < b && sk[q] == k) {
+      a = a + sv[q];
+      ++q;
+    }
+    stripes[k] = a;
+  }
+}
+
+// One wave per chain start for long chains: lanes load 64 values ahead, lane 0 folds them.
+__global__ __launch_bounds__(64) void k_chain_fold_wave(const uint64_t *__restrict__ starts,
+                                                         uint64_t nchains,
+                                                         const uint64_t *__restrict__ sk,
+                                                         const double *__restrict__ sv, uint64_t b,
+                                                         double *stripes) {
+  const int l = lane_id();
+  for (uint64_t ch = blockIdx.x; ch < nchains; ch += gridDim.x) {
+    const uint64_t p0 = starts[ch];
+    const uint64_t p1 = ch + 1 < nchains ? starts[ch + 1] : b;
+    const uint64_t k = sk[p0];
+    double a = 0.0;
+    for (uint64_t q = p0; q < p1; q += 64) {
+      const uint64_t r = q + l;
+      const double v = r < p1 ? sv[r] : 0.0;
+      const int cnt = static_cast<int>(p1 - q < 64 ? p1 - q : 64);
+      for (int j = 0; j < cnt; ++j) {
+        const double u = __shfl(v, j);
+        a = a + u;
+      }
+    }
+    if (l == 0) stripes[k] = a;
+  }
+}
+
+__global__ void k_chain_starts(const uint64_t *__restrict__ sk, uint64_t b, uint8_t *flag) {
+  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
+       p += uint64_t(gridDim.x) * blockDim.x)
+    flag[p] = (p == 0 || sk[p - 1] != sk[p]) ? 1 : 0;
+}
+
+__global__ void k_combine_stripes(const double *__restrict__ stripes, uint64_t buckets, double *out) {
+  for (uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; k < buckets;
+       k += uint64_t(gridDim.x) * blockDim.x) {
+    double a[16];
+    for (int j = 0; j < 16; ++j) a[j] = stripes[k * 16 + j];
+    for (int step = 8; step > 0; step /= 2)
+      for (int j = 0; j < step; ++j) a[j] = a[j] + a[j + step];
+    out[k] = a[0];
+  }
+}
+
+__global__ void k_fold_tail(const int64_t *keys, const double *vals, uint64_t b, uint64_t n,
+                            double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint64_t i = b; i < n; ++i) out[keys[i]] = out[keys[i]] + vals[i];
+}
+
+template <typename T>
+__global__ void k_fill(T *p, uint64_t n, T v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+void bucket_init(Ctx &c, int op, int32_t type, void *out, uint64_t buckets) {
+  if (buckets == 0) return;
+  const uint32_t g = stream_grid(buckets, kBlock, 4096);
+  if (op <= 1) {
+    k_fill<uint64_t><<<g, kBlock, 0, c.stream>>>(static_cast<uint64_t *>(out), buckets, 0);
+  } else if (type == VK_F64) {
+    const double id = op == 2 ? __builtin_inf() : -__builtin_inf();
+    k_fill<double><<<g, kBlock, 0, c.stream>>>(static_cast<double *>(out), buckets, id);
+  } else {
+    const int64_t id = op == 2 ? INT64_MAX : INT64_MIN;
+    k_fill<int64_t><<<g, kBlock, 0, c.stream>>>(static_cast<int64_t *>(out), buckets, id);
+  }
+  check_hip(hipGetLastError(), "bucket init");
+}
+
+void bucket_rows(Ctx &c, int op, int32_t type, const int64_t *keys, const void *vals, uint64_t n,
+                 uint64_t buckets, void *out, uint64_t *d_bad) {
+  fill_u64(c, d_bad, 1, ~uint64_t(0));
+  if (n == 0) return;
+  const uint32_t g = stream_grid(n, kBlock * 8, 2048);
+  auto *o = static_cast<unsigned long long *>(out);
+  const auto *v = static_cast<const int64_t *>(vals);
+  if (op <= 1) {
+    const bool lds = buckets <= kLdsBuckets;
+    if (op == 0) {
+      if (lds) k_bucket_add<0, true><<<g, kBlock, 0, c.stream>>>(keys, v, n, buckets, o, d_bad);
+      else k_bucket_add<0, false><<<g, kBlock, 0, c.stream>>>(keys, v, n, buckets, o, d_bad);
+    } else {
+      if (lds) k_bucket_add<1, true><<<g, kBlock, 0, c.stream>>>(keys, v, n, buckets, o, d_bad);
+      else k_bucket_add<1, false><<<g, kBlock, 0, c.stream>>>(keys, v, n, buckets, o, d_bad);
+    }
+  } else if (type == VK_F64) {
+    auto *vf = static_cast<const double *>(vals);
+    auto *of = static_cast<double *>(out);
+    if (op == 2) k_bucket_ext<double, false><<<g, kBlock, 0, c.stream>>>(keys, vf, n, buckets, of, d_bad);
+    else k_bucket_ext<double, true><<<g, kBlock, 0, c.stream>>>(keys, vf, n, buckets, of, d_bad);
+  } else {
+    auto *oi = static_cast<int64_t *>(out);
+    if (op == 2) k_bucket_ext<int64_t, false><<<g, kBlock, 0, c.stream>>>(keys, v, n, buckets, oi, d_bad);
+    else k_bucket_ext<int64_t, true><<<g, kBlock, 0, c.stream>>>(keys, v, n, buckets, oi, d_bad);
+  }
+  check_hip(hipGetLastError(), "bucket launch");
+}
+
+void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint64_t *d_bad) {
+  fill_u64(c, d_bad, 1, ~uint64_t(0));
+  if (n == 0) return;
+  k_check_keys<<<stream_grid(n, kBlock * 8, 2048), kBlock, 0, c.stream>>>(keys, n, buckets, d_bad);
+  check_hip(hipGetLastError(), "check keys");
+}
+
+// Keys are already checked to lie in 0..buckets.
+void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
+                         uint64_t buckets, double *out) {
+  if (buckets == 0) return;
+  const uint64_t b = n - n % 16;
+  double *stripes = static_cast<double *>(tmp_alloc(c, buckets * 16 * 8));
+  k_fill<double><<<stream_grid(buckets * 16, kBlock, 4096), kBlock, 0, c.stream>>>(stripes, buckets * 16, 0.0);
+  if (b > 0) {
+    uint64_t *ck = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
+    uint64_t *sk = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
+    int64_t *perm = static_cast<int64_t *>(tmp_alloc(c, b * 8));
+    double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
+    const uint32_t g = stream_grid(b, kBlock * 4, 8192);
+    k_chain_keys<<<g, kBlock, 0, c.stream>>>(keys, b, ck);
+    sort_u64_pairs(c, ck, b, sk, perm);
+    k_gather_vals<<<g, kBlock, 0, c.stream>>>(vals, perm, b, sv);
+    const uint64_t chains_est = buckets * 16 < b ? buckets * 16 : b;
+    if (b / chains_est >= 256) {
+      // long chains: compact the chain starts, one wave per chain
+      uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
+      uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
+      k_chain_starts<<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+      SelectSpec ss{2, VK_I64, 0, 0, 0};
+      uint64_t *d_cnt = c.scratch.slots + 20;
+      select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
+      uint64_t nchains = 0;
+      read_back(c, d_cnt, 8, &nchains);
+      k_chain_fold_wave<<<stream_grid(nchains, 1, 16384), 64, 0, c.stream>>>(starts, nchains, sk, sv,
+                                                                             b, stripes);
+      tmp_free(c, flag);
+      tmp_free(c, starts);
+    } else {
+      k_chain_fold_thread<<<g, kBlock, 0, c.stream>>>(sk, sv, b, stripes);
+    }
+    tmp_free(c, ck);
+    tmp_free(c, sk);
+    tmp_free(c, perm);
+    tmp_free(c, sv);
+  }
+  k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, buckets, out);
+  if (n > b) k_fold_tail<<<1, 64, 0, c.stream>>>(keys, vals, b, n, out);
+  check_hip(hipGetLastError(), "f64 reduce_into");
+  tmp_free(c, stripes);
+}
+
+}  // namespace vk

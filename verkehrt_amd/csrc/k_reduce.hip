@@ -1,0 +1,312 @@
+// Streaming reductions whose result does not depend on the order rows combine in:
+// counts (count_<cmp>_<t>, fused.rs:142-179; count, reduce.rs:527-529), wrapping integer
+// sums that widen (IntSum, reduce.rs:325-377), IEEE minimum/maximum (Extreme,
+// reduce.rs:379-412), first-index arg extremes (ArgExtreme, reduce.rs:414-468) and bit
+// folds (reduce.rs:531-575).
+//
+// One launch reads its rows once with 128-bit loads spread over the grid, reduces per
+// thread, per wave (DPP/permute shuffles), per block (LDS), writes one partial per block,
+// and the last block to finish folds the partials into the reducer's device state.
+// HBM-bound: algorithmic bytes = n x width.
+#include "vk_device.h"
+#include "vk_internal.h"
+
+namespace vk {
+namespace {
+
+// ---------------------------------------------------------------- accumulators
+
+struct SumAcc {
+  uint64_t s;
+};
+struct ExtAcc {
+  uint64_t bits;  // value bits
+  uint64_t has;
+};
+struct ArgAcc {
+  uint64_t bits;  // best value bits
+  int64_t idx;    // best index, INT64_MAX when none
+  int64_t nan;    // first NaN index, INT64_MAX when none
+};
+
+constexpr int64_t kNone = INT64_MAX;
+
+// Rows widen to u64 for sums: i32 sign-extends, u8 zero-extends (reduce.rs:375-377).
+template <typename T>
+__device__ __forceinline__ uint64_t widen(T v) {
+  return static_cast<uint64_t>(static_cast<int64_t>(v));
+}
+template <>
+__device__ __forceinline__ uint64_t widen<uint8_t>(uint8_t v) { return v; }
+template <>
+__device__ __forceinline__ uint64_t widen<uint64_t>(uint64_t v) { return v; }
+
+// Ordered::less (reduce.rs:183-199): -0.0 ranks below +0.0; ints compare directly.
+template <typename T>
+__device__ __forceinline__ bool ord_less(T a, T b) { return a < b; }
+template <>
+__device__ __forceinline__ bool ord_less<double>(double a, double b) {
+  return a < b || (a == b && __signbit(a) && !__signbit(b));
+}
+template <>
+__device__ __forceinline__ bool ord_less<float>(float a, float b) {
+  return a < b || (a == b && __signbit(a) && !__signbit(b));
+}
+template <typename T>
+__device__ __forceinline__ bool is_nan(T) { return false; }
+template <>
+__device__ __forceinline__ bool is_nan<double>(double v) { return v != v; }
+template <>
+__device__ __forceinline__ bool is_nan<float>(float v) { return v != v; }
+
+// -------------------------------------------------------------- the operators
+
+template <typename T>
+struct OpSum {
+  using Acc = SumAcc;
+  __device__ Acc identity() const { return {0}; }
+  __device__ void add(Acc &a, T v, uint64_t) const { a.s += widen(v); }
+  __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
+};
+
+template <typename T>
+struct OpCount {
+  using Acc = SumAcc;
+  int cmp;
+  T c;
+  __device__ Acc identity() const { return {0}; }
+  __device__ void add(Acc &a, T v, uint64_t) const { a.s += cmp_test(cmp, v, c) ? 1 : 0; }
+  __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
+};
+
+struct OpCountBool {
+  using Acc = SumAcc;
+  __device__ Acc identity() const { return {0}; }
+  __device__ void add(Acc &a, uint8_t v, uint64_t) const { a.s += v != 0; }
+  __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
+};
+
+template <typename T>
+struct OpBits {
+  using Acc = SumAcc;
+  int op;  // 0 and, 1 or, 2 xor
+  __device__ Acc identity() const {
+    return {op == 0 ? (sizeof(T) == 8 ? ~uint64_t(0) : (uint64_t(1) << (8 * sizeof(T))) - 1) : 0};
+  }
+  __device__ void add(Acc &a, T v, uint64_t) const {
+    uint64_t b = to_bits(v);
+    a.s = op == 0 ? (a.s & b) : op == 1 ? (a.s | b) : (a.s ^ b);
+  }
+  __device__ void merge(Acc &a, const Acc &b) const {
+    a.s = op == 0 ? (a.s & b.s) : op == 1 ? (a.s | b.s) : (a.s ^ b.s);
+  }
+};
+
+template <typename T>
+struct OpExtreme {
+  using Acc = ExtAcc;
+  bool want_max;
+  __device__ Acc identity() const { return {0, 0}; }
+  __device__ void add(Acc &a, T v, uint64_t) const {
+    if (!a.has) {
+      a.bits = to_bits(v);
+      a.has = 1;
+    } else {
+      T cur = from_bits<T>(a.bits);
+      a.bits = to_bits(want_max ? max2(cur, v) : min2(cur, v));
+    }
+  }
+  __device__ void merge(Acc &a, const Acc &b) const {
+    if (!b.has) return;
+    add(a, from_bits<T>(b.bits), 0);
+  }
+};
+
+template <typename T>
+struct OpArg {
+  using Acc = ArgAcc;
+  bool want_max;
+  uint64_t base;
+  __device__ Acc identity() const { return {0, kNone, kNone}; }
+  // Whether (v, i) beats (w, j): strictly more extreme, or equally ranked and earlier.
+  __device__ bool beats(T v, int64_t i, T w, int64_t j) const {
+    bool more = want_max ? ord_less(w, v) : ord_less(v, w);
+    bool less = want_max ? ord_less(v, w) : ord_less(w, v);
+    return more || (!less && i < j);
+  }
+  __device__ void add(Acc &a, T v, uint64_t row) const {
+    int64_t i = static_cast<int64_t>(base + row);
+    if (is_nan(v)) {
+      if (i < a.nan) a.nan = i;
+      return;
+    }
+    if (a.idx == kNone || beats(v, i, from_bits<T>(a.bits), a.idx)) {
+      a.bits = to_bits(v);
+      a.idx = i;
+    }
+  }
+  __device__ void merge(Acc &a, const Acc &b) const {
+    if (b.nan < a.nan) a.nan = b.nan;
+    if (b.idx == kNone) return;
+    if (a.idx == kNone || beats(from_bits<T>(b.bits), b.idx, from_bits<T>(a.bits), a.idx)) {
+      a.bits = b.bits;
+      a.idx = b.idx;
+    }
+  }
+};
+
+// ------------------------------------------------------------------- the kernel
+
+template <class Op>
+__device__ __forceinline__ typename Op::Acc block_merge(const Op &op, typename Op::Acc a) {
+  using Acc = typename Op::Acc;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) op.merge(a, shfl_xor_pod(a, m));
+  __shared__ Acc s_w[kBlock / kWave];
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) s_w[w] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / kWave; ++k) op.merge(a, s_w[k]);
+  }
+  return a;  // valid in thread 0
+}
+
+template <typename T, class Op>
+__global__ __launch_bounds__(kBlock) void k_reduce(Op op, const T *__restrict__ x, uint64_t n,
+                                                   typename Op::Acc *partials, uint32_t *done,
+                                                   typename Op::Acc *state) {
+  using Acc = typename Op::Acc;
+  Acc a = op.identity();
+  grid_rows(x, n, [&](T v, uint64_t row) { op.add(a, v, row); });
+  a = block_merge(op, a);
+  __shared__ bool s_last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = a;
+    __threadfence();
+    uint32_t prev = atomicAdd(done, 1u);
+    s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  Acc b = op.identity();
+  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+    Acc p;
+    // Partials of other XCDs: read through the agent-coherent path.
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(&partials[i]);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(&p);
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(sizeof(Acc) / 8); ++k) dst[k] = ld_rlx(src + k);
+    op.merge(b, p);
+  }
+  __syncthreads();
+  b = block_merge(op, b);
+  if (threadIdx.x == 0) {
+    Acc s = *state;
+    op.merge(s, b);
+    *state = s;
+    atomicExch(done, 0u);
+  }
+}
+
+template <typename T, class Op>
+void launch(Ctx &c, const Op &op, const void *x, uint64_t n, void *state) {
+  using Acc = typename Op::Acc;
+  const uint32_t per_block = kBlock * (16 / sizeof(T)) * 4;
+  const uint32_t grid = stream_grid(n, per_block, 2048);
+  Acc *parts = static_cast<Acc *>(partials(c, sizeof(Acc) * grid));
+  k_reduce<T, Op><<<grid, kBlock, 0, c.stream>>>(op, static_cast<const T *>(x), n, parts,
+                                                 c.scratch.counters + 1, static_cast<Acc *>(state));
+  check_hip(hipGetLastError(), "reduce launch");
+}
+
+template <template <typename> class OpT, typename... Extra>
+void by_type(Ctx &c, int32_t type, const void *x, uint64_t n, void *state, Extra... extra) {
+  switch (type) {
+    case VK_F64: launch<double>(c, OpT<double>{extra...}, x, n, state); break;
+    case VK_F32: launch<float>(c, OpT<float>{extra...}, x, n, state); break;
+    case VK_I64: launch<int64_t>(c, OpT<int64_t>{extra...}, x, n, state); break;
+    case VK_U64: launch<uint64_t>(c, OpT<uint64_t>{extra...}, x, n, state); break;
+    case VK_I32: launch<int32_t>(c, OpT<int32_t>{extra...}, x, n, state); break;
+    case VK_U8: case VK_BOOL: launch<uint8_t>(c, OpT<uint8_t>{extra...}, x, n, state); break;
+    default: usage_error("unsupported element type");
+  }
+}
+
+template <typename T>
+OpCount<T> make_count(int cmp, uint64_t cbits) {
+  T c;
+  __builtin_memcpy(&c, &cbits, sizeof(T));
+  return OpCount<T>{cmp, c};
+}
+
+__global__ void k_init_state(uint64_t *s, int words, uint64_t w0, uint64_t w1, uint64_t w2) {
+  if (threadIdx.x == 0) {
+    if (words > 0) s[0] = w0;
+    if (words > 1) s[1] = w1;
+    if (words > 2) s[2] = w2;
+  }
+}
+
+}  // namespace
+
+void reduce_state_init(Ctx &c, const ReduceSpec &s, void *state) {
+  uint64_t *p = static_cast<uint64_t *>(state);
+  switch (s.kind) {
+    case ReduceSpec::BITS: {
+      const uint32_t w = type_width(s.type);
+      uint64_t all = w == 8 ? ~uint64_t(0) : (uint64_t(1) << (8 * w)) - 1;
+      if (s.type == VK_BOOL) all = 1;  // the identity of bool `and` is true
+      k_init_state<<<1, 64, 0, c.stream>>>(p, 1, s.bitop == 0 ? all : 0, 0, 0);
+      break;
+    }
+    case ReduceSpec::EXTREME: k_init_state<<<1, 64, 0, c.stream>>>(p, 2, 0, 0, 0); break;
+    case ReduceSpec::ARG_EXTREME:
+      k_init_state<<<1, 64, 0, c.stream>>>(p, 3, 0, uint64_t(kNone), uint64_t(kNone));
+      break;
+    default: k_init_state<<<1, 64, 0, c.stream>>>(p, 1, 0, 0, 0); break;
+  }
+  check_hip(hipGetLastError(), "reduce init");
+}
+
+void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *state) {
+  if (n == 0) return;
+  switch (s.kind) {
+    case ReduceSpec::SUM_INT:
+      switch (s.type) {
+        case VK_I64: launch<int64_t>(c, OpSum<int64_t>{}, x, n, state); break;
+        case VK_U64: launch<uint64_t>(c, OpSum<uint64_t>{}, x, n, state); break;
+        case VK_I32: launch<int32_t>(c, OpSum<int32_t>{}, x, n, state); break;
+        case VK_U8: launch<uint8_t>(c, OpSum<uint8_t>{}, x, n, state); break;
+        default: usage_error("unsupported integer sum type");
+      }
+      break;
+    case ReduceSpec::COUNT_CMP:
+      switch (s.type) {
+        case VK_F64: launch<double>(c, make_count<double>(s.cmp, s.cbits), x, n, state); break;
+        case VK_F32: launch<float>(c, make_count<float>(s.cmp, s.cbits), x, n, state); break;
+        case VK_I64: launch<int64_t>(c, make_count<int64_t>(s.cmp, s.cbits), x, n, state); break;
+        case VK_U64: launch<uint64_t>(c, make_count<uint64_t>(s.cmp, s.cbits), x, n, state); break;
+        case VK_I32: launch<int32_t>(c, make_count<int32_t>(s.cmp, s.cbits), x, n, state); break;
+        default: usage_error("unsupported count type");
+      }
+      break;
+    case ReduceSpec::COUNT_BOOL: launch<uint8_t>(c, OpCountBool{}, x, n, state); break;
+    case ReduceSpec::EXTREME: by_type<OpExtreme>(c, s.type, x, n, state, s.want_max); break;
+    case ReduceSpec::ARG_EXTREME:
+      by_type<OpArg>(c, s.type, x, n, state, s.want_max, s.base_row);
+      break;
+    case ReduceSpec::BITS:
+      switch (s.type) {
+        case VK_I64: launch<int64_t>(c, OpBits<int64_t>{s.bitop}, x, n, state); break;
+        case VK_U64: launch<uint64_t>(c, OpBits<uint64_t>{s.bitop}, x, n, state); break;
+        case VK_I32: launch<int32_t>(c, OpBits<int32_t>{s.bitop}, x, n, state); break;
+        case VK_U8: case VK_BOOL: launch<uint8_t>(c, OpBits<uint8_t>{s.bitop}, x, n, state); break;
+        default: usage_error("unsupported bit fold type");
+      }
+      break;
+  }
+}
+
+}  // namespace vk

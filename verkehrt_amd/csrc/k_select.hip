@@ -1,0 +1,174 @@
+// Selections: compress_<t> (select.rs:79-92), filter_<cmp>_<t> (fused.rs:212-246) and where
+// (select.rs:94-104). The kept rows keep their row order.
+//
+// One pass, no count pass: each 4096-row tile flags its rows, ranks the kept ones with
+// wave64 ballots and mbcnt, learns the number of kept rows before it by decoupled
+// look-back over the tiles' published counts, and writes its survivors. Algorithmic bytes:
+// n x (width [+1 mask byte]) read, kept x width written.
+#include "vk_device.h"
+#include "vk_internal.h"
+
+namespace vk {
+namespace {
+
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;  // 4096 rows
+constexpr int kWaves = kBlock / kWave;
+
+// Look-back word: epoch (16 bits) | flag (2 bits: 1 aggregate, 2 inclusive) | count (46 bits).
+__device__ __forceinline__ uint64_t pack(uint32_t epoch, uint64_t flag, uint64_t v) {
+  return (uint64_t(epoch) << 48) | (flag << 46) | v;
+}
+constexpr uint64_t kCountMask = (uint64_t(1) << 46) - 1;
+
+// The number of kept rows of all tiles before `tile`, by one wave: lanes read the status of
+// 64 predecessors at once, wait until all are published, add aggregates back to the nearest
+// inclusive prefix.
+__device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile) {
+  uint64_t sum = 0;
+  int64_t end = static_cast<int64_t>(tile);  // exclusive
+  const int l = lane_id();
+  while (end > 0) {
+    const int64_t idx = end - 64 + l;
+    uint64_t w = 0;
+    bool ready;
+    do {
+      w = idx >= 0 ? ld_rlx(&status[idx]) : pack(epoch, 2, 0);
+      ready = (w >> 48) == epoch && ((w >> 46) & 3) != 0;
+      if (__all(ready)) break;
+      backoff();
+    } while (true);
+    const uint64_t incl = __ballot(((w >> 46) & 3) == 2);
+    // lanes at or above the highest inclusive lane contribute
+    const int top = incl ? 63 - __builtin_clzll(incl) : -1;
+    uint64_t v = (l >= top) ? (w & kCountMask) : 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    sum += v;
+    if (top >= 0) break;
+    end -= 64;
+  }
+  return sum;
+}
+
+// kind: 0 compress (keep = mask != 0), 1 filter (keep = x cmp c), 2 where (value base + row).
+template <typename T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_select(const T *__restrict__ x,
+                                                   const uint8_t *__restrict__ mask, uint64_t n,
+                                                   int cmp, T c, uint64_t base, void *out_v,
+                                                   uint64_t *status, uint32_t epoch,
+                                                   uint32_t *ticket, uint32_t ntiles,
+                                                   uint64_t *d_count) {
+  using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
+  Out *out = static_cast<Out *>(out_v);
+  const uint32_t tile = take_ticket(ticket, ntiles);
+  const uint64_t t0 = uint64_t(tile) * kTile;
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+
+  Out val[kItems];
+  uint64_t keepm[kItems];
+  __shared__ uint32_t s_cnt[kItems][kWaves];
+  __shared__ uint32_t s_excl[kItems][kWaves];
+  __shared__ uint64_t s_prefix;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t r = t0 + uint64_t(i) * kBlock + t;
+    bool keep = false;
+    if (r < n) {
+      if (KIND == 0) {
+        keep = mask[r] != 0;
+        val[i] = static_cast<Out>(x[r]);
+      } else if (KIND == 1) {
+        T v = x[r];
+        keep = cmp_test(cmp, v, c);
+        val[i] = static_cast<Out>(v);
+      } else {
+        keep = mask[r] != 0;
+        val[i] = static_cast<Out>(base + r);
+      }
+    }
+    keepm[i] = __ballot(keep);
+    if (l == 0) s_cnt[i][w] = __popcll(keepm[i]);
+  }
+  __syncthreads();
+  // Exclusive scan of the kItems x kWaves counts in row order (item-major), by wave 0.
+  if (w == 0) {
+    const int k = l;  // kItems * kWaves == 64
+    uint32_t v = s_cnt[k / kWaves][k % kWaves];
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t o = __shfl_up(inc, d);
+      if (l >= d) inc += o;
+    }
+    s_excl[k / kWaves][k % kWaves] = inc - v;
+    const uint32_t total = __shfl(inc, 63);
+    // publish and look back
+    uint64_t prefix = 0;
+    if (tile == 0) {
+      if (l == 0) st_rel(&status[0], pack(epoch, 2, total));
+    } else {
+      if (l == 0) st_rel(&status[tile], pack(epoch, 1, total));
+      prefix = look_back(status, epoch, tile);
+      if (l == 0) st_rel(&status[tile], pack(epoch, 2, prefix + total));
+    }
+    if (l == 0) {
+      s_prefix = prefix;
+      if (tile == ntiles - 1) *d_count = prefix + total;
+    }
+  }
+  __syncthreads();
+  const uint64_t prefix = s_prefix;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if ((keepm[i] >> l) & 1) out[prefix + s_excl[i][w] + rank_below(keepm[i])] = val[i];
+  }
+}
+
+template <typename T, int KIND>
+void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uint64_t cbits,
+            uint64_t base, void *out, uint64_t *d_count) {
+  const uint64_t tiles = (n + kTile - 1) / kTile;
+  if (tiles > 0xFFFFFFFFull) usage_error("selection too large");
+  uint64_t *status = status_words(c, tiles);
+  const uint32_t epoch = next_epoch(c);
+  T cv;
+  __builtin_memcpy(&cv, &cbits, sizeof(T));
+  k_select<T, KIND><<<static_cast<uint32_t>(tiles), kBlock, 0, c.stream>>>(
+      static_cast<const T *>(x), mask, n, cmp, cv, base, out, status, epoch, c.scratch.counters,
+      static_cast<uint32_t>(tiles), d_count);
+  check_hip(hipGetLastError(), "select launch");
+}
+
+template <int KIND>
+void by_width(Ctx &c, int32_t type, const void *x, const uint8_t *mask, uint64_t n, int cmp,
+              uint64_t cbits, uint64_t base, void *out, uint64_t *d_count) {
+  switch (type) {
+    case VK_F64: launch<double, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    case VK_F32: launch<float, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    case VK_I64: launch<int64_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    case VK_U64: launch<uint64_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    case VK_I32: launch<int32_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    case VK_U8: case VK_BOOL: launch<uint8_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    default: usage_error("unsupported selection type");
+  }
+}
+
+__global__ void k_zero_count(uint64_t *d) { *d = 0; }
+
+}  // namespace
+
+void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
+                 void *out, uint64_t *d_count) {
+  if (n == 0) {
+    k_zero_count<<<1, 1, 0, c.stream>>>(d_count);
+    return;
+  }
+  switch (s.kind) {
+    case 0: by_width<0>(c, s.type, x, mask, n, 0, 0, 0, out, d_count); break;
+    case 1: by_width<1>(c, s.type, x, nullptr, n, s.cmp, s.cbits, 0, out, d_count); break;
+    default: launch<uint8_t, 2>(c, nullptr, mask, n, 0, 0, s.base, out, d_count); break;
+  }
+}
+
+}  // namespace vk

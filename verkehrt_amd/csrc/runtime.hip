@@ -1,0 +1,205 @@
+// The context, the caching device pool, device array views and per-context scratch.
+#include <cstdio>
+#include <cstring>
+
+#include "vk_device.h"
+#include "vk_internal.h"
+
+namespace vk {
+
+void fail(int code, std::string msg) { throw Error{code, std::move(msg)}; }
+void runtime_error(std::string msg) { fail(VK_ERR_RUNTIME, std::move(msg)); }
+void usage_error(std::string msg) { fail(VK_ERR_USAGE, std::move(msg)); }
+
+void check_hip(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(VK_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const char *type_name(int32_t t) {
+  switch (t) {
+    case VK_F64: return "f64";
+    case VK_F32: return "f32";
+    case VK_I64: return "i64";
+    case VK_U64: return "u64";
+    case VK_I32: return "i32";
+    case VK_U8: return "u8";
+    case VK_BOOL: return "bool";
+    default: return "?";
+  }
+}
+bool valid_type(int32_t t) { return t >= VK_F64 && t <= VK_BOOL; }
+
+// ------------------------------------------------------------------- pool
+
+void *Pool::alloc(uint64_t bytes) {
+  if (bytes == 0) bytes = 256;
+  bytes = (bytes + 255) & ~uint64_t(255);
+  auto it = free_.lower_bound(bytes);
+  if (it != free_.end() && it->first <= bytes + bytes / 4) {
+    void *p = it->second;
+    in_use_ += it->first;
+    free_.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // Give cached blocks back and try once more before reporting.
+    (void)hipGetLastError();
+    trim();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  size_of_[p] = bytes;
+  held_ += bytes;
+  in_use_ += bytes;
+  return p;
+}
+
+void Pool::free(void *p) {
+  if (!p) return;
+  auto it = size_of_.find(p);
+  if (it == size_of_.end()) return;
+  in_use_ -= it->second;
+  free_.emplace(it->second, p);
+}
+
+void Pool::trim() {
+  for (auto &kv : free_) {
+    (void)hipFree(kv.second);
+    held_ -= kv.first;
+    size_of_.erase(kv.second);
+  }
+  free_.clear();
+}
+
+Pool::~Pool() {
+  trim();
+  for (auto &kv : size_of_) (void)hipFree(kv.first);
+}
+
+// ----------------------------------------------------------------- arrays
+
+vk_array *new_array(Ctx &c, int32_t type, uint64_t len) {
+  const uint64_t w = type_width(type);
+  if (len > (uint64_t(1) << 62) / w)
+    runtime_error("array of " + std::to_string(len) + " elements is too large");
+  void *p = c.pool->alloc(len * w);
+  if (!p) runtime_error("cannot allocate an array of " + std::to_string(len) + " elements");
+  auto *b = new Buffer{&c, p, len * w, true, 1};
+  return new vk_array{1, b, 0, len, type};
+}
+
+void release(vk_array *a) {
+  if (!a) return;
+  if (--a->refs > 0) return;
+  Buffer *b = a->buf;
+  if (--b->refs == 0) {
+    if (b->owned) b->ctx->pool->free(b->ptr);
+    delete b;
+  }
+  delete a;
+}
+
+// ---------------------------------------------------------------- scratch
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+void fill_u64(Ctx &c, uint64_t *p, uint64_t n, uint64_t value) {
+  if (n == 0) return;
+  k_fill_u64<<<stream_grid(n, kBlock), kBlock, 0, c.stream>>>(p, n, value);
+  check_hip(hipGetLastError(), "fill");
+}
+
+uint64_t *status_words(Ctx &c, uint64_t words) {
+  if (words > c.scratch.status_words) {
+    uint64_t want = words < 4096 ? 4096 : words + words / 2;
+    if (c.scratch.status) c.pool->free(c.scratch.status);
+    c.scratch.status = static_cast<uint64_t *>(c.pool->alloc(want * 8));
+    if (!c.scratch.status) runtime_error("cannot allocate look-back status words");
+    c.scratch.status_words = want;
+    fill_u64(c, c.scratch.status, want, 0);
+  }
+  return c.scratch.status;
+}
+
+uint32_t next_epoch(Ctx &c) {
+  if (++c.epoch > 0xFFFF) {
+    c.epoch = 1;
+    fill_u64(c, c.scratch.status, c.scratch.status_words, 0);
+  }
+  return c.epoch;
+}
+
+void *partials(Ctx &c, uint64_t bytes) {
+  if (bytes > c.scratch.partial_bytes) {
+    if (c.scratch.partials) c.pool->free(c.scratch.partials);
+    c.scratch.partials = c.pool->alloc(bytes);
+    if (!c.scratch.partials) runtime_error("cannot allocate reduction partials");
+    c.scratch.partial_bytes = bytes;
+  }
+  return c.scratch.partials;
+}
+
+void *tmp_alloc(Ctx &c, uint64_t bytes) {
+  void *p = c.pool->alloc(bytes);
+  if (!p) runtime_error("cannot allocate " + std::to_string(bytes) + " bytes of device scratch");
+  return p;
+}
+void tmp_free(Ctx &c, void *p) { c.pool->free(p); }
+
+void read_back(Ctx &c, const void *src, uint64_t bytes, void *host) {
+  check_hip(hipMemcpyAsync(c.host_slots, src, bytes, hipMemcpyDeviceToHost, c.stream), "read back");
+  check_hip(hipStreamSynchronize(c.stream), "stream sync");
+  std::memcpy(host, c.host_slots, bytes);
+}
+
+void copy_d2d(Ctx &c, void *dst, const void *src, uint64_t bytes) {
+  if (bytes == 0) return;
+  check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
+}
+
+Ctx *create_ctx(int device) {
+  check_hip(hipSetDevice(device), "hipSetDevice");
+  auto *c = new Ctx();
+  c->device = device;
+  check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream create");
+  c->pool = new Pool(device);
+  c->epoch = 0;
+  c->trace = false;
+  c->last_us = 0;
+  check_hip(hipEventCreate(&c->ev0), "event");
+  check_hip(hipEventCreate(&c->ev1), "event");
+  check_hip(hipHostMalloc(reinterpret_cast<void **>(&c->host_slots), 64 * 8, hipHostMallocDefault),
+            "pinned slots");
+  void *counters = c->pool->alloc(256);
+  void *slots = c->pool->alloc(64 * 8);
+  if (!counters || !slots) runtime_error("cannot allocate context scratch");
+  c->scratch.counters = static_cast<uint32_t *>(counters);
+  c->scratch.slots = static_cast<uint64_t *>(slots);
+  fill_u64(*c, static_cast<uint64_t *>(counters), 256 / 8, 0);
+  status_words(*c, 4096);
+  check_hip(hipStreamSynchronize(c->stream), "stream sync");
+  return c;
+}
+
+void destroy_ctx(Ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete c->pool;
+  (void)hipHostFree(c->host_slots);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // namespace vk

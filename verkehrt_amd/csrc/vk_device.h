@@ -1,0 +1,175 @@
+// Device-side helpers shared by the gfx950 kernels: wave64 primitives, agent-scope atomics
+// for look-back, IEEE 754-2019 minimum/maximum, and the row iteration of streaming kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace vk {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Number of lanes below this one whose bit is set in `mask`.
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Agent-scope atomics: visible across the eight XCDs, whose L2s are not coherent.
+__device__ __forceinline__ uint64_t ld_acq(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_rlx(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A dynamic tile id: tiles are numbered in the order their blocks start, so a tile only
+// ever waits on tiles that are already running. The block that draws the last ticket
+// resets the counter for the next launch of the context.
+__device__ __forceinline__ uint32_t take_ticket(uint32_t *ticket, uint32_t ntiles) {
+  __shared__ uint32_t s_tile;
+  if (threadIdx.x == 0) {
+    uint32_t t = atomicAdd(ticket, 1u);
+    if (t == ntiles - 1) atomicExch(ticket, 0u);
+    s_tile = t;
+  }
+  __syncthreads();
+  return s_tile;
+}
+
+__device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(1); }
+
+// IEEE 754-2019 minimum/maximum with explicit compares: NaN propagates, -0.0 < +0.0
+// (src/kernels/scalar/elementwise.rs:403-441). Hardware min/max are not used because their
+// NaN and signed-zero rules differ.
+__device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000000000000ll); }
+__device__ __forceinline__ double min2(double a, double b) {
+  if (a != a || b != b) return qnan();
+  if (a == b) return __signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+__device__ __forceinline__ double max2(double a, double b) {
+  if (a != a || b != b) return qnan();
+  if (a == b) return __signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+__device__ __forceinline__ float min2(float a, float b) {
+  if (a != a || b != b) return __int_as_float(0x7fc00000);
+  if (a == b) return __signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+__device__ __forceinline__ float max2(float a, float b) {
+  if (a != a || b != b) return __int_as_float(0x7fc00000);
+  if (a == b) return __signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+template <typename T>
+__device__ __forceinline__ T min2(T a, T b) { return b < a ? b : a; }
+template <typename T>
+__device__ __forceinline__ T max2(T a, T b) { return a < b ? b : a; }
+
+// Cmp::test (src/kernels/scalar/fused.rs:42-55).
+template <typename T>
+__device__ __forceinline__ bool cmp_test(int cmp, T a, T b) {
+  switch (cmp) {
+    case 0: return a == b;
+    case 1: return a != b;
+    case 2: return a < b;
+    case 3: return a <= b;
+    case 4: return a > b;
+    default: return a >= b;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T from_bits(uint64_t b) {
+  T v;
+  __builtin_memcpy(&v, &b, sizeof(T));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ uint64_t to_bits(T v) {
+  uint64_t b = 0;
+  __builtin_memcpy(&b, &v, sizeof(T));
+  return b;
+}
+
+// 16 bytes of rows, the unit of a 128-bit load.
+template <typename T>
+struct alignas(16) Vec16 {
+  T e[16 / sizeof(T)];
+};
+
+// Calls f(value, row) for every row of x[0..n), rows spread over the whole grid with
+// 128-bit loads on the aligned body and scalar loads on the unaligned head and the tail.
+template <typename T, typename F>
+__device__ __forceinline__ void grid_rows(const T *__restrict__ x, uint64_t n, F &&f) {
+  constexpr int V = 16 / sizeof(T);
+  const uint64_t tid = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const uint64_t nth = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t head = ((16 - (reinterpret_cast<uintptr_t>(x) & 15)) & 15) / sizeof(T);
+  if (head > n) head = n;
+  if (tid < head) f(x[tid], tid);
+  const uint64_t nv = (n - head) / V;
+  const Vec16<T> *xv = reinterpret_cast<const Vec16<T> *>(x + head);
+  uint64_t i = tid;
+  for (; i + 3 * nth < nv; i += 4 * nth) {
+    Vec16<T> v0 = xv[i], v1 = xv[i + nth], v2 = xv[i + 2 * nth], v3 = xv[i + 3 * nth];
+#pragma unroll
+    for (int k = 0; k < V; ++k) f(v0.e[k], head + i * V + k);
+#pragma unroll
+    for (int k = 0; k < V; ++k) f(v1.e[k], head + (i + nth) * V + k);
+#pragma unroll
+    for (int k = 0; k < V; ++k) f(v2.e[k], head + (i + 2 * nth) * V + k);
+#pragma unroll
+    for (int k = 0; k < V; ++k) f(v3.e[k], head + (i + 3 * nth) * V + k);
+  }
+  for (; i < nv; i += nth) {
+    Vec16<T> v = xv[i];
+#pragma unroll
+    for (int k = 0; k < V; ++k) f(v.e[k], head + i * V + k);
+  }
+  for (uint64_t r = head + nv * V + tid; r < n; r += nth) f(x[r], r);
+}
+
+// Shuffles a trivially copyable value word by word.
+template <typename T>
+__device__ __forceinline__ T shfl_xor_pod(const T &v, int mask) {
+  static_assert(sizeof(T) % 4 == 0, "pod shuffle needs whole words");
+  T out;
+  const int *src = reinterpret_cast<const int *>(&v);
+  int *dst = reinterpret_cast<int *>(&out);
+#pragma unroll
+  for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __shfl_xor(src[w], mask);
+  return out;
+}
+template <typename T>
+__device__ __forceinline__ T shfl_up_pod(const T &v, int delta) {
+  T out;
+  const int *src = reinterpret_cast<const int *>(&v);
+  int *dst = reinterpret_cast<int *>(&out);
+#pragma unroll
+  for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __shfl_up(src[w], delta);
+  return out;
+}
+template <typename T>
+__device__ __forceinline__ T shfl_pod(const T &v, int src_lane) {
+  T out;
+  const int *src = reinterpret_cast<const int *>(&v);
+  int *dst = reinterpret_cast<int *>(&out);
+#pragma unroll
+  for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __shfl(src[w], src_lane);
+  return out;
+}
+
+}  // namespace vk

@@ -1,0 +1,238 @@
+// Internal declarations of the hip backend: the context, device buffers and array views,
+// the error type, and the host launchers of every kernel family. Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vk_hip.h"
+
+namespace vk {
+
+// An error that crosses the C ABI as a vk_status: `code` is VK_ERR_*, `msg` the text.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, std::string msg);
+[[noreturn]] void runtime_error(std::string msg);  // VK_ERR_RUNTIME, reference text
+[[noreturn]] void usage_error(std::string msg);    // VK_ERR_USAGE, reference text
+void check_hip(hipError_t e, const char *what);
+
+inline uint32_t type_width(int32_t t) {
+  switch (t) {
+    case VK_F64: case VK_I64: case VK_U64: return 8;
+    case VK_F32: case VK_I32: return 4;
+    default: return 1;
+  }
+}
+const char *type_name(int32_t t);  // ScalarType::name, src/value.rs:57-71
+bool valid_type(int32_t t);
+
+// A caching device allocator for one context. Freed blocks are kept by size and reused
+// for requests of at most 1.25x their size; all reuse is ordered by the context's single
+// stream, so a block freed after its last use was enqueued is safe to hand out again.
+class Pool {
+ public:
+  explicit Pool(int device) : device_(device) {}
+  void *alloc(uint64_t bytes);
+  void free(void *p);
+  void trim();
+  uint64_t held() const { return held_; }
+  uint64_t in_use() const { return in_use_; }
+  ~Pool();
+
+ private:
+  int device_;
+  std::multimap<uint64_t, void *> free_;
+  std::map<void *, uint64_t> size_of_;
+  uint64_t held_ = 0, in_use_ = 0;
+};
+
+}  // namespace vk
+
+struct vk_ctx;
+
+namespace vk {
+
+// A device allocation shared by array views. `owned` buffers return to the pool.
+struct Buffer {
+  vk_ctx *ctx;
+  void *ptr;
+  uint64_t bytes;
+  bool owned;
+  int refs;
+};
+
+}  // namespace vk
+
+struct vk_array {
+  int refs;
+  vk::Buffer *buf;
+  uint64_t offset;  // bytes from buf->ptr to row 0
+  uint64_t len;
+  int32_t type;
+};
+
+namespace vk {
+
+// Per-context device scratch reused by every launch of the context: look-back status
+// words, tickets, done counters and small result slots. Grows on demand (between calls).
+struct Scratch {
+  uint64_t *status = nullptr;  // look-back words
+  uint64_t status_words = 0;
+  uint32_t *counters = nullptr;  // [0] ticket, [1] done, [2..] spare; always returned to 0
+  uint64_t *slots = nullptr;     // 64 x 8 bytes of small device results
+  void *partials = nullptr;      // per-block partial results
+  uint64_t partial_bytes = 0;
+};
+
+}  // namespace vk
+
+struct vk_ctx {
+  int device;
+  hipStream_t stream;
+  vk::Pool *pool;
+  vk::Scratch scratch;
+  uint32_t epoch;  // look-back epoch, 1..65535
+  uint64_t *host_slots;  // pinned 64 x 8 bytes for small D2H results
+  bool trace;
+  std::string trace_buf;
+  double last_us;
+  hipEvent_t ev0, ev1;
+};
+
+namespace vk {
+
+using Ctx = vk_ctx;
+
+// Arrays ------------------------------------------------------------------
+vk_array *new_array(Ctx &c, int32_t type, uint64_t len);  // uninitialised
+inline void *dptr(const vk_array *a) { return static_cast<char *>(a->buf->ptr) + a->offset; }
+inline uint64_t abytes(const vk_array *a) { return a->len * type_width(a->type); }
+void release(vk_array *a);
+
+// Scratch -----------------------------------------------------------------
+uint64_t *status_words(Ctx &c, uint64_t words);  // zeroed at growth
+uint32_t next_epoch(Ctx &c);                     // clears status on wrap
+void *partials(Ctx &c, uint64_t bytes);
+void *tmp_alloc(Ctx &c, uint64_t bytes);  // pool memory for one call
+void tmp_free(Ctx &c, void *p);
+// Copies `bytes` of device memory at `src` to the pinned host slots and waits.
+void read_back(Ctx &c, const void *src, uint64_t bytes, void *host);
+void fill_u64(Ctx &c, uint64_t *p, uint64_t n, uint64_t value);
+
+// Launch geometry -----------------------------------------------------------
+constexpr int kBlock = 256;
+inline uint32_t stream_grid(uint64_t work_items, uint32_t per_block, uint32_t cap = 2048) {
+  uint64_t g = (work_items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<uint32_t>(g);
+}
+
+// Comparison codes in the order of CMPS (src/kernels/scalar/fused.rs:33-40).
+enum Cmp { CMP_EQ = 0, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE };
+
+// ---------------------------------------------------------------- launchers
+// All launchers enqueue on c.stream. Device pointers are rows of the named type.
+
+// Streaming reductions (k_reduce.hip). `state` is the reducer's device state; each
+// launch merges the reduction of its rows into it.
+struct ReduceSpec {
+  enum Kind { SUM_INT, COUNT_CMP, COUNT_BOOL, EXTREME, ARG_EXTREME, BITS } kind;
+  int32_t type;       // element type of x
+  int cmp;            // COUNT_CMP
+  uint64_t cbits;     // COUNT_CMP constant bits
+  bool want_max;      // EXTREME / ARG_EXTREME
+  int bitop;          // BITS: 0 and, 1 or, 2 xor
+  uint64_t base_row;  // ARG_EXTREME: rows consumed before this update
+};
+// Reduce state layouts (device): SUM_INT/COUNT/BITS: u64; EXTREME: {u64 bits, u64 has};
+// ARG_EXTREME: {u64 best_bits, i64 best_idx, i64 nan_idx}.
+void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *state);
+void reduce_state_init(Ctx &c, const ReduceSpec &s, void *state);
+
+// Canonical float reductions (k_canon.hip). kind: 0 sum, 1 dot, 2 sumsq, 3 condsum, 4 mul.
+// `acc` is 16 doubles; blocks of 16 rows fold into them in the canonical order.
+struct CanonSpec {
+  int kind;
+  int cmp;
+  double c, e;
+};
+void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t blocks,
+                  double *acc);
+// Writes the row values of rows [0, rest) (rest < 16) to `tail`.
+void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t rest,
+                double *tail);
+void canon_finish(Ctx &c, bool mul, const double *acc, const double *tail, int tail_len,
+                  double *out);
+void canon_init(Ctx &c, bool mul, double *acc);
+
+// Selections (k_select.hip): append the kept rows at `out`, the count to *d_count (device).
+// kind 0 compress (mask), 1 filter (cmp with cbits), 2 where (base).
+struct SelectSpec {
+  int kind;
+  int32_t type;  // row type (compress/filter); where writes i64
+  int cmp;
+  uint64_t cbits;
+  uint64_t base;
+};
+void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
+                 void *out, uint64_t *d_count);
+
+// Scans (k_scan.hip). `carry` is the scanner's device state {u64 acc, u64 has}.
+// op 0 add (out width 8, input widened from `type`), 1 min, 2 max.
+struct ScanSpec {
+  int op;
+  int32_t type;
+  bool by;
+};
+void scan_rows(Ctx &c, const ScanSpec &s, const void *x, const uint8_t *starts, uint64_t n,
+               void *out, uint64_t *carry);
+void scan_add_f64(Ctx &c, const double *x, uint64_t n, double *out, double *state);
+
+// Gather (k_gather.hip): out[i] = xs[idx[i]]; the first bad row (or ~0) to *d_bad.
+void gather_rows(Ctx &c, uint32_t width, const void *xs, uint64_t xlen, const int64_t *idx,
+                 uint64_t n, void *out, uint64_t *d_bad);
+
+// Buckets (k_bucket.hip): histogram / reduce_into; first bad key row to *d_bad.
+// op: 0 count (histogram), 1 add i64, 2 min, 3 max (type f64 or i64).
+void bucket_init(Ctx &c, int op, int32_t type, void *out, uint64_t buckets);
+void bucket_rows(Ctx &c, int op, int32_t type, const int64_t *keys, const void *vals, uint64_t n,
+                 uint64_t buckets, void *out, uint64_t *d_bad);
+void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
+                         uint64_t buckets, double *out);
+void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint64_t *d_bad);
+
+// Argsort (k_sort.hip): stable ascending positions as i64.
+void argsort(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out);
+// Stable sort of u64 keys: sorted keys and the positions they came from.
+void sort_u64_pairs(Ctx &c, const uint64_t *keys, uint64_t n, uint64_t *sorted_keys, int64_t *perm);
+
+// group_id (k_group.hip): first-occurrence ids. `b` may be null (one key column).
+void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
+              int64_t *out);
+
+// Construction and elementwise (k_misc.hip).
+void rand_f64(Ctx &c, uint64_t n, uint64_t seed, double *out);
+void rand_i64(Ctx &c, uint64_t n, uint64_t seed, int64_t *out);
+void iota(Ctx &c, uint64_t n, int64_t *out);
+void fill_bits(Ctx &c, uint32_t width, uint64_t n, uint64_t bits, void *out);
+// Elementwise binary op over rows/splats: op codes in k_misc.hip.
+struct EwSpec {
+  int op;
+  int32_t in_type;
+  bool a_splat, b_splat;
+  uint64_t a_bits, b_bits;
+};
+void elementwise(Ctx &c, const EwSpec &s, const void *a, const void *b, uint64_t n, void *out);
+void copy_d2d(Ctx &c, void *dst, const void *src, uint64_t bytes);
+
+}  // namespace vk
