@@ -45,6 +45,14 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host-side model of the exact canonical fold, loaded only by the tests
+    model = os.path.join(LIB_DIR, "libvk_model.so")
+    cmd = [HIPCC, "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+           os.path.join(CSRC, "model", "canon_model.cpp"), "-o", model + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"model build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(model + ".tmp", model)
     if verbose:
         print(LIB)
     return LIB

@@ -1,0 +1,166 @@
+// Exact parallel evaluation of one canonical accumulator chain, S <- fl(S + v) in index
+// order (the chain of accumulator j of src/kernels/scalar/reduce.rs:40-62), without changing
+// a single bit of the sequential result.
+//
+// The fact used: while the running value S and every exact sum S + v stay inside one binade
+// [2^e, 2^(e+1)) of fixed sign, every representable value there is an integer multiple of
+// u = 2^(e-52), so fl(S + v) = S + u * r(v, parity), where r rounds v/u to an integer, ties
+// to the even result. r depends on S only through the parity of S/u. A run of elements is
+// therefore a function of one bit: for each start parity p it adds D_p units of u, and its
+// running offset stays within [lo_p, hi_p]. These functions compose associatively, so runs
+// are summarised in parallel and joined in order; a start value S then checks the summary
+// (S is in the binade, and S + lo_p, S + hi_p stay strictly inside it) and, when the check
+// holds, the sequential result is exactly S + u * D_p. When it fails (a binade crossing, an
+// element too large, NaN or infinity, a start of zero or a subnormal) the caller folds that
+// run sequentially instead, so the result is always the sequential one.
+//
+// r(v, p) is computed with IEEE adds from reference points inside the binade: Z0 = +-1.5 * 2^e
+// (an even multiple of u) and Z1 = Z0 + u (odd), r(v, p) = (fl(Zp + v) - Zp) / u, exact by
+// Sterbenz whenever |v| < 2^(e-1). Offsets are integers held exactly in doubles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vk {
+
+// A binade hypothesis: exponent e of the running value and its sign. `ok` false means no
+// hypothesis (the run is folded sequentially).
+struct Binade {
+  int e;
+  int neg;
+  int ok;
+  int pad;
+};
+
+// The summary of a run under one hypothesis, in units of u, for start parities 0 and 1.
+struct RunFn {
+  double d[2];   // offset after the run
+  double lo[2];  // least offset after any element of the run (+inf for an empty run)
+  double hi[2];  // greatest offset after any element (-inf for an empty run)
+  double bad;    // nonzero when some element was too large for the binade, NaN or infinite
+  double pad;
+};
+
+__host__ __device__ __forceinline__ double pow2(int k) {
+  // 2^k for -1022 <= k <= 1023, built from its bits
+  const uint64_t b = static_cast<uint64_t>(k + 1023) << 52;
+  double r;
+  __builtin_memcpy(&r, &b, 8);
+  return r;
+}
+
+__host__ __device__ __forceinline__ uint64_t dbits_(double v) {
+  uint64_t b;
+  __builtin_memcpy(&b, &v, 8);
+  return b;
+}
+
+// The hypothesis of a running value near `approx`; none for zero, subnormal, non-finite or
+// extreme magnitudes (where u or 1/u would leave the normal range).
+__host__ __device__ __forceinline__ Binade binade_of(double approx) {
+  Binade h{0, 0, 0, 0};
+  const uint64_t b = dbits_(approx);
+  const int ex = static_cast<int>((b >> 52) & 0x7FF);
+  if (ex == 0 || ex == 0x7FF) return h;
+  const int e = ex - 1023;
+  if (e < -960 || e > 960) return h;
+  h.e = e;
+  h.neg = static_cast<int>(b >> 63);
+  h.ok = 1;
+  return h;
+}
+
+__host__ __device__ __forceinline__ bool same_binade(const Binade &a, const Binade &b) {
+  return a.ok && b.ok && a.e == b.e && a.neg == b.neg;
+}
+
+__host__ __device__ __forceinline__ RunFn run_identity() {
+  const double inf = __builtin_inf();
+  return RunFn{{0.0, 0.0}, {inf, inf}, {-inf, -inf}, 0.0, 0.0};
+}
+
+// Parity of an integer-valued double t: 1 when t is odd.
+__host__ __device__ __forceinline__ int parity(double t) {
+  const double h = t * 0.5;
+  return h != __builtin_floor(h) ? 1 : 0;
+}
+
+// Per-run constants of a hypothesis.
+struct BinadeRef {
+  double z0, z1;   // reference points, even and odd multiples of u
+  double inv_u;    // 1/u
+  double lim;      // elements must satisfy |v| < lim = 2^(e-1)
+};
+
+__host__ __device__ __forceinline__ BinadeRef binade_ref(const Binade &h) {
+  const double two_e = pow2(h.e);
+  const double u = pow2(h.e - 52);
+  const double z0 = (h.neg ? -1.5 : 1.5) * two_e;
+  return BinadeRef{z0, z0 + u, pow2(52 - h.e), 0.5 * two_e};
+}
+
+// Appends one element v to the run summary f.
+__host__ __device__ __forceinline__ void run_push(RunFn &f, const BinadeRef &r, double v) {
+  const double a0 = ((r.z0 + v) - r.z0) * r.inv_u;
+  const double a1 = ((r.z1 + v) - r.z1) * r.inv_u;
+  const double av = v < 0 ? -v : v;
+  if (!(av < r.lim)) f.bad = 1.0;
+  // track p: the current parity is p xor parity(offset)
+  {
+    const double d = parity(f.d[0]) ? a1 : a0;
+    f.d[0] = f.d[0] + d;
+    f.lo[0] = f.d[0] < f.lo[0] ? f.d[0] : f.lo[0];
+    f.hi[0] = f.d[0] > f.hi[0] ? f.d[0] : f.hi[0];
+  }
+  {
+    const double d = parity(f.d[1]) ? a0 : a1;
+    f.d[1] = f.d[1] + d;
+    f.lo[1] = f.d[1] < f.lo[1] ? f.d[1] : f.lo[1];
+    f.hi[1] = f.d[1] > f.hi[1] ? f.d[1] : f.hi[1];
+  }
+}
+
+// The summary of run a followed by run b (both under the same hypothesis).
+__host__ __device__ __forceinline__ RunFn run_join(const RunFn &a, const RunFn &b) {
+  RunFn o;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int q = p ^ parity(a.d[p]);
+    o.d[p] = a.d[p] + b.d[q];
+    const double blo = a.d[p] + b.lo[q], bhi = a.d[p] + b.hi[q];
+    o.lo[p] = a.lo[p] < blo ? a.lo[p] : blo;
+    o.hi[p] = a.hi[p] > bhi ? a.hi[p] : bhi;
+  }
+  o.bad = (a.bad != 0.0 || b.bad != 0.0) ? 1.0 : 0.0;
+  o.pad = 0.0;
+  return o;
+}
+
+// Applies run f (under hypothesis h) to the exact start value s. Returns true and the exact
+// end value in *out when the summary proves the sequential fold equals it.
+__host__ __device__ __forceinline__ bool run_apply(const RunFn &f, const Binade &h, double s,
+                                                   double *out) {
+  if (!h.ok || f.bad != 0.0) return false;
+  const Binade hs = binade_of(s);
+  if (!same_binade(hs, h)) return false;
+  const double inv_u = pow2(52 - h.e);
+  const double k = s * inv_u;  // exact integer, 2^52 <= |k| < 2^53
+  const int p = parity(k);
+  const double two52 = 4503599627370496.0, two53 = 9007199254740992.0;
+  const double lo = k + f.lo[p], hi = k + f.hi[p];
+  bool ok;
+  if (!h.neg) ok = lo >= two52 + 1.0 && hi <= two53 - 1.0;
+  else ok = hi <= -(two52 + 1.0) && lo >= -(two53 - 1.0);
+  // an empty run (lo = +inf, hi = -inf) leaves s unchanged
+  if (f.lo[p] > f.hi[p]) {
+    *out = s;
+    return true;
+  }
+  if (!ok) return false;
+  *out = (k + f.d[p]) * pow2(h.e - 52);
+  return true;
+}
+
+}  // namespace vk

@@ -3,8 +3,11 @@
 // (fused.rs:72-210). Accumulator j folds rows j, j+16, ... in index order with plain IEEE
 // adds (or multiplies); no FMA anywhere (built with -ffp-contract=off).
 //
-// This file holds the sequential form: one wave, lane j < 16 owning accumulator j. Its
-// bound is the dependent-add latency, n/16 adds in a row, not HBM bandwidth.
+// Two forms. The sequential form is one wave with lane j < 16 owning accumulator j; its bound
+// is the dependent-add latency, n/16 adds in a row. The fast form (sums only) evaluates the
+// same 16 chains in parallel with binade run summaries (canon_binade.h) and returns the same
+// bits; it is HBM-bound on data whose running sums stay inside a binade for long stretches.
+#include "canon_binade.h"
 #include "vk_device.h"
 #include "vk_internal.h"
 
@@ -63,6 +66,213 @@ __global__ __launch_bounds__(64) void k_canon_chain(RowVal rv, bool mul, const d
   if (l < 16) acc[l] = a;
 }
 
+// ------------------------------------------------------------------ fast form
+// The 16 chains over tiles of 256 blocks (4096 rows), one pass (canon_binade.h):
+//  1. thread (chain j, run r) loads the 16 blocks 16r..16r+15 of its chain (4 x 128-byte
+//     segments per wave load) and keeps the row values in LDS for a sequential fallback;
+//  2. per-chain approximate sums, and a decoupled look-back over them, give each chain an
+//     approximate start and so a binade hypothesis;
+//  3. every thread summarises its run under the hypothesis, runs join in order (two
+//     shuffles, then across the four waves) into the tile summary;
+//  4. a second decoupled look-back joins the summaries of preceding tiles with the same
+//     hypothesis back to a tile whose exact end values are published, applies the joined
+//     summary to that exact value, and publishes this tile's exact end values. A failed
+//     check waits for the previous tile's exact values and folds the tile's rows of the
+//     chain sequentially from LDS.
+constexpr int kTileBlocks = 256;
+constexpr uint64_t kFastMinBlocks = 512;
+constexpr int kStride = 192;  // status words per tile
+constexpr int W_AAGG = 1, W_AINC = 17, W_F = 33, W_EXACT = 161;
+constexpr uint64_t F_AAGG = 1, F_AINC = 2, F_FN = 4, F_EXACT = 8;
+
+__device__ __forceinline__ uint64_t flag_word(uint32_t epoch, uint64_t bits) {
+  return (uint64_t(epoch) << 8) | bits;
+}
+__device__ __forceinline__ uint64_t wait_flag(const uint64_t *st, uint32_t epoch, uint64_t need) {
+  while (true) {
+    const uint64_t w = ld_acq(st);
+    if ((w >> 8) == epoch && (w & need)) return w;
+    backoff();
+  }
+}
+__device__ __forceinline__ double ld_d(const uint64_t *p) {
+  return __longlong_as_double(static_cast<long long>(ld_rlx(p)));
+}
+__device__ __forceinline__ void st_d(uint64_t *p, double v) {
+  st_rlx(p, static_cast<uint64_t>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ uint64_t pack_binade(const Binade &h) {
+  return (static_cast<uint64_t>(static_cast<uint32_t>(h.e + 2048)) << 2) | (uint64_t(h.neg) << 1) |
+         uint64_t(h.ok);
+}
+__device__ __forceinline__ Binade unpack_binade(uint64_t w) {
+  return Binade{static_cast<int>(w >> 2) - 2048, static_cast<int>((w >> 1) & 1), static_cast<int>(w & 1), 0};
+}
+
+__global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *__restrict__ x,
+                                                       const double *__restrict__ y, uint64_t blocks,
+                                                       const double *acc_in, double *acc_out,
+                                                       uint64_t *status, uint32_t epoch,
+                                                       uint32_t *ticket, uint32_t ntiles) {
+  __shared__ double s_val[kTileBlocks * 16];
+  __shared__ double s_a[kBlock / kWave][16];
+  __shared__ RunFn s_f[kBlock / kWave][16];
+  __shared__ Binade s_h[16];
+  const uint32_t tile = take_ticket(ticket, ntiles);
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  const int j = t & 15, r = t >> 4;
+  const uint64_t b0 = uint64_t(tile) * kTileBlocks;
+  const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
+  uint64_t *st = status + uint64_t(tile) * kStride;
+
+  // 1. load the run
+  double v[16];
+  double a = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int lb = r * 16 + k;
+    if (lb < nb) {
+      v[k] = rv(x, y, (b0 + lb) * 16 + j);
+      s_val[lb * 16 + j] = v[k];
+      a = a + v[k];
+    } else {
+      v[k] = 0.0;
+    }
+  }
+  // 2. approximate chain sums of the tile
+  a = a + __shfl_xor(a, 16);
+  a = a + __shfl_xor(a, 32);
+  if (l < 16) s_a[w][j] = a;
+  __syncthreads();
+  if (w == 0) {
+    double atile = 0.0, pre = 0.0;
+    if (l < 16) {
+      atile = s_a[0][l] + s_a[1][l] + s_a[2][l] + s_a[3][l];
+      if (tile == 0) {
+        pre = acc_in[l];
+        st_d(st + W_AINC + l, pre + atile);
+      } else {
+        st_d(st + W_AAGG + l, atile);
+      }
+    }
+    __threadfence();
+    if (l == 0) st_rel(st, flag_word(epoch, tile == 0 ? (F_AAGG | F_AINC) : F_AAGG));
+    if (tile > 0 && l < 16) {
+      double sum = 0.0;
+      for (int64_t k = int64_t(tile) - 1;; --k) {
+        const uint64_t *sk = status + uint64_t(k) * kStride;
+        const uint64_t fw = wait_flag(sk, epoch, F_AAGG | F_AINC | F_EXACT);
+        if (fw & F_EXACT) {
+          sum = sum + ld_d(sk + W_EXACT + l);
+          break;
+        }
+        if (fw & F_AINC) {
+          sum = sum + ld_d(sk + W_AINC + l);
+          break;
+        }
+        sum = sum + ld_d(sk + W_AAGG + l);
+      }
+      pre = sum;
+      st_d(st + W_AINC + l, pre + atile);
+    }
+    if (tile > 0) {
+      __threadfence();
+      if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC));
+    }
+    if (l < 16) s_h[l] = binade_of(pre);
+  }
+  __syncthreads();
+  // 3. summarise the run under the chain's hypothesis, join the runs in order
+  const Binade h = s_h[j];
+  RunFn f = run_identity();
+  if (h.ok) {
+    const BinadeRef ref = binade_ref(h);
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (r * 16 + k < nb) run_push(f, ref, v[k]);
+  }
+  {
+    RunFn o = shfl_xor_pod(f, 16);
+    f = (l & 16) ? run_join(o, f) : run_join(f, o);
+    o = shfl_xor_pod(f, 32);
+    f = (l & 32) ? run_join(o, f) : run_join(f, o);
+  }
+  if (l < 16) s_f[w][j] = f;
+  __syncthreads();
+  if (w != 0) return;
+  // 4. publish the tile summary, then the exact look-back (lane = chain)
+  RunFn ft = run_identity();
+  Binade hj{0, 0, 0, 0};
+  if (l < 16) {
+    hj = s_h[l];
+    ft = run_join(run_join(s_f[0][l], s_f[1][l]), run_join(s_f[2][l], s_f[3][l]));
+    uint64_t *fp = st + W_F + 8 * l;
+    st_d(fp + 0, ft.d[0]);
+    st_d(fp + 1, ft.d[1]);
+    st_d(fp + 2, ft.lo[0]);
+    st_d(fp + 3, ft.lo[1]);
+    st_d(fp + 4, ft.hi[0]);
+    st_d(fp + 5, ft.hi[1]);
+    st_d(fp + 6, ft.bad);
+    st_rlx(fp + 7, pack_binade(hj));
+  }
+  __threadfence();
+  if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC | F_FN));
+  double exact = 0.0;
+  if (l < 16) {
+    RunFn g = run_identity();  // joined summaries of the tiles between `base` and this one
+    double base = 0.0;
+    int64_t k = int64_t(tile) - 1;
+    for (;; --k) {
+      if (k < 0) {
+        base = acc_in[l];
+        break;
+      }
+      const uint64_t *sk = status + uint64_t(k) * kStride;
+      const uint64_t fw = wait_flag(sk, epoch, F_FN | F_EXACT);
+      if (fw & F_EXACT) {
+        base = ld_d(sk + W_EXACT + l);
+        break;
+      }
+      const uint64_t *fp = sk + W_F + 8 * l;
+      const Binade hk = unpack_binade(ld_rlx(fp + 7));
+      if (!same_binade(hk, hj)) {
+        wait_flag(sk, epoch, F_EXACT);
+        base = ld_d(sk + W_EXACT + l);
+        break;
+      }
+      RunFn fk;
+      fk.d[0] = ld_d(fp + 0);
+      fk.d[1] = ld_d(fp + 1);
+      fk.lo[0] = ld_d(fp + 2);
+      fk.lo[1] = ld_d(fp + 3);
+      fk.hi[0] = ld_d(fp + 4);
+      fk.hi[1] = ld_d(fp + 5);
+      fk.bad = ld_d(fp + 6);
+      fk.pad = 0.0;
+      g = run_join(fk, g);
+    }
+    bool done = run_apply(run_join(g, ft), hj, base, &exact);
+    if (!done) {
+      if (k != int64_t(tile) - 1 && tile > 0) {
+        const uint64_t *sp = status + uint64_t(tile - 1) * kStride;
+        wait_flag(sp, epoch, F_EXACT);
+        base = ld_d(sp + W_EXACT + l);
+        done = run_apply(ft, hj, base, &exact);
+      }
+      if (!done) {
+        double s = base;
+        for (int lb = 0; lb < nb; ++lb) s = s + s_val[lb * 16 + l];
+        exact = s;
+      }
+    }
+    st_d(st + W_EXACT + l, exact);
+    if (tile == ntiles - 1) acc_out[l] = exact;
+  }
+  __threadfence();
+  if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC | F_FN | F_EXACT));
+}
+
 __global__ void k_canon_rest(RowVal rv, const double *x, const double *y, uint64_t rest,
                              double *tail) {
   const uint64_t i = threadIdx.x;
@@ -97,8 +307,20 @@ void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, 
                   double *acc) {
   if (blocks == 0) return;
   RowVal rv{s.kind, s.cmp, s.c, s.e};
-  k_canon_chain<<<1, 64, 0, c.stream>>>(rv, s.kind == 4, x, y ? y : x, blocks, acc);
-  check_hip(hipGetLastError(), "canon chain");
+  if (s.kind == 4 || blocks < kFastMinBlocks) {
+    // products have no binade form; short inputs are cheaper as one sequential wave
+    k_canon_chain<<<1, 64, 0, c.stream>>>(rv, s.kind == 4, x, y ? y : x, blocks, acc);
+    check_hip(hipGetLastError(), "canon chain");
+    return;
+  }
+  const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
+  uint64_t *status = status_words(c, ST_CANON, tiles * kStride);
+  const uint32_t epoch = next_epoch(c);
+  double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
+  copy_d2d(c, acc_in, acc, 16 * 8);
+  k_canon_fast<<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y ? y : x, blocks, acc_in, acc, status,
+                                                        epoch, c.scratch.counters, uint32_t(tiles));
+  check_hip(hipGetLastError(), "canon fast");
 }
 
 void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t rest,

@@ -218,7 +218,7 @@ template <typename TIn, typename U, int OP>
 void launch(Ctx &c, bool by, const void *x, const uint8_t *starts, uint64_t n, void *out,
             uint64_t *carry) {
   const uint64_t tiles = (n + kTile - 1) / kTile;
-  uint64_t *status = status_words(c, tiles * 4);
+  uint64_t *status = status_words(c, ST_SCAN, tiles * 4);
   const uint32_t epoch = next_epoch(c);
   uint64_t *carry_in = c.scratch.slots + 8;
   copy_d2d(c, carry_in, carry, 16);

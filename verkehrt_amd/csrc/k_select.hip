@@ -130,7 +130,7 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
             uint64_t base, void *out, uint64_t *d_count) {
   const uint64_t tiles = (n + kTile - 1) / kTile;
   if (tiles > 0xFFFFFFFFull) usage_error("selection too large");
-  uint64_t *status = status_words(c, tiles);
+  uint64_t *status = status_words(c, ST_SELECT, tiles);
   const uint32_t epoch = next_epoch(c);
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));

@@ -217,7 +217,7 @@ template <typename U, bool FIRST, bool LAST>
 void run_pass(Ctx &c, int32_t type, const void *kin, const uint32_t *pin, uint64_t n, int shift,
               const uint32_t *dbase, U *kout, uint32_t *pout, int64_t *out_pos) {
   const uint64_t tiles = (n + kTile - 1) / kTile;
-  uint64_t *status = status_words(c, tiles * kRadix);
+  uint64_t *status = status_words(c, ST_SORT, tiles * kRadix);
   const uint32_t epoch = next_epoch(c);
   k_pass<U, FIRST, LAST><<<uint32_t(tiles), kBlock, 0, c.stream>>>(
       type, kin, pin, n, shift, dbase, kout, pout, out_pos, status, epoch, c.scratch.counters,

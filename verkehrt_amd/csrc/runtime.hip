@@ -118,22 +118,23 @@ void fill_u64(Ctx &c, uint64_t *p, uint64_t n, uint64_t value) {
   check_hip(hipGetLastError(), "fill");
 }
 
-uint64_t *status_words(Ctx &c, uint64_t words) {
-  if (words > c.scratch.status_words) {
-    uint64_t want = words < 4096 ? 4096 : words + words / 2;
-    if (c.scratch.status) c.pool->free(c.scratch.status);
-    c.scratch.status = static_cast<uint64_t *>(c.pool->alloc(want * 8));
-    if (!c.scratch.status) runtime_error("cannot allocate look-back status words");
-    c.scratch.status_words = want;
-    fill_u64(c, c.scratch.status, want, 0);
+uint64_t *status_words(Ctx &c, int family, uint64_t words) {
+  if (words > c.scratch.status_words[family]) {
+    uint64_t want = words < 4096 ? 4096 : words + words / 4;
+    if (c.scratch.status[family]) c.pool->free(c.scratch.status[family]);
+    c.scratch.status[family] = static_cast<uint64_t *>(c.pool->alloc(want * 8));
+    if (!c.scratch.status[family]) runtime_error("cannot allocate look-back status words");
+    c.scratch.status_words[family] = want;
+    fill_u64(c, c.scratch.status[family], want, 0);
   }
-  return c.scratch.status;
+  return c.scratch.status[family];
 }
 
 uint32_t next_epoch(Ctx &c) {
   if (++c.epoch > 0xFFFF) {
     c.epoch = 1;
-    fill_u64(c, c.scratch.status, c.scratch.status_words, 0);
+    for (int f = 0; f < ST_FAMILIES; ++f)
+      if (c.scratch.status[f]) fill_u64(c, c.scratch.status[f], c.scratch.status_words[f], 0);
   }
   return c.epoch;
 }
@@ -185,7 +186,6 @@ Ctx *create_ctx(int device) {
   c->scratch.counters = static_cast<uint32_t *>(counters);
   c->scratch.slots = static_cast<uint64_t *>(slots);
   fill_u64(*c, static_cast<uint64_t *>(counters), 256 / 8, 0);
-  status_words(*c, 4096);
   check_hip(hipStreamSynchronize(c->stream), "stream sync");
   return c;
 }

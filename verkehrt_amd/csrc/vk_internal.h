@@ -84,9 +84,13 @@ namespace vk {
 
 // Per-context device scratch reused by every launch of the context: look-back status
 // words, tickets, done counters and small result slots. Grows on demand (between calls).
+// Look-back status buffers, one per kernel family, each with one fixed word layout, so a
+// word at a flag position was only ever written as a flag (older epochs read as not ready).
+enum StatusFamily { ST_SELECT = 0, ST_SCAN, ST_SORT, ST_CANON, ST_FAMILIES };
+
 struct Scratch {
-  uint64_t *status = nullptr;  // look-back words
-  uint64_t status_words = 0;
+  uint64_t *status[ST_FAMILIES] = {};  // look-back words
+  uint64_t status_words[ST_FAMILIES] = {};
   uint32_t *counters = nullptr;  // [0] ticket, [1] done, [2..] spare; always returned to 0
   uint64_t *slots = nullptr;     // 64 x 8 bytes of small device results
   void *partials = nullptr;      // per-block partial results
@@ -119,8 +123,8 @@ inline uint64_t abytes(const vk_array *a) { return a->len * type_width(a->type);
 void release(vk_array *a);
 
 // Scratch -----------------------------------------------------------------
-uint64_t *status_words(Ctx &c, uint64_t words);  // zeroed at growth
-uint32_t next_epoch(Ctx &c);                     // clears status on wrap
+uint64_t *status_words(Ctx &c, int family, uint64_t words);  // zeroed at growth
+uint32_t next_epoch(Ctx &c);  // 1..65535; clears every status buffer on wrap
 void *partials(Ctx &c, uint64_t bytes);
 void *tmp_alloc(Ctx &c, uint64_t bytes);  // pool memory for one call
 void tmp_free(Ctx &c, void *p);
