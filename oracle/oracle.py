@@ -386,3 +386,30 @@ def group_id(x, y=None) -> np.ndarray:
             raise OracleError(f"length mismatch: {a.size} vs {b.size}")
         lib().vko_group_id_pair_u64(_p(a), _p(b), a.size, _p(out))
     return out
+
+
+# ----------------------------------------------- best x86 CPU path (baseline only)
+
+BEST_PATH = os.path.join(HERE, "lib", "libvk_cpu_best.so")
+_best = None
+
+
+def best():
+    """ctypes handle of oracle/cpu_best.c, the restated x86-64-v4 backend timed as the CPU
+    baseline. Checked against the oracle functions above by tests/test_cpu_best.py."""
+    global _best
+    if _best is None:
+        L = C.CDLL(BEST_PATH)
+        u64, i64, dbl, vp = C.c_uint64, C.c_int64, C.c_double, C.c_void_p
+        for name, res, args in [
+            ("vkb_sum_f64", dbl, [vp, u64]), ("vkb_dot_f64", dbl, [vp, vp, u64]),
+            ("vkb_count_gt_f64", i64, [vp, u64, dbl]), ("vkb_min_f64", dbl, [vp, u64]),
+            ("vkb_argmax_f64", i64, [vp, u64]), ("vkb_scan_add_i64", None, [vp, u64, vp]),
+            ("vkb_filter_gt_f64", u64, [vp, u64, dbl, vp]), ("vkb_compress_f64", u64, [vp, vp, u64, vp]),
+            ("vkb_gather_f64", C.c_int, [vp, u64, vp, u64, vp, vp]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _best = L
+    return _best
