@@ -510,3 +510,66 @@ def test_errors_as_strings(D):
     assert e == ("err", "argmin of an empty array")
     e = err_of(lambda: D.call("reduce_max_i64", [D.array(np.zeros(0, np.int64))]))
     assert e == ("err", "reduce_max of an empty array")
+
+
+# ------------------------------------------------------------------ plan replay
+
+def test_readme_plan_replay_with_trace(D):
+    # README.md:55-69: the four plan lines, their outputs, and the trace line format
+    from verkehrt_amd.plan import Plan
+    P = Plan(D)
+    D.ctx.set_trace(True)
+    D.ctx.take_trace()
+    P.run("(rand_f64 1000000 1)", bind="x")
+    dot = P.run("(dot_f64 x (rand_f64 1000000 2))")
+    cnt = P.run("(count_gt_f64 x 0.5)")
+    ln = P.run("(length_f64 (filter_gt_f64 x 0.99))")
+    trace = D.ctx.take_trace().strip().splitlines()
+    D.ctx.set_trace(False)
+    assert repr(dot.value) == "249724.39002046228" and cnt.value == 499521 and ln.value == 9851
+    names = [t.split()[1] for t in trace]
+    assert names == ["rand_f64", "rand_f64", "dot_f64", "count_gt_f64", "filter_gt_f64", "length_f64"]
+    import re
+    pat = re.compile(r"^trace \w+ isa=hip n=\d+ read=\d+ write=\d+ time=\d+\.\d gbps=\d+\.\d\d$")
+    assert all(pat.match(t) for t in trace), trace
+    f = [t for t in trace if t.split()[1] == "filter_gt_f64"][0]
+    assert "n=1000000 read=8000000 write=78808" in f
+
+
+# --------------------------------------------- exact fast canonical fold (large n)
+
+def fast_path_sets(rng, n):
+    yield "uniform", rng.random(n)
+    yield "normal", rng.normal(size=n)
+    yield "varied", (rng.random(n) - 0.5) * 10.0 ** ((np.arange(n) % 9) * 2)
+    yield "specials-free", floats(rng, n, False)
+    yield "with-specials", floats(rng, n, True)
+    yield "halves", rng.integers(-4, 5, n) * 0.5 + (np.arange(n) < 16) * 2.0**52  # parity ties
+    drift = rng.random(n) - 0.5001
+    drift[:16] = 40.0
+    yield "drift-through-zero", drift
+
+
+@pytest.mark.parametrize("n", [8192, 8192 + 13, 65536 + 17, 1_000_003])
+def test_canonical_fast_path(D, n):
+    rng = np.random.default_rng(n)
+    for name, x in fast_path_sets(rng, n):
+        dx = view(D, x, 1)
+        assert same_float(O.canonical_sum_f64(x), D.call("reduce_add_f64", [dx]).value), (name, n)
+        y = rng.random(n)
+        assert same_float(O.dot_f64(x, y), D.call("dot_f64", [dx, D.array(y)]).value), (name, n)
+        want = O.condsum_f64(x, "gt", 0.25, -1.0)
+        got = D.call("condsum_gt_f64", [dx, S("f64", 0.25), S("f64", -1.0)]).value
+        assert same_float(want, got), (name, n)
+
+
+def test_canonical_fast_path_tiles(D):
+    rng = np.random.default_rng(77)
+    n = 300_000 + 5
+    x = rng.normal(size=n) * 1e3
+    dx = D.array(x)
+    r = D.reducer("reduce_add_f64")
+    cuts = [0, 16 * 1000, 16 * 9000, 16 * 9001, 16 * 18000, n]
+    for a, b in zip(cuts, cuts[1:]):
+        r.update([dx.slice(a, b - a)])
+    assert same_float(O.canonical_sum_f64(x), r.finish().value)
