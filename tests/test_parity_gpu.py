@@ -573,3 +573,37 @@ def test_canonical_fast_path_tiles(D):
     for a, b in zip(cuts, cuts[1:]):
         r.update([dx.slice(a, b - a)])
     assert same_float(O.canonical_sum_f64(x), r.finish().value)
+
+
+def test_grouping_programs_replayed(D):
+    # tests/programs/grouping.dd (group_sorted, buckets) as the kernel sequences a plan runs
+    I = lambda v: D.array(np.array(v, np.int64))
+    k, v = I([1, 1, 2, 2, 2, 5]), I([1, 2, 3, 4, 5, 6])
+    n = D.call("length_i64", [k]).value
+    it = D.call("iota", [S("i64", n)])
+    s = D.call("or_bool", [D.call("ne_i64", [k, D.call("prev_i64", [k, S("i64", 0)])]),
+                           D.call("eq_i64", [it, S("i64", 0)])])
+    nxt = D.call("gather_i64", [k, D.call("min_i64", [D.call("add_i64", [it, S("i64", 1)]), S("i64", n - 1)])])
+    e = D.call("or_bool", [D.call("ne_i64", [k, nxt]), D.call("eq_i64", [it, S("i64", n - 1)])])
+    assert list(D.call("compress_i64", [D.call("scan_by_add_i64", [v, s]), e]).to_numpy()) == [3, 12, 6]
+    ids = D.call("sub_i64", [D.call("scan_add_i64", [D.call("cast_bool_i64", [s])]), S("i64", 1)])
+    ng = D.call("count", [s])
+    assert list(D.call("reduce_into_add_i64", [ids, v, ng]).to_numpy()) == [3, 12, 6]
+    assert list(D.call("compress_i64", [k, s]).to_numpy()) == [1, 2, 5]
+    g = D.call("group_id_i64", [I([7, -1, 7, 3, -1, 7])])
+    assert list(g.to_numpy()) == [0, 1, 0, 2, 1, 0]
+    assert D.call("reduce_max_i64", [g]).value + 1 == 3
+
+
+def test_prev_and_tiles(D):
+    rng = np.random.default_rng(31)
+    x = rng.integers(-100, 100, 5000).astype(np.int64)
+    got = D.call("prev_i64", [D.array(x), S("i64", -9)]).to_numpy()
+    assert got[0] == -9 and np.array_equal(got[1:], x[:-1])
+    from verkehrt_amd import Array
+    out = Array.empty(D.ctx, "i64", x.size)
+    sc = D.scanner("prev_i64")
+    dx = D.array(x)
+    for a, b in [(0, 37), (37, 100), (100, 5000)]:
+        sc.update([dx.slice(a, b - a), S("i64", -9)], out.slice(a, b - a))
+    assert np.array_equal(out.to_numpy(), got)

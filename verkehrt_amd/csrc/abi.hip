@@ -28,7 +28,7 @@ struct In {
 enum Impl {
   R_CANON, R_SUM_INT, R_COUNT_CMP, R_COUNT_BOOL, R_EXTREME, R_ARG, R_BITS,
   S_COMPRESS, S_FILTER, S_WHERE,
-  SC_SCAN, SC_FSCAN,
+  SC_SCAN, SC_FSCAN, SC_PREV,
   W_GATHER, W_ARGSORT, W_GROUP_ID, W_GROUP_ID2, W_HISTOGRAM, W_REDUCE_INTO, W_RAND_F64, W_RAND_I64,
   W_IOTA, W_LENGTH, W_FILL,
   E_BINARY, E_UNARY
@@ -117,6 +117,8 @@ void build_table() {
   }
   // Not a reference kernel (builtins.dd:50-54): the order defined by this backend.
   add({"scan_add_f64", VK_SCAN, {A(VK_F64)}, true, VK_F64, SC_FSCAN});
+  for (int32_t t : all)  // prev_<t> (scan.rs:57-75)
+    add({"prev_" + tn(t), VK_SCAN, {A(t), S(t)}, true, t, SC_PREV, 0, 0, t});
   // whole kernels (index.rs:277-326, group.rs:330-365, construct.rs:200-272)
   for (int32_t t : all)
     add({"gather_" + tn(t), VK_WHOLE, {A(t), A(VK_I64)}, true, t, W_GATHER, 0, 0, t});
@@ -423,6 +425,19 @@ void reducer_free(vk_reducer *r) {
 void scanner_update(vk_scanner *s, const vk_arg *args, size_t nargs, vk_array *out) {
   Ctx &c = *s->c;
   const Def &d = g_defs[s->def];
+  if (d.impl == SC_PREV) {
+    // Prev::update (scan.rs:24-47)
+    if (nargs != 2) arity(2, nargs);
+    const vk_array *x = rows_arg(args[0], d.t0);
+    const uint64_t fill = splat_arg(args[1], d.t0);
+    if (!out || out->type != d.out_t) usage_error("bad destination: the output rows are not " + tn(d.out_t));
+    if (out->len != x->len)
+      usage_error("length mismatch: " + std::to_string(x->len) + " rows for a destination of " +
+                  std::to_string(out->len) + " rows");
+    prev_rows(c, type_width(d.t0), dptr(x), x->len, dptr(out), s->carry, fill);
+    s->consumed += x->len;
+    return;
+  }
   const bool by = d.impl == SC_SCAN && d.p1;
   if (nargs != (by ? 2u : 1u)) arity(by ? 2 : 1, nargs);
   const vk_array *x = rows_arg(args[0], d.in[0].t);

@@ -282,7 +282,47 @@ __global__ void k_fscan_rows(const double *__restrict__ x, uint64_t n, const dou
   }
 }
 
+// prev (Prev<T, BOOL>, scan.rs:14-55): row i is row i - 1 of x; row 0 is the last row of
+// the previous tile, or `fill` before the first row. The carry keeps the last row.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prev(const T *__restrict__ x, uint64_t n, T *__restrict__ out,
+                                                 const uint64_t *carry_in, uint64_t fill_bits,
+                                                 uint64_t *carry_out) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    if (i == 0) {
+      out[0] = carry_in[1] ? from_bits<T>(carry_in[0]) : from_bits<T>(fill_bits);
+    } else {
+      out[i] = x[i - 1];
+    }
+    if (i == n - 1) {
+      carry_out[0] = to_bits(x[n - 1]);
+      carry_out[1] = 1;
+    }
+  }
+}
+
 }  // namespace
+
+void prev_rows(Ctx &c, uint32_t width, const void *x, uint64_t n, void *out, uint64_t *carry,
+               uint64_t fill_bits) {
+  if (n == 0) return;
+  uint64_t *carry_in = c.scratch.slots + 10;
+  copy_d2d(c, carry_in, carry, 16);
+  const uint32_t g = stream_grid(n, kBlock * 4, 8192);
+  switch (width) {
+    case 8: k_prev<uint64_t><<<g, kBlock, 0, c.stream>>>(static_cast<const uint64_t *>(x), n,
+                                                         static_cast<uint64_t *>(out), carry_in, fill_bits, carry);
+      break;
+    case 4: k_prev<uint32_t><<<g, kBlock, 0, c.stream>>>(static_cast<const uint32_t *>(x), n,
+                                                         static_cast<uint32_t *>(out), carry_in, fill_bits, carry);
+      break;
+    default: k_prev<uint8_t><<<g, kBlock, 0, c.stream>>>(static_cast<const uint8_t *>(x), n,
+                                                         static_cast<uint8_t *>(out), carry_in, fill_bits, carry);
+      break;
+  }
+  check_hip(hipGetLastError(), "prev");
+}
 
 void scan_rows(Ctx &c, const ScanSpec &s, const void *x, const uint8_t *starts, uint64_t n,
                void *out, uint64_t *carry) {
