@@ -192,12 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(Op op, const T *__restrict__ 
   __threadfence();
   Acc b = op.identity();
   for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-    Acc p;
-    // Partials of other XCDs: read through the agent-coherent path.
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(&partials[i]);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(&p);
-#pragma unroll
-    for (int k = 0; k < static_cast<int>(sizeof(Acc) / 8); ++k) dst[k] = ld_rlx(src + k);
+    // partials of other XCDs: read through the agent-coherent path
+    const Acc p = ld_rlx_pod(&partials[i]);
     op.merge(b, p);
   }
   __syncthreads();

@@ -142,34 +142,56 @@ __device__ __forceinline__ void grid_rows(const T *__restrict__ x, uint64_t n, F
   for (uint64_t r = head + nv * V + tid; r < n; r += nth) f(x[r], r);
 }
 
-// Shuffles a trivially copyable value word by word.
+// Shuffles a trivially copyable value word by word. The words are copied in and out with
+// memcpy, never read through a cast pointer, so no access breaks type-based aliasing.
+template <typename T>
+struct PodWords {
+  static_assert(sizeof(T) % 4 == 0, "pod shuffle needs whole words");
+  static constexpr int N = static_cast<int>(sizeof(T) / 4);
+  int w[N];
+};
 template <typename T>
 __device__ __forceinline__ T shfl_xor_pod(const T &v, int mask) {
-  static_assert(sizeof(T) % 4 == 0, "pod shuffle needs whole words");
-  T out;
-  const int *src = reinterpret_cast<const int *>(&v);
-  int *dst = reinterpret_cast<int *>(&out);
+  PodWords<T> in, out;
+  __builtin_memcpy(in.w, &v, sizeof(T));
 #pragma unroll
-  for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __shfl_xor(src[w], mask);
-  return out;
+  for (int k = 0; k < PodWords<T>::N; ++k) out.w[k] = __shfl_xor(in.w[k], mask);
+  T r;
+  __builtin_memcpy(&r, out.w, sizeof(T));
+  return r;
 }
 template <typename T>
 __device__ __forceinline__ T shfl_up_pod(const T &v, int delta) {
-  T out;
-  const int *src = reinterpret_cast<const int *>(&v);
-  int *dst = reinterpret_cast<int *>(&out);
+  PodWords<T> in, out;
+  __builtin_memcpy(in.w, &v, sizeof(T));
 #pragma unroll
-  for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __shfl_up(src[w], delta);
-  return out;
+  for (int k = 0; k < PodWords<T>::N; ++k) out.w[k] = __shfl_up(in.w[k], delta);
+  T r;
+  __builtin_memcpy(&r, out.w, sizeof(T));
+  return r;
 }
 template <typename T>
 __device__ __forceinline__ T shfl_pod(const T &v, int src_lane) {
-  T out;
-  const int *src = reinterpret_cast<const int *>(&v);
-  int *dst = reinterpret_cast<int *>(&out);
+  PodWords<T> in, out;
+  __builtin_memcpy(in.w, &v, sizeof(T));
 #pragma unroll
-  for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) dst[w] = __shfl(src[w], src_lane);
-  return out;
+  for (int k = 0; k < PodWords<T>::N; ++k) out.w[k] = __shfl(in.w[k], src_lane);
+  T r;
+  __builtin_memcpy(&r, out.w, sizeof(T));
+  return r;
+}
+
+// Loads a trivially copyable value of 8-byte words through agent-coherent atomic loads.
+template <typename T>
+__device__ __forceinline__ T ld_rlx_pod(const T *p) {
+  static_assert(sizeof(T) % 8 == 0, "pod load needs whole 8-byte words");
+  uint64_t w[sizeof(T) / 8];
+  const uint64_t *src = reinterpret_cast<const uint64_t *>(p);
+#pragma unroll
+  for (int k = 0; k < static_cast<int>(sizeof(T) / 8); ++k) w[k] = ld_rlx(src + k);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
 }
 
 }  // namespace vk
