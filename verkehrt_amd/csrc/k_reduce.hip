@@ -163,13 +163,17 @@ __device__ __forceinline__ typename Op::Acc block_merge(const Op &op, typename O
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) op.merge(a, shfl_xor_pod(a, m));
   __shared__ Acc s_w[kBlock / kWave];
-  const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) s_w[w] = a;
+  const int w = threadIdx.x / kWave, l = lane_id();
+  if (l == 0) s_w[w] = a;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < kBlock / kWave; ++k) op.merge(a, s_w[k]);
+  // the four wave results meet in lanes 0..3 of wave 0 and merge with two more butterflies
+  if (w == 0) {
+    Acc b = l < kBlock / kWave ? s_w[l] : op.identity();
+    op.merge(b, shfl_xor_pod(b, 1));
+    op.merge(b, shfl_xor_pod(b, 2));
+    a = b;
   }
-  return a;  // valid in thread 0
+  return a;  // valid in wave 0
 }
 
 template <typename T, class Op>
