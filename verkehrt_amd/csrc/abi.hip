@@ -771,6 +771,126 @@ vk_value call(Ctx &c, const Def &d, const vk_arg *args, size_t nargs) {
   }
 }
 
+// ------------------------------------------------------------------ batching
+
+std::vector<vk_value> batch_call(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, size_t nitems) {
+  std::vector<vk_value> out(nitems, vk_value{VK_VALUE_ARRAY, nullptr, {0, 0}});
+  // every item's arguments pass the checks of a single call first
+  std::vector<int64_t> rows(nitems, -1);
+  for (size_t i = 0; i < nitems; ++i) {
+    const vk_arg *a = args + i * nargs;
+    if (d.cls == VK_WHOLE) {
+      if (nargs != d.in.size()) arity(d.in.size(), nargs);
+      for (size_t k = 0; k < nargs; ++k)
+        if (!accepts(d.in[k], a[k])) kind_error(d.in[k], a[k]);
+    } else {
+      rows[i] = check_args(d, a, nargs);
+    }
+  }
+  auto each = [&]() {
+    size_t done = 0;
+    try {
+      for (; done < nitems; ++done) out[done] = call(c, d, args + done * nargs, nargs);
+    } catch (...) {
+      for (size_t i = 0; i < done; ++i) vk_value_release(&out[i]);
+      throw;
+    }
+  };
+  for (size_t i = 0; i < nitems; ++i)
+    if (d.cls != VK_WHOLE && rows[i] < 0) {  // no array operand: the single call's own error
+      each();
+      return out;
+    }
+  if (d.cls == VK_SELECTION) {
+    const uint32_t w = type_width(d.out_t);
+    uint64_t total = 0;
+    std::vector<uint64_t> off(nitems);
+    for (size_t i = 0; i < nitems; ++i) {
+      off[i] = total;
+      total += rows[i] < 0 ? 0 : uint64_t(rows[i]);
+    }
+    vk_array *buf = new_array(c, d.out_t, total ? total : 1);
+    std::vector<BatchItem> items(nitems);
+    SelectSpec s{};
+    for (size_t i = 0; i < nitems; ++i) {
+      const vk_arg *a = args + i * nargs;
+      BatchItem &it = items[i];
+      it.n = rows[i] < 0 ? 0 : uint64_t(rows[i]);
+      it.out = static_cast<char *>(dptr(buf)) + off[i] * w;
+      if (d.impl == S_WHERE) {
+        it.a = dptr(a[0].rows);
+        it.aux = 0;
+      } else {
+        it.a = dptr(a[0].rows);
+        if (d.impl == S_COMPRESS) it.b = dptr(a[1].rows);
+      }
+    }
+    if (d.impl == S_COMPRESS) {
+      s.kind = 0;
+      s.type = d.t0;
+    } else if (d.impl == S_FILTER) {
+      s.kind = 1;
+      s.type = d.t0;
+      s.cmp = d.p0;
+      s.cbits = args[1].splat.bits;
+      for (size_t i = 1; i < nitems; ++i)
+        if (args[i * nargs + 1].splat.bits != s.cbits) {  // one constant per launch
+          release(buf);
+          each();
+          return out;
+        }
+    } else {
+      s.kind = 2;
+      s.type = VK_I64;
+    }
+    auto *counts = static_cast<uint64_t *>(tmp_alloc(c, nitems * 8));
+    select_batch(c, s, items, counts);
+    std::vector<uint64_t> got(nitems);
+    check_hip(hipMemcpyAsync(got.data(), counts, nitems * 8, hipMemcpyDeviceToHost, c.stream), "counts");
+    check_hip(hipStreamSynchronize(c.stream), "sync");
+    tmp_free(c, counts);
+    for (size_t i = 0; i < nitems; ++i) {
+      buf->buf->refs++;
+      out[i].array = new vk_array{1, buf->buf, buf->offset + off[i] * w, got[i], d.out_t};
+    }
+    release(buf);
+    return out;
+  }
+  if (d.impl == R_CANON && d.p0 != 4) {
+    std::vector<BatchItem> items(nitems);
+    CanonSpec cs{d.p0, d.p1, 0, 0};
+    for (size_t i = 0; i < nitems; ++i) {
+      const vk_arg *a = args + i * nargs;
+      items[i].a = dptr(a[0].rows);
+      items[i].b = d.p0 == 1 ? dptr(a[1].rows) : nullptr;
+      items[i].n = uint64_t(rows[i]);
+      if (d.p0 == 3) {
+        if (a[1].splat.bits != args[1].splat.bits || a[2].splat.bits != args[2].splat.bits) {
+          each();
+          return out;
+        }
+      }
+    }
+    if (d.p0 == 3) {
+      std::memcpy(&cs.c, &args[1].splat.bits, 8);
+      std::memcpy(&cs.e, &args[2].splat.bits, 8);
+    }
+    auto *res = static_cast<double *>(tmp_alloc(c, nitems * 8));
+    canon_batch(c, cs, items, res);
+    std::vector<uint64_t> bits(nitems);
+    check_hip(hipMemcpyAsync(bits.data(), res, nitems * 8, hipMemcpyDeviceToHost, c.stream), "results");
+    check_hip(hipStreamSynchronize(c.stream), "sync");
+    tmp_free(c, res);
+    for (size_t i = 0; i < nitems; ++i) {
+      out[i].kind = VK_VALUE_SCALAR;
+      out[i].scalar = scalar(VK_F64, bits[i]);
+    }
+    return out;
+  }
+  each();
+  return out;
+}
+
 int set_status(vk_status *st, int code, const std::string &msg) {
   if (st) {
     st->code = code;
@@ -981,6 +1101,21 @@ void vk_value_release(vk_value *v) {
     release(v->array);
     v->array = nullptr;
   }
+}
+
+int vk_batch_call(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, size_t nitems,
+                  vk_value *out, vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    TraceScope ts(*ctx, d.name.c_str(), args, nargs * nitems);
+    std::vector<vk_value> v = batch_call(*ctx, d, args, nargs, nitems);
+    uint64_t wb = 0;
+    for (size_t i = 0; i < nitems; ++i) {
+      out[i] = v[i];
+      if (v[i].kind == VK_VALUE_ARRAY && v[i].array) wb += abytes(v[i].array);
+    }
+    ts.done(wb);
+  });
 }
 
 }  // extern "C"

@@ -109,19 +109,36 @@ __device__ __forceinline__ Binade unpack_binade(uint64_t w) {
   return Binade{static_cast<int>(w >> 2) - 2048, static_cast<int>((w >> 1) & 1), static_cast<int>(w & 1), 0};
 }
 
+// BATCH: the tile's item gives the rows (a, b), the block count (n / 16), the first tile, and
+// the item's 16 start and end accumulators at acc_in / acc_out + 16 * item.
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *__restrict__ x,
                                                        const double *__restrict__ y, uint64_t blocks,
                                                        const double *acc_in, double *acc_out,
                                                        uint64_t *status, uint32_t epoch,
-                                                       uint32_t *ticket, uint32_t ntiles) {
+                                                       uint32_t *ticket, uint32_t ntiles,
+                                                       const BatchItem *items,
+                                                       const uint32_t *tile_item) {
   __shared__ double s_val[kTileBlocks * 16];
   __shared__ double s_a[kBlock / kWave][16];
   __shared__ RunFn s_f[kBlock / kWave][16];
   __shared__ Binade s_h[16];
   const uint32_t tile = take_ticket(ticket, ntiles);
+  uint32_t first = 0, last = ntiles - 1;
+  if (BATCH) {
+    const uint32_t it = tile_item[tile];
+    const BatchItem &I = items[it];
+    x = static_cast<const double *>(I.a);
+    y = I.b ? static_cast<const double *>(I.b) : x;
+    blocks = I.n / 16;
+    first = static_cast<uint32_t>(I.tile0);
+    last = first + static_cast<uint32_t>((blocks + kTileBlocks - 1) / kTileBlocks) - 1;
+    acc_in += 16 * uint64_t(it);
+    acc_out += 16 * uint64_t(it);
+  }
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const int j = t & 15, r = t >> 4;
-  const uint64_t b0 = uint64_t(tile) * kTileBlocks;
+  const uint64_t b0 = uint64_t(tile - first) * kTileBlocks;
   const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
   uint64_t *st = status + uint64_t(tile) * kStride;
 
@@ -148,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
     double atile = 0.0, pre = 0.0;
     if (l < 16) {
       atile = s_a[0][l] + s_a[1][l] + s_a[2][l] + s_a[3][l];
-      if (tile == 0) {
+      if (tile == first) {
         pre = acc_in[l];
         st_d(st + W_AINC + l, pre + atile);
       } else {
@@ -156,8 +173,8 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
       }
     }
     __threadfence();
-    if (l == 0) st_rel(st, flag_word(epoch, tile == 0 ? (F_AAGG | F_AINC) : F_AAGG));
-    if (tile > 0 && l < 16) {
+    if (l == 0) st_rel(st, flag_word(epoch, tile == first ? (F_AAGG | F_AINC) : F_AAGG));
+    if (tile > first && l < 16) {
       double sum = 0.0;
       for (int64_t k = int64_t(tile) - 1;; --k) {
         const uint64_t *sk = status + uint64_t(k) * kStride;
@@ -175,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
       pre = sum;
       st_d(st + W_AINC + l, pre + atile);
     }
-    if (tile > 0) {
+    if (tile > first) {
       __threadfence();
       if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC));
     }
@@ -224,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
     double base = 0.0;
     int64_t k = int64_t(tile) - 1;
     for (;; --k) {
-      if (k < 0) {
+      if (k < int64_t(first)) {
         base = acc_in[l];
         break;
       }
@@ -254,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
     }
     bool done = run_apply(run_join(g, ft), hj, base, &exact);
     if (!done) {
-      if (k != int64_t(tile) - 1 && tile > 0) {
+      if (k != int64_t(tile) - 1 && tile > first) {
         const uint64_t *sp = status + uint64_t(tile - 1) * kStride;
         wait_flag(sp, epoch, F_EXACT);
         base = ld_d(sp + W_EXACT + l);
@@ -267,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
       }
     }
     st_d(st + W_EXACT + l, exact);
-    if (tile == ntiles - 1) acc_out[l] = exact;
+    if (tile == last) acc_out[l] = exact;
   }
   __threadfence();
   if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC | F_FN | F_EXACT));
@@ -290,6 +307,31 @@ __global__ void k_canon_finish(bool mul, const double *acc_in, const double *tai
   double a = acc[0];
   for (int j = 0; j < tail_len; ++j) a = mul ? a * tail[j] : a + tail[j];
   *out = a;
+}
+
+__global__ void k_fill_d(double *p, uint64_t n, double v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// One thread per item: finish_lanes (reduce.rs:75-91) over the item's accumulators and its
+// tail rows 16 * (n / 16) .. n.
+__global__ void k_canon_batch_finish(RowVal rv, const BatchItem *items, uint64_t nitems,
+                                     const double *acc, double *out) {
+  const uint64_t it = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (it >= nitems) return;
+  const BatchItem &I = items[it];
+  const double *x = static_cast<const double *>(I.a);
+  const double *y = I.b ? static_cast<const double *>(I.b) : x;
+  const bool mul = rv.kind == 4;
+  double a[16];
+  for (int j = 0; j < 16; ++j) a[j] = acc[it * 16 + j];
+  for (int step = 8; step > 0; step /= 2)
+    for (int j = 0; j < step; ++j) a[j] = mul ? a[j] * a[j + step] : a[j] + a[j + step];
+  double s = a[0];
+  for (uint64_t r = I.n / 16 * 16; r < I.n; ++r) s = mul ? s * rv(x, y, r) : s + rv(x, y, r);
+  out[it] = s;
 }
 
 __global__ void k_canon_init(double *acc, double id) {
@@ -318,9 +360,42 @@ void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, 
   const uint32_t epoch = next_epoch(c);
   double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
   copy_d2d(c, acc_in, acc, 16 * 8);
-  k_canon_fast<<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y ? y : x, blocks, acc_in, acc, status,
-                                                        epoch, c.scratch.counters, uint32_t(tiles));
+  k_canon_fast<false><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y ? y : x, blocks, acc_in, acc,
+                                                               status, epoch, c.scratch.counters,
+                                                               uint32_t(tiles), nullptr, nullptr);
   check_hip(hipGetLastError(), "canon fast");
+}
+
+// Batched canonical sums: every item's 16 chains start at the identity; a finish kernel
+// combines each item's accumulators pairwise and folds its tail rows.
+void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, double *d_out) {
+  const uint64_t nitems = items.size();
+  if (nitems == 0) return;
+  RowVal rv{s.kind, s.cmp, s.c, s.e};
+  double *acc = static_cast<double *>(tmp_alloc(c, nitems * 16 * 8));
+  double *zero = static_cast<double *>(tmp_alloc(c, nitems * 16 * 8));
+  const double id = s.kind == 4 ? 1.0 : 0.0;
+  k_fill_d<<<stream_grid(nitems * 16, kBlock, 4096), kBlock, 0, c.stream>>>(acc, nitems * 16, id);
+  k_fill_d<<<stream_grid(nitems * 16, kBlock, 4096), kBlock, 0, c.stream>>>(zero, nitems * 16, id);
+  std::vector<BatchItem> plan_items(items);
+  for (auto &it : plan_items) it.n = it.n / 16 * 16;  // tiles cover whole blocks only
+  BatchPlan p = batch_plan(c, plan_items, uint64_t(kTileBlocks) * 16);
+  if (p.ntiles > 0) {
+    if (s.kind == 4) usage_error("batched products are not supported");
+    uint64_t *status = status_words(c, ST_CANON, uint64_t(p.ntiles) * kStride);
+    const uint32_t epoch = next_epoch(c);
+    k_canon_fast<true><<<p.ntiles, kBlock, 0, c.stream>>>(rv, nullptr, nullptr, 0, zero, acc, status, epoch,
+                                                         c.scratch.counters, p.ntiles, p.items, p.tile_item);
+    check_hip(hipGetLastError(), "canon batch");
+  }
+  BatchPlan q = batch_plan(c, items, 1);  // the full rows, for the tails
+  k_canon_batch_finish<<<stream_grid(nitems, kBlock, 4096), kBlock, 0, c.stream>>>(rv, q.items, nitems, acc,
+                                                                                  d_out);
+  check_hip(hipGetLastError(), "canon batch finish");
+  batch_plan_free(c, p);
+  batch_plan_free(c, q);
+  tmp_free(c, acc);
+  tmp_free(c, zero);
 }
 
 void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t rest,

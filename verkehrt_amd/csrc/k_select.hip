@@ -24,16 +24,17 @@ constexpr uint64_t kCountMask = (uint64_t(1) << 46) - 1;
 // The number of kept rows of all tiles before `tile`, by one wave: lanes read the status of
 // 64 predecessors at once, wait until all are published, add aggregates back to the nearest
 // inclusive prefix.
-__device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile) {
+// Tiles below `first` (another item of a batched launch, or none) count as an inclusive 0.
+__device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, uint32_t first) {
   uint64_t sum = 0;
   int64_t end = static_cast<int64_t>(tile);  // exclusive
   const int l = lane_id();
-  while (end > 0) {
+  while (end > static_cast<int64_t>(first)) {
     const int64_t idx = end - 64 + l;
     uint64_t w = 0;
     bool ready;
     do {
-      w = idx >= 0 ? ld_rlx(&status[idx]) : pack(epoch, 2, 0);
+      w = idx >= static_cast<int64_t>(first) ? ld_rlx(&status[idx]) : pack(epoch, 2, 0);
       ready = (w >> 48) == epoch && ((w >> 46) & 3) != 0;
       if (__all(ready)) break;
       backoff();
@@ -52,17 +53,33 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile) {
 }
 
 // kind: 0 compress (keep = mask != 0), 1 filter (keep = x cmp c), 2 where (value base + row).
-template <typename T, int KIND>
+// BATCH: the tile's item (row pointers, row count, output, first tile, index base) comes
+// from the batch tables, and the item's last tile writes its count to counts[item].
+template <typename T, int KIND, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
                                                    int cmp, T c, uint64_t base, void *out_v,
                                                    uint64_t *status, uint32_t epoch,
                                                    uint32_t *ticket, uint32_t ntiles,
-                                                   uint64_t *d_count) {
+                                                   uint64_t *d_count, const BatchItem *items,
+                                                   const uint32_t *tile_item, uint64_t *counts) {
   using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
-  Out *out = static_cast<Out *>(out_v);
   const uint32_t tile = take_ticket(ticket, ntiles);
-  const uint64_t t0 = uint64_t(tile) * kTile;
+  uint32_t first = 0, last = ntiles - 1;
+  if (BATCH) {
+    const uint32_t it = tile_item[tile];
+    const BatchItem &I = items[it];
+    x = static_cast<const T *>(I.a);
+    mask = KIND == 2 ? static_cast<const uint8_t *>(I.a) : static_cast<const uint8_t *>(I.b);
+    n = I.n;
+    out_v = I.out;
+    base = I.aux;
+    first = static_cast<uint32_t>(I.tile0);
+    last = first + static_cast<uint32_t>((n + kTile - 1) / kTile) - 1;
+    d_count = counts + it;
+  }
+  Out *out = static_cast<Out *>(out_v);
+  const uint64_t t0 = uint64_t(tile - first) * kTile;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
 
   Out val[kItems];
@@ -105,16 +122,16 @@ __global__ __launch_bounds__(kBlock) void k_select(const T *__restrict__ x,
     const uint32_t total = __shfl(inc, 63);
     // publish and look back
     uint64_t prefix = 0;
-    if (tile == 0) {
-      if (l == 0) st_rel(&status[0], pack(epoch, 2, total));
+    if (tile == first) {
+      if (l == 0) st_rel(&status[tile], pack(epoch, 2, total));
     } else {
       if (l == 0) st_rel(&status[tile], pack(epoch, 1, total));
-      prefix = look_back(status, epoch, tile);
+      prefix = look_back(status, epoch, tile, first);
       if (l == 0) st_rel(&status[tile], pack(epoch, 2, prefix + total));
     }
     if (l == 0) {
       s_prefix = prefix;
-      if (tile == ntiles - 1) *d_count = prefix + total;
+      if (tile == last) *d_count = prefix + total;
     }
   }
   __syncthreads();
@@ -134,10 +151,39 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
   const uint32_t epoch = next_epoch(c);
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
-  k_select<T, KIND><<<static_cast<uint32_t>(tiles), kBlock, 0, c.stream>>>(
+  k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kBlock, 0, c.stream>>>(
       static_cast<const T *>(x), mask, n, cmp, cv, base, out, status, epoch, c.scratch.counters,
-      static_cast<uint32_t>(tiles), d_count);
+      static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr);
   check_hip(hipGetLastError(), "select launch");
+}
+
+template <typename T, int KIND>
+void launch_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts) {
+  BatchPlan p = batch_plan(c, items, kTile);
+  if (p.ntiles > 0) {
+    uint64_t *status = status_words(c, ST_SELECT, p.ntiles);
+    const uint32_t epoch = next_epoch(c);
+    T cv;
+    __builtin_memcpy(&cv, &s.cbits, sizeof(T));
+    k_select<T, KIND, true><<<p.ntiles, kBlock, 0, c.stream>>>(
+        nullptr, nullptr, 0, s.cmp, cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
+        nullptr, p.items, p.tile_item, d_counts);
+    check_hip(hipGetLastError(), "select batch launch");
+  }
+  batch_plan_free(c, p);
+}
+
+template <int KIND>
+void batch_by_width(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts) {
+  switch (s.type) {
+    case VK_F64: launch_batch<double, KIND>(c, s, items, d_counts); break;
+    case VK_F32: launch_batch<float, KIND>(c, s, items, d_counts); break;
+    case VK_I64: launch_batch<int64_t, KIND>(c, s, items, d_counts); break;
+    case VK_U64: launch_batch<uint64_t, KIND>(c, s, items, d_counts); break;
+    case VK_I32: launch_batch<int32_t, KIND>(c, s, items, d_counts); break;
+    case VK_U8: case VK_BOOL: launch_batch<uint8_t, KIND>(c, s, items, d_counts); break;
+    default: usage_error("unsupported selection type");
+  }
 }
 
 template <int KIND>
@@ -168,6 +214,15 @@ void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask
     case 0: by_width<0>(c, s.type, x, mask, n, 0, 0, 0, out, d_count); break;
     case 1: by_width<1>(c, s.type, x, nullptr, n, s.cmp, s.cbits, 0, out, d_count); break;
     default: launch<uint8_t, 2>(c, nullptr, mask, n, 0, 0, s.base, out, d_count); break;
+  }
+}
+
+void select_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts) {
+  fill_u64(c, d_counts, items.size(), 0);  // items without rows keep a count of 0
+  switch (s.kind) {
+    case 0: batch_by_width<0>(c, s, items, d_counts); break;
+    case 1: batch_by_width<1>(c, s, items, d_counts); break;
+    default: launch_batch<uint8_t, 2>(c, s, items, d_counts); break;
   }
 }
 

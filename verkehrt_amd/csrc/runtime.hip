@@ -167,6 +167,39 @@ void copy_d2d(Ctx &c, void *dst, const void *src, uint64_t bytes) {
   check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream), "copy");
 }
 
+BatchPlan batch_plan(Ctx &c, std::vector<BatchItem> &items, uint64_t rows_per_tile) {
+  BatchPlan p;
+  uint64_t tiles = 0;
+  for (auto &it : items) {
+    it.tile0 = tiles;
+    tiles += (it.n + rows_per_tile - 1) / rows_per_tile;
+  }
+  if (tiles > 0xFFFFFFFFull) usage_error("batch has too many tiles");
+  p.ntiles = static_cast<uint32_t>(tiles);
+  if (items.empty()) return p;
+  std::vector<uint32_t> map(tiles);
+  for (size_t i = 0; i < items.size(); ++i) {
+    const uint64_t t1 = i + 1 < items.size() ? items[i + 1].tile0 : tiles;
+    for (uint64_t t = items[i].tile0; t < t1; ++t) map[t] = static_cast<uint32_t>(i);
+  }
+  p.items = static_cast<BatchItem *>(tmp_alloc(c, items.size() * sizeof(BatchItem)));
+  p.tile_item = static_cast<uint32_t *>(tmp_alloc(c, (tiles ? tiles : 1) * 4));
+  check_hip(hipMemcpyAsync(p.items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                           c.stream), "batch items");
+  if (tiles)
+    check_hip(hipMemcpyAsync(p.tile_item, map.data(), tiles * 4, hipMemcpyHostToDevice, c.stream),
+              "batch tiles");
+  // the host vectors die with this call: wait for the copies
+  check_hip(hipStreamSynchronize(c.stream), "batch plan");
+  return p;
+}
+
+void batch_plan_free(Ctx &c, BatchPlan &p) {
+  if (p.items) tmp_free(c, p.items);
+  if (p.tile_item) tmp_free(c, p.tile_item);
+  p = BatchPlan{};
+}
+
 Ctx *create_ctx(int device) {
   check_hip(hipSetDevice(device), "hipSetDevice");
   auto *c = new Ctx();

@@ -178,6 +178,7 @@ void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, ui
 void canon_finish(Ctx &c, bool mul, const double *acc, const double *tail, int tail_len,
                   double *out);
 void canon_init(Ctx &c, bool mul, double *acc);
+// Batched canonical reductions (declared after BatchItem below).
 
 // Selections (k_select.hip): append the kept rows at `out`, the count to *d_count (device).
 // kind 0 compress (mask), 1 filter (cmp with cbits), 2 where (base).
@@ -190,6 +191,32 @@ struct SelectSpec {
 };
 void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
                  void *out, uint64_t *d_count);
+
+// Batched launches: many independent arrays in one launch per kernel phase. Tiles are
+// numbered over all items; every tile belongs to one item, and look-backs stop at the
+// item's first tile.
+struct BatchItem {
+  const void *a;   // first input rows
+  const void *b;   // second input rows (mask, y), or null
+  void *out;       // output rows
+  uint64_t n;      // rows
+  uint64_t tile0;  // first tile of the item in the launch
+  uint64_t aux;    // kernel-specific (where: index base)
+};
+struct BatchPlan {
+  BatchItem *items = nullptr;     // device copy of the items
+  uint32_t *tile_item = nullptr;  // device: the item of each tile
+  uint32_t ntiles = 0;
+};
+// Sets tile0 of every item for `rows_per_tile` rows per tile and uploads the items and the
+// tile -> item map.
+BatchPlan batch_plan(Ctx &c, std::vector<BatchItem> &items, uint64_t rows_per_tile);
+void batch_plan_free(Ctx &c, BatchPlan &p);
+// One selection launch over all items; the kept count of item i goes to d_counts[i].
+void select_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts);
+// One canonical-reduction launch over all items (a = x, b = y or null); item i's result to
+// d_out[i].
+void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, double *d_out);
 
 // Scans (k_scan.hip). `carry` is the scanner's device state {u64 acc, u64 has}.
 // op 0 add (out width 8, input widened from `type`), 1 min, 2 max.

@@ -398,6 +398,28 @@ class Dispatch:
             return Scalar.from_c(out.scalar)
         return Array(self.ctx, out.array)
 
+    def batch_call(self, name: str, items: Sequence[Sequence[Value]]) -> list:
+        """Runs kernel `name` on every argument list of `items` with one launch per kernel
+        phase (vk_batch_call); result i equals call(name, items[i])."""
+        if not items:
+            return []
+        nargs = len(items[0])
+        if any(len(it) != nargs for it in items):
+            raise ValueError("every item needs the same number of arguments")
+        flat = [v for it in items for v in it]
+        a = _args(flat)
+        out = (N.CValue * len(items))()
+        st = N.Status()
+        _check(self._lib.vk_batch_call(self.ctx.handle, name.encode(), a, nargs, len(items), out,
+                                       C.byref(st)), st)
+        res = []
+        for o in out:
+            if o.kind == N.VALUE_SCALAR:
+                res.append(Scalar.from_c(o.scalar))
+            else:
+                res.append(Array(self.ctx, o.array))
+        return res
+
     def reducer(self, name: str) -> Reducer:
         return Reducer(self.ctx, name)
 
