@@ -550,7 +550,7 @@ def fast_path_sets(rng, n):
     yield "drift-through-zero", drift
 
 
-@pytest.mark.parametrize("n", [8192, 8192 + 13, 65536 + 17, 1_000_003])
+@pytest.mark.parametrize("n", [8192, 16384, 16384 + 13, 65536 + 17, 1_000_003, 4_000_037])
 def test_canonical_fast_path(D, n):
     rng = np.random.default_rng(n)
     for name, x in fast_path_sets(rng, n):
@@ -561,6 +561,30 @@ def test_canonical_fast_path(D, n):
         want = O.condsum_f64(x, "gt", 0.25, -1.0)
         got = D.call("condsum_gt_f64", [dx, S("f64", 0.25), S("f64", -1.0)]).value
         assert same_float(want, got), (name, n)
+
+
+def test_many_tiles(D):
+    # every look-back kernel over thousands of tiles
+    rng = np.random.default_rng(5)
+    n = 6_000_011
+    x = rng.random(n)
+    dx = D.array(x)
+    for c in (0.01, 0.5, 0.99):
+        got = D.call("filter_gt_f64", [dx, S("f64", c)]).to_numpy()
+        assert np.array_equal(got, x[x > c]), c
+    m = (x < 0.3).astype(np.uint8)
+    assert np.array_equal(D.call("where", [D.array(m, "bool")]).to_numpy(), np.flatnonzero(m))
+    xi = rng.integers(-2**40, 2**40, n).astype(np.int64)
+    dxi = D.array(xi)
+    assert np.array_equal(D.call("scan_add_i64", [dxi]).to_numpy(), np.cumsum(xi))
+    st = (rng.random(n) < 1e-4).astype(np.uint8)
+    assert np.array_equal(D.call("scan_by_add_i64", [dxi, D.array(st, "bool")]).to_numpy(),
+                          O.scan_by_add_i64(xi, st))
+    assert np.array_equal(D.call("scan_max_i64", [dxi]).to_numpy(), np.maximum.accumulate(xi))
+    k = rng.integers(0, 1 << 20, n).astype(np.int64)
+    assert np.array_equal(D.call("argsort_i64", [D.array(k)]).to_numpy(), np.argsort(k, kind="stable"))
+    assert D.call("count_gt_f64", [dx, S("f64", 0.5)]).value == int((x > 0.5).sum())
+    assert D.call("argmax_f64", [dx]).value == int(np.argmax(x))
 
 
 def test_canonical_fast_path_tiles(D):

@@ -35,12 +35,12 @@ struct Binade {
 };
 
 // The summary of a run under one hypothesis, in units of u, for start parities 0 and 1.
+// An element too large for the binade, NaN or infinite makes the run unusable: its bounds
+// become [-inf, +inf] (or NaN), which no start value passes, and joins keep them so.
 struct RunFn {
   double d[2];   // offset after the run
   double lo[2];  // least offset after any element of the run (+inf for an empty run)
   double hi[2];  // greatest offset after any element (-inf for an empty run)
-  double bad;    // nonzero when some element was too large for the binade, NaN or infinite
-  double pad;
 };
 
 __host__ __device__ __forceinline__ double pow2(int k) {
@@ -78,7 +78,7 @@ __host__ __device__ __forceinline__ bool same_binade(const Binade &a, const Bina
 
 __host__ __device__ __forceinline__ RunFn run_identity() {
   const double inf = __builtin_inf();
-  return RunFn{{0.0, 0.0}, {inf, inf}, {-inf, -inf}, 0.0, 0.0};
+  return RunFn{{0.0, 0.0}, {inf, inf}, {-inf, -inf}};
 }
 
 // Parity of an integer-valued double t: 1 when t is odd.
@@ -106,7 +106,7 @@ __host__ __device__ __forceinline__ void run_push(RunFn &f, const BinadeRef &r, 
   const double a0 = ((r.z0 + v) - r.z0) * r.inv_u;
   const double a1 = ((r.z1 + v) - r.z1) * r.inv_u;
   const double av = v < 0 ? -v : v;
-  if (!(av < r.lim)) f.bad = 1.0;
+  const double inf = __builtin_inf();
   // track p: the current parity is p xor parity(offset)
   {
     const double d = parity(f.d[0]) ? a1 : a0;
@@ -119,6 +119,10 @@ __host__ __device__ __forceinline__ void run_push(RunFn &f, const BinadeRef &r, 
     f.d[1] = f.d[1] + d;
     f.lo[1] = f.d[1] < f.lo[1] ? f.d[1] : f.lo[1];
     f.hi[1] = f.d[1] > f.hi[1] ? f.d[1] : f.hi[1];
+  }
+  if (!(av < r.lim)) {
+    f.lo[0] = f.lo[1] = -inf;
+    f.hi[0] = f.hi[1] = inf;
   }
 }
 
@@ -133,8 +137,6 @@ __host__ __device__ __forceinline__ RunFn run_join(const RunFn &a, const RunFn &
     o.lo[p] = a.lo[p] < blo ? a.lo[p] : blo;
     o.hi[p] = a.hi[p] > bhi ? a.hi[p] : bhi;
   }
-  o.bad = (a.bad != 0.0 || b.bad != 0.0) ? 1.0 : 0.0;
-  o.pad = 0.0;
   return o;
 }
 
@@ -142,7 +144,7 @@ __host__ __device__ __forceinline__ RunFn run_join(const RunFn &a, const RunFn &
 // end value in *out when the summary proves the sequential fold equals it.
 __host__ __device__ __forceinline__ bool run_apply(const RunFn &f, const Binade &h, double s,
                                                    double *out) {
-  if (!h.ok || f.bad != 0.0) return false;
+  if (!h.ok) return false;
   const Binade hs = binade_of(s);
   if (!same_binade(hs, h)) return false;
   const double inv_u = pow2(52 - h.e);

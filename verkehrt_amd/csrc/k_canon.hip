@@ -67,31 +67,42 @@ __global__ __launch_bounds__(64) void k_canon_chain(RowVal rv, bool mul, const d
 }
 
 // ------------------------------------------------------------------ fast form
-// The 16 chains over tiles of 256 blocks (4096 rows), one pass (canon_binade.h):
-//  1. thread (chain j, run r) loads the 16 blocks 16r..16r+15 of its chain (4 x 128-byte
-//     segments per wave load) and keeps the row values in LDS for a sequential fallback;
+// The 16 chains over tiles of 1024 blocks (16384 rows), one pass (canon_binade.h); a block
+// of 1024 threads, thread t owning chain j = t % 16 and run r = t / 16 (64 runs of 16 blocks):
+//  1. each thread loads the 16 rows of its chain in its run (a wave load covers four
+//     128-byte blocks) and keeps them in registers;
 //  2. per-chain approximate sums, and a decoupled look-back over them, give each chain an
 //     approximate start and so a binade hypothesis;
-//  3. every thread summarises its run under the hypothesis, runs join in order (two
-//     shuffles, then across the four waves) into the tile summary;
+//  3. every thread summarises its run under the hypothesis; runs join in order (two
+//     shuffles per wave, then a tree over the 16 waves) into the tile summary;
 //  4. a second decoupled look-back joins the summaries of preceding tiles with the same
 //     hypothesis back to a tile whose exact end values are published, applies the joined
 //     summary to that exact value, and publishes this tile's exact end values. A failed
 //     check waits for the previous tile's exact values and folds the tile's rows of the
-//     chain sequentially from LDS.
-constexpr int kTileBlocks = 256;
-constexpr uint64_t kFastMinBlocks = 512;
+//     chain sequentially, re-read from memory.
+// Tiles are large so that the tile ticket (one same-address atomic per tile) never limits
+// the row rate; look-back status is published with relaxed agent-scope stores ordered by
+// publish_fence, never with release/acquire (which would write back and invalidate L2).
+constexpr int kFastThreads = 1024;
+constexpr int kFastWaves = kFastThreads / kWave;
+constexpr int kRunBlocks = 16;
+constexpr int kTileBlocks = kFastThreads / 16 * kRunBlocks;  // 1024 blocks
+constexpr uint64_t kFastMinBlocks = kTileBlocks;
 constexpr int kStride = 192;  // status words per tile
 constexpr int W_AAGG = 1, W_AINC = 17, W_F = 33, W_EXACT = 161;
 constexpr uint64_t F_AAGG = 1, F_AINC = 2, F_FN = 4, F_EXACT = 8;
+static_assert(kFastWaves == 16, "the tile summary tree joins 16 waves as 4 x 4");
 
 __device__ __forceinline__ uint64_t flag_word(uint32_t epoch, uint64_t bits) {
   return (uint64_t(epoch) << 8) | bits;
 }
 __device__ __forceinline__ uint64_t wait_flag(const uint64_t *st, uint32_t epoch, uint64_t need) {
   while (true) {
-    const uint64_t w = ld_acq(st);
-    if ((w >> 8) == epoch && (w & need)) return w;
+    const uint64_t w = ld_rlx(st);
+    if ((w >> 8) == epoch && (w & need)) {
+      observe_fence();
+      return w;
+    }
     backoff();
   }
 }
@@ -109,20 +120,91 @@ __device__ __forceinline__ Binade unpack_binade(uint64_t w) {
   return Binade{static_cast<int>(w >> 2) - 2048, static_cast<int>((w >> 1) & 1), static_cast<int>(w & 1), 0};
 }
 
+// The exact fold of chain c's rows of blocks b0 .. b0 + nb - 1 from the exact value s, by
+// one wave: lane r holds run r (blocks 16r .. 16r + 15). Runs are summarised under the binade
+// of the current value and scanned in order; the runs before the first one whose check fails
+// are applied at once, that run is folded sequentially by its lane, and the rest is taken up
+// again from the new value. Each round passes at least one run, and a round only fails where
+// the value leaves its binade (or meets a huge, infinite or NaN element), so typical chains
+// take two rounds.
+__device__ double eval_chain(const RowVal &rv, const double *x, const double *y, uint64_t b0, int nb,
+                             int c, double s) {
+  const int l = lane_id();
+  double v[kRunBlocks];
+#pragma unroll
+  for (int k = 0; k < kRunBlocks; ++k) {
+    const int lb = l * kRunBlocks + k;
+    v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + c) : 0.0;
+  }
+  const int nruns = (nb + kRunBlocks - 1) / kRunBlocks;
+  double cur = s;
+  for (int r0 = 0; r0 < nruns;) {
+    const Binade h = binade_of(cur);
+    RunFn f = run_identity();
+    if (h.ok && l >= r0) {
+      const BinadeRef ref = binade_ref(h);
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k)
+        if (l * kRunBlocks + k < nb) run_push(f, ref, v[k]);
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const RunFn o = shfl_up_pod(f, d);
+      if (l >= d) f = run_join(o, f);
+    }
+    double val = cur;
+    const bool ok = l < r0 || run_apply(f, h, cur, &val);
+    const uint64_t fails = __ballot(!ok);
+    if (fails == 0) {
+      cur = __shfl(val, kWave - 1);
+      break;
+    }
+    const int m = __builtin_ctzll(fails);
+    double t = m > r0 ? __shfl(val, m - 1) : cur;
+    if (l == m) {
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k)
+        if (m * kRunBlocks + k < nb) t = t + v[k];
+    }
+    cur = __shfl(t, m);
+    r0 = m + 1;
+  }
+  return cur;
+}
+
+// Joins the summaries of one wave's four look-back slots (lane l holds slot l / 16 of chain
+// l % 16; a higher slot is an older tile) so that every lane ends with older-to-newer order.
+__device__ __forceinline__ RunFn join_slots(RunFn f, int l) {
+  RunFn o = shfl_xor_pod(f, 16);
+  f = (l & 16) ? run_join(f, o) : run_join(o, f);
+  o = shfl_xor_pod(f, 32);
+  f = (l & 32) ? run_join(f, o) : run_join(o, f);
+  return f;
+}
+
 // BATCH: the tile's item gives the rows (a, b), the block count (n / 16), the first tile, and
 // the item's 16 start and end accumulators at acc_in / acc_out + 16 * item.
+//
+// Both look-backs are windowed: thread t reads tile `end - 1 - i` for slot i = 4 * (t / 64) +
+// (t % 64) / 16 and chain t % 16, so one round reads 64 predecessors in parallel; a
+// block-wide minimum finds the nearest tile that ends the look-back (per chain for the exact
+// one), and the slots before it are summed (approximate) or joined in order (exact).
 template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *__restrict__ x,
-                                                       const double *__restrict__ y, uint64_t blocks,
-                                                       const double *acc_in, double *acc_out,
-                                                       uint64_t *status, uint32_t epoch,
-                                                       uint32_t *ticket, uint32_t ntiles,
-                                                       const BatchItem *items,
-                                                       const uint32_t *tile_item) {
-  __shared__ double s_val[kTileBlocks * 16];
-  __shared__ double s_a[kBlock / kWave][16];
-  __shared__ RunFn s_f[kBlock / kWave][16];
+__global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const double *__restrict__ x,
+                                                             const double *__restrict__ y, uint64_t blocks,
+                                                             const double *acc_in, double *acc_out,
+                                                             uint64_t *status, uint32_t epoch,
+                                                             uint32_t *ticket, uint32_t ntiles,
+                                                             const BatchItem *items,
+                                                             const uint32_t *tile_item) {
+  __shared__ double s_a[kFastWaves][16];
+  __shared__ RunFn s_f[kFastWaves][16];
   __shared__ Binade s_h[16];
+  __shared__ int s_stop[16];
+  __shared__ int s_more;
+  __shared__ double s_base[16];
+  __shared__ double s_exact[16];
+  __shared__ int s_fail[16];
   const uint32_t tile = take_ticket(ticket, ntiles);
   uint32_t first = 0, last = ntiles - 1;
   if (BATCH) {
@@ -138,75 +220,93 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
   }
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const int j = t & 15, r = t >> 4;
+  const int slot = 4 * w + (l >> 4);  // look-back slot of this thread (chain j)
   const uint64_t b0 = uint64_t(tile - first) * kTileBlocks;
   const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
   uint64_t *st = status + uint64_t(tile) * kStride;
 
   // 1. load the run
-  double v[16];
+  double v[kRunBlocks];
   double a = 0.0;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int lb = r * 16 + k;
-    if (lb < nb) {
-      v[k] = rv(x, y, (b0 + lb) * 16 + j);
-      s_val[lb * 16 + j] = v[k];
-      a = a + v[k];
-    } else {
-      v[k] = 0.0;
-    }
+  for (int k = 0; k < kRunBlocks; ++k) {
+    const int lb = r * kRunBlocks + k;
+    v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
+    a = a + v[k];
   }
-  // 2. approximate chain sums of the tile
+  // 2. approximate chain sums of the tile, published as an aggregate (or inclusive at the
+  //    item's first tile)
   a = a + __shfl_xor(a, 16);
   a = a + __shfl_xor(a, 32);
   if (l < 16) s_a[w][j] = a;
+  if (t == 0) s_stop[0] = 64;
   __syncthreads();
+  double atile = 0.0;  // lanes 0..15 of wave 0
   if (w == 0) {
-    double atile = 0.0, pre = 0.0;
     if (l < 16) {
-      atile = s_a[0][l] + s_a[1][l] + s_a[2][l] + s_a[3][l];
-      if (tile == first) {
-        pre = acc_in[l];
-        st_d(st + W_AINC + l, pre + atile);
+#pragma unroll
+      for (int k = 0; k < kFastWaves; ++k) atile = atile + s_a[k][l];
+      if (tile == first) st_d(st + W_AINC + l, acc_in[l] + atile);
+      else st_d(st + W_AAGG + l, atile);
+    }
+    publish_fence();
+    if (l == 0) st_rlx(st, flag_word(epoch, tile == first ? (F_AAGG | F_AINC) : F_AAGG));
+  }
+  // 2b. approximate look-back: the sum of aggregates back to an inclusive or exact tile
+  double pre = 0.0;  // lanes 0..15 of wave 0
+  if (tile == first) {
+    if (w == 0 && l < 16) pre = acc_in[l];
+  } else {
+    for (int64_t end = tile;; end -= 64) {
+      __syncthreads();  // s_a and s_stop[0] are free
+      const int64_t T = end - 1 - slot;
+      bool stop = true;
+      double val = 0.0;
+      if (T < int64_t(first)) {
+        val = acc_in[j];
       } else {
-        st_d(st + W_AAGG + l, atile);
-      }
-    }
-    __threadfence();
-    if (l == 0) st_rel(st, flag_word(epoch, tile == first ? (F_AAGG | F_AINC) : F_AAGG));
-    if (tile > first && l < 16) {
-      double sum = 0.0;
-      for (int64_t k = int64_t(tile) - 1;; --k) {
-        const uint64_t *sk = status + uint64_t(k) * kStride;
+        const uint64_t *sk = status + uint64_t(T) * kStride;
         const uint64_t fw = wait_flag(sk, epoch, F_AAGG | F_AINC | F_EXACT);
-        if (fw & F_EXACT) {
-          sum = sum + ld_d(sk + W_EXACT + l);
-          break;
+        if (fw & F_EXACT) val = ld_d(sk + W_EXACT + j);
+        else if (fw & F_AINC) val = ld_d(sk + W_AINC + j);
+        else {
+          val = ld_d(sk + W_AAGG + j);
+          stop = false;
         }
-        if (fw & F_AINC) {
-          sum = sum + ld_d(sk + W_AINC + l);
-          break;
-        }
-        sum = sum + ld_d(sk + W_AAGG + l);
       }
-      pre = sum;
-      st_d(st + W_AINC + l, pre + atile);
+      if (stop) atomicMin(&s_stop[0], slot);
+      __syncthreads();
+      const int sidx = s_stop[0];
+      double cv = slot <= sidx ? val : 0.0;
+      cv = cv + __shfl_xor(cv, 16);
+      cv = cv + __shfl_xor(cv, 32);
+      if (l < 16) s_a[w][j] = cv;
+      __syncthreads();
+      if (w == 0 && l < 16) {
+#pragma unroll
+        for (int k = 0; k < kFastWaves; ++k) pre = pre + s_a[k][l];
+      }
+      if (sidx < 64) break;
+      if (t == 0) s_stop[0] = 64;
     }
+  }
+  if (w == 0) {
     if (tile > first) {
-      __threadfence();
-      if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC));
+      if (l < 16) st_d(st + W_AINC + l, pre + atile);
+      publish_fence();
+      if (l == 0) st_rlx(st, flag_word(epoch, F_AAGG | F_AINC));
     }
     if (l < 16) s_h[l] = binade_of(pre);
   }
   __syncthreads();
-  // 3. summarise the run under the chain's hypothesis, join the runs in order
+  // 3. summarise the run under the chain's hypothesis, join the runs of the wave in order
   const Binade h = s_h[j];
   RunFn f = run_identity();
   if (h.ok) {
     const BinadeRef ref = binade_ref(h);
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
-      if (r * 16 + k < nb) run_push(f, ref, v[k]);
+    for (int k = 0; k < kRunBlocks; ++k)
+      if (r * kRunBlocks + k < nb) run_push(f, ref, v[k]);
   }
   {
     RunFn o = shfl_xor_pod(f, 16);
@@ -216,78 +316,140 @@ __global__ __launch_bounds__(kBlock) void k_canon_fast(RowVal rv, const double *
   }
   if (l < 16) s_f[w][j] = f;
   __syncthreads();
-  if (w != 0) return;
-  // 4. publish the tile summary, then the exact look-back (lane = chain)
+  // 4. the tile summary (wave 0: lane l joins waves 4q..4q+3 of chain l % 16, q = l / 16,
+  //    then the four quarters by shuffles), published
   RunFn ft = run_identity();
-  Binade hj{0, 0, 0, 0};
-  if (l < 16) {
-    hj = s_h[l];
-    ft = run_join(run_join(s_f[0][l], s_f[1][l]), run_join(s_f[2][l], s_f[3][l]));
-    uint64_t *fp = st + W_F + 8 * l;
-    st_d(fp + 0, ft.d[0]);
-    st_d(fp + 1, ft.d[1]);
-    st_d(fp + 2, ft.lo[0]);
-    st_d(fp + 3, ft.lo[1]);
-    st_d(fp + 4, ft.hi[0]);
-    st_d(fp + 5, ft.hi[1]);
-    st_d(fp + 6, ft.bad);
-    st_rlx(fp + 7, pack_binade(hj));
-  }
-  __threadfence();
-  if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC | F_FN));
-  double exact = 0.0;
-  if (l < 16) {
-    RunFn g = run_identity();  // joined summaries of the tiles between `base` and this one
-    double base = 0.0;
-    int64_t k = int64_t(tile) - 1;
-    for (;; --k) {
-      if (k < int64_t(first)) {
-        base = acc_in[l];
-        break;
-      }
-      const uint64_t *sk = status + uint64_t(k) * kStride;
-      const uint64_t fw = wait_flag(sk, epoch, F_FN | F_EXACT);
-      if (fw & F_EXACT) {
-        base = ld_d(sk + W_EXACT + l);
-        break;
-      }
-      const uint64_t *fp = sk + W_F + 8 * l;
-      const Binade hk = unpack_binade(ld_rlx(fp + 7));
-      if (!same_binade(hk, hj)) {
-        wait_flag(sk, epoch, F_EXACT);
-        base = ld_d(sk + W_EXACT + l);
-        break;
-      }
-      RunFn fk;
-      fk.d[0] = ld_d(fp + 0);
-      fk.d[1] = ld_d(fp + 1);
-      fk.lo[0] = ld_d(fp + 2);
-      fk.lo[1] = ld_d(fp + 3);
-      fk.hi[0] = ld_d(fp + 4);
-      fk.hi[1] = ld_d(fp + 5);
-      fk.bad = ld_d(fp + 6);
-      fk.pad = 0.0;
-      g = run_join(fk, g);
+  const Binade hj = s_h[j];
+  if (w == 0) {
+    const int q = l >> 4;
+    ft = run_join(run_join(s_f[4 * q][j], s_f[4 * q + 1][j]), run_join(s_f[4 * q + 2][j], s_f[4 * q + 3][j]));
+    RunFn o = shfl_xor_pod(ft, 16);
+    ft = (l & 16) ? run_join(o, ft) : run_join(ft, o);
+    o = shfl_xor_pod(ft, 32);
+    ft = (l & 32) ? run_join(o, ft) : run_join(ft, o);
+    if (l < 16) {
+      uint64_t *fp = st + W_F + 8 * l;
+      st_d(fp + 0, ft.d[0]);
+      st_d(fp + 1, ft.d[1]);
+      st_d(fp + 2, ft.lo[0]);
+      st_d(fp + 3, ft.lo[1]);
+      st_d(fp + 4, ft.hi[0]);
+      st_d(fp + 5, ft.hi[1]);
+      st_rlx(fp + 7, pack_binade(hj));
     }
-    bool done = run_apply(run_join(g, ft), hj, base, &exact);
-    if (!done) {
-      if (k != int64_t(tile) - 1 && tile > first) {
+    publish_fence();
+    if (l == 0) st_rlx(st, flag_word(epoch, F_AAGG | F_AINC | F_FN));
+  }
+  // 5. exact look-back: per chain, join the summaries of same-hypothesis tiles back to a tile
+  //    with published exact values (or the item's start); a tile with another hypothesis
+  //    ends it too, and its exact values are awaited
+  RunFn g = run_identity();  // wave 0 lanes 0..15: joined summaries after `base`
+  double base = 0.0;
+  int64_t base_tile = int64_t(tile) - 1;  // the tile whose end `base` is
+  if (tile == first) {
+    if (w == 0 && l < 16) base = acc_in[l];
+  } else {
+    if (t < 16) s_stop[t] = 64;
+    bool open = true;  // wave 0 lanes 0..15: chain still looking back
+    for (int64_t end = tile;; end -= 64) {
+      __syncthreads();  // s_f, s_stop and s_more are free
+      const int64_t T = end - 1 - slot;
+      const bool chain_open = s_stop[j] >= 0;  // -1 marks a finished chain
+      RunFn fk = run_identity();
+      bool stop = false;
+      if (chain_open) {
+        if (T < int64_t(first)) {
+          stop = true;
+        } else {
+          const uint64_t *sk = status + uint64_t(T) * kStride;
+          const uint64_t fw = wait_flag(sk, epoch, F_FN | F_EXACT);
+          const uint64_t *fp = sk + W_F + 8 * j;
+          if (fw & F_EXACT) {
+            stop = true;
+          } else if (!same_binade(unpack_binade(ld_rlx(fp + 7)), hj)) {
+            stop = true;
+          } else {
+            fk.d[0] = ld_d(fp + 0);
+            fk.d[1] = ld_d(fp + 1);
+            fk.lo[0] = ld_d(fp + 2);
+            fk.lo[1] = ld_d(fp + 3);
+            fk.hi[0] = ld_d(fp + 4);
+            fk.hi[1] = ld_d(fp + 5);
+          }
+        }
+      }
+      __syncthreads();  // every thread has read s_stop[j] for chain_open
+      if (stop) atomicMin(&s_stop[j], slot);
+      __syncthreads();
+      const int sidx = s_stop[j];
+      if (!(chain_open && slot < sidx)) fk = run_identity();
+      fk = join_slots(fk, l);
+      if (l < 16) s_f[w][j] = fk;
+      __syncthreads();
+      if (w == 0) {
+        const int q = l >> 4;
+        // waves 4q+3 (oldest) .. 4q, then the quarters, older first
+        RunFn gw = run_join(run_join(s_f[4 * q + 3][j], s_f[4 * q + 2][j]),
+                            run_join(s_f[4 * q + 1][j], s_f[4 * q][j]));
+        gw = join_slots(gw, l);
+        if (l < 16 && open) {
+          g = run_join(gw, g);
+          if (sidx < 64) {
+            const int64_t Ts = end - 1 - sidx;
+            base_tile = Ts;
+            if (Ts < int64_t(first)) {
+              base = acc_in[l];
+            } else {
+              const uint64_t *ss = status + uint64_t(Ts) * kStride;
+              wait_flag(ss, epoch, F_EXACT);
+              base = ld_d(ss + W_EXACT + l);
+            }
+            open = false;
+          }
+        }
+        const bool more = __ballot(l < 16 && open) != 0;
+        if (l < 16) s_stop[l] = open ? 64 : -1;
+        if (l == 0) s_more = more;
+      }
+      __syncthreads();
+      if (!s_more) break;
+    }
+  }
+  // 6. apply; a chain whose check fails is evaluated exactly by wave (chain), from the exact
+  //    value at the tile's start: `base` moved over the joined predecessors, or, where that
+  //    check fails too, the previous tile's published end
+  if (w == 0 && l < 16) {
+    double exact = 0.0;
+    const bool done = run_apply(run_join(g, ft), hj, base, &exact);
+    if (!done && base_tile != int64_t(tile) - 1) {
+      double start = 0.0;
+      if (run_apply(g, hj, base, &start)) {
+        base = start;
+      } else {
         const uint64_t *sp = status + uint64_t(tile - 1) * kStride;
         wait_flag(sp, epoch, F_EXACT);
         base = ld_d(sp + W_EXACT + l);
-        done = run_apply(ft, hj, base, &exact);
-      }
-      if (!done) {
-        double s = base;
-        for (int lb = 0; lb < nb; ++lb) s = s + s_val[lb * 16 + l];
-        exact = s;
       }
     }
+    s_base[l] = base;
+    s_exact[l] = exact;
+    s_fail[l] = !done;
+  }
+  __syncthreads();
+  static_assert(kFastWaves == 16, "one wave per chain");
+  if (s_fail[w]) {
+    const double e = eval_chain(rv, x, y, b0, nb, w, s_base[w]);
+    if (l == 0) s_exact[w] = e;
+  }
+  __syncthreads();
+  if (w != 0) return;
+  if (l < 16) {
+    const double exact = s_exact[l];
     st_d(st + W_EXACT + l, exact);
     if (tile == last) acc_out[l] = exact;
   }
-  __threadfence();
-  if (l == 0) st_rel(st, flag_word(epoch, F_AAGG | F_AINC | F_FN | F_EXACT));
+  publish_fence();
+  if (l == 0) st_rlx(st, flag_word(epoch, F_AAGG | F_AINC | F_FN | F_EXACT));
 }
 
 __global__ void k_canon_rest(RowVal rv, const double *x, const double *y, uint64_t rest,
@@ -360,7 +522,7 @@ void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, 
   const uint32_t epoch = next_epoch(c);
   double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
   copy_d2d(c, acc_in, acc, 16 * 8);
-  k_canon_fast<false><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y ? y : x, blocks, acc_in, acc,
+  k_canon_fast<false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y ? y : x, blocks, acc_in, acc,
                                                                status, epoch, c.scratch.counters,
                                                                uint32_t(tiles), nullptr, nullptr);
   check_hip(hipGetLastError(), "canon fast");
@@ -384,7 +546,7 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
     if (s.kind == 4) usage_error("batched products are not supported");
     uint64_t *status = status_words(c, ST_CANON, uint64_t(p.ntiles) * kStride);
     const uint32_t epoch = next_epoch(c);
-    k_canon_fast<true><<<p.ntiles, kBlock, 0, c.stream>>>(rv, nullptr, nullptr, 0, zero, acc, status, epoch,
+    k_canon_fast<true><<<p.ntiles, kFastThreads, 0, c.stream>>>(rv, nullptr, nullptr, 0, zero, acc, status, epoch,
                                                          c.scratch.counters, p.ntiles, p.items, p.tile_item);
     check_hip(hipGetLastError(), "canon batch");
   }

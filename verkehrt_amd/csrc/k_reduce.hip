@@ -186,14 +186,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce(Op op, const T *__restrict__ 
   a = block_merge(op, a);
   __shared__ bool s_last;
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = a;
-    __threadfence();
+    // the partial goes out through agent-coherent stores, complete before the count
+    st_rlx_pod(&partials[blockIdx.x], a);
+    publish_fence();
     uint32_t prev = atomicAdd(done, 1u);
     s_last = prev == gridDim.x - 1;
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
+  observe_fence();
   Acc b = op.identity();
   for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
     // partials of other XCDs: read through the agent-coherent path

@@ -3,7 +3,7 @@
 // min and max (IEEE minimum/maximum on floats, ScanExtreme scan.rs:164-183). A start byte
 // restarts the fold; row 0 of the whole array always starts.
 //
-// One pass: a 4096-row tile is staged through LDS into 16 consecutive rows per thread,
+// One pass: an 8192-row tile (512 threads x 16 rows) is staged through LDS into 16 consecutive rows per thread,
 // scanned per thread, per wave (shuffles) and per block, then joined to the tiles before
 // it by decoupled look-back on (aggregate, has-start) pairs: a predecessor whose aggregate
 // holds a start, or that published its inclusive value, ends the look-back. The operators
@@ -16,8 +16,9 @@ namespace vk {
 namespace {
 
 constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;
-constexpr int kWaves = kBlock / kWave;
+constexpr int kScanBlock = 512;
+constexpr int kTile = kScanBlock * kItems;  // 8192 rows: ~78 KB of LDS, two blocks per CU
+constexpr int kWaves = kScanBlock / kWave;
 
 template <typename U, int OP>
 __device__ __forceinline__ U apply(U a, U b) {
@@ -55,10 +56,11 @@ __device__ U look_back(uint64_t *status, uint32_t epoch, uint32_t tile, bool *ha
     const int64_t idx = end - 64 + l;
     uint64_t w;
     while (true) {
-      w = idx >= 0 ? ld_acq(&status[idx * 4 + 2]) : ((uint64_t(epoch) << 8) | 2);
+      w = idx >= 0 ? ld_rlx(&status[idx * 4 + 2]) : ((uint64_t(epoch) << 8) | 2);
       if (__all(stat_ready(w, epoch))) break;
       backoff();
     }
+    observe_fence();
     // A lane ends the look-back when it is inclusive or its aggregate holds a start.
     const bool stop = idx < 0 || (w & 2) || (w & 4);
     const uint64_t stops = __ballot(stop);
@@ -92,7 +94,7 @@ __device__ U look_back(uint64_t *status, uint32_t epoch, uint32_t tile, bool *ha
 }
 
 template <typename TIn, typename U, int OP, bool BY>
-__global__ __launch_bounds__(kBlock) void k_scan(const TIn *__restrict__ x,
+__global__ __launch_bounds__(kScanBlock) void k_scan(const TIn *__restrict__ x,
                                                  const uint8_t *__restrict__ starts, uint64_t n,
                                                  U *__restrict__ out, uint64_t *status,
                                                  uint32_t epoch, uint32_t *ticket, uint32_t ntiles,
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const TIn *__restrict__ x,
   // 1. coalesced load into LDS, padded one slot per 16 rows
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const int lr = k * kBlock + t;
+    const int lr = k * kScanBlock + t;
     const uint64_t r = t0 + lr;
     U v{};
     uint8_t f = 0;
@@ -123,19 +125,13 @@ __global__ __launch_bounds__(kBlock) void k_scan(const TIn *__restrict__ x,
     s_f[lr] = f;
   }
   __syncthreads();
-  // 2. sequential inclusive scan of 16 consecutive rows per thread
+  // 2. the aggregate of the thread's 16 consecutive rows (the rows stay in LDS)
   Seg<U> run{U{}, 0, 0};
-  U loc[kItems];
-  uint32_t locf = 0;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int lr = t * kItems + j;
-    const U v = s_v[lr + (lr >> 4)];
-    const uint32_t f = s_f[lr];
-    if (j == 0) run = Seg<U>{v, f, 0};
-    else run = join<U, OP>(run, Seg<U>{v, f, 0});
-    loc[j] = run.v;
-    locf |= f << j;
+    const Seg<U> e{s_v[lr + (lr >> 4)], s_f[lr], 0};
+    run = j == 0 ? e : join<U, OP>(run, e);
   }
   // 3. wave inclusive scan of thread aggregates
   Seg<U> inc = run;
@@ -161,7 +157,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(const TIn *__restrict__ x,
     } else {
       if (l == 0) {
         st_rlx(&status[uint64_t(tile) * 4 + 0], to_bits(agg.v));
-        st_rel(&status[uint64_t(tile) * 4 + 2], (uint64_t(epoch) << 8) | 1 | (agg.f ? 4 : 0));
+        publish_fence();
+        st_rlx(&status[uint64_t(tile) * 4 + 2], (uint64_t(epoch) << 8) | 1 | (agg.f ? 4 : 0));
       }
       bool h;
       U pv = look_back<U, OP>(status, epoch, tile, &h);
@@ -171,7 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(const TIn *__restrict__ x,
     const Seg<U> incl = have ? join<U, OP>(pre, agg) : agg;
     if (l == 0) {
       st_rlx(&status[uint64_t(tile) * 4 + 1], to_bits(incl.v));
-      st_rel(&status[uint64_t(tile) * 4 + 2], (uint64_t(epoch) << 8) | 2 | 1 | (agg.f ? 4 : 0));
+      publish_fence();
+      st_rlx(&status[uint64_t(tile) * 4 + 2], (uint64_t(epoch) << 8) | 2 | 1 | (agg.f ? 4 : 0));
       s_prefix = pre;
       s_prefix.f = have ? 1 : 0;  // here f marks "a prefix exists"
     }
@@ -189,19 +187,19 @@ __global__ __launch_bounds__(kBlock) void k_scan(const TIn *__restrict__ x,
     pre = have ? join<U, OP>(pre, excl_in_wave) : excl_in_wave;
     have = true;
   }
-  // rows before the thread's first start take the prefix
+  // the thread's rows again, folded from the prefix (a start restarts the fold)
+  U cur = pre.v;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
-    const bool started = (locf & ((2u << j) - 1)) != 0;
-    U v = loc[j];
-    if (!started && have) v = apply<U, OP>(pre.v, v);
     const int lr = t * kItems + j;
-    s_v[lr + (lr >> 4)] = v;
+    const U v = s_v[lr + (lr >> 4)];
+    cur = (s_f[lr] || (j == 0 && !have)) ? v : apply<U, OP>(cur, v);
+    s_v[lr + (lr >> 4)] = cur;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const int lr = k * kBlock + t;
+    const int lr = k * kScanBlock + t;
     const uint64_t r = t0 + lr;
     if (r < n) {
       const U v = s_v[lr + (lr >> 4)];
@@ -223,11 +221,11 @@ void launch(Ctx &c, bool by, const void *x, const uint8_t *starts, uint64_t n, v
   uint64_t *carry_in = c.scratch.slots + 8;
   copy_d2d(c, carry_in, carry, 16);
   if (by)
-    k_scan<TIn, U, OP, true><<<uint32_t(tiles), kBlock, 0, c.stream>>>(
+    k_scan<TIn, U, OP, true><<<uint32_t(tiles), kScanBlock, 0, c.stream>>>(
         static_cast<const TIn *>(x), starts, n, static_cast<U *>(out), status, epoch,
         c.scratch.counters, uint32_t(tiles), carry_in, carry);
   else
-    k_scan<TIn, U, OP, false><<<uint32_t(tiles), kBlock, 0, c.stream>>>(
+    k_scan<TIn, U, OP, false><<<uint32_t(tiles), kScanBlock, 0, c.stream>>>(
         static_cast<const TIn *>(x), starts, n, static_cast<U *>(out), status, epoch,
         c.scratch.counters, uint32_t(tiles), carry_in, carry);
   check_hip(hipGetLastError(), "scan launch");

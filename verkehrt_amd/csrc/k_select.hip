@@ -1,7 +1,7 @@
 // Selections: compress_<t> (select.rs:79-92), filter_<cmp>_<t> (fused.rs:212-246) and where
 // (select.rs:94-104). The kept rows keep their row order.
 //
-// One pass, no count pass: each 4096-row tile flags its rows, ranks the kept ones with
+// One pass, no count pass: each 16384-row tile flags its rows, ranks the kept ones with
 // wave64 ballots and mbcnt, learns the number of kept rows before it by decoupled
 // look-back over the tiles' published counts, and writes its survivors. Algorithmic bytes:
 // n x (width [+1 mask byte]) read, kept x width written.
@@ -11,9 +11,12 @@
 namespace vk {
 namespace {
 
+// 1024 threads x 16 rows: large enough that the tile ticket (one same-address atomic per
+// tile, about 11 ns apart at best) never limits the row rate.
+constexpr int kSelBlock = 1024;
 constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;  // 4096 rows
-constexpr int kWaves = kBlock / kWave;
+constexpr int kTile = kSelBlock * kItems;  // 16384 rows
+constexpr int kWaves = kSelBlock / kWave;
 
 // Look-back word: epoch (16 bits) | flag (2 bits: 1 aggregate, 2 inclusive) | count (46 bits).
 __device__ __forceinline__ uint64_t pack(uint32_t epoch, uint64_t flag, uint64_t v) {
@@ -56,7 +59,7 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
 // BATCH: the tile's item (row pointers, row count, output, first tile, index base) comes
 // from the batch tables, and the item's last tile writes its count to counts[item].
 template <typename T, int KIND, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_select(const T *__restrict__ x,
+__global__ __launch_bounds__(kSelBlock) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
                                                    int cmp, T c, uint64_t base, void *out_v,
                                                    uint64_t *status, uint32_t epoch,
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_select(const T *__restrict__ x,
   __shared__ uint64_t s_prefix;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const uint64_t r = t0 + uint64_t(i) * kBlock + t;
+    const uint64_t r = t0 + uint64_t(i) * kSelBlock + t;
     bool keep = false;
     if (r < n) {
       if (KIND == 0) {
@@ -108,26 +111,39 @@ __global__ __launch_bounds__(kBlock) void k_select(const T *__restrict__ x,
     if (l == 0) s_cnt[i][w] = __popcll(keepm[i]);
   }
   __syncthreads();
-  // Exclusive scan of the kItems x kWaves counts in row order (item-major), by wave 0.
+  // Exclusive scan of the kItems x kWaves counts in row order (item-major), by wave 0:
+  // lane l owns the four consecutive entries 4l..4l+3.
+  static_assert(kItems * kWaves == 4 * kWave, "four counts per lane");
   if (w == 0) {
-    const int k = l;  // kItems * kWaves == 64
-    uint32_t v = s_cnt[k / kWaves][k % kWaves];
+    uint32_t c4[4], v = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = 4 * l + q;
+      c4[q] = s_cnt[k / kWaves][k % kWaves];
+      v += c4[q];
+    }
     uint32_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       uint32_t o = __shfl_up(inc, d);
       if (l >= d) inc += o;
     }
-    s_excl[k / kWaves][k % kWaves] = inc - v;
+    uint32_t e = inc - v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = 4 * l + q;
+      s_excl[k / kWaves][k % kWaves] = e;
+      e += c4[q];
+    }
     const uint32_t total = __shfl(inc, 63);
-    // publish and look back
+    // publish and look back; the status word carries its payload, so relaxed stores suffice
     uint64_t prefix = 0;
     if (tile == first) {
-      if (l == 0) st_rel(&status[tile], pack(epoch, 2, total));
+      if (l == 0) st_rlx(&status[tile], pack(epoch, 2, total));
     } else {
-      if (l == 0) st_rel(&status[tile], pack(epoch, 1, total));
+      if (l == 0) st_rlx(&status[tile], pack(epoch, 1, total));
       prefix = look_back(status, epoch, tile, first);
-      if (l == 0) st_rel(&status[tile], pack(epoch, 2, prefix + total));
+      if (l == 0) st_rlx(&status[tile], pack(epoch, 2, prefix + total));
     }
     if (l == 0) {
       s_prefix = prefix;
@@ -151,7 +167,7 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
   const uint32_t epoch = next_epoch(c);
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
-  k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kBlock, 0, c.stream>>>(
+  k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
       static_cast<const T *>(x), mask, n, cmp, cv, base, out, status, epoch, c.scratch.counters,
       static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr);
   check_hip(hipGetLastError(), "select launch");
@@ -165,7 +181,7 @@ void launch_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, ui
     const uint32_t epoch = next_epoch(c);
     T cv;
     __builtin_memcpy(&cv, &s.cbits, sizeof(T));
-    k_select<T, KIND, true><<<p.ntiles, kBlock, 0, c.stream>>>(
+    k_select<T, KIND, true><<<p.ntiles, kSelBlock, 0, c.stream>>>(
         nullptr, nullptr, 0, s.cmp, cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
         nullptr, p.items, p.tile_item, d_counts);
     check_hip(hipGetLastError(), "select batch launch");

@@ -153,9 +153,9 @@ __global__ __launch_bounds__(kBlock) void k_pass(int32_t type, const void *keys_
     uint64_t *st = status + uint64_t(tile) * kRadix + dd;
     uint64_t prefix = 0;
     if (tile == 0) {
-      st_rel(st, pack(epoch, 2, cnt));
+      st_rlx(st, pack(epoch, 2, cnt));
     } else {
-      st_rel(st, pack(epoch, 1, cnt));
+      st_rlx(st, pack(epoch, 1, cnt));
       int64_t j = int64_t(tile) - 1;
       while (j >= 0) {
         const uint64_t wd = ld_rlx(status + uint64_t(j) * kRadix + dd);
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(int32_t type, const void *keys_
         if (flag == 2) break;
         --j;
       }
-      st_rel(st, pack(epoch, 2, prefix + cnt));
+      st_rlx(st, pack(epoch, 2, prefix + cnt));
     }
     s_gbase[dd] = uint64_t(digit_base[dd]) + prefix;
   }

@@ -33,6 +33,18 @@ __device__ __forceinline__ void st_rlx(uint64_t *p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Look-back publication without cache maintenance. Release/acquire at agent scope write back
+// or invalidate the whole per-XCD L2 on every publish and poll; look-back status is instead
+// only ever touched by agent-scope atomic loads and stores (above), which the hardware keeps
+// coherent across XCDs on their own. A publisher stores its payload words, then
+// publish_fence() waits until every earlier store of the wave is acknowledged, then it stores
+// the flag word: a wave that has seen the flag issues its payload loads after that and reads
+// the payload. observe_fence() keeps the compiler from hoisting payload loads above the flag
+// test. A publisher whose payload was stored by other waves of its block runs publish_fence()
+// in each of those waves before a __syncthreads() and then stores the flag.
+__device__ __forceinline__ void publish_fence() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void observe_fence() { asm volatile("" ::: "memory"); }
+
 // A dynamic tile id: tiles are numbered in the order their blocks start, so a tile only
 // ever waits on tiles that are already running. The block that draws the last ticket
 // resets the counter for the next launch of the context.
@@ -192,6 +204,17 @@ __device__ __forceinline__ T ld_rlx_pod(const T *p) {
   T r;
   __builtin_memcpy(&r, w, sizeof(T));
   return r;
+}
+
+// Stores a trivially copyable value of 8-byte words through agent-coherent atomic stores.
+template <typename T>
+__device__ __forceinline__ void st_rlx_pod(T *p, const T &v) {
+  static_assert(sizeof(T) % 8 == 0, "pod store needs whole 8-byte words");
+  uint64_t w[sizeof(T) / 8];
+  __builtin_memcpy(w, &v, sizeof(T));
+  uint64_t *dst = reinterpret_cast<uint64_t *>(p);
+#pragma unroll
+  for (int k = 0; k < static_cast<int>(sizeof(T) / 8); ++k) st_rlx(dst + k, w[k]);
 }
 
 }  // namespace vk
