@@ -43,6 +43,8 @@ def alg_bytes(name: str, n: int, kept: int = 0) -> int:
         return 9 * n + 8 * kept
     if name.startswith("gather"):
         return 24 * n
+    if name == "copy_f64":
+        return 16 * n
     raise KeyError(name)
 
 
@@ -72,6 +74,21 @@ def i64_uniform_40(n: int, seed: int) -> np.ndarray:
 def gather_indices(n: int, seed: int, table_rows: int) -> np.ndarray:
     """splitmix64(seed + i) mod table_rows (a power of two)."""
     return _chunked(n, lambda s0, c: (splitmix64_np(seed, s0, c) & np.uint64(table_rows - 1)).astype(np.int64))
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_streaming_2p28.md")
+
+
+def measured_traffic(label: str, log2n: int):
+    """HBM bytes per launch of `label` at 2^28 rows from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None when there is none for this label or size."""
+    if log2n != 28 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == label:
+            return int(parts[2])
+    return None
 
 
 def parse_trace(text: str):
@@ -156,6 +173,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="",
+                    help="comma-separated primitive labels to run (for profiling); 'copy_f64' adds a "
+                         "plain copy (add_f64 x + 0.0) used to calibrate HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this backend has no multi-GPU path (asked for {args.gpus})",
@@ -192,6 +212,13 @@ def main():
         calls.append((f"compress_f64@{c}", "compress_f64", [x1, masks[c]]))
     calls.append(("gather_f64", "gather_f64", [tbl, idx]))
     calls.append(("gather_i64", "gather_i64", [tbli, idx]))
+    if args.only:
+        want = set(args.only.split(","))
+        extra = [("copy_f64", "add_f64", [x1, S("f64", 0.0)])] if "copy_f64" in want else []
+        calls = [c for c in calls if c[0] in want] + extra
+        unknown = want - {c[0] for c in calls}
+        if unknown:
+            sys.exit(f"unknown primitive labels: {sorted(unknown)}")
 
     kept = {}
 
@@ -231,7 +258,7 @@ def main():
                         "frac": round(b / us / 1e3 / HBM_PEAK_GBPS, 4), "bytes": b}
     dom = max(prims, key=lambda k: prims[k]["us"])
     roofline = {"bound": "hbm", "kernel": dom, "achieved": prims[dom]["GBps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": None,
+                "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": measured_traffic(dom, args.log2n),
                 "bytes_per_launch": prims[dom]["bytes"], "launch_us": prims[dom]["us"]}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     line = {
