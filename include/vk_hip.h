@@ -204,10 +204,11 @@ void vk_value_release(vk_value *v);
  * phase for all items instead of one call per item, and writes item i's result to out[i].
  * Every result equals what vk_call returns for that item alone. Natively batched:
  * filter_<cmp>_<t> (one constant for all items), compress_<t>, where, reduce_add_f64,
- * dot_f64, sumsq_f64 and condsum_<cmp>_f64 (one constant pair); other kernels and mixed
- * constants run item by item. On an error no result is returned and `st` holds the first
- * failing item's error, the kernel's own text. Selection results of one call share one
- * device buffer. */
+ * dot_f64, sumsq_f64 and condsum_<cmp>_f64 (one constant pair), the other reductions
+ * (count_<cmp>_<t> with one constant), argsort_<t>, group_id_<t> and reduce_into_add_f64;
+ * other kernels and mixed constants run item by item. On an error no result is returned and `st` holds the first
+ * failing item's error, the kernel's own text. Array results of one call share one device
+ * buffer. */
 int vk_batch_call(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, size_t nitems,
                   vk_value *out, vk_status *st);
 

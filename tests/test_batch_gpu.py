@@ -118,3 +118,140 @@ def test_batch_errors_and_fallbacks(D):
     # an ordinary kernel runs item by item
     got = D.batch_call("count_gt_f64", [[D.array(x), S("f64", 3.0)] for x in xs])
     assert [g.value for g in got] == [1, 8996]
+
+
+@pytest.mark.parametrize("t", ["u64", "i64", "i32", "u8", "f64"])
+def test_argsort_batch(D, t):
+    rng = np.random.default_rng(15)
+    sizes = uneven(rng, 30, hi=100000)
+    xs = []
+    for n in sizes:
+        if t == "f64":
+            x = rng.normal(size=n)
+            if n > 10:
+                x[rng.integers(0, n, 3)] = 0.0
+                x[rng.integers(0, n, 3)] = -0.0
+        elif t == "u8":
+            x = rng.integers(0, 256, n).astype(np.uint8)
+        elif t == "i32":
+            x = rng.integers(-1000, 1000, n).astype(np.int32)
+        elif t == "u64":
+            x = rng.integers(0, 2**63, n, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+        else:
+            x = rng.integers(-2**40, 2**40, n).astype(np.int64)
+        xs.append(x)
+    got = D.batch_call(f"argsort_{t}", [[D.array(x, t)] for x in xs])
+    for x, g in zip(xs, got):
+        assert np.array_equal(g.to_numpy(), O.argsort(x)), (t, x.size)
+
+
+def test_argsort_batch_constant_digits(D):
+    # every item has the same high bytes: those passes are skipped for the whole batch; one
+    # item varies only in a byte that is constant in the others
+    xs = [np.arange(n, dtype=np.int64) % 200 for n in (5, 300, 20000)]
+    xs.append((np.arange(5000, dtype=np.int64) % 3) << 16)
+    xs.append(np.full(17, 7, np.int64))
+    got = D.batch_call("argsort_i64", [[D.array(x)] for x in xs])
+    for x, g in zip(xs, got):
+        assert np.array_equal(g.to_numpy(), np.argsort(x, kind="stable"))
+
+
+REDUCTIONS = ["reduce_add_i64", "reduce_add_u8", "reduce_min_f64", "reduce_max_f64", "argmin_f64",
+              "argmax_f64", "reduce_max_i64", "argmin_i32", "reduce_and_u8", "reduce_xor_i64",
+              "reduce_or_bool", "count", "any", "all", "count_gt_f64", "count_le_i64"]
+
+
+def reduction_input(rng, name, n):
+    if name.endswith("_f64"):
+        x = rng.normal(size=n)
+        if n > 100:
+            x[rng.integers(0, n)] = np.nan if name.startswith("arg") else x[0]
+        return x, "f64"
+    if name in ("count", "any", "all") or name.endswith("_bool"):
+        return (rng.random(n) < (0.999 if name == "all" else 0.001)).astype(np.uint8), "bool"
+    if name.endswith("_u8"):
+        return rng.integers(0, 256, n).astype(np.uint8), "u8"
+    if name.endswith("_i32"):
+        return rng.integers(-50, 50, n).astype(np.int32), "i32"
+    return rng.integers(-2**40, 2**40, n).astype(np.int64), "i64"
+
+
+@pytest.mark.parametrize("name", REDUCTIONS)
+def test_reduction_batch(D, name):
+    rng = np.random.default_rng(16)
+    sizes = [n for n in uneven(rng, 20, lo=1, hi=50000) if n > 0]
+    args = []
+    for n in sizes:
+        x, t = reduction_input(rng, name, n)
+        a = [D.array(x, t)]
+        if name.startswith("count_"):
+            a.append(S(t, 0.25 if t == "f64" else 7))
+        args.append(a)
+    got = D.batch_call(name, args)
+    for a, g in zip(args, got):
+        want = D.call(name, a)
+        if isinstance(want.value, float) and np.isnan(want.value):
+            assert np.isnan(g.value)
+        else:
+            assert g.value == want.value and g.type == want.type, (name, len(a[0]))
+
+
+def test_reduction_batch_empty_error(D):
+    from verkehrt_amd import KernelError
+    xs = [D.array(np.arange(5, dtype=np.float64)), D.array(np.zeros(0)), D.array(np.ones(3))]
+    with pytest.raises(KernelError) as e1:
+        D.batch_call("reduce_min_f64", [[x] for x in xs])
+    with pytest.raises(KernelError) as e2:
+        D.call("reduce_min_f64", [xs[1]])
+    assert str(e1.value) == str(e2.value)
+    assert [g.value for g in D.batch_call("reduce_add_i64", [[D.array(np.zeros(0, np.int64))]])] == [0]
+
+
+@pytest.mark.parametrize("t", ["i64", "u64", "i32", "u8"])
+def test_group_id_batch(D, t):
+    rng = np.random.default_rng(17)
+    sizes = uneven(rng, 25, hi=60000)
+    xs = []
+    for i, n in enumerate(sizes):
+        g = [3, 256, 5000][i % 3]
+        pool = {"i64": rng.integers(-2**62, 2**62, g), "u64": rng.integers(0, 2**63, g, dtype=np.uint64),
+                "i32": rng.integers(-2**30, 2**30, g).astype(np.int32),
+                "u8": rng.integers(0, 256, g).astype(np.uint8)}[t]
+        xs.append(pool[rng.integers(0, len(pool), n)])
+    got = D.batch_call(f"group_id_{t}", [[D.array(x, t)] for x in xs])
+    for x, g in zip(xs, got):
+        assert np.array_equal(g.to_numpy(), O.group_id(x)), (t, x.size)
+
+
+def test_reduce_into_add_f64_batch(D):
+    rng = np.random.default_rng(18)
+    sizes = uneven(rng, 25, hi=80000)
+    items, want = [], []
+    for i, n in enumerate(sizes):
+        nb = [1, 7, 256, 3000][i % 4]
+        k = rng.integers(0, nb, n).astype(np.int64)
+        v = rng.normal(2.0, 3.0, n)
+        items.append([D.array(k), D.array(v), S("i64", nb)])
+        want.append(O.reduce_into(k, v, nb, "add"))
+    got = D.batch_call("reduce_into_add_f64", items)
+    for it, w, g in zip(items, want, got):
+        assert np.array_equal(g.to_numpy().view(np.uint64), np.asarray(w, np.float64).view(np.uint64))
+        assert np.array_equal(D.call("reduce_into_add_f64", it).to_numpy().view(np.uint64),
+                              np.asarray(w, np.float64).view(np.uint64))
+
+
+def test_reduce_into_batch_errors(D):
+    from verkehrt_amd import KernelError
+    ok = [D.array(np.array([0, 1, 1], np.int64)), D.array(np.ones(3)), S("i64", 2)]
+    badk = [D.array(np.array([0, 5, 1, 9], np.int64)), D.array(np.ones(4)), S("i64", 3)]
+    with pytest.raises(KernelError) as e1:
+        D.batch_call("reduce_into_add_f64", [ok, badk])
+    with pytest.raises(KernelError) as e2:
+        D.call("reduce_into_add_f64", badk)
+    assert str(e1.value) == str(e2.value) and "bucket key 5" in str(e1.value)
+    neg = [D.array(np.array([0], np.int64)), D.array(np.ones(1)), S("i64", -1)]
+    with pytest.raises(KernelError) as e3:
+        D.batch_call("reduce_into_add_f64", [ok, neg])
+    with pytest.raises(KernelError) as e4:
+        D.call("reduce_into_add_f64", neg)
+    assert str(e3.value) == str(e4.value)

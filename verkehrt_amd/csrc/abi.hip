@@ -383,18 +383,8 @@ void reducer_update(vk_reducer *r, const vk_arg *args, size_t nargs) {
   r->consumed += x->len;
 }
 
-vk_scalar reducer_finish(vk_reducer *r) {
-  Ctx &c = *r->c;
-  const Def &d = g_defs[r->def];
-  if (d.impl == R_CANON) {
-    double *acc = static_cast<double *>(r->state);
-    canon_finish(c, d.p0 == 4, acc, acc + 16, r->tail_len, reinterpret_cast<double *>(c.scratch.slots));
-    uint64_t b;
-    read_back(c, c.scratch.slots, 8, &b);
-    return scalar(VK_F64, b);
-  }
-  uint64_t w[3];
-  read_back(c, r->state, 24, w);
+// The scalar of a finished non-canonical reduction state (Reducer::finish).
+vk_scalar finish_words(const Def &d, const uint64_t *w) {
   switch (d.impl) {
     case R_EXTREME:
       if (!w[1]) runtime_error(std::string("reduce_") + (d.p0 ? "max" : "min") + " of an empty array");
@@ -413,6 +403,21 @@ vk_scalar reducer_finish(vk_reducer *r) {
     }
     default: return scalar(d.out_t, w[0]);
   }
+}
+
+vk_scalar reducer_finish(vk_reducer *r) {
+  Ctx &c = *r->c;
+  const Def &d = g_defs[r->def];
+  if (d.impl == R_CANON) {
+    double *acc = static_cast<double *>(r->state);
+    canon_finish(c, d.p0 == 4, acc, acc + 16, r->tail_len, reinterpret_cast<double *>(c.scratch.slots));
+    uint64_t b;
+    read_back(c, c.scratch.slots, 8, &b);
+    return scalar(VK_F64, b);
+  }
+  uint64_t w[3];
+  read_back(c, r->state, 24, w);
+  return finish_words(d, w);
 }
 
 void reducer_free(vk_reducer *r) {
@@ -885,6 +890,151 @@ std::vector<vk_value> batch_call(Ctx &c, const Def &d, const vk_arg *args, size_
       out[i].kind = VK_VALUE_SCALAR;
       out[i].scalar = scalar(VK_F64, bits[i]);
     }
+    return out;
+  }
+  if (d.cls == VK_REDUCTION && d.impl != R_CANON) {
+    // the other reductions: one fold of every item's tiles, one merge per item
+    ReduceSpec rs = reduce_spec(d, 0);
+    if (d.impl == R_COUNT_CMP) {
+      rs.cbits = splat_arg(args[1], d.t0);
+      for (size_t i = 1; i < nitems; ++i)
+        if (args[i * nargs + 1].splat.bits != rs.cbits) {  // one constant per launch
+          each();
+          return out;
+        }
+    }
+    std::vector<BatchItem> items(nitems);
+    for (size_t i = 0; i < nitems; ++i) {
+      const vk_array *x = args[i * nargs].rows;
+      items[i].a = dptr(x);
+      items[i].n = x->len;
+    }
+    const uint32_t words = reduce_state_words(rs);
+    auto *states = static_cast<uint64_t *>(tmp_alloc(c, nitems * words * 8));
+    std::vector<uint64_t> host(nitems * words);
+    try {
+      reduce_batch(c, rs, items, states);
+      check_hip(hipMemcpyAsync(host.data(), states, nitems * words * 8, hipMemcpyDeviceToHost, c.stream),
+                "states");
+      check_hip(hipStreamSynchronize(c.stream), "sync");
+    } catch (...) {
+      tmp_free(c, states);
+      throw;
+    }
+    tmp_free(c, states);
+    for (size_t i = 0; i < nitems; ++i) {
+      uint64_t w[3] = {0, 0, 0};
+      for (uint32_t k = 0; k < words; ++k) w[k] = host[i * words + k];
+      out[i].kind = VK_VALUE_SCALAR;
+      out[i].scalar = finish_words(d, w);
+    }
+    return out;
+  }
+  if (d.impl == W_GROUP_ID) {
+    std::vector<const vk_array *> xs(nitems);
+    std::vector<uint64_t> off(nitems);
+    uint64_t total = 0;
+    for (size_t i = 0; i < nitems; ++i) {
+      xs[i] = array_arg(args[i * nargs], d.t0);
+      off[i] = total;
+      total += xs[i]->len;
+    }
+    vk_array *buf = new_array(c, VK_I64, total ? total : 1);
+    std::vector<BatchItem> items(nitems);
+    for (size_t i = 0; i < nitems; ++i) {
+      items[i].a = dptr(xs[i]);
+      items[i].n = xs[i]->len;
+      items[i].out = static_cast<int64_t *>(dptr(buf)) + off[i];
+    }
+    auto *groups = static_cast<uint64_t *>(tmp_alloc(c, nitems * 8));
+    try {
+      group_id_batch(c, d.t0, items, groups);
+    } catch (...) {
+      tmp_free(c, groups);
+      release(buf);
+      throw;
+    }
+    tmp_free(c, groups);
+    for (size_t i = 0; i < nitems; ++i) {
+      buf->buf->refs++;
+      out[i].array = new vk_array{1, buf->buf, buf->offset + off[i] * 8, xs[i]->len, VK_I64};
+    }
+    release(buf);
+    return out;
+  }
+  if (d.impl == W_REDUCE_INTO && d.p0 == 4) {
+    // host-side errors (bucket count, lengths) keep the single call's order item by item
+    std::vector<BatchItem> items(nitems);
+    std::vector<const vk_array *> ks(nitems);
+    std::vector<uint64_t> off(nitems);
+    uint64_t total = 0;
+    for (size_t i = 0; i < nitems; ++i) {
+      const vk_arg *a = args + i * nargs;
+      const vk_array *vals = array_arg(a[1], VK_F64);
+      ks[i] = array_arg(a[0], VK_I64);
+      const int64_t nb = i64_arg(a[2]);
+      if (nb < 0 || ks[i]->len != vals->len) {
+        each();
+        return out;
+      }
+      items[i].a = dptr(ks[i]);
+      items[i].b = dptr(vals);
+      items[i].n = ks[i]->len;
+      items[i].aux2 = static_cast<uint64_t>(nb);
+      off[i] = total;
+      total += static_cast<uint64_t>(nb);
+    }
+    auto *bad = static_cast<uint64_t *>(tmp_alloc(c, nitems * 8));
+    std::vector<uint64_t> hbad(nitems);
+    check_keys_batch(c, items, bad);
+    check_hip(hipMemcpyAsync(hbad.data(), bad, nitems * 8, hipMemcpyDeviceToHost, c.stream), "bad keys");
+    check_hip(hipStreamSynchronize(c.stream), "sync");
+    tmp_free(c, bad);
+    for (size_t i = 0; i < nitems; ++i)
+      if (hbad[i] != ~uint64_t(0)) bucket_error(c, ks[i], hbad[i], static_cast<int64_t>(items[i].aux2));
+    vk_array *buf = new_array(c, VK_F64, total ? total : 1);
+    for (size_t i = 0; i < nitems; ++i) items[i].out = static_cast<double *>(dptr(buf)) + off[i];
+    try {
+      reduce_into_add_f64_batch(c, items);
+    } catch (...) {
+      release(buf);
+      throw;
+    }
+    for (size_t i = 0; i < nitems; ++i) {
+      buf->buf->refs++;
+      out[i].array = new vk_array{1, buf->buf, buf->offset + off[i] * 8, items[i].aux2, VK_F64};
+    }
+    release(buf);
+    return out;
+  }
+  if (d.impl == W_ARGSORT) {
+    std::vector<const vk_array *> xs(nitems);
+    std::vector<uint64_t> off(nitems);
+    uint64_t total = 0;
+    for (size_t i = 0; i < nitems; ++i) {
+      xs[i] = array_arg(args[i * nargs], d.t0);
+      if (xs[i]->len > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
+      off[i] = total;
+      total += xs[i]->len;
+    }
+    vk_array *buf = new_array(c, VK_I64, total ? total : 1);
+    std::vector<BatchItem> items(nitems);
+    for (size_t i = 0; i < nitems; ++i) {
+      items[i].a = dptr(xs[i]);
+      items[i].n = xs[i]->len;
+      items[i].out = static_cast<int64_t *>(dptr(buf)) + off[i];
+    }
+    try {
+      argsort_batch(c, d.t0, items);
+    } catch (...) {
+      release(buf);
+      throw;
+    }
+    for (size_t i = 0; i < nitems; ++i) {
+      buf->buf->refs++;
+      out[i].array = new vk_array{1, buf->buf, buf->offset + off[i] * 8, xs[i]->len, VK_I64};
+    }
+    release(buf);
     return out;
   }
   each();

@@ -100,26 +100,22 @@ __global__ void k_check_keys(const int64_t *__restrict__ keys, uint64_t n, uint6
 
 // ----------------------------------------------------------- f64 reduce_into add
 
-// chain key of row i < b: bucket * 16 + i % 16
-__global__ void k_chain_keys(const int64_t *__restrict__ keys, uint64_t b, uint64_t *ck) {
+// chain key of row i < b: bucket * 16 + i % 16 (u32 when 16 x buckets fits, else u64)
+template <typename K>
+__global__ void k_chain_keys(const int64_t *__restrict__ keys, uint64_t b, K *ck) {
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < b;
        i += uint64_t(gridDim.x) * blockDim.x)
-    ck[i] = static_cast<uint64_t>(keys[i]) * 16 + (i & 15);
+    ck[i] = static_cast<K>(static_cast<uint64_t>(keys[i]) * 16 + (i & 15));
 }
 
-__global__ void k_gather_vals(const double *__restrict__ vals, const int64_t *__restrict__ perm,
-                              uint64_t b, double *sv) {
-  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
-       p += uint64_t(gridDim.x) * blockDim.x)
-    sv[p] = vals[perm[p]];
-}
-
-// One thread per chain start: folds the chain's values (sorted positions, so index order).
-__global__ void k_chain_fold_thread(const uint64_t *__restrict__ sk, const double *__restrict__ sv,
+// One thread per chain start: folds the chain's values (a stable sort by chain key keeps
+// them in index order).
+template <typename K>
+__global__ void k_chain_fold_thread(const K *__restrict__ sk, const double *__restrict__ sv,
                                     uint64_t b, double *stripes) {
   for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
        p += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t k = sk[p];
+    const K k = sk[p];
     if (p > 0 && sk[p - 1] == k) continue;
     double a = 0.0;
     uint64_t q = p;
@@ -132,9 +128,10 @@ __global__ void k_chain_fold_thread(const uint64_t *__restrict__ sk, const doubl
 }
 
 // One wave per chain start for long chains: lanes load 64 values ahead, lane 0 folds them.
+template <typename K>
 __global__ __launch_bounds__(64) void k_chain_fold_wave(const uint64_t *__restrict__ starts,
                                                          uint64_t nchains,
-                                                         const uint64_t *__restrict__ sk,
+                                                         const K *__restrict__ sk,
                                                          const double *__restrict__ sv, uint64_t b,
                                                          double *stripes) {
   const int l = lane_id();
@@ -156,7 +153,8 @@ __global__ __launch_bounds__(64) void k_chain_fold_wave(const uint64_t *__restri
   }
 }
 
-__global__ void k_chain_starts(const uint64_t *__restrict__ sk, uint64_t b, uint8_t *flag) {
+template <typename K>
+__global__ void k_chain_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag) {
   for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
        p += uint64_t(gridDim.x) * blockDim.x)
     flag[p] = (p == 0 || sk[p - 1] != sk[p]) ? 1 : 0;
@@ -240,6 +238,201 @@ void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint6
 }
 
 // Keys are already checked to lie in 0..buckets.
+namespace {
+
+// The chains of rows 0..b (bucket, i % 16): a stable sort of (chain key, value) pairs puts each
+// chain's values together in index order; then each chain is folded sequentially.
+template <typename K>
+void fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
+                 double *stripes) {
+  constexpr bool wide = sizeof(K) == 8;
+  K *ck = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
+  K *sk = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
+  double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
+  const uint32_t g = stream_grid(b, kBlock * 4, 8192);
+  k_chain_keys<K><<<g, kBlock, 0, c.stream>>>(keys, b, ck);
+  check_hip(hipGetLastError(), "chain keys");
+  sort_key_values(c, wide, ck, reinterpret_cast<const uint64_t *>(vals), b, sk,
+                  reinterpret_cast<uint64_t *>(sv));
+  const uint64_t chains_est = buckets * 16 < b ? buckets * 16 : b;
+  if (b / chains_est >= 256) {
+    // long chains: compact the chain starts, one wave per chain
+    uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
+    uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
+    k_chain_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+    SelectSpec ss{2, VK_I64, 0, 0, 0};
+    uint64_t *d_cnt = c.scratch.slots + 20;
+    select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
+    uint64_t nchains = 0;
+    read_back(c, d_cnt, 8, &nchains);
+    k_chain_fold_wave<K><<<stream_grid(nchains, 1, 16384), 64, 0, c.stream>>>(starts, nchains, sk, sv, b,
+                                                                                stripes);
+    tmp_free(c, flag);
+    tmp_free(c, starts);
+  } else {
+    k_chain_fold_thread<K><<<g, kBlock, 0, c.stream>>>(sk, sv, b, stripes);
+  }
+  check_hip(hipGetLastError(), "chain fold");
+  tmp_free(c, ck);
+  tmp_free(c, sk);
+  tmp_free(c, sv);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batched form
+// Items: a = keys (i64), b = values (f64), n rows, out = buckets f64, aux2 = buckets. The
+// chain keys of all items are sorted with their values in one batched sort (each item in its
+// own rows, from aux), then every chain is folded, and each item combines its stripes and
+// folds its tail rows.
+namespace {
+
+constexpr uint64_t kBTile = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_check_keys_batch(const BatchItem *items,
+                                                             const uint32_t *tile_item, uint64_t *bad) {
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile, t1 = t0 + kBTile < I.n ? t0 + kBTile : I.n;
+  const int64_t *keys = static_cast<const int64_t *>(I.a);
+  uint64_t my_bad = ~uint64_t(0);
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock)
+    if (static_cast<uint64_t>(keys[r]) >= I.aux2 && r < my_bad) my_bad = r;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const uint64_t o = __shfl_xor(my_bad, m);
+    my_bad = o < my_bad ? o : my_bad;
+  }
+  if (lane_id() == 0 && my_bad != ~uint64_t(0)) atomicMin(reinterpret_cast<unsigned long long *>(&bad[it]), my_bad);
+}
+
+// chain keys of the whole blocks of every item, at the item's concatenated rows (aux)
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_chain_keys_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                             K *ck) {
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile, t1 = t0 + kBTile < I.n ? t0 + kBTile : I.n;
+  const int64_t *keys = static_cast<const int64_t *>(I.a);
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock)
+    ck[I.aux + r] = static_cast<K>(static_cast<uint64_t>(keys[r]) * 16 + (r & 15));
+}
+
+// One thread per chain start of every item: fold the chain, eight rows loaded ahead.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_chain_fold_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                             const K *sk, const double *sv,
+                                                             double *stripes) {
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile, t1 = t0 + kBTile < I.n ? t0 + kBTile : I.n;
+  const K *k_ = sk + I.aux;
+  const double *v_ = sv + I.aux;
+  double *st = stripes + I.aux3;
+  for (uint64_t p = t0 + threadIdx.x; p < t1; p += kBlock) {
+    const K k = k_[p];
+    if (p > 0 && k_[p - 1] == k) continue;
+    double a = 0.0;
+    for (uint64_t q = p;; q += 8) {
+      K kk[8];
+      double vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool in = q + u < I.n;
+        kk[u] = in ? k_[q + u] : static_cast<K>(~k);
+        vv[u] = in ? v_[q + u] : 0.0;
+      }
+      bool stop = false;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (!stop && kk[u] == k) a = a + vv[u];
+        else stop = true;
+      }
+      if (stop) break;
+    }
+    st[k] = a;
+  }
+}
+
+// One block per item: the pairwise combine of every bucket's 16 stripes (finish_lanes,
+// reduce.rs:75-91), then thread 0 folds the item's tail rows in order.
+__global__ __launch_bounds__(kBlock) void k_combine_batch(const BatchItem *items, const double *stripes) {
+  const BatchItem &I = items[blockIdx.x];
+  const double *st = stripes + I.aux3;
+  double *out = static_cast<double *>(I.out);
+  for (uint64_t k = threadIdx.x; k < I.aux2; k += kBlock) {
+    double a[16];
+    for (int j = 0; j < 16; ++j) a[j] = st[k * 16 + j];
+    for (int step = 8; step > 0; step /= 2)
+      for (int j = 0; j < step; ++j) a[j] = a[j] + a[j + step];
+    out[k] = a[0];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t *keys = static_cast<const int64_t *>(I.a);
+    const double *vals = static_cast<const double *>(I.b);
+    for (uint64_t r = I.n - I.n % 16; r < I.n; ++r) out[keys[r]] = out[keys[r]] + vals[r];
+  }
+}
+
+}  // namespace
+
+void check_keys_batch(Ctx &c, std::vector<BatchItem> &items, uint64_t *d_bad) {
+  const uint64_t nitems = items.size();
+  fill_u64(c, d_bad, nitems, ~uint64_t(0));
+  BatchPlan p = batch_plan(c, items, kBTile);
+  if (p.ntiles) k_check_keys_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, d_bad);
+  check_hip(hipGetLastError(), "check keys batch");
+  batch_plan_free(c, p);
+}
+
+void reduce_into_add_f64_batch(Ctx &c, std::vector<BatchItem> &items) {
+  const uint64_t nitems = items.size();
+  if (nitems == 0) return;
+  // item rows and stripe offsets
+  uint64_t stripes_n = 0, maxb = 0;
+  std::vector<BatchItem> whole(items);  // whole blocks only, for the chains
+  for (size_t i = 0; i < nitems; ++i) {
+    items[i].aux3 = stripes_n;
+    stripes_n += items[i].aux2 * 16;
+    maxb = items[i].aux2 > maxb ? items[i].aux2 : maxb;
+    whole[i] = items[i];
+    whole[i].n = items[i].n - items[i].n % 16;
+  }
+  const bool wide = maxb * 16 > 0xFFFFFFFFull;
+  double *stripes = static_cast<double *>(tmp_alloc(c, (stripes_n ? stripes_n : 1) * 8));
+  k_fill<double><<<stream_grid(stripes_n, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, stripes_n, 0.0);
+  uint64_t total = 0;
+  for (auto &it : whole) {
+    it.aux = total;
+    total += it.n;
+  }
+  if (total > 0) {
+    const size_t kw = wide ? 8 : 4;
+    void *ck = tmp_alloc(c, total * kw);
+    void *sk = tmp_alloc(c, total * kw);
+    double *sv = static_cast<double *>(tmp_alloc(c, total * 8));
+    BatchPlan p = batch_plan(c, whole, kBTile);
+    if (wide) k_chain_keys_batch<uint64_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<uint64_t *>(ck));
+    else k_chain_keys_batch<uint32_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<uint32_t *>(ck));
+    check_hip(hipGetLastError(), "chain keys batch");
+    // the sort reads each item's chain keys at a and its values at b
+    std::vector<BatchItem> sorting(whole);
+    for (auto &it : sorting) it.a = static_cast<char *>(ck) + it.aux * kw;
+    sort_key_values_batch(c, wide, sorting, sk, reinterpret_cast<uint64_t *>(sv));
+    if (wide) k_chain_fold_batch<uint64_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<const uint64_t *>(sk), sv, stripes);
+    else k_chain_fold_batch<uint32_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<const uint32_t *>(sk), sv, stripes);
+    check_hip(hipGetLastError(), "chain fold batch");
+    batch_plan_free(c, p);
+    tmp_free(c, ck);
+    tmp_free(c, sk);
+    tmp_free(c, sv);
+  }
+  BatchPlan q = batch_plan(c, items, ~uint64_t(0) >> 1);  // just the item table
+  k_combine_batch<<<uint32_t(nitems), kBlock, 0, c.stream>>>(q.items, stripes);
+  check_hip(hipGetLastError(), "combine batch");
+  batch_plan_free(c, q);
+  tmp_free(c, stripes);
+}
+
 void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
                          uint64_t buckets, double *out) {
   if (buckets == 0) return;
@@ -247,36 +440,8 @@ void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64
   double *stripes = static_cast<double *>(tmp_alloc(c, buckets * 16 * 8));
   k_fill<double><<<stream_grid(buckets * 16, kBlock, 4096), kBlock, 0, c.stream>>>(stripes, buckets * 16, 0.0);
   if (b > 0) {
-    uint64_t *ck = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-    uint64_t *sk = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-    int64_t *perm = static_cast<int64_t *>(tmp_alloc(c, b * 8));
-    double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
-    const uint32_t g = stream_grid(b, kBlock * 4, 8192);
-    k_chain_keys<<<g, kBlock, 0, c.stream>>>(keys, b, ck);
-    sort_u64_pairs(c, ck, b, sk, perm);
-    k_gather_vals<<<g, kBlock, 0, c.stream>>>(vals, perm, b, sv);
-    const uint64_t chains_est = buckets * 16 < b ? buckets * 16 : b;
-    if (b / chains_est >= 256) {
-      // long chains: compact the chain starts, one wave per chain
-      uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
-      uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-      k_chain_starts<<<g, kBlock, 0, c.stream>>>(sk, b, flag);
-      SelectSpec ss{2, VK_I64, 0, 0, 0};
-      uint64_t *d_cnt = c.scratch.slots + 20;
-      select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
-      uint64_t nchains = 0;
-      read_back(c, d_cnt, 8, &nchains);
-      k_chain_fold_wave<<<stream_grid(nchains, 1, 16384), 64, 0, c.stream>>>(starts, nchains, sk, sv,
-                                                                             b, stripes);
-      tmp_free(c, flag);
-      tmp_free(c, starts);
-    } else {
-      k_chain_fold_thread<<<g, kBlock, 0, c.stream>>>(sk, sv, b, stripes);
-    }
-    tmp_free(c, ck);
-    tmp_free(c, sk);
-    tmp_free(c, perm);
-    tmp_free(c, sv);
+    if (buckets * 16 <= 0xFFFFFFFFull) fold_chains<uint32_t>(c, keys, vals, b, buckets, stripes);
+    else fold_chains<uint64_t>(c, keys, vals, b, buckets, stripes);
   }
   k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, buckets, out);
   if (n > b) k_fold_tail<<<1, 64, 0, c.stream>>>(keys, vals, b, n, out);

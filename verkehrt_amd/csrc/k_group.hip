@@ -109,7 +109,150 @@ __global__ void k_ids(const uint32_t *__restrict__ slot, const uint32_t *__restr
     out[r] = id_of_slot[slot[r]];
 }
 
+// ------------------------------------------------------------------ batched form
+// The same four passes for many arrays at once: each item has its own table (words aux2 ..
+// aux2 + 2^aux3 of one allocation) and its own rows in the concatenated slot and first-flag
+// buffers (from row aux); one launch per pass covers the tiles of every item.
+constexpr uint64_t kGTile = 4096;
+
+struct ItemRows {
+  uint64_t t0, t1;
+  __device__ ItemRows(const BatchItem &I, uint32_t tile) {
+    t0 = (tile - I.tile0) * kGTile;
+    t1 = t0 + kGTile < I.n ? t0 + kGTile : I.n;
+  }
+};
+
+__global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const BatchItem *items,
+                                                         const uint32_t *tile_item,
+                                                         unsigned long long *tables) {
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const ItemRows R(I, blockIdx.x);
+  const KeyCol a{type, I.a};
+  unsigned long long *table = tables + I.aux2;
+  const int lg = static_cast<int>(I.aux3);
+  const uint64_t mask = (uint64_t(1) << lg) - 1;
+  for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) {
+    const uint64_t ka = a.bits(r);
+    uint64_t h = slot_of<false>(a, a, ka, 0, 64 - lg);
+    while (true) {
+      unsigned long long s = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (s == 0) {
+        const unsigned long long old = atomicCAS(&table[h], 0ull, r + 1);
+        if (old == 0) break;
+        s = old;
+      }
+      if (a.bits(s - 1) == ka) {
+        if (r + 1 < s) atomicMin(&table[h], r + 1);
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const BatchItem *items,
+                                                        const uint32_t *tile_item,
+                                                        const unsigned long long *tables,
+                                                        uint32_t *slot, uint8_t *first) {
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const ItemRows R(I, blockIdx.x);
+  const KeyCol a{type, I.a};
+  const unsigned long long *table = tables + I.aux2;
+  const int lg = static_cast<int>(I.aux3);
+  const uint64_t mask = (uint64_t(1) << lg) - 1;
+  for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) {
+    const uint64_t ka = a.bits(r);
+    uint64_t h = slot_of<false>(a, a, ka, 0, 64 - lg);
+    while (true) {
+      const uint64_t w = table[h];
+      if (w == 0) {  // unreachable: every key was inserted on its own probe path
+        slot[I.aux + r] = 0;
+        first[I.aux + r] = 0;
+        break;
+      }
+      if (a.bits(w - 1) == ka) {
+        slot[I.aux + r] = static_cast<uint32_t>(h);
+        first[I.aux + r] = (w - 1) == r;
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+// Row j < groups of an item is the item's j-th first row: its slot gets id j.
+__global__ __launch_bounds__(kBlock) void k_number_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                         const int64_t *firsts, const uint64_t *groups,
+                                                         const uint32_t *slot, unsigned long long *tables) {
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const ItemRows R(I, blockIdx.x);
+  const uint64_t g = groups[it];
+  uint32_t *id_of_slot = reinterpret_cast<uint32_t *>(tables + I.aux2);
+  for (uint64_t j = R.t0 + threadIdx.x; j < R.t1 && j < g; j += kBlock)
+    id_of_slot[slot[I.aux + firsts[I.aux + j]]] = static_cast<uint32_t>(j);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                      const uint32_t *slot,
+                                                      const unsigned long long *tables) {
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const ItemRows R(I, blockIdx.x);
+  const uint32_t *id_of_slot = reinterpret_cast<const uint32_t *>(tables + I.aux2);
+  int64_t *out = static_cast<int64_t *>(I.out);
+  for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) out[r] = id_of_slot[slot[I.aux + r]];
+}
+
 }  // namespace
+
+void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_t *d_groups) {
+  uint64_t total = 0, words = 0;
+  for (auto &it : items) {
+    if (it.n > 0x7FFFFFFFull) usage_error("group_id supports at most 2^31 - 1 rows");
+    uint64_t slots = 2;
+    int lg = 1;
+    while (slots < 2 * it.n) {
+      slots <<= 1;
+      ++lg;
+    }
+    it.aux = total;
+    it.aux2 = words;
+    it.aux3 = static_cast<uint64_t>(lg);
+    total += it.n;
+    words += slots;
+  }
+  const uint64_t nitems = items.size();
+  fill_u64(c, d_groups, nitems, 0);
+  if (total == 0) return;
+  auto *tables = static_cast<unsigned long long *>(tmp_alloc(c, words * 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(tables), words, 0);
+  auto *slot = static_cast<uint32_t *>(tmp_alloc(c, total * 4));
+  auto *first = static_cast<uint8_t *>(tmp_alloc(c, total));
+  auto *firsts = static_cast<int64_t *>(tmp_alloc(c, total * 8));
+  BatchPlan p = batch_plan(c, items, kGTile);
+  k_insert_batch<<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, tables);
+  k_probe_batch<<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, tables, slot, first);
+  check_hip(hipGetLastError(), "group_id batch table");
+  // each item's first rows in row order
+  std::vector<BatchItem> w(nitems);
+  for (uint64_t i = 0; i < nitems; ++i) {
+    w[i] = BatchItem{};
+    w[i].a = first + items[i].aux;
+    w[i].out = firsts + items[i].aux;
+    w[i].n = items[i].n;
+  }
+  SelectSpec ss{2, VK_I64, 0, 0, 0};
+  select_batch(c, ss, w, d_groups);
+  k_number_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, firsts, d_groups, slot, tables);
+  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, slot, tables);
+  check_hip(hipGetLastError(), "group_id batch ids");
+  batch_plan_free(c, p);
+  tmp_free(c, tables);
+  tmp_free(c, slot);
+  tmp_free(c, first);
+  tmp_free(c, firsts);
+}
 
 void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
               int64_t *out) {

@@ -222,19 +222,6 @@ void launch(Ctx &c, const Op &op, const void *x, uint64_t n, void *state) {
   check_hip(hipGetLastError(), "reduce launch");
 }
 
-template <template <typename> class OpT, typename... Extra>
-void by_type(Ctx &c, int32_t type, const void *x, uint64_t n, void *state, Extra... extra) {
-  switch (type) {
-    case VK_F64: launch<double>(c, OpT<double>{extra...}, x, n, state); break;
-    case VK_F32: launch<float>(c, OpT<float>{extra...}, x, n, state); break;
-    case VK_I64: launch<int64_t>(c, OpT<int64_t>{extra...}, x, n, state); break;
-    case VK_U64: launch<uint64_t>(c, OpT<uint64_t>{extra...}, x, n, state); break;
-    case VK_I32: launch<int32_t>(c, OpT<int32_t>{extra...}, x, n, state); break;
-    case VK_U8: case VK_BOOL: launch<uint8_t>(c, OpT<uint8_t>{extra...}, x, n, state); break;
-    default: usage_error("unsupported element type");
-  }
-}
-
 template <typename T>
 OpCount<T> make_count(int cmp, uint64_t cbits) {
   T c;
@@ -271,43 +258,126 @@ void reduce_state_init(Ctx &c, const ReduceSpec &s, void *state) {
   check_hip(hipGetLastError(), "reduce init");
 }
 
-void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *state) {
-  if (n == 0) return;
+namespace {
+
+// Calls f(op, (T *)nullptr) with the operator and element type of the spec.
+template <class F>
+void with_op(const ReduceSpec &s, F &&f) {
   switch (s.kind) {
     case ReduceSpec::SUM_INT:
       switch (s.type) {
-        case VK_I64: launch<int64_t>(c, OpSum<int64_t>{}, x, n, state); break;
-        case VK_U64: launch<uint64_t>(c, OpSum<uint64_t>{}, x, n, state); break;
-        case VK_I32: launch<int32_t>(c, OpSum<int32_t>{}, x, n, state); break;
-        case VK_U8: launch<uint8_t>(c, OpSum<uint8_t>{}, x, n, state); break;
+        case VK_I64: f(OpSum<int64_t>{}, static_cast<int64_t *>(nullptr)); break;
+        case VK_U64: f(OpSum<uint64_t>{}, static_cast<uint64_t *>(nullptr)); break;
+        case VK_I32: f(OpSum<int32_t>{}, static_cast<int32_t *>(nullptr)); break;
+        case VK_U8: f(OpSum<uint8_t>{}, static_cast<uint8_t *>(nullptr)); break;
         default: usage_error("unsupported integer sum type");
       }
       break;
     case ReduceSpec::COUNT_CMP:
       switch (s.type) {
-        case VK_F64: launch<double>(c, make_count<double>(s.cmp, s.cbits), x, n, state); break;
-        case VK_F32: launch<float>(c, make_count<float>(s.cmp, s.cbits), x, n, state); break;
-        case VK_I64: launch<int64_t>(c, make_count<int64_t>(s.cmp, s.cbits), x, n, state); break;
-        case VK_U64: launch<uint64_t>(c, make_count<uint64_t>(s.cmp, s.cbits), x, n, state); break;
-        case VK_I32: launch<int32_t>(c, make_count<int32_t>(s.cmp, s.cbits), x, n, state); break;
+        case VK_F64: f(make_count<double>(s.cmp, s.cbits), static_cast<double *>(nullptr)); break;
+        case VK_F32: f(make_count<float>(s.cmp, s.cbits), static_cast<float *>(nullptr)); break;
+        case VK_I64: f(make_count<int64_t>(s.cmp, s.cbits), static_cast<int64_t *>(nullptr)); break;
+        case VK_U64: f(make_count<uint64_t>(s.cmp, s.cbits), static_cast<uint64_t *>(nullptr)); break;
+        case VK_I32: f(make_count<int32_t>(s.cmp, s.cbits), static_cast<int32_t *>(nullptr)); break;
         default: usage_error("unsupported count type");
       }
       break;
-    case ReduceSpec::COUNT_BOOL: launch<uint8_t>(c, OpCountBool{}, x, n, state); break;
-    case ReduceSpec::EXTREME: by_type<OpExtreme>(c, s.type, x, n, state, s.want_max); break;
-    case ReduceSpec::ARG_EXTREME:
-      by_type<OpArg>(c, s.type, x, n, state, s.want_max, s.base_row);
+    case ReduceSpec::COUNT_BOOL: f(OpCountBool{}, static_cast<uint8_t *>(nullptr)); break;
+    case ReduceSpec::EXTREME:
+    case ReduceSpec::ARG_EXTREME: {
+      const bool arg = s.kind == ReduceSpec::ARG_EXTREME;
+      auto go = [&](auto *tag) {
+        using T = typename std::remove_pointer<decltype(tag)>::type;
+        if (arg) f(OpArg<T>{s.want_max, s.base_row}, tag);
+        else f(OpExtreme<T>{s.want_max}, tag);
+      };
+      switch (s.type) {
+        case VK_F64: go(static_cast<double *>(nullptr)); break;
+        case VK_F32: go(static_cast<float *>(nullptr)); break;
+        case VK_I64: go(static_cast<int64_t *>(nullptr)); break;
+        case VK_U64: go(static_cast<uint64_t *>(nullptr)); break;
+        case VK_I32: go(static_cast<int32_t *>(nullptr)); break;
+        case VK_U8: case VK_BOOL: go(static_cast<uint8_t *>(nullptr)); break;
+        default: usage_error("unsupported element type");
+      }
       break;
+    }
     case ReduceSpec::BITS:
       switch (s.type) {
-        case VK_I64: launch<int64_t>(c, OpBits<int64_t>{s.bitop}, x, n, state); break;
-        case VK_U64: launch<uint64_t>(c, OpBits<uint64_t>{s.bitop}, x, n, state); break;
-        case VK_I32: launch<int32_t>(c, OpBits<int32_t>{s.bitop}, x, n, state); break;
-        case VK_U8: case VK_BOOL: launch<uint8_t>(c, OpBits<uint8_t>{s.bitop}, x, n, state); break;
+        case VK_I64: f(OpBits<int64_t>{s.bitop}, static_cast<int64_t *>(nullptr)); break;
+        case VK_U64: f(OpBits<uint64_t>{s.bitop}, static_cast<uint64_t *>(nullptr)); break;
+        case VK_I32: f(OpBits<int32_t>{s.bitop}, static_cast<int32_t *>(nullptr)); break;
+        case VK_U8: case VK_BOOL: f(OpBits<uint8_t>{s.bitop}, static_cast<uint8_t *>(nullptr)); break;
         default: usage_error("unsupported bit fold type");
       }
       break;
   }
+}
+
+// ------------------------------------------------------------------ batched form
+// One launch folds every tile of every item into a per-tile partial, a second merges each
+// item's partials. Every operator here is commutative and associative (the arg operators
+// break ties by index), so the merge order does not matter.
+constexpr uint64_t kBatchTile = 8192;
+
+template <typename T, class Op>
+__global__ __launch_bounds__(kBlock) void k_reduce_tiles(Op op, const BatchItem *items,
+                                                         const uint32_t *tile_item,
+                                                         typename Op::Acc *partials) {
+  using Acc = typename Op::Acc;
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const T *x = static_cast<const T *>(I.a);
+  const uint64_t t0 = (blockIdx.x - I.tile0) * kBatchTile;
+  const uint64_t t1 = t0 + kBatchTile < I.n ? t0 + kBatchTile : I.n;
+  Acc a = op.identity();
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) op.add(a, x[r], r);
+  a = block_merge(op, a);
+  if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+template <class Op>
+__global__ void k_reduce_items(Op op, const BatchItem *items, uint64_t nitems,
+                               const typename Op::Acc *partials, typename Op::Acc *states) {
+  const uint64_t it = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (it >= nitems) return;
+  const BatchItem &I = items[it];
+  const uint64_t tiles = (I.n + kBatchTile - 1) / kBatchTile;
+  typename Op::Acc s = op.identity();
+  for (uint64_t k = 0; k < tiles; ++k) op.merge(s, partials[I.tile0 + k]);
+  states[it] = s;
+}
+
+}  // namespace
+
+void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *state) {
+  if (n == 0) return;
+  with_op(s, [&](auto op, auto *tag) {
+    using T = typename std::remove_pointer<decltype(tag)>::type;
+    launch<T>(c, op, x, n, state);
+  });
+}
+
+uint32_t reduce_state_words(const ReduceSpec &s) {
+  return s.kind == ReduceSpec::EXTREME ? 2 : s.kind == ReduceSpec::ARG_EXTREME ? 3 : 1;
+}
+
+void reduce_batch(Ctx &c, const ReduceSpec &s, std::vector<BatchItem> &items, uint64_t *states) {
+  const uint64_t nitems = items.size();
+  if (nitems == 0) return;
+  BatchPlan p = batch_plan(c, items, kBatchTile);
+  with_op(s, [&](auto op, auto *tag) {
+    using T = typename std::remove_pointer<decltype(tag)>::type;
+    using Acc = typename decltype(op)::Acc;
+    Acc *parts = static_cast<Acc *>(tmp_alloc(c, (p.ntiles ? p.ntiles : 1) * sizeof(Acc)));
+    if (p.ntiles)
+      k_reduce_tiles<T, decltype(op)><<<p.ntiles, kBlock, 0, c.stream>>>(op, p.items, p.tile_item, parts);
+    k_reduce_items<decltype(op)><<<stream_grid(nitems, kBlock, 1u << 20), kBlock, 0, c.stream>>>(
+        op, p.items, nitems, parts, reinterpret_cast<Acc *>(states));
+    check_hip(hipGetLastError(), "reduce batch");
+    tmp_free(c, parts);
+  });
+  batch_plan_free(c, p);
 }
 
 }  // namespace vk

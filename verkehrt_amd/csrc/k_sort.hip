@@ -26,6 +26,7 @@ constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;  // 4096 rows
 constexpr int kWaves = kBlock / kWave;
 constexpr int kRadix = 256;
+constexpr int32_t kRawU32 = 100;  // internal key type: u32 keys sorted as they are
 
 template <typename U>
 __device__ __forceinline__ U image(int32_t type, const void *keys, uint64_t r) {
@@ -37,6 +38,7 @@ __device__ __forceinline__ U image(int32_t type, const void *keys, uint64_t r) {
     }
     case VK_I32: return U(static_cast<const uint32_t *>(keys)[r] ^ 0x80000000u);
     case VK_U8: return U(static_cast<const uint8_t *>(keys)[r]);
+    case kRawU32: return U(static_cast<const uint32_t *>(keys)[r]);
     default: return U(static_cast<const uint64_t *>(keys)[r]);  // u64
   }
 }
@@ -70,30 +72,86 @@ __device__ __forceinline__ uint64_t pack(uint32_t epoch, uint64_t flag, uint64_t
 }
 constexpr uint64_t kCountMask = (uint64_t(1) << 46) - 1;
 
-// FIRST: keys are the typed input (imaged here) and positions are the row numbers.
-// LAST: positions go to out_pos as i64 (and keys to keys_out when given).
-template <typename U, bool FIRST, bool LAST>
-__global__ __launch_bounds__(kBlock) void k_pass(int32_t type, const void *keys_in,
-                                                 const uint32_t *__restrict__ pos_in, uint64_t n,
-                                                 int shift, const uint32_t *__restrict__ digit_base,
-                                                 U *__restrict__ keys_out, uint32_t *__restrict__ pos_out,
-                                                 int64_t *__restrict__ out_pos, uint64_t *status,
-                                                 uint32_t epoch, uint32_t *ticket, uint32_t ntiles) {
+// The arguments of one pass. Batched passes (BATCH) take each tile's item from the batch
+// tables: the item's typed keys (`a`), rows (`n`), first tile (`tile0`), first row in the
+// concatenated key and position buffers (`aux`) and its i64 output (`out`); its digit bases
+// are at digit_base + (item * P + d) * 256.
+// The payload V travels with the key: u32 row positions (argsort: FIRST takes the row number,
+// LAST writes it to out_pos as i64) or u64 value bits (FIRST takes vals[row], LAST writes the
+// sorted keys and values; batched items take their values from `b`).
+template <typename U, typename V>
+struct PassArgs {
+  int32_t type;
+  const void *keys_in;  // FIRST: typed keys; otherwise key images
+  const V *pay_in;
+  const uint64_t *vals;  // FIRST with u64 payload: the values
+  uint64_t n;
+  int shift;
+  const uint32_t *digit_base;
+  U *keys_out;
+  V *pay_out;
+  int64_t *out_pos;
+  uint64_t *status;
+  uint32_t epoch;
+  uint32_t *ticket;
+  uint32_t ntiles;
+  const BatchItem *items;
+  const uint32_t *tile_item;
+  int P, d;
+};
+
+// FIRST: keys are the typed input (imaged here); the payload starts as described above.
+// LAST: positions go to out_pos as i64 (and keys to keys_out when given); values go with
+// their keys to keys_out / pay_out.
+template <typename U, typename V, bool FIRST, bool LAST, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
+  constexpr bool POS = std::is_same<V, uint32_t>::value;
   __shared__ uint32_t s_wcnt[kWaves][kRadix];  // per-wave counters, then per-wave offsets
   __shared__ uint32_t s_excl[kRadix];           // tile-local exclusive digit offsets
   __shared__ uint64_t s_gbase[kRadix];          // global position of the tile's first row per digit
   __shared__ U s_keys[kTile];
-  __shared__ uint32_t s_pos[kTile];
+  __shared__ V s_pos[kTile];
   __shared__ uint32_t s_wsum[kWaves];
 
-  const uint32_t tile = take_ticket(ticket, ntiles);
-  const uint64_t t0 = uint64_t(tile) * kTile;
+  const int32_t type = a.type;
+  const uint32_t tile = take_ticket(a.ticket, a.ntiles);
+  uint32_t first = 0;
+  const void *keys_in = a.keys_in;
+  const V *__restrict__ pos_in = a.pay_in;
+  const uint64_t *__restrict__ vals = a.vals;
+  uint64_t n = a.n;
+  const int shift = a.shift;
+  const uint32_t *__restrict__ digit_base = a.digit_base;
+  U *__restrict__ keys_out = a.keys_out;
+  V *__restrict__ pos_out = a.pay_out;
+  int64_t *__restrict__ out_pos = a.out_pos;
+  uint64_t *status = a.status;
+  const uint32_t epoch = a.epoch;
+  if (BATCH) {
+    const uint32_t it = a.tile_item[tile];
+    const BatchItem &I = a.items[it];
+    n = I.n;
+    first = static_cast<uint32_t>(I.tile0);
+    const uint64_t row0 = I.aux;
+    keys_in = FIRST ? I.a : static_cast<const void *>(static_cast<const U *>(a.keys_in) + row0);
+    if (FIRST) vals = static_cast<const uint64_t *>(I.b);
+    else pos_in += row0;
+    digit_base += (uint64_t(it) * a.P + a.d) * kRadix;
+    if (LAST && POS) {
+      out_pos = static_cast<int64_t *>(I.out);
+      keys_out = nullptr;
+    } else {
+      keys_out += row0;
+      pos_out += row0;
+    }
+  }
+  const uint64_t t0 = uint64_t(tile - first) * kTile;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   for (int k = t; k < kWaves * kRadix; k += kBlock) (&s_wcnt[0][0])[k] = 0;
   __syncthreads();
 
   U key[kItems];
-  uint32_t pos[kItems];
+  V pos[kItems];
   uint32_t rnk[kItems];
   const uint64_t wbase = t0 + uint64_t(w) * (kTile / kWaves);
   const uint64_t below = (uint64_t(1) << l) - 1;
@@ -103,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_pass(int32_t type, const void *keys_
     const bool valid = r < n;
     if (valid) {
       key[i] = FIRST ? image<U>(type, keys_in, r) : static_cast<const U *>(keys_in)[r];
-      pos[i] = FIRST ? static_cast<uint32_t>(r) : pos_in[r];
+      if (FIRST) pos[i] = POS ? static_cast<V>(r) : static_cast<V>(vals[r]);
+      else pos[i] = pos_in[r];
     } else {
       key[i] = 0;
       pos[i] = 0;
@@ -152,12 +211,12 @@ __global__ __launch_bounds__(kBlock) void k_pass(int32_t type, const void *keys_
     const int dd = t;
     uint64_t *st = status + uint64_t(tile) * kRadix + dd;
     uint64_t prefix = 0;
-    if (tile == 0) {
+    if (tile == first) {
       st_rlx(st, pack(epoch, 2, cnt));
     } else {
       st_rlx(st, pack(epoch, 1, cnt));
       int64_t j = int64_t(tile) - 1;
-      while (j >= 0) {
+      while (j >= int64_t(first)) {
         const uint64_t wd = ld_rlx(status + uint64_t(j) * kRadix + dd);
         const uint64_t flag = (wd >> 46) & 3;
         if ((wd >> 48) != epoch || flag == 0) {
@@ -190,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(int32_t type, const void *keys_
     const U k = s_keys[j];
     const uint32_t d = (k >> shift) & 255;
     const uint64_t g = s_gbase[d] + (j - s_excl[d]);
-    if (LAST) {
+    if (LAST && POS) {
       out_pos[g] = static_cast<int64_t>(s_pos[j]);
       if (keys_out) keys_out[g] = k;
     } else {
@@ -213,22 +272,189 @@ __global__ void k_copy_images(int32_t type, const void *keys, uint64_t n, U *out
     out[i] = image<U>(type, keys, i);
 }
 
-template <typename U, bool FIRST, bool LAST>
-void run_pass(Ctx &c, int32_t type, const void *kin, const uint32_t *pin, uint64_t n, int shift,
-              const uint32_t *dbase, U *kout, uint32_t *pout, int64_t *out_pos) {
+template <typename U, typename V, bool FIRST, bool LAST>
+void run_pass(Ctx &c, int32_t type, const void *kin, const V *pin, const uint64_t *vals, uint64_t n,
+              int shift, const uint32_t *dbase, U *kout, V *pout, int64_t *out_pos) {
   const uint64_t tiles = (n + kTile - 1) / kTile;
-  uint64_t *status = status_words(c, ST_SORT, tiles * kRadix);
-  const uint32_t epoch = next_epoch(c);
-  k_pass<U, FIRST, LAST><<<uint32_t(tiles), kBlock, 0, c.stream>>>(
-      type, kin, pin, n, shift, dbase, kout, pout, out_pos, status, epoch, c.scratch.counters,
-      uint32_t(tiles));
+  PassArgs<U, V> a{};
+  a.type = type;
+  a.keys_in = kin;
+  a.pay_in = pin;
+  a.vals = vals;
+  a.n = n;
+  a.shift = shift;
+  a.digit_base = dbase;
+  a.keys_out = kout;
+  a.pay_out = pout;
+  a.out_pos = out_pos;
+  a.status = status_words(c, ST_SORT, tiles * kRadix);
+  a.epoch = next_epoch(c);
+  a.ticket = c.scratch.counters;
+  a.ntiles = uint32_t(tiles);
+  k_pass<U, V, FIRST, LAST, false><<<uint32_t(tiles), kBlock, 0, c.stream>>>(a);
   check_hip(hipGetLastError(), "radix pass");
 }
 
-// Sorts rows by image; positions of the sorted rows to out_pos, sorted images to
-// sorted_keys when not null.
+// ------------------------------------------------------------------ batched form
+// Every item sorts in its own rows: one launch per pass for all items, the tiles of an item
+// looking back only to the item's first tile, per-item digit histograms and bases. Digits
+// that are constant in every item are skipped; a digit constant in some items only is a
+// stable identity pass for those.
+
 template <typename U>
-void radix(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out_pos, U *sorted_keys) {
+__global__ __launch_bounds__(kBlock) void k_hist_batch(int32_t type, const BatchItem *items,
+                                                       const uint32_t *tile_item, uint32_t *hist) {
+  constexpr int P = sizeof(U);
+  __shared__ uint32_t s_h[P][kRadix];
+  for (int k = threadIdx.x; k < P * kRadix; k += kBlock) (&s_h[0][0])[k] = 0;
+  __syncthreads();
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
+  const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) {
+    const U k = image<U>(type, I.a, r);
+#pragma unroll
+    for (int d = 0; d < P; ++d) atomicAdd(&s_h[d][(k >> (8 * d)) & 255], 1u);
+  }
+  __syncthreads();
+  uint32_t *h = hist + uint64_t(it) * P * kRadix;
+  for (int k = threadIdx.x; k < P * kRadix; k += kBlock) {
+    const uint32_t v = (&s_h[0][0])[k];
+    if (v) atomicAdd(&h[k], v);
+  }
+}
+
+// One thread per (item, digit): counts to exclusive bases in place; active[d] counts the
+// items in which digit d takes more than one value.
+__global__ void k_base_batch(const BatchItem *items, uint64_t nitems, int P, uint32_t *hist,
+                             uint32_t *active) {
+  const uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (k >= nitems * P) return;
+  const uint64_t n = items[k / P].n;
+  uint32_t *h = hist + k * kRadix;
+  uint32_t s = 0;
+  bool constant = false;
+  for (int v = 0; v < kRadix; ++v) {
+    const uint32_t cnt = h[v];
+    h[v] = s;
+    s += cnt;
+    if (cnt == n) constant = true;
+  }
+  if (n > 0 && !constant) atomicAdd(&active[k % P], 1u);
+}
+
+__global__ void k_iota_batch(const BatchItem *items, const uint32_t *tile_item) {
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
+  const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
+  int64_t *out = static_cast<int64_t *>(I.out);
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) out[r] = static_cast<int64_t>(r);
+}
+
+template <typename U, typename V, bool FIRST, bool LAST>
+void run_pass_batch(Ctx &c, int32_t type, const BatchPlan &p, int P, int d, const void *kin,
+                    const V *pin, const uint32_t *hist, U *kout, V *pout) {
+  PassArgs<U, V> a{};
+  a.type = type;
+  a.keys_in = kin;
+  a.pay_in = pin;
+  a.shift = 8 * d;
+  a.digit_base = hist;
+  a.keys_out = kout;
+  a.pay_out = pout;
+  a.status = status_words(c, ST_SORT, uint64_t(p.ntiles) * kRadix);
+  a.epoch = next_epoch(c);
+  a.ticket = c.scratch.counters;
+  a.ntiles = p.ntiles;
+  a.items = p.items;
+  a.tile_item = p.tile_item;
+  a.P = P;
+  a.d = d;
+  k_pass<U, V, FIRST, LAST, true><<<p.ntiles, kBlock, 0, c.stream>>>(a);
+  check_hip(hipGetLastError(), "radix batch pass");
+}
+
+// Batched sort: positions mode (V = u32) writes each item's i64 positions to item.out;
+// values mode (V = u64) sorts (key, value bits of item.b) pairs into sorted_keys / sorted_vals
+// at each item's first row (item.aux, set here) of the concatenated rows.
+template <typename U, typename V>
+void radix_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, U *sorted_keys = nullptr,
+                 V *sorted_vals = nullptr) {
+  constexpr bool POS = std::is_same<V, uint32_t>::value;
+  constexpr int P = sizeof(U);
+  uint64_t total = 0;
+  for (auto &it : items) {
+    if (it.n > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
+    it.aux = total;
+    total += it.n;
+  }
+  if (total == 0) return;
+  const uint64_t nitems = items.size();
+  BatchPlan p = batch_plan(c, items, kTile);
+  const uint64_t hwords = nitems * P * kRadix;
+  uint32_t *hist = static_cast<uint32_t *>(tmp_alloc(c, hwords * 4 + 64));
+  uint32_t *active = hist + hwords;
+  fill_u64(c, reinterpret_cast<uint64_t *>(hist), (hwords * 4 + 64) / 8, 0);
+  k_hist_batch<U><<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, hist);
+  check_hip(hipGetLastError(), "radix batch histogram");
+  k_base_batch<<<stream_grid(nitems * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(p.items, nitems, P, hist,
+                                                                                 active);
+  check_hip(hipGetLastError(), "radix batch bases");
+  uint32_t act[8] = {0};
+  check_hip(hipMemcpyAsync(act, active, P * 4, hipMemcpyDeviceToHost, c.stream), "active digits");
+  check_hip(hipStreamSynchronize(c.stream), "sync");
+  std::vector<int> digits;
+  for (int d = 0; d < P; ++d)
+    if (act[d]) digits.push_back(d);
+  // with no digit to sort by, one stable pass on digit 0 still moves the rows as they are
+  if (digits.empty() && !POS) digits.push_back(0);
+  if (digits.empty()) {
+    k_iota_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item);
+    check_hip(hipGetLastError(), "radix batch iota");
+  } else {
+    const int np = static_cast<int>(digits.size());
+    U *ka = nullptr, *kb = nullptr;
+    V *pa = nullptr, *pb = nullptr;
+    if (np > 1) {
+      ka = static_cast<U *>(tmp_alloc(c, total * sizeof(U)));
+      pa = static_cast<V *>(tmp_alloc(c, total * sizeof(V)));
+      if (np > 2) {
+        kb = static_cast<U *>(tmp_alloc(c, total * sizeof(U)));
+        pb = static_cast<V *>(tmp_alloc(c, total * sizeof(V)));
+      }
+    }
+    const U *kin = nullptr;
+    const V *pin = nullptr;
+    for (int k = 0; k < np; ++k) {
+      const int d = digits[k];
+      const bool first = k == 0, last = k == np - 1;
+      U *ko = last ? sorted_keys : (k % 2 == 0) ? ka : kb;
+      V *po = last ? sorted_vals : (k % 2 == 0) ? pa : pb;
+      if (first && last) run_pass_batch<U, V, true, true>(c, type, p, P, d, nullptr, nullptr, hist, ko, po);
+      else if (first) run_pass_batch<U, V, true, false>(c, type, p, P, d, nullptr, nullptr, hist, ko, po);
+      else if (last) run_pass_batch<U, V, false, true>(c, type, p, P, d, kin, pin, hist, ko, po);
+      else run_pass_batch<U, V, false, false>(c, type, p, P, d, kin, pin, hist, ko, po);
+      kin = ko;
+      pin = po;
+    }
+    if (ka) tmp_free(c, ka);
+    if (pa) tmp_free(c, pa);
+    if (kb) tmp_free(c, kb);
+    if (pb) tmp_free(c, pb);
+  }
+  tmp_free(c, hist);
+  batch_plan_free(c, p);
+}
+
+// Sorts rows by image. Positions mode (V = u32): positions of the sorted rows to out_pos,
+// sorted images to sorted_keys when not null. Values mode (V = u64): the sorted images and
+// the value bits of `vals` that go with them to sorted_keys / sorted_vals.
+template <typename U, typename V>
+void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_t n, int64_t *out_pos,
+           U *sorted_keys, V *sorted_vals) {
+  constexpr bool POS = std::is_same<V, uint32_t>::value;
   if (n == 0) return;
   if (n > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
   constexpr int P = sizeof(U);
@@ -255,6 +481,7 @@ void radix(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out_pos,
   check_hip(hipMemcpyAsync(hist, base.data(), P * kRadix * 4, hipMemcpyHostToDevice, c.stream), "base");
   check_hip(hipStreamSynchronize(c.stream), "sync");
   const uint32_t g = stream_grid(n, kBlock * 4, 8192);
+  if (active.empty() && !POS) active.push_back(0);  // one stable pass moves the pairs
   if (active.empty()) {
     k_iota64<<<g, kBlock, 0, c.stream>>>(n, out_pos);
     if (sorted_keys) k_copy_images<U><<<g, kBlock, 0, c.stream>>>(type, keys, n, sorted_keys);
@@ -263,31 +490,28 @@ void radix(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out_pos,
   }
   const int np = static_cast<int>(active.size());
   U *ka = nullptr, *kb = nullptr;
-  uint32_t *pa = nullptr, *pb = nullptr;
+  V *pa = nullptr, *pb = nullptr;
   if (np > 1) {
     ka = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
-    pa = static_cast<uint32_t *>(tmp_alloc(c, n * 4));
+    pa = static_cast<V *>(tmp_alloc(c, n * sizeof(V)));
     if (np > 2) {
       kb = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
-      pb = static_cast<uint32_t *>(tmp_alloc(c, n * 4));
+      pb = static_cast<V *>(tmp_alloc(c, n * sizeof(V)));
     }
   }
   const U *kin = nullptr;
-  const uint32_t *pin = nullptr;
+  const V *pin = nullptr;
   for (int k = 0; k < np; ++k) {
     const int d = active[k];
     const uint32_t *db = hist + d * kRadix;
     const bool first = k == 0, last = k == np - 1;
-    U *ko = (k % 2 == 0) ? ka : kb;
-    uint32_t *po = (k % 2 == 0) ? pa : pb;
-    if (first && last)
-      run_pass<U, true, true>(c, type, keys, nullptr, n, 8 * d, db, sorted_keys, nullptr, out_pos);
-    else if (first)
-      run_pass<U, true, false>(c, type, keys, nullptr, n, 8 * d, db, ko, po, nullptr);
-    else if (last)
-      run_pass<U, false, true>(c, type, kin, pin, n, 8 * d, db, sorted_keys, nullptr, out_pos);
-    else
-      run_pass<U, false, false>(c, type, kin, pin, n, 8 * d, db, ko, po, nullptr);
+    U *ko = last ? sorted_keys : (k % 2 == 0) ? ka : kb;
+    V *po = last ? sorted_vals : (k % 2 == 0) ? pa : pb;
+    int64_t *op = last ? out_pos : nullptr;
+    if (first && last) run_pass<U, V, true, true>(c, type, keys, nullptr, vals, n, 8 * d, db, ko, po, op);
+    else if (first) run_pass<U, V, true, false>(c, type, keys, nullptr, vals, n, 8 * d, db, ko, po, op);
+    else if (last) run_pass<U, V, false, true>(c, type, kin, pin, vals, n, 8 * d, db, ko, po, op);
+    else run_pass<U, V, false, false>(c, type, kin, pin, vals, n, 8 * d, db, ko, po, op);
     kin = ko;
     pin = po;
   }
@@ -302,15 +526,42 @@ void radix(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out_pos,
 
 void argsort(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out) {
   switch (type) {
-    case VK_U64: case VK_I64: case VK_F64: radix<uint64_t>(c, type, keys, n, out, nullptr); break;
-    case VK_I32: radix<uint32_t>(c, type, keys, n, out, nullptr); break;
-    case VK_U8: radix<uint8_t>(c, type, keys, n, out, nullptr); break;
+    case VK_U64: case VK_I64: case VK_F64:
+      radix<uint64_t, uint32_t>(c, type, keys, nullptr, n, out, nullptr, nullptr);
+      break;
+    case VK_I32: radix<uint32_t, uint32_t>(c, type, keys, nullptr, n, out, nullptr, nullptr); break;
+    case VK_U8: radix<uint8_t, uint32_t>(c, type, keys, nullptr, n, out, nullptr, nullptr); break;
+    default: usage_error("unsupported argsort type");
+  }
+}
+
+void argsort_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items) {
+  switch (type) {
+    case VK_U64: case VK_I64: case VK_F64: radix_batch<uint64_t, uint32_t>(c, type, items); break;
+    case VK_I32: radix_batch<uint32_t, uint32_t>(c, type, items); break;
+    case VK_U8: radix_batch<uint8_t, uint32_t>(c, type, items); break;
     default: usage_error("unsupported argsort type");
   }
 }
 
 void sort_u64_pairs(Ctx &c, const uint64_t *keys, uint64_t n, uint64_t *sorted_keys, int64_t *perm) {
-  radix<uint64_t>(c, VK_U64, keys, n, perm, sorted_keys);
+  radix<uint64_t, uint32_t>(c, VK_U64, keys, nullptr, n, perm, sorted_keys, nullptr);
+}
+
+void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, uint64_t n,
+                     void *sorted_keys, uint64_t *sorted_vals) {
+  if (wide)
+    radix<uint64_t, uint64_t>(c, VK_U64, keys, vals, n, nullptr, static_cast<uint64_t *>(sorted_keys), sorted_vals);
+  else
+    radix<uint32_t, uint64_t>(c, kRawU32, keys, vals, n, nullptr, static_cast<uint32_t *>(sorted_keys), sorted_vals);
+}
+
+void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
+                           uint64_t *sorted_vals) {
+  if (wide)
+    radix_batch<uint64_t, uint64_t>(c, VK_U64, items, static_cast<uint64_t *>(sorted_keys), sorted_vals);
+  else
+    radix_batch<uint32_t, uint64_t>(c, kRawU32, items, static_cast<uint32_t *>(sorted_keys), sorted_vals);
 }
 
 }  // namespace vk

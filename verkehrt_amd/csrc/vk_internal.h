@@ -201,7 +201,9 @@ struct BatchItem {
   void *out;       // output rows
   uint64_t n;      // rows
   uint64_t tile0;  // first tile of the item in the launch
-  uint64_t aux;    // kernel-specific (where: index base)
+  uint64_t aux;    // kernel-specific (where: index base; sorts: first concatenated row)
+  uint64_t aux2;   // kernel-specific (group_id: first table word)
+  uint64_t aux3;   // kernel-specific (group_id: log2 of the table size)
 };
 struct BatchPlan {
   BatchItem *items = nullptr;     // device copy of the items
@@ -212,6 +214,19 @@ struct BatchPlan {
 // tile -> item map.
 BatchPlan batch_plan(Ctx &c, std::vector<BatchItem> &items, uint64_t rows_per_tile);
 void batch_plan_free(Ctx &c, BatchPlan &p);
+// Batched reductions: item i's final state (the reduce state layout, 8-byte words) at
+// states + i * reduce_state_words(s); items are the rows a (n).
+uint32_t reduce_state_words(const ReduceSpec &s);
+void reduce_batch(Ctx &c, const ReduceSpec &s, std::vector<BatchItem> &items, uint64_t *states);
+// Batched group_id_<t>: item i numbers its keys a (n rows) into the i64 ids at out; the
+// number of groups of item i goes to d_groups[i] (device). Sets aux, aux2, aux3.
+void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_t *d_groups);
+// Batched key checks of reduce_into: item i (keys a, n rows, aux2 buckets) lowers d_bad[i] to
+// its first row whose key is outside 0..buckets (all ones when none).
+void check_keys_batch(Ctx &c, std::vector<BatchItem> &items, uint64_t *d_bad);
+// Batched reduce_into_add_f64 (keys checked): item i folds values b by keys a (n rows) into
+// the aux2 f64 buckets at out, in the order of reduce_into_add_f64. Sets aux3.
+void reduce_into_add_f64_batch(Ctx &c, std::vector<BatchItem> &items);
 // One selection launch over all items; the kept count of item i goes to d_counts[i].
 void select_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts);
 // One canonical-reduction launch over all items (a = x, b = y or null); item i's result to
@@ -247,8 +262,17 @@ void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint6
 
 // Argsort (k_sort.hip): stable ascending positions as i64.
 void argsort(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out);
+// Batched argsort: item i sorts keys a (n rows) into the i64 positions at out; aux is set.
+void argsort_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items);
 // Stable sort of u64 keys: sorted keys and the positions they came from.
 void sort_u64_pairs(Ctx &c, const uint64_t *keys, uint64_t n, uint64_t *sorted_keys, int64_t *perm);
+// Stable sort of (key, 8-byte value) pairs by key: keys are u64 (wide) or u32.
+void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, uint64_t n,
+                     void *sorted_keys, uint64_t *sorted_vals);
+// Batched: item i sorts its keys a with the values b (n rows) into sorted_keys / sorted_vals
+// from row aux (set here) of the concatenated rows.
+void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
+                           uint64_t *sorted_vals);
 
 // group_id (k_group.hip): first-occurrence ids. `b` may be null (one key column).
 void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
