@@ -255,3 +255,20 @@ def test_reduce_into_batch_errors(D):
     with pytest.raises(KernelError) as e4:
         D.call("reduce_into_add_f64", neg)
     assert str(e3.value) == str(e4.value)
+
+
+def test_batched_workload_matches_plain(D):
+    """The batched form of the many-arrays program returns the plain dispatch's results."""
+    from verkehrt_amd import batch as B
+    wl = B.make_workload(D, count=48, lo=1 << 10, hi=1 << 16, seed=7)
+    p = B.plain(D, wl)
+    b = B.batched(D, wl)
+    assert B.same(p, b)
+    # spot-check against the oracle
+    for i in (0, 17, 47):
+        x, k = (a.to_numpy() for a in wl.item(i))
+        assert b.sums[i].bits() == np.float64(O.canonical_sum_f64(x[x > 0.9])).view(np.uint64)
+        assert np.array_equal(b.perms[i].to_numpy(), O.argsort(k))
+        g = O.group_id(k)
+        want = O.reduce_into(g, x, int(g.max()) + 1, "add")
+        assert np.array_equal(b.groups[i].to_numpy().view(np.uint64), want.view(np.uint64))

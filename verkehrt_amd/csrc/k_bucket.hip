@@ -317,38 +317,40 @@ __global__ __launch_bounds__(kBlock) void k_chain_keys_batch(const BatchItem *it
     ck[I.aux + r] = static_cast<K>(static_cast<uint64_t>(keys[r]) * 16 + (r & 15));
 }
 
-// One thread per chain start of every item: fold the chain, eight rows loaded ahead.
+// One wave per 64 sorted rows of an item: the wave folds, one after another, the chains that
+// start in its rows, reading each chain 64 rows at a time (lanes load, lane order folds).
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_chain_fold_batch(const BatchItem *items, const uint32_t *tile_item,
                                                              const K *sk, const double *sv,
                                                              double *stripes) {
   const BatchItem &I = items[tile_item[blockIdx.x]];
-  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile, t1 = t0 + kBTile < I.n ? t0 + kBTile : I.n;
+  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile;
   const K *k_ = sk + I.aux;
   const double *v_ = sv + I.aux;
   double *st = stripes + I.aux3;
-  for (uint64_t p = t0 + threadIdx.x; p < t1; p += kBlock) {
-    const K k = k_[p];
-    if (p > 0 && k_[p - 1] == k) continue;
-    double a = 0.0;
-    for (uint64_t q = p;; q += 8) {
-      K kk[8];
-      double vv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const bool in = q + u < I.n;
-        kk[u] = in ? k_[q + u] : static_cast<K>(~k);
-        vv[u] = in ? v_[q + u] : 0.0;
+  const int l = lane_id();
+  for (uint64_t r0 = t0 + (threadIdx.x / kWave) * kWave; r0 < t0 + kBTile && r0 < I.n;
+       r0 += kBlock) {
+    const uint64_t r = r0 + l;
+    const bool start = r < I.n && (r == 0 || k_[r - 1] != k_[r]);
+    uint64_t starts = __ballot(start);
+    while (starts) {
+      const int s = __builtin_ctzll(starts);
+      starts &= starts - 1;
+      const uint64_t p = r0 + s;
+      const K k = k_[p];
+      double a = 0.0;
+      for (uint64_t q = p;; q += kWave) {
+        const uint64_t rr = q + l;
+        const bool in = rr < I.n && k_[rr] == k;
+        const double v = in ? v_[rr] : 0.0;
+        const uint64_t out = __ballot(!in);
+        const int cnt = out ? __builtin_ctzll(out) : kWave;
+        for (int j = 0; j < cnt; ++j) a = a + __shfl(v, j);
+        if (cnt < kWave) break;
       }
-      bool stop = false;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (!stop && kk[u] == k) a = a + vv[u];
-        else stop = true;
-      }
-      if (stop) break;
+      if (l == 0) st[k] = a;
     }
-    st[k] = a;
   }
 }
 

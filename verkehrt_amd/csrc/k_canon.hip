@@ -550,7 +550,7 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
                                                          c.scratch.counters, p.ntiles, p.items, p.tile_item);
     check_hip(hipGetLastError(), "canon batch");
   }
-  BatchPlan q = batch_plan(c, items, 1);  // the full rows, for the tails
+  BatchPlan q = batch_plan(c, items, ~uint64_t(0) >> 1);  // the item table (full rows), for the tails
   k_canon_batch_finish<<<stream_grid(nitems, kBlock, 4096), kBlock, 0, c.stream>>>(rv, q.items, nitems, acc,
                                                                                   d_out);
   check_hip(hipGetLastError(), "canon batch finish");
