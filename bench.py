@@ -164,6 +164,28 @@ def cpu_baseline(sample_rows: int, budget_s: float = 12.0):
                       f"C restatement of the reference's best x86-64-v4 path on one core of {cpu}"}
 
 
+def batched_suite(D, count: int):
+    """The many-arrays form: `count` arrays with log-uniform lengths in [2^10, 2^20] rows, each
+    running filter_gt(0.9)+sum, argsort and a group-by sum over 256 keys; one call per array
+    and statement (plain) against one batched call per statement. Results are compared bit for
+    bit (all sums, every 16th array's permutation and group sums)."""
+    from verkehrt_amd import batch as B
+    wl = B.make_workload(D, count=count)
+    B.batched(D, wl)  # warm-up
+    tb, rb = B.timed(B.batched, D, wl, repeat=3)
+    tp, rp = B.timed(B.plain, D, wl, repeat=1)
+    sample = B.Results(rb.sums, rb.perms[::16], rb.groups[::16])
+    ref = B.Results(rp.sums, rp.perms[::16], rp.groups[::16])
+    same = B.same(sample, ref)
+    if not same:
+        sys.exit("batched results differ from plain dispatch")
+    return {"arrays": count, "rows": wl.rows, "lengths": "log-uniform in [2^10, 2^20] rows",
+            "program": "filter_gt_f64(x, 0.9) then reduce_add_f64; argsort_i64(k); group_id_i64(k), "
+                       "reduce_max_i64 + 1, reduce_into_add_f64(g, x, ng); keys from 256 values",
+            "plain_ms": round(tp * 1e3, 1), "batched_ms": round(tb * 1e3, 1),
+            "speedup": round(tp / tb, 2), "identical": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +195,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=1 << 24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-arrays batched comparison")
+    ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--only", default="",
                     help="comma-separated primitive labels to run (for profiling); 'copy_f64' adds a "
                          "plain copy (add_f64 x + 0.0) used to calibrate HBM counters")
@@ -260,6 +284,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": prims[dom]["GBps"], "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": measured_traffic(dom, args.log2n),
                 "bytes_per_launch": prims[dom]["bytes"], "launch_us": prims[dom]["us"]}
+    batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     line = {
         "metric": "HBM GB/s per primitive (% of peak); batched-vs-plain speedup; serve p99 ms",
@@ -273,6 +298,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "primitives": prims,
+        "batched": batched,
     }
     print(json.dumps(line))
 

@@ -215,17 +215,32 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
       st_rlx(st, pack(epoch, 2, cnt));
     } else {
       st_rlx(st, pack(epoch, 1, cnt));
+      // the walk reads kLook predecessors per step (independent loads, one latency); a
+      // tile before `first` counts as an inclusive zero
+      constexpr int kLook = 8;
       int64_t j = int64_t(tile) - 1;
-      while (j >= int64_t(first)) {
-        const uint64_t wd = ld_rlx(status + uint64_t(j) * kRadix + dd);
-        const uint64_t flag = (wd >> 46) & 3;
-        if ((wd >> 48) != epoch || flag == 0) {
-          backoff();
-          continue;
+      bool done = false;
+      while (!done) {
+        uint64_t wd[kLook];
+#pragma unroll
+        for (int u = 0; u < kLook; ++u) {
+          const int64_t jj = j - u;
+          wd[u] = jj >= int64_t(first) ? ld_rlx(status + uint64_t(jj) * kRadix + dd) : pack(epoch, 2, 0);
         }
-        prefix += wd & kCountMask;
-        if (flag == 2) break;
-        --j;
+        int u = 0;
+        for (; u < kLook; ++u) {
+          const uint64_t flag = (wd[u] >> 46) & 3;
+          if ((wd[u] >> 48) != epoch || flag == 0) break;  // not published yet: resume there
+          prefix += wd[u] & kCountMask;
+          if (flag == 2) {
+            done = true;
+            break;
+          }
+        }
+        if (!done) {
+          j -= u;
+          if (u < kLook) backoff();
+        }
       }
       st_rlx(st, pack(epoch, 2, prefix + cnt));
     }

@@ -32,6 +32,7 @@ bool valid_type(int32_t t) { return t >= VK_F64 && t <= VK_BOOL; }
 // ------------------------------------------------------------------- pool
 
 void *Pool::alloc(uint64_t bytes) {
+  std::lock_guard<std::mutex> lock(mu_);
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~uint64_t(255);
   auto it = free_.lower_bound(bytes);
@@ -46,7 +47,7 @@ void *Pool::alloc(uint64_t bytes) {
   if (e != hipSuccess) {
     // Give cached blocks back and try once more before reporting.
     (void)hipGetLastError();
-    trim();
+    trim_locked();
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -61,6 +62,7 @@ void *Pool::alloc(uint64_t bytes) {
 
 void Pool::free(void *p) {
   if (!p) return;
+  std::lock_guard<std::mutex> lock(mu_);
   auto it = size_of_.find(p);
   if (it == size_of_.end()) return;
   in_use_ -= it->second;
@@ -68,6 +70,11 @@ void Pool::free(void *p) {
 }
 
 void Pool::trim() {
+  std::lock_guard<std::mutex> lock(mu_);
+  trim_locked();
+}
+
+void Pool::trim_locked() {
   for (auto &kv : free_) {
     (void)hipFree(kv.second);
     held_ -= kv.first;
@@ -77,7 +84,7 @@ void Pool::trim() {
 }
 
 Pool::~Pool() {
-  trim();
+  trim_locked();
   for (auto &kv : size_of_) (void)hipFree(kv.first);
 }
 

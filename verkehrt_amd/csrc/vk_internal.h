@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -41,6 +42,9 @@ bool valid_type(int32_t t);
 class Pool {
  public:
   explicit Pool(int device) : device_(device) {}
+  // Thread safe: an array's last reference may be dropped on any thread. Reuse is ordered by
+  // the owning context's stream; memory used on another context's stream must not be
+  // released before that stream has finished with it.
   void *alloc(uint64_t bytes);
   void free(void *p);
   void trim();
@@ -49,7 +53,9 @@ class Pool {
   ~Pool();
 
  private:
+  void trim_locked();
   int device_;
+  std::mutex mu_;
   std::multimap<uint64_t, void *> free_;
   std::map<void *, uint64_t> size_of_;
   uint64_t held_ = 0, in_use_ = 0;
@@ -67,13 +73,13 @@ struct Buffer {
   void *ptr;
   uint64_t bytes;
   bool owned;
-  int refs;
+  std::atomic<int> refs;
 };
 
 }  // namespace vk
 
 struct vk_array {
-  int refs;
+  std::atomic<int> refs;
   vk::Buffer *buf;
   uint64_t offset;  // bytes from buf->ptr to row 0
   uint64_t len;
