@@ -84,6 +84,8 @@ def measured_traffic(label: str, log2n: int):
     WRITE_SIZE summary (copy-calibrated), or None when there is none for this label or size."""
     if log2n != 28 or not os.path.exists(TRAFFIC_PROFILE):
         return None
+    # gather_i64 runs the same kernel (8-byte rows) on the same sizes as gather_f64
+    label = {"gather_i64": "gather_f64"}.get(label, label)
     for line in open(TRAFFIC_PROFILE):
         parts = line.split()
         if len(parts) == 3 and parts[0] == "traffic" and parts[1] == label:
@@ -186,6 +188,28 @@ def batched_suite(D, count: int):
             "speedup": round(tp / tb, 2), "identical": same}
 
 
+def serve_suite(requests: int, qps: float):
+    """The serving loop: a seeded Poisson stream of `requests` programs at `qps`, sizes
+    log-uniform in [10K, 64M] rows, 40% reusing resident columns, 4 workers, a 200 GB HBM
+    budget; latency percentiles at that rate, and the throughput of the same requests all
+    arriving at once."""
+    from verkehrt_amd import serve as SV
+    reqs = SV.make_requests(requests, qps=qps, seed=11)
+    srv = SV.Server(workers=4, budget=200 * 10**9, key_rows=64 << 20)
+    srv.run(SV.make_requests(50, qps=1e9, seed=99, hi=1 << 20), paced=False)  # warm-up
+    srv.close()
+    paced = srv.run(reqs).summary()
+    srv.close()
+    sat = srv.run(reqs, paced=False).summary()
+    srv.close()
+    return {"requests": requests, "qps": qps, "sizes": "log-uniform in [10K, 64M] rows",
+            "programs": "sum_filter, dot, count, scan, argsort, group_sum (uniform)",
+            "reuse": 0.4, "workers": 4, "budget_GB": 200,
+            "p50_ms": paced["p50_ms"], "p99_ms": paced["p99_ms"], "GBps_at_qps": paced["GBps"],
+            "max_concurrent": paced["max_concurrent"], "columns_created": paced["columns_created"],
+            "saturated_GBps": sat["GBps"], "saturated_wall_s": sat["wall_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +221,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-arrays batched comparison")
     ap.add_argument("--batch-arrays", type=int, default=4096)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--only", default="",
                     help="comma-separated primitive labels to run (for profiling); 'copy_f64' adds a "
                          "plain copy (add_f64 x + 0.0) used to calibrate HBM counters")
@@ -285,6 +312,7 @@ def main():
                 "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": measured_traffic(dom, args.log2n),
                 "bytes_per_launch": prims[dom]["bytes"], "launch_us": prims[dom]["us"]}
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
+    serve = None if (args.no_serve or args.only) else serve_suite(args.serve_requests, args.serve_qps)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     line = {
         "metric": "HBM GB/s per primitive (% of peak); batched-vs-plain speedup; serve p99 ms",
@@ -299,6 +327,7 @@ def main():
         "cpu_baseline": cpu,
         "primitives": prims,
         "batched": batched,
+        "serve": serve,
     }
     print(json.dumps(line))
 

@@ -88,8 +88,12 @@ constexpr int kFastWaves = kFastThreads / kWave;
 constexpr int kRunBlocks = 16;
 constexpr int kTileBlocks = kFastThreads / 16 * kRunBlocks;  // 1024 blocks
 constexpr uint64_t kFastMinBlocks = kTileBlocks;
-constexpr int kStride = 192;  // status words per tile
-constexpr int W_AAGG = 1, W_AINC = 17, W_F = 33, W_EXACT = 161;
+// Status words per tile: [0] flags; approximate aggregate and inclusive sums per chain; the
+// tile summary per chain (6 words, the binade word at +7); exact end value per chain, and per
+// chain a flag word (epoch << 8 | 1) once that value is published. Exact values are published
+// chain by chain, so one chain's slow evaluation holds up no other chain.
+constexpr int kStride = 200;
+constexpr int W_AAGG = 1, W_AINC = 17, W_F = 33, W_EXACT = 161, W_XF = 177;
 constexpr uint64_t F_AAGG = 1, F_AINC = 2, F_FN = 4, F_EXACT = 8;
 static_assert(kFastWaves == 16, "the tile summary tree joins 16 waves as 4 x 4");
 
@@ -106,11 +110,36 @@ __device__ __forceinline__ uint64_t wait_flag(const uint64_t *st, uint32_t epoch
     backoff();
   }
 }
+// Waits until chain j of the tile at sk has a published exact value (returns true) or the tile
+// has published its summaries (returns false).
+__device__ __forceinline__ bool wait_exact_or_fn(const uint64_t *sk, uint32_t epoch, int j) {
+  while (true) {
+    // both words in one round trip
+    const uint64_t xf = ld_rlx(sk + W_XF + j);
+    const uint64_t w = ld_rlx(sk);
+    if ((xf >> 8) == epoch) {
+      observe_fence();
+      return true;
+    }
+    if ((w >> 8) == epoch && (w & F_FN)) {
+      observe_fence();
+      return false;
+    }
+    backoff();
+  }
+}
+// Waits until chain j of the tile at sk has a published exact value; returns it.
+__device__ __forceinline__ double wait_exact(const uint64_t *sk, uint32_t epoch, int j);
 __device__ __forceinline__ double ld_d(const uint64_t *p) {
   return __longlong_as_double(static_cast<long long>(ld_rlx(p)));
 }
 __device__ __forceinline__ void st_d(uint64_t *p, double v) {
   st_rlx(p, static_cast<uint64_t>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ double wait_exact(const uint64_t *sk, uint32_t epoch, int j) {
+  while ((ld_rlx(sk + W_XF + j) >> 8) != epoch) backoff();
+  observe_fence();
+  return ld_d(sk + W_EXACT + j);
 }
 __device__ __forceinline__ uint64_t pack_binade(const Binade &h) {
   return (static_cast<uint64_t>(static_cast<uint32_t>(h.e + 2048)) << 2) | (uint64_t(h.neg) << 1) |
@@ -121,53 +150,103 @@ __device__ __forceinline__ Binade unpack_binade(uint64_t w) {
 }
 
 // The exact fold of chain c's rows of blocks b0 .. b0 + nb - 1 from the exact value s, by
-// one wave: lane r holds run r (blocks 16r .. 16r + 15). Runs are summarised under the binade
-// of the current value and scanned in order; the runs before the first one whose check fails
-// are applied at once, that run is folded sequentially by its lane, and the rest is taken up
-// again from the new value. Each round passes at least one run, and a round only fails where
-// the value leaves its binade (or meets a huge, infinite or NaN element), so typical chains
-// take two rounds.
+// one wave: lane r holds run r (blocks 16r .. 16r + 15). Two ways, picked by how often the
+// binade of the approximate running value (s plus the approximate sums of the runs before)
+// changes between runs:
+//  - few changes (the usual single crossing): rounds. Runs are summarised under the binade of
+//    the current exact value and scanned in order; the runs before the first one whose check
+//    fails are applied at once, that run is folded element by element, and the next round
+//    starts after it. Each round passes at least one run.
+//  - many changes (a running value near zero): each run is summarised under the binade of its
+//    own approximate start and the runs are applied one after another; a run whose check
+//    fails is folded element by element. One pass whatever the data.
+// A check fails only where the value leaves its binade or meets a huge, infinite or NaN
+// element, and a failed check is always followed by the element-by-element fold, so the
+// result is the sequential one either way.
 __device__ double eval_chain(const RowVal &rv, const double *x, const double *y, uint64_t b0, int nb,
                              int c, double s) {
   const int l = lane_id();
   double v[kRunBlocks];
+  double asum = 0.0;
 #pragma unroll
   for (int k = 0; k < kRunBlocks; ++k) {
     const int lb = l * kRunBlocks + k;
     v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + c) : 0.0;
+    asum = asum + v[k];
   }
   const int nruns = (nb + kRunBlocks - 1) / kRunBlocks;
+  // approximate start of each run: an inclusive scan of the run sums, less the own sum
+  double inc = asum;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double o = __shfl_up(inc, d);
+    if (l >= d) inc = inc + o;
+  }
+  const Binade ha = binade_of(s + (inc - asum));
+  const Binade hp{__shfl_up(ha.e, 1), __shfl_up(ha.neg, 1), __shfl_up(ha.ok, 1), 0};
+  const uint64_t changes = __ballot(l > 0 && l < nruns && !same_binade(ha, hp));
   double cur = s;
-  for (int r0 = 0; r0 < nruns;) {
-    const Binade h = binade_of(cur);
-    RunFn f = run_identity();
-    if (h.ok && l >= r0) {
-      const BinadeRef ref = binade_ref(h);
+  if (__popcll(changes) <= 2) {
+    for (int r0 = 0; r0 < nruns;) {
+      const Binade h = binade_of(cur);
+      RunFn f = run_identity();
+      if (h.ok && l >= r0) {
+        const BinadeRef ref = binade_ref(h);
 #pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k)
-        if (l * kRunBlocks + k < nb) run_push(f, ref, v[k]);
-    }
+        for (int k = 0; k < kRunBlocks; ++k)
+          if (l * kRunBlocks + k < nb) run_push(f, ref, v[k]);
+      }
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const RunFn o = shfl_up_pod(f, d);
-      if (l >= d) f = run_join(o, f);
-    }
-    double val = cur;
-    const bool ok = l < r0 || run_apply(f, h, cur, &val);
-    const uint64_t fails = __ballot(!ok);
-    if (fails == 0) {
-      cur = __shfl(val, kWave - 1);
-      break;
-    }
-    const int m = __builtin_ctzll(fails);
-    double t = m > r0 ? __shfl(val, m - 1) : cur;
-    if (l == m) {
+      for (int d = 1; d < kWave; d <<= 1) {
+        const RunFn o = shfl_up_pod(f, d);
+        if (l >= d) f = run_join(o, f);
+      }
+      double val = cur;
+      const bool ok = l < r0 || run_apply(f, h, cur, &val);
+      const uint64_t fails = __ballot(!ok);
+      if (fails == 0) {
+        cur = __shfl(val, kWave - 1);
+        break;
+      }
+      const int m = __builtin_ctzll(fails);
+      double t = m > r0 ? __shfl(val, m - 1) : cur;
+      if (l == m) {
 #pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k)
-        if (m * kRunBlocks + k < nb) t = t + v[k];
+        for (int k = 0; k < kRunBlocks; ++k)
+          if (m * kRunBlocks + k < nb) t = t + v[k];
+      }
+      cur = __shfl(t, m);
+      r0 = m + 1;
     }
-    cur = __shfl(t, m);
-    r0 = m + 1;
+    return cur;
+  }
+  RunFn f = run_identity();
+  if (ha.ok) {
+    const BinadeRef ref = binade_ref(ha);
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k)
+      if (l * kRunBlocks + k < nb) run_push(f, ref, v[k]);
+  }
+  for (int r = 0; r < nruns; ++r) {
+    RunFn fr;
+    fr.d[0] = __shfl(f.d[0], r);
+    fr.d[1] = __shfl(f.d[1], r);
+    fr.lo[0] = __shfl(f.lo[0], r);
+    fr.lo[1] = __shfl(f.lo[1], r);
+    fr.hi[0] = __shfl(f.hi[0], r);
+    fr.hi[1] = __shfl(f.hi[1], r);
+    const Binade hr{__shfl(ha.e, r), __shfl(ha.neg, r), __shfl(ha.ok, r), 0};
+    double out;
+    if (run_apply(fr, hr, cur, &out)) {
+      cur = out;
+    } else {
+      const int nk = nb - r * kRunBlocks < kRunBlocks ? nb - r * kRunBlocks : kRunBlocks;
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k) {
+        const double u = __shfl(v[k], r);
+        if (k < nk) cur = cur + u;
+      }
+    }
   }
   return cur;
 }
@@ -335,7 +414,12 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const do
       st_d(fp + 3, ft.lo[1]);
       st_d(fp + 4, ft.hi[0]);
       st_d(fp + 5, ft.hi[1]);
-      st_rlx(fp + 7, pack_binade(hj));
+      // A summary that fails even at the approximate start (a binade crossing or a huge,
+      // infinite or NaN element inside the tile) is published as opaque: later tiles stop at
+      // it and wait for its exact value instead of joining a summary that cannot apply.
+      double probe;
+      const bool usable = run_apply(ft, hj, pre, &probe);
+      st_rlx(fp + 7, pack_binade(usable ? hj : Binade{0, 0, 0, 0}));
     }
     publish_fence();
     if (l == 0) st_rlx(st, flag_word(epoch, F_AAGG | F_AINC | F_FN));
@@ -362,9 +446,9 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const do
           stop = true;
         } else {
           const uint64_t *sk = status + uint64_t(T) * kStride;
-          const uint64_t fw = wait_flag(sk, epoch, F_FN | F_EXACT);
+          const bool exact_there = wait_exact_or_fn(sk, epoch, j);
           const uint64_t *fp = sk + W_F + 8 * j;
-          if (fw & F_EXACT) {
+          if (exact_there) {
             stop = true;
           } else if (!same_binade(unpack_binade(ld_rlx(fp + 7)), hj)) {
             stop = true;
@@ -400,9 +484,7 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const do
             if (Ts < int64_t(first)) {
               base = acc_in[l];
             } else {
-              const uint64_t *ss = status + uint64_t(Ts) * kStride;
-              wait_flag(ss, epoch, F_EXACT);
-              base = ld_d(ss + W_EXACT + l);
+              base = wait_exact(status + uint64_t(Ts) * kStride, epoch, l);
             }
             open = false;
           }
@@ -418,37 +500,39 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const do
   // 6. apply; a chain whose check fails is evaluated exactly by wave (chain), from the exact
   //    value at the tile's start: `base` moved over the joined predecessors, or, where that
   //    check fails too, the previous tile's published end
-  if (w == 0 && l < 16) {
+  //    Chains that apply are published at once; the others after their evaluation.
+  if (w == 0) {
     double exact = 0.0;
-    const bool done = run_apply(run_join(g, ft), hj, base, &exact);
-    if (!done && base_tile != int64_t(tile) - 1) {
-      double start = 0.0;
-      if (run_apply(g, hj, base, &start)) {
-        base = start;
-      } else {
-        const uint64_t *sp = status + uint64_t(tile - 1) * kStride;
-        wait_flag(sp, epoch, F_EXACT);
-        base = ld_d(sp + W_EXACT + l);
-      }
+    bool done = true;
+    if (l < 16) {
+      done = run_apply(run_join(g, ft), hj, base, &exact);
+      s_exact[l] = exact;
+      s_fail[l] = !done;
+      if (done) st_d(st + W_EXACT + l, exact);
     }
-    s_base[l] = base;
-    s_exact[l] = exact;
-    s_fail[l] = !done;
+    publish_fence();
+    if (l < 16 && done) st_rlx(st + W_XF + l, flag_word(epoch, 1));
+    if (l < 16 && !done && base_tile != int64_t(tile) - 1) {
+      double start = 0.0;
+      if (run_apply(g, hj, base, &start)) base = start;
+      else base = wait_exact(status + uint64_t(tile - 1) * kStride, epoch, l);
+    }
+    if (l < 16) s_base[l] = base;
   }
   __syncthreads();
   static_assert(kFastWaves == 16, "one wave per chain");
   if (s_fail[w]) {
     const double e = eval_chain(rv, x, y, b0, nb, w, s_base[w]);
-    if (l == 0) s_exact[w] = e;
+    if (l == 0) {
+      s_exact[w] = e;
+      st_d(st + W_EXACT + w, e);
+    }
+    publish_fence();
+    if (l == 0) st_rlx(st + W_XF + w, flag_word(epoch, 1));
   }
   __syncthreads();
   if (w != 0) return;
-  if (l < 16) {
-    const double exact = s_exact[l];
-    st_d(st + W_EXACT + l, exact);
-    if (tile == last) acc_out[l] = exact;
-  }
-  publish_fence();
+  if (l < 16 && tile == last) acc_out[l] = s_exact[l];
   if (l == 0) st_rlx(st, flag_word(epoch, F_AAGG | F_AINC | F_FN | F_EXACT));
 }
 
