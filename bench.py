@@ -188,6 +188,68 @@ def batched_suite(D, count: int):
             "speedup": round(tp / tb, 2), "identical": same}
 
 
+def sort_suite(D):
+    """Sort and grouping cases: argsort of 2^27 keys (u64 uniform splitmix64, i64 Zipf(1.1)
+    over 2^20 values, f64 normal(0,1) with +-0), group-by sum of rand_f64 values over i64 keys
+    with 1K, 1M and 64M distinct values at 2^28 rows (group_id, reduce_max + 1,
+    reduce_into_add_f64), and scan_by add over 2^28 rows with starts of probability 1/16.
+    Times are the median of three calls from HIP events; GB/s counts the algorithmic bytes."""
+    from verkehrt_amd import Array, Scalar as S
+    ctx = D.ctx
+
+    def timed(name, a, reps=3):
+        D.call(name, a)
+        ctx.sync()
+        ctx.set_trace(True)
+        ctx.take_trace()
+        for _ in range(reps):
+            out = D.call(name, a)
+            del out
+        ctx.sync()
+        tr = parse_trace(ctx.take_trace())
+        ctx.set_trace(False)
+        return float(np.median([us for _, us in tr]))
+
+    out = {}
+    n = 1 << 27
+    rng = np.random.default_rng(3)
+    keys = {"u64": Array.from_numpy(ctx, splitmix64_np(7, 0, n), "u64"),
+            "i64_zipf": Array.from_numpy(ctx, np.minimum(rng.zipf(1.1, n), 1 << 20).astype(np.int64)),
+            "f64_normal": None}
+    f = rng.normal(size=n)
+    f[rng.integers(0, n, 1000)] = 0.0
+    f[rng.integers(0, n, 1000)] = -0.0
+    keys["f64_normal"] = Array.from_numpy(ctx, f)
+    del f
+    for label, a in keys.items():
+        name = {"u64": "argsort_u64", "i64_zipf": "argsort_i64", "f64_normal": "argsort_f64"}[label]
+        us = timed(name, [a])
+        out[f"argsort_{label}"] = {"rows": n, "us": round(us, 1)}
+    del keys
+    n = 1 << 28
+    vals = D.call("rand_f64", [S("i64", n), S("i64", 1)])
+    for g in (1 << 10, 1 << 20, 1 << 26):
+        pool = splitmix64_np(11, 0, g).view(np.int64)
+        sel = (splitmix64_np(12, 0, n) % np.uint64(g)).astype(np.int64)
+        k = Array.from_numpy(ctx, pool[sel])
+        del sel
+        t_id = timed("group_id_i64", [k], reps=2)
+        ids = D.call("group_id_i64", [k])
+        ng = D.call("reduce_max_i64", [ids]).value + 1
+        t_ri = timed("reduce_into_add_f64", [ids, vals, S("i64", ng)], reps=2)
+        out[f"group_sum_{g}"] = {"rows": n, "group_id_us": round(t_id, 1), "reduce_into_us": round(t_ri, 1),
+                                 "GBps_min_io": round((16 * n + 8 * g) / (t_id + t_ri) / 1e3, 1)}
+        del k, ids
+    st = np.zeros(n, np.uint8)
+    st[(splitmix64_np(13, 0, n) % np.uint64(16)) == 0] = 1
+    sa = Array.from_numpy(ctx, st, "bool")
+    del st
+    iv = Array.from_numpy(ctx, i64_uniform_40(n, 4))
+    us = timed("scan_by_add_i64", [iv, sa])
+    out["scan_by_add_i64"] = {"rows": n, "us": round(us, 1), "GBps": round(17 * n / us / 1e3, 1)}
+    return out
+
+
 def serve_suite(requests: int, qps: float):
     """The serving loop: a seeded Poisson stream of `requests` programs at `qps`, sizes
     log-uniform in [10K, 64M] rows, 40% reusing resident columns, 4 workers, a 200 GB HBM
@@ -222,6 +284,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the many-arrays batched comparison")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--sort", action="store_true",
+                    help="also time the sort and grouping cases (argsort of 2^27 keys, group-by sum "
+                         "over 1K/1M/64M groups at 2^28 rows, scan_by add)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--only", default="",
@@ -313,6 +378,7 @@ def main():
                 "bytes_per_launch": prims[dom]["bytes"], "launch_us": prims[dom]["us"]}
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
     serve = None if (args.no_serve or args.only) else serve_suite(args.serve_requests, args.serve_qps)
+    sort = sort_suite(D) if args.sort else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     line = {
         "metric": "HBM GB/s per primitive (% of peak); batched-vs-plain speedup; serve p99 ms",
@@ -328,6 +394,7 @@ def main():
         "primitives": prims,
         "batched": batched,
         "serve": serve,
+        "sort": sort,
     }
     print(json.dumps(line))
 

@@ -18,7 +18,10 @@
  * All entries of one `vk_ctx` run in order on that context's HIP stream. An entry returns
  * after its result is final: a returned scalar or error is known on the host; a returned
  * array is complete for every later use on the same context (and for vk_array_to_host).
- * A context is not thread safe: use one context per host thread.
+ * A context is not thread safe: use one context per host thread. Arrays may be shared between
+ * contexts: reference counts are atomic and memory returns to its creator's pool from any
+ * thread. An array made on context A is ready for context B after vk_ctx_sync(A), and B's
+ * user keeps a reference until B has finished with it (vk_ctx_sync(B)).
  *
  * Every function returns 0 on success. On failure it returns nonzero and fills `st`
  * (when not NULL): code VK_ERR_RUNTIME carries a runtime error of the kernel, whose text
