@@ -176,49 +176,43 @@ __device__ __forceinline__ typename Op::Acc block_merge(const Op &op, typename O
   return a;  // valid in wave 0
 }
 
+// Each block folds its grid-stride share into one partial; a one-block kernel merges the
+// partials into the reducer state. The kernel boundary orders the partials before the merge, so
+// no block waits on a counter (a same-address atomic per block costs ~11 ns, serialised).
 template <typename T, class Op>
 __global__ __launch_bounds__(kBlock) void k_reduce(Op op, const T *__restrict__ x, uint64_t n,
-                                                   typename Op::Acc *partials, uint32_t *done,
-                                                   typename Op::Acc *state) {
+                                                   typename Op::Acc *partials) {
   using Acc = typename Op::Acc;
   Acc a = op.identity();
   grid_rows(x, n, [&](T v, uint64_t row) { op.add(a, v, row); });
   a = block_merge(op, a);
-  __shared__ bool s_last;
-  if (threadIdx.x == 0) {
-    // the partial goes out through agent-coherent stores, complete before the count
-    st_rlx_pod(&partials[blockIdx.x], a);
-    publish_fence();
-    uint32_t prev = atomicAdd(done, 1u);
-    s_last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  observe_fence();
+  if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_reduce_merge(Op op, const typename Op::Acc *partials,
+                                                         uint32_t nparts, typename Op::Acc *state) {
+  using Acc = typename Op::Acc;
   Acc b = op.identity();
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-    // partials of other XCDs: read through the agent-coherent path
-    const Acc p = ld_rlx_pod(&partials[i]);
-    op.merge(b, p);
-  }
-  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) op.merge(b, partials[i]);
   b = block_merge(op, b);
   if (threadIdx.x == 0) {
     Acc s = *state;
     op.merge(s, b);
     *state = s;
-    atomicExch(done, 0u);
   }
 }
+
+constexpr uint32_t kReduceGrid = 2048;
 
 template <typename T, class Op>
 void launch(Ctx &c, const Op &op, const void *x, uint64_t n, void *state) {
   using Acc = typename Op::Acc;
   const uint32_t per_block = kBlock * (16 / sizeof(T)) * 4;
-  const uint32_t grid = stream_grid(n, per_block, 2048);
+  const uint32_t grid = stream_grid(n, per_block, kReduceGrid);
   Acc *parts = static_cast<Acc *>(partials(c, sizeof(Acc) * grid));
-  k_reduce<T, Op><<<grid, kBlock, 0, c.stream>>>(op, static_cast<const T *>(x), n, parts,
-                                                 c.scratch.counters + 1, static_cast<Acc *>(state));
+  k_reduce<T, Op><<<grid, kBlock, 0, c.stream>>>(op, static_cast<const T *>(x), n, parts);
+  k_reduce_merge<Op><<<1, kBlock, 0, c.stream>>>(op, parts, grid, static_cast<Acc *>(state));
   check_hip(hipGetLastError(), "reduce launch");
 }
 

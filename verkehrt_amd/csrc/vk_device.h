@@ -134,19 +134,25 @@ __device__ __forceinline__ void grid_rows(const T *__restrict__ x, uint64_t n, F
   if (tid < head) f(x[tid], tid);
   const uint64_t nv = (n - head) / V;
   const Vec16<T> *xv = reinterpret_cast<const Vec16<T> *>(x + head);
-  uint64_t i = tid;
-  for (; i + 3 * nth < nv; i += 4 * nth) {
-    Vec16<T> v0 = xv[i], v1 = xv[i + nth], v2 = xv[i + 2 * nth], v3 = xv[i + 3 * nth];
+  // each block streams one contiguous chunk (a whole number of block-wide steps), four
+  // 16-byte loads in flight per thread; measured faster than a grid-wide stride
+  const uint64_t bd = blockDim.x;
+  const uint64_t per = ((nv + gridDim.x - 1) / gridDim.x + bd - 1) / bd * bd;
+  const uint64_t b0 = blockIdx.x * per < nv ? blockIdx.x * per : nv;
+  const uint64_t b1 = b0 + per < nv ? b0 + per : nv;
+  uint64_t i = b0 + threadIdx.x;
+  for (; i + 3 * bd < b1; i += 4 * bd) {
+    Vec16<T> v0 = xv[i], v1 = xv[i + bd], v2 = xv[i + 2 * bd], v3 = xv[i + 3 * bd];
 #pragma unroll
     for (int k = 0; k < V; ++k) f(v0.e[k], head + i * V + k);
 #pragma unroll
-    for (int k = 0; k < V; ++k) f(v1.e[k], head + (i + nth) * V + k);
+    for (int k = 0; k < V; ++k) f(v1.e[k], head + (i + bd) * V + k);
 #pragma unroll
-    for (int k = 0; k < V; ++k) f(v2.e[k], head + (i + 2 * nth) * V + k);
+    for (int k = 0; k < V; ++k) f(v2.e[k], head + (i + 2 * bd) * V + k);
 #pragma unroll
-    for (int k = 0; k < V; ++k) f(v3.e[k], head + (i + 3 * nth) * V + k);
+    for (int k = 0; k < V; ++k) f(v3.e[k], head + (i + 3 * bd) * V + k);
   }
-  for (; i < nv; i += nth) {
+  for (; i < b1; i += bd) {
     Vec16<T> v = xv[i];
 #pragma unroll
     for (int k = 0; k < V; ++k) f(v.e[k], head + i * V + k);
