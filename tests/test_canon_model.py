@@ -22,6 +22,8 @@ def model():
     L = C.CDLL(LIB)
     L.vkm_canonical_fold.restype = C.c_int
     L.vkm_canonical_fold.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]
+    L.vkm_canonical_fold_streamed.restype = C.c_int
+    L.vkm_canonical_fold_streamed.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]
     return L
 
 
@@ -36,11 +38,12 @@ def chains_oracle(vals, acc):
     return [float(np.cumsum(np.concatenate([[acc[j]], vals[j::16]]))[-1]) for j in range(16)]
 
 
-def run(model, vals, acc, lag):
+def run(model, vals, acc, lag, streamed=False):
     vals = np.ascontiguousarray(vals, np.float64)
     a = np.array(acc, np.float64)
     stats = np.zeros(4, np.uint64)
-    model.vkm_canonical_fold(vals.ctypes.data, vals.size // 16, a.ctypes.data, lag, stats.ctypes.data)
+    f = model.vkm_canonical_fold_streamed if streamed else model.vkm_canonical_fold
+    f(vals.ctypes.data, vals.size // 16, a.ctypes.data, lag, stats.ctypes.data)
     return a, stats
 
 
@@ -104,3 +107,36 @@ def test_model_gives_canonical_sum(model):
         s = s + float(v)
     assert bits(s) == bits(O.canonical_sum_f64(x))
     assert stats[0] > 10 * stats[1]  # uniform data takes the fast path almost everywhere
+
+
+def check_bits(name, got, want):
+    for j in range(16):
+        if np.isnan(want[j]):
+            assert np.isnan(got[j]), (name, j)
+        else:
+            assert bits(got[j]) == bits(want[j]), (name, j, got[j], want[j])
+
+
+@pytest.mark.parametrize("group_tiles", [1, 4, 64])
+def test_streamed_model_matches_sequential_chains(model, group_tiles):
+    """The streamed form (prefix hypotheses, clean groups, the walk) on the same data; small
+    groups make joined spans, failed spans and segments all occur on 41 tiles."""
+    st = np.zeros(4, np.uint64)
+    for name, vals, acc in datasets():
+        got, stats = run(model, vals, acc, group_tiles, streamed=True)
+        check_bits(name, got, chains_oracle(vals, acc))
+        st += stats
+    assert st[0] > 0 and st[3] > 0  # joined spans applied, tiles evaluated exactly
+    if group_tiles < 64:
+        assert st[2] > 0  # segments of non-clean groups applied at once
+
+
+def test_streamed_model_long_uniform(model):
+    """Uniform data over 300 tiles: everything in joined clean spans except the tiles where a
+    chain's running sum crosses into the next binade (about 512 per tile, so from 2^9 to 2^17:
+    9 crossings) and the first tile, which starts at zero."""
+    x = O.rand_f64(16 * 1024 * 300, 5)
+    got, stats = run(model, x, np.zeros(16), 64, streamed=True)
+    check_bits("uniform-300", got, chains_oracle(x, np.zeros(16)))
+    assert 0 < stats[3] <= 16 * 11
+    assert stats[0] > 0

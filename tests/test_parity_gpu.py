@@ -587,6 +587,34 @@ def test_many_tiles(D):
     assert D.call("argmax_f64", [dx]).value == int(np.argmax(x))
 
 
+def test_canonical_streamed_long(D):
+    # above 64 tiles of 16K rows the streamed form runs; 80M rows make 4883 tiles, so the
+    # walk over groups of 64 tiles goes past one wave's 64 groups
+    rng = np.random.default_rng(80)
+    n = 80_000_021
+    y = rng.random(n)
+    dy = D.array(y)
+    for name, x in (("uniform", rng.random(n)), ("normal", rng.normal(size=n)),
+                    ("drift-through-zero", np.where(np.arange(n) < 16, 4000.0, rng.random(n) - 0.5001))):
+        dx = D.array(x)
+        assert same_float(O.canonical_sum_f64(x), D.call("reduce_add_f64", [dx]).value), name
+        assert same_float(O.dot_f64(x, y), D.call("dot_f64", [dx, dy]).value), name
+        del dx
+
+
+def test_canonical_streamed_slices(D):
+    # the streamed form continuing a running state, between slices of the single-pass form
+    rng = np.random.default_rng(78)
+    n = 5_000_003
+    x = rng.normal(size=n) * 1e3 + 0.25
+    dx = D.array(x)
+    r = D.reducer("reduce_add_f64")
+    cuts = [0, 16 * 40000, 16 * 40000 + 16 * 70000, 16 * 300000, n]
+    for a, b in zip(cuts, cuts[1:]):
+        r.update([dx.slice(a, b - a)])
+    assert same_float(O.canonical_sum_f64(x), r.finish().value)
+
+
 def test_canonical_fast_path_tiles(D):
     rng = np.random.default_rng(77)
     n = 300_000 + 5

@@ -101,29 +101,61 @@ __host__ __device__ __forceinline__ BinadeRef binade_ref(const Binade &h) {
   return BinadeRef{z0, z0 + u, pow2(52 - h.e), 0.5 * two_e};
 }
 
-// Appends one element v to the run summary f.
-__host__ __device__ __forceinline__ void run_push(RunFn &f, const BinadeRef &r, double v) {
-  const double a0 = ((r.z0 + v) - r.z0) * r.inv_u;
-  const double a1 = ((r.z1 + v) - r.z1) * r.inv_u;
+// A run summary being built: the summary so far, the current parity for each start parity
+// (p xor parity(offset), tracked as bits from the increments' parities so that the chain from
+// one element to the next is a single add), and whether an element the binade cannot take
+// (too large, NaN, infinite) has been seen. The parity bits equal those of the offsets while
+// the offsets stay exact, which every summary that passes a check does.
+struct RunAcc {
+  RunFn f;
+  bool q[2];
+  bool bad;
+};
+
+__host__ __device__ __forceinline__ RunAcc run_start() { return RunAcc{run_identity(), {false, true}, false}; }
+
+__host__ __device__ __forceinline__ void run_push(RunAcc &a, const BinadeRef &r, double v) {
+  const double a0 = ((r.z0 + v) - r.z0) * r.inv_u;  // rounding from an even value
+  const double a1 = ((r.z1 + v) - r.z1) * r.inv_u;  // from an odd one
+  const bool p0 = parity(a0) != 0, p1 = parity(a1) != 0;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const bool q = a.q[p];
+    a.f.d[p] = a.f.d[p] + (q ? a1 : a0);
+    a.q[p] = q ? !p1 : p0;
+    // bounds are never NaN here (they start at +-inf and take minNum/maxNum with the offsets)
+    a.f.lo[p] = __builtin_fmin(a.f.lo[p], a.f.d[p]);
+    a.f.hi[p] = __builtin_fmax(a.f.hi[p], a.f.d[p]);
+  }
   const double av = v < 0 ? -v : v;
-  const double inf = __builtin_inf();
-  // track p: the current parity is p xor parity(offset)
-  {
-    const double d = parity(f.d[0]) ? a1 : a0;
-    f.d[0] = f.d[0] + d;
-    f.lo[0] = f.d[0] < f.lo[0] ? f.d[0] : f.lo[0];
-    f.hi[0] = f.d[0] > f.hi[0] ? f.d[0] : f.hi[0];
-  }
-  {
-    const double d = parity(f.d[1]) ? a0 : a1;
-    f.d[1] = f.d[1] + d;
-    f.lo[1] = f.d[1] < f.lo[1] ? f.d[1] : f.lo[1];
-    f.hi[1] = f.d[1] > f.hi[1] ? f.d[1] : f.hi[1];
-  }
-  if (!(av < r.lim)) {
+  a.bad = a.bad || !(av < r.lim);
+}
+
+// The finished summary: a run with a bad element gets bounds [-inf, +inf], which no start
+// value passes.
+__host__ __device__ __forceinline__ RunFn run_end(const RunAcc &a) {
+  RunFn f = a.f;
+  if (a.bad) {
+    const double inf = __builtin_inf();
     f.lo[0] = f.lo[1] = -inf;
     f.hi[0] = f.hi[1] = inf;
   }
+  return f;
+}
+
+// The summary of the first cnt (at most N) of the elements v.
+template <int N>
+__host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, const BinadeRef &r) {
+  RunAcc a = run_start();
+  if (cnt >= N) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) run_push(a, r, v[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      if (k < cnt) run_push(a, r, v[k]);
+  }
+  return run_end(a);
 }
 
 // The summary of run a followed by run b (both under the same hypothesis).

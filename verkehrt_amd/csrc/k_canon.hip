@@ -14,16 +14,25 @@
 namespace vk {
 namespace {
 
-struct RowVal {
-  int kind;  // 0 sum, 1 dot, 2 sumsq, 3 condsum, 4 mul
-  int cmp;
+// Row i's value in a canonical reduction of kind K: x (0: sums, and the factors of products),
+// x * y (1: dot), x * x (2: sumsq), x where its comparison with c holds, else e (3: condsum).
+// K is a template argument so that row loads are plain, independent loads.
+template <int K>
+struct Row {
   double c, e;
+  int keep;  // condsum: the outcomes of comparing x with c that keep x (bit 0 <, 1 ==, 2 >, 3 unordered)
   __device__ __forceinline__ double operator()(const double *x, const double *y, uint64_t i) const {
-    switch (kind) {
-      case 1: return x[i] * y[i];
-      case 2: return x[i] * x[i];
-      case 3: { double v = x[i]; return cmp_test(cmp, v, c) ? v : e; }
-      default: return x[i];
+    if constexpr (K == 1) {
+      return x[i] * y[i];
+    } else if constexpr (K == 2) {
+      const double v = x[i];
+      return v * v;
+    } else if constexpr (K == 3) {
+      const double v = x[i];
+      const int o = v < c ? 1 : v == c ? 2 : v > c ? 4 : 8;
+      return (keep & o) ? v : e;
+    } else {
+      return x[i];
     }
   }
 };
@@ -31,7 +40,8 @@ struct RowVal {
 constexpr int kGroup = 8;  // 64-row chunks loaded ahead of the dependent chain
 
 // acc[16] <- fold of rows [0, 16*blocks) in the canonical order.
-__global__ __launch_bounds__(64) void k_canon_chain(RowVal rv, bool mul, const double *__restrict__ x,
+template <class RV>
+__global__ __launch_bounds__(64) void k_canon_chain(RV rv, bool mul, const double *__restrict__ x,
                                                      const double *__restrict__ y, uint64_t blocks,
                                                      double *acc) {
   const int l = lane_id();
@@ -88,6 +98,7 @@ constexpr int kFastWaves = kFastThreads / kWave;
 constexpr int kRunBlocks = 16;
 constexpr int kTileBlocks = kFastThreads / 16 * kRunBlocks;  // 1024 blocks
 constexpr uint64_t kFastMinBlocks = kTileBlocks;
+constexpr uint64_t kStreamMinTiles = 64;  // from 1M rows the streamed form (below) is used
 // Status words per tile: [0] flags; approximate aggregate and inclusive sums per chain; the
 // tile summary per chain (6 words, the binade word at +7); exact end value per chain, and per
 // chain a flag word (epoch << 8 | 1) once that value is published. Exact values are published
@@ -163,7 +174,8 @@ __device__ __forceinline__ Binade unpack_binade(uint64_t w) {
 // A check fails only where the value leaves its binade or meets a huge, infinite or NaN
 // element, and a failed check is always followed by the element-by-element fold, so the
 // result is the sequential one either way.
-__device__ double eval_chain(const RowVal &rv, const double *x, const double *y, uint64_t b0, int nb,
+template <class RV>
+__device__ double eval_chain(const RV &rv, const double *x, const double *y, uint64_t b0, int nb,
                              int c, double s) {
   const int l = lane_id();
   double v[kRunBlocks];
@@ -190,12 +202,7 @@ __device__ double eval_chain(const RowVal &rv, const double *x, const double *y,
     for (int r0 = 0; r0 < nruns;) {
       const Binade h = binade_of(cur);
       RunFn f = run_identity();
-      if (h.ok && l >= r0) {
-        const BinadeRef ref = binade_ref(h);
-#pragma unroll
-        for (int k = 0; k < kRunBlocks; ++k)
-          if (l * kRunBlocks + k < nb) run_push(f, ref, v[k]);
-      }
+      if (h.ok && l >= r0) f = run_of(v, nb - l * kRunBlocks, binade_ref(h));
 #pragma unroll
       for (int d = 1; d < kWave; d <<= 1) {
         const RunFn o = shfl_up_pod(f, d);
@@ -221,12 +228,7 @@ __device__ double eval_chain(const RowVal &rv, const double *x, const double *y,
     return cur;
   }
   RunFn f = run_identity();
-  if (ha.ok) {
-    const BinadeRef ref = binade_ref(ha);
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k)
-      if (l * kRunBlocks + k < nb) run_push(f, ref, v[k]);
-  }
+  if (ha.ok) f = run_of(v, nb - l * kRunBlocks, binade_ref(ha));
   for (int r = 0; r < nruns; ++r) {
     RunFn fr;
     fr.d[0] = __shfl(f.d[0], r);
@@ -268,8 +270,8 @@ __device__ __forceinline__ RunFn join_slots(RunFn f, int l) {
 // (t % 64) / 16 and chain t % 16, so one round reads 64 predecessors in parallel; a
 // block-wide minimum finds the nearest tile that ends the look-back (per chain for the exact
 // one), and the slots before it are summed (approximate) or joined in order (exact).
-template <bool BATCH>
-__global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const double *__restrict__ x,
+template <class RV, bool BATCH>
+__global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double *__restrict__ x,
                                                              const double *__restrict__ y, uint64_t blocks,
                                                              const double *acc_in, double *acc_out,
                                                              uint64_t *status, uint32_t epoch,
@@ -381,12 +383,7 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const do
   // 3. summarise the run under the chain's hypothesis, join the runs of the wave in order
   const Binade h = s_h[j];
   RunFn f = run_identity();
-  if (h.ok) {
-    const BinadeRef ref = binade_ref(h);
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k)
-      if (r * kRunBlocks + k < nb) run_push(f, ref, v[k]);
-  }
+  if (h.ok) f = run_of(v, nb - r * kRunBlocks, binade_ref(h));
   {
     RunFn o = shfl_xor_pod(f, 16);
     f = (l & 16) ? run_join(o, f) : run_join(f, o);
@@ -536,7 +533,350 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RowVal rv, const do
   if (l == 0) st_rlx(st, flag_word(epoch, F_AAGG | F_AINC | F_FN | F_EXACT));
 }
 
-__global__ void k_canon_rest(RowVal rv, const double *x, const double *y, uint64_t rest,
+// ---------------------------------------------------------------- streamed form
+// For long inputs the same exact evaluation in five plain launches, with no look-back:
+//  1. k_canon_asum: approximate per-chain sums of every tile (a streaming read);
+//  2. k_canon_prefix: the exclusive prefix of those sums per chain, from the start state: each
+//     tile's approximate start, so its binade hypothesis (any order of these approximate sums
+//     serves: the applies below prove every result);
+//  3. k_canon_summ: every tile's run summaries under its hypothesis, joined into a tile
+//     summary, marked usable when it applies at the approximate start (a streaming read);
+//  4. k_canon_groups: the ordered join of each group of 64 tiles whose summaries are all usable
+//     under one hypothesis ("clean" groups);
+//  5. k_canon_walk: one wave per chain walks the groups in order from the exact start value:
+//     consecutive clean groups with one hypothesis join, and their joined summary is applied
+//     once; other groups are split into segments of usable same-hypothesis tiles, each applied
+//     once; a summary that fails is retried tile by tile, and a tile that fails is evaluated
+//     exactly by the wave (eval_chain).
+// The applies prove each result equal to the sequential fold, as in the single-pass form.
+constexpr int kGroupTiles = 64;
+constexpr uint64_t kUsableBit = uint64_t(1) << 16;
+
+struct TileSum {
+  double d0, d1, lo0, lo1, hi0, hi1;
+  uint64_t hword;  // pack_binade(h) | kUsableBit when the summary applies at the approximate start
+  uint64_t pad;
+};
+struct GroupSum {
+  double d0, d1, lo0, lo1, hi0, hi1;
+  uint64_t hword;  // the group's hypothesis
+  uint64_t clean;  // every tile usable under that hypothesis
+};
+
+__device__ __forceinline__ RunFn fn_of(double d0, double d1, double lo0, double lo1, double hi0, double hi1) {
+  RunFn f;
+  f.d[0] = d0;
+  f.d[1] = d1;
+  f.lo[0] = lo0;
+  f.lo[1] = lo1;
+  f.hi[0] = hi0;
+  f.hi[1] = hi1;
+  return f;
+}
+
+// 1. 256 threads per tile; thread t folds rows t, t + 256, ... of the tile, all of chain t % 16.
+//    Sums are stored chain-major: asum[j * tiles + tile].
+template <class RV>
+__global__ __launch_bounds__(kBlock) void k_canon_asum(RV rv, const double *__restrict__ x,
+                                                       const double *__restrict__ y, uint64_t blocks,
+                                                       uint64_t tiles, double *asum) {
+  __shared__ double s_a[kBlock / kWave][16];
+  const uint64_t tile = blockIdx.x;
+  const uint64_t b0 = tile * kTileBlocks;
+  const uint64_t nb = blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks;
+  const uint64_t r0 = b0 * 16, rows = nb * 16;
+  const int t = threadIdx.x, l = lane_id();
+  double a = 0.0;
+  uint64_t i = t;
+  for (; i + 7 * kBlock < rows; i += 8 * kBlock) {
+    double u[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u[k] = rv(x, y, r0 + i + k * kBlock);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a = a + u[k];
+  }
+  for (; i < rows; i += kBlock) a = a + rv(x, y, r0 + i);
+  a = a + __shfl_xor(a, 16);
+  a = a + __shfl_xor(a, 32);
+  if (l < 16) s_a[t / kWave][l] = a;
+  __syncthreads();
+  if (t < 16) asum[t * tiles + tile] = s_a[0][t] + s_a[1][t] + s_a[2][t] + s_a[3][t];
+}
+
+// 2. One block per chain: thread p sums part p of the tiles (ceil(tiles / 1024) each), the part
+//    sums are scanned in LDS (Hillis-Steele), then each thread writes its tiles' approximate
+//    starts, pre[tile * 16 + j].
+constexpr int kPrefixParts = 1024;
+__global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asum, uint64_t tiles,
+                                                               const double *acc_in, double *pre) {
+  __shared__ double s_part[kPrefixParts];
+  const int j = blockIdx.x, p = threadIdx.x;
+  const double *a = asum + uint64_t(j) * tiles;
+  const uint64_t per = (tiles + kPrefixParts - 1) / kPrefixParts;
+  const uint64_t t0 = p * per < tiles ? p * per : tiles;
+  const uint64_t t1 = t0 + per < tiles ? t0 + per : tiles;
+  double sum = 0.0;
+  uint64_t k = t0;
+  for (; k + 8 <= t1; k += 8) {
+    double u[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) u[m] = a[k + m];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) sum = sum + u[m];
+  }
+  for (; k < t1; ++k) sum = sum + a[k];
+  s_part[p] = sum;
+  __syncthreads();
+  for (int d = 1; d < kPrefixParts; d <<= 1) {
+    const double o = p >= d ? s_part[p - d] : 0.0;
+    __syncthreads();
+    if (p >= d) s_part[p] = s_part[p] + o;
+    __syncthreads();
+  }
+  double run = acc_in[j] + (p > 0 ? s_part[p - 1] : 0.0);
+  k = t0;
+  for (; k + 8 <= t1; k += 8) {
+    double u[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) u[m] = a[k + m];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      pre[(k + m) * 16 + j] = run;
+      run = run + u[m];
+    }
+  }
+  for (; k < t1; ++k) {
+    pre[k * 16 + j] = run;
+    run = run + a[k];
+  }
+}
+
+// 3. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the start.
+template <class RV>
+__global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double *__restrict__ x,
+                                                             const double *__restrict__ y, uint64_t blocks,
+                                                             const double *pre, uint64_t tiles,
+                                                             TileSum *sums) {
+  __shared__ RunFn s_f[kFastWaves][16];
+  const uint64_t tile = blockIdx.x;
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  const int j = t & 15, r = t >> 4;
+  const uint64_t b0 = tile * kTileBlocks;
+  const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
+  const double start = pre[tile * 16 + j];
+  const Binade h = binade_of(start);
+  RunFn f = run_identity();
+  if (h.ok) {
+    double v[kRunBlocks];
+    const int cnt = nb - r * kRunBlocks;
+    if (cnt >= kRunBlocks) {
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j) : 0.0;
+    }
+    f = run_of(v, cnt, binade_ref(h));
+  }
+  {
+    RunFn o = shfl_xor_pod(f, 16);
+    f = (l & 16) ? run_join(o, f) : run_join(f, o);
+    o = shfl_xor_pod(f, 32);
+    f = (l & 32) ? run_join(o, f) : run_join(f, o);
+  }
+  if (l < 16) s_f[w][j] = f;
+  __syncthreads();
+  if (w != 0) return;
+  const int q = l >> 4;
+  RunFn ft = run_join(run_join(s_f[4 * q][j], s_f[4 * q + 1][j]), run_join(s_f[4 * q + 2][j], s_f[4 * q + 3][j]));
+  RunFn o = shfl_xor_pod(ft, 16);
+  ft = (l & 16) ? run_join(o, ft) : run_join(ft, o);
+  o = shfl_xor_pod(ft, 32);
+  ft = (l & 32) ? run_join(o, ft) : run_join(ft, o);
+  if (l < 16) {
+    double probe;
+    const bool usable = run_apply(ft, h, start, &probe);
+    TileSum ts{ft.d[0], ft.d[1], ft.lo[0], ft.lo[1], ft.hi[0], ft.hi[1],
+               pack_binade(h) | (usable ? kUsableBit : 0), 0};
+    sums[uint64_t(l) * tiles + tile] = ts;
+  }
+}
+
+// Ordered join of the 64 lanes' summaries (lane order is tile order); every lane gets it.
+__device__ __forceinline__ RunFn join_lanes(RunFn f, int l) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    const RunFn o = shfl_xor_pod(f, m);
+    f = (l & m) ? run_join(o, f) : run_join(f, o);
+  }
+  return f;
+}
+
+// 4. One wave per (chain, group of 64 tiles).
+__global__ __launch_bounds__(kBlock) void k_canon_groups(const TileSum *sums, uint64_t tiles, uint64_t groups,
+                                                         GroupSum *gsum) {
+  const uint64_t wid = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
+  if (wid >= 16 * groups) return;
+  const int l = lane_id();
+  const uint64_t j = wid / groups, g = wid % groups;
+  const uint64_t tile = g * kGroupTiles + l;
+  const bool valid = tile < tiles;
+  RunFn f = run_identity();
+  uint64_t hw = 0;
+  if (valid) {
+    const TileSum ts = sums[j * tiles + tile];
+    f = fn_of(ts.d0, ts.d1, ts.lo0, ts.lo1, ts.hi0, ts.hi1);
+    hw = ts.hword;
+  }
+  const uint64_t h0 = __shfl(hw, 0);
+  const bool ok = !valid || (hw == h0 && (hw & kUsableBit));
+  const bool clean = __ballot(!ok) == 0;
+  f = join_lanes(f, l);
+  if (l == 0) {
+    GroupSum gs{f.d[0], f.d[1], f.lo[0], f.lo[1], f.hi[0], f.hi[1], h0 & ~kUsableBit, clean ? 1ull : 0ull};
+    gsum[j * groups + g] = gs;
+  }
+}
+
+// The tiles of group g from the exact value cur, by segments of usable same-hypothesis tiles
+// (each applied once, or tile by tile when its joined summary fails); `tiles_only` skips the
+// segment attempt.
+template <class RV>
+__device__ double walk_group(const RV &rv, const double *x, const double *y, uint64_t blocks,
+                             const TileSum *sums, uint64_t tiles, int j, uint64_t g, double cur,
+                             bool tiles_only) {
+  const int l = lane_id();
+  const uint64_t tile = g * kGroupTiles + l;
+  const bool valid = tile < tiles;
+  RunFn f = run_identity();
+  uint64_t hw = 0;
+  if (valid) {
+    const TileSum ts = sums[uint64_t(j) * tiles + tile];
+    f = fn_of(ts.d0, ts.d1, ts.lo0, ts.lo1, ts.hi0, ts.hi1);
+    hw = ts.hword;
+  }
+  const bool usable = valid && (hw & kUsableBit);
+  const uint64_t hprev = __shfl_up(hw, 1);
+  const bool uprev = __shfl_up(static_cast<int>(usable), 1) != 0;
+  // a segment starts at lane 0, at an unusable tile, after one, and where the hypothesis changes
+  const bool start = l == 0 || !usable || !uprev || hw != hprev || tiles_only;
+  // segmented inclusive scan: (f, s) then (f', s') = s' ? (f', 1) : (f . f', s)
+  RunFn sc = f;
+  int ss = start ? 1 : 0;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const RunFn o = shfl_up_pod(sc, d);
+    const int os = __shfl_up(ss, d);
+    if (l >= d && !ss) {
+      sc = run_join(o, sc);
+      ss = os;
+    }
+  }
+  const uint64_t vmask = __ballot(valid);
+  const uint64_t starts = __ballot(start && valid);
+  const int nvalid = __popcll(vmask);
+  // segment ends: the lane before each start, and the last valid lane
+  uint64_t ends = (starts >> 1) | (uint64_t(1) << (nvalid - 1));
+  ends &= vmask;
+  int first = 0;
+  while (ends) {
+    const int e = __builtin_ctzll(ends);
+    ends &= ends - 1;
+    const uint64_t hw_e = __shfl(hw, e);
+    const int us_e = __shfl(static_cast<int>(usable), e);
+    RunFn tot;
+    tot.d[0] = __shfl(sc.d[0], e);
+    tot.d[1] = __shfl(sc.d[1], e);
+    tot.lo[0] = __shfl(sc.lo[0], e);
+    tot.lo[1] = __shfl(sc.lo[1], e);
+    tot.hi[0] = __shfl(sc.hi[0], e);
+    tot.hi[1] = __shfl(sc.hi[1], e);
+    double out;
+    if (us_e && run_apply(tot, unpack_binade(hw_e & ~kUsableBit), cur, &out)) {
+      cur = out;
+    } else {
+      // tile by tile through the segment first..e
+      for (int r = first; r <= e; ++r) {
+        const uint64_t hw_r = __shfl(hw, r);
+        RunFn fr;
+        fr.d[0] = __shfl(f.d[0], r);
+        fr.d[1] = __shfl(f.d[1], r);
+        fr.lo[0] = __shfl(f.lo[0], r);
+        fr.lo[1] = __shfl(f.lo[1], r);
+        fr.hi[0] = __shfl(f.hi[0], r);
+        fr.hi[1] = __shfl(f.hi[1], r);
+        if (!((hw_r & kUsableBit) && run_apply(fr, unpack_binade(hw_r & ~kUsableBit), cur, &out))) {
+          const uint64_t tb = (g * kGroupTiles + r) * kTileBlocks;
+          const int nb = static_cast<int>(blocks - tb < uint64_t(kTileBlocks) ? blocks - tb : kTileBlocks);
+          out = eval_chain(rv, x, y, tb, nb, j, cur);
+        }
+        cur = out;
+      }
+    }
+    first = e + 1;
+  }
+  return cur;
+}
+
+// 5. One wave per chain.
+template <class RV>
+__global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__restrict__ x,
+                                                      const double *__restrict__ y, uint64_t blocks,
+                                                      const TileSum *sums, uint64_t tiles,
+                                                      const GroupSum *gsum, uint64_t groups, double *acc) {
+  const int j = blockIdx.x, l = lane_id();
+  double cur = acc[j];
+  RunFn pend = run_identity();  // joined clean groups since `cur`
+  uint64_t pend_h = 0, pend_g0 = 0;
+  bool pending = false;
+  auto flush = [&](uint64_t g_end) {
+    double out;
+    if (run_apply(pend, unpack_binade(pend_h), cur, &out)) {
+      cur = out;
+    } else {
+      for (uint64_t gg = pend_g0; gg < g_end; ++gg)
+        cur = walk_group(rv, x, y, blocks, sums, tiles, j, gg, cur, true);
+    }
+    pending = false;
+  };
+  for (uint64_t g0 = 0; g0 < groups; g0 += kWave) {
+    // lane l holds group g0 + l
+    GroupSum mine{};
+    if (g0 + l < groups) mine = gsum[uint64_t(j) * groups + g0 + l];
+    const int cnt = groups - g0 < uint64_t(kWave) ? static_cast<int>(groups - g0) : kWave;
+    for (int k = 0; k < cnt; ++k) {
+      const uint64_t g = g0 + k;
+      const uint64_t clean = __shfl(mine.clean, k);
+      const uint64_t hw = __shfl(mine.hword, k);
+      if (clean) {
+        RunFn gf;
+        gf.d[0] = __shfl(mine.d0, k);
+        gf.d[1] = __shfl(mine.d1, k);
+        gf.lo[0] = __shfl(mine.lo0, k);
+        gf.lo[1] = __shfl(mine.lo1, k);
+        gf.hi[0] = __shfl(mine.hi0, k);
+        gf.hi[1] = __shfl(mine.hi1, k);
+        if (pending && hw == pend_h) {
+          pend = run_join(pend, gf);
+        } else {
+          if (pending) flush(g);
+          pend = gf;
+          pend_h = hw;
+          pend_g0 = g;
+          pending = true;
+        }
+      } else {
+        if (pending) flush(g);
+        cur = walk_group(rv, x, y, blocks, sums, tiles, j, g, cur, false);
+      }
+    }
+  }
+  if (pending) flush(groups);
+  if (l == 0) acc[j] = cur;
+}
+
+template <class RV>
+__global__ void k_canon_rest(RV rv, const double *x, const double *y, uint64_t rest,
                              double *tail) {
   const uint64_t i = threadIdx.x;
   if (i < rest) tail[i] = rv(x, y, i);
@@ -563,14 +903,14 @@ __global__ void k_fill_d(double *p, uint64_t n, double v) {
 
 // One thread per item: finish_lanes (reduce.rs:75-91) over the item's accumulators and its
 // tail rows 16 * (n / 16) .. n.
-__global__ void k_canon_batch_finish(RowVal rv, const BatchItem *items, uint64_t nitems,
+template <class RV>
+__global__ void k_canon_batch_finish(RV rv, bool mul, const BatchItem *items, uint64_t nitems,
                                      const double *acc, double *out) {
   const uint64_t it = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (it >= nitems) return;
   const BatchItem &I = items[it];
   const double *x = static_cast<const double *>(I.a);
   const double *y = I.b ? static_cast<const double *>(I.b) : x;
-  const bool mul = rv.kind == 4;
   double a[16];
   for (int j = 0; j < 16; ++j) a[j] = acc[it * 16 + j];
   for (int step = 8; step > 0; step /= 2)
@@ -584,6 +924,62 @@ __global__ void k_canon_init(double *acc, double id) {
   if (threadIdx.x < 16) acc[threadIdx.x] = id;
 }
 
+// Calls f with the Row of the spec's kind (products read plain rows).
+template <class F>
+void with_row(const CanonSpec &s, F &&f) {
+  // comparison codes 0 eq, 1 ne, 2 lt, 3 le, 4 gt, 5 ge as the outcome sets they accept
+  static const int keep[6] = {2, 1 | 4 | 8, 1, 1 | 2, 4, 4 | 2};
+  switch (s.kind) {
+    case 1: f(Row<1>{s.c, s.e, 0}); break;
+    case 2: f(Row<2>{s.c, s.e, 0}); break;
+    case 3: f(Row<3>{s.c, s.e, keep[s.cmp >= 0 && s.cmp < 6 ? s.cmp : 5]}); break;
+    default: f(Row<0>{s.c, s.e, 0}); break;
+  }
+}
+
+}  // namespace
+
+namespace {
+
+template <class RV>
+void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, uint64_t blocks,
+                     double *acc) {
+  if (mul || blocks < kFastMinBlocks) {
+    // products have no binade form; short inputs are cheaper as one sequential wave
+    k_canon_chain<<<1, 64, 0, c.stream>>>(rv, mul, x, y, blocks, acc);
+    check_hip(hipGetLastError(), "canon chain");
+    return;
+  }
+  const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
+  if (tiles >= kStreamMinTiles) {
+    const uint64_t groups = (tiles + kGroupTiles - 1) / kGroupTiles;
+    double *asum = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
+    double *pre = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
+    auto *sums = static_cast<TileSum *>(tmp_alloc(c, tiles * 16 * sizeof(TileSum)));
+    auto *gsum = static_cast<GroupSum *>(tmp_alloc(c, groups * 16 * sizeof(GroupSum)));
+    k_canon_asum<RV><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y, blocks, tiles, asum);
+    k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
+    k_canon_summ<RV><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums);
+    k_canon_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(sums, tiles, groups,
+                                                                                              gsum);
+    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, gsum, groups, acc);
+    check_hip(hipGetLastError(), "canon streamed");
+    tmp_free(c, asum);
+    tmp_free(c, pre);
+    tmp_free(c, sums);
+    tmp_free(c, gsum);
+    return;
+  }
+  uint64_t *status = status_words(c, ST_CANON, tiles * kStride);
+  const uint32_t epoch = next_epoch(c);
+  double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
+  copy_d2d(c, acc_in, acc, 16 * 8);
+  k_canon_fast<RV, false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, acc_in, acc,
+                                                               status, epoch, c.scratch.counters,
+                                                               uint32_t(tiles), nullptr, nullptr);
+  check_hip(hipGetLastError(), "canon fast");
+}
+
 }  // namespace
 
 void canon_init(Ctx &c, bool mul, double *acc) {
@@ -594,22 +990,7 @@ void canon_init(Ctx &c, bool mul, double *acc) {
 void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t blocks,
                   double *acc) {
   if (blocks == 0) return;
-  RowVal rv{s.kind, s.cmp, s.c, s.e};
-  if (s.kind == 4 || blocks < kFastMinBlocks) {
-    // products have no binade form; short inputs are cheaper as one sequential wave
-    k_canon_chain<<<1, 64, 0, c.stream>>>(rv, s.kind == 4, x, y ? y : x, blocks, acc);
-    check_hip(hipGetLastError(), "canon chain");
-    return;
-  }
-  const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
-  uint64_t *status = status_words(c, ST_CANON, tiles * kStride);
-  const uint32_t epoch = next_epoch(c);
-  double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
-  copy_d2d(c, acc_in, acc, 16 * 8);
-  k_canon_fast<false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y ? y : x, blocks, acc_in, acc,
-                                                               status, epoch, c.scratch.counters,
-                                                               uint32_t(tiles), nullptr, nullptr);
-  check_hip(hipGetLastError(), "canon fast");
+  with_row(s, [&](auto rv) { canon_blocks_as(c, rv, s.kind == 4, x, y ? y : x, blocks, acc); });
 }
 
 // Batched canonical sums: every item's 16 chains start at the identity; a finish kernel
@@ -617,7 +998,6 @@ void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, 
 void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, double *d_out) {
   const uint64_t nitems = items.size();
   if (nitems == 0) return;
-  RowVal rv{s.kind, s.cmp, s.c, s.e};
   double *acc = static_cast<double *>(tmp_alloc(c, nitems * 16 * 8));
   double *zero = static_cast<double *>(tmp_alloc(c, nitems * 16 * 8));
   const double id = s.kind == 4 ? 1.0 : 0.0;
@@ -630,13 +1010,17 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
     if (s.kind == 4) usage_error("batched products are not supported");
     uint64_t *status = status_words(c, ST_CANON, uint64_t(p.ntiles) * kStride);
     const uint32_t epoch = next_epoch(c);
-    k_canon_fast<true><<<p.ntiles, kFastThreads, 0, c.stream>>>(rv, nullptr, nullptr, 0, zero, acc, status, epoch,
-                                                         c.scratch.counters, p.ntiles, p.items, p.tile_item);
+    with_row(s, [&](auto rv) {
+      k_canon_fast<decltype(rv), true><<<p.ntiles, kFastThreads, 0, c.stream>>>(
+          rv, nullptr, nullptr, 0, zero, acc, status, epoch, c.scratch.counters, p.ntiles, p.items, p.tile_item);
+    });
     check_hip(hipGetLastError(), "canon batch");
   }
   BatchPlan q = batch_plan(c, items, ~uint64_t(0) >> 1);  // the item table (full rows), for the tails
-  k_canon_batch_finish<<<stream_grid(nitems, kBlock, 4096), kBlock, 0, c.stream>>>(rv, q.items, nitems, acc,
-                                                                                  d_out);
+  with_row(s, [&](auto rv) {
+    k_canon_batch_finish<<<stream_grid(nitems, kBlock, 4096), kBlock, 0, c.stream>>>(rv, s.kind == 4, q.items,
+                                                                                    nitems, acc, d_out);
+  });
   check_hip(hipGetLastError(), "canon batch finish");
   batch_plan_free(c, p);
   batch_plan_free(c, q);
@@ -647,8 +1031,7 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
 void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t rest,
                 double *tail) {
   if (rest == 0) return;
-  RowVal rv{s.kind, s.cmp, s.c, s.e};
-  k_canon_rest<<<1, 64, 0, c.stream>>>(rv, x, y ? y : x, rest, tail);
+  with_row(s, [&](auto rv) { k_canon_rest<<<1, 64, 0, c.stream>>>(rv, x, y ? y : x, rest, tail); });
   check_hip(hipGetLastError(), "canon rest");
 }
 

@@ -1,6 +1,7 @@
-// Host-side model of the tiled exact canonical fold (canon_binade.h, k_canon.hip), for CPU
-// verification of its exactness against the sequential definition. It takes the decisions of
-// the device kernel one tile after another, for one chain at a time:
+// Host-side models of the tiled exact canonical fold (canon_binade.h, k_canon.hip), for CPU
+// verification of their exactness against the sequential definition.
+//
+// vkm_canonical_fold follows the single-pass kernel, one tile after another, one chain at a time:
 //  - an approximate running sum per chain picks the tile's binade hypothesis;
 //  - each run of 16 blocks is summarised under it, and the runs join into the tile summary;
 //  - a tile whose summary fails at its own approximate start publishes it as opaque;
@@ -37,7 +38,9 @@ double eval_chain(const double *vals, uint64_t b0, uint64_t b1, int j, double s,
     RunFn f = run_identity();
     if (h.ok) {
       const BinadeRef ref = binade_ref(h);
-      for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b) run_push(f, ref, vals[b * 16 + j]);
+      RunAcc a = run_start();
+      for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b) run_push(a, ref, vals[b * 16 + j]);
+      f = run_end(a);
     }
     return f;
   };
@@ -122,9 +125,9 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
       if (h.ok) {
         const BinadeRef ref = binade_ref(h);
         for (uint64_t rb = b0; rb < b1; rb += kRun) {
-          RunFn f = run_identity();
-          for (uint64_t b = rb; b < std::min(b1, rb + kRun); ++b) run_push(f, ref, vals[b * 16 + j]);
-          ft = run_join(ft, f);
+          RunAcc a = run_start();
+          for (uint64_t b = rb; b < std::min(b1, rb + kRun); ++b) run_push(a, ref, vals[b * 16 + j]);
+          ft = run_join(ft, run_end(a));
         }
       }
       hyp[t] = h;
@@ -170,5 +173,154 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
     stats[2] = rounds;
     stats[3] = mismatch;
   }
+  return 0;
+}
+
+// The streamed form of k_canon.hip for long inputs, one chain at a time: approximate tile sums
+// (summed in the device's order), their exclusive prefix from the start state (in 1024 parts,
+// as the prefix kernel does), tile summaries under the binade of that prefix, usable
+// where they apply at it; groups of `group_tiles` tiles are clean when every tile is usable
+// under one hypothesis; the walk joins consecutive clean groups of one hypothesis and applies
+// the joined summary to the exact value (group by group, tile by tile where that fails), and
+// splits other groups into segments of usable same-hypothesis tiles, each applied once or
+// tile by tile, a tile that fails evaluated by the fallback wave.
+// stats: [0] joined clean spans applied, [1] spans that failed, [2] segments applied,
+// [3] tiles evaluated by the fallback.
+extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, double *acc,
+                                           int group_tiles, uint64_t *stats) {
+  const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
+  if (group_tiles < 1) group_tiles = 1;
+  const uint64_t gt = static_cast<uint64_t>(group_tiles);
+  const uint64_t groups = (tiles + gt - 1) / gt;
+  uint64_t st[4] = {0, 0, 0, 0}, rounds = 0;
+  for (int j = 0; j < 16; ++j) {
+    // 1. approximate tile sums: 16 partial sums of blocks b0 + p + 16 m, combined as the
+    //    device's lanes (xor 16, xor 32) and waves are
+    std::vector<double> asum(tiles);
+    for (uint64_t t = 0; t < tiles; ++t) {
+      const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
+      double part[16];
+      for (int p = 0; p < 16; ++p) {
+        double a = 0.0;
+        for (uint64_t b = b0 + p; b < b1; b += 16) a = a + vals[b * 16 + j];
+        part[p] = a;
+      }
+      double w[4];
+      for (int q = 0; q < 4; ++q) {
+        const double x0 = part[4 * q] + part[4 * q + 1], x1 = part[4 * q + 2] + part[4 * q + 3];
+        w[q] = x0 + x1;
+      }
+      asum[t] = w[0] + w[1] + w[2] + w[3];
+    }
+    // 2. exclusive prefix in 1024 parts, the part sums scanned as the device's Hillis-Steele
+    std::vector<double> pre(tiles);
+    const uint64_t parts = 1024, per = (tiles + parts - 1) / parts;
+    std::vector<double> psum(parts, 0.0);
+    for (uint64_t p = 0; p < parts; ++p)
+      for (uint64_t k = std::min(tiles, p * per); k < std::min(tiles, (p + 1) * per); ++k) psum[p] = psum[p] + asum[k];
+    for (uint64_t d = 1; d < parts; d <<= 1) {
+      std::vector<double> nxt(psum);
+      for (uint64_t p = d; p < parts; ++p) nxt[p] = psum[p] + psum[p - d];
+      psum = nxt;
+    }
+    for (uint64_t p = 0; p < parts; ++p) {
+      double run = acc[j] + (p > 0 ? psum[p - 1] : 0.0);
+      for (uint64_t k = std::min(tiles, p * per); k < std::min(tiles, (p + 1) * per); ++k) {
+        pre[k] = run;
+        run = run + asum[k];
+      }
+    }
+    // 3. tile summaries
+    std::vector<RunFn> fn(tiles);
+    std::vector<Binade> hyp(tiles);
+    std::vector<char> usable(tiles);
+    for (uint64_t t = 0; t < tiles; ++t) {
+      const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
+      const Binade h = binade_of(pre[t]);
+      RunFn ft = run_identity();
+      if (h.ok) {
+        const BinadeRef ref = binade_ref(h);
+        for (uint64_t rb = b0; rb < b1; rb += kRun) {
+          RunAcc a = run_start();
+          for (uint64_t b = rb; b < std::min(b1, rb + kRun); ++b) run_push(a, ref, vals[b * 16 + j]);
+          ft = run_join(ft, run_end(a));
+        }
+      }
+      double probe;
+      hyp[t] = h;
+      fn[t] = ft;
+      usable[t] = run_apply(ft, h, pre[t], &probe);
+    }
+    auto one = [&](uint64_t t, double cur) {
+      double out;
+      if (usable[t] && run_apply(fn[t], hyp[t], cur, &out)) return out;
+      ++st[3];
+      const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
+      return eval_chain(vals, b0, b1, j, cur, &rounds);
+    };
+    auto walk_group = [&](uint64_t g, double cur, bool tiles_only) {
+      const uint64_t t0 = g * gt, t1 = std::min(tiles, t0 + gt);
+      for (uint64_t s = t0; s < t1;) {
+        uint64_t e = s + 1;
+        if (!tiles_only && usable[s])
+          while (e < t1 && usable[e] && same_binade(hyp[e], hyp[s])) ++e;
+        RunFn f = run_identity();
+        for (uint64_t t = s; t < e; ++t) f = run_join(f, fn[t]);
+        double out;
+        if (usable[s] && run_apply(f, hyp[s], cur, &out)) {
+          ++st[2];
+          cur = out;
+        } else {
+          for (uint64_t t = s; t < e; ++t) cur = one(t, cur);
+        }
+        s = e;
+      }
+      return cur;
+    };
+    // 4-5. clean groups and the walk
+    double cur = acc[j];
+    RunFn pend = run_identity();
+    Binade pend_h{};
+    uint64_t pend_g0 = 0;
+    bool pending = false;
+    auto flush = [&](uint64_t g_end) {
+      double out;
+      if (run_apply(pend, pend_h, cur, &out)) {
+        ++st[0];
+        cur = out;
+      } else {
+        ++st[1];
+        for (uint64_t g = pend_g0; g < g_end; ++g) cur = walk_group(g, cur, true);
+      }
+      pending = false;
+    };
+    for (uint64_t g = 0; g < groups; ++g) {
+      const uint64_t t0 = g * gt, t1 = std::min(tiles, t0 + gt);
+      bool clean = true;
+      RunFn gf = run_identity();
+      for (uint64_t t = t0; t < t1; ++t) {
+        clean = clean && usable[t] && same_binade(hyp[t], hyp[t0]);
+        gf = run_join(gf, fn[t]);
+      }
+      if (clean) {
+        if (pending && same_binade(hyp[t0], pend_h)) {
+          pend = run_join(pend, gf);
+        } else {
+          if (pending) flush(g);
+          pend = gf;
+          pend_h = hyp[t0];
+          pend_g0 = g;
+          pending = true;
+        }
+      } else {
+        if (pending) flush(g);
+        cur = walk_group(g, cur, false);
+      }
+    }
+    if (pending) flush(groups);
+    acc[j] = cur;
+  }
+  if (stats)
+    for (int k = 0; k < 4; ++k) stats[k] = st[k];
   return 0;
 }
