@@ -92,13 +92,14 @@ struct BinadeRef {
   double z0, z1;   // reference points, even and odd multiples of u
   double inv_u;    // 1/u
   double lim;      // elements must satisfy |v| < lim = 2^(e-1)
+  double half_u;   // u / 2
 };
 
 __host__ __device__ __forceinline__ BinadeRef binade_ref(const Binade &h) {
   const double two_e = pow2(h.e);
   const double u = pow2(h.e - 52);
   const double z0 = (h.neg ? -1.5 : 1.5) * two_e;
-  return BinadeRef{z0, z0 + u, pow2(52 - h.e), 0.5 * two_e};
+  return BinadeRef{z0, z0 + u, pow2(52 - h.e), 0.5 * two_e, pow2(h.e - 53)};
 }
 
 // A run summary being built: the summary so far, the current parity for each start parity
@@ -143,18 +144,37 @@ __host__ __device__ __forceinline__ RunFn run_end(const RunAcc &a) {
   return f;
 }
 
-// The summary of the first cnt (at most N) of the elements v.
+// The summary of the first cnt (at most N) of the elements v. Without ties the rounding does
+// not depend on the parity, so both start parities give one offset track: that is computed
+// first, with a tie (the rounding error of Z0 + v is exactly u / 2; Z0 + v rounds to a multiple
+// of u and fl(Z0 + v) - Z0 - v is exact by Fast2Sum, |Z0| > |v|) sending the run to the
+// two-track form. NaN and infinite elements give no tie but make the run bad either way.
 template <int N>
 __host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, const BinadeRef &r) {
-  RunAcc a = run_start();
-  if (cnt >= N) {
+  const double inf = __builtin_inf();
+  double d = 0.0, lo = inf, hi = -inf;
+  bool tie = false, bad = false;
 #pragma unroll
-    for (int k = 0; k < N; ++k) run_push(a, r, v[k]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-      if (k < cnt) run_push(a, r, v[k]);
+  for (int k = 0; k < N; ++k) {
+    if (k < cnt) {
+      const double w = (r.z0 + v[k]) - r.z0;
+      const double e = v[k] - w;
+      tie = tie || (e < 0 ? -e : e) == r.half_u;
+      d = d + w * r.inv_u;
+      lo = __builtin_fmin(lo, d);
+      hi = __builtin_fmax(hi, d);
+      const double av = v[k] < 0 ? -v[k] : v[k];
+      bad = bad || !(av < r.lim);
+    }
   }
+  if (__builtin_expect(!tie, 1)) {
+    RunAcc a{RunFn{{d, d}, {lo, lo}, {hi, hi}}, {false, true}, bad};
+    return run_end(a);
+  }
+  RunAcc a = run_start();
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (k < cnt) run_push(a, r, v[k]);
   return run_end(a);
 }
 

@@ -534,21 +534,27 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double
 }
 
 // ---------------------------------------------------------------- streamed form
-// For long inputs the same exact evaluation in five plain launches, with no look-back:
-//  1. k_canon_asum: approximate per-chain sums of every tile (a streaming read);
-//  2. k_canon_prefix: the exclusive prefix of those sums per chain, from the start state: each
-//     tile's approximate start, so its binade hypothesis (any order of these approximate sums
-//     serves: the applies below prove every result);
-//  3. k_canon_summ: every tile's run summaries under its hypothesis, joined into a tile
-//     summary, marked usable when it applies at the approximate start (a streaming read);
-//  4. k_canon_groups: the ordered join of each group of 64 tiles whose summaries are all usable
+// For long inputs the same exact evaluation in plain launches, with no look-back:
+//  1. k_canon_sample, k_canon_prefix: predicted approximate starts of every tile per chain,
+//     from every 16th block (a sixteenth of the rows) scaled by 16 and prefixed from the start
+//     state; their binades are the first hypotheses;
+//  2. k_canon_summ<false>: every tile's run summaries under its predicted hypothesis, joined
+//     into a tile summary, and the tile's approximate chain sums (one streaming read);
+//  3. k_canon_prefix: the exclusive prefix of those sums from the start state, the approximate
+//     start of every tile (any order of these approximate sums serves: the applies below prove
+//     every result);
+//  4. k_canon_summ<true>: tiles with a chain whose prediction missed the binade of its
+//     approximate start are summarised again (re-read); every summary is marked usable when it
+//     applies at the approximate start;
+//  5. k_canon_groups: the ordered join of each group of 64 tiles whose summaries are all usable
 //     under one hypothesis ("clean" groups);
-//  5. k_canon_walk: one wave per chain walks the groups in order from the exact start value:
+//  6. k_canon_walk: one wave per chain walks the groups in order from the exact start value:
 //     consecutive clean groups with one hypothesis join, and their joined summary is applied
 //     once; other groups are split into segments of usable same-hypothesis tiles, each applied
 //     once; a summary that fails is retried tile by tile, and a tile that fails is evaluated
 //     exactly by the wave (eval_chain).
-// The applies prove each result equal to the sequential fold, as in the single-pass form.
+// The applies prove each result equal to the sequential fold, as in the single-pass form. A
+// wrong prediction costs only the re-read of its tile in step 4.
 constexpr int kGroupTiles = 64;
 constexpr uint64_t kUsableBit = uint64_t(1) << 16;
 
@@ -574,36 +580,32 @@ __device__ __forceinline__ RunFn fn_of(double d0, double d1, double lo0, double 
   return f;
 }
 
-// 1. 256 threads per tile; thread t folds rows t, t + 256, ... of the tile, all of chain t % 16.
-//    Sums are stored chain-major: asum[j * tiles + tile].
+// 1. Predicted chain sums of a tile, asum[j * tiles + tile]: 16 times the sum of the tile's
+//    blocks 0, 16, 32, ...; thread t reads chain t % 16 of blocks 16 (t / 16 + 16 m).
 template <class RV>
-__global__ __launch_bounds__(kBlock) void k_canon_asum(RV rv, const double *__restrict__ x,
-                                                       const double *__restrict__ y, uint64_t blocks,
-                                                       uint64_t tiles, double *asum) {
+__global__ __launch_bounds__(kBlock) void k_canon_sample(RV rv, const double *__restrict__ x,
+                                                         const double *__restrict__ y, uint64_t blocks,
+                                                         uint64_t tiles, double *asum) {
   __shared__ double s_a[kBlock / kWave][16];
   const uint64_t tile = blockIdx.x;
   const uint64_t b0 = tile * kTileBlocks;
   const uint64_t nb = blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks;
-  const uint64_t r0 = b0 * 16, rows = nb * 16;
-  const int t = threadIdx.x, l = lane_id();
-  double a = 0.0;
-  uint64_t i = t;
-  for (; i + 7 * kBlock < rows; i += 8 * kBlock) {
-    double u[8];
+  const int t = threadIdx.x, l = lane_id(), j = t & 15;
+  double u[4];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) u[k] = rv(x, y, r0 + i + k * kBlock);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) a = a + u[k];
+  for (int m = 0; m < 4; ++m) {
+    const uint64_t b = 16 * uint64_t((t >> 4) + 16 * m);
+    u[m] = b < nb ? rv(x, y, (b0 + b) * 16 + j) : 0.0;
   }
-  for (; i < rows; i += kBlock) a = a + rv(x, y, r0 + i);
+  double a = (u[0] + u[1]) + (u[2] + u[3]);
   a = a + __shfl_xor(a, 16);
   a = a + __shfl_xor(a, 32);
   if (l < 16) s_a[t / kWave][l] = a;
   __syncthreads();
-  if (t < 16) asum[t * tiles + tile] = s_a[0][t] + s_a[1][t] + s_a[2][t] + s_a[3][t];
+  if (t < 16) asum[t * tiles + tile] = 16.0 * ((s_a[0][t] + s_a[1][t]) + (s_a[2][t] + s_a[3][t]));
 }
 
-// 2. One block per chain: thread p sums part p of the tiles (ceil(tiles / 1024) each), the part
+// Prefix: one block per chain: thread p sums part p of the tiles (ceil(tiles / 1024) each), the part
 //    sums are scanned in LDS (Hillis-Steele), then each thread writes its tiles' approximate
 //    starts, pre[tile * 16 + j].
 constexpr int kPrefixParts = 1024;
@@ -651,13 +653,19 @@ __global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asu
   }
 }
 
-// 3. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the start.
-template <class RV>
+// 2 and 4. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the
+// start. The first pass (FIX false) also writes the tile's approximate chain sums: each run's
+// sum in order, then runs joined in the order of the summary tree. The fix-up (FIX true) keeps
+// a chain whose stored hypothesis is the binade of its start and only re-checks usability; a
+// tile with any other chain is summarised again whole.
+template <class RV, bool FIX>
 __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double *__restrict__ x,
                                                              const double *__restrict__ y, uint64_t blocks,
                                                              const double *pre, uint64_t tiles,
-                                                             TileSum *sums) {
+                                                             TileSum *sums, double *asum) {
   __shared__ RunFn s_f[kFastWaves][16];
+  __shared__ double s_a[kFastWaves][16];
+  __shared__ int s_redo;
   const uint64_t tile = blockIdx.x;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const int j = t & 15, r = t >> 4;
@@ -665,18 +673,42 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double
   const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
   const double start = pre[tile * 16 + j];
   const Binade h = binade_of(start);
-  RunFn f = run_identity();
-  if (h.ok) {
-    double v[kRunBlocks];
-    const int cnt = nb - r * kRunBlocks;
-    if (cnt >= kRunBlocks) {
-#pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j) : 0.0;
+  if constexpr (FIX) {
+    if (t == 0) s_redo = 0;
+    __syncthreads();
+    if (t < 16) {
+      TileSum *ts = sums + uint64_t(t) * tiles + tile;
+      const uint64_t hw = ts->hword;
+      if ((hw & ~kUsableBit) == pack_binade(h)) {
+        double probe;
+        const bool usable = run_apply(fn_of(ts->d0, ts->d1, ts->lo0, ts->lo1, ts->hi0, ts->hi1), h, start, &probe);
+        const uint64_t nw = pack_binade(h) | (usable ? kUsableBit : 0);
+        if (nw != hw) ts->hword = nw;
+      } else {
+        s_redo = 1;
+      }
     }
-    f = run_of(v, cnt, binade_ref(h));
+    __syncthreads();
+    if (!s_redo) return;
+  }
+  double v[kRunBlocks];
+  const int cnt = nb - r * kRunBlocks;
+  if (cnt >= kRunBlocks) {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j) : 0.0;
+  }
+  RunFn f = run_identity();
+  if (h.ok) f = run_of(v, cnt, binade_ref(h));
+  if constexpr (!FIX) {
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
+    a = a + __shfl_xor(a, 16);
+    a = a + __shfl_xor(a, 32);
+    if (l < 16) s_a[w][j] = a;
   }
   {
     RunFn o = shfl_xor_pod(f, 16);
@@ -693,6 +725,12 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double
   ft = (l & 16) ? run_join(o, ft) : run_join(ft, o);
   o = shfl_xor_pod(ft, 32);
   ft = (l & 32) ? run_join(o, ft) : run_join(ft, o);
+  if constexpr (!FIX) {
+    double a = (s_a[4 * q][j] + s_a[4 * q + 1][j]) + (s_a[4 * q + 2][j] + s_a[4 * q + 3][j]);
+    a = a + __shfl_xor(a, 16);
+    a = a + __shfl_xor(a, 32);
+    if (l < 16) asum[uint64_t(l) * tiles + tile] = a;
+  }
   if (l < 16) {
     double probe;
     const bool usable = run_apply(ft, h, start, &probe);
@@ -957,9 +995,11 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
     double *pre = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
     auto *sums = static_cast<TileSum *>(tmp_alloc(c, tiles * 16 * sizeof(TileSum)));
     auto *gsum = static_cast<GroupSum *>(tmp_alloc(c, groups * 16 * sizeof(GroupSum)));
-    k_canon_asum<RV><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y, blocks, tiles, asum);
+    k_canon_sample<RV><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y, blocks, tiles, asum);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
-    k_canon_summ<RV><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums);
+    k_canon_summ<RV, false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum);
+    k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
+    k_canon_summ<RV, true><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum);
     k_canon_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(sums, tiles, groups,
                                                                                               gsum);
     k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, gsum, groups, acc);

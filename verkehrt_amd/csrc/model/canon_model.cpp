@@ -26,6 +26,14 @@ namespace {
 
 constexpr uint64_t kTileBlocks = 1024, kRun = 16;
 
+// The summary of chain j's blocks [b, min(b + 16, b1)) through the device's run_of.
+RunFn run_at(const double *vals, uint64_t b, uint64_t b1, int j, const BinadeRef &ref) {
+  double v[kRun];
+  const int cnt = static_cast<int>(std::min<uint64_t>(kRun, b1 - b));
+  for (int k = 0; k < static_cast<int>(kRun); ++k) v[k] = k < cnt ? vals[(b + k) * 16 + j] : 0.0;
+  return run_of(v, cnt, ref);
+}
+
 // The fallback wave of k_canon.hip: chain j's blocks [b0, b1) from the exact value s. With at
 // most two changes of the per-run binade hypotheses (each run's approximate start is s plus
 // the approximate sums of the runs before it, summed in the device's scan order), rounds:
@@ -36,12 +44,7 @@ double eval_chain(const double *vals, uint64_t b0, uint64_t b1, int j, double s,
   const uint64_t nruns = (b1 - b0 + kRun - 1) / kRun;
   auto run_of = [&](uint64_t r, const Binade &h) {
     RunFn f = run_identity();
-    if (h.ok) {
-      const BinadeRef ref = binade_ref(h);
-      RunAcc a = run_start();
-      for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b) run_push(a, ref, vals[b * 16 + j]);
-      f = run_end(a);
-    }
+    if (h.ok) f = run_at(vals, b0 + r * kRun, b1, j, binade_ref(h));
     return f;
   };
   auto fold_run = [&](uint64_t r, double t) {
@@ -125,9 +128,7 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
       if (h.ok) {
         const BinadeRef ref = binade_ref(h);
         for (uint64_t rb = b0; rb < b1; rb += kRun) {
-          RunAcc a = run_start();
-          for (uint64_t b = rb; b < std::min(b1, rb + kRun); ++b) run_push(a, ref, vals[b * 16 + j]);
-          ft = run_join(ft, run_end(a));
+          ft = run_join(ft, run_at(vals, rb, b1, j, ref));
         }
       }
       hyp[t] = h;
@@ -176,8 +177,9 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
   return 0;
 }
 
-// The streamed form of k_canon.hip for long inputs, one chain at a time: approximate tile sums
-// (summed in the device's order), their exclusive prefix from the start state (in 1024 parts,
+// The streamed form of k_canon.hip for long inputs, one chain at a time (its predicted first
+// hypotheses only decide which tiles it summarises twice, so they are left out): approximate
+// tile sums (summed in the device's order), their exclusive prefix from the start state (in 1024 parts,
 // as the prefix kernel does), tile summaries under the binade of that prefix, usable
 // where they apply at it; groups of `group_tiles` tiles are clean when every tile is usable
 // under one hypothesis; the walk joins consecutive clean groups of one hypothesis and applies
@@ -194,23 +196,25 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
   const uint64_t groups = (tiles + gt - 1) / gt;
   uint64_t st[4] = {0, 0, 0, 0}, rounds = 0;
   for (int j = 0; j < 16; ++j) {
-    // 1. approximate tile sums: 16 partial sums of blocks b0 + p + 16 m, combined as the
-    //    device's lanes (xor 16, xor 32) and waves are
+    // 1. approximate tile sums as the first summary pass forms them: each run's elements in
+    //    order (from 0.0, missing elements as 0.0), runs in fours, fours in fours, and so on
     std::vector<double> asum(tiles);
     for (uint64_t t = 0; t < tiles; ++t) {
       const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
-      double part[16];
-      for (int p = 0; p < 16; ++p) {
+      double part[64];
+      for (uint64_t r = 0; r < 64; ++r) {
         double a = 0.0;
-        for (uint64_t b = b0 + p; b < b1; b += 16) a = a + vals[b * 16 + j];
-        part[p] = a;
+        for (uint64_t k = 0; k < kRun; ++k) {
+          const uint64_t b = b0 + r * kRun + k;
+          a = a + (b < b1 ? vals[b * 16 + j] : 0.0);
+        }
+        part[r] = a;
       }
-      double w[4];
-      for (int q = 0; q < 4; ++q) {
-        const double x0 = part[4 * q] + part[4 * q + 1], x1 = part[4 * q + 2] + part[4 * q + 3];
-        w[q] = x0 + x1;
-      }
-      asum[t] = w[0] + w[1] + w[2] + w[3];
+      double w[16];
+      for (int i = 0; i < 16; ++i) w[i] = (part[4 * i] + part[4 * i + 1]) + (part[4 * i + 2] + part[4 * i + 3]);
+      double q[4];
+      for (int i = 0; i < 4; ++i) q[i] = (w[4 * i] + w[4 * i + 1]) + (w[4 * i + 2] + w[4 * i + 3]);
+      asum[t] = (q[0] + q[1]) + (q[2] + q[3]);
     }
     // 2. exclusive prefix in 1024 parts, the part sums scanned as the device's Hillis-Steele
     std::vector<double> pre(tiles);
@@ -241,9 +245,7 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
       if (h.ok) {
         const BinadeRef ref = binade_ref(h);
         for (uint64_t rb = b0; rb < b1; rb += kRun) {
-          RunAcc a = run_start();
-          for (uint64_t b = rb; b < std::min(b1, rb + kRun); ++b) run_push(a, ref, vals[b * 16 + j]);
-          ft = run_join(ft, run_end(a));
+          ft = run_join(ft, run_at(vals, rb, b1, j, ref));
         }
       }
       double probe;
