@@ -23,7 +23,8 @@ def model():
     L.vkm_canonical_fold.restype = C.c_int
     L.vkm_canonical_fold.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]
     L.vkm_canonical_fold_streamed.restype = C.c_int
-    L.vkm_canonical_fold_streamed.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]
+    L.vkm_canonical_fold_streamed.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
+                                              C.c_void_p]
     return L
 
 
@@ -38,12 +39,15 @@ def chains_oracle(vals, acc):
     return [float(np.cumsum(np.concatenate([[acc[j]], vals[j::16]]))[-1]) for j in range(16)]
 
 
-def run(model, vals, acc, lag, streamed=False):
+def run(model, vals, acc, lag, streamed=False, records=True):
     vals = np.ascontiguousarray(vals, np.float64)
     a = np.array(acc, np.float64)
-    stats = np.zeros(4, np.uint64)
-    f = model.vkm_canonical_fold_streamed if streamed else model.vkm_canonical_fold
-    f(vals.ctypes.data, vals.size // 16, a.ctypes.data, lag, stats.ctypes.data)
+    stats = np.zeros(6, np.uint64)
+    if streamed:
+        model.vkm_canonical_fold_streamed(vals.ctypes.data, vals.size // 16, a.ctypes.data, lag, int(records),
+                                          stats.ctypes.data)
+    else:
+        model.vkm_canonical_fold(vals.ctypes.data, vals.size // 16, a.ctypes.data, lag, stats.ctypes.data)
     return a, stats
 
 
@@ -117,18 +121,22 @@ def check_bits(name, got, want):
             assert bits(got[j]) == bits(want[j]), (name, j, got[j], want[j])
 
 
+@pytest.mark.parametrize("records", [True, False])
 @pytest.mark.parametrize("group_tiles", [1, 4, 64])
-def test_streamed_model_matches_sequential_chains(model, group_tiles):
+def test_streamed_model_matches_sequential_chains(model, group_tiles, records):
     """The streamed form (prefix hypotheses, clean groups, the walk) on the same data; small
-    groups make joined spans, failed spans and segments all occur on 41 tiles."""
-    st = np.zeros(4, np.uint64)
+    groups make joined spans, failed spans and segments all occur on 41 tiles. Tiles whose
+    summary fails go through run records or, without them, the fallback wave."""
+    st = np.zeros(6, np.uint64)
     for name, vals, acc in datasets():
-        got, stats = run(model, vals, acc, group_tiles, streamed=True)
+        got, stats = run(model, vals, acc, group_tiles, streamed=True, records=records)
         check_bits(name, got, chains_oracle(vals, acc))
         st += stats
     assert st[0] > 0 and st[3] > 0  # joined spans applied, tiles evaluated exactly
     if group_tiles < 64:
         assert st[2] > 0  # segments of non-clean groups applied at once
+    if records:
+        assert st[4] > 0 and st[5] > 0  # run segments applied, runs folded
 
 
 def test_streamed_model_long_uniform(model):

@@ -538,23 +538,27 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double
 //  1. k_canon_sample, k_canon_prefix: predicted approximate starts of every tile per chain,
 //     from every 16th block (a sixteenth of the rows) scaled by 16 and prefixed from the start
 //     state; their binades are the first hypotheses;
-//  2. k_canon_summ<false>: every tile's run summaries under its predicted hypothesis, joined
+//  2. k_canon_summ: every tile's run summaries under its predicted hypothesis, joined
 //     into a tile summary, and the tile's approximate chain sums (one streaming read);
 //  3. k_canon_prefix: the exclusive prefix of those sums from the start state, the approximate
 //     start of every tile (any order of these approximate sums serves: the applies below prove
 //     every result);
-//  4. k_canon_summ<true>: tiles with a chain whose prediction missed the binade of its
-//     approximate start are summarised again (re-read); every summary is marked usable when it
-//     applies at the approximate start;
-//  5. k_canon_groups: the ordered join of each group of 64 tiles whose summaries are all usable
+//  4. k_canon_check: every chain's summary is checked at its approximate start; tiles with a
+//     chain whose prediction missed the binade of its approximate start, or whose summary is not
+//     usable there, are listed;
+//  5. k_canon_fix: the listed tiles are re-read: missed chains are summarised again, and every
+//     chain still not usable gets run records (each run summarised under the binade of its own
+//     approximate start);
+//  6. k_canon_groups: the ordered join of each group of 64 tiles whose summaries are all usable
 //     under one hypothesis ("clean" groups);
-//  6. k_canon_walk: one wave per chain walks the groups in order from the exact start value:
+//  7. k_canon_walk: one wave per chain walks the groups in order from the exact start value:
 //     consecutive clean groups with one hypothesis join, and their joined summary is applied
 //     once; other groups are split into segments of usable same-hypothesis tiles, each applied
-//     once; a summary that fails is retried tile by tile, and a tile that fails is evaluated
-//     exactly by the wave (eval_chain).
+//     once; a summary that fails is retried tile by tile; a tile that fails goes the same way
+//     through its run records (segments of runs, then runs, then a run's elements one by one),
+//     or, without records, is evaluated exactly by the wave (eval_chain).
 // The applies prove each result equal to the sequential fold, as in the single-pass form. A
-// wrong prediction costs only the re-read of its tile in step 4.
+// wrong prediction costs only the re-read of its tile in step 5.
 constexpr int kGroupTiles = 64;
 constexpr uint64_t kUsableBit = uint64_t(1) << 16;
 
@@ -569,6 +573,12 @@ struct GroupSum {
   uint64_t clean;  // every tile usable under that hypothesis
 };
 
+struct RunRec {
+  double d0, d1, lo0, lo1, hi0, hi1;
+  uint64_t hword;  // pack_binade(h) | kUsableBit when the run applies at its approximate start
+  uint64_t pad;
+};
+
 __device__ __forceinline__ RunFn fn_of(double d0, double d1, double lo0, double lo1, double hi0, double hi1) {
   RunFn f;
   f.d[0] = d0;
@@ -578,6 +588,31 @@ __device__ __forceinline__ RunFn fn_of(double d0, double d1, double lo0, double 
   f.hi[0] = hi0;
   f.hi[1] = hi1;
   return f;
+}
+
+// Chain j's rows of blocks b .. b + min(cnt, 16) - 1 into v (zeros past cnt).
+template <class RV>
+__device__ __forceinline__ void load_run(const RV &rv, const double *x, const double *y, uint64_t b, int j,
+                                         int cnt, double (&v)[kRunBlocks]) {
+  if (cnt >= kRunBlocks) {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b + k) * 16 + j);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b + k) * 16 + j) : 0.0;
+  }
+}
+
+// The tile summary of chain j from the 16 waves' joined runs: wave 0's lanes join four waves
+// each (lane group l / 16), then the four groups (xor 16, 32); every lane of chain j gets it.
+__device__ __forceinline__ RunFn tile_tree(const RunFn (&s_f)[kFastWaves][16], int j, int l) {
+  const int q = l >> 4;
+  RunFn ft = run_join(run_join(s_f[4 * q][j], s_f[4 * q + 1][j]), run_join(s_f[4 * q + 2][j], s_f[4 * q + 3][j]));
+  RunFn o = shfl_xor_pod(ft, 16);
+  ft = (l & 16) ? run_join(o, ft) : run_join(ft, o);
+  o = shfl_xor_pod(ft, 32);
+  ft = (l & 32) ? run_join(o, ft) : run_join(ft, o);
+  return ft;
 }
 
 // 1. Predicted chain sums of a tile, asum[j * tiles + tile]: 16 times the sum of the tile's
@@ -653,63 +688,35 @@ __global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asu
   }
 }
 
-// 2 and 4. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the
-// start. The first pass (FIX false) also writes the tile's approximate chain sums: each run's
-// sum in order, then runs joined in the order of the summary tree. The fix-up (FIX true) keeps
-// a chain whose stored hypothesis is the binade of its start and only re-checks usability; a
-// tile with any other chain is summarised again whole.
-template <class RV, bool FIX>
+// 2. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the start;
+// also writes the tile's approximate chain sums: each run's sum in order, then runs joined in
+// the order of the summary tree. Block 0 clears the two counters of steps 4 and 5.
+template <class RV>
 __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double *__restrict__ x,
                                                              const double *__restrict__ y, uint64_t blocks,
                                                              const double *pre, uint64_t tiles,
-                                                             TileSum *sums, double *asum) {
+                                                             TileSum *sums, double *asum, uint32_t *counts) {
   __shared__ RunFn s_f[kFastWaves][16];
   __shared__ double s_a[kFastWaves][16];
-  __shared__ int s_redo;
   const uint64_t tile = blockIdx.x;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const int j = t & 15, r = t >> 4;
+  if (tile == 0 && t < 2) counts[t] = 0;
   const uint64_t b0 = tile * kTileBlocks;
   const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
   const double start = pre[tile * 16 + j];
   const Binade h = binade_of(start);
-  if constexpr (FIX) {
-    if (t == 0) s_redo = 0;
-    __syncthreads();
-    if (t < 16) {
-      TileSum *ts = sums + uint64_t(t) * tiles + tile;
-      const uint64_t hw = ts->hword;
-      if ((hw & ~kUsableBit) == pack_binade(h)) {
-        double probe;
-        const bool usable = run_apply(fn_of(ts->d0, ts->d1, ts->lo0, ts->lo1, ts->hi0, ts->hi1), h, start, &probe);
-        const uint64_t nw = pack_binade(h) | (usable ? kUsableBit : 0);
-        if (nw != hw) ts->hword = nw;
-      } else {
-        s_redo = 1;
-      }
-    }
-    __syncthreads();
-    if (!s_redo) return;
-  }
   double v[kRunBlocks];
   const int cnt = nb - r * kRunBlocks;
-  if (cnt >= kRunBlocks) {
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j);
-  } else {
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b0 + r * kRunBlocks + k) * 16 + j) : 0.0;
-  }
+  load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
   RunFn f = run_identity();
   if (h.ok) f = run_of(v, cnt, binade_ref(h));
-  if constexpr (!FIX) {
-    double a = 0.0;
+  double a = 0.0;
 #pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
-    a = a + __shfl_xor(a, 16);
-    a = a + __shfl_xor(a, 32);
-    if (l < 16) s_a[w][j] = a;
-  }
+  for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
+  a = a + __shfl_xor(a, 16);
+  a = a + __shfl_xor(a, 32);
+  if (l < 16) s_a[w][j] = a;
   {
     RunFn o = shfl_xor_pod(f, 16);
     f = (l & 16) ? run_join(o, f) : run_join(f, o);
@@ -719,24 +726,136 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double
   if (l < 16) s_f[w][j] = f;
   __syncthreads();
   if (w != 0) return;
-  const int q = l >> 4;
-  RunFn ft = run_join(run_join(s_f[4 * q][j], s_f[4 * q + 1][j]), run_join(s_f[4 * q + 2][j], s_f[4 * q + 3][j]));
-  RunFn o = shfl_xor_pod(ft, 16);
-  ft = (l & 16) ? run_join(o, ft) : run_join(ft, o);
-  o = shfl_xor_pod(ft, 32);
-  ft = (l & 32) ? run_join(o, ft) : run_join(ft, o);
-  if constexpr (!FIX) {
-    double a = (s_a[4 * q][j] + s_a[4 * q + 1][j]) + (s_a[4 * q + 2][j] + s_a[4 * q + 3][j]);
-    a = a + __shfl_xor(a, 16);
-    a = a + __shfl_xor(a, 32);
-    if (l < 16) asum[uint64_t(l) * tiles + tile] = a;
-  }
+  const RunFn ft = tile_tree(s_f, j, l);
+  a = (s_a[4 * (l >> 4)][j] + s_a[4 * (l >> 4) + 1][j]) + (s_a[4 * (l >> 4) + 2][j] + s_a[4 * (l >> 4) + 3][j]);
+  a = a + __shfl_xor(a, 16);
+  a = a + __shfl_xor(a, 32);
   if (l < 16) {
+    asum[uint64_t(l) * tiles + tile] = a;
     double probe;
     const bool usable = run_apply(ft, h, start, &probe);
     TileSum ts{ft.d[0], ft.d[1], ft.lo[0], ft.lo[1], ft.hi[0], ft.hi[1],
                pack_binade(h) | (usable ? kUsableBit : 0), 0};
     sums[uint64_t(l) * tiles + tile] = ts;
+  }
+}
+
+// 4. One thread per tile: a chain whose stored hypothesis is the binade of its approximate
+// start only has its usability re-checked there; a tile with a chain whose hypothesis differs,
+// or with a chain whose summary is not usable, is listed for step 5.
+__global__ __launch_bounds__(kBlock) void k_canon_check(const double *pre, uint64_t tiles, TileSum *sums,
+                                                        uint32_t *list, uint32_t *counts) {
+  const uint64_t tile = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  if (tile >= tiles) return;
+  bool listed = false;
+  for (int j = 0; j < 16; ++j) {
+    TileSum *ts = sums + uint64_t(j) * tiles + tile;
+    const double start = pre[tile * 16 + j];
+    const Binade h = binade_of(start);
+    const uint64_t hw = ts->hword;
+    if ((hw & ~kUsableBit) == pack_binade(h)) {
+      double probe;
+      const bool usable = run_apply(fn_of(ts->d0, ts->d1, ts->lo0, ts->lo1, ts->hi0, ts->hi1), h, start, &probe);
+      const uint64_t nw = pack_binade(h) | (usable ? kUsableBit : 0);
+      if (nw != hw) ts->hword = nw;
+      listed = listed || !usable;
+    } else {
+      listed = true;
+    }
+  }
+  if (listed) list[atomicAdd(counts, 1u)] = static_cast<uint32_t>(tile);
+}
+
+// 5. The listed tiles, one block each (a grid-stride loop over the list): chains whose
+// hypothesis differs are summarised again under the binade of their approximate start; every
+// chain whose summary is then not usable gets a slot of run records, while slots last (its
+// TileSum.pad is the slot + 1): each run's summary under the binade of its own approximate
+// start (the tile's plus the approximate sums of the runs before it), marked usable when it
+// applies there.
+template <class RV>
+__global__ __launch_bounds__(kFastThreads) void k_canon_fix(RV rv, const double *__restrict__ x,
+                                                            const double *__restrict__ y, uint64_t blocks,
+                                                            const double *pre, uint64_t tiles, TileSum *sums,
+                                                            const uint32_t *list, uint32_t *counts,
+                                                            RunRec *recs, uint32_t rcap) {
+  __shared__ RunFn s_f[kFastWaves][16];
+  __shared__ double s_a[kFastWaves][16];
+  __shared__ int s_mis[16];
+  __shared__ uint32_t s_slot[16];
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  const int j = t & 15, r = t >> 4, q = l >> 4;
+  const uint32_t n = counts[0];
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint64_t tile = list[i];
+    const uint64_t b0 = tile * kTileBlocks;
+    const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
+    const double start = pre[tile * 16 + j];
+    const Binade h = binade_of(start);
+    if (t < 16) s_mis[t] = (sums[uint64_t(t) * tiles + tile].hword & ~kUsableBit) != pack_binade(h);
+    double v[kRunBlocks];
+    const int cnt = nb - r * kRunBlocks;
+    load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
+    // run sums, their inclusive prefix over the wave's four runs of the chain, wave totals
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
+    double inc = a;
+    {
+      const double o = __shfl_up(inc, 16);
+      if (q >= 1) inc = o + inc;
+      const double o2 = __shfl_up(inc, 32);
+      if (q >= 2) inc = o2 + inc;
+    }
+    if (q == 3) s_a[w][j] = inc;
+    __syncthreads();
+    const bool mis = s_mis[j] != 0;
+    RunFn f = run_identity();
+    if (mis && h.ok) f = run_of(v, cnt, binade_ref(h));
+    {
+      RunFn o = shfl_xor_pod(f, 16);
+      f = (l & 16) ? run_join(o, f) : run_join(f, o);
+      o = shfl_xor_pod(f, 32);
+      f = (l & 32) ? run_join(o, f) : run_join(f, o);
+    }
+    if (l < 16) s_f[w][j] = f;
+    __syncthreads();
+    if (w == 0) {
+      const RunFn ft = tile_tree(s_f, j, l);
+      if (l < 16) {
+        TileSum *ts = sums + uint64_t(l) * tiles + tile;
+        bool usable;
+        if (s_mis[l]) {
+          double probe;
+          usable = run_apply(ft, h, start, &probe);
+          *ts = TileSum{ft.d[0], ft.d[1], ft.lo[0], ft.lo[1], ft.hi[0], ft.hi[1],
+                        pack_binade(h) | (usable ? kUsableBit : 0), 0};
+        } else {
+          usable = (ts->hword & kUsableBit) != 0;
+        }
+        uint32_t slot = 0;
+        if (!usable) {
+          const uint32_t k = atomicAdd(counts + 1, 1u);
+          if (k < rcap) slot = k + 1;
+        }
+        s_slot[l] = slot;
+        ts->pad = slot;
+      }
+    }
+    __syncthreads();
+    const uint32_t slot = s_slot[j];
+    if (slot) {
+      double rs = start;
+      for (int ww = 0; ww < w; ++ww) rs = rs + s_a[ww][j];
+      rs = rs + (inc - a);
+      const Binade hr = binade_of(rs);
+      RunFn fr = run_identity();
+      if (hr.ok) fr = run_of(v, cnt, binade_ref(hr));
+      double probe;
+      const bool usable = r * kRunBlocks < nb && run_apply(fr, hr, rs, &probe);
+      recs[uint64_t(slot - 1) * kWave + r] = RunRec{fr.d[0], fr.d[1], fr.lo[0], fr.lo[1], fr.hi[0], fr.hi[1],
+                                                    pack_binade(hr) | (usable ? kUsableBit : 0), 0};
+    }
+    __syncthreads();
   }
 }
 
@@ -750,7 +869,7 @@ __device__ __forceinline__ RunFn join_lanes(RunFn f, int l) {
   return f;
 }
 
-// 4. One wave per (chain, group of 64 tiles).
+// 6. One wave per (chain, group of 64 tiles).
 __global__ __launch_bounds__(kBlock) void k_canon_groups(const TileSum *sums, uint64_t tiles, uint64_t groups,
                                                          GroupSum *gsum) {
   const uint64_t wid = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
@@ -776,28 +895,20 @@ __global__ __launch_bounds__(kBlock) void k_canon_groups(const TileSum *sums, ui
   }
 }
 
-// The tiles of group g from the exact value cur, by segments of usable same-hypothesis tiles
-// (each applied once, or tile by tile when its joined summary fails); `tiles_only` skips the
-// segment attempt.
-template <class RV>
-__device__ double walk_group(const RV &rv, const double *x, const double *y, uint64_t blocks,
-                             const TileSum *sums, uint64_t tiles, int j, uint64_t g, double cur,
-                             bool tiles_only) {
+// Lanes 0 .. n - 1 of the wave in order from the exact value cur: lane i holds a summary f
+// under the hypothesis word hw (kUsableBit: it applied at its approximate start). Segments of
+// consecutive usable lanes with one hypothesis are applied at once (with `singles`, every lane
+// is its own segment); a segment that fails goes lane by lane, and a lane whose summary does
+// not apply goes through one(i, cur), which returns the exact value after lane i.
+template <class One>
+__device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, double cur, One &&one) {
   const int l = lane_id();
-  const uint64_t tile = g * kGroupTiles + l;
-  const bool valid = tile < tiles;
-  RunFn f = run_identity();
-  uint64_t hw = 0;
-  if (valid) {
-    const TileSum ts = sums[uint64_t(j) * tiles + tile];
-    f = fn_of(ts.d0, ts.d1, ts.lo0, ts.lo1, ts.hi0, ts.hi1);
-    hw = ts.hword;
-  }
+  const bool valid = l < n;
   const bool usable = valid && (hw & kUsableBit);
   const uint64_t hprev = __shfl_up(hw, 1);
   const bool uprev = __shfl_up(static_cast<int>(usable), 1) != 0;
-  // a segment starts at lane 0, at an unusable tile, after one, and where the hypothesis changes
-  const bool start = l == 0 || !usable || !uprev || hw != hprev || tiles_only;
+  // a segment starts at lane 0, at an unusable lane, after one, and where the hypothesis changes
+  const bool start = l == 0 || !usable || !uprev || hw != hprev || singles;
   // segmented inclusive scan: (f, s) then (f', s') = s' ? (f', 1) : (f . f', s)
   RunFn sc = f;
   int ss = start ? 1 : 0;
@@ -810,44 +921,22 @@ __device__ double walk_group(const RV &rv, const double *x, const double *y, uin
       ss = os;
     }
   }
-  const uint64_t vmask = __ballot(valid);
   const uint64_t starts = __ballot(start && valid);
-  const int nvalid = __popcll(vmask);
-  // segment ends: the lane before each start, and the last valid lane
-  uint64_t ends = (starts >> 1) | (uint64_t(1) << (nvalid - 1));
-  ends &= vmask;
+  // segment ends: the lane before each start, and lane n - 1
+  uint64_t ends = ((starts >> 1) | (uint64_t(1) << (n - 1))) & (n == kWave ? ~uint64_t(0) : (uint64_t(1) << n) - 1);
   int first = 0;
   while (ends) {
     const int e = __builtin_ctzll(ends);
     ends &= ends - 1;
-    const uint64_t hw_e = __shfl(hw, e);
-    const int us_e = __shfl(static_cast<int>(usable), e);
-    RunFn tot;
-    tot.d[0] = __shfl(sc.d[0], e);
-    tot.d[1] = __shfl(sc.d[1], e);
-    tot.lo[0] = __shfl(sc.lo[0], e);
-    tot.lo[1] = __shfl(sc.lo[1], e);
-    tot.hi[0] = __shfl(sc.hi[0], e);
-    tot.hi[1] = __shfl(sc.hi[1], e);
+    const uint64_t hw_e = bcast_pod(hw, e);
     double out;
-    if (us_e && run_apply(tot, unpack_binade(hw_e & ~kUsableBit), cur, &out)) {
+    if ((hw_e & kUsableBit) && run_apply(bcast_pod(sc, e), unpack_binade(hw_e & ~kUsableBit), cur, &out)) {
       cur = out;
     } else {
-      // tile by tile through the segment first..e
-      for (int r = first; r <= e; ++r) {
-        const uint64_t hw_r = __shfl(hw, r);
-        RunFn fr;
-        fr.d[0] = __shfl(f.d[0], r);
-        fr.d[1] = __shfl(f.d[1], r);
-        fr.lo[0] = __shfl(f.lo[0], r);
-        fr.lo[1] = __shfl(f.lo[1], r);
-        fr.hi[0] = __shfl(f.hi[0], r);
-        fr.hi[1] = __shfl(f.hi[1], r);
-        if (!((hw_r & kUsableBit) && run_apply(fr, unpack_binade(hw_r & ~kUsableBit), cur, &out))) {
-          const uint64_t tb = (g * kGroupTiles + r) * kTileBlocks;
-          const int nb = static_cast<int>(blocks - tb < uint64_t(kTileBlocks) ? blocks - tb : kTileBlocks);
-          out = eval_chain(rv, x, y, tb, nb, j, cur);
-        }
+      for (int i = first; i <= e; ++i) {
+        const uint64_t hw_i = bcast_pod(hw, i);
+        if (!((hw_i & kUsableBit) && run_apply(bcast_pod(f, i), unpack_binade(hw_i & ~kUsableBit), cur, &out)))
+          out = one(i, cur);
         cur = out;
       }
     }
@@ -856,11 +945,77 @@ __device__ double walk_group(const RV &rv, const double *x, const double *y, uin
   return cur;
 }
 
-// 5. One wave per chain.
+// Chain j's blocks b0 .. b0 + nb - 1 (nb >= 1) from the exact value s with the tile's run records
+// (lane = run); a run whose record does not apply is folded element by element, its values
+// loaded ahead when its record was not usable at its approximate start.
+template <class RV>
+__device__ double walk_runs(const RV &rv, const double *x, const double *y, const RunRec *rec, uint64_t b0,
+                            int nb, int j, double s) {
+  const int l = lane_id();
+  const int nruns = (nb + kRunBlocks - 1) / kRunBlocks;
+  RunFn f = run_identity();
+  uint64_t hw = 0;
+  if (l < nruns) {
+    const RunRec rr = rec[l];
+    f = fn_of(rr.d0, rr.d1, rr.lo0, rr.lo1, rr.hi0, rr.hi1);
+    hw = rr.hword;
+  }
+  const int cnt = nb - l * kRunBlocks;
+  double v[kRunBlocks];
+  const bool ahead = l < nruns && !(hw & kUsableBit);
+#pragma unroll
+  for (int k = 0; k < kRunBlocks; ++k) v[k] = 0.0;
+  if (ahead) load_run(rv, x, y, b0 + l * kRunBlocks, j, cnt, v);
+  const uint64_t loaded = __ballot(ahead);
+  return walk_lanes(f, hw, nruns, false, s, [&](int i, double cur) {
+    const int ci = nb - i * kRunBlocks < kRunBlocks ? nb - i * kRunBlocks : kRunBlocks;
+    double u[kRunBlocks];
+    if ((loaded >> i) & 1) {
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k) u[k] = bcast_pod(v[k], i);
+    } else {
+      // lane k < ci loads element k of run i
+      const double e = l < ci ? rv(x, y, (b0 + uint64_t(i) * kRunBlocks + l) * 16 + j) : 0.0;
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k) u[k] = bcast_pod(e, k);
+    }
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k)
+      if (k < ci) cur = cur + u[k];
+    return cur;
+  });
+}
+
+// The tiles of group g from the exact value cur: lane = tile, through walk_lanes; a tile whose
+// summary does not apply goes by its run records, or, without a slot, through eval_chain.
+template <class RV>
+__device__ double walk_group(const RV &rv, const double *x, const double *y, uint64_t blocks,
+                             const TileSum *sums, uint64_t tiles, const RunRec *recs, int j, uint64_t g,
+                             double cur, bool singles) {
+  const int l = lane_id();
+  const uint64_t tile = g * kGroupTiles + l;
+  const int n = tiles - g * kGroupTiles < uint64_t(kWave) ? static_cast<int>(tiles - g * kGroupTiles) : kWave;
+  RunFn f = run_identity();
+  uint64_t hw = 0, slot = 0;
+  if (l < n) {
+    const TileSum ts = sums[uint64_t(j) * tiles + tile];
+    f = fn_of(ts.d0, ts.d1, ts.lo0, ts.lo1, ts.hi0, ts.hi1);
+    hw = ts.hword;
+    slot = ts.pad;
+  }
+  return walk_lanes(f, hw, n, singles, cur, [&](int i, double c) {
+    const uint64_t tb = (g * kGroupTiles + i) * kTileBlocks;
+    const int nb = static_cast<int>(blocks - tb < uint64_t(kTileBlocks) ? blocks - tb : kTileBlocks);
+    const uint64_t si = bcast_pod(slot, i);
+    return si ? walk_runs(rv, x, y, recs + (si - 1) * kWave, tb, nb, j, c) : eval_chain(rv, x, y, tb, nb, j, c);
+  });
+}
+
+// 7. One wave per chain.
 template <class RV>
 __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__restrict__ x,
                                                       const double *__restrict__ y, uint64_t blocks,
-                                                      const TileSum *sums, uint64_t tiles,
+                                                      const TileSum *sums, uint64_t tiles, const RunRec *recs,
                                                       const GroupSum *gsum, uint64_t groups, double *acc) {
   const int j = blockIdx.x, l = lane_id();
   double cur = acc[j];
@@ -873,7 +1028,7 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
       cur = out;
     } else {
       for (uint64_t gg = pend_g0; gg < g_end; ++gg)
-        cur = walk_group(rv, x, y, blocks, sums, tiles, j, gg, cur, true);
+        cur = walk_group(rv, x, y, blocks, sums, tiles, recs, j, gg, cur, true);
     }
     pending = false;
   };
@@ -884,16 +1039,11 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
     const int cnt = groups - g0 < uint64_t(kWave) ? static_cast<int>(groups - g0) : kWave;
     for (int k = 0; k < cnt; ++k) {
       const uint64_t g = g0 + k;
-      const uint64_t clean = __shfl(mine.clean, k);
-      const uint64_t hw = __shfl(mine.hword, k);
+      const uint64_t clean = bcast_pod(mine.clean, k);
+      const uint64_t hw = bcast_pod(mine.hword, k);
       if (clean) {
-        RunFn gf;
-        gf.d[0] = __shfl(mine.d0, k);
-        gf.d[1] = __shfl(mine.d1, k);
-        gf.lo[0] = __shfl(mine.lo0, k);
-        gf.lo[1] = __shfl(mine.lo1, k);
-        gf.hi[0] = __shfl(mine.hi0, k);
-        gf.hi[1] = __shfl(mine.hi1, k);
+        const RunFn gf = fn_of(bcast_pod(mine.d0, k), bcast_pod(mine.d1, k), bcast_pod(mine.lo0, k),
+                               bcast_pod(mine.lo1, k), bcast_pod(mine.hi0, k), bcast_pod(mine.hi1, k));
         if (pending && hw == pend_h) {
           pend = run_join(pend, gf);
         } else {
@@ -905,7 +1055,7 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
         }
       } else {
         if (pending) flush(g);
-        cur = walk_group(rv, x, y, blocks, sums, tiles, j, g, cur, false);
+        cur = walk_group(rv, x, y, blocks, sums, tiles, recs, j, g, cur, false);
       }
     }
   }
@@ -991,23 +1141,34 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
   const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
   if (tiles >= kStreamMinTiles) {
     const uint64_t groups = (tiles + kGroupTiles - 1) / kGroupTiles;
+    const uint64_t pairs = tiles * 16;
+    const uint32_t rcap = static_cast<uint32_t>(pairs < 8192 + pairs / 8 ? pairs : 8192 + pairs / 8);
     double *asum = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
     double *pre = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
     auto *sums = static_cast<TileSum *>(tmp_alloc(c, tiles * 16 * sizeof(TileSum)));
     auto *gsum = static_cast<GroupSum *>(tmp_alloc(c, groups * 16 * sizeof(GroupSum)));
+    auto *recs = static_cast<RunRec *>(tmp_alloc(c, uint64_t(rcap) * kWave * sizeof(RunRec)));
+    auto *list = static_cast<uint32_t *>(tmp_alloc(c, tiles * 4 + 8));
+    uint32_t *counts = list + tiles;  // [0] listed tiles, [1] record slots taken
+    const uint32_t fix_grid = static_cast<uint32_t>(tiles < 512 ? tiles : 512);
     k_canon_sample<RV><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y, blocks, tiles, asum);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
-    k_canon_summ<RV, false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum);
+    k_canon_summ<RV><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum,
+                                                                    counts);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
-    k_canon_summ<RV, true><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum);
+    k_canon_check<<<uint32_t((tiles + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(pre, tiles, sums, list, counts);
+    k_canon_fix<RV><<<fix_grid, kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, list, counts, recs,
+                                                            rcap);
     k_canon_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(sums, tiles, groups,
                                                                                               gsum);
-    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, gsum, groups, acc);
+    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, recs, gsum, groups, acc);
     check_hip(hipGetLastError(), "canon streamed");
     tmp_free(c, asum);
     tmp_free(c, pre);
     tmp_free(c, sums);
     tmp_free(c, gsum);
+    tmp_free(c, recs);
+    tmp_free(c, list);
     return;
   }
   uint64_t *status = status_words(c, ST_CANON, tiles * kStride);

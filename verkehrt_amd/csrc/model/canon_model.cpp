@@ -185,16 +185,17 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
 // under one hypothesis; the walk joins consecutive clean groups of one hypothesis and applies
 // the joined summary to the exact value (group by group, tile by tile where that fails), and
 // splits other groups into segments of usable same-hypothesis tiles, each applied once or
-// tile by tile, a tile that fails evaluated by the fallback wave.
-// stats: [0] joined clean spans applied, [1] spans that failed, [2] segments applied,
-// [3] tiles evaluated by the fallback.
+// tile by tile, a tile that fails evaluated through its run records (with `records`) or by the
+// fallback wave.
+// stats: [0] joined clean spans applied, [1] spans that failed, [2] segments applied, [3] tiles
+// whose summary did not apply, [4] run segments applied, [5] runs folded element by element.
 extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, double *acc,
-                                           int group_tiles, uint64_t *stats) {
+                                           int group_tiles, int records, uint64_t *stats) {
   const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
   if (group_tiles < 1) group_tiles = 1;
   const uint64_t gt = static_cast<uint64_t>(group_tiles);
   const uint64_t groups = (tiles + gt - 1) / gt;
-  uint64_t st[4] = {0, 0, 0, 0}, rounds = 0;
+  uint64_t st[6] = {0, 0, 0, 0, 0, 0}, rounds = 0;
   for (int j = 0; j < 16; ++j) {
     // 1. approximate tile sums as the first summary pass forms them: each run's elements in
     //    order (from 0.0, missing elements as 0.0), runs in fours, fours in fours, and so on
@@ -253,10 +254,52 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
       fn[t] = ft;
       usable[t] = run_apply(ft, h, pre[t], &probe);
     }
+    // a tile whose summary does not apply: by its run records (each run summarised under the
+    // binade of its own approximate start, the tile's plus the run sums before it; segments of
+    // usable runs under one hypothesis applied at once, then run by run, then element by
+    // element), or, without records, by the fallback wave
+    auto by_runs = [&](uint64_t t, double cur) {
+      const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
+      const uint64_t nruns = (b1 - b0 + kRun - 1) / kRun;
+      std::vector<RunFn> rf(nruns);
+      std::vector<Binade> rh(nruns);
+      std::vector<char> ru(nruns);
+      double rs = pre[t];
+      for (uint64_t r = 0; r < nruns; ++r) {
+        const uint64_t rb = b0 + r * kRun;
+        rh[r] = binade_of(rs);
+        rf[r] = rh[r].ok ? run_at(vals, rb, b1, j, binade_ref(rh[r])) : run_identity();
+        double probe;
+        ru[r] = run_apply(rf[r], rh[r], rs, &probe);
+        for (uint64_t b = rb; b < std::min(b1, rb + kRun); ++b) rs = rs + vals[b * 16 + j];
+      }
+      auto fold = [&](uint64_t r, double c) {
+        ++st[5];
+        for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b) c = c + vals[b * 16 + j];
+        return c;
+      };
+      for (uint64_t s0 = 0; s0 < nruns;) {
+        uint64_t e = s0 + 1;
+        if (ru[s0])
+          while (e < nruns && ru[e] && same_binade(rh[e], rh[s0])) ++e;
+        RunFn f = run_identity();
+        for (uint64_t r = s0; r < e; ++r) f = run_join(f, rf[r]);
+        double out;
+        if (ru[s0] && run_apply(f, rh[s0], cur, &out)) {
+          ++st[4];
+          cur = out;
+        } else {
+          for (uint64_t r = s0; r < e; ++r) cur = ru[r] && run_apply(rf[r], rh[r], cur, &out) ? out : fold(r, cur);
+        }
+        s0 = e;
+      }
+      return cur;
+    };
     auto one = [&](uint64_t t, double cur) {
       double out;
       if (usable[t] && run_apply(fn[t], hyp[t], cur, &out)) return out;
       ++st[3];
+      if (records) return by_runs(t, cur);
       const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
       return eval_chain(vals, b0, b1, j, cur, &rounds);
     };
@@ -323,6 +366,6 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
     acc[j] = cur;
   }
   if (stats)
-    for (int k = 0; k < 4; ++k) stats[k] = st[k];
+    for (int k = 0; k < 6; ++k) stats[k] = st[k];
   return 0;
 }

@@ -188,6 +188,17 @@ __device__ __forceinline__ T shfl_up_pod(const T &v, int delta) {
   __builtin_memcpy(&r, out.w, sizeof(T));
   return r;
 }
+// v from lane `lane`, the same lane for the whole wave (readlane: no LDS round trip).
+template <typename T>
+__device__ __forceinline__ T bcast_pod(const T &v, int lane) {
+  PodWords<T> in, out;
+  __builtin_memcpy(in.w, &v, sizeof(T));
+#pragma unroll
+  for (int k = 0; k < PodWords<T>::N; ++k) out.w[k] = __builtin_amdgcn_readlane(in.w[k], lane);
+  T r;
+  __builtin_memcpy(&r, out.w, sizeof(T));
+  return r;
+}
 template <typename T>
 __device__ __forceinline__ T shfl_pod(const T &v, int src_lane) {
   PodWords<T> in, out;
