@@ -640,9 +640,9 @@ __global__ __launch_bounds__(kBlock) void k_canon_sample(RV rv, const double *__
   if (t < 16) asum[t * tiles + tile] = 16.0 * ((s_a[0][t] + s_a[1][t]) + (s_a[2][t] + s_a[3][t]));
 }
 
-// Prefix: one block per chain: thread p sums part p of the tiles (ceil(tiles / 1024) each), the part
-//    sums are scanned in LDS (Hillis-Steele), then each thread writes its tiles' approximate
-//    starts, pre[tile * 16 + j].
+// Prefix: one block per chain: thread p sums part p of the tiles (ceil(tiles / 1024) each), the
+//    part sums are scanned, then each thread writes its tiles' approximate starts,
+//    pre[tile * 16 + j].
 constexpr int kPrefixParts = 1024;
 __global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asum, uint64_t tiles,
                                                                const double *acc_in, double *pre) {
@@ -662,14 +662,31 @@ __global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asu
     for (int m = 0; m < 8; ++m) sum = sum + u[m];
   }
   for (; k < t1; ++k) sum = sum + a[k];
-  s_part[p] = sum;
-  __syncthreads();
-  for (int d = 1; d < kPrefixParts; d <<= 1) {
-    const double o = p >= d ? s_part[p - d] : 0.0;
-    __syncthreads();
-    if (p >= d) s_part[p] = s_part[p] + o;
-    __syncthreads();
+  // inclusive scan of the part sums: within each wave (Hillis-Steele over lanes), then the
+  // wave totals (Hillis-Steele over the 16 waves), each part adding the totals before its wave
+  const int l = lane_id(), w = p / kWave;
+  double inc = sum;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double o = __shfl_up(inc, d);
+    if (l >= d) inc = o + inc;
   }
+  __shared__ double s_wave[kPrefixParts / kWave];
+  if (l == kWave - 1) s_wave[w] = inc;
+  __syncthreads();
+  if (p < kWave) {
+    double t = p < kPrefixParts / kWave ? s_wave[p] : 0.0;
+#pragma unroll
+    for (int d = 1; d < kPrefixParts / kWave; d <<= 1) {
+      const double o = __shfl_up(t, d);
+      if (p >= d) t = o + t;
+    }
+    if (p < kPrefixParts / kWave) s_wave[p] = t;
+  }
+  __syncthreads();
+  if (w > 0) inc = s_wave[w - 1] + inc;
+  s_part[p] = inc;
+  __syncthreads();
   double run = acc_in[j] + (p > 0 ? s_part[p - 1] : 0.0);
   k = t0;
   for (; k + 8 <= t1; k += 8) {
@@ -740,15 +757,17 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double
   }
 }
 
-// 4. One thread per tile: a chain whose stored hypothesis is the binade of its approximate
+// 4. One thread per tile and chain: a chain whose stored hypothesis is the binade of its approximate
 // start only has its usability re-checked there; a tile with a chain whose hypothesis differs,
 // or with a chain whose summary is not usable, is listed for step 5.
 __global__ __launch_bounds__(kBlock) void k_canon_check(const double *pre, uint64_t tiles, TileSum *sums,
                                                         uint32_t *list, uint32_t *counts) {
-  const uint64_t tile = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
-  if (tile >= tiles) return;
+  // thread (tile, chain j): j = t % 16, so a tile's chains are 16 lanes of one wave
+  const uint64_t g = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+  const uint64_t tile = g / 16;
+  const int j = static_cast<int>(g % 16), l = lane_id();
   bool listed = false;
-  for (int j = 0; j < 16; ++j) {
+  if (tile < tiles) {
     TileSum *ts = sums + uint64_t(j) * tiles + tile;
     const double start = pre[tile * 16 + j];
     const Binade h = binade_of(start);
@@ -758,12 +777,13 @@ __global__ __launch_bounds__(kBlock) void k_canon_check(const double *pre, uint6
       const bool usable = run_apply(fn_of(ts->d0, ts->d1, ts->lo0, ts->lo1, ts->hi0, ts->hi1), h, start, &probe);
       const uint64_t nw = pack_binade(h) | (usable ? kUsableBit : 0);
       if (nw != hw) ts->hword = nw;
-      listed = listed || !usable;
+      listed = !usable;
     } else {
       listed = true;
     }
   }
-  if (listed) list[atomicAdd(counts, 1u)] = static_cast<uint32_t>(tile);
+  const uint64_t any = __ballot(listed);
+  if (tile < tiles && j == 0 && ((any >> (l & ~15)) & 0xFFFF)) list[atomicAdd(counts, 1u)] = static_cast<uint32_t>(tile);
 }
 
 // 5. The listed tiles, one block each (a grid-stride loop over the list): chains whose
@@ -1011,7 +1031,9 @@ __device__ double walk_group(const RV &rv, const double *x, const double *y, uin
   });
 }
 
-// 7. One wave per chain.
+// 7. One wave per chain: groups 64 at a time through walk_lanes (lane = group; a clean group's
+// joined summary is usable under its hypothesis), an unclean group, or a clean one whose
+// summary fails, through walk_group.
 template <class RV>
 __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__restrict__ x,
                                                       const double *__restrict__ y, uint64_t blocks,
@@ -1019,47 +1041,19 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
                                                       const GroupSum *gsum, uint64_t groups, double *acc) {
   const int j = blockIdx.x, l = lane_id();
   double cur = acc[j];
-  RunFn pend = run_identity();  // joined clean groups since `cur`
-  uint64_t pend_h = 0, pend_g0 = 0;
-  bool pending = false;
-  auto flush = [&](uint64_t g_end) {
-    double out;
-    if (run_apply(pend, unpack_binade(pend_h), cur, &out)) {
-      cur = out;
-    } else {
-      for (uint64_t gg = pend_g0; gg < g_end; ++gg)
-        cur = walk_group(rv, x, y, blocks, sums, tiles, recs, j, gg, cur, true);
-    }
-    pending = false;
-  };
   for (uint64_t g0 = 0; g0 < groups; g0 += kWave) {
-    // lane l holds group g0 + l
-    GroupSum mine{};
-    if (g0 + l < groups) mine = gsum[uint64_t(j) * groups + g0 + l];
-    const int cnt = groups - g0 < uint64_t(kWave) ? static_cast<int>(groups - g0) : kWave;
-    for (int k = 0; k < cnt; ++k) {
-      const uint64_t g = g0 + k;
-      const uint64_t clean = bcast_pod(mine.clean, k);
-      const uint64_t hw = bcast_pod(mine.hword, k);
-      if (clean) {
-        const RunFn gf = fn_of(bcast_pod(mine.d0, k), bcast_pod(mine.d1, k), bcast_pod(mine.lo0, k),
-                               bcast_pod(mine.lo1, k), bcast_pod(mine.hi0, k), bcast_pod(mine.hi1, k));
-        if (pending && hw == pend_h) {
-          pend = run_join(pend, gf);
-        } else {
-          if (pending) flush(g);
-          pend = gf;
-          pend_h = hw;
-          pend_g0 = g;
-          pending = true;
-        }
-      } else {
-        if (pending) flush(g);
-        cur = walk_group(rv, x, y, blocks, sums, tiles, recs, j, g, cur, false);
-      }
+    const int n = groups - g0 < uint64_t(kWave) ? static_cast<int>(groups - g0) : kWave;
+    RunFn f = run_identity();
+    uint64_t hw = 0;
+    if (l < n) {
+      const GroupSum gs = gsum[uint64_t(j) * groups + g0 + l];
+      f = fn_of(gs.d0, gs.d1, gs.lo0, gs.lo1, gs.hi0, gs.hi1);
+      hw = gs.hword | (gs.clean ? kUsableBit : 0);
     }
+    cur = walk_lanes(f, hw, n, false, cur, [&](int i, double c) {
+      return walk_group(rv, x, y, blocks, sums, tiles, recs, j, g0 + i, c, false);
+    });
   }
-  if (pending) flush(groups);
   if (l == 0) acc[j] = cur;
 }
 
@@ -1156,7 +1150,8 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
     k_canon_summ<RV><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum,
                                                                     counts);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
-    k_canon_check<<<uint32_t((tiles + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(pre, tiles, sums, list, counts);
+    k_canon_check<<<uint32_t((tiles * 16 + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(pre, tiles, sums, list,
+                                                                                       counts);
     k_canon_fix<RV><<<fix_grid, kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, list, counts, recs,
                                                             rcap);
     k_canon_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(sums, tiles, groups,

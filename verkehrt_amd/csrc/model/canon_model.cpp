@@ -182,10 +182,10 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
 // tile sums (summed in the device's order), their exclusive prefix from the start state (in 1024 parts,
 // as the prefix kernel does), tile summaries under the binade of that prefix, usable
 // where they apply at it; groups of `group_tiles` tiles are clean when every tile is usable
-// under one hypothesis; the walk joins consecutive clean groups of one hypothesis and applies
-// the joined summary to the exact value (group by group, tile by tile where that fails), and
-// splits other groups into segments of usable same-hypothesis tiles, each applied once or
-// tile by tile, a tile that fails evaluated through its run records (with `records`) or by the
+// under one hypothesis; the walk takes 64 groups at a time, joins consecutive clean groups of
+// one hypothesis and applies the joined summary to the exact value (group by group where that
+// fails), and splits other groups into segments of usable same-hypothesis tiles, each applied
+// once or tile by tile, a tile that fails evaluated through its run records (with `records`) or by the
 // fallback wave.
 // stats: [0] joined clean spans applied, [1] spans that failed, [2] segments applied, [3] tiles
 // whose summary did not apply, [4] run segments applied, [5] runs folded element by element.
@@ -217,17 +217,27 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
       for (int i = 0; i < 4; ++i) q[i] = (w[4 * i] + w[4 * i + 1]) + (w[4 * i + 2] + w[4 * i + 3]);
       asum[t] = (q[0] + q[1]) + (q[2] + q[3]);
     }
-    // 2. exclusive prefix in 1024 parts, the part sums scanned as the device's Hillis-Steele
+    // 2. exclusive prefix in 1024 parts, the part sums scanned as the device does: Hillis-Steele
+    //    within each wave of 64 parts, then over the 16 wave totals, which are added after
     std::vector<double> pre(tiles);
     const uint64_t parts = 1024, per = (tiles + parts - 1) / parts;
     std::vector<double> psum(parts, 0.0);
     for (uint64_t p = 0; p < parts; ++p)
       for (uint64_t k = std::min(tiles, p * per); k < std::min(tiles, (p + 1) * per); ++k) psum[p] = psum[p] + asum[k];
-    for (uint64_t d = 1; d < parts; d <<= 1) {
-      std::vector<double> nxt(psum);
-      for (uint64_t p = d; p < parts; ++p) nxt[p] = psum[p] + psum[p - d];
-      psum = nxt;
+    auto hs = [](double *v, int n) {
+      for (int d = 1; d < n; d <<= 1) {
+        std::vector<double> nxt(v, v + n);
+        for (int i = d; i < n; ++i) nxt[i] = v[i - d] + v[i];
+        std::copy(nxt.begin(), nxt.end(), v);
+      }
+    };
+    double tot[16];
+    for (int w = 0; w < 16; ++w) {
+      hs(&psum[64 * w], 64);
+      tot[w] = psum[64 * w + 63];
     }
+    hs(tot, 16);
+    for (uint64_t p = 64; p < parts; ++p) psum[p] = tot[p / 64 - 1] + psum[p];
     for (uint64_t p = 0; p < parts; ++p) {
       double run = acc[j] + (p > 0 ? psum[p - 1] : 0.0);
       for (uint64_t k = std::min(tiles, p * per); k < std::min(tiles, (p + 1) * per); ++k) {
@@ -322,23 +332,12 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
       }
       return cur;
     };
-    // 4-5. clean groups and the walk
+    // 4-5. clean groups and the walk: 64 groups at a time, segments of consecutive clean groups
+    //      of one hypothesis joined and applied, a failing segment group by group, a group that
+    //      is not clean or fails through walk_group
     double cur = acc[j];
-    RunFn pend = run_identity();
-    Binade pend_h{};
-    uint64_t pend_g0 = 0;
-    bool pending = false;
-    auto flush = [&](uint64_t g_end) {
-      double out;
-      if (run_apply(pend, pend_h, cur, &out)) {
-        ++st[0];
-        cur = out;
-      } else {
-        ++st[1];
-        for (uint64_t g = pend_g0; g < g_end; ++g) cur = walk_group(g, cur, true);
-      }
-      pending = false;
-    };
+    std::vector<RunFn> gfn(groups);
+    std::vector<char> gclean(groups);
     for (uint64_t g = 0; g < groups; ++g) {
       const uint64_t t0 = g * gt, t1 = std::min(tiles, t0 + gt);
       bool clean = true;
@@ -347,22 +346,29 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
         clean = clean && usable[t] && same_binade(hyp[t], hyp[t0]);
         gf = run_join(gf, fn[t]);
       }
-      if (clean) {
-        if (pending && same_binade(hyp[t0], pend_h)) {
-          pend = run_join(pend, gf);
+      gfn[g] = gf;
+      gclean[g] = clean;
+    }
+    for (uint64_t b0 = 0; b0 < groups; b0 += 64) {
+      const uint64_t b1 = std::min(groups, b0 + 64);
+      for (uint64_t s0 = b0; s0 < b1;) {
+        uint64_t e = s0 + 1;
+        if (gclean[s0])
+          while (e < b1 && gclean[e] && same_binade(hyp[e * gt], hyp[s0 * gt])) ++e;
+        RunFn f = run_identity();
+        for (uint64_t g = s0; g < e; ++g) f = run_join(f, gfn[g]);
+        double out;
+        if (gclean[s0] && run_apply(f, hyp[s0 * gt], cur, &out)) {
+          ++st[0];
+          cur = out;
         } else {
-          if (pending) flush(g);
-          pend = gf;
-          pend_h = hyp[t0];
-          pend_g0 = g;
-          pending = true;
+          if (gclean[s0]) ++st[1];
+          for (uint64_t g = s0; g < e; ++g)
+            cur = gclean[g] && run_apply(gfn[g], hyp[g * gt], cur, &out) ? out : walk_group(g, cur, false);
         }
-      } else {
-        if (pending) flush(g);
-        cur = walk_group(g, cur, false);
+        s0 = e;
       }
     }
-    if (pending) flush(groups);
     acc[j] = cur;
   }
   if (stats)
