@@ -55,13 +55,25 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
   return sum;
 }
 
+// The comparison outcome of v against c as one bit: 1 less, 2 equal, 4 greater, 8 unordered.
+template <typename T>
+__device__ __forceinline__ uint32_t outcome(T v, T c) {
+  return v < c ? 1u : v == c ? 2u : v > c ? 4u : 8u;
+}
+
 // kind: 0 compress (keep = mask != 0), 1 filter (keep = x cmp c), 2 where (value base + row).
+// `keepset` is the set of comparison outcomes a filter keeps (outcome()).
 // BATCH: the tile's item (row pointers, row count, output, first tile, index base) comes
 // from the batch tables, and the item's last tile writes its count to counts[item].
+// All of a thread's loads are issued before the first ballot (a ballot is convergent, so
+// loads interleaved with ballots are not hoisted and would complete one at a time); rows past
+// the end of the last tile load the last row and are never kept, so no load is predicated.
+// Two blocks per compute unit (8 waves per SIMD, at most 64 VGPRs) keep one block loading while
+// the other waits on its look-back.
 template <typename T, int KIND, bool BATCH>
-__global__ __launch_bounds__(kSelBlock) void k_select(const T *__restrict__ x,
+__global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
-                                                   int cmp, T c, uint64_t base, void *out_v,
+                                                   uint32_t keepset, T c, uint64_t base, void *out_v,
                                                    uint64_t *status, uint32_t epoch,
                                                    uint32_t *ticket, uint32_t ntiles,
                                                    uint64_t *d_count, const BatchItem *items,
@@ -83,31 +95,33 @@ __global__ __launch_bounds__(kSelBlock) void k_select(const T *__restrict__ x,
   }
   Out *out = static_cast<Out *>(out_v);
   const uint64_t t0 = uint64_t(tile - first) * kTile;
+  const uint64_t rem = n - t0;
+  const uint32_t lim = rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);  // rows of the tile
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
 
   Out val[kItems];
+  uint8_t mv[kItems];
   uint64_t keepm[kItems];
   __shared__ uint32_t s_cnt[kItems][kWaves];
   __shared__ uint32_t s_excl[kItems][kWaves];
   __shared__ uint64_t s_prefix;
+  // item i of thread t is row t0 + i * kSelBlock + t: every wave load is 64 consecutive rows
+  {
+    const T *xt = KIND == 2 ? nullptr : x + t0;
+    const uint8_t *mt = KIND == 1 ? nullptr : mask + t0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const uint32_t lr = uint32_t(i) * kSelBlock + t;
+      const uint32_t rr = lr < lim ? lr : lim - 1;
+      if (KIND != 1) mv[i] = mt[rr];
+      if (KIND != 2) val[i] = static_cast<Out>(xt[rr]);
+    }
+  }
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const uint64_t r = t0 + uint64_t(i) * kSelBlock + t;
-    bool keep = false;
-    if (r < n) {
-      if (KIND == 0) {
-        keep = mask[r] != 0;
-        val[i] = static_cast<Out>(x[r]);
-      } else if (KIND == 1) {
-        T v = x[r];
-        keep = cmp_test(cmp, v, c);
-        val[i] = static_cast<Out>(v);
-      } else {
-        keep = mask[r] != 0;
-        val[i] = static_cast<Out>(base + r);
-      }
-    }
-    keepm[i] = __ballot(keep);
+    const uint32_t lr = uint32_t(i) * kSelBlock + t;
+    const bool keep = KIND == 1 ? (keepset & outcome(static_cast<T>(val[i]), c)) != 0 : mv[i] != 0;
+    keepm[i] = __ballot(keep && lr < lim);
     if (l == 0) s_cnt[i][w] = __popcll(keepm[i]);
   }
   __syncthreads();
@@ -151,11 +165,21 @@ __global__ __launch_bounds__(kSelBlock) void k_select(const T *__restrict__ x,
     }
   }
   __syncthreads();
-  const uint64_t prefix = s_prefix;
+  Out *ot = out + s_prefix;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    if ((keepm[i] >> l) & 1) out[prefix + s_excl[i][w] + rank_below(keepm[i])] = val[i];
+    if ((keepm[i] >> l) & 1) {
+      const uint32_t at = s_excl[i][w] + rank_below(keepm[i]);
+      if (KIND == 2) ot[at] = static_cast<Out>(base + t0 + uint32_t(i) * kSelBlock + t);
+      else ot[at] = val[i];
+    }
   }
+}
+
+// Comparison codes 0 eq, 1 ne, 2 lt, 3 le, 4 gt, 5 ge as the outcome sets they keep.
+inline uint32_t keepset_of(int cmp) {
+  static const uint32_t sets[6] = {2, 1 | 4 | 8, 1, 1 | 2, 4, 4 | 2};
+  return sets[cmp >= 0 && cmp < 6 ? cmp : 5];
 }
 
 template <typename T, int KIND>
@@ -168,8 +192,8 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
   k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
-      static_cast<const T *>(x), mask, n, cmp, cv, base, out, status, epoch, c.scratch.counters,
-      static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr);
+      static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, status, epoch,
+      c.scratch.counters, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr);
   check_hip(hipGetLastError(), "select launch");
 }
 
@@ -182,7 +206,7 @@ void launch_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, ui
     T cv;
     __builtin_memcpy(&cv, &s.cbits, sizeof(T));
     k_select<T, KIND, true><<<p.ntiles, kSelBlock, 0, c.stream>>>(
-        nullptr, nullptr, 0, s.cmp, cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
+        nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
         nullptr, p.items, p.tile_item, d_counts);
     check_hip(hipGetLastError(), "select batch launch");
   }
