@@ -154,18 +154,23 @@ __host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, 
   const double inf = __builtin_inf();
   double d = 0.0, lo = inf, hi = -inf;
   bool tie = false, bad = false;
+  auto one = [&](double x) {
+    const double w = (r.z0 + x) - r.z0;
+    const double e = x - w;
+    tie = tie || (e < 0 ? -e : e) == r.half_u;
+    d = d + w * r.inv_u;
+    lo = __builtin_fmin(lo, d);
+    hi = __builtin_fmax(hi, d);
+    const double ax = x < 0 ? -x : x;
+    bad = bad || !(ax < r.lim);
+  };
+  if (cnt >= N) {  // a whole run: straight-line code
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    if (k < cnt) {
-      const double w = (r.z0 + v[k]) - r.z0;
-      const double e = v[k] - w;
-      tie = tie || (e < 0 ? -e : e) == r.half_u;
-      d = d + w * r.inv_u;
-      lo = __builtin_fmin(lo, d);
-      hi = __builtin_fmax(hi, d);
-      const double av = v[k] < 0 ? -v[k] : v[k];
-      bad = bad || !(av < r.lim);
-    }
+    for (int k = 0; k < N; ++k) one(v[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      if (k < cnt) one(v[k]);
   }
   if (__builtin_expect(!tie, 1)) {
     RunAcc a{RunFn{{d, d}, {lo, lo}, {hi, hi}}, {false, true}, bad};

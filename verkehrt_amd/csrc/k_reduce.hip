@@ -4,9 +4,9 @@
 // reduce.rs:379-412), first-index arg extremes (ArgExtreme, reduce.rs:414-468) and bit
 // folds (reduce.rs:531-575).
 //
-// One launch reads its rows once with 128-bit loads spread over the grid, reduces per
-// thread, per wave (DPP/permute shuffles), per block (LDS), writes one partial per block,
-// and the last block to finish folds the partials into the reducer's device state.
+// One launch reads its rows once with 128-bit loads, one batch of loads per block, reduces
+// per thread, per wave (shuffles) and per block (LDS) into one partial per block; a second,
+// one-block launch folds the partials into the reducer's device state.
 // HBM-bound: algorithmic bytes = n x width.
 #include "vk_device.h"
 #include "vk_internal.h"
@@ -72,10 +72,10 @@ struct OpSum {
 template <typename T>
 struct OpCount {
   using Acc = SumAcc;
-  int cmp;
+  uint32_t keepset;  // the comparison outcomes counted (outcome())
   T c;
   __device__ Acc identity() const { return {0}; }
-  __device__ void add(Acc &a, T v, uint64_t) const { a.s += cmp_test(cmp, v, c) ? 1 : 0; }
+  __device__ void add(Acc &a, T v, uint64_t) const { a.s += (keepset & outcome(v, c)) != 0 ? 1 : 0; }
   __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
 };
 
@@ -176,26 +176,91 @@ __device__ __forceinline__ typename Op::Acc block_merge(const Op &op, typename O
   return a;  // valid in wave 0
 }
 
-// Each block folds its grid-stride share into one partial; a one-block kernel merges the
-// partials into the reducer state. The kernel boundary orders the partials before the merge, so
-// no block waits on a counter (a same-address atomic per block costs ~11 ns, serialised).
-template <typename T, class Op>
-__global__ __launch_bounds__(kBlock) void k_reduce(Op op, const T *__restrict__ x, uint64_t n,
-                                                   typename Op::Acc *partials) {
+// One batch of loads per block: a block of 1024 threads reads 8 x 16 bytes per thread (128 KB,
+// all loads issued before the first is used) and folds them into one partial; the hardware
+// starts a new block as each one ends, which keeps more loads in flight than a loop inside
+// long-lived blocks (measured 6.4 against 5.5 TB/s for a count). Block 0 also folds the rows
+// before the first 16-byte boundary and after the last whole 16 bytes. A one-block kernel merges
+// the partials into the reducer state; the kernel boundary orders the partials before the merge.
+constexpr int kRedThreads = 1024;
+// 16-byte loads per thread: 8 for 8-byte rows, fewer for narrower rows so that a thread folds
+// at most 32 rows (wider batches of narrow rows spill their unpacked rows)
+template <typename T>
+constexpr int red_loads() { return sizeof(T) >= 8 ? 8 : sizeof(T) == 4 ? 4 : 2; }
+template <typename T>
+constexpr uint64_t red_vecs() { return uint64_t(kRedThreads) * red_loads<T>(); }  // per block
+
+template <class Op>
+__device__ __forceinline__ typename Op::Acc block_merge_big(const Op &op, typename Op::Acc a) {
   using Acc = typename Op::Acc;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) op.merge(a, shfl_xor_pod(a, m));
+  __shared__ Acc s_w[kRedThreads / kWave];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  if (l == 0) s_w[w] = a;
+  __syncthreads();
+  // the 16 wave results meet in lanes 0..15 of wave 0
+  if (w == 0) {
+    Acc b = l < kRedThreads / kWave ? s_w[l] : op.identity();
+#pragma unroll
+    for (int m = 1; m < kRedThreads / kWave; m <<= 1) op.merge(b, shfl_xor_pod(b, m));
+    a = b;
+  }
+  return a;  // valid in wave 0
+}
+
+template <typename T, class Op>
+__global__ __launch_bounds__(kRedThreads) void k_reduce(Op op, const T *__restrict__ x, uint64_t n,
+                                                        uint64_t head, typename Op::Acc *partials) {
+  using Acc = typename Op::Acc;
+  constexpr int V = 16 / sizeof(T);
+  const int t = threadIdx.x;
+  constexpr int kRedLoads = red_loads<T>();
+  constexpr uint64_t kRedVecs = red_vecs<T>();
+  const uint64_t nv = (n - head) / V;
+  const Vec16<T> *xv = reinterpret_cast<const Vec16<T> *>(x + head);
+  const uint64_t base = blockIdx.x * kRedVecs;
   Acc a = op.identity();
-  grid_rows(x, n, [&](T v, uint64_t row) { op.add(a, v, row); });
-  a = block_merge(op, a);
-  if (threadIdx.x == 0) partials[blockIdx.x] = a;
+  Vec16<T> v[kRedLoads];
+  if (base + kRedVecs <= nv) {
+#pragma unroll
+    for (int u = 0; u < kRedLoads; ++u) v[u] = xv[base + uint64_t(u) * kRedThreads + t];
+#pragma unroll
+    for (int u = 0; u < kRedLoads; ++u) {
+      const uint64_t i = base + uint64_t(u) * kRedThreads + t;
+#pragma unroll
+      for (int k = 0; k < V; ++k) op.add(a, v[u].e[k], head + i * V + k);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kRedLoads; ++u) {
+      const uint64_t i = base + uint64_t(u) * kRedThreads + t;
+      if (i < nv) v[u] = xv[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kRedLoads; ++u) {
+      const uint64_t i = base + uint64_t(u) * kRedThreads + t;
+      if (i < nv) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) op.add(a, v[u].e[k], head + i * V + k);
+      }
+    }
+  }
+  if (blockIdx.x == 0) {
+    if (uint64_t(t) < head) op.add(a, x[t], t);
+    for (uint64_t r = head + nv * V + t; r < n; r += kRedThreads) op.add(a, x[r], r);
+  }
+  a = block_merge_big(op, a);
+  if (t == 0) partials[blockIdx.x] = a;
 }
 
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_reduce_merge(Op op, const typename Op::Acc *partials,
-                                                         uint32_t nparts, typename Op::Acc *state) {
+__global__ __launch_bounds__(kRedThreads) void k_reduce_merge(Op op, const typename Op::Acc *partials,
+                                                              uint32_t nparts, typename Op::Acc *state) {
   using Acc = typename Op::Acc;
   Acc b = op.identity();
   for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) op.merge(b, partials[i]);
-  b = block_merge(op, b);
+  b = block_merge_big(op, b);
   if (threadIdx.x == 0) {
     Acc s = *state;
     op.merge(s, b);
@@ -203,16 +268,18 @@ __global__ __launch_bounds__(kBlock) void k_reduce_merge(Op op, const typename O
   }
 }
 
-constexpr uint32_t kReduceGrid = 2048;
-
 template <typename T, class Op>
 void launch(Ctx &c, const Op &op, const void *x, uint64_t n, void *state) {
   using Acc = typename Op::Acc;
-  const uint32_t per_block = kBlock * (16 / sizeof(T)) * 4;
-  const uint32_t grid = stream_grid(n, per_block, kReduceGrid);
+  uint64_t head = ((16 - (reinterpret_cast<uintptr_t>(x) & 15)) & 15) / sizeof(T);
+  if (head > n) head = n;
+  const uint64_t nv = (n - head) / (16 / sizeof(T));
+  const uint64_t blocks = nv == 0 ? 1 : (nv + red_vecs<T>() - 1) / red_vecs<T>();
+  if (blocks > 0xFFFFFFFFull) usage_error("reduction too large");
+  const uint32_t grid = static_cast<uint32_t>(blocks);
   Acc *parts = static_cast<Acc *>(partials(c, sizeof(Acc) * grid));
-  k_reduce<T, Op><<<grid, kBlock, 0, c.stream>>>(op, static_cast<const T *>(x), n, parts);
-  k_reduce_merge<Op><<<1, kBlock, 0, c.stream>>>(op, parts, grid, static_cast<Acc *>(state));
+  k_reduce<T, Op><<<grid, kRedThreads, 0, c.stream>>>(op, static_cast<const T *>(x), n, head, parts);
+  k_reduce_merge<Op><<<1, kRedThreads, 0, c.stream>>>(op, parts, grid, static_cast<Acc *>(state));
   check_hip(hipGetLastError(), "reduce launch");
 }
 
@@ -220,7 +287,7 @@ template <typename T>
 OpCount<T> make_count(int cmp, uint64_t cbits) {
   T c;
   __builtin_memcpy(&c, &cbits, sizeof(T));
-  return OpCount<T>{cmp, c};
+  return OpCount<T>{keepset_of(cmp), c};
 }
 
 __global__ void k_init_state(uint64_t *s, int words, uint64_t w0, uint64_t w1, uint64_t w2) {

@@ -3,22 +3,30 @@
 // min and max (IEEE minimum/maximum on floats, ScanExtreme scan.rs:164-183). A start byte
 // restarts the fold; row 0 of the whole array always starts.
 //
-// One pass: an 8192-row tile (512 threads x 16 rows) is staged through LDS into 16 consecutive rows per thread,
-// scanned per thread, per wave (shuffles) and per block, then joined to the tiles before
-// it by decoupled look-back on (aggregate, has-start) pairs: a predecessor whose aggregate
-// holds a start, or that published its inclusive value, ends the look-back. The operators
-// are associative and commutative, so any association order gives the reference's bits.
-// Algorithmic bytes: n x (in width [+1 start byte] + 8).
+// One pass over 8192-row tiles (512 threads x 16 rows), data in registers only: wave w of
+// a tile owns its 1024 consecutive rows, so every wave load and store covers 64 (or, with
+// 16-byte accesses, 128) consecutive rows. Each wave scans its rows item by item with DPP
+// row shifts and broadcasts (no LDS), carrying the running value from item to item; the
+// eight wave aggregates meet in wave 0, which publishes the tile aggregate and joins it to
+// the tiles before by decoupled look-back on (aggregate, has-start) pairs: a predecessor whose
+// aggregate holds a start, or that published its inclusive value, ends the look-back.
+// The fold starts from the operator's identity (0, +inf, -inf, or the integer bounds), which
+// is the same as starting at row 0; the operators are associative and commutative, so any
+// association order gives the reference's bits. Algorithmic bytes: n x (in width [+1 start
+// byte] + 8).
 #include "vk_device.h"
 #include "vk_internal.h"
+
+#include <limits>
 
 namespace vk {
 namespace {
 
-constexpr int kItems = 16;
 constexpr int kScanBlock = 512;
-constexpr int kTile = kScanBlock * kItems;  // 8192 rows: ~78 KB of LDS, two blocks per CU
+constexpr int kRows = 16;                     // rows per thread
+constexpr int kTile = kScanBlock * kRows;     // 8192 rows
 constexpr int kWaves = kScanBlock / kWave;
+constexpr int kWaveRows = kWave * kRows;      // 1024 rows per wave
 
 template <typename U, int OP>
 __device__ __forceinline__ U apply(U a, U b) {
@@ -27,17 +35,67 @@ __device__ __forceinline__ U apply(U a, U b) {
   else return max2(a, b);
 }
 
+template <typename U, int OP>
+__device__ __forceinline__ U identity() {
+  if constexpr (OP == 0) return U{};
+  else if constexpr (std::is_floating_point<U>::value)
+    return OP == 1 ? std::numeric_limits<U>::infinity() : -std::numeric_limits<U>::infinity();
+  else return OP == 1 ? std::numeric_limits<U>::max() : std::numeric_limits<U>::lowest();
+}
+
+// A span of rows: its fold, and whether a start lies in it (BY scans; always 0 otherwise).
 template <typename U>
 struct Seg {
   U v;
-  uint32_t f;  // a start lies in this span
-  uint32_t pad;
+  uint32_t f;
 };
 
+// a then b: b's value stands alone if b holds a start.
 template <typename U, int OP>
 __device__ __forceinline__ Seg<U> join(Seg<U> a, Seg<U> b) {
-  // a then b: b's value stands alone if b holds a start.
-  return Seg<U>{b.f ? b.v : apply<U, OP>(a.v, b.v), a.f | b.f, 0};
+  return Seg<U>{b.f ? b.v : apply<U, OP>(a.v, b.v), a.f | b.f};
+}
+
+// DPP move of every 32-bit word of a segment; lanes with no source (or masked rows) take `id`.
+template <int CTRL, int ROWS, typename U>
+__device__ __forceinline__ Seg<U> dpp_seg(Seg<U> x, Seg<U> id) {
+  static_assert(sizeof(U) == 8 || sizeof(U) == 4, "4- or 8-byte values");
+  constexpr int N = sizeof(U) / 4;
+  int xw[N], iw[N];
+  __builtin_memcpy(xw, &x.v, sizeof(U));
+  __builtin_memcpy(iw, &id.v, sizeof(U));
+  int ow[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) ow[k] = __builtin_amdgcn_update_dpp(iw[k], xw[k], CTRL, ROWS, 0xF, false);
+  Seg<U> r;
+  __builtin_memcpy(&r.v, ow, sizeof(U));
+  r.f = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(id.f), static_cast<int>(x.f), CTRL, ROWS, 0xF, false));
+  return r;
+}
+
+// Inclusive scan over the 64 lanes of a wave (lane order = row order): Kogge-Stone within
+// rows of 16 lanes (row_shr 1, 2, 4, 8), then row_bcast:15 into rows 1 and 3 and
+// row_bcast:31 into rows 2 and 3.
+template <typename U, int OP, bool BY>
+__device__ __forceinline__ Seg<U> wave_scan(Seg<U> x) {
+  const Seg<U> id{identity<U, OP>(), 0};
+  auto f = [&](Seg<U> o) { x = BY ? join<U, OP>(o, x) : Seg<U>{apply<U, OP>(o.v, x.v), 0}; };
+  f(dpp_seg<0x111, 0xF>(x, id));
+  f(dpp_seg<0x112, 0xF>(x, id));
+  f(dpp_seg<0x114, 0xF>(x, id));
+  f(dpp_seg<0x118, 0xF>(x, id));
+  f(dpp_seg<0x142, 0xA>(x, id));
+  f(dpp_seg<0x143, 0xC>(x, id));
+  return x;
+}
+
+// The value of lane 63 in every lane.
+template <typename U>
+__device__ __forceinline__ Seg<U> lane63(Seg<U> x) {
+  Seg<U> r;
+  r.v = bcast_pod(x.v, 63);
+  r.f = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x.f), 63));
+  return r;
 }
 
 // Status of a tile: [0] aggregate, [1] inclusive, [2] flag word
@@ -46,10 +104,10 @@ __device__ __forceinline__ bool stat_ready(uint64_t w, uint32_t epoch) {
   return (w >> 8) == epoch && (w & 3) != 0;
 }
 
+// The fold of all tiles before `tile` back to the nearest inclusive value or start, starting
+// from `acc` (the identity, or the carried value for tile 0's predecessors).
 template <typename U, int OP>
-__device__ U look_back(uint64_t *status, uint32_t epoch, uint32_t tile, bool *has_any) {
-  U acc{};
-  bool have = false;
+__device__ U look_back(uint64_t *status, uint32_t epoch, uint32_t tile, U acc) {
   int64_t end = tile;
   const int l = lane_id();
   while (end > 0) {
@@ -65,169 +123,199 @@ __device__ U look_back(uint64_t *status, uint32_t epoch, uint32_t tile, bool *ha
     const bool stop = idx < 0 || (w & 2) || (w & 4);
     const uint64_t stops = __ballot(stop);
     const int top = stops ? 63 - __builtin_clzll(stops) : -1;
-    U v{};
-    bool mine = l >= top && idx >= 0;
-    if (mine) {
-      uint64_t bits = (w & 2) ? ld_rlx(&status[idx * 4 + 1]) : ld_rlx(&status[idx * 4 + 0]);
+    U v = identity<U, OP>();
+    if (l >= top && idx >= 0) {
+      const uint64_t bits = (w & 2) ? ld_rlx(&status[idx * 4 + 1]) : ld_rlx(&status[idx * 4 + 0]);
       v = from_bits<U>(bits);
     }
-    // Fold the contributing lanes (any order: the operators commute).
-    const uint64_t contrib = __ballot(mine);
-    if (contrib) {
-      U r = v;
-      bool rh = mine;
+    // fold the contributing lanes (any order: the operators commute; only the top lane may
+    // hold a start, and it is the earliest)
 #pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) {
-        U o = shfl_xor_pod(r, m);
-        int oh = __shfl_xor(static_cast<int>(rh), m);
-        if (oh) r = rh ? apply<U, OP>(r, o) : o;
-        rh = rh || oh;
-      }
-      acc = have ? apply<U, OP>(acc, r) : r;
-      have = true;
-    }
+    for (int m = 32; m >= 1; m >>= 1) v = apply<U, OP>(v, shfl_xor_pod(v, m));
+    acc = apply<U, OP>(acc, v);
     if (top >= 0) break;
     end -= 64;
   }
-  *has_any = have;
   return acc;
 }
 
-template <typename TIn, typename U, int OP, bool BY>
+// VEC rows per access (1, or 2 with 16-byte output accesses when every pointer is aligned).
+template <typename TIn, typename U, int OP, bool BY, int VEC>
 __global__ __launch_bounds__(kScanBlock) void k_scan(const TIn *__restrict__ x,
                                                  const uint8_t *__restrict__ starts, uint64_t n,
                                                  U *__restrict__ out, uint64_t *status,
                                                  uint32_t epoch, uint32_t *ticket, uint32_t ntiles,
                                                  const uint64_t *carry_in, uint64_t *carry_out) {
-  __shared__ U s_v[kTile + kTile / 16];
-  __shared__ uint8_t s_f[kTile];
+  constexpr int L = kRows / VEC;  // accesses per thread
   __shared__ Seg<U> s_wave[kWaves];
-  __shared__ Seg<U> s_prefix;
+  __shared__ U s_pre[kWaves];
   const uint32_t tile = take_ticket(ticket, ntiles);
-  const uint64_t t0 = uint64_t(tile) * kTile;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  // carry from the previous update (tile contract, scan.rs:251-259)
-  const bool has_carry = carry_in[1] != 0;
-
-  // 1. coalesced load into LDS, padded one slot per 16 rows
+  const uint64_t t0 = uint64_t(tile) * kTile;
+  const uint64_t rem = n - t0;
+  const uint32_t lim = rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);
+  const bool full = lim == uint32_t(kTile);
+  // access k of lane l covers tile rows w * 1024 + k * 64 * VEC + l * VEC + (0 .. VEC - 1)
+  auto lrow = [&](int k, int e) -> uint32_t {
+    return uint32_t(w) * kWaveRows + uint32_t(k) * (kWave * VEC) + uint32_t(l) * VEC + e;
+  };
+  U v[kRows];
+  uint8_t f[kRows];
+  {
+    const TIn *xt = x + t0;
+    const uint8_t *st = BY ? starts + t0 : nullptr;
+    if (VEC == 2 && full) {
+      using In2 = struct alignas(2 * sizeof(TIn)) { TIn a, b; };
+      using St2 = struct alignas(2) { uint8_t a, b; };
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int lr = k * kScanBlock + t;
-    const uint64_t r = t0 + lr;
-    U v{};
-    uint8_t f = 0;
-    if (r < n) {
-      v = static_cast<U>(x[r]);
-      if (BY) f = starts[r] != 0;
-      if (r == 0 && !has_carry) f = 1;
-    }
-    s_v[lr + (lr >> 4)] = v;
-    s_f[lr] = f;
-  }
-  __syncthreads();
-  // 2. the aggregate of the thread's 16 consecutive rows (the rows stay in LDS)
-  Seg<U> run{U{}, 0, 0};
-#pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const int lr = t * kItems + j;
-    const Seg<U> e{s_v[lr + (lr >> 4)], s_f[lr], 0};
-    run = j == 0 ? e : join<U, OP>(run, e);
-  }
-  // 3. wave inclusive scan of thread aggregates
-  Seg<U> inc = run;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    Seg<U> o = shfl_up_pod(inc, d);
-    if (l >= d) inc = join<U, OP>(o, inc);
-  }
-  if (l == 63) s_wave[w] = inc;
-  Seg<U> excl_in_wave = shfl_up_pod(inc, 1);
-  __syncthreads();
-  // 4. tile aggregate, publish, look back (wave 0)
-  if (w == 0) {
-    Seg<U> agg = s_wave[0];
-    for (int k = 1; k < kWaves; ++k) agg = join<U, OP>(agg, s_wave[k]);
-    Seg<U> pre{U{}, 0, 0};
-    bool have = false;
-    if (tile == 0) {
-      if (has_carry) {
-        pre = Seg<U>{from_bits<U>(carry_in[0]), 0, 0};
-        have = true;
+      for (int k = 0; k < L; ++k) {
+        const uint32_t r = lrow(k, 0);
+        const In2 p = *reinterpret_cast<const In2 *>(xt + r);
+        v[2 * k] = static_cast<U>(p.a);
+        v[2 * k + 1] = static_cast<U>(p.b);
+        if (BY) {
+          const St2 q = *reinterpret_cast<const St2 *>(st + r);
+          f[2 * k] = q.a;
+          f[2 * k + 1] = q.b;
+        }
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < L; ++k)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          const uint32_t r = lrow(k, e);
+          const uint32_t rr = r < lim ? r : lim - 1;
+          v[k * VEC + e] = static_cast<U>(xt[rr]);
+          if (BY) f[k * VEC + e] = st[rr];
+        }
+    }
+  }
+  // rows past the end are identities (they only ever follow the valid rows)
+  if (!full) {
+#pragma unroll
+    for (int k = 0; k < L; ++k)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+        if (lrow(k, e) >= lim) {
+          v[k * VEC + e] = identity<U, OP>();
+          if (BY) f[k * VEC + e] = 0;
+        }
+  }
+  // wave scan, item by item, carrying the wave's running value; v[] becomes the inclusive
+  // fold from the wave's first row (flag in f[]: a start since the wave's first row)
+  Seg<U> carry{identity<U, OP>(), 0};
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    Seg<U> e0{v[k * VEC], BY ? uint32_t(f[k * VEC] != 0) : 0u};
+    Seg<U> lane = e0;
+    if (VEC == 2) lane = join<U, OP>(e0, Seg<U>{v[k * VEC + 1], BY ? uint32_t(f[k * VEC + 1] != 0) : 0u});
+    const Seg<U> inc = wave_scan<U, OP, BY>(lane);
+    if (VEC == 2) {
+      // row 0 of the pair: the lanes before (wave_shr:1 of the inclusive) then e0
+      const Seg<U> id{identity<U, OP>(), 0};
+      const Seg<U> excl = dpp_seg<0x138, 0xF>(inc, id);
+      const Seg<U> r0 = join<U, OP>(carry, join<U, OP>(excl, e0));
+      v[k * VEC] = r0.v;
+      if (BY) f[k * VEC] = uint8_t(r0.f);
+    }
+    const Seg<U> r1 = join<U, OP>(carry, inc);
+    v[k * VEC + VEC - 1] = r1.v;
+    if (BY) f[k * VEC + VEC - 1] = uint8_t(r1.f);
+    carry = lane63(r1);
+  }
+  if (l == 0) s_wave[w] = carry;
+  __syncthreads();
+  // wave 0: the tile aggregate, publish, look back, then the prefix of every wave
+  if (w == 0) {
+    const Seg<U> id{identity<U, OP>(), 0};
+    Seg<U> a = l < kWaves ? s_wave[l] : id;
+    const Seg<U> inc = wave_scan<U, OP, BY>(a);  // lane l: waves 0..l
+    const Seg<U> agg = lane63(inc);
+    U pre = identity<U, OP>();
+    if (tile == 0) {
+      if (carry_in[1] != 0) pre = from_bits<U>(carry_in[0]);
     } else {
       if (l == 0) {
         st_rlx(&status[uint64_t(tile) * 4 + 0], to_bits(agg.v));
         publish_fence();
         st_rlx(&status[uint64_t(tile) * 4 + 2], (uint64_t(epoch) << 8) | 1 | (agg.f ? 4 : 0));
       }
-      bool h;
-      U pv = look_back<U, OP>(status, epoch, tile, &h);
-      pre = Seg<U>{pv, 0, 0};
-      have = h;
+      pre = look_back<U, OP>(status, epoch, tile, pre);
     }
-    const Seg<U> incl = have ? join<U, OP>(pre, agg) : agg;
+    const Seg<U> incl = join<U, OP>(Seg<U>{pre, 0}, agg);
     if (l == 0) {
       st_rlx(&status[uint64_t(tile) * 4 + 1], to_bits(incl.v));
       publish_fence();
       st_rlx(&status[uint64_t(tile) * 4 + 2], (uint64_t(epoch) << 8) | 2 | 1 | (agg.f ? 4 : 0));
-      s_prefix = pre;
-      s_prefix.f = have ? 1 : 0;  // here f marks "a prefix exists"
     }
+    // wave l's prefix: the tile prefix joined with waves 0 .. l - 1
+    const Seg<U> excl = dpp_seg<0x138, 0xF>(inc, id);
+    if (l < kWaves) s_pre[l] = join<U, OP>(Seg<U>{pre, 0}, excl).v;
   }
   __syncthreads();
-  // 5. this thread's exclusive prefix = tile prefix, waves before, lanes before
-  Seg<U> pre{U{}, 0, 0};
-  bool have = s_prefix.f != 0;
-  if (have) pre = Seg<U>{s_prefix.v, 0, 0};
-  for (int k = 0; k < w; ++k) {
-    pre = have ? join<U, OP>(pre, s_wave[k]) : s_wave[k];
-    have = true;
-  }
-  if (l > 0) {
-    pre = have ? join<U, OP>(pre, excl_in_wave) : excl_in_wave;
-    have = true;
-  }
-  // the thread's rows again, folded from the prefix (a start restarts the fold)
-  U cur = pre.v;
+  const U wp = s_pre[w];
+  U *ot = out + t0;
+  auto finish = [&](int i) -> U { return BY && f[i] ? v[i] : apply<U, OP>(wp, v[i]); };
+  if (VEC == 2 && full) {
+    using Out2 = struct alignas(2 * sizeof(U)) { U a, b; };
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const int lr = t * kItems + j;
-    const U v = s_v[lr + (lr >> 4)];
-    cur = (s_f[lr] || (j == 0 && !have)) ? v : apply<U, OP>(cur, v);
-    s_v[lr + (lr >> 4)] = cur;
-  }
-  __syncthreads();
+    for (int k = 0; k < L; ++k) {
+      Out2 o{finish(2 * k), finish(2 * k + 1)};
+      *reinterpret_cast<Out2 *>(ot + lrow(k, 0)) = o;
+    }
+  } else {
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int lr = k * kScanBlock + t;
-    const uint64_t r = t0 + lr;
-    if (r < n) {
-      const U v = s_v[lr + (lr >> 4)];
-      out[r] = v;
-      if (r == n - 1) {
-        carry_out[0] = to_bits(v);
-        carry_out[1] = 1;
+    for (int k = 0; k < L; ++k)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        const uint32_t r = lrow(k, e);
+        if (r < lim) ot[r] = finish(k * VEC + e);
       }
-    }
   }
+  // the carry for the next update: the last row's value
+  if (tile == ntiles - 1) {
+    const uint32_t last = lim - 1;
+#pragma unroll
+    for (int k = 0; k < L; ++k)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+        if (lrow(k, e) == last) {
+          carry_out[0] = to_bits(finish(k * VEC + e));
+          carry_out[1] = 1;
+        }
+  }
+}
+
+template <typename TIn, typename U, int OP, bool BY, int VEC>
+void launch_as(Ctx &c, const void *x, const uint8_t *starts, uint64_t n, void *out, uint64_t *carry_in,
+               uint64_t *carry, uint64_t *status, uint32_t epoch, uint32_t tiles) {
+  k_scan<TIn, U, OP, BY, VEC><<<tiles, kScanBlock, 0, c.stream>>>(static_cast<const TIn *>(x), starts, n,
+                                                                   static_cast<U *>(out), status, epoch,
+                                                                   c.scratch.counters, tiles, carry_in, carry);
 }
 
 template <typename TIn, typename U, int OP>
 void launch(Ctx &c, bool by, const void *x, const uint8_t *starts, uint64_t n, void *out,
             uint64_t *carry) {
   const uint64_t tiles = (n + kTile - 1) / kTile;
+  if (tiles > 0xFFFFFFFFull) usage_error("scan too large");
   uint64_t *status = status_words(c, ST_SCAN, tiles * 4);
   const uint32_t epoch = next_epoch(c);
   uint64_t *carry_in = c.scratch.slots + 8;
   copy_d2d(c, carry_in, carry, 16);
-  if (by)
-    k_scan<TIn, U, OP, true><<<uint32_t(tiles), kScanBlock, 0, c.stream>>>(
-        static_cast<const TIn *>(x), starts, n, static_cast<U *>(out), status, epoch,
-        c.scratch.counters, uint32_t(tiles), carry_in, carry);
-  else
-    k_scan<TIn, U, OP, false><<<uint32_t(tiles), kScanBlock, 0, c.stream>>>(
-        static_cast<const TIn *>(x), starts, n, static_cast<U *>(out), status, epoch,
-        c.scratch.counters, uint32_t(tiles), carry_in, carry);
+  // 16-byte output accesses when the input pairs, start pairs and outputs are aligned
+  const bool vec = reinterpret_cast<uintptr_t>(x) % (2 * sizeof(TIn)) == 0 &&
+                   reinterpret_cast<uintptr_t>(out) % (2 * sizeof(U)) == 0 &&
+                   (!by || reinterpret_cast<uintptr_t>(starts) % 2 == 0);
+  const uint32_t nt = static_cast<uint32_t>(tiles);
+  if (by) {
+    if (vec) launch_as<TIn, U, OP, true, 2>(c, x, starts, n, out, carry_in, carry, status, epoch, nt);
+    else launch_as<TIn, U, OP, true, 1>(c, x, starts, n, out, carry_in, carry, status, epoch, nt);
+  } else {
+    if (vec) launch_as<TIn, U, OP, false, 2>(c, x, starts, n, out, carry_in, carry, status, epoch, nt);
+    else launch_as<TIn, U, OP, false, 1>(c, x, starts, n, out, carry_in, carry, status, epoch, nt);
+  }
   check_hip(hipGetLastError(), "scan launch");
 }
 

@@ -55,12 +55,6 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
   return sum;
 }
 
-// The comparison outcome of v against c as one bit: 1 less, 2 equal, 4 greater, 8 unordered.
-template <typename T>
-__device__ __forceinline__ uint32_t outcome(T v, T c) {
-  return v < c ? 1u : v == c ? 2u : v > c ? 4u : 8u;
-}
-
 // kind: 0 compress (keep = mask != 0), 1 filter (keep = x cmp c), 2 where (value base + row).
 // `keepset` is the set of comparison outcomes a filter keeps (outcome()).
 // BATCH: the tile's item (row pointers, row count, output, first tile, index base) comes
@@ -174,12 +168,6 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
       else ot[at] = val[i];
     }
   }
-}
-
-// Comparison codes 0 eq, 1 ne, 2 lt, 3 le, 4 gt, 5 ge as the outcome sets they keep.
-inline uint32_t keepset_of(int cmp) {
-  static const uint32_t sets[6] = {2, 1 | 4 | 8, 1, 1 | 2, 4, 4 | 2};
-  return sets[cmp >= 0 && cmp < 6 ? cmp : 5];
 }
 
 template <typename T, int KIND>
