@@ -144,15 +144,17 @@ __host__ __device__ __forceinline__ RunFn run_end(const RunAcc &a) {
   return f;
 }
 
-// The summary of the first cnt (at most N) of the elements v. Without ties the rounding does
-// not depend on the parity, so both start parities give one offset track: that is computed
-// first, with a tie (the rounding error of Z0 + v is exactly u / 2; Z0 + v rounds to a multiple
-// of u and fl(Z0 + v) - Z0 - v is exact by Fast2Sum, |Z0| > |v|) sending the run to the
-// two-track form. NaN and infinite elements give no tie but make the run bad either way.
+// The one-track summary of the first cnt (at most N) of the elements v, and their plain sum in
+// index order (*asum). Without ties the rounding does not depend on the parity, so both start
+// parities give one offset track. A tie (the rounding error of Z0 + v is exactly u / 2; Z0 + v
+// rounds to a multiple of u and fl(Z0 + v) - Z0 - v is exact by Fast2Sum, |Z0| > |v|) makes
+// the result false: the run needs the two-track form (run_of_two). NaN and infinite elements
+// give no tie but make the run bad either way.
 template <int N>
-__host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, const BinadeRef &r) {
+__host__ __device__ __forceinline__ bool run_of_one(const double (&v)[N], int cnt, const BinadeRef &r, RunFn *f,
+                                                    double *asum) {
   const double inf = __builtin_inf();
-  double d = 0.0, lo = inf, hi = -inf;
+  double d = 0.0, lo = inf, hi = -inf, a = 0.0;
   bool tie = false, bad = false;
   auto one = [&](double x) {
     const double w = (r.z0 + x) - r.z0;
@@ -163,6 +165,7 @@ __host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, 
     hi = __builtin_fmax(hi, d);
     const double ax = x < 0 ? -x : x;
     bad = bad || !(ax < r.lim);
+    a = a + x;
   };
   if (cnt >= N) {  // a whole run: straight-line code
 #pragma unroll
@@ -172,15 +175,29 @@ __host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, 
     for (int k = 0; k < N; ++k)
       if (k < cnt) one(v[k]);
   }
-  if (__builtin_expect(!tie, 1)) {
-    RunAcc a{RunFn{{d, d}, {lo, lo}, {hi, hi}}, {false, true}, bad};
-    return run_end(a);
-  }
+  *asum = a;
+  RunAcc acc{RunFn{{d, d}, {lo, lo}, {hi, hi}}, {false, true}, bad};
+  *f = run_end(acc);
+  return !tie;
+}
+
+// The two-track summary (start parities 0 and 1 followed separately) of the first cnt elements.
+template <int N>
+__host__ __device__ __forceinline__ RunFn run_of_two(const double (&v)[N], int cnt, const BinadeRef &r) {
   RunAcc a = run_start();
 #pragma unroll
   for (int k = 0; k < N; ++k)
     if (k < cnt) run_push(a, r, v[k]);
   return run_end(a);
+}
+
+// The summary of the first cnt (at most N) of the elements v.
+template <int N>
+__host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, const BinadeRef &r) {
+  RunFn f;
+  double a;
+  if (__builtin_expect(run_of_one(v, cnt, r, &f, &a), 1)) return f;
+  return run_of_two(v, cnt, r);
 }
 
 // The summary of run a followed by run b (both under the same hypothesis).

@@ -723,14 +723,26 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double
   const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
   const double start = pre[tile * 16 + j];
   const Binade h = binade_of(start);
-  double v[kRunBlocks];
   const int cnt = nb - r * kRunBlocks;
-  load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
   RunFn f = run_identity();
-  if (h.ok) f = run_of(v, cnt, binade_ref(h));
   double a = 0.0;
-#pragma unroll
-  for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
+  {
+    double v[kRunBlocks];
+    load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
+    // the one-track pass also sums the run; a tie (rare) reloads the run for the two-track
+    // form, so the loaded rows die as the pass consumes them
+    RunFn f1;
+    const bool one = run_of_one(v, cnt, binade_ref(h), &f1, &a);
+    if (h.ok) {
+      if (__builtin_expect(one, 1)) {
+        f = f1;
+      } else {
+        double w2[kRunBlocks];
+        load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, w2);
+        f = run_of_two(w2, cnt, binade_ref(h));
+      }
+    }
+  }
   a = a + __shfl_xor(a, 16);
   a = a + __shfl_xor(a, 32);
   if (l < 16) s_a[w][j] = a;
