@@ -181,11 +181,26 @@ def batched_suite(D, count: int):
     same = B.same(sample, ref)
     if not same:
         sys.exit("batched results differ from plain dispatch")
+    # the first statement pair alone, three ways: one call per array, two batched calls, and a
+    # batch plan (tiles planned on the device, tile shapes autotuned, one hipGraph replay)
+    tfp, rfp = B.timed(B.filter_sum_plain, D, wl, repeat=1)
+    tfb, rfb = B.timed(B.filter_sum_batched, D, wl, repeat=3)
+    plan = B.filter_sum_plan(D, wl, tune=True)
+    B.plan_sums(D, plan)  # warm-up replay
+    tfg, rfg = B.timed(B.plan_sums, D, plan, repeat=5)
+    tuning = plan.tuning().strip()
+    plan.close()
+    if not all(a.bits() == b.bits() == c.bits() for a, b, c in zip(rfp, rfb, rfg)):
+        sys.exit("filter+sum results differ between plain, batched and the batch plan")
+    filter_sum = {"statements": "s = reduce_add_f64(filter_gt_f64(x, 0.9)) per array",
+                  "plain_ms": round(tfp * 1e3, 1), "batched_ms": round(tfb * 1e3, 1),
+                  "plan_ms": round(tfg * 1e3, 2), "plan_speedup_vs_plain": round(tfp / tfg, 1),
+                  "autotuned": tuning, "identical": True}
     return {"arrays": count, "rows": wl.rows, "lengths": "log-uniform in [2^10, 2^20] rows",
             "program": "filter_gt_f64(x, 0.9) then reduce_add_f64; argsort_i64(k); group_id_i64(k), "
                        "reduce_max_i64 + 1, reduce_into_add_f64(g, x, ng); keys from 256 values",
             "plain_ms": round(tp * 1e3, 1), "batched_ms": round(tb * 1e3, 1),
-            "speedup": round(tp / tb, 2), "identical": same}
+            "speedup": round(tp / tb, 2), "identical": same, "filter_sum": filter_sum}
 
 
 def sort_suite(D):

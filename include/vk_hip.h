@@ -215,6 +215,54 @@ void vk_value_release(vk_value *v);
 int vk_batch_call(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, size_t nitems,
                   vk_value *out, vk_status *st);
 
+/* ---------------------------------------------------------------- batch plans */
+
+/* No counterpart in the reference; for callers that run the same statements over the same
+ * resident arrays again and again (a serving loop, a repeated query). A plan is a sequence of
+ * batched statements over `nitems` items: each statement is one kernel by name, run on every
+ * item, whose arguments refer to a plan input of the item, to an earlier statement's array
+ * result for the item, or to one scalar for all items. Compiling allocates every result,
+ * optionally times the tile shapes of each statement on the plan's own inputs and keeps the
+ * fastest, and captures into one hipGraph the sequence of kernel launches, whose tiles are
+ * planned on the device from the row counts earlier statements leave there (a selection's kept
+ * rows); a run replays the graph with no host work and no read-back. Results equal vk_call on
+ * each item alone, bit for bit.
+ * Plannable kernels: filter_<cmp>_<t> (over a plan input or an earlier selection),
+ * compress_<t> (plan inputs), reduce_add_f64, sumsq_f64 and condsum_<cmp>_f64 (over a plan
+ * input or a selection), dot_f64 (plan inputs), and the other reductions (over a plan input or
+ * a selection). A plan retains its inputs; they must not change while it is in use. */
+typedef struct vk_bplan vk_bplan;
+
+enum { VK_BREF_INPUT = 0, VK_BREF_STMT = 1, VK_BREF_SCALAR = 2 };
+
+/* A plan argument: a plan input (index = input number), an earlier statement's array result
+ * (index = statement number), or a scalar (`scalar`). */
+typedef struct vk_bref {
+  int32_t kind;
+  int32_t index;
+  vk_scalar scalar;
+} vk_bref;
+
+/* A plan over `nitems` items with `ninputs` arrays each: item i's input k is
+ * inputs[i * ninputs + k]. */
+int vk_bplan_new(vk_ctx *ctx, vk_array *const *inputs, size_t ninputs, size_t nitems, vk_bplan **out,
+                 vk_status *st);
+/* Appends a statement; its number goes to *stmt. Checks kinds, element types and lengths. */
+int vk_bplan_add(vk_bplan *plan, const char *name, const vk_bref *args, size_t nargs, size_t *stmt,
+                 vk_status *st);
+/* Allocates every result, autotunes the tile shapes when `tune` is nonzero (selection rows per
+ * thread 8 or 16, reduction tiles of 4096, 8192 or 16384 rows) and captures the graph. */
+int vk_bplan_compile(vk_bplan *plan, int tune, vk_status *st);
+/* Enqueues one replay of the plan's graph on the context's stream and returns. */
+int vk_bplan_run(vk_bplan *plan, vk_status *st);
+/* Statement `stmt`'s result for item `item` after the last run (waits for it): an array (a view
+ * of the plan's buffer, one reference owned by the caller) or a scalar. */
+int vk_bplan_result(vk_bplan *plan, size_t stmt, size_t item, vk_value *out, vk_status *st);
+/* The autotuner's measurements and choices, one line per tuned statement; returns the full
+ * length (copies at most cap - 1 bytes and a terminating zero). */
+size_t vk_bplan_tuning(const vk_bplan *plan, char *buf, size_t cap);
+void vk_bplan_free(vk_bplan *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,8 +4,8 @@ primitives, behind a C ABI (include/vk_hip.h) that mirrors verkehrt's kernel-tab
 ``Registry`` / ``Dispatch`` mirror the reference's kernel-table API over that ABI; arrays
 stay resident in HBM between calls.
 """
-from .kernels import (Array, Builder, Context, Dispatch, KernelError, Reducer, Registry, Scalar,
-                      Scanner, TYPES)
+from .kernels import (Array, BatchPlan, Builder, Context, Dispatch, Input, KernelError, Reducer,
+                      Registry, Scalar, Scanner, Stmt, TYPES)
 
-__all__ = ["Array", "Builder", "Context", "Dispatch", "KernelError", "Reducer", "Registry",
-           "Scalar", "Scanner", "TYPES"]
+__all__ = ["Array", "BatchPlan", "Builder", "Context", "Dispatch", "Input", "KernelError", "Reducer",
+           "Registry", "Scalar", "Scanner", "Stmt", "TYPES"]

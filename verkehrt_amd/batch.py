@@ -131,6 +131,42 @@ def batched(D: Dispatch, wl: Workload) -> Results:
     return Results(sums, perms, groups)
 
 
+def filter_sum_plain(D: Dispatch, wl: Workload) -> List[Scalar]:
+    """The statement pair s = reduce_add_f64(filter_gt_f64(x, 0.9)), one call per array."""
+    out = []
+    for i in range(len(wl.lengths)):
+        x, _ = wl.item(i)
+        out.append(D.call("reduce_add_f64", [D.call("filter_gt_f64", [x, Scalar("f64", 0.9)])]))
+    D.sync()
+    return out
+
+
+def filter_sum_batched(D: Dispatch, wl: Workload) -> List[Scalar]:
+    """The same pair as two batched calls."""
+    xs = [wl.item(i)[0] for i in range(len(wl.lengths))]
+    fs = D.batch_call("filter_gt_f64", [[x, Scalar("f64", 0.9)] for x in xs])
+    out = D.batch_call("reduce_add_f64", [[f] for f in fs])
+    D.sync()
+    return out
+
+
+def filter_sum_plan(D: Dispatch, wl: Workload, tune: bool = True):
+    """The same pair as a batch plan: statement 0 filters, statement 1 sums its rows."""
+    from .kernels import Input, Stmt
+    plan = D.batch_plan([[wl.item(i)[0]] for i in range(len(wl.lengths))])
+    plan.add("filter_gt_f64", [Input(0), Scalar("f64", 0.9)])
+    plan.add("reduce_add_f64", [Stmt(0)])
+    plan.compile(tune=tune)
+    return plan
+
+
+def plan_sums(D: Dispatch, plan) -> List[Scalar]:
+    """One replay of a filter-then-sum plan and its sums."""
+    plan.run()
+    D.sync()
+    return [plan.result(1, i) for i in range(plan.nitems)]
+
+
 def same(a: Results, b: Results) -> bool:
     """Bit-identical results."""
     if len(a.sums) != len(b.sums):

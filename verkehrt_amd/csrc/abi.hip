@@ -325,6 +325,7 @@ ReduceSpec reduce_spec(const Def &d, uint64_t base) {
 
 vk_reducer *reducer_new(Ctx &c, int def) {
   auto *r = new vk_reducer{&c, def, nullptr, 0, 0};
+  ctx_retain(c);
   r->state = tmp_alloc(c, 256);
   const Def &d = g_defs[def];
   if (d.impl == R_CANON) canon_init(c, d.p0 == 4, static_cast<double *>(r->state));
@@ -422,8 +423,10 @@ vk_scalar reducer_finish(vk_reducer *r) {
 
 void reducer_free(vk_reducer *r) {
   if (!r) return;
-  tmp_free(*r->c, r->state);
+  Ctx *c = r->c;
+  tmp_free(*c, r->state);
   delete r;
+  ctx_release(c);
 }
 
 // Scanner::update (scan.rs:251-326) with tile_rows' checks (scan.rs:226-248).
@@ -467,6 +470,7 @@ void scanner_update(vk_scanner *s, const vk_arg *args, size_t nargs, vk_array *o
 
 vk_scanner *scanner_new(Ctx &c, int def) {
   auto *s = new vk_scanner{&c, def, nullptr, 0};
+  ctx_retain(c);
   s->carry = static_cast<uint64_t *>(tmp_alloc(c, 256));
   fill_u64(c, s->carry, 32, 0);
   return s;
@@ -474,8 +478,10 @@ vk_scanner *scanner_new(Ctx &c, int def) {
 
 void scanner_free(vk_scanner *s) {
   if (!s) return;
-  tmp_free(*s->c, s->carry);
+  Ctx *c = s->c;
+  tmp_free(*c, s->carry);
   delete s;
+  ctx_release(c);
 }
 
 // ------------------------------------------------------------------ builder
@@ -546,6 +552,7 @@ vk_array *builder_take(vk_builder *b) {
     if (!b->ptr) runtime_error("cannot allocate an array of 0 elements");
   }
   auto *buf = new Buffer{&c, b->ptr, b->cap * w, true, 1};
+  ctx_retain(c);
   auto *a = new vk_array{1, buf, 0, b->len, b->type};
   b->ptr = nullptr;
   b->cap = b->len = 0;
@@ -1061,6 +1068,409 @@ int guarded(vk_status *st, F &&f) {
   }
 }
 
+// ------------------------------------------------------------- batch plans
+// A plan of batched statements over a fixed batch of resident arrays (vk_bplan_*): every
+// statement is one launch sequence over all items, its tiles planned on the device from the
+// row counts earlier statements left there (a selection's kept counts), so the whole plan
+// runs with no host read-back, as one sequence of kernels with arguments fixed at compile time,
+// captured once into a hipGraph and replayed. Statements:
+// selections (filter_<cmp>_<t> over a plan input or an earlier selection, compress_<t> over
+// plan inputs), canonical float sums (reduce_add_f64, sumsq_f64, condsum_<cmp>_f64 over a plan
+// input or a selection; dot_f64 over plan inputs) and the other reductions (over a plan input
+// or a selection). Argument references: a plan input (per item), an earlier statement's
+// per-item array, or a scalar.
+
+__global__ void k_plan_bump(uint32_t *epoch, uint32_t *wrapped) {
+  const uint32_t e = *epoch;
+  const uint32_t n = e >= 65535u ? 1u : e + 1u;
+  *epoch = n;
+  *wrapped = n == 1u ? 1u : 0u;
+}
+
+// Clears look-back status words when the plan epoch has wrapped (never in between), so a
+// word from 65535 runs ago cannot pass for this run's.
+__global__ void k_plan_clear(uint64_t *status, uint64_t words, const uint32_t *wrapped) {
+  if (!*wrapped) return;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x)
+    status[i] = 0;
+}
+
+// Compiling a plan captures its launch sequence (kernels only, arguments fixed at compile time)
+// into a hipGraph; a run replays the graph.
+constexpr bool kCapturePlans = true;
+
+struct PlanStmt {
+  const Def *d = nullptr;
+  int kind = 0;  // 0 selection, 1 canonical sum, 2 other reduction
+  std::vector<vk_bref> args;
+  std::vector<uint64_t> bound;  // per item: an upper bound of its rows
+  int src = -1;                 // statement whose arrays are the rows (or -1: plan inputs)
+  DevPlan dp;
+  uint64_t status_words = 0;
+  uint64_t *counts = nullptr;   // selection: kept rows per item
+  vk_array *outbuf = nullptr;   // selection: the kept rows of item i from row off[i]
+  std::vector<uint64_t> off;
+  uint64_t *res = nullptr;      // reductions: nitems x words result words
+  double *acc = nullptr, *zero = nullptr;
+  void *partials = nullptr;
+  uint32_t words = 1;
+  SelectSpec ss{};
+  CanonSpec cs{};
+  ReduceSpec rs{};
+  int items_per_thread = 16;    // selection rows per thread (autotuned: 8 or 16)
+  uint64_t tile_rows = 8192;    // reduction rows per tile (autotuned: 4096, 8192 or 16384)
+  std::vector<uint64_t> host;   // results copied back after a run
+};
+
+}  // namespace
+}  // namespace vk
+
+struct vk_bplan {
+  vk::Ctx *c = nullptr;
+  size_t nitems = 0, ninputs = 0;
+  std::vector<vk_array *> inputs;  // item-major, retained
+  std::vector<vk::PlanStmt> st;
+  uint32_t *epoch = nullptr;       // [0] epoch, [1] wrapped flag
+  std::vector<void *> allocs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool compiled = false, fresh = false;
+  std::vector<std::string> tuned;  // autotuner decisions, one line per tuned statement
+};
+
+namespace vk {
+namespace {
+
+void *plan_alloc(vk_bplan &p, uint64_t bytes) {
+  void *q = tmp_alloc(*p.c, bytes ? bytes : 8);
+  p.allocs.push_back(q);
+  return q;
+}
+
+// The rows of argument `a` of item i: pointer (null for a selection not compiled yet), type
+// and bound (rows known on the host or an upper bound when they come from a selection's device
+// count).
+struct RowsRef {
+  const void *ptr;
+  int32_t type;
+  uint64_t bound;
+  int src;  // -1 plan input, else the selection statement
+};
+
+RowsRef rows_ref(const vk_bplan &p, const vk_bref &a, size_t i) {
+  if (a.kind == VK_BREF_INPUT) {
+    if (a.index < 0 || size_t(a.index) >= p.ninputs) usage_error("plan input out of range");
+    const vk_array *x = p.inputs[i * p.ninputs + size_t(a.index)];
+    return RowsRef{dptr(x), x->type, x->len, -1};
+  }
+  if (a.kind == VK_BREF_STMT) {
+    if (a.index < 0 || size_t(a.index) >= p.st.size()) usage_error("plan statement out of range");
+    const PlanStmt &s = p.st[size_t(a.index)];
+    if (s.kind != 0) usage_error("statement " + std::to_string(a.index) + " has no array result");
+    // the statement's buffer exists once it is compiled (compiling goes in statement order)
+    const uint32_t w = type_width(s.d->out_t);
+    const void *ptr = s.outbuf ? static_cast<const char *>(dptr(s.outbuf)) + s.off[i] * w : nullptr;
+    return RowsRef{ptr, s.d->out_t, s.bound[i], a.index};
+  }
+  usage_error("a plan argument of " + std::string("kind ") + std::to_string(a.kind) + " is not rows");
+}
+
+uint64_t tiles_for(uint64_t rows, uint64_t unit, uint64_t per) { return (rows / unit + per - 1) / per; }
+
+// Statement s's launches over plan state (capturable: kernels only).
+void plan_launch(vk_bplan &p, PlanStmt &s) {
+  Ctx &c = *p.c;
+  const uint32_t ni = static_cast<uint32_t>(p.nitems);
+  const uint64_t *dev_n = s.src >= 0 ? p.st[size_t(s.src)].counts : nullptr;
+  if (s.kind == 0) {
+    plan_tiles_dev(c, s.dp, ni, dev_n, 1, select_tile_rows(s.items_per_thread));
+    select_batch_dev(c, s.ss, s.items_per_thread, s.dp, ni, s.counts);
+  } else if (s.kind == 1) {
+    plan_tiles_dev(c, s.dp, ni, dev_n, 16, canon_tile_blocks());
+    canon_batch_dev(c, s.cs, s.dp, ni, s.acc, s.zero, reinterpret_cast<double *>(s.res));
+  } else {
+    plan_tiles_dev(c, s.dp, ni, dev_n, 1, s.tile_rows);
+    reduce_batch_dev(c, s.rs, s.tile_rows, s.dp, ni, s.partials, s.res);
+  }
+}
+
+void plan_prologue(vk_bplan &p) {
+  Ctx &c = *p.c;
+  k_plan_bump<<<1, 1, 0, c.stream>>>(p.epoch, p.epoch + 1);
+  for (auto &s : p.st)
+    if (s.status_words)
+      k_plan_clear<<<stream_grid(s.status_words, kBlock, 256), kBlock, 0, c.stream>>>(s.dp.status, s.status_words,
+                                                                                     p.epoch + 1);
+  check_hip(hipGetLastError(), "plan prologue");
+}
+
+void plan_run_eager(vk_bplan &p) {
+  plan_prologue(p);
+  for (auto &s : p.st) plan_launch(p, s);
+}
+
+// Autotuning: each tunable statement is timed with every candidate on the plan's own inputs
+// (its sources already computed by an eager run), best of three launches each.
+void plan_tune(vk_bplan &p) {
+  Ctx &c = *p.c;
+  hipEvent_t e0, e1;
+  check_hip(hipEventCreate(&e0), "event");
+  check_hip(hipEventCreate(&e1), "event");
+  auto time_stmt = [&](PlanStmt &s) {
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      plan_prologue(p);  // a new epoch for every timed launch of a look-back kernel
+      check_hip(hipEventRecord(e0, c.stream), "event");
+      plan_launch(p, s);
+      check_hip(hipEventRecord(e1, c.stream), "event");
+      check_hip(hipEventSynchronize(e1), "event");
+      float ms = 0;
+      check_hip(hipEventElapsedTime(&ms, e0, e1), "event");
+      if (ms < best) best = ms;
+    }
+    return best;
+  };
+  plan_run_eager(p);
+  for (size_t k = 0; k < p.st.size(); ++k) {
+    PlanStmt &s = p.st[k];
+    std::string line = "statement " + std::to_string(k) + " " + s.d->name + ":";
+    if (s.kind == 0) {
+      int best = 16;
+      float bt = 1e30f;
+      for (int ipt : {8, 16}) {
+        s.items_per_thread = ipt;
+        const float t = time_stmt(s);
+        line += " " + std::to_string(ipt) + " rows/thread " + std::to_string(t * 1000.0f) + " us;";
+        if (t < bt) { bt = t; best = ipt; }
+      }
+      s.items_per_thread = best;
+      line += " chose " + std::to_string(best);
+    } else if (s.kind == 2) {
+      uint64_t best = 8192;
+      float bt = 1e30f;
+      for (uint64_t tr : {uint64_t(4096), uint64_t(8192), uint64_t(16384)}) {
+        s.tile_rows = tr;
+        const float t = time_stmt(s);
+        line += " " + std::to_string(tr) + "-row tiles " + std::to_string(t * 1000.0f) + " us;";
+        if (t < bt) { bt = t; best = tr; }
+      }
+      s.tile_rows = best;
+      line += " chose " + std::to_string(best);
+    } else {
+      continue;  // canonical sums have one tile shape (16K rows, the exact form's)
+    }
+    p.tuned.push_back(line);
+    // later statements read this one's outputs: recompute them with the choice
+    plan_prologue(p);
+    plan_launch(p, s);
+  }
+  check_hip(hipStreamSynchronize(c.stream), "tune");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+void plan_add(vk_bplan &p, const char *name, const vk_bref *args, size_t nargs) {
+  if (p.compiled) usage_error("the plan is already compiled");
+  const Def &d = def_of(name);
+  PlanStmt s;
+  s.d = &d;
+  s.args.assign(args, args + nargs);
+  if (nargs != d.in.size()) arity(d.in.size(), nargs);
+  auto rows_type = [&](size_t k) {
+    if (args[k].kind == VK_BREF_SCALAR) usage_error("expected " + tn(d.in[k].t) + "[], got " + tn(args[k].scalar.type));
+    const RowsRef r = rows_ref(p, args[k], 0);
+    if (r.type != d.in[k].t) usage_error("expected " + tn(d.in[k].t) + "[], got " + tn(r.type) + "[]");
+    return r;
+  };
+  auto scalar_of = [&](size_t k) {
+    if (args[k].kind != VK_BREF_SCALAR || args[k].scalar.type != d.in[k].t)
+      usage_error("expected " + tn(d.in[k].t) + " as argument " + std::to_string(k));
+    return args[k].scalar.bits;
+  };
+  if (d.cls == VK_SELECTION && (d.impl == S_FILTER || d.impl == S_COMPRESS)) {
+    s.kind = 0;
+    const RowsRef r0 = rows_type(0);
+    s.src = r0.src;
+    if (d.impl == S_FILTER) {
+      s.ss.kind = 1;
+      s.ss.type = d.t0;
+      s.ss.cmp = d.p0;
+      s.ss.cbits = scalar_of(1);
+    } else {
+      const RowsRef r1 = rows_type(1);
+      if (r0.src >= 0 || r1.src >= 0) usage_error("a planned compress takes plan inputs");
+      s.ss.kind = 0;
+      s.ss.type = d.t0;
+    }
+  } else if (d.cls == VK_REDUCTION && d.impl == R_CANON && d.p0 != 4) {
+    s.kind = 1;
+    const RowsRef r0 = rows_type(0);
+    s.src = r0.src;
+    s.cs = CanonSpec{d.p0, d.p1, 0, 0};
+    if (d.p0 == 1) {
+      const RowsRef r1 = rows_type(1);
+      if (r0.src >= 0 || r1.src >= 0) usage_error("a planned dot_f64 takes plan inputs");
+    }
+    if (d.p0 == 3) {
+      const uint64_t cb = scalar_of(1), eb = scalar_of(2);
+      std::memcpy(&s.cs.c, &cb, 8);
+      std::memcpy(&s.cs.e, &eb, 8);
+    }
+  } else if (d.cls == VK_REDUCTION && d.impl != R_CANON) {
+    s.kind = 2;
+    const RowsRef r0 = rows_type(0);
+    s.src = r0.src;
+    s.rs = reduce_spec(d, 0);
+    if (d.impl == R_COUNT_CMP) s.rs.cbits = scalar_of(1);
+    s.words = reduce_state_words(s.rs);
+  } else {
+    usage_error(d.name + " cannot be planned (selections, canonical sums and reductions can)");
+  }
+  // per-item bounds and equal lengths of two-array statements
+  s.bound.resize(p.nitems);
+  for (size_t i = 0; i < p.nitems; ++i) {
+    const RowsRef r0 = rows_ref(p, s.args[0], i);
+    s.bound[i] = r0.bound;
+    if ((s.kind == 0 && d.impl == S_COMPRESS) || (s.kind == 1 && d.p0 == 1)) {
+      const RowsRef r1 = rows_ref(p, s.args[1], i);
+      if (r1.bound != r0.bound)
+        usage_error("length mismatch: " + std::to_string(r0.bound) + " vs " + std::to_string(r1.bound));
+    }
+  }
+  p.st.push_back(std::move(s));
+}
+
+void plan_compile(vk_bplan &p, bool tune) {
+  if (p.compiled) usage_error("the plan is already compiled");
+  Ctx &c = *p.c;
+  const size_t ni = p.nitems;
+  p.epoch = static_cast<uint32_t *>(plan_alloc(p, 64));
+  fill_u64(c, reinterpret_cast<uint64_t *>(p.epoch), 8, 0);
+  for (size_t k = 0; k < p.st.size(); ++k) {
+    PlanStmt &s = p.st[k];
+    // tile bounds for the smallest tile any candidate uses
+    uint64_t unit = 1, per = 1;
+    if (s.kind == 0) per = select_tile_rows(8);
+    else if (s.kind == 1) unit = 16, per = canon_tile_blocks();
+    else per = 4096;
+    uint64_t max_tiles = 0;
+    for (size_t i = 0; i < ni; ++i) max_tiles += tiles_for(s.bound[i], unit, per);
+    if (max_tiles > 0x7FFFFFFFull) usage_error("plan statement too large");
+    // device items: pointers and the rows known on the host (device counts replace them)
+    std::vector<BatchItem> items(ni);
+    if (s.kind == 0) {
+      uint64_t total = 0;
+      s.off.resize(ni);
+      for (size_t i = 0; i < ni; ++i) {
+        s.off[i] = total;
+        total += s.bound[i];
+      }
+      s.outbuf = new_array(c, s.d->out_t, total ? total : 1);
+      s.counts = static_cast<uint64_t *>(plan_alloc(p, ni * 8));
+    }
+    const uint32_t w_out = type_width(s.d->out_t);
+    for (size_t i = 0; i < ni; ++i) {
+      const RowsRef r0 = rows_ref(p, s.args[0], i);
+      BatchItem &it = items[i];
+      it.a = r0.ptr;
+      it.n = r0.src >= 0 ? 0 : r0.bound;
+      if ((s.kind == 0 && s.ss.kind == 0) || (s.kind == 1 && s.cs.kind == 1)) it.b = rows_ref(p, s.args[1], i).ptr;
+      if (s.kind == 0) it.out = static_cast<char *>(dptr(s.outbuf)) + s.off[i] * w_out;
+    }
+    s.dp.items = static_cast<BatchItem *>(plan_alloc(p, ni * sizeof(BatchItem)));
+    check_hip(hipMemcpyAsync(s.dp.items, items.data(), ni * sizeof(BatchItem), hipMemcpyHostToDevice, c.stream),
+              "plan items");
+    s.dp.max_tiles = static_cast<uint32_t>(max_tiles);
+    s.dp.tile_item = static_cast<uint32_t *>(plan_alloc(p, max_tiles * 4));
+    s.dp.ntiles = static_cast<uint32_t *>(plan_alloc(p, 8));
+    s.dp.ticket = static_cast<uint32_t *>(plan_alloc(p, 8));
+    fill_u64(c, reinterpret_cast<uint64_t *>(s.dp.ticket), 1, 0);
+    s.dp.epoch = p.epoch;
+    if (s.kind == 0) s.status_words = max_tiles;
+    else if (s.kind == 1) s.status_words = max_tiles * canon_status_words_per_tile();
+    if (s.status_words) {
+      s.dp.status = static_cast<uint64_t *>(plan_alloc(p, s.status_words * 8));
+      fill_u64(c, s.dp.status, s.status_words, 0);
+    }
+    if (s.kind == 1) {
+      s.res = static_cast<uint64_t *>(plan_alloc(p, ni * 8));
+      s.acc = static_cast<double *>(plan_alloc(p, ni * 16 * 8));
+      s.zero = static_cast<double *>(plan_alloc(p, ni * 16 * 8));
+    } else if (s.kind == 2) {
+      s.res = static_cast<uint64_t *>(plan_alloc(p, ni * s.words * 8));
+      s.partials = plan_alloc(p, max_tiles * 24);
+    }
+  }
+  check_hip(hipStreamSynchronize(c.stream), "plan setup");
+  if (tune) plan_tune(p);
+  p.compiled = true;
+  if (!kCapturePlans) return;  // then a run enqueues the launch sequence itself
+  // capture: kernels only (no allocation, no copy, no memset, no synchronisation)
+  check_hip(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    plan_run_eager(p);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  check_hip(hipStreamEndCapture(c.stream, &p.graph), "end capture");
+  check_hip(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0), "graph instantiate");
+}
+
+void plan_fetch(vk_bplan &p) {
+  if (p.fresh) return;
+  Ctx &c = *p.c;
+  for (auto &s : p.st) {
+    if (s.kind == 0) {
+      s.host.resize(p.nitems);
+      check_hip(hipMemcpyAsync(s.host.data(), s.counts, p.nitems * 8, hipMemcpyDeviceToHost, c.stream), "counts");
+    } else {
+      const uint64_t w = s.kind == 1 ? 1 : s.words;
+      s.host.resize(p.nitems * w);
+      check_hip(hipMemcpyAsync(s.host.data(), s.res, p.nitems * w * 8, hipMemcpyDeviceToHost, c.stream), "results");
+    }
+  }
+  check_hip(hipStreamSynchronize(c.stream), "plan results");
+  p.fresh = true;
+}
+
+vk_value plan_result(vk_bplan &p, size_t k, size_t i) {
+  if (!p.compiled) usage_error("the plan is not compiled");
+  if (k >= p.st.size() || i >= p.nitems) usage_error("plan result out of range");
+  plan_fetch(p);
+  PlanStmt &s = p.st[k];
+  vk_value v{VK_VALUE_SCALAR, nullptr, {0, 0}};
+  if (s.kind == 0) {
+    v.kind = VK_VALUE_ARRAY;
+    s.outbuf->buf->refs++;
+    v.array = new vk_array{1, s.outbuf->buf, s.outbuf->offset + s.off[i] * type_width(s.d->out_t), s.host[i],
+                           s.d->out_t};
+  } else if (s.kind == 1) {
+    v.scalar = scalar(VK_F64, s.host[i]);
+  } else {
+    uint64_t w[3] = {0, 0, 0};
+    for (uint32_t q = 0; q < s.words; ++q) w[q] = s.host[i * s.words + q];
+    v.scalar = finish_words(*s.d, w);
+  }
+  return v;
+}
+
+void plan_free(vk_bplan *p) {
+  if (!p) return;
+  Ctx &c = *p->c;
+  (void)hipStreamSynchronize(c.stream);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (auto &s : p->st)
+    if (s.outbuf) release(s.outbuf);
+  for (void *q : p->allocs) tmp_free(c, q);
+  for (vk_array *a : p->inputs) release(a);
+  delete p;
+  ctx_release(&c);
+}
+
 }  // namespace
 }  // namespace vk
 
@@ -1208,18 +1618,24 @@ int vk_builder_new(vk_ctx *ctx, int32_t type, vk_builder **out, vk_status *st) {
   return guarded(st, [&] {
     if (!valid_type(type)) usage_error("unknown element type");
     *out = new vk_builder{ctx, type, nullptr, 0, 0};
+    ctx_retain(*ctx);
   });
 }
 uint64_t vk_builder_len(const vk_builder *b) { return b->len; }
 int vk_builder_finish(vk_builder *b, vk_array **out, vk_status *st) {
   int rc = guarded(st, [&] { *out = builder_take(b); });
+  Ctx *c = b->c;
+  if (b->ptr) c->pool->free(b->ptr);  // left only when taking failed
   delete b;
+  ctx_release(c);
   return rc;
 }
 void vk_builder_free(vk_builder *b) {
   if (!b) return;
-  if (b->ptr) b->c->pool->free(b->ptr);
+  Ctx *c = b->c;
+  if (b->ptr) c->pool->free(b->ptr);
   delete b;
+  ctx_release(c);
 }
 
 int vk_select_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, uint64_t base,
@@ -1267,5 +1683,63 @@ int vk_batch_call(vk_ctx *ctx, const char *name, const vk_arg *args, size_t narg
     ts.done(wb);
   });
 }
+
+int vk_bplan_new(vk_ctx *ctx, vk_array *const *inputs, size_t ninputs, size_t nitems, vk_bplan **out,
+                 vk_status *st) {
+  return guarded(st, [&] {
+    if (nitems == 0) usage_error("a plan needs at least one item");
+    auto *p = new vk_bplan();
+    p->c = ctx;
+    p->nitems = nitems;
+    p->ninputs = ninputs;
+    p->inputs.assign(inputs, inputs + ninputs * nitems);
+    for (vk_array *a : p->inputs) {
+      if (!a) {
+        delete p;
+        usage_error("a plan input is null");
+      }
+    }
+    for (vk_array *a : p->inputs) a->refs++;
+    ctx_retain(*ctx);
+    *out = p;
+  });
+}
+
+int vk_bplan_add(vk_bplan *plan, const char *name, const vk_bref *args, size_t nargs, size_t *stmt, vk_status *st) {
+  return guarded(st, [&] {
+    plan_add(*plan, name, args, nargs);
+    if (stmt) *stmt = plan->st.size() - 1;
+  });
+}
+
+int vk_bplan_compile(vk_bplan *plan, int tune, vk_status *st) {
+  return guarded(st, [&] { plan_compile(*plan, tune != 0); });
+}
+
+int vk_bplan_run(vk_bplan *plan, vk_status *st) {
+  return guarded(st, [&] {
+    if (!plan->compiled) usage_error("the plan is not compiled");
+    if (plan->exec) check_hip(hipGraphLaunch(plan->exec, plan->c->stream), "graph launch");
+    else plan_run_eager(*plan);
+    plan->fresh = false;
+  });
+}
+
+int vk_bplan_result(vk_bplan *plan, size_t stmt, size_t item, vk_value *out, vk_status *st) {
+  return guarded(st, [&] { *out = plan_result(*plan, stmt, item); });
+}
+
+size_t vk_bplan_tuning(const vk_bplan *plan, char *buf, size_t cap) {
+  std::string all;
+  for (const auto &l : plan->tuned) all += l + "\n";
+  if (buf && cap) {
+    const size_t k = all.size() < cap - 1 ? all.size() : cap - 1;
+    std::memcpy(buf, all.data(), k);
+    buf[k] = 0;
+  }
+  return all.size();
+}
+
+void vk_bplan_free(vk_bplan *plan) { plan_free(plan); }
 
 }  // extern "C"

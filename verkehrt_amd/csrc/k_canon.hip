@@ -277,7 +277,9 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double
                                                              uint64_t *status, uint32_t epoch,
                                                              uint32_t *ticket, uint32_t ntiles,
                                                              const BatchItem *items,
-                                                             const uint32_t *tile_item) {
+                                                             const uint32_t *tile_item,
+                                                             const uint32_t *dev_ntiles = nullptr,
+                                                             const uint32_t *dev_epoch = nullptr) {
   __shared__ double s_a[kFastWaves][16];
   __shared__ RunFn s_f[kFastWaves][16];
   __shared__ Binade s_h[16];
@@ -287,6 +289,9 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double
   __shared__ double s_exact[16];
   __shared__ int s_fail[16];
   const uint32_t tile = take_ticket(ticket, ntiles);
+  // tiles planned on the device: blocks past the planned count exit, the epoch is the plan's
+  if (dev_ntiles && tile >= *dev_ntiles) return;
+  if (dev_epoch) epoch = *dev_epoch;
   uint32_t first = 0, last = ntiles - 1;
   if (BATCH) {
     const uint32_t it = tile_item[tile];
@@ -1235,6 +1240,29 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
   tmp_free(c, acc);
   tmp_free(c, zero);
 }
+
+uint64_t canon_tile_blocks() { return kTileBlocks; }
+
+// Batched canonical sums over device-planned tiles (plan_tiles_dev with unit 16 and
+// canon_tile_blocks() blocks per tile): acc and zero are nitems x 16 doubles of scratch.
+void canon_batch_dev(Ctx &c, const CanonSpec &s, const DevPlan &p, uint32_t nitems, double *acc, double *zero,
+                     double *d_out) {
+  if (nitems == 0) return;
+  if (s.kind == 4) usage_error("batched products are not supported");
+  k_fill_d<<<stream_grid(uint64_t(nitems) * 16, kBlock, 4096), kBlock, 0, c.stream>>>(acc, uint64_t(nitems) * 16, 0.0);
+  k_fill_d<<<stream_grid(uint64_t(nitems) * 16, kBlock, 4096), kBlock, 0, c.stream>>>(zero, uint64_t(nitems) * 16, 0.0);
+  with_row(s, [&](auto rv) {
+    if (p.max_tiles > 0)
+      k_canon_fast<decltype(rv), true><<<p.max_tiles, kFastThreads, 0, c.stream>>>(
+          rv, nullptr, nullptr, 0, zero, acc, p.status, 0, p.ticket, p.max_tiles, p.items, p.tile_item, p.ntiles,
+          p.epoch);
+    k_canon_batch_finish<<<stream_grid(nitems, kBlock, 4096), kBlock, 0, c.stream>>>(rv, false, p.items, nitems, acc,
+                                                                                    d_out);
+  });
+  check_hip(hipGetLastError(), "canon batch (device plan)");
+}
+
+uint64_t canon_status_words_per_tile() { return kStride; }
 
 void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t rest,
                 double *tail) {

@@ -382,28 +382,32 @@ void with_op(const ReduceSpec &s, F &&f) {
 // break ties by index), so the merge order does not matter.
 constexpr uint64_t kBatchTile = 8192;
 
-template <typename T, class Op>
+// TILE rows per block (TILE / 256 rows per thread; batched plans pick it by autotuning). With
+// tiles planned on the device, blocks past dev_ntiles exit.
+template <typename T, class Op, uint64_t TILE = kBatchTile>
 __global__ __launch_bounds__(kBlock) void k_reduce_tiles(Op op, const BatchItem *items,
                                                          const uint32_t *tile_item,
-                                                         typename Op::Acc *partials) {
+                                                         typename Op::Acc *partials,
+                                                         const uint32_t *dev_ntiles = nullptr) {
   using Acc = typename Op::Acc;
+  if (dev_ntiles && blockIdx.x >= *dev_ntiles) return;
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const T *x = static_cast<const T *>(I.a);
-  const uint64_t t0 = (blockIdx.x - I.tile0) * kBatchTile;
-  const uint64_t t1 = t0 + kBatchTile < I.n ? t0 + kBatchTile : I.n;
+  const uint64_t t0 = (blockIdx.x - I.tile0) * TILE;
+  const uint64_t t1 = t0 + TILE < I.n ? t0 + TILE : I.n;
   Acc a = op.identity();
   for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) op.add(a, x[r], r);
   a = block_merge(op, a);
   if (threadIdx.x == 0) partials[blockIdx.x] = a;
 }
 
-template <class Op>
+template <class Op, uint64_t TILE = kBatchTile>
 __global__ void k_reduce_items(Op op, const BatchItem *items, uint64_t nitems,
                                const typename Op::Acc *partials, typename Op::Acc *states) {
   const uint64_t it = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (it >= nitems) return;
   const BatchItem &I = items[it];
-  const uint64_t tiles = (I.n + kBatchTile - 1) / kBatchTile;
+  const uint64_t tiles = (I.n + TILE - 1) / TILE;
   typename Op::Acc s = op.identity();
   for (uint64_t k = 0; k < tiles; ++k) op.merge(s, partials[I.tile0 + k]);
   states[it] = s;
@@ -417,6 +421,30 @@ void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *s
     using T = typename std::remove_pointer<decltype(tag)>::type;
     launch<T>(c, op, x, n, state);
   });
+}
+
+// Batched reductions over device-planned tiles of tile_rows rows (plan_tiles_dev with unit 1):
+// tile_rows is 4096, 8192 or 16384; partials holds p.max_tiles accumulators (24 bytes each).
+void reduce_batch_dev(Ctx &c, const ReduceSpec &s, uint64_t tile_rows, const DevPlan &p, uint32_t nitems,
+                      void *partials, uint64_t *states) {
+  if (nitems == 0) return;
+  with_op(s, [&](auto op, auto *tag) {
+    using T = typename std::remove_pointer<decltype(tag)>::type;
+    using O = decltype(op);
+    using Acc = typename O::Acc;
+    auto *parts = static_cast<Acc *>(partials);
+    auto go = [&](auto tile) {
+      constexpr uint64_t TL = decltype(tile)::value;
+      if (p.max_tiles)
+        k_reduce_tiles<T, O, TL><<<p.max_tiles, kBlock, 0, c.stream>>>(op, p.items, p.tile_item, parts, p.ntiles);
+      k_reduce_items<O, TL><<<stream_grid(nitems, kBlock, 1u << 20), kBlock, 0, c.stream>>>(
+          op, p.items, nitems, parts, reinterpret_cast<Acc *>(states));
+    };
+    if (tile_rows == 4096) go(std::integral_constant<uint64_t, 4096>());
+    else if (tile_rows == 16384) go(std::integral_constant<uint64_t, 16384>());
+    else go(std::integral_constant<uint64_t, 8192>());
+  });
+  check_hip(hipGetLastError(), "reduce batch (device plan)");
 }
 
 uint32_t reduce_state_words(const ReduceSpec &s) {

@@ -12,7 +12,8 @@ namespace vk {
 namespace {
 
 // 1024 threads x 16 rows: large enough that the tile ticket (one same-address atomic per
-// tile, about 11 ns apart at best) never limits the row rate.
+// tile, about 11 ns apart at best) never limits the row rate. Batched launches may use 8 rows
+// per thread (chosen by the plan autotuner).
 constexpr int kSelBlock = 1024;
 constexpr int kItems = 16;
 constexpr int kTile = kSelBlock * kItems;  // 16384 rows
@@ -64,16 +65,23 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
 // the end of the last tile load the last row and are never kept, so no load is predicated.
 // Two blocks per compute unit (8 waves per SIMD, at most 64 VGPRs) keep one block loading while
 // the other waits on its look-back.
-template <typename T, int KIND, bool BATCH>
+// ITEMS rows per thread (a tile is 1024 x ITEMS rows). A launch whose tiles were planned on
+// the device passes dev_ntiles (blocks past it exit) and dev_epoch (the epoch to publish with).
+template <typename T, int KIND, bool BATCH, int ITEMS = kItems>
 __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
                                                    uint32_t keepset, T c, uint64_t base, void *out_v,
                                                    uint64_t *status, uint32_t epoch,
                                                    uint32_t *ticket, uint32_t ntiles,
                                                    uint64_t *d_count, const BatchItem *items,
-                                                   const uint32_t *tile_item, uint64_t *counts) {
+                                                   const uint32_t *tile_item, uint64_t *counts,
+                                                   const uint32_t *dev_ntiles, const uint32_t *dev_epoch) {
   using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
+  constexpr int kTile = kSelBlock * ITEMS;
+  constexpr int kItems = ITEMS;
   const uint32_t tile = take_ticket(ticket, ntiles);
+  if (dev_ntiles && tile >= *dev_ntiles) return;
+  if (dev_epoch) epoch = *dev_epoch;
   uint32_t first = 0, last = ntiles - 1;
   if (BATCH) {
     const uint32_t it = tile_item[tile];
@@ -120,13 +128,14 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   }
   __syncthreads();
   // Exclusive scan of the kItems x kWaves counts in row order (item-major), by wave 0:
-  // lane l owns the four consecutive entries 4l..4l+3.
-  static_assert(kItems * kWaves == 4 * kWave, "four counts per lane");
+  // lane l owns the PER consecutive entries PER l .. PER l + PER - 1.
+  constexpr int PER = kItems * kWaves / kWave;
+  static_assert(PER * kWave == kItems * kWaves, "whole counts per lane");
   if (w == 0) {
-    uint32_t c4[4], v = 0;
+    uint32_t c4[PER], v = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = 4 * l + q;
+    for (int q = 0; q < PER; ++q) {
+      const int k = PER * l + q;
       c4[q] = s_cnt[k / kWaves][k % kWaves];
       v += c4[q];
     }
@@ -138,8 +147,8 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
     }
     uint32_t e = inc - v;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = 4 * l + q;
+    for (int q = 0; q < PER; ++q) {
+      const int k = PER * l + q;
       s_excl[k / kWaves][k % kWaves] = e;
       e += c4[q];
     }
@@ -181,7 +190,7 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
   __builtin_memcpy(&cv, &cbits, sizeof(T));
   k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
       static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, status, epoch,
-      c.scratch.counters, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr);
+      c.scratch.counters, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);
   check_hip(hipGetLastError(), "select launch");
 }
 
@@ -195,7 +204,7 @@ void launch_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, ui
     __builtin_memcpy(&cv, &s.cbits, sizeof(T));
     k_select<T, KIND, true><<<p.ntiles, kSelBlock, 0, c.stream>>>(
         nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
-        nullptr, p.items, p.tile_item, d_counts);
+        nullptr, p.items, p.tile_item, d_counts, nullptr, nullptr);
     check_hip(hipGetLastError(), "select batch launch");
   }
   batch_plan_free(c, p);
@@ -243,6 +252,52 @@ void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask
     case 1: by_width<1>(c, s.type, x, nullptr, n, s.cmp, s.cbits, 0, out, d_count); break;
     default: launch<uint8_t, 2>(c, nullptr, mask, n, 0, 0, s.base, out, d_count); break;
   }
+}
+
+namespace {
+
+template <typename T, int KIND, int ITEMS>
+void launch_dev(Ctx &c, const SelectSpec &s, const DevPlan &p, uint64_t *d_counts) {
+  T cv;
+  __builtin_memcpy(&cv, &s.cbits, sizeof(T));
+  k_select<T, KIND, true, ITEMS><<<p.max_tiles, kSelBlock, 0, c.stream>>>(
+      nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, p.status, 0, p.ticket, p.max_tiles, nullptr,
+      p.items, p.tile_item, d_counts, p.ntiles, p.epoch);
+}
+
+template <int KIND, int ITEMS>
+void dev_by_width(Ctx &c, const SelectSpec &s, const DevPlan &p, uint64_t *d_counts) {
+  switch (s.type) {
+    case VK_F64: launch_dev<double, KIND, ITEMS>(c, s, p, d_counts); break;
+    case VK_F32: launch_dev<float, KIND, ITEMS>(c, s, p, d_counts); break;
+    case VK_I64: launch_dev<int64_t, KIND, ITEMS>(c, s, p, d_counts); break;
+    case VK_U64: launch_dev<uint64_t, KIND, ITEMS>(c, s, p, d_counts); break;
+    case VK_I32: launch_dev<int32_t, KIND, ITEMS>(c, s, p, d_counts); break;
+    case VK_U8: case VK_BOOL: launch_dev<uint8_t, KIND, ITEMS>(c, s, p, d_counts); break;
+    default: usage_error("unsupported selection type");
+  }
+}
+
+template <int ITEMS>
+void dev_by_kind(Ctx &c, const SelectSpec &s, const DevPlan &p, uint64_t *d_counts) {
+  switch (s.kind) {
+    case 0: dev_by_width<0, ITEMS>(c, s, p, d_counts); break;
+    case 1: dev_by_width<1, ITEMS>(c, s, p, d_counts); break;
+    default: launch_dev<uint8_t, 2, ITEMS>(c, s, p, d_counts); break;
+  }
+}
+
+}  // namespace
+
+uint64_t select_tile_rows(int items_per_thread) { return uint64_t(kSelBlock) * (items_per_thread == 8 ? 8 : 16); }
+
+void select_batch_dev(Ctx &c, const SelectSpec &s, int items_per_thread, const DevPlan &p, uint32_t nitems,
+                      uint64_t *d_counts) {
+  fill_u64(c, d_counts, nitems, 0);  // items without rows keep a count of 0
+  if (p.max_tiles == 0) return;
+  if (items_per_thread == 8) dev_by_kind<8>(c, s, p, d_counts);
+  else dev_by_kind<16>(c, s, p, d_counts);
+  check_hip(hipGetLastError(), "select batch (device plan)");
 }
 
 void select_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts) {

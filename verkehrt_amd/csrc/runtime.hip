@@ -97,6 +97,7 @@ vk_array *new_array(Ctx &c, int32_t type, uint64_t len) {
   void *p = c.pool->alloc(len * w);
   if (!p) runtime_error("cannot allocate an array of " + std::to_string(len) + " elements");
   auto *b = new Buffer{&c, p, len * w, true, 1};
+  ctx_retain(c);
   return new vk_array{1, b, 0, len, type};
 }
 
@@ -105,7 +106,10 @@ void release(vk_array *a) {
   if (--a->refs > 0) return;
   Buffer *b = a->buf;
   if (--b->refs == 0) {
-    if (b->owned) b->ctx->pool->free(b->ptr);
+    if (b->owned) {
+      b->ctx->pool->free(b->ptr);
+      ctx_release(b->ctx);
+    }
     delete b;
   }
   delete a;
@@ -201,6 +205,73 @@ BatchPlan batch_plan(Ctx &c, std::vector<BatchItem> &items, uint64_t rows_per_ti
   return p;
 }
 
+namespace {
+
+constexpr int kPlanThreads = 1024;
+
+// Tile counts and first tiles of every item, by one block: thread t takes a contiguous range
+// of items, the range totals are scanned in LDS, then each thread writes its items' tile0.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_items(BatchItem *items, uint32_t nitems,
+                                                             const uint64_t *dev_n, uint32_t unit,
+                                                             uint64_t per_tile, uint32_t max_tiles,
+                                                             uint32_t *ntiles) {
+  __shared__ uint64_t s_part[kPlanThreads];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (nitems + kPlanThreads - 1) / kPlanThreads;
+  const uint32_t i0 = t * per < nitems ? t * per : nitems;
+  const uint32_t i1 = i0 + per < nitems ? i0 + per : nitems;
+  auto tiles_of = [&](uint64_t n) { return (n / unit + per_tile - 1) / per_tile; };
+  uint64_t sum = 0;
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint64_t n = dev_n ? dev_n[i] : items[i].n;
+    if (dev_n) items[i].n = n;
+    sum += tiles_of(n);
+  }
+  s_part[t] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < kPlanThreads; d <<= 1) {  // inclusive scan (Hillis-Steele)
+    const uint64_t o = t >= d ? s_part[t - d] : 0;
+    __syncthreads();
+    s_part[t] += o;
+    __syncthreads();
+  }
+  uint64_t run = s_part[t] - sum;
+  for (uint32_t i = i0; i < i1; ++i) {
+    items[i].tile0 = run;
+    run += tiles_of(items[i].n);
+  }
+  // the grid of the launches is max_tiles, an upper bound the host proved from input sizes;
+  // the clamp only keeps every access inside the map
+  if (t == kPlanThreads - 1) *ntiles = static_cast<uint32_t>(s_part[t] < max_tiles ? s_part[t] : max_tiles);
+}
+
+// Tile t belongs to the last item whose first tile is at most t (items without tiles share
+// their successor's first tile, so the search never stops on one).
+__global__ void k_plan_map(const BatchItem *items, uint32_t nitems, uint32_t max_tiles, const uint32_t *ntiles,
+                           uint32_t *tile_item) {
+  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (t >= max_tiles || t >= *ntiles) return;
+  uint32_t lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo + 1) / 2;
+    if (items[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  tile_item[t] = lo;
+}
+
+}  // namespace
+
+void plan_tiles_dev(Ctx &c, const DevPlan &p, uint32_t nitems, const uint64_t *dev_n, uint32_t unit,
+                    uint64_t per_tile) {
+  if (nitems == 0) return;
+  k_plan_items<<<1, kPlanThreads, 0, c.stream>>>(p.items, nitems, dev_n, unit, per_tile, p.max_tiles, p.ntiles);
+  if (p.max_tiles)
+    k_plan_map<<<(p.max_tiles + kBlock - 1) / kBlock, kBlock, 0, c.stream>>>(p.items, nitems, p.max_tiles, p.ntiles,
+                                                                           p.tile_item);
+  check_hip(hipGetLastError(), "device tile plan");
+}
+
 void batch_plan_free(Ctx &c, BatchPlan &p) {
   if (p.items) tmp_free(c, p.items);
   if (p.tile_item) tmp_free(c, p.tile_item);
@@ -250,8 +321,10 @@ Ctx *create_ctx(int device) {
   return c;
 }
 
-void destroy_ctx(Ctx *c) {
-  if (!c) return;
+void ctx_retain(Ctx &c) { c.refs.fetch_add(1); }
+
+void ctx_release(Ctx *c) {
+  if (!c || c->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   delete c->pool;
@@ -261,5 +334,7 @@ void destroy_ctx(Ctx *c) {
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
+
+void destroy_ctx(Ctx *c) { ctx_release(c); }
 
 }  // namespace vk

@@ -118,11 +118,18 @@ struct vk_ctx {
   hipEvent_t ev0, ev1;
   uint32_t pticket = 0;  // next ticket base of persistent launches (counters[4])
   int cus = 0;           // compute units of the device
+  // Holders of the context: its creator (until vk_ctx_destroy), every owned buffer, reducer,
+  // scanner, builder and plan. The stream, pool and scratch go when the last one lets go, so
+  // an array released after vk_ctx_destroy still returns its memory to a live pool.
+  std::atomic<int> refs{1};
 };
 
 namespace vk {
 
 using Ctx = vk_ctx;
+
+void ctx_retain(Ctx &c);
+void ctx_release(Ctx *c);  // the last release tears the context down
 
 // Arrays ------------------------------------------------------------------
 vk_array *new_array(Ctx &c, int32_t type, uint64_t len);  // uninitialised
@@ -247,6 +254,39 @@ void check_keys_batch(Ctx &c, std::vector<BatchItem> &items, uint64_t *d_bad);
 // Batched reduce_into_add_f64 (keys checked): item i folds values b by keys a (n rows) into
 // the aux2 f64 buckets at out, in the order of reduce_into_add_f64. Sets aux3.
 void reduce_into_add_f64_batch(Ctx &c, std::vector<BatchItem> &items);
+// Batched launches whose tiles are planned on the device, so that no host read-back sits
+// between statements and a statement sequence can be captured into a hipGraph: item row
+// counts may come from an earlier launch's device results (a selection's kept counts), the
+// launch grid is an upper bound of the tiles, and blocks past the planned count exit. The
+// look-back kernels take their epoch from `epoch` (bumped on the device once per run) and their
+// tickets from their own counter, so one plan's launches share no scratch with other calls.
+struct DevPlan {
+  BatchItem *items = nullptr;     // device items; n (from dev_n) and tile0 set on the device
+  uint32_t *tile_item = nullptr;  // device tile -> item map, max_tiles entries
+  uint32_t *ntiles = nullptr;     // device: the planned tile count
+  uint32_t max_tiles = 0;         // the launch grid, an upper bound of the planned tiles
+  uint32_t *epoch = nullptr;      // device epoch word of the look-back kernels
+  uint64_t *status = nullptr;     // look-back status words (max_tiles x words per tile)
+  uint32_t *ticket = nullptr;     // ticket counter, zero before a launch and after it
+};
+// Item i has n_i rows (dev_n[i] when dev_n is set, written to items[i].n; else items[i].n) and
+// ceil(floor(n_i / unit) / per_tile) tiles; writes items[i].tile0, the map and the tile count.
+void plan_tiles_dev(Ctx &c, const DevPlan &p, uint32_t nitems, const uint64_t *dev_n, uint32_t unit,
+                    uint64_t per_tile);
+// Rows per tile of the batched selection with 8 or 16 rows per thread.
+uint64_t select_tile_rows(int items_per_thread);
+void select_batch_dev(Ctx &c, const SelectSpec &s, int items_per_thread, const DevPlan &p, uint32_t nitems,
+                      uint64_t *d_counts);
+// Batched canonical sums (plan unit 16, canon_tile_blocks() blocks per tile); acc and zero are
+// nitems x 16 doubles of scratch.
+uint64_t canon_tile_blocks();
+uint64_t canon_status_words_per_tile();
+void canon_batch_dev(Ctx &c, const CanonSpec &s, const DevPlan &p, uint32_t nitems, double *acc, double *zero,
+                     double *d_out);
+// Batched reductions (plan unit 1, tile_rows rows per tile: 4096, 8192 or 16384); partials
+// holds max_tiles accumulators of at most 24 bytes.
+void reduce_batch_dev(Ctx &c, const ReduceSpec &s, uint64_t tile_rows, const DevPlan &p, uint32_t nitems,
+                      void *partials, uint64_t *states);
 // One selection launch over all items; the kept count of item i goes to d_counts[i].
 void select_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, uint64_t *d_counts);
 // One canonical-reduction launch over all items (a = x, b = y or null); item i's result to

@@ -420,6 +420,11 @@ class Dispatch:
                 res.append(Array(self.ctx, o.array))
         return res
 
+    def batch_plan(self, inputs: Sequence[Sequence[Array]]) -> "BatchPlan":
+        """A plan of batched statements over the items' input arrays (vk_bplan_new):
+        inputs[i][k] is item i's input k."""
+        return BatchPlan(self.ctx, inputs)
+
     def reducer(self, name: str) -> Reducer:
         return Reducer(self.ctx, name)
 
@@ -447,3 +452,92 @@ class Dispatch:
 
     def sync(self):
         self.ctx.sync()
+
+
+class Input:
+    """A plan argument: the item's plan input number `k`."""
+
+    def __init__(self, k: int):
+        self.k = k
+
+
+class Stmt:
+    """A plan argument: the item's array result of plan statement `s`."""
+
+    def __init__(self, s: int):
+        self.s = s
+
+
+class BatchPlan:
+    """Batched statements over fixed resident arrays (vk_bplan_*): add statements, compile
+    (optionally autotuning tile shapes; the launch sequence, with tiles planned on the device,
+    is captured into one hipGraph), then run (one graph replay, no host read-back) and read
+    results; every result equals the single call on that item alone."""
+
+    def __init__(self, ctx: Context, inputs: Sequence[Sequence[Array]]):
+        self.ctx = ctx
+        self._lib = N.lib()
+        self.nitems = len(inputs)
+        self.ninputs = len(inputs[0]) if inputs else 0
+        if any(len(it) != self.ninputs for it in inputs):
+            raise ValueError("every item needs the same number of inputs")
+        flat = (C.c_void_p * max(1, self.nitems * self.ninputs))()
+        for i, it in enumerate(inputs):
+            for k, a in enumerate(it):
+                flat[i * self.ninputs + k] = a.handle
+        h = C.c_void_p()
+        st = N.Status()
+        _check(self._lib.vk_bplan_new(ctx.handle, flat, self.ninputs, self.nitems, C.byref(h), C.byref(st)), st)
+        self.handle = h
+        self.kinds = []
+
+    def add(self, name: str, args: Sequence) -> int:
+        refs = (N.CBref * max(1, len(args)))()
+        for i, a in enumerate(args):
+            if isinstance(a, Input):
+                refs[i].kind, refs[i].index = N.BREF_INPUT, a.k
+            elif isinstance(a, Stmt):
+                refs[i].kind, refs[i].index = N.BREF_STMT, a.s
+            elif isinstance(a, Scalar):
+                refs[i].kind = N.BREF_SCALAR
+                refs[i].scalar.type = TYPE_CODE[a.type]
+                refs[i].scalar.bits = a.bits()
+            else:
+                raise TypeError(f"plan argument {a!r}")
+        out = C.c_size_t()
+        st = N.Status()
+        _check(self._lib.vk_bplan_add(self.handle, name.encode(), refs, len(args), C.byref(out), C.byref(st)), st)
+        return out.value
+
+    def compile(self, tune: bool = False):
+        st = N.Status()
+        _check(self._lib.vk_bplan_compile(self.handle, 1 if tune else 0, C.byref(st)), st)
+
+    def run(self):
+        st = N.Status()
+        _check(self._lib.vk_bplan_run(self.handle, C.byref(st)), st)
+
+    def result(self, stmt: int, item: int):
+        out = N.CValue()
+        st = N.Status()
+        _check(self._lib.vk_bplan_result(self.handle, stmt, item, C.byref(out), C.byref(st)), st)
+        if out.kind == N.VALUE_SCALAR:
+            return Scalar.from_c(out.scalar)
+        return Array(self.ctx, out.array)
+
+    def tuning(self) -> str:
+        n = self._lib.vk_bplan_tuning(self.handle, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self._lib.vk_bplan_tuning(self.handle, buf, n + 1)
+        return buf.value.decode()
+
+    def close(self):
+        if self.handle and _ALIVE[0]:
+            self._lib.vk_bplan_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -34,6 +34,13 @@ class CValue(C.Structure):
     _fields_ = [("kind", C.c_int32), ("array", C.c_void_p), ("scalar", CScalar)]
 
 
+BREF_INPUT, BREF_STMT, BREF_SCALAR = 0, 1, 2
+
+
+class CBref(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_int32), ("scalar", CScalar)]
+
+
 # Every symbol the header declares, with its ctypes signature.
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
@@ -81,6 +88,13 @@ SIGNATURES = {
     "vk_value_release": (None, [C.POINTER(CValue)]),
     "vk_batch_call": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, C.c_size_t,
                                 C.POINTER(CValue), _ST]),
+    "vk_bplan_new": (C.c_int, [_P, _PP, C.c_size_t, C.c_size_t, _PP, _ST]),
+    "vk_bplan_add": (C.c_int, [_P, C.c_char_p, C.POINTER(CBref), C.c_size_t, C.POINTER(C.c_size_t), _ST]),
+    "vk_bplan_compile": (C.c_int, [_P, C.c_int, _ST]),
+    "vk_bplan_run": (C.c_int, [_P, _ST]),
+    "vk_bplan_result": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(CValue), _ST]),
+    "vk_bplan_tuning": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
+    "vk_bplan_free": (None, [_P]),
 }
 
 _lib = None
