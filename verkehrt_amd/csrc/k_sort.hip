@@ -109,8 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
   __shared__ uint32_t s_wcnt[kWaves][kRadix];  // per-wave counters, then per-wave offsets
   __shared__ uint32_t s_excl[kRadix];           // tile-local exclusive digit offsets
   __shared__ uint64_t s_gbase[kRadix];          // global position of the tile's first row per digit
-  __shared__ U s_keys[kTile];
-  __shared__ V s_pos[kTile];
+  __shared__ uint64_t s_stage[kTile];          // one column of the sorted tile at a time
   __shared__ uint32_t s_wsum[kWaves];
 
   const int32_t type = a.type;
@@ -247,29 +246,52 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
     s_gbase[dd] = uint64_t(digit_base[dd]) + prefix;
   }
   __syncthreads();
-  // stage the rows sorted by digit within the tile
+  // Stage the tile sorted by digit one column at a time through one LDS buffer (32 KB instead
+  // of 48 for keys and positions together, so four blocks fit a compute unit): the key round
+  // gives every sorted slot its destination, kept in registers for the payload round.
+  const uint32_t rows = static_cast<uint32_t>(n - t0 < uint64_t(kTile) ? n - t0 : uint64_t(kTile));
+  uint32_t at[kItems];
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const uint64_t r = wbase + uint64_t(i) * kWave + l;
-    if (r < n) {
-      const uint32_t d = (key[i] >> shift) & 255;
-      const uint32_t at = s_excl[d] + s_wcnt[w][d] + rnk[i];
-      s_keys[at] = key[i];
-      s_pos[at] = pos[i];
+    const uint32_t d = (key[i] >> shift) & 255;
+    at[i] = r < n ? s_excl[d] + s_wcnt[w][d] + rnk[i] : 0u;
+  }
+  U *sk = reinterpret_cast<U *>(s_stage);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t r = wbase + uint64_t(i) * kWave + l;
+    if (r < n) sk[at[i]] = key[i];
+  }
+  __syncthreads();
+  constexpr int kSlots = kTile / kBlock;
+  uint32_t gj[kSlots];  // destinations below 2^32: at most 2^32 - 1 rows per sort
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) {
+    const uint32_t j = uint32_t(t) + uint32_t(q) * kBlock;
+    gj[q] = 0;
+    if (j < rows) {
+      const U k = sk[j];
+      const uint32_t d = (k >> shift) & 255;
+      const uint64_t g = s_gbase[d] + (j - s_excl[d]);
+      gj[q] = static_cast<uint32_t>(g);
+      if (keys_out) keys_out[g] = k;
     }
   }
   __syncthreads();
-  const uint64_t rows = n - t0 < uint64_t(kTile) ? n - t0 : uint64_t(kTile);
-  for (uint32_t j = t; j < rows; j += kBlock) {
-    const U k = s_keys[j];
-    const uint32_t d = (k >> shift) & 255;
-    const uint64_t g = s_gbase[d] + (j - s_excl[d]);
-    if (LAST && POS) {
-      out_pos[g] = static_cast<int64_t>(s_pos[j]);
-      if (keys_out) keys_out[g] = k;
-    } else {
-      keys_out[g] = k;
-      pos_out[g] = s_pos[j];
+  V *sp = reinterpret_cast<V *>(s_stage);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t r = wbase + uint64_t(i) * kWave + l;
+    if (r < n) sp[at[i]] = pos[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kSlots; ++q) {
+    const uint32_t j = uint32_t(t) + uint32_t(q) * kBlock;
+    if (j < rows) {
+      if (LAST && POS) out_pos[gj[q]] = static_cast<int64_t>(sp[j]);
+      else pos_out[gj[q]] = sp[j];
     }
   }
 }
