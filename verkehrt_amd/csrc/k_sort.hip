@@ -6,7 +6,7 @@
 // skipped (radix.rs:187-259).
 //
 // MI355X form: the counting pass builds all digit histograms in LDS per block. Each
-// remaining digit is one pass over 4096-row tiles: the rows of a wave are ranked among
+// remaining digit is one pass over 8192-row tiles (512 threads x 16 rows): the rows of a wave are ranked among
 // equal digits with eight ballots (a wave64 match), per-wave digit counters in LDS give
 // ranks in row order, each tile publishes its 256 digit counts and learns the counts of
 // the tiles before it by decoupled look-back per digit, and the rows are staged sorted
@@ -23,8 +23,9 @@ namespace vk {
 namespace {
 
 constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;  // 4096 rows
-constexpr int kWaves = kBlock / kWave;
+constexpr int kSortBlock = 512;              // threads of a pass block
+constexpr int kTile = kSortBlock * kItems;   // 8192 rows
+constexpr int kWaves = kSortBlock / kWave;
 constexpr int kRadix = 256;
 constexpr int32_t kRawU32 = 100;  // internal key type: u32 keys sorted as they are
 
@@ -104,7 +105,7 @@ struct PassArgs {
 // LAST: positions go to out_pos as i64 (and keys to keys_out when given); values go with
 // their keys to keys_out / pay_out.
 template <typename U, typename V, bool FIRST, bool LAST, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
+__global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
   constexpr bool POS = std::is_same<V, uint32_t>::value;
   __shared__ uint32_t s_wcnt[kWaves][kRadix];  // per-wave counters, then per-wave offsets
   __shared__ uint32_t s_excl[kRadix];           // tile-local exclusive digit offsets
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
   }
   const uint64_t t0 = uint64_t(tile - first) * kTile;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  for (int k = t; k < kWaves * kRadix; k += kBlock) (&s_wcnt[0][0])[k] = 0;
+  for (int k = t; k < kWaves * kRadix; k += kSortBlock) (&s_wcnt[0][0])[k] = 0;
   __syncthreads();
 
   U key[kItems];
@@ -182,9 +183,10 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
     rnk[i] = r_in;
   }
   __syncthreads();
-  // per digit: wave offsets and the tile count (thread t owns digit t)
-  uint32_t cnt = 0;
-  {
+  // per digit: wave offsets and the tile count (thread t < 256 owns digit t; the first four
+  // waves)
+  uint32_t cnt = 0, inc = 0;
+  if (t < kRadix) {
     const int dd = t;
 #pragma unroll
     for (int k = 0; k < kWaves; ++k) {
@@ -192,21 +194,21 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
       s_wcnt[k][dd] = cnt;
       cnt += v;
     }
-  }
-  // tile-local exclusive scan of the 256 digit counts
-  uint32_t inc = cnt;
+    // tile-local exclusive scan of the 256 digit counts
+    inc = cnt;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(inc, d);
-    if (l >= d) inc += o;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d);
+      if (l >= d) inc += o;
+    }
+    if (l == 63) s_wsum[w] = inc;
   }
-  if (l == 63) s_wsum[w] = inc;
   __syncthreads();
-  uint32_t wpre = 0;
-  for (int k = 0; k < w; ++k) wpre += s_wsum[k];
-  s_excl[t] = wpre + inc - cnt;
   // publish this tile's count of digit t and look back for the tiles before it
-  {
+  if (t < kRadix) {
+    uint32_t wpre = 0;
+    for (int k = 0; k < w; ++k) wpre += s_wsum[k];
+    s_excl[t] = wpre + inc - cnt;
     const int dd = t;
     uint64_t *st = status + uint64_t(tile) * kRadix + dd;
     uint64_t prefix = 0;
@@ -246,8 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
     s_gbase[dd] = uint64_t(digit_base[dd]) + prefix;
   }
   __syncthreads();
-  // Stage the tile sorted by digit one column at a time through one LDS buffer (32 KB instead
-  // of 48 for keys and positions together, so four blocks fit a compute unit): the key round
+  // Stage the tile sorted by digit one column at a time through one LDS buffer (64 KB instead
+  // of 96 for keys and positions together, so two blocks fit a compute unit): the key round
   // gives every sorted slot its destination, kept in registers for the payload round.
   const uint32_t rows = static_cast<uint32_t>(n - t0 < uint64_t(kTile) ? n - t0 : uint64_t(kTile));
   uint32_t at[kItems];
@@ -264,11 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
     if (r < n) sk[at[i]] = key[i];
   }
   __syncthreads();
-  constexpr int kSlots = kTile / kBlock;
+  constexpr int kSlots = kTile / kSortBlock;
   uint32_t gj[kSlots];  // destinations below 2^32: at most 2^32 - 1 rows per sort
 #pragma unroll
   for (int q = 0; q < kSlots; ++q) {
-    const uint32_t j = uint32_t(t) + uint32_t(q) * kBlock;
+    const uint32_t j = uint32_t(t) + uint32_t(q) * kSortBlock;
     gj[q] = 0;
     if (j < rows) {
       const U k = sk[j];
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassArgs<U, V> a) {
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kSlots; ++q) {
-    const uint32_t j = uint32_t(t) + uint32_t(q) * kBlock;
+    const uint32_t j = uint32_t(t) + uint32_t(q) * kSortBlock;
     if (j < rows) {
       if (LAST && POS) out_pos[gj[q]] = static_cast<int64_t>(sp[j]);
       else pos_out[gj[q]] = sp[j];
@@ -328,7 +330,7 @@ void run_pass(Ctx &c, int32_t type, const void *kin, const V *pin, const uint64_
   a.epoch = next_epoch(c);
   a.ticket = c.scratch.counters;
   a.ntiles = uint32_t(tiles);
-  k_pass<U, V, FIRST, LAST, false><<<uint32_t(tiles), kBlock, 0, c.stream>>>(a);
+  k_pass<U, V, FIRST, LAST, false><<<uint32_t(tiles), kSortBlock, 0, c.stream>>>(a);
   check_hip(hipGetLastError(), "radix pass");
 }
 
@@ -409,7 +411,7 @@ void run_pass_batch(Ctx &c, int32_t type, const BatchPlan &p, int P, int d, cons
   a.tile_item = p.tile_item;
   a.P = P;
   a.d = d;
-  k_pass<U, V, FIRST, LAST, true><<<p.ntiles, kBlock, 0, c.stream>>>(a);
+  k_pass<U, V, FIRST, LAST, true><<<p.ntiles, kSortBlock, 0, c.stream>>>(a);
   check_hip(hipGetLastError(), "radix batch pass");
 }
 
