@@ -6,7 +6,7 @@
 // skipped (radix.rs:187-259).
 //
 // MI355X form: the counting pass builds all digit histograms in LDS per block. Each
-// remaining digit is one pass over 8192-row tiles (512 threads x 16 rows): the rows of a wave are ranked among
+// remaining digit is one pass over 16384-row tiles (1024 threads x 16 rows): the rows of a wave are ranked among
 // equal digits with eight ballots (a wave64 match), per-wave digit counters in LDS give
 // ranks in row order, each tile publishes its 256 digit counts and learns the counts of
 // the tiles before it by decoupled look-back per digit, and the rows are staged sorted
@@ -23,8 +23,8 @@ namespace vk {
 namespace {
 
 constexpr int kItems = 16;
-constexpr int kSortBlock = 512;              // threads of a pass block
-constexpr int kTile = kSortBlock * kItems;   // 8192 rows
+constexpr int kSortBlock = 1024;             // threads of a pass block
+constexpr int kTile = kSortBlock * kItems;   // 16384 rows
 constexpr int kWaves = kSortBlock / kWave;
 constexpr int kRadix = 256;
 constexpr int32_t kRawU32 = 100;  // internal key type: u32 keys sorted as they are
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
     s_gbase[dd] = uint64_t(digit_base[dd]) + prefix;
   }
   __syncthreads();
-  // Stage the tile sorted by digit one column at a time through one LDS buffer (64 KB instead
-  // of 96 for keys and positions together, so two blocks fit a compute unit): the key round
+  // Stage the tile sorted by digit one column at a time through one LDS buffer (128 KB; keys
+  // and positions together would not fit the 160 KB of a compute unit): the key round
   // gives every sorted slot its destination, kept in registers for the payload round.
   const uint32_t rows = static_cast<uint32_t>(n - t0 < uint64_t(kTile) ? n - t0 : uint64_t(kTile));
   uint32_t at[kItems];
