@@ -190,7 +190,7 @@ def batched_suite(D, count: int):
     tfg, rfg = B.timed(B.plan_sums, D, plan, repeat=5)
     tuning = plan.tuning().strip()
     plan.close()
-    if not all(a.bits() == b.bits() == c.bits() for a, b, c in zip(rfp, rfb, rfg)):
+    if not all(a.bits() == b.bits() == int(c) for a, b, c in zip(rfp, rfb, rfg)):
         sys.exit("filter+sum results differ between plain, batched and the batch plan")
     filter_sum = {"statements": "s = reduce_add_f64(filter_gt_f64(x, 0.9)) per array",
                   "plain_ms": round(tfp * 1e3, 1), "batched_ms": round(tfb * 1e3, 1),

@@ -258,6 +258,9 @@ int vk_bplan_run(vk_bplan *plan, vk_status *st);
 /* Statement `stmt`'s result for item `item` after the last run (waits for it): an array (a view
  * of the plan's buffer, one reference owned by the caller) or a scalar. */
 int vk_bplan_result(vk_bplan *plan, size_t stmt, size_t item, vk_value *out, vk_status *st);
+/* Statement `stmt`'s results for every item at once after the last run (waits for it): a
+ * selection's kept row counts, or every item's scalar result bits (out has nitems words). */
+int vk_bplan_counts(vk_bplan *plan, size_t stmt, uint64_t *out, vk_status *st);
 /* The autotuner's measurements and choices, one line per tuned statement; returns the full
  * length (copies at most cap - 1 bytes and a terminating zero). */
 size_t vk_bplan_tuning(const vk_bplan *plan, char *buf, size_t cap);

@@ -91,6 +91,9 @@ def test_plan_filter_sum_and_reductions(D, tune):
             # the oracle for the filter and its canonical sum
             kept = O.filter_cmp(x, "gt", 0.9)
             assert plan.result(1, i).bits() == np.float64(O.canonical_sum_f64(kept)).view(np.uint64)
+        # every item's results at once: the kept counts and the sums' bits
+        assert [int(v) for v in plan.bits(0)] == [len(want[i][0]) for i in range(len(xs))]
+        assert [int(v) for v in plan.bits(1)] == [want[i][1].bits() for i in range(len(xs))]
         # eager work between replays: other calls on the same context, then a sync
         junk = D.call("filter_gt_f64", [arrs[-1], S("f64", 0.3)])
         D.call("reduce_add_f64", [junk])

@@ -1729,6 +1729,26 @@ int vk_bplan_result(vk_bplan *plan, size_t stmt, size_t item, vk_value *out, vk_
   return guarded(st, [&] { *out = plan_result(*plan, stmt, item); });
 }
 
+int vk_bplan_counts(vk_bplan *plan, size_t stmt, uint64_t *out, vk_status *st) {
+  return guarded(st, [&] {
+    if (!plan->compiled) usage_error("the plan is not compiled");
+    if (stmt >= plan->st.size()) usage_error("plan result out of range");
+    plan_fetch(*plan);
+    const PlanStmt &s = plan->st[stmt];
+    if (s.kind == 0) {
+      std::memcpy(out, s.host.data(), plan->nitems * 8);
+    } else if (s.kind == 1) {
+      std::memcpy(out, s.host.data(), plan->nitems * 8);
+    } else {
+      for (size_t i = 0; i < plan->nitems; ++i) {
+        uint64_t w[3] = {0, 0, 0};
+        for (uint32_t q = 0; q < s.words; ++q) w[q] = s.host[i * s.words + q];
+        out[i] = finish_words(*s.d, w).bits;
+      }
+    }
+  });
+}
+
 size_t vk_bplan_tuning(const vk_bplan *plan, char *buf, size_t cap) {
   std::string all;
   for (const auto &l : plan->tuned) all += l + "\n";

@@ -525,6 +525,15 @@ class BatchPlan:
             return Scalar.from_c(out.scalar)
         return Array(self.ctx, out.array)
 
+    def bits(self, stmt: int) -> np.ndarray:
+        """Statement `stmt`'s result for every item as u64 words (vk_bplan_counts): a
+        selection's kept row counts, or each item's scalar bits."""
+        out = np.empty(self.nitems, np.uint64)
+        st = N.Status()
+        _check(self._lib.vk_bplan_counts(self.handle, stmt, out.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         C.byref(st)), st)
+        return out
+
     def tuning(self) -> str:
         n = self._lib.vk_bplan_tuning(self.handle, None, 0)
         buf = C.create_string_buffer(n + 1)

@@ -93,6 +93,7 @@ SIGNATURES = {
     "vk_bplan_compile": (C.c_int, [_P, C.c_int, _ST]),
     "vk_bplan_run": (C.c_int, [_P, _ST]),
     "vk_bplan_result": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(CValue), _ST]),
+    "vk_bplan_counts": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64), _ST]),
     "vk_bplan_tuning": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
     "vk_bplan_free": (None, [_P]),
 }
