@@ -52,6 +52,24 @@ template <>
 __device__ __forceinline__ bool ord_less<float>(float a, float b) {
   return a < b || (a == b && __signbit(a) && !__signbit(b));
 }
+// An integer image that orders like Ordered::less (reduce.rs:183-199) for non-NaN values:
+// floats map negative values to their complemented magnitude (so -0.0 < +0.0), integers to
+// themselves (u64 with the sign bit flipped).
+template <typename T>
+__device__ __forceinline__ int64_t ord_key(T v) {
+  if constexpr (std::is_same<T, double>::value) {
+    const int64_t b = static_cast<int64_t>(to_bits(v));
+    return b < 0 ? (b ^ INT64_MAX) : b;
+  } else if constexpr (std::is_same<T, float>::value) {
+    const int32_t b = static_cast<int32_t>(to_bits(v));
+    return b < 0 ? int64_t(b ^ INT32_MAX) : int64_t(b);
+  } else if constexpr (std::is_same<T, uint64_t>::value) {
+    return static_cast<int64_t>(v ^ 0x8000000000000000ull);
+  } else {
+    return static_cast<int64_t>(v);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ bool is_nan(T) { return false; }
 template <>
@@ -145,6 +163,18 @@ struct OpArg {
       a.idx = i;
     }
   }
+  // The same fold for rows arriving in increasing order: an equal value never wins, so a row
+  // wins when its ordered image (ord_key) is strictly more extreme, one integer compare; the
+  // first NaN is the smallest NaN row.
+  __device__ void add_inorder(Acc &a, T v, uint64_t row) const {
+    const int64_t i = static_cast<int64_t>(base + row);
+    const bool nan = is_nan(v);
+    a.nan = nan && i < a.nan ? i : a.nan;
+    const int64_t kv = ord_key(v), kb = ord_key(from_bits<T>(a.bits));
+    const bool win = !nan && (a.idx == kNone || (want_max ? kv > kb : kv < kb));
+    a.bits = win ? to_bits(v) : a.bits;
+    a.idx = win ? i : a.idx;
+  }
   __device__ void merge(Acc &a, const Acc &b) const {
     if (b.nan < a.nan) a.nan = b.nan;
     if (b.idx == kNone) return;
@@ -154,6 +184,18 @@ struct OpArg {
     }
   }
 };
+
+// Rows a thread folds in increasing row order: an operator may offer add_inorder, a cheaper
+// fold that relies on it (see OpArg); the others fold as usual.
+template <class Op, class Acc, typename T>
+__device__ __forceinline__ auto add_inorder(const Op &op, Acc &a, T v, uint64_t row, int)
+    -> decltype(op.add_inorder(a, v, row), void()) {
+  op.add_inorder(a, v, row);
+}
+template <class Op, class Acc, typename T>
+__device__ __forceinline__ void add_inorder(const Op &op, Acc &a, T v, uint64_t row, long) {
+  op.add(a, v, row);
+}
 
 // ------------------------------------------------------------------- the kernel
 
@@ -229,7 +271,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(Op op, const T *__restri
     for (int u = 0; u < kRedLoads; ++u) {
       const uint64_t i = base + uint64_t(u) * kRedThreads + t;
 #pragma unroll
-      for (int k = 0; k < V; ++k) op.add(a, v[u].e[k], head + i * V + k);
+      for (int k = 0; k < V; ++k) add_inorder(op, a, v[u].e[k], head + i * V + k, 0);
     }
   } else {
 #pragma unroll
@@ -242,13 +284,17 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(Op op, const T *__restri
       const uint64_t i = base + uint64_t(u) * kRedThreads + t;
       if (i < nv) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) op.add(a, v[u].e[k], head + i * V + k);
+        for (int k = 0; k < V; ++k) add_inorder(op, a, v[u].e[k], head + i * V + k, 0);
       }
     }
   }
   if (blockIdx.x == 0) {
-    if (uint64_t(t) < head) op.add(a, x[t], t);
-    for (uint64_t r = head + nv * V + t; r < n; r += kRedThreads) op.add(a, x[r], r);
+    // rows before the first 16-byte boundary and after the last: out of this thread's order,
+    // so folded apart and merged
+    Acc b = op.identity();
+    if (uint64_t(t) < head) op.add(b, x[t], t);
+    for (uint64_t r = head + nv * V + t; r < n; r += kRedThreads) op.add(b, x[r], r);
+    op.merge(a, b);
   }
   a = block_merge_big(op, a);
   if (t == 0) partials[blockIdx.x] = a;
