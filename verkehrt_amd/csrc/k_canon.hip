@@ -19,6 +19,11 @@ namespace {
 // K is a template argument so that row loads are plain, independent loads.
 template <int K>
 struct Row {
+  // the summary kernel's passes per thread and the waves per SIMD it is built for: two-input
+  // rows (dot) take one pass at four waves, one-input rows two passes at six (spilling no more
+  // than a pair of values)
+  static constexpr int kSummPasses = K == 1 ? 1 : 2;
+  static constexpr int kSummWaves = K == 1 ? 1 : 6;
   double c, e;
   int keep;  // condsum: the outcomes of comparing x with c that keep x (bit 0 <, 1 ==, 2 >, 3 unordered)
   __device__ __forceinline__ double operator()(const double *x, const double *y, uint64_t i) const {
@@ -713,8 +718,19 @@ __global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asu
 // 2. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the start;
 // also writes the tile's approximate chain sums: each run's sum in order, then runs joined in
 // the order of the summary tree. Block 0 clears the two counters of steps 4 and 5.
+// With P = RV::kSummPasses passes, a block has 1024 / P threads and thread t summarises runs
+// t / 16 + k * 64 / P (k < P) of chain t % 16 in turn. One-input rows take two passes: a block
+// needs half the registers of one run per thread and three blocks fit a compute unit. Two-input
+// rows (dot) hold twice the loads per run and take one pass over 1024 threads. The runs of wave
+// w in pass k are run group w + k * 16 / P, so the 16 groups keep the tile's run order for the
+// summary tree.
 template <class RV>
-__global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double *__restrict__ x,
+constexpr int summ_threads() {
+  return 1024 / RV::kSummPasses;
+}
+
+template <class RV>
+__global__ __launch_bounds__(summ_threads<RV>()) __attribute__((amdgpu_waves_per_eu(RV::kSummWaves, 8))) void k_canon_summ(RV rv, const double *__restrict__ x,
                                                              const double *__restrict__ y, uint64_t blocks,
                                                              const double *pre, uint64_t tiles,
                                                              TileSum *sums, double *asum, uint32_t *counts) {
@@ -722,46 +738,52 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_summ(RV rv, const double
   __shared__ double s_a[kFastWaves][16];
   const uint64_t tile = blockIdx.x;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  const int j = t & 15, r = t >> 4;
+  const int j = t & 15;
   if (tile == 0 && t < 2) counts[t] = 0;
   const uint64_t b0 = tile * kTileBlocks;
   const int nb = static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks);
   const double start = pre[tile * 16 + j];
   const Binade h = binade_of(start);
-  const int cnt = nb - r * kRunBlocks;
-  RunFn f = run_identity();
-  double a = 0.0;
-  {
-    double v[kRunBlocks];
-    load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
-    // the one-track pass also sums the run; a tie (rare) reloads the run for the two-track
-    // form, so the loaded rows die as the pass consumes them
-    RunFn f1;
-    const bool one = run_of_one(v, cnt, binade_ref(h), &f1, &a);
-    if (h.ok) {
-      if (__builtin_expect(one, 1)) {
-        f = f1;
-      } else {
-        double w2[kRunBlocks];
-        load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, w2);
-        f = run_of_two(w2, cnt, binade_ref(h));
+  constexpr int kThreads = summ_threads<RV>();
+#pragma unroll 1
+  for (int pass = 0; pass < RV::kSummPasses; ++pass) {
+    const int r = (t >> 4) + pass * (kThreads / 16);
+    const int cnt = nb - r * kRunBlocks;
+    RunFn f = run_identity();
+    double a = 0.0;
+    {
+      double v[kRunBlocks];
+      load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
+      // the one-track pass also sums the run; a tie (rare) reloads the run for the two-track
+      // form, so the loaded rows die as the pass consumes them
+      RunFn f1;
+      const bool one = run_of_one(v, cnt, binade_ref(h), &f1, &a);
+      if (h.ok) {
+        if (__builtin_expect(one, 1)) {
+          f = f1;
+        } else {
+          double w2[kRunBlocks];
+          load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, w2);
+          f = run_of_two(w2, cnt, binade_ref(h));
+        }
       }
     }
+    a = a + __shfl_xor(a, 16);
+    a = a + __shfl_xor(a, 32);
+    const int grp = w + pass * (kThreads / kWave);
+    if (l < 16) s_a[grp][j] = a;
+    {
+      RunFn o = shfl_xor_pod(f, 16);
+      f = (l & 16) ? run_join(o, f) : run_join(f, o);
+      o = shfl_xor_pod(f, 32);
+      f = (l & 32) ? run_join(o, f) : run_join(f, o);
+    }
+    if (l < 16) s_f[grp][j] = f;
   }
-  a = a + __shfl_xor(a, 16);
-  a = a + __shfl_xor(a, 32);
-  if (l < 16) s_a[w][j] = a;
-  {
-    RunFn o = shfl_xor_pod(f, 16);
-    f = (l & 16) ? run_join(o, f) : run_join(f, o);
-    o = shfl_xor_pod(f, 32);
-    f = (l & 32) ? run_join(o, f) : run_join(f, o);
-  }
-  if (l < 16) s_f[w][j] = f;
   __syncthreads();
   if (w != 0) return;
   const RunFn ft = tile_tree(s_f, j, l);
-  a = (s_a[4 * (l >> 4)][j] + s_a[4 * (l >> 4) + 1][j]) + (s_a[4 * (l >> 4) + 2][j] + s_a[4 * (l >> 4) + 3][j]);
+  double a = (s_a[4 * (l >> 4)][j] + s_a[4 * (l >> 4) + 1][j]) + (s_a[4 * (l >> 4) + 2][j] + s_a[4 * (l >> 4) + 3][j]);
   a = a + __shfl_xor(a, 16);
   a = a + __shfl_xor(a, 32);
   if (l < 16) {
@@ -1164,7 +1186,7 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
     const uint32_t fix_grid = static_cast<uint32_t>(tiles < 512 ? tiles : 512);
     k_canon_sample<RV><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y, blocks, tiles, asum);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
-    k_canon_summ<RV><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum,
+    k_canon_summ<RV><<<uint32_t(tiles), summ_threads<RV>(), 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum,
                                                                     counts);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
     k_canon_check<<<uint32_t((tiles * 16 + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(pre, tiles, sums, list,
