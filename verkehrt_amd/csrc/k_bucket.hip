@@ -108,49 +108,101 @@ __global__ void k_chain_keys(const int64_t *__restrict__ keys, uint64_t b, K *ck
     ck[i] = static_cast<K>(static_cast<uint64_t>(keys[i]) * 16 + (i & 15));
 }
 
-// One thread per chain start: folds the chain's values (a stable sort by chain key keeps
-// them in index order).
-template <typename K>
-__global__ void k_chain_fold_thread(const K *__restrict__ sk, const double *__restrict__ sv,
-                                    uint64_t b, double *stripes) {
-  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
-       p += uint64_t(gridDim.x) * blockDim.x) {
-    const K k = sk[p];
-    if (p > 0 && sk[p - 1] == k) continue;
-    double a = 0.0;
-    uint64_t q = p;
-    while (q < b && sk[q] == k) {
-      a = a + sv[q];
-      ++q;
-    }
-    stripes[k] = a;
-  }
-}
+// One wave per chain start for long chains: the lanes load 64 values at a time, the wave folds
+// them in order with constant-lane reads (readlane, no LDS round trip) while the next chunks
+// load. Every lane holds the same running value.
+constexpr int kFoldWaves = 4;  // chains per block of the wave fold
 
-// One wave per chain start for long chains: lanes load 64 values ahead, lane 0 folds them.
 template <typename K>
-__global__ __launch_bounds__(64) void k_chain_fold_wave(const uint64_t *__restrict__ starts,
-                                                         uint64_t nchains,
-                                                         const K *__restrict__ sk,
-                                                         const double *__restrict__ sv, uint64_t b,
-                                                         double *stripes) {
+__global__ __launch_bounds__(kFoldWaves * kWave) void k_chain_fold_wave(const uint64_t *__restrict__ starts,
+                                                                       uint64_t nchains,
+                                                                       const K *__restrict__ sk,
+                                                                       const double *__restrict__ sv,
+                                                                       uint64_t b, double *stripes) {
   const int l = lane_id();
-  for (uint64_t ch = blockIdx.x; ch < nchains; ch += gridDim.x) {
+  const uint64_t w0 = blockIdx.x * uint64_t(kFoldWaves) + threadIdx.x / kWave;
+  for (uint64_t ch = w0; ch < nchains; ch += uint64_t(gridDim.x) * kFoldWaves) {
     const uint64_t p0 = starts[ch];
     const uint64_t p1 = ch + 1 < nchains ? starts[ch + 1] : b;
     const uint64_t k = sk[p0];
     double a = 0.0;
-    for (uint64_t q = p0; q < p1; q += 64) {
-      const uint64_t r = q + l;
-      const double v = r < p1 ? sv[r] : 0.0;
-      const int cnt = static_cast<int>(p1 - q < 64 ? p1 - q : 64);
-      for (int j = 0; j < cnt; ++j) {
-        const double u = __shfl(v, j);
-        a = a + u;
+    // kAhead chunks of 64 values in flight: chunk d of each round is folded, then refilled
+    // with the chunk kAhead further on
+    constexpr int kAhead = 8;
+    double v[kAhead];
+#pragma unroll
+    for (int d = 0; d < kAhead; ++d) {
+      const uint64_t r = p0 + d * 64 + l;
+      v[d] = r < p1 ? sv[r] : 0.0;
+    }
+    for (uint64_t q = p0; q < p1; q += kAhead * 64) {
+#pragma unroll
+      for (int d = 0; d < kAhead; ++d) {
+        const uint64_t qd = q + d * 64;
+        if (qd >= p1) break;
+        if (p1 - qd >= 64) {
+#pragma unroll
+          for (int j = 0; j < 64; ++j) a = a + bcast_pod(v[d], j);
+        } else {
+          const int cnt = static_cast<int>(p1 - qd);
+          for (int j = 0; j < cnt; ++j) a = a + __shfl(v[d], j);
+        }
+        const uint64_t r = qd + kAhead * 64 + l;
+        v[d] = r < p1 ? sv[r] : 0.0;
       }
     }
     if (l == 0) stripes[k] = a;
   }
+}
+
+// One thread per bucket (the compacted bucket starts) for short chains: the bucket's rows are
+// its chains in stripe order, each in index order; every chain folds from 0.0 and the 16
+// stripes (0.0 where a stripe has no rows) combine pairwise into out[bucket], as
+// k_combine_stripes does.
+template <typename K>
+__global__ void k_bucket_fold(const uint64_t *__restrict__ starts, uint64_t nstarts,
+                              const K *__restrict__ sk, const double *__restrict__ sv, uint64_t b,
+                              double *out) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nstarts;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t p0 = starts[i];
+    const uint64_t p1 = i + 1 < nstarts ? starts[i + 1] : b;
+    const uint64_t bucket = static_cast<uint64_t>(sk[p0]) >> 4;
+    double a[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = 0.0;
+    int cur = static_cast<int>(sk[p0] & 15);
+    double acc = 0.0;
+    auto put = [&](int s, double v) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j == s) a[j] = v;
+    };
+    for (uint64_t q = p0; q < p1; ++q) {
+      const int s = static_cast<int>(sk[q] & 15);
+      const double v = sv[q];
+      if (s != cur) {
+        put(cur, acc);
+        acc = 0.0;
+        cur = s;
+      }
+      acc = acc + v;
+    }
+    put(cur, acc);
+#pragma unroll
+    for (int step = 8; step > 0; step /= 2)
+#pragma unroll
+      for (int j = 0; j < step; ++j) a[j] = a[j] + a[j + step];
+    out[bucket] = a[0];
+  }
+}
+
+// flag[p] = 1 where row p of the sorted chain keys starts a bucket (chain key / 16 changes)
+template <typename K>
+__global__ void k_bucket_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag) {
+  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
+       p += uint64_t(gridDim.x) * blockDim.x)
+    flag[p] = (p == 0 || (sk[p - 1] >> 4) != (sk[p] >> 4)) ? 1 : 0;
 }
 
 template <typename K>
@@ -241,10 +293,13 @@ void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint6
 namespace {
 
 // The chains of rows 0..b (bucket, i % 16): a stable sort of (chain key, value) pairs puts each
-// chain's values together in index order; then each chain is folded sequentially.
+// chain's values together in index order, and a bucket's chains together in stripe order.
+// Long chains (few buckets) fold one wave per chain into the stripes, which then combine per
+// bucket; short chains fold one thread per bucket, straight into out (no stripe buffer).
+// Returns true when out holds the combined buckets.
 template <typename K>
-void fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
-                 double *stripes) {
+bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
+                 double *stripes, double *out) {
   constexpr bool wide = sizeof(K) == 8;
   K *ck = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
   K *sk = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
@@ -254,28 +309,33 @@ void fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
   check_hip(hipGetLastError(), "chain keys");
   sort_key_values(c, wide, ck, reinterpret_cast<const uint64_t *>(vals), b, sk,
                   reinterpret_cast<uint64_t *>(sv));
+  tmp_free(c, ck);
   const uint64_t chains_est = buckets * 16 < b ? buckets * 16 : b;
-  if (b / chains_est >= 256) {
-    // long chains: compact the chain starts, one wave per chain
-    uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
-    uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-    k_chain_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
-    SelectSpec ss{2, VK_I64, 0, 0, 0};
-    uint64_t *d_cnt = c.scratch.slots + 20;
-    select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
-    uint64_t nchains = 0;
-    read_back(c, d_cnt, 8, &nchains);
-    k_chain_fold_wave<K><<<stream_grid(nchains, 1, 16384), 64, 0, c.stream>>>(starts, nchains, sk, sv, b,
-                                                                                stripes);
-    tmp_free(c, flag);
-    tmp_free(c, starts);
+  const bool long_chains = b / chains_est >= 256;
+  // the starts of chains (long) or buckets (short), compacted in order
+  uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
+  uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
+  if (long_chains) k_chain_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+  else k_bucket_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+  SelectSpec ss{2, VK_I64, 0, 0, 0};
+  uint64_t *d_cnt = c.scratch.slots + 20;
+  select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
+  uint64_t nstarts = 0;
+  read_back(c, d_cnt, 8, &nstarts);
+  if (long_chains) {
+    k_chain_fold_wave<K><<<stream_grid(nstarts, kFoldWaves, 16384), kFoldWaves * kWave, 0, c.stream>>>(
+        starts, nstarts, sk, sv, b, stripes);
   } else {
-    k_chain_fold_thread<K><<<g, kBlock, 0, c.stream>>>(sk, sv, b, stripes);
+    k_fill<double><<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(out, buckets, 0.0);
+    k_bucket_fold<K><<<stream_grid(nstarts, kBlock, 8192), kBlock, 0, c.stream>>>(starts, nstarts, sk, sv, b,
+                                                                                  out);
   }
   check_hip(hipGetLastError(), "chain fold");
-  tmp_free(c, ck);
+  tmp_free(c, flag);
+  tmp_free(c, starts);
   tmp_free(c, sk);
   tmp_free(c, sv);
+  return !long_chains;
 }
 
 }  // namespace
@@ -439,16 +499,26 @@ void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64
                          uint64_t buckets, double *out) {
   if (buckets == 0) return;
   const uint64_t b = n - n % 16;
-  double *stripes = static_cast<double *>(tmp_alloc(c, buckets * 16 * 8));
-  k_fill<double><<<stream_grid(buckets * 16, kBlock, 4096), kBlock, 0, c.stream>>>(stripes, buckets * 16, 0.0);
-  if (b > 0) {
-    if (buckets * 16 <= 0xFFFFFFFFull) fold_chains<uint32_t>(c, keys, vals, b, buckets, stripes);
-    else fold_chains<uint64_t>(c, keys, vals, b, buckets, stripes);
+  // the stripe buffer is needed only for long chains (buckets <= b / 4096)
+  const uint64_t chains_est = b == 0 ? 0 : (buckets * 16 < b ? buckets * 16 : b);
+  const bool long_chains = b > 0 && b / chains_est >= 256;
+  double *stripes = nullptr;
+  if (long_chains) {
+    stripes = static_cast<double *>(tmp_alloc(c, buckets * 16 * 8));
+    k_fill<double><<<stream_grid(buckets * 16, kBlock, 4096), kBlock, 0, c.stream>>>(stripes, buckets * 16, 0.0);
   }
-  k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, buckets, out);
+  bool combined = false;
+  if (b > 0) {
+    if (buckets * 16 <= 0xFFFFFFFFull) combined = fold_chains<uint32_t>(c, keys, vals, b, buckets, stripes, out);
+    else combined = fold_chains<uint64_t>(c, keys, vals, b, buckets, stripes, out);
+  }
+  if (long_chains)
+    k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, buckets, out);
+  else if (!combined)
+    k_fill<double><<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(out, buckets, 0.0);
   if (n > b) k_fold_tail<<<1, 64, 0, c.stream>>>(keys, vals, b, n, out);
   check_hip(hipGetLastError(), "f64 reduce_into");
-  tmp_free(c, stripes);
+  if (stripes) tmp_free(c, stripes);
 }
 
 }  // namespace vk

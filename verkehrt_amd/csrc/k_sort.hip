@@ -46,6 +46,25 @@ __device__ __forceinline__ U image(int32_t type, const void *keys, uint64_t r) {
 
 // ------------------------------------------------------------- counting pass
 
+// Adds one row's digits to the block's LDS histograms. A digit that every active lane of the
+// wave shares (the constant high digits of small keys, runs of equal keys) is added once, by
+// the first active lane, instead of by 64 atomics on one LDS word.
+template <typename U>
+__device__ __forceinline__ void hist_add(uint32_t (&s_h)[sizeof(U)][kRadix], U k) {
+  const uint64_t act = __ballot(1);
+  const int lead = __ffsll(static_cast<unsigned long long>(act)) - 1;
+#pragma unroll
+  for (int d = 0; d < static_cast<int>(sizeof(U)); ++d) {
+    const uint32_t dig = static_cast<uint32_t>(k >> (8 * d)) & 255;
+    const uint32_t first = __builtin_amdgcn_readfirstlane(dig);
+    if (__ballot(dig == first) == act) {
+      if (static_cast<int>(lane_id()) == lead) atomicAdd(&s_h[d][first], static_cast<uint32_t>(__popcll(act)));
+    } else {
+      atomicAdd(&s_h[d][dig], 1u);
+    }
+  }
+}
+
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys, uint64_t n,
                                                  uint32_t *hist) {
@@ -55,9 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys,
   __syncthreads();
   for (uint64_t r = blockIdx.x * uint64_t(kBlock) + threadIdx.x; r < n;
        r += uint64_t(gridDim.x) * kBlock) {
-    const U k = image<U>(type, keys, r);
-#pragma unroll
-    for (int d = 0; d < P; ++d) atomicAdd(&s_h[d][(k >> (8 * d)) & 255], 1u);
+    hist_add<U>(s_h, image<U>(type, keys, r));
   }
   __syncthreads();
   for (int k = threadIdx.x; k < P * kRadix; k += kBlock) {
@@ -352,9 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_hist_batch(int32_t type, const Batch
   const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
   const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
   for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) {
-    const U k = image<U>(type, I.a, r);
-#pragma unroll
-    for (int d = 0; d < P; ++d) atomicAdd(&s_h[d][(k >> (8 * d)) & 255], 1u);
+    hist_add<U>(s_h, image<U>(type, I.a, r));
   }
   __syncthreads();
   uint32_t *h = hist + uint64_t(it) * P * kRadix;
