@@ -479,6 +479,25 @@ def test_group_id(D):
         assert np.array_equal(D.call("group_id_u8", [view(D, xu, 3)]).to_numpy(), O.group_id(xu))
 
 
+def test_group_id_large(D):
+    """Columns past 2^20 rows: a few distinct keys (the small global table, every key held by
+    the blocks' LDS tables), more keys than an LDS table takes, more keys than the small table
+    takes (its overflow, then the full table), and the LDS empty marker's bit pattern as a key."""
+    rng = np.random.default_rng(40)
+    n = 3_000_000
+    marker = np.array([0xFFF8DEADBEEF1357], np.uint64).view(np.int64)[0]
+    for d in (1, 7, 5000, 100_000, n):
+        base = rng.integers(-2**63, 2**63 - 1, d, dtype=np.int64)
+        base[d // 2] = marker
+        x = base[rng.integers(0, d, n)] if d < n else base
+        got = D.call("group_id_i64", [view(D, x, 1)]).to_numpy()
+        assert np.array_equal(got, O.group_id(x)), d
+    xi = rng.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+    assert np.array_equal(D.call("group_id_i32", [D.array(xi)]).to_numpy(), O.group_id(xi))
+    xu = rng.integers(0, 256, n).astype(np.uint8)
+    assert np.array_equal(D.call("group_id_u8", [D.array(xu)]).to_numpy(), O.group_id(xu))
+
+
 # ----------------------------------------------------------------- elementwise
 
 def test_elementwise(D):

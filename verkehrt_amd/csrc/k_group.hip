@@ -3,7 +3,8 @@
 // depend only on the order in which distinct keys first appear (group.rs:1-5), so the table
 // layout is free.
 //
-// MI355X form, four passes:
+// The two-key form and the batched form, four passes over a global table (the one-key form
+// puts a per-block LDS table in front of it; see the single-key section):
 //  1. insert: every row claims or finds its key's slot in an open-addressing table of
 //     u64 (min row + 1) words, claiming with compare-and-swap and lowering the word with
 //     atomicMin, so the word ends at the key's first row + 1;
@@ -107,6 +108,230 @@ __global__ void k_ids(const uint32_t *__restrict__ slot, const uint32_t *__restr
   for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
        r += uint64_t(gridDim.x) * blockDim.x)
     out[r] = id_of_slot[slot[r]];
+}
+
+// ------------------------------------------------------------------ single-key form
+// group_id_<t> over one key column, in five passes around a per-block LDS hash table that sits
+// in front of the global table, so that the rows of a column with few distinct keys stay on
+// the compute unit:
+//  1. insert (k_insert1): rows fold into the block's LDS table (key -> least row); a key the
+//     LDS table cannot take (its new-entry budget spent, a probe run too long, or the key equal
+//     to the LDS empty marker) goes straight to the global table, and at the end each LDS entry
+//     is inserted into the global table once. The global table starts at kSmallSlots slots; a
+//     probe run longer than kMaxRun there marks overflow, and the pass is redone on 2n slots;
+//  2. every occupied global slot flags its least row (k_first_flags);
+//  3. the flagged rows in row order (the selection kernel) are the keys' first rows;
+//  4. the slot of the j-th first row gets id j (k_number1), and every slot's key is copied next
+//     to the table (k_slot_keys), so that searches compare cached slot keys;
+//  5. every row maps its key to its id through the LDS table again, each key new to the block
+//     resolved once from the global table, and writes it (k_ids1); with more than kLdsGroups
+//     keys, where LDS tables seldom hit, every row searches the global table (k_ids2).
+// When the small table overflows (more than about a million distinct keys), passes 2-5 are
+// those of the two-key form instead (probe, select, number, ids): the table is then too large
+// for a copy of its keys to stay in the caches.
+constexpr int kGBlock = 1024;
+constexpr int kGRows = 8;  // rows per thread per chunk
+constexpr uint64_t kGChunk = uint64_t(kGBlock) * kGRows;
+constexpr int kLBits = 12;
+constexpr int kLSlots = 1 << kLBits;  // LDS entries: 32 KB of keys, 16 KB of values
+constexpr uint32_t kLFull = kLSlots / 2;  // no new entries beyond half full
+constexpr int kLProbes = 32;
+constexpr uint64_t kLEmpty = 0xFFF8DEADBEEF1357ull;  // LDS empty marker
+constexpr int kSmallLg = 21;
+constexpr uint64_t kSmallSlots = uint64_t(1) << kSmallLg;
+constexpr int kMaxRun = 64;  // longest probe run allowed in the small table
+constexpr int kNoLimit = 0x7FFFFFFF;
+constexpr uint64_t kLdsGroups = uint64_t(1) << 16;  // more keys than this: k_ids2 instead of k_ids1
+
+// The LDS entry of key k, or -1. An empty entry is claimed (*claimed set) while fewer than
+// kLFull entries are taken; after that only existing entries are found.
+__device__ __forceinline__ int lds_find(unsigned long long *s_key, uint32_t *s_cnt, uint64_t k, bool *claimed) {
+  *claimed = false;
+  if (k == kLEmpty) return -1;
+  int h = static_cast<int>(mix(k) >> (64 - kLBits));
+  const bool grow = __hip_atomic_load(s_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < kLFull;
+  for (int p = 0; p < kLProbes; ++p) {
+    const unsigned long long cur = __hip_atomic_load(&s_key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == k) return h;
+    if (cur == kLEmpty) {
+      if (!grow) return -1;
+      const unsigned long long old = atomicCAS(&s_key[h], kLEmpty, static_cast<unsigned long long>(k));
+      if (old == kLEmpty) {
+        *claimed = true;
+        atomicAdd(s_cnt, 1u);
+        return h;
+      }
+      if (old == k) return h;
+    }
+    h = (h + 1) & (kLSlots - 1);
+  }
+  return -1;
+}
+
+// Inserts (key k, row r) into the global table: claims an empty slot with r + 1, or lowers the
+// word of the slot holding k to r + 1. False when the probe run reaches max_run slots.
+__device__ bool global_insert(const KeyCol &a, unsigned long long *table, uint64_t mask, int shift,
+                              uint64_t k, uint64_t r, int max_run) {
+  uint64_t h = mix(k) >> shift;
+  for (int p = 0; p < max_run; ++p) {
+    unsigned long long s = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s == 0) {
+      const unsigned long long old = atomicCAS(&table[h], 0ull, r + 1);
+      if (old == 0) return true;
+      s = old;
+    }
+    if (a.bits(s - 1) == k) {
+      if (r + 1 < s) atomicMin(&table[h], r + 1);
+      return true;
+    }
+    h = (h + 1) & mask;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, unsigned long long *table,
+                                                     uint64_t mask, int shift, int max_run,
+                                                     uint32_t *overflow) {
+  __shared__ unsigned long long s_key[kLSlots];
+  __shared__ uint32_t s_row[kLSlots];
+  __shared__ uint32_t s_cnt;
+  const int t = threadIdx.x;
+  for (int i = t; i < kLSlots; i += kGBlock) {
+    s_key[i] = kLEmpty;
+    s_row[i] = 0xFFFFFFFFu;
+  }
+  if (t == 0) s_cnt = 0;
+  __syncthreads();
+  bool over = false;
+  for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += uint64_t(gridDim.x) * kGChunk) {
+    if (max_run != kNoLimit) {  // leave early once any block has overflowed (one read per wave)
+      uint32_t o = 0;
+      if (lane_id() == 0) o = __hip_atomic_load(overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__shfl(o, 0) != 0) {
+        over = true;
+        break;
+      }
+    }
+    uint64_t k[kGRows];
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+      k[j] = r < n ? a.bits(r) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+      if (r >= n) continue;
+      bool claimed;
+      const int h = lds_find(s_key, &s_cnt, k[j], &claimed);
+      if (h >= 0) {
+        // rows arrive in increasing order per thread, so the entry is mostly lower already
+        if (__hip_atomic_load(&s_row[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > r)
+          atomicMin(&s_row[h], static_cast<uint32_t>(r));
+      } else if (!global_insert(a, table, mask, shift, k[j], r, max_run)) over = true;
+    }
+    if (over) atomicOr(overflow, 1u);
+    over = false;
+  }
+  __syncthreads();
+  if (max_run != kNoLimit && __hip_atomic_load(overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    return;  // the pass is redone on the full table
+  for (int i = t; i < kLSlots; i += kGBlock) {
+    const uint64_t k = s_key[i];
+    if (k != kLEmpty && !global_insert(a, table, mask, shift, k, s_row[i], max_run)) atomicOr(overflow, 1u);
+  }
+}
+
+__global__ void k_first_flags(const unsigned long long *__restrict__ table, uint64_t slots, uint8_t *first) {
+  for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < slots;
+       s += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t w = table[s];
+    if (w != 0) first[w - 1] = 1;
+  }
+}
+
+// The slot of first row firsts[j] is the one whose word is firsts[j] + 1.
+__global__ void k_number1(KeyCol a, const int64_t *__restrict__ firsts, uint64_t groups,
+                          const unsigned long long *__restrict__ table, uint64_t mask, int shift,
+                          uint32_t *id_of_slot) {
+  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < groups;
+       j += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t r = static_cast<uint64_t>(firsts[j]);
+    uint64_t h = mix(a.bits(r)) >> shift;
+    for (uint64_t p = 0; p <= mask && table[h] != r + 1; ++p) h = (h + 1) & mask;
+    id_of_slot[h] = static_cast<uint32_t>(j);
+  }
+}
+
+// The key of every occupied slot, next to the table (the table is small here, so later
+// searches compare against cached slot keys instead of reading the key column at random).
+__global__ void k_slot_keys(KeyCol a, const unsigned long long *__restrict__ table, uint64_t slots,
+                            uint64_t *slot_key) {
+  for (uint64_t h = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; h < slots;
+       h += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t w = table[h];
+    slot_key[h] = w != 0 ? a.bits(w - 1) : 0;
+  }
+}
+
+// The id of key k from the global table (every key is in it).
+__device__ __forceinline__ uint32_t global_id(const unsigned long long *table, const uint64_t *slot_key,
+                                              uint64_t mask, int shift, const uint32_t *id_of_slot, uint64_t k) {
+  uint64_t h = mix(k) >> shift;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    if (table[h] != 0 && slot_key[h] == k) break;
+    h = (h + 1) & mask;
+  }
+  return id_of_slot[h];
+}
+
+// Every row's id from the global table, for more keys than the LDS tables hold.
+__global__ __launch_bounds__(kBlock) void k_ids2(KeyCol a, uint64_t n, const unsigned long long *table,
+                                                 const uint64_t *slot_key, uint64_t mask, int shift,
+                                                 const uint32_t *id_of_slot, int64_t *out) {
+  for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+       r += uint64_t(gridDim.x) * blockDim.x)
+    out[r] = global_id(table, slot_key, mask, shift, id_of_slot, a.bits(r));
+}
+
+// Phase 1 of each chunk finds (or claims) the LDS entry of every row's key, the claiming lane
+// resolving the key's id; after the barrier, phase 2 reads the ids. An entry's id is written
+// once, before the barrier that follows its claim, and never changes.
+__global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const unsigned long long *table,
+                                                  const uint64_t *slot_key, uint64_t mask, int shift,
+                                                  const uint32_t *id_of_slot, int64_t *out) {
+  __shared__ unsigned long long s_key[kLSlots];
+  __shared__ uint32_t s_id[kLSlots];
+  __shared__ uint32_t s_cnt;
+  const int t = threadIdx.x;
+  for (int i = t; i < kLSlots; i += kGBlock) s_key[i] = kLEmpty;
+  if (t == 0) s_cnt = 0;
+  __syncthreads();
+  for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += uint64_t(gridDim.x) * kGChunk) {
+    uint64_t k[kGRows];
+    int h[kGRows];
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+      k[j] = r < n ? a.bits(r) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+      h[j] = -1;
+      if (r >= n) continue;
+      bool claimed;
+      h[j] = lds_find(s_key, &s_cnt, k[j], &claimed);
+      if (claimed) s_id[h[j]] = global_id(table, slot_key, mask, shift, id_of_slot, k[j]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+      if (r >= n) continue;
+      out[r] = h[j] >= 0 ? s_id[h[j]] : global_id(table, slot_key, mask, shift, id_of_slot, k[j]);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ batched form
@@ -254,31 +479,19 @@ void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_
   tmp_free(c, firsts);
 }
 
-void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
-              int64_t *out) {
-  if (n == 0) return;
-  if (n > 0x7FFFFFFFull) usage_error("group_id supports at most 2^31 - 1 rows");
-  uint64_t slots = 2;
-  int lg = 1;
-  while (slots < 2 * n) {
-    slots <<= 1;
-    ++lg;
-  }
-  auto *table = static_cast<unsigned long long *>(tmp_alloc(c, slots * 8));
-  fill_u64(c, reinterpret_cast<uint64_t *>(table), slots, 0);
+namespace {
+
+// Passes 2-4 of the file header over a filled global table: probe (per-row slot and first
+// flag), the first rows in row order, number, ids.
+void ids_from_slots(Ctx &c, KeyCol ka, KeyCol kb, bool pair, uint64_t n, unsigned long long *table,
+                    uint64_t slots, int lg, int64_t *out) {
   auto *slot = static_cast<uint32_t *>(tmp_alloc(c, n * 4));
   auto *first = static_cast<uint8_t *>(tmp_alloc(c, n));
-  KeyCol ka{ta, a}, kb{tb, b};
   const uint32_t g = stream_grid(n, kBlock * 4, 8192);
   const int shift = 64 - lg;
-  if (b) {
-    k_insert<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift);
-    k_probe<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
-  } else {
-    k_insert<false><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift);
-    k_probe<false><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
-  }
-  check_hip(hipGetLastError(), "group_id table");
+  if (pair) k_probe<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
+  else k_probe<false><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
+  check_hip(hipGetLastError(), "group_id probe");
   // first rows in row order
   auto *firsts = static_cast<int64_t *>(tmp_alloc(c, n * 8));
   uint64_t *d_cnt = c.scratch.slots + 21;
@@ -291,10 +504,106 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
   k_number<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(firsts, groups, slot, id_of_slot);
   k_ids<<<g, kBlock, 0, c.stream>>>(slot, id_of_slot, n, out);
   check_hip(hipGetLastError(), "group_id ids");
-  tmp_free(c, table);
   tmp_free(c, slot);
   tmp_free(c, first);
   tmp_free(c, firsts);
+}
+
+// The two-key form: the four passes of the file header over the global table alone.
+void group_id_pair(Ctx &c, KeyCol ka, KeyCol kb, uint64_t n, int64_t *out) {
+  uint64_t slots = 2;
+  int lg = 1;
+  while (slots < 2 * n) {
+    slots <<= 1;
+    ++lg;
+  }
+  auto *table = static_cast<unsigned long long *>(tmp_alloc(c, slots * 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(table), slots, 0);
+  const uint32_t g = stream_grid(n, kBlock * 4, 8192);
+  k_insert<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, 64 - lg);
+  check_hip(hipGetLastError(), "group_id table");
+  ids_from_slots(c, ka, kb, true, n, table, slots, lg, out);
+  tmp_free(c, table);
+}
+
+}  // namespace
+
+void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
+              int64_t *out) {
+  if (n == 0) return;
+  if (n > 0x7FFFFFFFull) usage_error("group_id supports at most 2^31 - 1 rows");
+  KeyCol ka{ta, a}, kb{tb, b};
+  if (b) {
+    group_id_pair(c, ka, kb, n, out);
+    return;
+  }
+  uint64_t full = 2;
+  int lg_full = 1;
+  while (full < 2 * n) {
+    full <<= 1;
+    ++lg_full;
+  }
+  const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>((n + kGChunk - 1) / kGChunk, 512));
+  auto *d_over = reinterpret_cast<uint32_t *>(c.scratch.slots + 22);
+  unsigned long long *table = nullptr;
+  uint64_t slots = full;
+  int lg = lg_full;
+  if (full > kSmallSlots) {
+    // few distinct keys fit the small table; a long probe run there means too many
+    table = static_cast<unsigned long long *>(tmp_alloc(c, kSmallSlots * 8));
+    fill_u64(c, reinterpret_cast<uint64_t *>(table), kSmallSlots, 0);
+    fill_u64(c, c.scratch.slots + 22, 1, 0);
+    k_insert1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, kSmallSlots - 1, 64 - kSmallLg, kMaxRun, d_over);
+    check_hip(hipGetLastError(), "group_id insert");
+    uint32_t over = 0;
+    read_back(c, d_over, 4, &over);
+    if (over) {
+      tmp_free(c, table);
+      table = nullptr;
+    } else {
+      slots = kSmallSlots;
+      lg = kSmallLg;
+    }
+  }
+  if (!table) {
+    table = static_cast<unsigned long long *>(tmp_alloc(c, full * 8));
+    fill_u64(c, reinterpret_cast<uint64_t *>(table), full, 0);
+    k_insert1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, full - 1, 64 - lg_full, kNoLimit, d_over);
+    check_hip(hipGetLastError(), "group_id insert");
+    if (full > kSmallSlots) {  // many keys: per-row slots from one probe pass
+      ids_from_slots(c, ka, kb, false, n, table, full, lg_full, out);
+      tmp_free(c, table);
+      return;
+    }
+  }
+  const uint64_t mask = slots - 1;
+  const int shift = 64 - lg;
+  // first rows flagged from the table, then listed in row order
+  auto *first = static_cast<uint8_t *>(tmp_alloc(c, (n + 7) / 8 * 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(first), (n + 7) / 8, 0);
+  k_first_flags<<<stream_grid(slots, kBlock * 4, 8192), kBlock, 0, c.stream>>>(table, slots, first);
+  auto *firsts = static_cast<int64_t *>(tmp_alloc(c, n * 8));
+  uint64_t *d_cnt = c.scratch.slots + 21;
+  SelectSpec ss{2, VK_I64, 0, 0, 0};
+  select_rows(c, ss, nullptr, first, n, firsts, d_cnt);
+  uint64_t groups = 0;
+  read_back(c, d_cnt, 8, &groups);
+  auto *slot_key = static_cast<uint64_t *>(tmp_alloc(c, slots * 8));
+  k_slot_keys<<<stream_grid(slots, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, table, slots, slot_key);
+  auto *id_of_slot = static_cast<uint32_t *>(tmp_alloc(c, slots * 4));
+  k_number1<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(ka, firsts, groups, table, mask, shift,
+                                                                         id_of_slot);
+  if (groups <= kLdsGroups)
+    k_ids1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, slot_key, mask, shift, id_of_slot, out);
+  else
+    k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, table, slot_key, mask, shift,
+                                                                      id_of_slot, out);
+  check_hip(hipGetLastError(), "group_id ids");
+  tmp_free(c, table);
+  tmp_free(c, first);
+  tmp_free(c, firsts);
+  tmp_free(c, slot_key);
+  tmp_free(c, id_of_slot);
 }
 
 }  // namespace vk
