@@ -205,6 +205,75 @@ __global__ void k_bucket_starts(const K *__restrict__ sk, uint64_t b, uint8_t *f
     flag[p] = (p == 0 || (sk[p - 1] >> 4) != (sk[p] >> 4)) ? 1 : 0;
 }
 
+// One lane per chain when there are many long chains: a wave takes 64 consecutive chains and
+// moves 64 values of each through LDS per round (one coalesced 512-byte load per chain, all 64
+// issued before the first is stored; rows padded to 65 values so that a lane's row walk
+// spreads over the banks); lane l then folds its own chain's 64 values in index order. Two
+// LDS buffers let the loads of round q + 1 fly while round q is folded. A chain shorter than
+// the round's window adds nothing past its end.
+constexpr int kLaneFoldPad = 65;
+
+template <typename K>
+__global__ __launch_bounds__(kWave) void k_chain_fold_lanes(const uint64_t *__restrict__ starts,
+                                                             uint64_t nchains, const K *__restrict__ sk,
+                                                             const double *__restrict__ sv, uint64_t b,
+                                                             double *stripes) {
+  __shared__ double s_v[2][kWave * kLaneFoldPad];
+  const int l = lane_id();
+  for (uint64_t c0 = blockIdx.x * uint64_t(kWave); c0 < nchains; c0 += uint64_t(gridDim.x) * kWave) {
+    const uint64_t ch = c0 + l;
+    const bool live = ch < nchains;
+    const uint64_t p0 = live ? starts[ch] : 0;
+    const uint64_t p1 = live ? (ch + 1 < nchains ? starts[ch + 1] : b) : 0;
+    const uint64_t len = p1 - p0;
+    // the longest chain of the wave sets the rounds
+    uint64_t most = len;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint64_t o = __shfl_xor(most, m);
+      most = o > most ? o : most;
+    }
+    double v[kWave];
+    // chain i's values q .. q + 63, for lane l the value q + l
+    auto load_round = [&](uint64_t q) {
+      const uint64_t r = q + l;
+#pragma unroll
+      for (int i = 0; i < kWave; ++i) {
+        const uint64_t pi = bcast_pod(p0, i), li = bcast_pod(len, i);
+        v[i] = r < li ? sv[pi + r] : 0.0;
+      }
+    };
+    auto store_round = [&](int buf) {
+#pragma unroll
+      for (int i = 0; i < kWave; ++i) s_v[buf][i * kLaneFoldPad + l] = v[i];
+    };
+    double a = 0.0;
+    if (most > 0) {
+      load_round(0);
+      store_round(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (uint64_t q = 0; q < most; q += kWave) {
+      const bool more = q + kWave < most;
+      if (more) load_round(q + kWave);
+      const double *row = &s_v[buf][l * kLaneFoldPad];
+      if (len >= q + kWave) {
+#pragma unroll 16
+        for (int k = 0; k < kWave; ++k) a = a + row[k];
+      } else if (len > q) {
+        const int cnt = static_cast<int>(len - q);
+        for (int k = 0; k < cnt; ++k) a = a + row[k];
+      }
+      if (more) store_round(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+    if (live) stripes[sk[p0]] = a;
+    __syncthreads();
+  }
+}
+
 template <typename K>
 __global__ void k_chain_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag) {
   for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
@@ -294,8 +363,12 @@ namespace {
 
 // The chains of rows 0..b (bucket, i % 16): a stable sort of (chain key, value) pairs puts each
 // chain's values together in index order, and a bucket's chains together in stripe order.
-// Long chains (few buckets) fold one wave per chain into the stripes, which then combine per
-// bucket; short chains fold one thread per bucket, straight into out (no stripe buffer).
+// Long chains (few buckets) fold into the stripes, which then combine per bucket: one lane per
+// chain when there are at least kLaneFoldChains of them (a wave's 64 lanes busy on 64 chains),
+// else one wave per chain (every lane of a wave on one chain, for few chains that need all
+// the parallel hardware they can get); short chains fold one thread per bucket, straight into
+// out (no stripe buffer).
+constexpr uint64_t kLaneFoldChains = 4096;
 // Returns true when out holds the combined buckets.
 template <typename K>
 bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
@@ -322,7 +395,10 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
   select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
   uint64_t nstarts = 0;
   read_back(c, d_cnt, 8, &nstarts);
-  if (long_chains) {
+  if (long_chains && nstarts >= kLaneFoldChains) {
+    k_chain_fold_lanes<K><<<stream_grid(nstarts, kWave, 16384), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
+                                                                                      stripes);
+  } else if (long_chains) {
     k_chain_fold_wave<K><<<stream_grid(nstarts, kFoldWaves, 16384), kFoldWaves * kWave, 0, c.stream>>>(
         starts, nstarts, sk, sv, b, stripes);
   } else {
