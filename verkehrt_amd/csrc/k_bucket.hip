@@ -155,45 +155,82 @@ __global__ __launch_bounds__(kFoldWaves * kWave) void k_chain_fold_wave(const ui
   }
 }
 
-// One thread per bucket (the compacted bucket starts) for short chains: the bucket's rows are
-// its chains in stripe order, each in index order; every chain folds from 0.0 and the 16
-// stripes (0.0 where a stripe has no rows) combine pairwise into out[bucket], as
-// k_combine_stripes does.
+// Short chains: lane l of a wave folds bucket w * 64 + l (the compacted bucket starts). The
+// wave's 64 buckets are consecutive rows of the sorted pairs, which it streams through LDS in
+// chunks of kBucketChunk rows with coalesced loads; each lane folds its own rows of the chunk.
+// A bucket's rows are its chains in stripe order, each in index order; every chain folds
+// from 0.0 and the 16 stripes (0.0 where a stripe has no rows) combine pairwise into
+// out[bucket], as k_combine_stripes does.
+constexpr int kBucketChunk = 1024;
+
 template <typename K>
-__global__ void k_bucket_fold(const uint64_t *__restrict__ starts, uint64_t nstarts,
-                              const K *__restrict__ sk, const double *__restrict__ sv, uint64_t b,
-                              double *out) {
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nstarts;
-       i += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t p0 = starts[i];
-    const uint64_t p1 = i + 1 < nstarts ? starts[i + 1] : b;
-    const uint64_t bucket = static_cast<uint64_t>(sk[p0]) >> 4;
+__global__ __launch_bounds__(kWave) void k_bucket_fold(const uint64_t *__restrict__ starts, uint64_t nstarts,
+                                                       const K *__restrict__ sk, const double *__restrict__ sv,
+                                                       uint64_t b, double *out) {
+  __shared__ K s_k[kBucketChunk];
+  __shared__ double s_v[kBucketChunk];
+  const int l = lane_id();
+  for (uint64_t w0 = blockIdx.x * uint64_t(kWave); w0 < nstarts; w0 += uint64_t(gridDim.x) * kWave) {
+    const uint64_t bi = w0 + l;
+    const bool live = bi < nstarts;
+    const uint64_t p0 = live ? starts[bi] : b;
+    const uint64_t p1 = live ? (bi + 1 < nstarts ? starts[bi + 1] : b) : b;
+    // the wave's rows: from lane 0's first to the last live lane's end
+    const uint64_t r0 = bcast_pod(p0, 0);
+    uint64_t r1 = live ? p1 : 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint64_t o = __shfl_xor(r1, m);
+      r1 = o > r1 ? o : r1;
+    }
     double a[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) a[j] = 0.0;
-    int cur = static_cast<int>(sk[p0] & 15);
+    int cur = -1;
     double acc = 0.0;
     auto put = [&](int s, double v) {
 #pragma unroll
       for (int j = 0; j < 16; ++j)
         if (j == s) a[j] = v;
     };
-    for (uint64_t q = p0; q < p1; ++q) {
-      const int s = static_cast<int>(sk[q] & 15);
-      const double v = sv[q];
-      if (s != cur) {
-        put(cur, acc);
-        acc = 0.0;
-        cur = s;
+    for (uint64_t c = r0; c < r1; c += kBucketChunk) {
+      K kk[kBucketChunk / kWave];
+      double vv[kBucketChunk / kWave];
+#pragma unroll
+      for (int j = 0; j < kBucketChunk / kWave; ++j) {
+        const uint64_t r = c + uint64_t(j) * kWave + l;
+        kk[j] = r < r1 ? sk[r] : K(0);
+        vv[j] = r < r1 ? sv[r] : 0.0;
       }
-      acc = acc + v;
+#pragma unroll
+      for (int j = 0; j < kBucketChunk / kWave; ++j) {
+        s_k[j * kWave + l] = kk[j];
+        s_v[j * kWave + l] = vv[j];
+      }
+      __syncthreads();
+      const uint64_t e = c + kBucketChunk;
+      const uint64_t q0 = p0 > c ? p0 : c;
+      const uint64_t q1 = p1 < e ? p1 : e;
+      for (uint64_t q = q0; q < q1; ++q) {
+        const int s = static_cast<int>(s_k[q - c] & 15);
+        const double v = s_v[q - c];
+        if (s != cur) {
+          if (cur >= 0) put(cur, acc);
+          acc = 0.0;
+          cur = s;
+        }
+        acc = acc + v;
+      }
+      __syncthreads();
     }
-    put(cur, acc);
+    if (live) {
+      put(cur, acc);
 #pragma unroll
-    for (int step = 8; step > 0; step /= 2)
+      for (int step = 8; step > 0; step /= 2)
 #pragma unroll
-      for (int j = 0; j < step; ++j) a[j] = a[j] + a[j + step];
-    out[bucket] = a[0];
+        for (int j = 0; j < step; ++j) a[j] = a[j] + a[j + step];
+      out[static_cast<uint64_t>(sk[p0]) >> 4] = a[0];
+    }
   }
 }
 
@@ -366,7 +403,7 @@ namespace {
 // Long chains (few buckets) fold into the stripes, which then combine per bucket: one lane per
 // chain when there are at least kLaneFoldChains of them (a wave's 64 lanes busy on 64 chains),
 // else one wave per chain (every lane of a wave on one chain, for few chains that need all
-// the parallel hardware they can get); short chains fold one thread per bucket, straight into
+// the parallel hardware they can get); short chains fold one lane per bucket, straight into
 // out (no stripe buffer).
 constexpr uint64_t kLaneFoldChains = 4096;
 // Returns true when out holds the combined buckets.
@@ -403,7 +440,7 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
         starts, nstarts, sk, sv, b, stripes);
   } else {
     k_fill<double><<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(out, buckets, 0.0);
-    k_bucket_fold<K><<<stream_grid(nstarts, kBlock, 8192), kBlock, 0, c.stream>>>(starts, nstarts, sk, sv, b,
+    k_bucket_fold<K><<<stream_grid(nstarts, kWave, 65535), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
                                                                                   out);
   }
   check_hip(hipGetLastError(), "chain fold");
