@@ -278,25 +278,6 @@ void batch_plan_free(Ctx &c, BatchPlan &p) {
   p = BatchPlan{};
 }
 
-uint32_t resident_grid(Ctx &c, const void *kernel, int threads, uint32_t ntiles) {
-  static std::mutex mu;
-  static std::map<const void *, int> per_cu;
-  int nb = 0;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    auto it = per_cu.find(kernel);
-    if (it == per_cu.end()) {
-      check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0), "occupancy");
-      if (nb < 1) nb = 1;
-      per_cu[kernel] = nb;
-    } else {
-      nb = it->second;
-    }
-  }
-  const uint64_t g = uint64_t(nb) * uint64_t(c.cus > 0 ? c.cus : 1);
-  return static_cast<uint32_t>(g < ntiles ? g : ntiles);
-}
-
 Ctx *create_ctx(int device) {
   check_hip(hipSetDevice(device), "hipSetDevice");
   auto *c = new Ctx();
@@ -306,7 +287,6 @@ Ctx *create_ctx(int device) {
   c->epoch = 0;
   c->trace = false;
   c->last_us = 0;
-  check_hip(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
   check_hip(hipEventCreate(&c->ev0), "event");
   check_hip(hipEventCreate(&c->ev1), "event");
   check_hip(hipHostMalloc(reinterpret_cast<void **>(&c->host_slots), 64 * 8, hipHostMallocDefault),

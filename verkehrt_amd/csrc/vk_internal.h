@@ -116,8 +116,6 @@ struct vk_ctx {
   std::string trace_buf;
   double last_us;
   hipEvent_t ev0, ev1;
-  uint32_t pticket = 0;  // next ticket base of persistent launches (counters[4])
-  int cus = 0;           // compute units of the device
   // Holders of the context: its creator (until vk_ctx_destroy), every owned buffer, reducer,
   // scanner, builder and plan. The stream, pool and scratch go when the last one lets go, so
   // an array released after vk_ctx_destroy still returns its memory to a live pool.
@@ -149,18 +147,6 @@ void fill_u64(Ctx &c, uint64_t *p, uint64_t n, uint64_t value);
 
 // Launch geometry -----------------------------------------------------------
 constexpr int kBlock = 256;
-// Persistent launches: `grid` resident blocks draw `ntiles` tiles in ticket order from
-// counters[4]. Every block draws one ticket past the last tile before it exits, so a launch
-// takes exactly ntiles + grid tickets and the next launch's base is known on the host.
-constexpr int kPersistentCounter = 4;
-inline uint32_t persistent_base(Ctx &c, uint32_t ntiles, uint32_t grid) {
-  const uint32_t b = c.pticket;
-  c.pticket += ntiles + grid;
-  return b;
-}
-// Blocks of `threads` threads of `kernel` that are resident on the device at once (at least one
-// per compute unit), capped at `ntiles`.
-uint32_t resident_grid(Ctx &c, const void *kernel, int threads, uint32_t ntiles);
 inline uint32_t stream_grid(uint64_t work_items, uint32_t per_block, uint32_t cap = 2048) {
   uint64_t g = (work_items + per_block - 1) / per_block;
   if (g < 1) g = 1;
