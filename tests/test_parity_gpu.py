@@ -435,6 +435,12 @@ def test_reduce_into_add_f64_defined_order(D):
     v = floats(rng, n, False)
     got = D.call("reduce_into_add_f64", [view(D, k, 1), view(D, v, 2), S("i64", 300)]).to_numpy()
     assert same_array(O.reduce_into(k, v, 300), got)
+    # many buckets (one lane per bucket, no stripe buffer), some of them empty
+    n = 300_007
+    k = rng.integers(0, 100_000, n).astype(np.int64)
+    v = floats(rng, n, False)
+    got = D.call("reduce_into_add_f64", [view(D, k, 3), view(D, v, 1), S("i64", 100_000)]).to_numpy()
+    assert same_array(O.reduce_into(k, v, 100_000), got)
     # every key equal gives sum's bits
     v = floats(rng, 10_007, False)
     got = D.call("reduce_into_add_f64", [D.array(np.zeros(v.size, np.int64)), D.array(v), S("i64", 1)]).to_numpy()

@@ -400,13 +400,16 @@ namespace {
 
 // The chains of rows 0..b (bucket, i % 16): a stable sort of (chain key, value) pairs puts each
 // chain's values together in index order, and a bucket's chains together in stripe order.
-// Long chains (few buckets) fold into the stripes, which then combine per bucket: one lane per
-// chain when there are at least kLaneFoldChains of them (a wave's 64 lanes busy on 64 chains),
-// else one wave per chain (every lane of a wave on one chain, for few chains that need all
-// the parallel hardware they can get); short chains fold one lane per bucket, straight into
-// out (no stripe buffer).
-constexpr uint64_t kLaneFoldChains = 4096;
+// Below kBucketFoldMin buckets the chains fold into the stripes, which then combine per
+// bucket: one lane per chain (k_chain_fold_lanes) unless there are fewer than kLaneFoldChains
+// chains averaging 256 rows or more, which fold one wave per chain (every lane of a wave on
+// one chain, for few long chains that need all the parallel hardware they can get). From
+// kBucketFoldMin buckets on, the buckets alone give the parallelism: one lane per bucket
+// folds straight into out (k_bucket_fold, no stripe buffer of 16 x buckets values).
 // Returns true when out holds the combined buckets.
+constexpr uint64_t kLaneFoldChains = 4096;
+constexpr uint64_t kBucketFoldMin = uint64_t(1) << 16;
+
 template <typename K>
 bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
                  double *stripes, double *out) {
@@ -420,35 +423,34 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
   sort_key_values(c, wide, ck, reinterpret_cast<const uint64_t *>(vals), b, sk,
                   reinterpret_cast<uint64_t *>(sv));
   tmp_free(c, ck);
-  const uint64_t chains_est = buckets * 16 < b ? buckets * 16 : b;
-  const bool long_chains = b / chains_est >= 256;
-  // the starts of chains (long) or buckets (short), compacted in order
+  const bool by_bucket = buckets >= kBucketFoldMin;
+  // the starts of chains (stripes) or buckets (by_bucket), compacted in order
   uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
   uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-  if (long_chains) k_chain_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
-  else k_bucket_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+  if (by_bucket) k_bucket_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+  else k_chain_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
   SelectSpec ss{2, VK_I64, 0, 0, 0};
   uint64_t *d_cnt = c.scratch.slots + 20;
   select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
   uint64_t nstarts = 0;
   read_back(c, d_cnt, 8, &nstarts);
-  if (long_chains && nstarts >= kLaneFoldChains) {
-    k_chain_fold_lanes<K><<<stream_grid(nstarts, kWave, 16384), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
-                                                                                      stripes);
-  } else if (long_chains) {
-    k_chain_fold_wave<K><<<stream_grid(nstarts, kFoldWaves, 16384), kFoldWaves * kWave, 0, c.stream>>>(
-        starts, nstarts, sk, sv, b, stripes);
-  } else {
+  if (by_bucket) {
     k_fill<double><<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(out, buckets, 0.0);
     k_bucket_fold<K><<<stream_grid(nstarts, kWave, 65535), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
                                                                                   out);
+  } else if (nstarts >= kLaneFoldChains || b / nstarts < 256) {
+    k_chain_fold_lanes<K><<<stream_grid(nstarts, kWave, 16384), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
+                                                                                      stripes);
+  } else {
+    k_chain_fold_wave<K><<<stream_grid(nstarts, kFoldWaves, 16384), kFoldWaves * kWave, 0, c.stream>>>(
+        starts, nstarts, sk, sv, b, stripes);
   }
   check_hip(hipGetLastError(), "chain fold");
   tmp_free(c, flag);
   tmp_free(c, starts);
   tmp_free(c, sk);
   tmp_free(c, sv);
-  return !long_chains;
+  return by_bucket;
 }
 
 }  // namespace
@@ -612,11 +614,10 @@ void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64
                          uint64_t buckets, double *out) {
   if (buckets == 0) return;
   const uint64_t b = n - n % 16;
-  // the stripe buffer is needed only for long chains (buckets <= b / 4096)
-  const uint64_t chains_est = b == 0 ? 0 : (buckets * 16 < b ? buckets * 16 : b);
-  const bool long_chains = b > 0 && b / chains_est >= 256;
+  // the stripe buffer (16 values per bucket) serves fewer than kBucketFoldMin buckets
+  const bool stripe_form = buckets < kBucketFoldMin;
   double *stripes = nullptr;
-  if (long_chains) {
+  if (stripe_form) {
     stripes = static_cast<double *>(tmp_alloc(c, buckets * 16 * 8));
     k_fill<double><<<stream_grid(buckets * 16, kBlock, 4096), kBlock, 0, c.stream>>>(stripes, buckets * 16, 0.0);
   }
@@ -625,7 +626,7 @@ void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64
     if (buckets * 16 <= 0xFFFFFFFFull) combined = fold_chains<uint32_t>(c, keys, vals, b, buckets, stripes, out);
     else combined = fold_chains<uint64_t>(c, keys, vals, b, buckets, stripes, out);
   }
-  if (long_chains)
+  if (stripe_form)
     k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, buckets, out);
   else if (!combined)
     k_fill<double><<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(out, buckets, 0.0);
