@@ -304,8 +304,20 @@ template <class Op>
 __global__ __launch_bounds__(kRedThreads) void k_reduce_merge(Op op, const typename Op::Acc *partials,
                                                               uint32_t nparts, typename Op::Acc *state) {
   using Acc = typename Op::Acc;
+  // thread t merges partials t, t + 1024, t + 2048, ... in that order, eight loads in flight
+  constexpr int kAhead = 8;
   Acc b = op.identity();
-  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) op.merge(b, partials[i]);
+  for (uint32_t i0 = threadIdx.x; i0 < nparts; i0 += kRedThreads * kAhead) {
+    Acc p[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const uint32_t i = i0 + k * kRedThreads;
+      p[k] = i < nparts ? partials[i] : op.identity();
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k)
+      if (i0 + k * kRedThreads < nparts) op.merge(b, p[k]);
+  }
   b = block_merge_big(op, b);
   if (threadIdx.x == 0) {
     Acc s = *state;
