@@ -121,14 +121,15 @@ __global__ void k_ids(const uint32_t *__restrict__ slot, const uint32_t *__restr
 //     probe run longer than kMaxRun there marks overflow, and the pass is redone on 2n slots;
 //  2. every occupied global slot flags its least row (k_first_flags);
 //  3. the flagged rows in row order (the selection kernel) are the keys' first rows;
-//  4. the slot of the j-th first row gets id j (k_number1), and every slot's key is copied next
-//     to the table (k_slot_keys), so that searches compare cached slot keys;
+//  4. every slot's key is copied into a 16-byte slot entry (k_slot_entries), and the entry of
+//     the j-th first row's slot gets id j (k_number1), so that a search step is one load of a
+//     cached entry;
 //  5. every row maps its key to its id through the LDS table again, each key new to the block
-//     resolved once from the global table, and writes it (k_ids1); with more than kLdsGroups
-//     keys, where LDS tables seldom hit, every row searches the global table (k_ids2).
-// When the small table overflows (more than about a million distinct keys), passes 2-5 are
-// those of the two-key form instead (probe, select, number, ids): the table is then too large
-// for a copy of its keys to stay in the caches.
+//     resolved once from the slot entries, and writes it (k_ids1); with more than kLdsGroups
+//     keys, where LDS tables seldom hit, every row searches the slot entries (k_ids2).
+// When the small table overflows (more than about a million distinct keys), the insert is
+// redone on a table of 2n slots and passes 2-5 follow on it; a search step is then one load
+// of a 16-byte entry from HBM, instead of a table word, the key column and an id apart.
 constexpr int kGBlock = 1024;
 constexpr int kGRows = 8;  // rows per thread per chunk
 constexpr uint64_t kGChunk = uint64_t(kGBlock) * kGRows;
@@ -250,56 +251,62 @@ __global__ void k_first_flags(const unsigned long long *__restrict__ table, uint
   }
 }
 
-// The slot of first row firsts[j] is the one whose word is firsts[j] + 1.
+// A slot of the global table as the id passes read it: its key, its id and whether it is
+// used, in one 16-byte entry, so a search step is one load.
+struct alignas(16) SlotEntry {
+  uint64_t key;
+  uint32_t id;
+  uint32_t used;
+};
+
+// The key and use of every slot (the table is small here, so later searches read cached
+// entries instead of the key column at random).
+__global__ void k_slot_entries(KeyCol a, const unsigned long long *__restrict__ table, uint64_t slots,
+                               SlotEntry *entry) {
+  for (uint64_t h = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; h < slots;
+       h += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t w = table[h];
+    entry[h] = SlotEntry{w != 0 ? a.bits(w - 1) : 0, 0u, w != 0 ? 1u : 0u};
+  }
+}
+
+// The slot of first row firsts[j] is the one whose word is firsts[j] + 1: its entry gets id j.
 __global__ void k_number1(KeyCol a, const int64_t *__restrict__ firsts, uint64_t groups,
                           const unsigned long long *__restrict__ table, uint64_t mask, int shift,
-                          uint32_t *id_of_slot) {
+                          SlotEntry *entry) {
   for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < groups;
        j += uint64_t(gridDim.x) * blockDim.x) {
     const uint64_t r = static_cast<uint64_t>(firsts[j]);
     uint64_t h = mix(a.bits(r)) >> shift;
     for (uint64_t p = 0; p <= mask && table[h] != r + 1; ++p) h = (h + 1) & mask;
-    id_of_slot[h] = static_cast<uint32_t>(j);
+    entry[h].id = static_cast<uint32_t>(j);
   }
 }
 
-// The key of every occupied slot, next to the table (the table is small here, so later
-// searches compare against cached slot keys instead of reading the key column at random).
-__global__ void k_slot_keys(KeyCol a, const unsigned long long *__restrict__ table, uint64_t slots,
-                            uint64_t *slot_key) {
-  for (uint64_t h = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; h < slots;
-       h += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t w = table[h];
-    slot_key[h] = w != 0 ? a.bits(w - 1) : 0;
-  }
-}
-
-// The id of key k from the global table (every key is in it).
-__device__ __forceinline__ uint32_t global_id(const unsigned long long *table, const uint64_t *slot_key,
-                                              uint64_t mask, int shift, const uint32_t *id_of_slot, uint64_t k) {
+// The id of key k from the slot entries (every key is in them).
+__device__ __forceinline__ uint32_t global_id(const SlotEntry *entry, uint64_t mask, int shift, uint64_t k) {
   uint64_t h = mix(k) >> shift;
   for (uint64_t p = 0; p <= mask; ++p) {
-    if (table[h] != 0 && slot_key[h] == k) break;
+    const SlotEntry e = entry[h];
+    if (e.used && e.key == k) return e.id;
     h = (h + 1) & mask;
   }
-  return id_of_slot[h];
+  return 0;  // unreachable: every key has a used slot on its probe path
 }
 
-// Every row's id from the global table, for more keys than the LDS tables hold.
-__global__ __launch_bounds__(kBlock) void k_ids2(KeyCol a, uint64_t n, const unsigned long long *table,
-                                                 const uint64_t *slot_key, uint64_t mask, int shift,
-                                                 const uint32_t *id_of_slot, int64_t *out) {
+// Every row's id from the slot entries, for more keys than the LDS tables hold.
+__global__ __launch_bounds__(kBlock) void k_ids2(KeyCol a, uint64_t n, const SlotEntry *entry, uint64_t mask,
+                                                 int shift, int64_t *out) {
   for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
        r += uint64_t(gridDim.x) * blockDim.x)
-    out[r] = global_id(table, slot_key, mask, shift, id_of_slot, a.bits(r));
+    out[r] = global_id(entry, mask, shift, a.bits(r));
 }
 
 // Phase 1 of each chunk finds (or claims) the LDS entry of every row's key, the claiming lane
 // resolving the key's id; after the barrier, phase 2 reads the ids. An entry's id is written
 // once, before the barrier that follows its claim, and never changes.
-__global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const unsigned long long *table,
-                                                  const uint64_t *slot_key, uint64_t mask, int shift,
-                                                  const uint32_t *id_of_slot, int64_t *out) {
+__global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const SlotEntry *entry, uint64_t mask,
+                                                  int shift, int64_t *out) {
   __shared__ unsigned long long s_key[kLSlots];
   __shared__ uint32_t s_id[kLSlots];
   __shared__ uint32_t s_cnt;
@@ -322,14 +329,14 @@ __global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const un
       if (r >= n) continue;
       bool claimed;
       h[j] = lds_find(s_key, &s_cnt, k[j], &claimed);
-      if (claimed) s_id[h[j]] = global_id(table, slot_key, mask, shift, id_of_slot, k[j]);
+      if (claimed) s_id[h[j]] = global_id(entry, mask, shift, k[j]);
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kGRows; ++j) {
       const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
       if (r >= n) continue;
-      out[r] = h[j] >= 0 ? s_id[h[j]] : global_id(table, slot_key, mask, shift, id_of_slot, k[j]);
+      out[r] = h[j] >= 0 ? s_id[h[j]] : global_id(entry, mask, shift, k[j]);
     }
   }
 }
@@ -570,11 +577,6 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
     fill_u64(c, reinterpret_cast<uint64_t *>(table), full, 0);
     k_insert1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, full - 1, 64 - lg_full, kNoLimit, d_over);
     check_hip(hipGetLastError(), "group_id insert");
-    if (full > kSmallSlots) {  // many keys: per-row slots from one probe pass
-      ids_from_slots(c, ka, kb, false, n, table, full, lg_full, out);
-      tmp_free(c, table);
-      return;
-    }
   }
   const uint64_t mask = slots - 1;
   const int shift = 64 - lg;
@@ -588,22 +590,19 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
   select_rows(c, ss, nullptr, first, n, firsts, d_cnt);
   uint64_t groups = 0;
   read_back(c, d_cnt, 8, &groups);
-  auto *slot_key = static_cast<uint64_t *>(tmp_alloc(c, slots * 8));
-  k_slot_keys<<<stream_grid(slots, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, table, slots, slot_key);
-  auto *id_of_slot = static_cast<uint32_t *>(tmp_alloc(c, slots * 4));
+  auto *entry = static_cast<SlotEntry *>(tmp_alloc(c, slots * sizeof(SlotEntry)));
+  k_slot_entries<<<stream_grid(slots, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, table, slots, entry);
   k_number1<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(ka, firsts, groups, table, mask, shift,
-                                                                         id_of_slot);
+                                                                         entry);
   if (groups <= kLdsGroups)
-    k_ids1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, slot_key, mask, shift, id_of_slot, out);
+    k_ids1<<<g, kGBlock, 0, c.stream>>>(ka, n, entry, mask, shift, out);
   else
-    k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, table, slot_key, mask, shift,
-                                                                      id_of_slot, out);
+    k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, entry, mask, shift, out);
   check_hip(hipGetLastError(), "group_id ids");
   tmp_free(c, table);
   tmp_free(c, first);
   tmp_free(c, firsts);
-  tmp_free(c, slot_key);
-  tmp_free(c, id_of_slot);
+  tmp_free(c, entry);
 }
 
 }  // namespace vk
