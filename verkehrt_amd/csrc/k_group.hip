@@ -116,20 +116,19 @@ __global__ void k_ids(const uint32_t *__restrict__ slot, const uint32_t *__restr
 // the compute unit:
 //  1. insert (k_insert1): rows fold into the block's LDS table (key -> least row); a key the
 //     LDS table cannot take (its new-entry budget spent, a probe run too long, or the key equal
-//     to the LDS empty marker) goes straight to the global table, and at the end each LDS entry
-//     is inserted into the global table once. The global table starts at kSmallSlots slots; a
-//     probe run longer than kMaxRun there marks overflow, and the pass is redone on 2n slots;
-//  2. every occupied global slot flags its least row (k_first_flags);
+//     to the empty marker) goes straight to the global table, and at the end each LDS entry
+//     is inserted into the global table once. The global table's slots hold the key itself
+//     (claimed by compare-and-swap from the empty marker), the key's least row (atomicMin)
+//     and, later, its id, in one 16-byte entry: a search step reads one line and the least-row
+//     update hits the same line. The key equal to the empty marker has a slot of its own past
+//     the table's end. The table starts at kSmallSlots slots; a probe run longer than kMaxRun
+//     there marks overflow, and the pass is redone on 2n slots;
+//  2. every used slot flags its least row (k_first_flags);
 //  3. the flagged rows in row order (the selection kernel) are the keys' first rows;
-//  4. every slot's key is copied into a 16-byte slot entry (k_slot_entries), and the entry of
-//     the j-th first row's slot gets id j (k_number1), so that a search step is one load of a
-//     cached entry;
+//  4. the slot of the j-th first row's key gets id j (k_number1);
 //  5. every row maps its key to its id through the LDS table again, each key new to the block
-//     resolved once from the slot entries, and writes it (k_ids1); with more than kLdsGroups
-//     keys, where LDS tables seldom hit, every row searches the slot entries (k_ids2).
-// When the small table overflows (more than about a million distinct keys), the insert is
-// redone on a table of 2n slots and passes 2-5 follow on it; a search step is then one load
-// of a 16-byte entry from HBM, instead of a table word, the key column and an id apart.
+//     resolved once from the global table, and writes it (k_ids1); with more than kLdsGroups
+//     keys, where LDS tables seldom hit, every row searches the global table (k_ids2).
 constexpr int kGBlock = 1024;
 constexpr int kGRows = 8;  // rows per thread per chunk
 constexpr uint64_t kGChunk = uint64_t(kGBlock) * kGRows;
@@ -169,20 +168,34 @@ __device__ __forceinline__ int lds_find(unsigned long long *s_key, uint32_t *s_c
   return -1;
 }
 
-// Inserts (key k, row r) into the global table: claims an empty slot with r + 1, or lowers the
-// word of the slot holding k to r + 1. False when the probe run reaches max_run slots.
-__device__ bool global_insert(const KeyCol &a, unsigned long long *table, uint64_t mask, int shift,
-                              uint64_t k, uint64_t r, int max_run) {
+// A slot of the single-key global table: its key (kLEmpty when unused), the key's least row and
+// its id. Slot `slots` (past the end) belongs to the key equal to kLEmpty.
+struct alignas(16) Slot {
+  unsigned long long key;
+  uint32_t row;
+  uint32_t id;
+};
+
+__global__ void k_slots_init(Slot *t, uint64_t n) {
+  for (uint64_t h = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; h < n; h += uint64_t(gridDim.x) * blockDim.x)
+    t[h] = Slot{kLEmpty, 0xFFFFFFFFu, 0u};
+}
+
+// Inserts (key k, row r) into the global table of mask + 1 slots: finds or claims the slot of
+// k and lowers its least row to r. False when the probe run reaches max_run slots.
+__device__ bool global_insert(Slot *t, uint64_t mask, int shift, uint64_t k, uint32_t r, int max_run) {
+  if (k == kLEmpty) {
+    atomicMin(&t[mask + 1].row, r);
+    return true;
+  }
   uint64_t h = mix(k) >> shift;
   for (int p = 0; p < max_run; ++p) {
-    unsigned long long s = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (s == 0) {
-      const unsigned long long old = atomicCAS(&table[h], 0ull, r + 1);
-      if (old == 0) return true;
-      s = old;
-    }
-    if (a.bits(s - 1) == k) {
-      if (r + 1 < s) atomicMin(&table[h], r + 1);
+    unsigned long long cur = __hip_atomic_load(&t[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == kLEmpty) cur = atomicCAS(&t[h].key, kLEmpty, static_cast<unsigned long long>(k));
+    if (cur == kLEmpty || cur == k) {
+      // the least row only decreases, so a value read at any time that is already at most r
+      // needs no atomic (mostly so: rows arrive in increasing order per thread)
+      if (__hip_atomic_load(&t[h].row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > r) atomicMin(&t[h].row, r);
       return true;
     }
     h = (h + 1) & mask;
@@ -190,9 +203,8 @@ __device__ bool global_insert(const KeyCol &a, unsigned long long *table, uint64
   return false;
 }
 
-__global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, unsigned long long *table,
-                                                     uint64_t mask, int shift, int max_run,
-                                                     uint32_t *overflow) {
+__global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, Slot *table, uint64_t mask, int shift,
+                                                     int max_run, uint32_t *overflow) {
   __shared__ unsigned long long s_key[kLSlots];
   __shared__ uint32_t s_row[kLSlots];
   __shared__ uint32_t s_cnt;
@@ -229,7 +241,9 @@ __global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, unsig
         // rows arrive in increasing order per thread, so the entry is mostly lower already
         if (__hip_atomic_load(&s_row[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > r)
           atomicMin(&s_row[h], static_cast<uint32_t>(r));
-      } else if (!global_insert(a, table, mask, shift, k[j], r, max_run)) over = true;
+      } else if (!global_insert(table, mask, shift, k[j], static_cast<uint32_t>(r), max_run)) {
+        over = true;
+      }
     }
     if (over) atomicOr(overflow, 1u);
     over = false;
@@ -239,73 +253,61 @@ __global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, unsig
     return;  // the pass is redone on the full table
   for (int i = t; i < kLSlots; i += kGBlock) {
     const uint64_t k = s_key[i];
-    if (k != kLEmpty && !global_insert(a, table, mask, shift, k, s_row[i], max_run)) atomicOr(overflow, 1u);
+    if (k != kLEmpty && !global_insert(table, mask, shift, k, s_row[i], max_run)) atomicOr(overflow, 1u);
   }
 }
 
-__global__ void k_first_flags(const unsigned long long *__restrict__ table, uint64_t slots, uint8_t *first) {
-  for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < slots;
-       s += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t w = table[s];
-    if (w != 0) first[w - 1] = 1;
-  }
-}
-
-// A slot of the global table as the id passes read it: its key, its id and whether it is
-// used, in one 16-byte entry, so a search step is one load.
-struct alignas(16) SlotEntry {
-  uint64_t key;
-  uint32_t id;
-  uint32_t used;
-};
-
-// The key and use of every slot (the table is small here, so later searches read cached
-// entries instead of the key column at random).
-__global__ void k_slot_entries(KeyCol a, const unsigned long long *__restrict__ table, uint64_t slots,
-                               SlotEntry *entry) {
-  for (uint64_t h = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; h < slots;
+// Slots 0 .. slots (the last one the empty marker's own): a used slot flags its least row.
+__global__ void k_first_flags(const Slot *__restrict__ table, uint64_t slots, uint8_t *first) {
+  for (uint64_t h = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; h <= slots;
        h += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t w = table[h];
-    entry[h] = SlotEntry{w != 0 ? a.bits(w - 1) : 0, 0u, w != 0 ? 1u : 0u};
+    const Slot e = table[h];
+    if (h < slots ? e.key != kLEmpty : e.row != 0xFFFFFFFFu) first[e.row] = 1;
   }
 }
 
-// The slot of first row firsts[j] is the one whose word is firsts[j] + 1: its entry gets id j.
-__global__ void k_number1(KeyCol a, const int64_t *__restrict__ firsts, uint64_t groups,
-                          const unsigned long long *__restrict__ table, uint64_t mask, int shift,
-                          SlotEntry *entry) {
-  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < groups;
-       j += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t r = static_cast<uint64_t>(firsts[j]);
-    uint64_t h = mix(a.bits(r)) >> shift;
-    for (uint64_t p = 0; p <= mask && table[h] != r + 1; ++p) h = (h + 1) & mask;
-    entry[h].id = static_cast<uint32_t>(j);
-  }
-}
-
-// The id of key k from the slot entries (every key is in them).
-__device__ __forceinline__ uint32_t global_id(const SlotEntry *entry, uint64_t mask, int shift, uint64_t k) {
+// The slot of key k (every key is in the table).
+__device__ __forceinline__ uint64_t slot_of_key(const Slot *table, uint64_t mask, int shift, uint64_t k) {
+  if (k == kLEmpty) return mask + 1;
   uint64_t h = mix(k) >> shift;
   for (uint64_t p = 0; p <= mask; ++p) {
-    const SlotEntry e = entry[h];
-    if (e.used && e.key == k) return e.id;
+    if (table[h].key == k) return h;
     h = (h + 1) & mask;
   }
-  return 0;  // unreachable: every key has a used slot on its probe path
+  return mask + 1;  // unreachable: every key has a slot on its probe path
 }
 
-// Every row's id from the slot entries, for more keys than the LDS tables hold.
-__global__ __launch_bounds__(kBlock) void k_ids2(KeyCol a, uint64_t n, const SlotEntry *entry, uint64_t mask,
-                                                 int shift, int64_t *out) {
+// The slot of the j-th first row's key gets id j.
+__global__ void k_number1(KeyCol a, const int64_t *__restrict__ firsts, uint64_t groups, Slot *table, uint64_t mask,
+                          int shift) {
+  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < groups;
+       j += uint64_t(gridDim.x) * blockDim.x)
+    table[slot_of_key(table, mask, shift, a.bits(static_cast<uint64_t>(firsts[j])))].id = static_cast<uint32_t>(j);
+}
+
+__device__ __forceinline__ uint32_t global_id(const Slot *table, uint64_t mask, int shift, uint64_t k) {
+  if (k == kLEmpty) return table[mask + 1].id;
+  uint64_t h = mix(k) >> shift;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    const Slot e = table[h];
+    if (e.key == k) return e.id;
+    h = (h + 1) & mask;
+  }
+  return 0;  // unreachable: every key has a slot on its probe path
+}
+
+// Every row's id from the global table, for more keys than the LDS tables hold.
+__global__ __launch_bounds__(kBlock) void k_ids2(KeyCol a, uint64_t n, const Slot *table, uint64_t mask, int shift,
+                                                 int64_t *out) {
   for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
        r += uint64_t(gridDim.x) * blockDim.x)
-    out[r] = global_id(entry, mask, shift, a.bits(r));
+    out[r] = global_id(table, mask, shift, a.bits(r));
 }
 
 // Phase 1 of each chunk finds (or claims) the LDS entry of every row's key, the claiming lane
 // resolving the key's id; after the barrier, phase 2 reads the ids. An entry's id is written
 // once, before the barrier that follows its claim, and never changes.
-__global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const SlotEntry *entry, uint64_t mask,
+__global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const Slot *table, uint64_t mask,
                                                   int shift, int64_t *out) {
   __shared__ unsigned long long s_key[kLSlots];
   __shared__ uint32_t s_id[kLSlots];
@@ -329,14 +331,14 @@ __global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const Sl
       if (r >= n) continue;
       bool claimed;
       h[j] = lds_find(s_key, &s_cnt, k[j], &claimed);
-      if (claimed) s_id[h[j]] = global_id(entry, mask, shift, k[j]);
+      if (claimed) s_id[h[j]] = global_id(table, mask, shift, k[j]);
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kGRows; ++j) {
       const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
       if (r >= n) continue;
-      out[r] = h[j] >= 0 ? s_id[h[j]] : global_id(entry, mask, shift, k[j]);
+      out[r] = h[j] >= 0 ? s_id[h[j]] : global_id(table, mask, shift, k[j]);
     }
   }
 }
@@ -552,13 +554,17 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
   }
   const uint32_t g = static_cast<uint32_t>(std::min<uint64_t>((n + kGChunk - 1) / kGChunk, 512));
   auto *d_over = reinterpret_cast<uint32_t *>(c.scratch.slots + 22);
-  unsigned long long *table = nullptr;
+  auto new_table = [&](uint64_t slots) {  // slots + 1 entries: the last for the empty marker's key
+    auto *t = static_cast<Slot *>(tmp_alloc(c, (slots + 1) * sizeof(Slot)));
+    k_slots_init<<<stream_grid(slots + 1, kBlock * 4, 8192), kBlock, 0, c.stream>>>(t, slots + 1);
+    return t;
+  };
+  Slot *table = nullptr;
   uint64_t slots = full;
   int lg = lg_full;
   if (full > kSmallSlots) {
     // few distinct keys fit the small table; a long probe run there means too many
-    table = static_cast<unsigned long long *>(tmp_alloc(c, kSmallSlots * 8));
-    fill_u64(c, reinterpret_cast<uint64_t *>(table), kSmallSlots, 0);
+    table = new_table(kSmallSlots);
     fill_u64(c, c.scratch.slots + 22, 1, 0);
     k_insert1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, kSmallSlots - 1, 64 - kSmallLg, kMaxRun, d_over);
     check_hip(hipGetLastError(), "group_id insert");
@@ -573,8 +579,7 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
     }
   }
   if (!table) {
-    table = static_cast<unsigned long long *>(tmp_alloc(c, full * 8));
-    fill_u64(c, reinterpret_cast<uint64_t *>(table), full, 0);
+    table = new_table(full);
     k_insert1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, full - 1, 64 - lg_full, kNoLimit, d_over);
     check_hip(hipGetLastError(), "group_id insert");
   }
@@ -583,26 +588,22 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
   // first rows flagged from the table, then listed in row order
   auto *first = static_cast<uint8_t *>(tmp_alloc(c, (n + 7) / 8 * 8));
   fill_u64(c, reinterpret_cast<uint64_t *>(first), (n + 7) / 8, 0);
-  k_first_flags<<<stream_grid(slots, kBlock * 4, 8192), kBlock, 0, c.stream>>>(table, slots, first);
+  k_first_flags<<<stream_grid(slots + 1, kBlock * 4, 8192), kBlock, 0, c.stream>>>(table, slots, first);
   auto *firsts = static_cast<int64_t *>(tmp_alloc(c, n * 8));
   uint64_t *d_cnt = c.scratch.slots + 21;
   SelectSpec ss{2, VK_I64, 0, 0, 0};
   select_rows(c, ss, nullptr, first, n, firsts, d_cnt);
   uint64_t groups = 0;
   read_back(c, d_cnt, 8, &groups);
-  auto *entry = static_cast<SlotEntry *>(tmp_alloc(c, slots * sizeof(SlotEntry)));
-  k_slot_entries<<<stream_grid(slots, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, table, slots, entry);
-  k_number1<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(ka, firsts, groups, table, mask, shift,
-                                                                         entry);
+  k_number1<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(ka, firsts, groups, table, mask, shift);
   if (groups <= kLdsGroups)
-    k_ids1<<<g, kGBlock, 0, c.stream>>>(ka, n, entry, mask, shift, out);
+    k_ids1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, mask, shift, out);
   else
-    k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, entry, mask, shift, out);
+    k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, table, mask, shift, out);
   check_hip(hipGetLastError(), "group_id ids");
   tmp_free(c, table);
   tmp_free(c, first);
   tmp_free(c, firsts);
-  tmp_free(c, entry);
 }
 
 }  // namespace vk
