@@ -1186,7 +1186,7 @@ void plan_launch(vk_bplan &p, PlanStmt &s) {
     plan_tiles_dev(c, s.dp, ni, dev_n, 1, select_tile_rows(s.items_per_thread));
     select_batch_dev(c, s.ss, s.items_per_thread, s.dp, ni, s.counts);
   } else if (s.kind == 1) {
-    plan_tiles_dev(c, s.dp, ni, dev_n, 16, canon_tile_blocks());
+    plan_tiles_dev(c, s.dp, ni, dev_n, 16, canon_tile_blocks(), canon_tile_min_rows());
     canon_batch_dev(c, s.cs, s.dp, ni, s.acc, s.zero, reinterpret_cast<double *>(s.res));
   } else {
     plan_tiles_dev(c, s.dp, ni, dev_n, 1, s.tile_rows);

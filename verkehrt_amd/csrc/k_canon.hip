@@ -81,6 +81,52 @@ __global__ __launch_bounds__(64) void k_canon_chain(RV rv, bool mul, const doubl
   if (l < 16) acc[l] = a;
 }
 
+// Batched items of fewer than kChainItemRows rows, one wave each: acc[item * 16 ..] <- the
+// item's whole blocks folded in order from acc (as k_canon_chain). Many items run at once, so
+// the dependent chains overlap; the tile form's per-tile machinery would cost more than the
+// chain of a small item. Larger items are left to the tile form.
+constexpr uint64_t kChainItemRows = uint64_t(1) << 16;
+
+template <class RV>
+__global__ __launch_bounds__(64) void k_canon_chain_batch(RV rv, const BatchItem *items, uint64_t nitems,
+                                                           double *acc) {
+  const int l = lane_id();
+  for (uint64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItem &I = items[it];
+    if (I.n >= kChainItemRows) continue;
+    const double *x = static_cast<const double *>(I.a);
+    const double *y = I.b ? static_cast<const double *>(I.b) : x;
+    const uint64_t rows = I.n / 16 * 16;
+    double a = l < 16 ? acc[it * 16 + l] : 0.0;
+    const uint64_t chunks = (rows + 63) / 64;
+    const uint64_t groups = (chunks + kGroup - 1) / kGroup;
+    double cur[kGroup], nxt[kGroup];
+    auto load = [&](double *v, uint64_t g) {
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) {
+        const uint64_t r = (g * kGroup + k) * 64 + l;
+        v[k] = r < rows ? rv(x, y, r) : 0.0;
+      }
+    };
+    if (groups > 0) load(cur, 0);
+    for (uint64_t g = 0; g < groups; ++g) {
+      if (g + 1 < groups) load(nxt, g + 1);
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) {
+        const uint64_t base = (g * kGroup + k) * 64;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double v = __shfl(cur[k], q * 16 + (l & 15));
+          if (l < 16 && base + q * 16 < rows) a = a + v;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) cur[k] = nxt[k];
+    }
+    if (l < 16) acc[it * 16 + l] = a;
+  }
+}
+
 // ------------------------------------------------------------------ fast form
 // The 16 chains over tiles of 1024 blocks (16384 rows), one pass (canon_binade.h); a block
 // of 1024 threads, thread t owning chain j = t % 16 and run r = t / 16 (64 runs of 16 blocks):
@@ -1238,11 +1284,22 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
   const double id = s.kind == 4 ? 1.0 : 0.0;
   k_fill_d<<<stream_grid(nitems * 16, kBlock, 4096), kBlock, 0, c.stream>>>(acc, nitems * 16, id);
   k_fill_d<<<stream_grid(nitems * 16, kBlock, 4096), kBlock, 0, c.stream>>>(zero, nitems * 16, id);
+  if (s.kind == 4) usage_error("batched products are not supported");
   std::vector<BatchItem> plan_items(items);
-  for (auto &it : plan_items) it.n = it.n / 16 * 16;  // tiles cover whole blocks only
+  bool small = false;
+  for (auto &it : plan_items) {  // tiles cover whole blocks of the larger items only
+    small = small || (it.n < kChainItemRows && it.n >= 16);
+    it.n = it.n < kChainItemRows ? 0 : it.n / 16 * 16;
+  }
   BatchPlan p = batch_plan(c, plan_items, uint64_t(kTileBlocks) * 16);
+  BatchPlan q = batch_plan(c, items, ~uint64_t(0) >> 1);  // the item table (full rows)
+  if (small) {
+    with_row(s, [&](auto rv) {
+      k_canon_chain_batch<<<uint32_t(nitems < 65535 ? nitems : 65535), 64, 0, c.stream>>>(rv, q.items, nitems, acc);
+    });
+    check_hip(hipGetLastError(), "canon batch chains");
+  }
   if (p.ntiles > 0) {
-    if (s.kind == 4) usage_error("batched products are not supported");
     uint64_t *status = status_words(c, ST_CANON, uint64_t(p.ntiles) * kStride);
     const uint32_t epoch = next_epoch(c);
     with_row(s, [&](auto rv) {
@@ -1251,7 +1308,6 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
     });
     check_hip(hipGetLastError(), "canon batch");
   }
-  BatchPlan q = batch_plan(c, items, ~uint64_t(0) >> 1);  // the item table (full rows), for the tails
   with_row(s, [&](auto rv) {
     k_canon_batch_finish<<<stream_grid(nitems, kBlock, 4096), kBlock, 0, c.stream>>>(rv, s.kind == 4, q.items,
                                                                                     nitems, acc, d_out);
@@ -1265,6 +1321,8 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
 
 uint64_t canon_tile_blocks() { return kTileBlocks; }
 
+uint64_t canon_tile_min_rows() { return kChainItemRows; }
+
 // Batched canonical sums over device-planned tiles (plan_tiles_dev with unit 16 and
 // canon_tile_blocks() blocks per tile): acc and zero are nitems x 16 doubles of scratch.
 void canon_batch_dev(Ctx &c, const CanonSpec &s, const DevPlan &p, uint32_t nitems, double *acc, double *zero,
@@ -1274,6 +1332,7 @@ void canon_batch_dev(Ctx &c, const CanonSpec &s, const DevPlan &p, uint32_t nite
   k_fill_d<<<stream_grid(uint64_t(nitems) * 16, kBlock, 4096), kBlock, 0, c.stream>>>(acc, uint64_t(nitems) * 16, 0.0);
   k_fill_d<<<stream_grid(uint64_t(nitems) * 16, kBlock, 4096), kBlock, 0, c.stream>>>(zero, uint64_t(nitems) * 16, 0.0);
   with_row(s, [&](auto rv) {
+    k_canon_chain_batch<<<nitems < 65535 ? nitems : 65535, 64, 0, c.stream>>>(rv, p.items, nitems, acc);
     if (p.max_tiles > 0)
       k_canon_fast<decltype(rv), true><<<p.max_tiles, kFastThreads, 0, c.stream>>>(
           rv, nullptr, nullptr, 0, zero, acc, p.status, 0, p.ticket, p.max_tiles, p.items, p.tile_item, p.ntiles,

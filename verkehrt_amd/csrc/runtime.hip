@@ -213,14 +213,15 @@ constexpr int kPlanThreads = 1024;
 // of items, the range totals are scanned in LDS, then each thread writes its items' tile0.
 __global__ __launch_bounds__(kPlanThreads) void k_plan_items(BatchItem *items, uint32_t nitems,
                                                              const uint64_t *dev_n, uint32_t unit,
-                                                             uint64_t per_tile, uint32_t max_tiles,
-                                                             uint32_t *ntiles) {
+                                                             uint64_t per_tile, uint64_t min_rows,
+                                                             uint32_t max_tiles, uint32_t *ntiles) {
   __shared__ uint64_t s_part[kPlanThreads];
   const uint32_t t = threadIdx.x;
   const uint32_t per = (nitems + kPlanThreads - 1) / kPlanThreads;
   const uint32_t i0 = t * per < nitems ? t * per : nitems;
   const uint32_t i1 = i0 + per < nitems ? i0 + per : nitems;
-  auto tiles_of = [&](uint64_t n) { return (n / unit + per_tile - 1) / per_tile; };
+  // items below min_rows rows get no tiles (another kernel of the statement takes them)
+  auto tiles_of = [&](uint64_t n) { return n < min_rows ? 0 : (n / unit + per_tile - 1) / per_tile; };
   uint64_t sum = 0;
   for (uint32_t i = i0; i < i1; ++i) {
     const uint64_t n = dev_n ? dev_n[i] : items[i].n;
@@ -263,9 +264,10 @@ __global__ void k_plan_map(const BatchItem *items, uint32_t nitems, uint32_t max
 }  // namespace
 
 void plan_tiles_dev(Ctx &c, const DevPlan &p, uint32_t nitems, const uint64_t *dev_n, uint32_t unit,
-                    uint64_t per_tile) {
+                    uint64_t per_tile, uint64_t min_rows) {
   if (nitems == 0) return;
-  k_plan_items<<<1, kPlanThreads, 0, c.stream>>>(p.items, nitems, dev_n, unit, per_tile, p.max_tiles, p.ntiles);
+  k_plan_items<<<1, kPlanThreads, 0, c.stream>>>(p.items, nitems, dev_n, unit, per_tile, min_rows, p.max_tiles,
+                                                 p.ntiles);
   if (p.max_tiles)
     k_plan_map<<<(p.max_tiles + kBlock - 1) / kBlock, kBlock, 0, c.stream>>>(p.items, nitems, p.max_tiles, p.ntiles,
                                                                            p.tile_item);

@@ -256,9 +256,10 @@ struct DevPlan {
   uint32_t *ticket = nullptr;     // ticket counter, zero before a launch and after it
 };
 // Item i has n_i rows (dev_n[i] when dev_n is set, written to items[i].n; else items[i].n) and
-// ceil(floor(n_i / unit) / per_tile) tiles; writes items[i].tile0, the map and the tile count.
+// ceil(floor(n_i / unit) / per_tile) tiles, none when n_i < min_rows; writes items[i].tile0,
+// the map and the tile count.
 void plan_tiles_dev(Ctx &c, const DevPlan &p, uint32_t nitems, const uint64_t *dev_n, uint32_t unit,
-                    uint64_t per_tile);
+                    uint64_t per_tile, uint64_t min_rows = 0);
 // Rows per tile of the batched selection with 8 or 16 rows per thread.
 uint64_t select_tile_rows(int items_per_thread);
 void select_batch_dev(Ctx &c, const SelectSpec &s, int items_per_thread, const DevPlan &p, uint32_t nitems,
@@ -266,6 +267,9 @@ void select_batch_dev(Ctx &c, const SelectSpec &s, int items_per_thread, const D
 // Batched canonical sums (plan unit 16, canon_tile_blocks() blocks per tile); acc and zero are
 // nitems x 16 doubles of scratch.
 uint64_t canon_tile_blocks();
+// Batched canonical sums fold items of fewer rows than this one wave each, in order; larger
+// items go by tiles (plan their tiles with min_rows = canon_tile_min_rows()).
+uint64_t canon_tile_min_rows();
 uint64_t canon_status_words_per_tile();
 void canon_batch_dev(Ctx &c, const CanonSpec &s, const DevPlan &p, uint32_t nitems, double *acc, double *zero,
                      double *d_out);
