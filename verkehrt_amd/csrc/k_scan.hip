@@ -3,9 +3,9 @@
 // min and max (IEEE minimum/maximum on floats, ScanExtreme scan.rs:164-183). A start byte
 // restarts the fold; row 0 of the whole array always starts.
 //
-// One pass over 8192-row tiles (512 threads x 16 rows), data in registers only: wave w of
-// a tile owns its 1024 consecutive rows, so every wave load and store covers 64 (or, with
-// 16-byte accesses, 128) consecutive rows. Each wave scans its rows item by item with DPP
+// One pass over 16384-row tiles (512 threads x 32 rows; scan_by: 8192-row tiles of 16 rows
+// per thread), data in registers only: wave w of a tile owns its 2048 (1024) consecutive rows,
+// so every wave load and store covers 64 (or, with 16-byte accesses, 128) consecutive rows. Each wave scans its rows item by item with DPP
 // row shifts and broadcasts (no LDS), carrying the running value from item to item; the
 // eight wave aggregates meet in wave 0, which publishes the tile aggregate and joins it to
 // the tiles before by decoupled look-back on (aggregate, has-start) pairs: a predecessor whose
@@ -23,10 +23,12 @@ namespace vk {
 namespace {
 
 constexpr int kScanBlock = 512;
-constexpr int kRows = 16;                     // rows per thread
-constexpr int kTile = kScanBlock * kRows;     // 8192 rows
 constexpr int kWaves = kScanBlock / kWave;
-constexpr int kWaveRows = kWave * kRows;      // 1024 rows per wave
+// Rows per thread: 32 for plain scans (16K-row tiles: half the tile tickets and look-backs of
+// 8K-row tiles, measured faster although one block fits a compute unit), 16 for scan_by, whose
+// start flags would need the registers of two waves per SIMD at 32 rows.
+constexpr int scan_rows(bool by) { return by ? 16 : 32; }
+constexpr int scan_tile(bool by) { return kScanBlock * scan_rows(by); }
 
 template <typename U, int OP>
 __device__ __forceinline__ U apply(U a, U b) {
@@ -146,7 +148,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const TIn *__restrict__ x,
                                                  U *__restrict__ out, uint64_t *status,
                                                  uint32_t epoch, uint32_t *ticket, uint32_t ntiles,
                                                  const uint64_t *carry_in, uint64_t *carry_out) {
-  constexpr int L = kRows / VEC;  // accesses per thread
+  constexpr int kRows = scan_rows(BY);
+  constexpr int kTile = scan_tile(BY);
+  constexpr int kWaveRows = kWave * kRows;  // rows per wave
+  constexpr int L = kRows / VEC;            // accesses per thread
   __shared__ Seg<U> s_wave[kWaves];
   __shared__ U s_pre[kWaves];
   const uint32_t tile = take_ticket(ticket, ntiles);
@@ -298,7 +303,8 @@ void launch_as(Ctx &c, const void *x, const uint8_t *starts, uint64_t n, void *o
 template <typename TIn, typename U, int OP>
 void launch(Ctx &c, bool by, const void *x, const uint8_t *starts, uint64_t n, void *out,
             uint64_t *carry) {
-  const uint64_t tiles = (n + kTile - 1) / kTile;
+  const uint64_t tile = scan_tile(by);
+  const uint64_t tiles = (n + tile - 1) / tile;
   if (tiles > 0xFFFFFFFFull) usage_error("scan too large");
   uint64_t *status = status_words(c, ST_SCAN, tiles * 4);
   const uint32_t epoch = next_epoch(c);
