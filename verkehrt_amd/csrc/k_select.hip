@@ -63,14 +63,12 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
 // All of a thread's loads are issued before the first ballot (a ballot is convergent, so
 // loads interleaved with ballots are not hoisted and would complete one at a time); rows past
 // the end of the last tile load the last row and are never kept, so no load is predicated.
-// Up to 16 rows per thread, two blocks per compute unit (8 waves per SIMD, at most 64 VGPRs)
-// keep one block loading while the other waits on its look-back; single calls take 32 rows per
-// thread (32K-row tiles, one block per compute unit at up to 128 VGPRs): half the tickets and
-// look-backs per row, measured 9-16% faster at 2^28 rows where at least half the rows are kept.
+// Two blocks per compute unit (8 waves per SIMD, at most 64 VGPRs) keep one block loading while
+// the other waits on its look-back.
 // ITEMS rows per thread (a tile is 1024 x ITEMS rows). A launch whose tiles were planned on
 // the device passes dev_ntiles (blocks past it exit) and dev_epoch (the epoch to publish with).
 template <typename T, int KIND, bool BATCH, int ITEMS = kItems>
-__global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(ITEMS >= 32 ? 4 : 8, 8))) void k_select(const T *__restrict__ x,
+__global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
                                                    uint32_t keepset, T c, uint64_t base, void *out_v,
                                                    uint64_t *status, uint32_t epoch,
@@ -184,14 +182,13 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(ITEMS
 template <typename T, int KIND>
 void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uint64_t cbits,
             uint64_t base, void *out, uint64_t *d_count) {
-  constexpr int kI = 32;  // rows per thread of a single call
-  const uint64_t tiles = (n + uint64_t(kSelBlock) * kI - 1) / (uint64_t(kSelBlock) * kI);
+  const uint64_t tiles = (n + kTile - 1) / kTile;
   if (tiles > 0xFFFFFFFFull) usage_error("selection too large");
   uint64_t *status = status_words(c, ST_SELECT, tiles);
   const uint32_t epoch = next_epoch(c);
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
-  k_select<T, KIND, false, kI><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
+  k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
       static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, status, epoch,
       c.scratch.counters, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);
   check_hip(hipGetLastError(), "select launch");
