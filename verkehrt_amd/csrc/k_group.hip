@@ -134,7 +134,6 @@ constexpr int kGRows = 8;  // rows per thread per chunk
 constexpr uint64_t kGChunk = uint64_t(kGBlock) * kGRows;
 constexpr int kLBits = 12;
 constexpr int kLSlots = 1 << kLBits;  // LDS entries: 32 KB of keys, 16 KB of values
-constexpr uint32_t kLFull = kLSlots / 2;  // no new entries beyond half full
 constexpr int kLProbes = 32;
 constexpr uint64_t kLEmpty = 0xFFF8DEADBEEF1357ull;  // LDS empty marker
 constexpr int kSmallLg = 21;
@@ -143,13 +142,16 @@ constexpr int kMaxRun = 64;  // longest probe run allowed in the small table
 constexpr int kNoLimit = 0x7FFFFFFF;
 constexpr uint64_t kLdsGroups = uint64_t(1) << 16;  // more keys than this: k_ids2 instead of k_ids1
 
-// The LDS entry of key k, or -1. An empty entry is claimed (*claimed set) while fewer than
-// kLFull entries are taken; after that only existing entries are found.
+// The LDS entry of key k in a table of 2^BITS entries, or -1. An empty entry is claimed
+// (*claimed set) while fewer than half the entries are taken; after that only existing entries
+// are found.
+template <int BITS = kLBits>
 __device__ __forceinline__ int lds_find(unsigned long long *s_key, uint32_t *s_cnt, uint64_t k, bool *claimed) {
+  constexpr int kSlots = 1 << BITS;
   *claimed = false;
   if (k == kLEmpty) return -1;
-  int h = static_cast<int>(mix(k) >> (64 - kLBits));
-  const bool grow = __hip_atomic_load(s_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < kLFull;
+  int h = static_cast<int>(mix(k) >> (64 - BITS));
+  const bool grow = __hip_atomic_load(s_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < uint32_t(kSlots / 2);
   for (int p = 0; p < kLProbes; ++p) {
     const unsigned long long cur = __hip_atomic_load(&s_key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (cur == k) return h;
@@ -163,7 +165,7 @@ __device__ __forceinline__ int lds_find(unsigned long long *s_key, uint32_t *s_c
       }
       if (old == k) return h;
     }
-    h = (h + 1) & (kLSlots - 1);
+    h = (h + 1) & (kSlots - 1);
   }
   return -1;
 }
@@ -357,61 +359,145 @@ struct ItemRows {
   }
 };
 
+// The batched tables hold u64 words (first row + 1); a key is compared through the column.
+// Inserts (key k, row r): claims an empty slot with r + 1 or lowers the word of k's slot.
+__device__ void word_insert(const KeyCol &a, unsigned long long *table, uint64_t mask, int shift, uint64_t k,
+                            uint64_t r) {
+  uint64_t h = mix(k) >> shift;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    unsigned long long s = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s == 0) {
+      const unsigned long long old = atomicCAS(&table[h], 0ull, r + 1);
+      if (old == 0) return;
+      s = old;
+    }
+    if (a.bits(s - 1) == k) {
+      if (r + 1 < s) atomicMin(&table[h], r + 1);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// The slot of key k (every key is in the table).
+__device__ uint64_t word_slot(const KeyCol &a, const unsigned long long *table, uint64_t mask, int shift,
+                              uint64_t k) {
+  uint64_t h = mix(k) >> shift;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    const uint64_t w = table[h];
+    if (w != 0 && a.bits(w - 1) == k) return h;
+    h = (h + 1) & mask;
+  }
+  return 0;  // unreachable: every key was inserted on its own probe path
+}
+
+// A tile of an item (kGTile rows, 16 per thread) goes through an LDS table of 2^kBLBits keys
+// first, as in the single-key form: rows of a key the tile has seen stay in LDS, and each of
+// the tile's keys reaches the item's table once.
+constexpr int kBLBits = 11;
+constexpr int kBLSlots = 1 << kBLBits;
+constexpr int kBRows = static_cast<int>(kGTile / kBlock);
+
 __global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const BatchItem *items,
                                                          const uint32_t *tile_item,
                                                          unsigned long long *tables) {
+  __shared__ unsigned long long s_key[kBLSlots];
+  __shared__ uint32_t s_row[kBLSlots];
+  __shared__ uint32_t s_cnt;
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const ItemRows R(I, blockIdx.x);
   const KeyCol a{type, I.a};
   unsigned long long *table = tables + I.aux2;
   const int lg = static_cast<int>(I.aux3);
   const uint64_t mask = (uint64_t(1) << lg) - 1;
-  for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) {
-    const uint64_t ka = a.bits(r);
-    uint64_t h = slot_of<false>(a, a, ka, 0, 64 - lg);
-    while (true) {
-      unsigned long long s = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (s == 0) {
-        const unsigned long long old = atomicCAS(&table[h], 0ull, r + 1);
-        if (old == 0) break;
-        s = old;
-      }
-      if (a.bits(s - 1) == ka) {
-        if (r + 1 < s) atomicMin(&table[h], r + 1);
-        break;
-      }
-      h = (h + 1) & mask;
+  const int t = threadIdx.x;
+  for (int i = t; i < kBLSlots; i += kBlock) {
+    s_key[i] = kLEmpty;
+    s_row[i] = 0xFFFFFFFFu;
+  }
+  if (t == 0) s_cnt = 0;
+  __syncthreads();
+  uint64_t k[kBRows];
+#pragma unroll
+  for (int j = 0; j < kBRows; ++j) {
+    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+    k[j] = r < R.t1 ? a.bits(r) : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kBRows; ++j) {
+    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+    if (r >= R.t1) continue;
+    bool claimed;
+    const int h = lds_find<kBLBits>(s_key, &s_cnt, k[j], &claimed);
+    if (h >= 0) {
+      if (__hip_atomic_load(&s_row[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > r)
+        atomicMin(&s_row[h], static_cast<uint32_t>(r));
+    } else {
+      word_insert(a, table, mask, 64 - lg, k[j], r);
     }
+  }
+  __syncthreads();
+  for (int i = t; i < kBLSlots; i += kBlock) {
+    const uint64_t kk = s_key[i];
+    if (kk != kLEmpty) word_insert(a, table, mask, 64 - lg, kk, s_row[i]);
   }
 }
 
+// Phase 1 finds (or claims) every row's LDS entry, the claiming lane resolving the key's slot
+// and first row; after the barrier, phase 2 writes each row's slot and first-row flag.
 __global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const BatchItem *items,
                                                         const uint32_t *tile_item,
                                                         const unsigned long long *tables,
                                                         uint32_t *slot, uint8_t *first) {
+  __shared__ unsigned long long s_key[kBLSlots];
+  __shared__ uint32_t s_slot[kBLSlots];
+  __shared__ uint32_t s_rep[kBLSlots];
+  __shared__ uint32_t s_cnt;
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const ItemRows R(I, blockIdx.x);
   const KeyCol a{type, I.a};
   const unsigned long long *table = tables + I.aux2;
   const int lg = static_cast<int>(I.aux3);
   const uint64_t mask = (uint64_t(1) << lg) - 1;
-  for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) {
-    const uint64_t ka = a.bits(r);
-    uint64_t h = slot_of<false>(a, a, ka, 0, 64 - lg);
-    while (true) {
-      const uint64_t w = table[h];
-      if (w == 0) {  // unreachable: every key was inserted on its own probe path
-        slot[I.aux + r] = 0;
-        first[I.aux + r] = 0;
-        break;
-      }
-      if (a.bits(w - 1) == ka) {
-        slot[I.aux + r] = static_cast<uint32_t>(h);
-        first[I.aux + r] = (w - 1) == r;
-        break;
-      }
-      h = (h + 1) & mask;
+  const int t = threadIdx.x;
+  for (int i = t; i < kBLSlots; i += kBlock) s_key[i] = kLEmpty;
+  if (t == 0) s_cnt = 0;
+  __syncthreads();
+  uint64_t k[kBRows];
+  int h[kBRows];
+#pragma unroll
+  for (int j = 0; j < kBRows; ++j) {
+    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+    k[j] = r < R.t1 ? a.bits(r) : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kBRows; ++j) {
+    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+    h[j] = -1;
+    if (r >= R.t1) continue;
+    bool claimed;
+    h[j] = lds_find<kBLBits>(s_key, &s_cnt, k[j], &claimed);
+    if (claimed) {
+      const uint64_t g = word_slot(a, table, mask, 64 - lg, k[j]);
+      s_slot[h[j]] = static_cast<uint32_t>(g);
+      s_rep[h[j]] = static_cast<uint32_t>(table[g] - 1);
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kBRows; ++j) {
+    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+    if (r >= R.t1) continue;
+    uint64_t g, rep;
+    if (h[j] >= 0) {
+      g = s_slot[h[j]];
+      rep = s_rep[h[j]];
+    } else {
+      g = word_slot(a, table, mask, 64 - lg, k[j]);
+      rep = table[g] - 1;
+    }
+    slot[I.aux + r] = static_cast<uint32_t>(g);
+    first[I.aux + r] = rep == r;
   }
 }
 
@@ -490,16 +576,15 @@ void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_
 
 namespace {
 
-// Passes 2-4 of the file header over a filled global table: probe (per-row slot and first
+// Passes 2-4 of the file header over a filled two-key table: probe (per-row slot and first
 // flag), the first rows in row order, number, ids.
-void ids_from_slots(Ctx &c, KeyCol ka, KeyCol kb, bool pair, uint64_t n, unsigned long long *table,
-                    uint64_t slots, int lg, int64_t *out) {
+void ids_from_slots(Ctx &c, KeyCol ka, KeyCol kb, uint64_t n, unsigned long long *table, uint64_t slots,
+                    int lg, int64_t *out) {
   auto *slot = static_cast<uint32_t *>(tmp_alloc(c, n * 4));
   auto *first = static_cast<uint8_t *>(tmp_alloc(c, n));
   const uint32_t g = stream_grid(n, kBlock * 4, 8192);
   const int shift = 64 - lg;
-  if (pair) k_probe<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
-  else k_probe<false><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
+  k_probe<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, shift, slot, first);
   check_hip(hipGetLastError(), "group_id probe");
   // first rows in row order
   auto *firsts = static_cast<int64_t *>(tmp_alloc(c, n * 8));
@@ -531,7 +616,7 @@ void group_id_pair(Ctx &c, KeyCol ka, KeyCol kb, uint64_t n, int64_t *out) {
   const uint32_t g = stream_grid(n, kBlock * 4, 8192);
   k_insert<true><<<g, kBlock, 0, c.stream>>>(ka, kb, n, table, slots - 1, 64 - lg);
   check_hip(hipGetLastError(), "group_id table");
-  ids_from_slots(c, ka, kb, true, n, table, slots, lg, out);
+  ids_from_slots(c, ka, kb, n, table, slots, lg, out);
   tmp_free(c, table);
 }
 
