@@ -100,14 +100,6 @@ __global__ void k_check_keys(const int64_t *__restrict__ keys, uint64_t n, uint6
 
 // ----------------------------------------------------------- f64 reduce_into add
 
-// chain key of row i < b: bucket * 16 + i % 16 (u32 when 16 x buckets fits, else u64)
-template <typename K>
-__global__ void k_chain_keys(const int64_t *__restrict__ keys, uint64_t b, K *ck) {
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < b;
-       i += uint64_t(gridDim.x) * blockDim.x)
-    ck[i] = static_cast<K>(static_cast<uint64_t>(keys[i]) * 16 + (i & 15));
-}
-
 // One wave per chain start for long chains: the lanes load 64 values at a time, the wave folds
 // them in order with constant-lane reads (readlane, no LDS round trip) while the next chunks
 // load. Every lane holds the same running value.
@@ -414,15 +406,12 @@ template <typename K>
 bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
                  double *stripes, double *out) {
   constexpr bool wide = sizeof(K) == 8;
-  K *ck = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
   K *sk = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
   double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
   const uint32_t g = stream_grid(b, kBlock * 4, 8192);
-  k_chain_keys<K><<<g, kBlock, 0, c.stream>>>(keys, b, ck);
-  check_hip(hipGetLastError(), "chain keys");
-  sort_key_values(c, wide, ck, reinterpret_cast<const uint64_t *>(vals), b, sk,
-                  reinterpret_cast<uint64_t *>(sv));
-  tmp_free(c, ck);
+  // the sort computes each row's chain key from its bucket key as it reads it
+  sort_chain_values(c, wide, keys, reinterpret_cast<const uint64_t *>(vals), b, sk,
+                    reinterpret_cast<uint64_t *>(sv));
   const bool by_bucket = buckets >= kBucketFoldMin;
   // the starts of chains (stripes) or buckets (by_bucket), compacted in order
   uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
@@ -479,17 +468,6 @@ __global__ __launch_bounds__(kBlock) void k_check_keys_batch(const BatchItem *it
     my_bad = o < my_bad ? o : my_bad;
   }
   if (lane_id() == 0 && my_bad != ~uint64_t(0)) atomicMin(reinterpret_cast<unsigned long long *>(&bad[it]), my_bad);
-}
-
-// chain keys of the whole blocks of every item, at the item's concatenated rows (aux)
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_chain_keys_batch(const BatchItem *items, const uint32_t *tile_item,
-                                                             K *ck) {
-  const BatchItem &I = items[tile_item[blockIdx.x]];
-  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile, t1 = t0 + kBTile < I.n ? t0 + kBTile : I.n;
-  const int64_t *keys = static_cast<const int64_t *>(I.a);
-  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock)
-    ck[I.aux + r] = static_cast<K>(static_cast<uint64_t>(keys[r]) * 16 + (r & 15));
 }
 
 // One wave per 64 sorted rows of an item: the wave folds, one after another, the chains that
@@ -584,22 +562,16 @@ void reduce_into_add_f64_batch(Ctx &c, std::vector<BatchItem> &items) {
   }
   if (total > 0) {
     const size_t kw = wide ? 8 : 4;
-    void *ck = tmp_alloc(c, total * kw);
     void *sk = tmp_alloc(c, total * kw);
     double *sv = static_cast<double *>(tmp_alloc(c, total * 8));
     BatchPlan p = batch_plan(c, whole, kBTile);
-    if (wide) k_chain_keys_batch<uint64_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<uint64_t *>(ck));
-    else k_chain_keys_batch<uint32_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<uint32_t *>(ck));
-    check_hip(hipGetLastError(), "chain keys batch");
-    // the sort reads each item's chain keys at a and its values at b
-    std::vector<BatchItem> sorting(whole);
-    for (auto &it : sorting) it.a = static_cast<char *>(ck) + it.aux * kw;
-    sort_key_values_batch(c, wide, sorting, sk, reinterpret_cast<uint64_t *>(sv));
+    // the sort reads each item's bucket keys at a (chain keys computed as it reads them) and
+    // its values at b
+    sort_chain_values_batch(c, wide, whole, sk, reinterpret_cast<uint64_t *>(sv));
     if (wide) k_chain_fold_batch<uint64_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<const uint64_t *>(sk), sv, stripes);
     else k_chain_fold_batch<uint32_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<const uint32_t *>(sk), sv, stripes);
     check_hip(hipGetLastError(), "chain fold batch");
     batch_plan_free(c, p);
-    tmp_free(c, ck);
     tmp_free(c, sk);
     tmp_free(c, sv);
   }

@@ -27,7 +27,8 @@ constexpr int kSortBlock = 1024;             // threads of a pass block
 constexpr int kTile = kSortBlock * kItems;   // 16384 rows
 constexpr int kWaves = kSortBlock / kWave;
 constexpr int kRadix = 256;
-constexpr int32_t kRawU32 = 100;  // internal key type: u32 keys sorted as they are
+constexpr int32_t kRawU32 = 100;    // internal key type: u32 keys sorted as they are
+constexpr int32_t kChainKey = 101;  // internal key type: i64 bucket k of row r as k * 16 + r % 16
 
 template <typename U>
 __device__ __forceinline__ U image(int32_t type, const void *keys, uint64_t r) {
@@ -40,6 +41,7 @@ __device__ __forceinline__ U image(int32_t type, const void *keys, uint64_t r) {
     case VK_I32: return U(static_cast<const uint32_t *>(keys)[r] ^ 0x80000000u);
     case VK_U8: return U(static_cast<const uint8_t *>(keys)[r]);
     case kRawU32: return U(static_cast<const uint32_t *>(keys)[r]);
+    case kChainKey: return U(static_cast<const uint64_t *>(keys)[r] * 16 + (r & 15));
     default: return U(static_cast<const uint64_t *>(keys)[r]);  // u64
   }
 }
@@ -608,6 +610,22 @@ void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, 
     radix<uint64_t, uint64_t>(c, VK_U64, keys, vals, n, nullptr, static_cast<uint64_t *>(sorted_keys), sorted_vals);
   else
     radix<uint32_t, uint64_t>(c, kRawU32, keys, vals, n, nullptr, static_cast<uint32_t *>(sorted_keys), sorted_vals);
+}
+
+void sort_chain_values(Ctx &c, bool wide, const int64_t *keys, const uint64_t *vals, uint64_t n,
+                       void *sorted_keys, uint64_t *sorted_vals) {
+  if (wide)
+    radix<uint64_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint64_t *>(sorted_keys), sorted_vals);
+  else
+    radix<uint32_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint32_t *>(sorted_keys), sorted_vals);
+}
+
+void sort_chain_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
+                             uint64_t *sorted_vals) {
+  if (wide)
+    radix_batch<uint64_t, uint64_t>(c, kChainKey, items, static_cast<uint64_t *>(sorted_keys), sorted_vals);
+  else
+    radix_batch<uint32_t, uint64_t>(c, kChainKey, items, static_cast<uint32_t *>(sorted_keys), sorted_vals);
 }
 
 void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,

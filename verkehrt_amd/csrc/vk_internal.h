@@ -319,10 +319,18 @@ void sort_u64_pairs(Ctx &c, const uint64_t *keys, uint64_t n, uint64_t *sorted_k
 // Stable sort of (key, 8-byte value) pairs by key: keys are u64 (wide) or u32.
 void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, uint64_t n,
                      void *sorted_keys, uint64_t *sorted_vals);
+// The same for chain keys computed on the fly: row r of bucket keys[r] (already checked to lie
+// in 0..buckets) has chain key keys[r] * 16 + r % 16 (u32 unless wide).
+void sort_chain_values(Ctx &c, bool wide, const int64_t *keys, const uint64_t *vals, uint64_t n,
+                       void *sorted_keys, uint64_t *sorted_vals);
 // Batched: item i sorts its keys a with the values b (n rows) into sorted_keys / sorted_vals
 // from row aux (set here) of the concatenated rows.
 void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
                            uint64_t *sorted_vals);
+// The batched form of sort_chain_values: each item's bucket keys at a, its values at b, rows
+// relative to the item.
+void sort_chain_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
+                             uint64_t *sorted_vals);
 
 // group_id (k_group.hip): first-occurrence ids. `b` may be null (one key column).
 void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
