@@ -174,11 +174,12 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
   uint32_t rnk[kItems];
   const uint64_t wbase = t0 + uint64_t(w) * (kTile / kWaves);
   const uint64_t below = (uint64_t(1) << l) - 1;
+  // every load of the tile first: the ballots below are convergent, so loads written in the
+  // same loop would each wait for the previous item's ballots
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const uint64_t r = wbase + uint64_t(i) * kWave + l;
-    const bool valid = r < n;
-    if (valid) {
+    if (r < n) {
       key[i] = FIRST ? image<U>(type, keys_in, r) : static_cast<const U *>(keys_in)[r];
       if (FIRST) pos[i] = POS ? static_cast<V>(r) : static_cast<V>(vals[r]);
       else pos[i] = pos_in[r];
@@ -186,6 +187,11 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
       key[i] = 0;
       pos[i] = 0;
     }
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t r = wbase + uint64_t(i) * kWave + l;
+    const bool valid = r < n;
     const uint32_t d = (key[i] >> shift) & 255;
     uint64_t peers = __ballot(valid);
 #pragma unroll
