@@ -67,6 +67,8 @@ __device__ __forceinline__ void hist_add(uint32_t (&s_h)[sizeof(U)][kRadix], U k
   }
 }
 
+constexpr int kHistAhead = 8;
+
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys, uint64_t n,
                                                  uint32_t *hist) {
@@ -74,9 +76,19 @@ __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys,
   __shared__ uint32_t s_h[P][kRadix];
   for (int k = threadIdx.x; k < P * kRadix; k += kBlock) (&s_h[0][0])[k] = 0;
   __syncthreads();
-  for (uint64_t r = blockIdx.x * uint64_t(kBlock) + threadIdx.x; r < n;
-       r += uint64_t(gridDim.x) * kBlock) {
-    hist_add<U>(s_h, image<U>(type, keys, r));
+  // kHistAhead rows per thread loaded before their ballots (which are convergent: a load in the
+  // same iteration as a ballot would wait for the previous row's)
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t r0 = blockIdx.x * uint64_t(kBlock) + threadIdx.x; r0 < n; r0 += stride * kHistAhead) {
+    U k[kHistAhead];
+#pragma unroll
+    for (int u = 0; u < kHistAhead; ++u) {
+      const uint64_t r = r0 + u * stride;
+      k[u] = r < n ? image<U>(type, keys, r) : U(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kHistAhead; ++u)
+      if (r0 + u * stride < n) hist_add<U>(s_h, k[u]);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < P * kRadix; k += kBlock) {
@@ -376,8 +388,16 @@ __global__ __launch_bounds__(kBlock) void k_hist_batch(int32_t type, const Batch
   const BatchItem &I = items[it];
   const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
   const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
-  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) {
-    hist_add<U>(s_h, image<U>(type, I.a, r));
+  for (uint64_t r0 = t0 + threadIdx.x; r0 < t1; r0 += uint64_t(kBlock) * kHistAhead) {
+    U k[kHistAhead];
+#pragma unroll
+    for (int u = 0; u < kHistAhead; ++u) {
+      const uint64_t r = r0 + uint64_t(u) * kBlock;
+      k[u] = r < t1 ? image<U>(type, I.a, r) : U(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kHistAhead; ++u)
+      if (r0 + uint64_t(u) * kBlock < t1) hist_add<U>(s_h, k[u]);
   }
   __syncthreads();
   uint32_t *h = hist + uint64_t(it) * P * kRadix;
