@@ -107,16 +107,40 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   __shared__ uint32_t s_cnt[kItems][kWaves];
   __shared__ uint32_t s_excl[kItems][kWaves];
   __shared__ uint64_t s_prefix;
-  // item i of thread t is row t0 + i * kSelBlock + t: every wave load is 64 consecutive rows
+  // item i of thread t is row t0 + i * kSelBlock + t: every wave load is 64 consecutive rows.
+  // The mask bytes of a wave's rows are 16 (kItems) segments of 64 bytes: in a full tile of an
+  // aligned mask, lane l loads 16 bytes of segment l / 4 and passes them through LDS, one load
+  // per lane instead of one byte load per item.
+  __shared__ uint8_t s_mask[KIND == 1 ? 1 : kWaves][KIND == 1 ? 1 : kItems][kWave];
   {
     const T *xt = KIND == 2 ? nullptr : x + t0;
     const uint8_t *mt = KIND == 1 ? nullptr : mask + t0;
+    bool wide = false;
+    if constexpr (KIND != 1) {
+      static_assert(kItems * 4 <= kWave, "a lane loads 16 bytes of one segment");
+      wide = lim == uint32_t(kTile) && (reinterpret_cast<uintptr_t>(mt) & 15) == 0;
+      if (wide && (l >> 2) < kItems) {
+        struct alignas(16) Q {
+          uint32_t a[4];
+        };
+        const int i = l >> 2, part = (l & 3) * 16;
+        const Q q = *reinterpret_cast<const Q *>(mt + uint32_t(i) * kSelBlock + uint32_t(w) * kWave + part);
+        *reinterpret_cast<Q *>(&s_mask[w][i][part]) = q;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       const uint32_t lr = uint32_t(i) * kSelBlock + t;
       const uint32_t rr = lr < lim ? lr : lim - 1;
-      if (KIND != 1) mv[i] = mt[rr];
+      if (KIND != 1 && !wide) mv[i] = mt[rr];
       if (KIND != 2) val[i] = static_cast<Out>(xt[rr]);
+    }
+    if constexpr (KIND != 1) {
+      // the wave reads back what it wrote (LDS operations of one wave complete in order)
+      if (wide) {
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) mv[i] = s_mask[w][i][l];
+      }
     }
   }
 #pragma unroll
