@@ -166,22 +166,50 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const TIn *__restrict__ x,
   };
   U v[kRows];
   uint8_t f[kRows];
+  // scan_by in a full tile with 16-byte aligned starts: a wave's start bytes (its rows are
+  // consecutive) are read with one 16-byte load per lane and passed to the row owners through
+  // LDS, instead of a 2-byte load per access
+  __shared__ uint8_t s_st[BY ? kWaves : 1][BY ? kWaveRows : 1];
   {
     const TIn *xt = x + t0;
     const uint8_t *st = BY ? starts + t0 : nullptr;
     if (VEC == 2 && full) {
       using In2 = struct alignas(2 * sizeof(TIn)) { TIn a, b; };
       using St2 = struct alignas(2) { uint8_t a, b; };
+      bool wide = false;
+      if constexpr (BY) {
+        static_assert(kWaveRows == kWave * 16, "one 16-byte load per lane covers a wave's starts");
+        wide = (reinterpret_cast<uintptr_t>(st) & 15) == 0;
+        if (wide) {
+          struct alignas(16) Q {
+            uint32_t a[4];
+          };
+          *reinterpret_cast<Q *>(&s_st[w][l * 16]) =
+              *reinterpret_cast<const Q *>(st + uint32_t(w) * kWaveRows + uint32_t(l) * 16);
+        }
+      }
 #pragma unroll
       for (int k = 0; k < L; ++k) {
         const uint32_t r = lrow(k, 0);
         const In2 p = *reinterpret_cast<const In2 *>(xt + r);
         v[2 * k] = static_cast<U>(p.a);
         v[2 * k + 1] = static_cast<U>(p.b);
-        if (BY) {
+        if (BY && !wide) {
           const St2 q = *reinterpret_cast<const St2 *>(st + r);
           f[2 * k] = q.a;
           f[2 * k + 1] = q.b;
+        }
+      }
+      if constexpr (BY) {
+        // the wave reads back what it wrote (LDS operations of one wave complete in order)
+        if (wide) {
+#pragma unroll
+          for (int k = 0; k < L; ++k) {
+            const uint32_t o = lrow(k, 0) - uint32_t(w) * kWaveRows;
+            const St2 q = *reinterpret_cast<const St2 *>(&s_st[w][o]);
+            f[2 * k] = q.a;
+            f[2 * k + 1] = q.b;
+          }
         }
       }
     } else {
