@@ -226,12 +226,29 @@ __global__ __launch_bounds__(kWave) void k_bucket_fold(const uint64_t *__restric
   }
 }
 
-// flag[p] = 1 where row p of the sorted chain keys starts a bucket (chain key / 16 changes)
-template <typename K>
-__global__ void k_bucket_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag) {
-  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
-       p += uint64_t(gridDim.x) * blockDim.x)
-    flag[p] = (p == 0 || (sk[p - 1] >> 4) != (sk[p] >> 4)) ? 1 : 0;
+// flag[p] = 1 where row p of the sorted chain keys starts a chain (SHIFT 0) or a bucket
+// (SHIFT 4: chain key / 16 changes). A thread takes four consecutive rows and stores their
+// flags as one word (flag is a fresh, aligned allocation).
+template <int SHIFT, typename K>
+__global__ void k_key_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag) {
+  const uint64_t quads = (b + 3) / 4;
+  for (uint64_t q = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; q < quads;
+       q += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t p = q * 4;
+    K k[5];
+    k[0] = p > 0 ? sk[p - 1] : K(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u + 1] = p + u < b ? sk[p + u] : K(0);
+    uint32_t word = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (p + u == 0 || (k[u] >> SHIFT) != (k[u + 1] >> SHIFT)) word |= 1u << (8 * u);
+    if (p + 4 <= b) {
+      *reinterpret_cast<uint32_t *>(flag + p) = word;
+    } else {
+      for (int u = 0; p + u < b; ++u) flag[p + u] = static_cast<uint8_t>((word >> (8 * u)) & 1);
+    }
+  }
 }
 
 // One lane per chain when there are many long chains: a wave takes 64 consecutive chains and
@@ -301,13 +318,6 @@ __global__ __launch_bounds__(kWave) void k_chain_fold_lanes(const uint64_t *__re
     if (live) stripes[sk[p0]] = a;
     __syncthreads();
   }
-}
-
-template <typename K>
-__global__ void k_chain_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag) {
-  for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < b;
-       p += uint64_t(gridDim.x) * blockDim.x)
-    flag[p] = (p == 0 || sk[p - 1] != sk[p]) ? 1 : 0;
 }
 
 __global__ void k_combine_stripes(const double *__restrict__ stripes, uint64_t buckets, double *out) {
@@ -408,7 +418,6 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
   constexpr bool wide = sizeof(K) == 8;
   K *sk = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
   double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
-  const uint32_t g = stream_grid(b, kBlock * 4, 8192);
   // the sort computes each row's chain key from its bucket key as it reads it
   sort_chain_values(c, wide, keys, reinterpret_cast<const uint64_t *>(vals), b, sk,
                     reinterpret_cast<uint64_t *>(sv));
@@ -416,8 +425,9 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
   // the starts of chains (stripes) or buckets (by_bucket), compacted in order
   uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
   uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-  if (by_bucket) k_bucket_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
-  else k_chain_starts<K><<<g, kBlock, 0, c.stream>>>(sk, b, flag);
+  const uint32_t gq = stream_grid((b + 3) / 4, kBlock * 4, 8192);
+  if (by_bucket) k_key_starts<4, K><<<gq, kBlock, 0, c.stream>>>(sk, b, flag);
+  else k_key_starts<0, K><<<gq, kBlock, 0, c.stream>>>(sk, b, flag);
   SelectSpec ss{2, VK_I64, 0, 0, 0};
   uint64_t *d_cnt = c.scratch.slots + 20;
   select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
