@@ -151,12 +151,13 @@ __device__ __forceinline__ int lds_find(unsigned long long *s_key, uint32_t *s_c
   *claimed = false;
   if (k == kLEmpty) return -1;
   int h = static_cast<int>(mix(k) >> (64 - BITS));
-  const bool grow = __hip_atomic_load(s_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < uint32_t(kSlots / 2);
   for (int p = 0; p < kLProbes; ++p) {
     const unsigned long long cur = __hip_atomic_load(&s_key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (cur == k) return h;
     if (cur == kLEmpty) {
-      if (!grow) return -1;
+      // the count is read only by rows about to claim: rows of known keys never touch it
+      if (__hip_atomic_load(s_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= uint32_t(kSlots / 2))
+        return -1;
       const unsigned long long old = atomicCAS(&s_key[h], kLEmpty, static_cast<unsigned long long>(k));
       if (old == kLEmpty) {
         *claimed = true;
@@ -219,19 +220,21 @@ __global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, Slot 
   __syncthreads();
   bool over = false;
   for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += uint64_t(gridDim.x) * kGChunk) {
-    if (max_run != kNoLimit) {  // leave early once any block has overflowed (one read per wave)
+    uint64_t k[kGRows];
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+      k[j] = r < n ? a.bits(r) : 0;
+    }
+    if (max_run != kNoLimit) {
+      // leave early once any block has overflowed (one read per wave, issued after the
+      // chunk's loads so that its round trip overlaps theirs)
       uint32_t o = 0;
       if (lane_id() == 0) o = __hip_atomic_load(overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (__shfl(o, 0) != 0) {
         over = true;
         break;
       }
-    }
-    uint64_t k[kGRows];
-#pragma unroll
-    for (int j = 0; j < kGRows; ++j) {
-      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
-      k[j] = r < n ? a.bits(r) : 0;
     }
 #pragma unroll
     for (int j = 0; j < kGRows; ++j) {
