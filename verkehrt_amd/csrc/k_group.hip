@@ -321,14 +321,23 @@ __global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const Sl
   for (int i = t; i < kLSlots; i += kGBlock) s_key[i] = kLEmpty;
   if (t == 0) s_cnt = 0;
   __syncthreads();
-  for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += uint64_t(gridDim.x) * kGChunk) {
+  // the next chunk's keys load while this chunk is resolved and written
+  const uint64_t stride = uint64_t(gridDim.x) * kGChunk;
+  uint64_t kn[kGRows];
+  auto load = [&](uint64_t c) {
+#pragma unroll
+    for (int j = 0; j < kGRows; ++j) {
+      const uint64_t r = c + uint64_t(j) * kGBlock + t;
+      kn[j] = r < n ? a.bits(r) : 0;
+    }
+  };
+  if (blockIdx.x * kGChunk < n) load(blockIdx.x * kGChunk);
+  for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += stride) {
     uint64_t k[kGRows];
     int h[kGRows];
 #pragma unroll
-    for (int j = 0; j < kGRows; ++j) {
-      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
-      k[j] = r < n ? a.bits(r) : 0;
-    }
+    for (int j = 0; j < kGRows; ++j) k[j] = kn[j];
+    if (c0 + stride < n) load(c0 + stride);
 #pragma unroll
     for (int j = 0; j < kGRows; ++j) {
       const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
