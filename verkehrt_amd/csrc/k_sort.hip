@@ -542,7 +542,7 @@ void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_
   constexpr int P = sizeof(U);
   uint32_t *hist = static_cast<uint32_t *>(tmp_alloc(c, P * kRadix * 4));
   fill_u64(c, reinterpret_cast<uint64_t *>(hist), P * kRadix / 2, 0);
-  k_hist<U><<<stream_grid(n, kBlock * 8, 1024), kBlock, 0, c.stream>>>(type, keys, n, hist);
+  k_hist<U><<<stream_grid(n, kBlock * 8, 4096), kBlock, 0, c.stream>>>(type, keys, n, hist);
   check_hip(hipGetLastError(), "radix histogram");
   std::vector<uint32_t> h(P * kRadix);
   check_hip(hipMemcpyAsync(h.data(), hist, P * kRadix * 4, hipMemcpyDeviceToHost, c.stream), "hist");
