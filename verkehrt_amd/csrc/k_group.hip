@@ -123,9 +123,12 @@ __global__ void k_ids(const uint32_t *__restrict__ slot, const uint32_t *__restr
 //     update hits the same line. The key equal to the empty marker has a slot of its own past
 //     the table's end. The table starts at kSmallSlots slots; a probe run longer than kMaxRun
 //     there marks overflow, and the pass is redone on 2n slots;
-//  2. every used slot flags its least row (k_first_flags);
-//  3. the flagged rows in row order (the selection kernel) are the keys' first rows;
-//  4. the slot of the j-th first row's key gets id j (k_number1);
+//  2-4. on the small table (at most kSmallSlots slots, so at most that many keys) the used
+//     slots list themselves with their least rows (k_used_slots), the list is sorted by row,
+//     and the slot at place j gets id j (k_number_slots). On the full table every used slot
+//     flags its least row (k_first_flags), the flagged rows in row order (the selection
+//     kernel) are the keys' first rows, and the slot of the j-th first row's key gets id j
+//     (k_number1);
 //  5. every row maps its key to its id through the LDS table again, each key new to the block
 //     resolved once from the global table, and writes it (k_ids1); with more than kLdsGroups
 //     keys, where LDS tables seldom hit, every row searches the global table (k_ids2).
@@ -288,6 +291,67 @@ __global__ void k_number1(KeyCol a, const int64_t *__restrict__ firsts, uint64_t
   for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < groups;
        j += uint64_t(gridDim.x) * blockDim.x)
     table[slot_of_key(table, mask, shift, a.bits(static_cast<uint64_t>(firsts[j])))].id = static_cast<uint32_t>(j);
+}
+
+// Every used slot of a table of slots + 1 entries lists its least row and its index, in no
+// particular order; *count ends at the number of keys. One atomic per block places its list.
+constexpr int kUsedPer = 16;
+__global__ __launch_bounds__(kBlock) void k_used_slots(const Slot *__restrict__ table, uint64_t slots,
+                                                       uint32_t *rows, uint64_t *where,
+                                                       unsigned long long *count) {
+  __shared__ uint32_t s_wave[kBlock / kWave];
+  __shared__ unsigned long long s_base;
+  const uint64_t base = blockIdx.x * uint64_t(kBlock * kUsedPer) + threadIdx.x;
+  uint32_t row[kUsedPer];
+  uint32_t used = 0, cnt = 0;
+#pragma unroll
+  for (int i = 0; i < kUsedPer; ++i) {
+    const uint64_t h = base + uint64_t(i) * kBlock;
+    row[i] = 0xFFFFFFFFu;
+    if (h <= slots) {
+      const Slot e = table[h];
+      if (h < slots ? e.key != kLEmpty : e.row != 0xFFFFFFFFu) row[i] = e.row;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kUsedPer; ++i) {
+    const uint32_t u = row[i] != 0xFFFFFFFFu;
+    used |= u << i;
+    cnt += u;
+  }
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  uint32_t inc = cnt;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d);
+    if (lane >= d) inc += o;
+  }
+  if (lane == kWave - 1) s_wave[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      const uint32_t x = s_wave[w];
+      s_wave[w] = t;
+      t += x;
+    }
+    s_base = t ? atomicAdd(count, static_cast<unsigned long long>(t)) : 0;
+  }
+  __syncthreads();
+  uint64_t pos = s_base + s_wave[wave] + inc - cnt;
+#pragma unroll
+  for (int i = 0; i < kUsedPer; ++i)
+    if (used >> i & 1) {
+      rows[pos] = row[i];
+      where[pos] = base + uint64_t(i) * kBlock;
+      ++pos;
+    }
+}
+
+// The slot of the j-th least first row gets id j.
+__global__ void k_number_slots(const uint64_t *__restrict__ where, uint64_t groups, Slot *table) {
+  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < groups;
+       j += uint64_t(gridDim.x) * blockDim.x)
+    table[where[j]].id = static_cast<uint32_t>(j);
 }
 
 __device__ __forceinline__ uint32_t global_id(const Slot *table, uint64_t mask, int shift, uint64_t k) {
@@ -682,25 +746,45 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
   }
   const uint64_t mask = slots - 1;
   const int shift = 64 - lg;
-  // first rows flagged from the table, then listed in row order
-  auto *first = static_cast<uint8_t *>(tmp_alloc(c, (n + 7) / 8 * 8));
-  fill_u64(c, reinterpret_cast<uint64_t *>(first), (n + 7) / 8, 0);
-  k_first_flags<<<stream_grid(slots + 1, kBlock * 4, 8192), kBlock, 0, c.stream>>>(table, slots, first);
-  auto *firsts = static_cast<int64_t *>(tmp_alloc(c, n * 8));
   uint64_t *d_cnt = c.scratch.slots + 21;
-  SelectSpec ss{2, VK_I64, 0, 0, 0};
-  select_rows(c, ss, nullptr, first, n, firsts, d_cnt);
   uint64_t groups = 0;
-  read_back(c, d_cnt, 8, &groups);
-  k_number1<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(ka, firsts, groups, table, mask, shift);
+  if (slots <= kSmallSlots) {
+    // the used slots listed with their least rows, then numbered in row order
+    auto *rows = static_cast<uint32_t *>(tmp_alloc(c, (slots + 1) * 4));
+    auto *where = static_cast<uint64_t *>(tmp_alloc(c, (slots + 1) * 8));
+    fill_u64(c, d_cnt, 1, 0);
+    constexpr uint64_t kPerBlock = uint64_t(kBlock) * kUsedPer;
+    k_used_slots<<<static_cast<uint32_t>((slots + kPerBlock) / kPerBlock), kBlock, 0, c.stream>>>(
+        table, slots, rows, where, reinterpret_cast<unsigned long long *>(d_cnt));
+    check_hip(hipGetLastError(), "group_id slots");
+    read_back(c, d_cnt, 8, &groups);
+    auto *sorted_rows = tmp_alloc(c, groups * 4);
+    auto *sorted_where = static_cast<uint64_t *>(tmp_alloc(c, groups * 8));
+    sort_key_values(c, false, rows, where, groups, sorted_rows, sorted_where);
+    k_number_slots<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(sorted_where, groups, table);
+    tmp_free(c, rows);
+    tmp_free(c, where);
+    tmp_free(c, sorted_rows);
+    tmp_free(c, sorted_where);
+  } else {
+    // first rows flagged from the table, then listed in row order
+    auto *first = static_cast<uint8_t *>(tmp_alloc(c, (n + 7) / 8 * 8));
+    fill_u64(c, reinterpret_cast<uint64_t *>(first), (n + 7) / 8, 0);
+    k_first_flags<<<stream_grid(slots + 1, kBlock * 4, 8192), kBlock, 0, c.stream>>>(table, slots, first);
+    auto *firsts = static_cast<int64_t *>(tmp_alloc(c, n * 8));
+    SelectSpec ss{2, VK_I64, 0, 0, 0};
+    select_rows(c, ss, nullptr, first, n, firsts, d_cnt);
+    read_back(c, d_cnt, 8, &groups);
+    k_number1<<<stream_grid(groups, kBlock, 8192), kBlock, 0, c.stream>>>(ka, firsts, groups, table, mask, shift);
+    tmp_free(c, first);
+    tmp_free(c, firsts);
+  }
   if (groups <= kLdsGroups)
     k_ids1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, mask, shift, out);
   else
     k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, table, mask, shift, out);
   check_hip(hipGetLastError(), "group_id ids");
   tmp_free(c, table);
-  tmp_free(c, first);
-  tmp_free(c, firsts);
 }
 
 }  // namespace vk
