@@ -421,6 +421,31 @@ def test_histogram_and_reduce_into(D):
                 assert same_array(O.reduce_into(k, f, nb, op), got)
 
 
+def test_reduce_into_add_f64_bad_keys(D):
+    # the f64 add checks its keys inside the sort's counting pass (rows of whole 16-row blocks)
+    # and on its own for the tail rows: the first bad row is named wherever it falls, for
+    # u32 chain keys (few buckets) and u64 chain keys (more than 2^28 buckets)
+    rng = np.random.default_rng(31)
+    cases = [(4, 3, {1: 7}), (40, 3, {5: 3}), (40, 3, {35: -1}), (40, 3, {35: 9, 7: -4}),
+             (100_000, 50, {99_999: 50}), (100_000, 50, {70_001: -1, 123: 1 << 40}),
+             (100_000, 70_000, {16: 70_000, 99_990: 80_000}), (33, 0, {}), (37, 1 << 29, {20: -2, 34: 1 << 29})]
+    for n, nb, bad in cases:
+        k = rng.integers(0, max(nb, 1), n).astype(np.int64)
+        for r, key in bad.items():
+            k[r] = key
+        v = floats(rng, n, False)
+        want = err_of(lambda: O.reduce_into(k, v, nb))
+        assert want[0] == "err"
+        got = err_of(lambda: D.call("reduce_into_add_f64", [D.array(k), D.array(v), S("i64", nb)]))
+        assert got == want, (n, nb, got, want)
+    # the context stays usable, with the same keys made good
+    n, nb = 100_000, 50
+    k = rng.integers(0, nb, n).astype(np.int64)
+    v = floats(rng, n, False)
+    got = D.call("reduce_into_add_f64", [D.array(k), D.array(v), S("i64", nb)]).to_numpy()
+    assert same_array(O.reduce_into(k, v, nb), got)
+
+
 def test_reduce_into_add_f64_defined_order(D):
     rng = np.random.default_rng(19)
     for n in LENGTHS + [100_000]:

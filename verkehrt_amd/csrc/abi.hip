@@ -634,7 +634,7 @@ vk_value run_whole(Ctx &c, const Def &d, const vk_arg *args, size_t nargs) {
       uint64_t *d_bad = c.scratch.slots + 18;
       uint64_t bad;
       const int op = hist ? 0 : d.p0;
-      if (op == 4 || (!hist && keys->len != vals->len)) {
+      if (!hist && keys->len != vals->len) {
         check_keys(c, static_cast<const int64_t *>(dptr(keys)), keys->len, uint64_t(buckets), d_bad);
         read_back(c, d_bad, 8, &bad);
         if (bad != ~uint64_t(0)) bucket_error(c, keys, bad, buckets);
@@ -644,9 +644,14 @@ vk_value run_whole(Ctx &c, const Def &d, const vk_arg *args, size_t nargs) {
       }
       vk_array *out = alloc_result(c, d.out_t, uint64_t(buckets));
       if (op == 4) {
-        reduce_into_add_f64(c, static_cast<const int64_t *>(dptr(keys)),
-                            static_cast<const double *>(dptr(vals)), keys->len, uint64_t(buckets),
-                            static_cast<double *>(dptr(out)));
+        // the keys are checked by the sort inside, before anything is folded
+        bad = reduce_into_add_f64(c, static_cast<const int64_t *>(dptr(keys)),
+                                  static_cast<const double *>(dptr(vals)), keys->len, uint64_t(buckets),
+                                  static_cast<double *>(dptr(out)));
+        if (bad != ~uint64_t(0)) {
+          release(out);
+          bucket_error(c, keys, bad, buckets);
+        }
       } else {
         bucket_init(c, op, d.out_t, dptr(out), uint64_t(buckets));
         bucket_rows(c, op, d.out_t, static_cast<const int64_t *>(dptr(keys)), vals ? dptr(vals) : nullptr,

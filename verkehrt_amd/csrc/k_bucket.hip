@@ -397,7 +397,7 @@ void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint6
   check_hip(hipGetLastError(), "check keys");
 }
 
-// Keys are already checked to lie in 0..buckets.
+// The sort checks the keys of rows 0..b before anything folds.
 namespace {
 
 // The chains of rows 0..b (bucket, i % 16): a stable sort of (chain key, value) pairs puts each
@@ -414,13 +414,19 @@ constexpr uint64_t kBucketFoldMin = uint64_t(1) << 16;
 
 template <typename K>
 bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
-                 double *stripes, double *out) {
+                 double *stripes, double *out, uint64_t *bad_row) {
   constexpr bool wide = sizeof(K) == 8;
   K *sk = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
   double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
-  // the sort computes each row's chain key from its bucket key as it reads it
+  // the sort computes each row's chain key from its bucket key as it reads it, and checks the
+  // bucket keys in its counting pass
   sort_chain_values(c, wide, keys, reinterpret_cast<const uint64_t *>(vals), b, sk,
-                    reinterpret_cast<uint64_t *>(sv));
+                    reinterpret_cast<uint64_t *>(sv), buckets, bad_row);
+  if (*bad_row != ~uint64_t(0)) {
+    tmp_free(c, sk);
+    tmp_free(c, sv);
+    return false;
+  }
   const bool by_bucket = buckets >= kBucketFoldMin;
   // the starts of chains (stripes) or buckets (by_bucket), compacted in order
   uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
@@ -592,10 +598,19 @@ void reduce_into_add_f64_batch(Ctx &c, std::vector<BatchItem> &items) {
   tmp_free(c, stripes);
 }
 
-void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
-                         uint64_t buckets, double *out) {
-  if (buckets == 0) return;
+uint64_t reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
+                             uint64_t buckets, double *out) {
+  if (buckets == 0) return n == 0 ? ~uint64_t(0) : 0;  // every key is outside an empty range
+  uint64_t bad = ~uint64_t(0);
   const uint64_t b = n - n % 16;
+  if (n > b) {
+    // the tail rows (after the last whole block of 16) are not sorted: checked on their own
+    uint64_t *d_bad = c.scratch.slots + 25;
+    check_keys(c, keys + b, n - b, buckets, d_bad);
+    read_back(c, d_bad, 8, &bad);
+    if (bad != ~uint64_t(0)) bad += b;
+  }
+  if (bad != ~uint64_t(0) && b == 0) return bad;
   // the stripe buffer (16 values per bucket) serves fewer than kBucketFoldMin buckets
   const bool stripe_form = buckets < kBucketFoldMin;
   double *stripes = nullptr;
@@ -605,8 +620,16 @@ void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64
   }
   bool combined = false;
   if (b > 0) {
-    if (buckets * 16 <= 0xFFFFFFFFull) combined = fold_chains<uint32_t>(c, keys, vals, b, buckets, stripes, out);
-    else combined = fold_chains<uint64_t>(c, keys, vals, b, buckets, stripes, out);
+    uint64_t bad_sorted = ~uint64_t(0);
+    if (buckets * 16 <= 0xFFFFFFFFull)
+      combined = fold_chains<uint32_t>(c, keys, vals, b, buckets, stripes, out, &bad_sorted);
+    else
+      combined = fold_chains<uint64_t>(c, keys, vals, b, buckets, stripes, out, &bad_sorted);
+    if (bad_sorted != ~uint64_t(0)) bad = bad_sorted;  // rows below b come first
+  }
+  if (bad != ~uint64_t(0)) {
+    if (stripes) tmp_free(c, stripes);
+    return bad;
   }
   if (stripe_form)
     k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(stripes, buckets, out);
@@ -615,6 +638,7 @@ void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64
   if (n > b) k_fold_tail<<<1, 64, 0, c.stream>>>(keys, vals, b, n, out);
   check_hip(hipGetLastError(), "f64 reduce_into");
   if (stripes) tmp_free(c, stripes);
+  return bad;
 }
 
 }  // namespace vk

@@ -69,13 +69,17 @@ __device__ __forceinline__ void hist_add(uint32_t (&s_h)[sizeof(U)][kRadix], U k
 
 constexpr int kHistAhead = 8;
 
-template <typename U>
+// CHECK (chain keys only): the bucket keys are also checked against 0..limit, the least row
+// holding one outside lowering *bad, so the caller can refuse the input before any pass runs.
+template <typename U, bool CHECK = false>
 __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys, uint64_t n,
-                                                 uint32_t *hist) {
+                                                 uint32_t *hist, uint64_t limit = 0,
+                                                 uint64_t *bad = nullptr) {
   constexpr int P = sizeof(U);
   __shared__ uint32_t s_h[P][kRadix];
   for (int k = threadIdx.x; k < P * kRadix; k += kBlock) (&s_h[0][0])[k] = 0;
   __syncthreads();
+  uint64_t my_bad = ~uint64_t(0);
   // kHistAhead rows per thread loaded before their ballots (which are convergent: a load in the
   // same iteration as a ballot would wait for the previous row's)
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
@@ -84,11 +88,26 @@ __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys,
 #pragma unroll
     for (int u = 0; u < kHistAhead; ++u) {
       const uint64_t r = r0 + u * stride;
-      k[u] = r < n ? image<U>(type, keys, r) : U(0);
+      if (CHECK) {
+        const uint64_t raw = r < n ? static_cast<const uint64_t *>(keys)[r] : 0;
+        k[u] = U(raw * 16 + (r & 15));
+        if (r < n && raw >= limit && r < my_bad) my_bad = r;
+      } else {
+        k[u] = r < n ? image<U>(type, keys, r) : U(0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < kHistAhead; ++u)
       if (r0 + u * stride < n) hist_add<U>(s_h, k[u]);
+  }
+  if (CHECK) {
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) {
+      const uint64_t o = __shfl_xor(my_bad, m);
+      my_bad = o < my_bad ? o : my_bad;
+    }
+    if (lane_id() == 0 && my_bad != ~uint64_t(0))
+      atomicMin(reinterpret_cast<unsigned long long *>(bad), static_cast<unsigned long long>(my_bad));
   }
   __syncthreads();
   for (int k = threadIdx.x; k < P * kRadix; k += kBlock) {
@@ -535,18 +554,36 @@ void radix_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, U *sorted_
 // the value bits of `vals` that go with them to sorted_keys / sorted_vals.
 template <typename U, typename V>
 void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_t n, int64_t *out_pos,
-           U *sorted_keys, V *sorted_vals) {
+           U *sorted_keys, V *sorted_vals, uint64_t limit = 0, uint64_t *bad_row = nullptr) {
   constexpr bool POS = std::is_same<V, uint32_t>::value;
   if (n == 0) return;
   if (n > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
   constexpr int P = sizeof(U);
   uint32_t *hist = static_cast<uint32_t *>(tmp_alloc(c, P * kRadix * 4));
   fill_u64(c, reinterpret_cast<uint64_t *>(hist), P * kRadix / 2, 0);
-  k_hist<U><<<stream_grid(n, kBlock * 8, 4096), kBlock, 0, c.stream>>>(type, keys, n, hist);
+  // with bad_row (chain keys): the least row whose bucket key is outside 0..limit, or ~0; the
+  // passes run only when there is none
+  uint64_t *d_bad = c.scratch.slots + 24;
+  const uint32_t hg = stream_grid(n, kBlock * 8, 4096);
+  if (bad_row) {
+    fill_u64(c, d_bad, 1, ~uint64_t(0));
+    k_hist<U, true><<<hg, kBlock, 0, c.stream>>>(type, keys, n, hist, limit, d_bad);
+  } else {
+    k_hist<U><<<hg, kBlock, 0, c.stream>>>(type, keys, n, hist);
+  }
   check_hip(hipGetLastError(), "radix histogram");
   std::vector<uint32_t> h(P * kRadix);
+  uint64_t bad = ~uint64_t(0);
   check_hip(hipMemcpyAsync(h.data(), hist, P * kRadix * 4, hipMemcpyDeviceToHost, c.stream), "hist");
+  if (bad_row) check_hip(hipMemcpyAsync(&bad, d_bad, 8, hipMemcpyDeviceToHost, c.stream), "bad row");
   check_hip(hipStreamSynchronize(c.stream), "sync");
+  if (bad_row) {
+    *bad_row = bad;
+    if (bad != ~uint64_t(0)) {
+      tmp_free(c, hist);
+      return;
+    }
+  }
   // exclusive offsets per digit; passes whose digit is the same in every row are skipped
   std::vector<int> active;
   std::vector<uint32_t> base(P * kRadix);
@@ -639,11 +676,13 @@ void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, 
 }
 
 void sort_chain_values(Ctx &c, bool wide, const int64_t *keys, const uint64_t *vals, uint64_t n,
-                       void *sorted_keys, uint64_t *sorted_vals) {
+                       void *sorted_keys, uint64_t *sorted_vals, uint64_t buckets, uint64_t *bad_row) {
   if (wide)
-    radix<uint64_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint64_t *>(sorted_keys), sorted_vals);
+    radix<uint64_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint64_t *>(sorted_keys), sorted_vals,
+                              buckets, bad_row);
   else
-    radix<uint32_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint32_t *>(sorted_keys), sorted_vals);
+    radix<uint32_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint32_t *>(sorted_keys), sorted_vals,
+                              buckets, bad_row);
 }
 
 void sort_chain_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,

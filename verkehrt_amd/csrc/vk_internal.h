@@ -306,8 +306,10 @@ void gather_rows(Ctx &c, uint32_t width, const void *xs, uint64_t xlen, const in
 void bucket_init(Ctx &c, int op, int32_t type, void *out, uint64_t buckets);
 void bucket_rows(Ctx &c, int op, int32_t type, const int64_t *keys, const void *vals, uint64_t n,
                  uint64_t buckets, void *out, uint64_t *d_bad);
-void reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
-                         uint64_t buckets, double *out);
+// reduce_into_add_f64 checks its keys itself (inside the sort's counting pass): it returns the
+// first row whose key is outside 0..buckets, or ~0, and writes out only when there is none.
+uint64_t reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, uint64_t n,
+                             uint64_t buckets, double *out);
 void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint64_t *d_bad);
 
 // Argsort (k_sort.hip): stable ascending positions as i64.
@@ -319,10 +321,12 @@ void sort_u64_pairs(Ctx &c, const uint64_t *keys, uint64_t n, uint64_t *sorted_k
 // Stable sort of (key, 8-byte value) pairs by key: keys are u64 (wide) or u32.
 void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, uint64_t n,
                      void *sorted_keys, uint64_t *sorted_vals);
-// The same for chain keys computed on the fly: row r of bucket keys[r] (already checked to lie
-// in 0..buckets) has chain key keys[r] * 16 + r % 16 (u32 unless wide).
+// The same for chain keys computed on the fly: row r of bucket keys[r] has chain key
+// keys[r] * 16 + r % 16 (u32 unless wide). The counting pass also checks the bucket keys:
+// *bad_row is set to the first row whose key is outside 0..buckets (or ~0), and nothing is
+// sorted when there is one.
 void sort_chain_values(Ctx &c, bool wide, const int64_t *keys, const uint64_t *vals, uint64_t n,
-                       void *sorted_keys, uint64_t *sorted_vals);
+                       void *sorted_keys, uint64_t *sorted_vals, uint64_t buckets, uint64_t *bad_row);
 // Batched: item i sorts its keys a with the values b (n rows) into sorted_keys / sorted_vals
 // from row aux (set here) of the concatenated rows.
 void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
