@@ -100,15 +100,31 @@ __global__ void k_check_keys(const int64_t *__restrict__ keys, uint64_t n, uint6
 
 // ----------------------------------------------------------- f64 reduce_into add
 
-// One wave per chain start for long chains: the lanes load 64 values at a time, the wave folds
-// them in order with constant-lane reads (readlane, no LDS round trip) while the next chunks
-// load. Every lane holds the same running value.
+// The first row of every chain key 0..nchains - 1 in the sorted chain keys (its lower bound,
+// by binary search), empty chains included: the chain starts without a pass over the rows.
+template <typename K>
+__global__ void k_chain_bounds(const K *__restrict__ sk, uint64_t b, uint64_t nchains, uint64_t *starts) {
+  for (uint64_t ch = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; ch < nchains;
+       ch += uint64_t(gridDim.x) * blockDim.x) {
+    const K key = static_cast<K>(ch);
+    uint64_t lo = 0, hi = b;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (sk[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    starts[ch] = lo;
+  }
+}
+
+// One wave per chain for long chains (starts[ch] is chain key ch's first row; the last chain
+// ends at b): the lanes load 64 values at a time, the wave folds them in order with
+// constant-lane reads (readlane, no LDS round trip) while the next chunks load. Every lane
+// holds the same running value. An empty chain leaves its stripe at 0.0.
 constexpr int kFoldWaves = 4;  // chains per block of the wave fold
 
-template <typename K>
 __global__ __launch_bounds__(kFoldWaves * kWave) void k_chain_fold_wave(const uint64_t *__restrict__ starts,
                                                                        uint64_t nchains,
-                                                                       const K *__restrict__ sk,
                                                                        const double *__restrict__ sv,
                                                                        uint64_t b, double *stripes) {
   const int l = lane_id();
@@ -116,7 +132,6 @@ __global__ __launch_bounds__(kFoldWaves * kWave) void k_chain_fold_wave(const ui
   for (uint64_t ch = w0; ch < nchains; ch += uint64_t(gridDim.x) * kFoldWaves) {
     const uint64_t p0 = starts[ch];
     const uint64_t p1 = ch + 1 < nchains ? starts[ch + 1] : b;
-    const uint64_t k = sk[p0];
     double a = 0.0;
     // kAhead chunks of 64 values in flight: chunk d of each round is folded, then refilled
     // with the chunk kAhead further on
@@ -143,7 +158,7 @@ __global__ __launch_bounds__(kFoldWaves * kWave) void k_chain_fold_wave(const ui
         v[d] = r < p1 ? sv[r] : 0.0;
       }
     }
-    if (l == 0) stripes[k] = a;
+    if (l == 0 && p1 > p0) stripes[ch] = a;
   }
 }
 
@@ -256,12 +271,12 @@ __global__ void k_key_starts(const K *__restrict__ sk, uint64_t b, uint8_t *flag
 // issued before the first is stored; rows padded to 65 values so that a lane's row walk
 // spreads over the banks); lane l then folds its own chain's 64 values in index order. Two
 // LDS buffers let the loads of round q + 1 fly while round q is folded. A chain shorter than
-// the round's window adds nothing past its end.
+// the round's window adds nothing past its end. Chains are indexed by chain key, as in the
+// wave fold; an empty chain leaves its stripe at 0.0.
 constexpr int kLaneFoldPad = 65;
 
-template <typename K>
 __global__ __launch_bounds__(kWave) void k_chain_fold_lanes(const uint64_t *__restrict__ starts,
-                                                             uint64_t nchains, const K *__restrict__ sk,
+                                                             uint64_t nchains,
                                                              const double *__restrict__ sv, uint64_t b,
                                                              double *stripes) {
   __shared__ double s_v[2][kWave * kLaneFoldPad];
@@ -315,7 +330,7 @@ __global__ __launch_bounds__(kWave) void k_chain_fold_lanes(const uint64_t *__re
       __syncthreads();
       buf ^= 1;
     }
-    if (live) stripes[sk[p0]] = a;
+    if (live && p1 > p0) stripes[ch] = a;
     __syncthreads();
   }
 }
@@ -402,8 +417,10 @@ namespace {
 
 // The chains of rows 0..b (bucket, i % 16): a stable sort of (chain key, value) pairs puts each
 // chain's values together in index order, and a bucket's chains together in stripe order.
-// Below kBucketFoldMin buckets the chains fold into the stripes, which then combine per
-// bucket: one lane per chain (k_chain_fold_lanes) unless there are fewer than kLaneFoldChains
+// Below kBucketFoldMin buckets every chain key's first row comes from a binary search of the
+// sorted keys (k_chain_bounds, no pass over the rows), and the chains fold into the stripes,
+// which then combine per bucket: one lane per chain (k_chain_fold_lanes) unless there are
+// fewer than kLaneFoldChains
 // chains averaging 256 rows or more, which fold one wave per chain (every lane of a wave on
 // one chain, for few long chains that need all the parallel hardware they can get). From
 // kBucketFoldMin buckets on, the buckets alone give the parallelism: one lane per bucket
@@ -428,31 +445,37 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
     return false;
   }
   const bool by_bucket = buckets >= kBucketFoldMin;
-  // the starts of chains (stripes) or buckets (by_bucket), compacted in order
-  uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
-  uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
-  const uint32_t gq = stream_grid((b + 3) / 4, kBlock * 4, 8192);
-  if (by_bucket) k_key_starts<4, K><<<gq, kBlock, 0, c.stream>>>(sk, b, flag);
-  else k_key_starts<0, K><<<gq, kBlock, 0, c.stream>>>(sk, b, flag);
-  SelectSpec ss{2, VK_I64, 0, 0, 0};
-  uint64_t *d_cnt = c.scratch.slots + 20;
-  select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
-  uint64_t nstarts = 0;
-  read_back(c, d_cnt, 8, &nstarts);
   if (by_bucket) {
+    // the bucket starts, compacted in order
+    uint8_t *flag = static_cast<uint8_t *>(tmp_alloc(c, b));
+    uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, b * 8));
+    const uint32_t gq = stream_grid((b + 3) / 4, kBlock * 4, 8192);
+    k_key_starts<4, K><<<gq, kBlock, 0, c.stream>>>(sk, b, flag);
+    SelectSpec ss{2, VK_I64, 0, 0, 0};
+    uint64_t *d_cnt = c.scratch.slots + 20;
+    select_rows(c, ss, nullptr, flag, b, starts, d_cnt);
+    uint64_t nstarts = 0;
+    read_back(c, d_cnt, 8, &nstarts);
     k_fill<double><<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(out, buckets, 0.0);
     k_bucket_fold<K><<<stream_grid(nstarts, kWave, 65535), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
                                                                                   out);
-  } else if (nstarts >= kLaneFoldChains || b / nstarts < 256) {
-    k_chain_fold_lanes<K><<<stream_grid(nstarts, kWave, 16384), kWave, 0, c.stream>>>(starts, nstarts, sk, sv, b,
-                                                                                      stripes);
+    check_hip(hipGetLastError(), "bucket fold");
+    tmp_free(c, flag);
+    tmp_free(c, starts);
   } else {
-    k_chain_fold_wave<K><<<stream_grid(nstarts, kFoldWaves, 16384), kFoldWaves * kWave, 0, c.stream>>>(
-        starts, nstarts, sk, sv, b, stripes);
+    // every chain key's first row (at most 16 x 64K chains), then the chains fold into the
+    // stripes
+    const uint64_t nchains = buckets * 16;
+    uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, nchains * 8));
+    k_chain_bounds<K><<<stream_grid(nchains, kBlock, 8192), kBlock, 0, c.stream>>>(sk, b, nchains, starts);
+    if (nchains >= kLaneFoldChains || b / nchains < 256)
+      k_chain_fold_lanes<<<stream_grid(nchains, kWave, 16384), kWave, 0, c.stream>>>(starts, nchains, sv, b, stripes);
+    else
+      k_chain_fold_wave<<<stream_grid(nchains, kFoldWaves, 16384), kFoldWaves * kWave, 0, c.stream>>>(
+          starts, nchains, sv, b, stripes);
+    check_hip(hipGetLastError(), "chain fold");
+    tmp_free(c, starts);
   }
-  check_hip(hipGetLastError(), "chain fold");
-  tmp_free(c, flag);
-  tmp_free(c, starts);
   tmp_free(c, sk);
   tmp_free(c, sv);
   return by_bucket;
