@@ -46,6 +46,22 @@ __device__ __forceinline__ U image(int32_t type, const void *keys, uint64_t r) {
   }
 }
 
+// Calls f(img) with the key-image function of `type` fixed, so that a loop inside f sees one
+// load form and can issue all its loads together (a switch on the type per row would sit
+// between them).
+template <typename U, typename F>
+__device__ __forceinline__ void with_image(int32_t type, const void *keys, F &&f) {
+  switch (type) {
+    case VK_I64: f([&](uint64_t r) { return image<U>(VK_I64, keys, r); }); break;
+    case VK_F64: f([&](uint64_t r) { return image<U>(VK_F64, keys, r); }); break;
+    case VK_I32: f([&](uint64_t r) { return image<U>(VK_I32, keys, r); }); break;
+    case VK_U8: f([&](uint64_t r) { return image<U>(VK_U8, keys, r); }); break;
+    case kRawU32: f([&](uint64_t r) { return image<U>(kRawU32, keys, r); }); break;
+    case kChainKey: f([&](uint64_t r) { return image<U>(kChainKey, keys, r); }); break;
+    default: f([&](uint64_t r) { return image<U>(VK_U64, keys, r); }); break;
+  }
+}
+
 // ------------------------------------------------------------- counting pass
 
 // Adds one row's digits to the block's LDS histograms. A digit that every active lane of the
@@ -85,16 +101,22 @@ __global__ __launch_bounds__(kBlock) void k_hist(int32_t type, const void *keys,
   const uint64_t stride = uint64_t(gridDim.x) * kBlock;
   for (uint64_t r0 = blockIdx.x * uint64_t(kBlock) + threadIdx.x; r0 < n; r0 += stride * kHistAhead) {
     U k[kHistAhead];
+    if (CHECK) {
 #pragma unroll
-    for (int u = 0; u < kHistAhead; ++u) {
-      const uint64_t r = r0 + u * stride;
-      if (CHECK) {
+      for (int u = 0; u < kHistAhead; ++u) {
+        const uint64_t r = r0 + u * stride;
         const uint64_t raw = r < n ? static_cast<const uint64_t *>(keys)[r] : 0;
         k[u] = U(raw * 16 + (r & 15));
         if (r < n && raw >= limit && r < my_bad) my_bad = r;
-      } else {
-        k[u] = r < n ? image<U>(type, keys, r) : U(0);
       }
+    } else {
+      with_image<U>(type, keys, [&](auto img) {
+#pragma unroll
+        for (int u = 0; u < kHistAhead; ++u) {
+          const uint64_t r = r0 + u * stride;
+          k[u] = r < n ? img(r) : U(0);
+        }
+      });
     }
 #pragma unroll
     for (int u = 0; u < kHistAhead; ++u)
@@ -207,18 +229,22 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
   const uint64_t below = (uint64_t(1) << l) - 1;
   // every load of the tile first: the ballots below are convergent, so loads written in the
   // same loop would each wait for the previous item's ballots
+  auto load_tile = [&](auto img) {
 #pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const uint64_t r = wbase + uint64_t(i) * kWave + l;
-    if (r < n) {
-      key[i] = FIRST ? image<U>(type, keys_in, r) : static_cast<const U *>(keys_in)[r];
-      if (FIRST) pos[i] = POS ? static_cast<V>(r) : static_cast<V>(vals[r]);
-      else pos[i] = pos_in[r];
-    } else {
-      key[i] = 0;
-      pos[i] = 0;
+    for (int i = 0; i < kItems; ++i) {
+      const uint64_t r = wbase + uint64_t(i) * kWave + l;
+      if (r < n) {
+        key[i] = img(r);
+        if (FIRST) pos[i] = POS ? static_cast<V>(r) : static_cast<V>(vals[r]);
+        else pos[i] = pos_in[r];
+      } else {
+        key[i] = 0;
+        pos[i] = 0;
+      }
     }
-  }
+  };
+  if (FIRST) with_image<U>(type, keys_in, load_tile);  // the key type fixed outside the loop
+  else load_tile([&](uint64_t r) { return static_cast<const U *>(keys_in)[r]; });
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const uint64_t r = wbase + uint64_t(i) * kWave + l;
@@ -409,11 +435,13 @@ __global__ __launch_bounds__(kBlock) void k_hist_batch(int32_t type, const Batch
   const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
   for (uint64_t r0 = t0 + threadIdx.x; r0 < t1; r0 += uint64_t(kBlock) * kHistAhead) {
     U k[kHistAhead];
+    with_image<U>(type, I.a, [&](auto img) {
 #pragma unroll
-    for (int u = 0; u < kHistAhead; ++u) {
-      const uint64_t r = r0 + uint64_t(u) * kBlock;
-      k[u] = r < t1 ? image<U>(type, I.a, r) : U(0);
-    }
+      for (int u = 0; u < kHistAhead; ++u) {
+        const uint64_t r = r0 + uint64_t(u) * kBlock;
+        k[u] = r < t1 ? img(r) : U(0);
+      }
+    });
 #pragma unroll
     for (int u = 0; u < kHistAhead; ++u)
       if (r0 + uint64_t(u) * kBlock < t1) hist_add<U>(s_h, k[u]);
