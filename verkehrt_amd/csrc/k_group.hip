@@ -29,6 +29,28 @@ struct KeyCol {
       default: return static_cast<const uint8_t *>(p)[r];
     }
   }
+  // Calls f(load) with the load of this column's width fixed, so that a loop inside f issues
+  // all its loads together (a width switch per row would sit between them).
+  template <typename F>
+  __device__ __forceinline__ void with_bits(F &&f) const {
+    switch (type_width_d(type)) {
+      case 8: {
+        const uint64_t *q = static_cast<const uint64_t *>(p);
+        f([q](uint64_t r) -> uint64_t { return q[r]; });
+        break;
+      }
+      case 4: {
+        const uint32_t *q = static_cast<const uint32_t *>(p);
+        f([q](uint64_t r) -> uint64_t { return q[r]; });
+        break;
+      }
+      default: {
+        const uint8_t *q = static_cast<const uint8_t *>(p);
+        f([q](uint64_t r) -> uint64_t { return q[r]; });
+        break;
+      }
+    }
+  }
   __device__ static __forceinline__ int type_width_d(int32_t t) {
     return (t == VK_F64 || t == VK_I64 || t == VK_U64) ? 8 : (t == VK_F32 || t == VK_I32) ? 4 : 1;
   }
@@ -224,11 +246,13 @@ __global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, Slot 
   bool over = false;
   for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += uint64_t(gridDim.x) * kGChunk) {
     uint64_t k[kGRows];
+    a.with_bits([&](auto bits) {
 #pragma unroll
-    for (int j = 0; j < kGRows; ++j) {
-      const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
-      k[j] = r < n ? a.bits(r) : 0;
-    }
+      for (int j = 0; j < kGRows; ++j) {
+        const uint64_t r = c0 + uint64_t(j) * kGBlock + t;
+        k[j] = r < n ? bits(r) : 0;
+      }
+    });
     if (max_run != kNoLimit) {
       // leave early once any block has overflowed (one read per wave, issued after the
       // chunk's loads so that its round trip overlaps theirs)
@@ -389,11 +413,13 @@ __global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const Sl
   const uint64_t stride = uint64_t(gridDim.x) * kGChunk;
   uint64_t kn[kGRows];
   auto load = [&](uint64_t c) {
+    a.with_bits([&](auto bits) {
 #pragma unroll
-    for (int j = 0; j < kGRows; ++j) {
-      const uint64_t r = c + uint64_t(j) * kGBlock + t;
-      kn[j] = r < n ? a.bits(r) : 0;
-    }
+      for (int j = 0; j < kGRows; ++j) {
+        const uint64_t r = c + uint64_t(j) * kGBlock + t;
+        kn[j] = r < n ? bits(r) : 0;
+      }
+    });
   };
   if (blockIdx.x * kGChunk < n) load(blockIdx.x * kGChunk);
   for (uint64_t c0 = blockIdx.x * kGChunk; c0 < n; c0 += stride) {
@@ -494,11 +520,13 @@ __global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const Bat
   if (t == 0) s_cnt = 0;
   __syncthreads();
   uint64_t k[kBRows];
+  a.with_bits([&](auto bits) {
 #pragma unroll
-  for (int j = 0; j < kBRows; ++j) {
-    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-    k[j] = r < R.t1 ? a.bits(r) : 0;
-  }
+    for (int j = 0; j < kBRows; ++j) {
+      const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+      k[j] = r < R.t1 ? bits(r) : 0;
+    }
+  });
 #pragma unroll
   for (int j = 0; j < kBRows; ++j) {
     const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
@@ -541,11 +569,13 @@ __global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const Batc
   __syncthreads();
   uint64_t k[kBRows];
   int h[kBRows];
+  a.with_bits([&](auto bits) {
 #pragma unroll
-  for (int j = 0; j < kBRows; ++j) {
-    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-    k[j] = r < R.t1 ? a.bits(r) : 0;
-  }
+    for (int j = 0; j < kBRows; ++j) {
+      const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
+      k[j] = r < R.t1 ? bits(r) : 0;
+    }
+  });
 #pragma unroll
   for (int j = 0; j < kBRows; ++j) {
     const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
