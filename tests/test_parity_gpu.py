@@ -466,6 +466,13 @@ def test_reduce_into_add_f64_defined_order(D):
     v = floats(rng, n, False)
     got = D.call("reduce_into_add_f64", [view(D, k, 3), view(D, v, 1), S("i64", 100_000)]).to_numpy()
     assert same_array(O.reduce_into(k, v, 100_000), got)
+    # many buckets, nearly all empty (whole waves of empty buckets), rows in the first and last
+    # buckets and a few between; and fewer rows than one 16-row block
+    for n, nb, keys in [(1_003, 200_000, None), (4_099, 131_072, [0, 131_071, 65_536, 7]), (9, 70_000, None)]:
+        k = (rng.integers(0, nb, n) if keys is None else rng.choice(np.array(keys), n)).astype(np.int64)
+        v = floats(rng, n, False)
+        got = D.call("reduce_into_add_f64", [D.array(k), D.array(v), S("i64", nb)]).to_numpy()
+        assert same_array(O.reduce_into(k, v, nb), got), (n, nb)
     # every key equal gives sum's bits
     v = floats(rng, 10_007, False)
     got = D.call("reduce_into_add_f64", [D.array(np.zeros(v.size, np.int64)), D.array(v), S("i64", 1)]).to_numpy()
