@@ -56,7 +56,7 @@ def _declare(L):
         "vko_sum_u64": (u64, [vp, u64]),
         "vko_sum_i32": (i64, [vp, u64]),
         "vko_sum_u8": (u64, [vp, u64]),
-        "vko_product_i64": (i64, [vp, u64]),
+        "vko_product_int": (u64, [vp, C.c_uint32, u64]),
         "vko_count_cmp_f64": (i64, [vp, u64, i32, dbl]),
         "vko_count_cmp_i64": (i64, [vp, u64, i32, i64]),
         "vko_count_bool": (i64, [vp, u64]),
@@ -74,11 +74,13 @@ def _declare(L):
         "vko_scan_by_add_i64": (None, [vp, vp, u64, vp]),
         "vko_scan_extreme_f64": (None, [vp, vp, u64, i32, vp]),
         "vko_scan_extreme_i64": (None, [vp, vp, u64, i32, vp]),
+        "vko_scan_extreme_u64": (None, [vp, vp, u64, i32, vp]),
         "vko_scan_add_f64": (None, [vp, u64, vp]),
         "vko_gather": (i32, [vp, C.c_uint32, u64, vp, u64, vp, vp]),
         "vko_histogram": (i32, [vp, u64, i64, vp, vp]),
         "vko_reduce_into_add_i64": (i32, [vp, vp, u64, i64, vp, vp]),
         "vko_reduce_into_extreme_f64": (i32, [vp, vp, u64, i64, i32, vp, vp]),
+        "vko_reduce_into_extreme_i64": (i32, [vp, vp, u64, i64, i32, vp, vp]),
         "vko_reduce_into_add_f64": (i32, [vp, vp, u64, i64, vp, vp]),
         "vko_argsort_u64": (None, [vp, u64, vp]),
         "vko_argsort_i64": (None, [vp, u64, vp]),
@@ -87,6 +89,10 @@ def _declare(L):
         "vko_argsort_f64": (None, [vp, u64, vp]),
         "vko_group_id_u64": (None, [vp, u64, vp]),
         "vko_group_id_pair_u64": (None, [vp, vp, u64, vp]),
+        "vko_ew_f64": (None, [i32, vp, vp, u64, vp]),
+        "vko_ew_f32": (None, [i32, vp, vp, u64, vp]),
+        "vko_ew_int": (None, [i32, i32, vp, vp, u64, vp]),
+        "vko_cast": (None, [i32, i32, vp, u64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -178,9 +184,13 @@ def sum_int(x) -> int:
     return f(_p(x), x.size)
 
 
-def product_i64(x) -> int:
-    x = _c(x, np.int64)
-    return lib().vko_product_i64(_p(x), x.size)
+def product_int(x) -> int:
+    """reduce_mul over an integer array: the wrapping product in the element type."""
+    x = np.ascontiguousarray(x)
+    bits = lib().vko_product_int(_p(x), x.dtype.itemsize, x.size)
+    if x.dtype.kind == "i" and bits >> (8 * x.dtype.itemsize - 1):
+        return bits - (1 << (8 * x.dtype.itemsize))
+    return bits
 
 
 def count_cmp(x, cmp: str, c) -> int:
@@ -286,6 +296,8 @@ def scan_extreme(x, want_max: bool, starts=None) -> np.ndarray:
     out = np.empty(x.size, x.dtype)
     if x.dtype == np.float64:
         lib().vko_scan_extreme_f64(_p(x), _p(s), x.size, int(want_max), _p(out))
+    elif x.dtype == np.uint64:
+        lib().vko_scan_extreme_u64(_p(x), _p(s), x.size, int(want_max), _p(out))
     else:
         lib().vko_scan_extreme_i64(_p(_c(x, np.int64)), _p(s), x.size, int(want_max), _p(out))
     return out
@@ -345,6 +357,9 @@ def reduce_into(keys, vals, buckets: int, op: str = "add") -> np.ndarray:
     elif vals.dtype == np.float64 and op in ("min", "max"):
         r = lib().vko_reduce_into_extreme_f64(_p(keys), _p(vals), keys.size, buckets,
                                               int(op == "max"), _p(out), _p(bad))
+    elif vals.dtype == np.int64 and op in ("min", "max"):
+        r = lib().vko_reduce_into_extreme_i64(_p(keys), _p(vals), keys.size, buckets,
+                                              int(op == "max"), _p(out), _p(bad))
     else:
         raise NotImplementedError(f"reduce_into {op} on {vals.dtype}")
     if r:
@@ -385,6 +400,69 @@ def group_id(x, y=None) -> np.ndarray:
         if a.size != b.size:
             raise OracleError(f"length mismatch: {a.size} vs {b.size}")
         lib().vko_group_id_pair_u64(_p(a), _p(b), a.size, _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- elementwise
+
+TYPE_CODE = {"f64": 0, "f32": 1, "i64": 2, "u64": 3, "i32": 4, "u8": 5, "bool": 6}
+NP = {"f64": np.float64, "f32": np.float32, "i64": np.int64, "u64": np.uint64, "i32": np.int32,
+      "u8": np.uint8, "bool": np.uint8}
+FLOAT_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "min": 4, "max": 5, "neg": 6, "abs": 7, "sqrt": 8}
+INT_OPS = {"add": 0, "sub": 1, "mul": 2, "div_const": 3, "min": 4, "max": 5, "neg": 6, "abs": 7,
+           "and": 8, "or": 9, "xor": 10, "not": 11}
+
+
+def _rows(v, t, n):
+    a = np.asarray(v, dtype=NP[t])
+    return np.ascontiguousarray(np.broadcast_to(a, (n,)) if a.ndim == 0 else a)
+
+
+def elementwise(op: str, t: str, a, b=None) -> np.ndarray:
+    """The elementwise kernel `<op>_<t>` over rows (scalars broadcast to the rows' length);
+    comparisons return 0/1 bytes. Raises OracleError for div_const by zero."""
+    n = max(np.size(a), np.size(b) if b is not None else 0)
+    if op in ("eq", "ne", "lt", "le", "gt", "ge"):
+        x, y = _rows(a, t, n), _rows(b, t, n)
+        if t in ("f64", "f32"):
+            x = x.astype(np.float64)
+            y = y.astype(np.float64)
+        else:
+            x = [int(v) for v in x]
+            y = [int(v) for v in y]
+        f = {"eq": lambda p, q: p == q, "ne": lambda p, q: p != q, "lt": lambda p, q: p < q,
+             "le": lambda p, q: p <= q, "gt": lambda p, q: p > q, "ge": lambda p, q: p >= q}[op]
+        return np.array([1 if f(float(p) if t in ("f64", "f32") else p, float(q) if t in ("f64", "f32") else q)
+                         else 0 for p, q in zip(x, y)], np.uint8)
+    x = _rows(a, t, n)
+    y = _rows(b, t, n) if b is not None else None
+    out = np.empty(n, NP[t])
+    if t == "f64":
+        lib().vko_ew_f64(FLOAT_OPS[op], _p(x), _p(y), n, _p(out))
+    elif t == "f32":
+        lib().vko_ew_f32(FLOAT_OPS[op], _p(x), _p(y), n, _p(out))
+    else:
+        if op == "div_const" and (y is None or not np.all(y != 0)):
+            raise OracleError("div_const: division by zero")
+        lib().vko_ew_int(INT_OPS[op], TYPE_CODE[t], _p(x), _p(y), n, _p(out))
+    return out
+
+
+def select(cond, t: str, a, b) -> np.ndarray:
+    """select_<t>: rows of `a` where cond is nonzero, else rows of `b` (elementwise.rs:105-120)."""
+    n = max(np.size(cond), np.size(a), np.size(b))
+    c, x, y = _rows(cond, "u8", n), _rows(a, t, n), _rows(b, t, n)
+    out = np.empty(n, NP[t])
+    for i in range(n):
+        out[i] = x[i] if c[i] else y[i]
+    return out
+
+
+def cast(x, src: str, dst: str) -> np.ndarray:
+    """cast_<src>_<dst> with Rust `as` semantics."""
+    x = np.ascontiguousarray(x, dtype=NP[src])
+    out = np.empty(x.size, NP[dst])
+    lib().vko_cast(TYPE_CODE[src], TYPE_CODE[dst], _p(x), x.size, _p(out))
     return out
 
 

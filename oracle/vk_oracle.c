@@ -220,11 +220,17 @@ uint64_t vko_sum_u8(const uint8_t *x, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) s += x[i];
   return s;
 }
-/* IntProduct wrapping in the element type, reduce.rs:358-366 */
-int64_t vko_product_i64(const int64_t *x, uint64_t n) {
+/* IntProduct<T> wrapping in the element type T (i64, u64, i32 or u8, `width` bytes),
+ * reduce.rs:358-366, identity 1: returns the product's bits in T, zero-extended. */
+uint64_t vko_product_int(const void *x, uint32_t width, uint64_t n) {
+  const uint8_t *b = (const uint8_t *)x;
   uint64_t p = 1;
-  for (uint64_t i = 0; i < n; ++i) p *= (uint64_t)x[i];
-  return (int64_t)p;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t v = 0;
+    memcpy(&v, b + i * width, width);
+    p *= v; /* the low `width` bytes of a product depend only on the low bytes of its factors */
+  }
+  return width == 8 ? p : p & ((1ull << (8 * width)) - 1);
 }
 
 /* count_cmp, fused.rs:142-147 (CountCmp fused.rs:149-179) */
@@ -376,6 +382,15 @@ void vko_scan_extreme_i64(const int64_t *x, const uint8_t *starts, uint64_t n, i
   }
 }
 
+void vko_scan_extreme_u64(const uint64_t *x, const uint8_t *starts, uint64_t n, int want_max, uint64_t *out) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    if (i == 0 || (starts && starts[i])) acc = x[i];
+    else acc = want_max ? (x[i] > acc ? x[i] : acc) : (x[i] < acc ? x[i] : acc);
+    out[i] = acc;
+  }
+}
+
 /*
  * f64 scan add: NOT defined by the reference (src/check/testdata/builtins.dd:50-54). This
  * project defines it by the tier 2 candidate order (docs/superpowers/specs/
@@ -450,6 +465,20 @@ int vko_reduce_into_extreme_f64(const int64_t *keys, const double *vals, uint64_
   for (int64_t b = 0; b < buckets; ++b) out[b] = want_max ? -INFINITY : INFINITY;
   for (uint64_t i = 0; i < n; ++i)
     out[keys[i]] = want_max ? vko_max2_f64(out[keys[i]], vals[i]) : vko_min2_f64(out[keys[i]], vals[i]);
+  return 0;
+}
+
+/* ReduceInto with ScanExtreme on i64 (index.rs:192-222, scan.rs:185-202): identity
+ * i64::MAX (min, Extremes::HIGHEST) / i64::MIN (max, LOWEST) in empty buckets. */
+int vko_reduce_into_extreme_i64(const int64_t *keys, const int64_t *vals, uint64_t n, int64_t buckets,
+                                int want_max, int64_t *out, int64_t *bad) {
+  int r = check_keys(keys, n, buckets, bad);
+  if (r) return r;
+  for (int64_t b = 0; b < buckets; ++b) out[b] = want_max ? INT64_MIN : INT64_MAX;
+  for (uint64_t i = 0; i < n; ++i) {
+    int64_t *o = &out[keys[i]];
+    if (want_max ? vals[i] > *o : vals[i] < *o) *o = vals[i];
+  }
   return 0;
 }
 
@@ -594,4 +623,137 @@ static void group_ids(const uint64_t *a, const uint64_t *b, uint64_t n, int64_t 
 void vko_group_id_u64(const uint64_t *x, uint64_t n, int64_t *out) { group_ids(x, 0, n, out); }
 void vko_group_id_pair_u64(const uint64_t *x, const uint64_t *y, uint64_t n, int64_t *out) {
   group_ids(x, y, n, out);
+}
+
+/* ------------------------------------------------------------ elementwise */
+
+/* Element type codes, in the order of ScalarType's storage types used here (vk_type):
+ * 0 f64, 1 f32, 2 i64, 3 u64, 4 i32, 5 u8, 6 bool (u8 storage). */
+static int type_width(int t) { return t <= 1 ? (t == 0 ? 8 : 4) : t <= 3 ? 8 : t == 4 ? 4 : 1; }
+
+/* Float ops, elementwise.rs:398-414 and :443-504 (Num for floats, Signed, Float): 0 add, 1 sub,
+ * 2 mul, 3 div, 4 min (IEEE minimum), 5 max, 6 neg, 7 abs, 8 sqrt. b is unused by the unary ops. */
+void vko_ew_f64(int op, const double *a, const double *b, uint64_t n, double *out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    double x = a[i], y = b ? b[i] : 0.0, r;
+    switch (op) {
+      case 0: r = x + y; break;
+      case 1: r = x - y; break;
+      case 2: r = x * y; break;
+      case 3: r = x / y; break;
+      case 4: r = vko_min2_f64(x, y); break;
+      case 5: r = vko_max2_f64(x, y); break;
+      case 6: r = -x; break;
+      case 7: r = fabs(x); break;
+      default: r = sqrt(x); break;
+    }
+    out[i] = r;
+  }
+}
+static float min2_f32(float a, float b) {
+  if (isnan(a) || isnan(b)) return NAN;
+  if (a == b) return signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+static float max2_f32(float a, float b) {
+  if (isnan(a) || isnan(b)) return NAN;
+  if (a == b) return signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+void vko_ew_f32(int op, const float *a, const float *b, uint64_t n, float *out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    float x = a[i], y = b ? b[i] : 0.0f, r;
+    switch (op) {
+      case 0: r = x + y; break;
+      case 1: r = x - y; break;
+      case 2: r = x * y; break;
+      case 3: r = x / y; break;
+      case 4: r = min2_f32(x, y); break;
+      case 5: r = max2_f32(x, y); break;
+      case 6: r = -x; break;
+      case 7: r = fabsf(x); break;
+      default: r = sqrtf(x); break;
+    }
+    out[i] = r;
+  }
+}
+
+static int64_t load_int(const void *p, int t, uint64_t i) {
+  switch (t) {
+    case 2: return ((const int64_t *)p)[i];
+    case 3: return (int64_t)((const uint64_t *)p)[i];
+    case 4: return ((const int32_t *)p)[i];
+    default: return ((const uint8_t *)p)[i];
+  }
+}
+static void store_int(void *p, int t, uint64_t i, uint64_t v) {
+  switch (t) {
+    case 2: case 3: ((uint64_t *)p)[i] = v; break;
+    case 4: ((uint32_t *)p)[i] = (uint32_t)v; break;
+    default: ((uint8_t *)p)[i] = (uint8_t)v; break;
+  }
+}
+
+/* Integer ops on type t (2 i64, 3 u64, 4 i32, 5 u8), elementwise.rs:380-397 (wrapping Num),
+ * :449-470 (wrapping neg/abs), :536-545 (bits), :612-737 (div_const, truncating, MIN / -1
+ * wraps): 0 add, 1 sub, 2 mul, 3 div_const by b[i] (nonzero), 4 min, 5 max, 6 neg, 7 abs,
+ * 8 and, 9 or, 10 xor, 11 not. Results keep the low bytes of the exact value. */
+void vko_ew_int(int op, int t, const void *a, const void *b, uint64_t n, void *out) {
+  const int sgn = t == 2 || t == 4, bitsz = 8 * type_width(t);
+  for (uint64_t i = 0; i < n; ++i) {
+    int64_t x = load_int(a, t, i), y = b ? load_int(b, t, i) : 0;
+    uint64_t ux = (uint64_t)x, uy = (uint64_t)y, r;
+    switch (op) {
+      case 0: r = ux + uy; break;
+      case 1: r = ux - uy; break;
+      case 2: r = ux * uy; break;
+      case 3: {
+        uint64_t mx = sgn && x < 0 ? 0 - ux : ux, my = sgn && y < 0 ? 0 - uy : uy;
+        if (!sgn && bitsz < 64) { mx &= (1ull << bitsz) - 1; my &= (1ull << bitsz) - 1; }
+        uint64_t q = mx / my;
+        r = sgn && ((x < 0) != (y < 0)) ? 0 - q : q;
+        break;
+      }
+      case 4: r = sgn ? (uint64_t)(x < y ? x : y) : (ux < uy ? ux : uy); break;
+      case 5: r = sgn ? (uint64_t)(x > y ? x : y) : (ux > uy ? ux : uy); break;
+      case 6: r = 0 - ux; break;
+      case 7: r = x < 0 ? 0 - ux : ux; break;
+      case 8: r = ux & uy; break;
+      case 9: r = ux | uy; break;
+      case 10: r = ux ^ uy; break;
+      default: r = ~ux; break;
+    }
+    store_int(out, t, i, r);
+  }
+}
+
+/* Rust `as` casts (elementwise.rs:574-610, CastOp): float to integer truncates toward zero,
+ * saturates at the bounds and maps NaN to 0; integer to integer keeps the low bytes of the
+ * sign- or zero-extended value; to a float rounds to nearest even; bool reads its byte. */
+void vko_cast(int from, int to, const void *x, uint64_t n, void *out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    if (to <= 1) {
+      double d;
+      float f;
+      if (from == 0) { d = ((const double *)x)[i]; f = (float)d; }
+      else if (from == 1) { f = ((const float *)x)[i]; d = f; }
+      else if (from == 3) { uint64_t u = ((const uint64_t *)x)[i]; d = (double)u; f = (float)u; }
+      else { int64_t v = load_int(x, from == 6 ? 5 : from, i); d = (double)v; f = (float)v; }
+      if (to == 0) ((double *)out)[i] = d; else ((float *)out)[i] = f;
+      continue;
+    }
+    uint64_t r;
+    if (from <= 1) {
+      double v = from == 0 ? ((const double *)x)[i] : ((const float *)x)[i];
+      int bitsz = 8 * type_width(to), sgn = to == 2 || to == 4;
+      double lo = sgn ? -ldexp(1.0, bitsz - 1) : 0.0, hi = sgn ? ldexp(1.0, bitsz - 1) : ldexp(1.0, bitsz);
+      if (isnan(v)) r = 0;
+      else if (v <= lo) r = sgn ? (uint64_t)1 << (bitsz - 1) : 0;
+      else if (v >= hi) r = sgn ? ((uint64_t)1 << (bitsz - 1)) - 1 : (bitsz == 64 ? ~0ull : (1ull << bitsz) - 1);
+      else r = sgn ? (uint64_t)(int64_t)trunc(v) : (uint64_t)trunc(v);
+    } else {
+      r = (uint64_t)load_int(x, from == 6 ? 5 : from, i);
+    }
+    store_int(out, to, i, r);
+  }
 }

@@ -179,6 +179,42 @@ def test_count_cmp_brute_force():
             assert O.count_cmp(x, cmp, c) == sum(1 for v in x if f(float(v), float(c))), (cmp, c)
 
 
+def test_count_cmp_i64_brute_force():
+    # count_<cmp>_i64 (fused.rs:142-179) for all six comparisons, with the extremes of i64 as
+    # rows and as constants: Python's own integer comparisons are the independent check.
+    lo, hi = -2**63, 2**63 - 1
+    x = np.array([lo, hi, 0, -1, 1, lo + 1, hi - 1, 5, 5, -5], np.int64)
+    rng = np.random.default_rng(7)
+    x = np.concatenate([x, rng.integers(-6, 7, 200).astype(np.int64)])
+    ops = {"eq": lambda a, b: a == b, "ne": lambda a, b: a != b, "lt": lambda a, b: a < b,
+           "le": lambda a, b: a <= b, "gt": lambda a, b: a > b, "ge": lambda a, b: a >= b}
+    for cmp, f in ops.items():
+        for c in [lo, hi, 0, -1, 5, lo + 1]:
+            want = sum(1 for v in x.tolist() if f(v, c))
+            assert O.count_cmp(x, cmp, c) == want, (cmp, c)
+    assert O.count_cmp(np.array([], np.int64), "eq", 0) == 0
+
+
+def test_integer_products_wrap():
+    # src/kernels/tests.rs:847-850,1337: reduce_mul wraps in the element type, identity 1
+    assert O.product_int(np.array([16, 17], np.uint8)) == 16
+    assert O.product_int(np.array([], np.int32)) == 1
+    assert O.product_int(np.array([2**63, 2], np.uint64)) == 0
+    assert O.product_int(np.array([2.0, 3.0, 4.0]).astype(np.int64)) == 24
+    # brute force: Python integers reduced modulo 2^width, read back signed for i64/i32
+    rng = np.random.default_rng(8)
+    for dt, w in [(np.int64, 64), (np.uint64, 64), (np.int32, 32), (np.uint8, 8)]:
+        info = np.iinfo(dt)
+        for n in [1, 2, 7, 300]:
+            x = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+            p = 1
+            for v in x.tolist():
+                p = (p * v) % (1 << w)
+            if info.min < 0 and p >> (w - 1):
+                p -= 1 << w
+            assert O.product_int(x) == p, (dt, n)
+
+
 # --------------------------------------------------------------- extremes
 
 def test_min_max_follow_ieee_minimum():
@@ -271,6 +307,22 @@ def test_scan_brute_force_and_last_equals_sum():
     assert list(O.scan_by_add_i64(x, starts)) == want
 
 
+def test_scan_extremes_segmented_brute_force():
+    # ScanExtreme with starts (scan.rs:164-202): a start restarts at the row itself; u64 rows
+    # compare unsigned, i64 signed
+    rng = np.random.default_rng(12)
+    n = 500
+    st = (rng.random(n) < 0.1).astype(np.uint8)
+    for x in [rng.integers(0, 2**64, n, dtype=np.uint64), rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64)]:
+        for mx in (False, True):
+            want, acc = [], None
+            for i, v in enumerate(x.tolist()):
+                acc = v if (i == 0 or st[i]) else (max(acc, v) if mx else min(acc, v))
+                want.append(acc)
+            assert O.scan_extreme(x, mx, st).tolist() == want
+            assert O.scan_extreme(x, mx).tolist() == (np.maximum.accumulate(x) if mx else np.minimum.accumulate(x)).tolist()
+
+
 def test_scan_add_f64_defined_order():
     # Build-defined order (tier 2 candidate): out[i] = canonical sum of x[0..=i].
     v = varied(100)
@@ -319,6 +371,63 @@ def test_histogram_and_reduce_into():
     assert list(mn) == [2.5, 0.5, math.inf, 9.0]
     with pytest.raises(O.OracleError, match="^length mismatch: 2 keys for 1 values$"):
         O.reduce_into([0, 0], np.array([1], np.int64), 1)
+
+
+def ieee_max(a, b):
+    # IEEE 754-2019 maximum written from the standard: NaN wins, +0.0 above -0.0
+    if math.isnan(a) or math.isnan(b):
+        return math.nan
+    if a == b:
+        return b if math.copysign(1.0, a) < 0 else a
+    return a if a > b else b
+
+
+def ieee_min(a, b):
+    if math.isnan(a) or math.isnan(b):
+        return math.nan
+    if a == b:
+        return a if math.copysign(1.0, a) < 0 else b
+    return a if a < b else b
+
+
+def test_reduce_into_extremes_identities():
+    # src/kernels/tests.rs:1653-1672: an empty bucket holds the operator's identity
+    assert list(O.reduce_into([1], np.array([2.5]), 3, "min")) == [math.inf, 2.5, math.inf]
+    mx = O.reduce_into([1], np.array([2.5]), 3, "max")
+    assert list(mx) == [-math.inf, 2.5, -math.inf]
+    assert list(O.reduce_into([1], np.array([4], np.int64), 2, "max")) == [-2**63, 4]
+    assert list(O.reduce_into([1], np.array([4], np.int64), 2, "min")) == [2**63 - 1, 4]
+
+
+@pytest.mark.parametrize("op", ["min", "max"])
+def test_reduce_into_extremes_brute_force(op):
+    # f64: NaN, +-0 and +-inf among the values, folded by ieee_min/ieee_max per bucket in row
+    # order from the identity; i64: the extremes of i64 among the values.
+    rng = np.random.default_rng(9 + (op == "max"))
+    n, nb = 2000, 37
+    keys = rng.integers(0, nb, n)
+    v = rng.normal(size=n)
+    special = rng.integers(0, 10, n)
+    v = np.where(special == 0, np.nan, np.where(special == 1, 0.0, np.where(special == 2, -0.0,
+                 np.where(special == 3, math.inf, np.where(special == 4, -math.inf, v)))))
+    keys[:3] = [5, 5, 6]
+    v[:3] = [0.0, -0.0, -0.0]  # bucket 5: {+0, -0}; bucket 6 starts with -0
+    f = ieee_max if op == "max" else ieee_min
+    want = [(-math.inf if op == "max" else math.inf)] * nb
+    for k, x in zip(keys.tolist(), v.tolist()):
+        want[k] = f(want[k], x)
+    got = O.reduce_into(keys, v, nb + 2, op)
+    assert len(got) == nb + 2
+    for k in range(nb):
+        w = want[k]
+        assert (math.isnan(w) and math.isnan(got[k])) or bits(w) == bits(float(got[k])), (op, k)
+    assert list(got[nb:]) == [-math.inf if op == "max" else math.inf] * 2  # empty buckets
+    iv = rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64, endpoint=True)
+    iv[:4] = [-2**63, 2**63 - 1, -2**63, 2**63 - 1]
+    wi = [(-2**63 if op == "max" else 2**63 - 1)] * nb
+    for k, x in zip(keys.tolist(), iv.tolist()):
+        wi[k] = max(wi[k], x) if op == "max" else min(wi[k], x)
+    assert list(O.reduce_into(keys, iv, nb, op)) == wi
 
 
 def test_reduce_into_f64_add_defined_order():
@@ -413,3 +522,122 @@ def test_group_sorted_program():
     ids = O.scan_add_i64(s.astype(np.int64)) - 1
     assert list(O.reduce_into(ids, v, O.count_bool(s))) == [3, 12, 6]
     assert list(O.compress(k, s)) == [1, 2, 5]
+
+
+# ---------------------------------------------------------------- elementwise
+
+def _f32_round(v: int) -> float:
+    # an integer rounded to the nearest float32, ties to even, by integer arithmetic
+    m, sgn = abs(v), -1 if v < 0 else 1
+    e = m.bit_length() - 24
+    if e > 0:
+        q, r = divmod(m, 1 << e)
+        half = 1 << (e - 1)
+        if r > half or (r == half and q & 1):
+            q += 1
+        m = q << e
+    return float(np.float32(sgn * float(m)))
+
+
+def test_elementwise_floats_follow_ieee():
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.normal(size=300) * 1e3, [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-310, -1e-310]])
+    y = np.concatenate([rng.normal(size=300), [-0.0, 0.0, np.inf, 1.0, 2.0, np.nan, 3.0]])
+    with np.errstate(all="ignore"):
+        for op, f in [("add", np.add), ("sub", np.subtract), ("mul", np.multiply), ("div", np.divide)]:
+            got = O.elementwise(op, "f64", x, y)
+            want = f(x, y)
+            assert np.array_equal(np.isnan(got), np.isnan(want))
+            assert np.array_equal(got[~np.isnan(got)].view(np.uint64), want[~np.isnan(want)].view(np.uint64)), op
+        got = O.elementwise("sqrt", "f64", np.abs(x))
+        assert np.array_equal(got.view(np.uint64), np.sqrt(np.abs(x)).view(np.uint64))
+        assert bits(O.elementwise("neg", "f64", [0.0])[0]) == bits(-0.0)
+        assert bits(O.elementwise("abs", "f64", [-0.0])[0]) == bits(0.0)
+        for op, f in [("min", ieee_min), ("max", ieee_max)]:
+            got = O.elementwise(op, "f64", x, y)
+            for g, a, b in zip(got.tolist(), x.tolist(), y.tolist()):
+                w = f(a, b)
+                assert (math.isnan(w) and math.isnan(g)) or bits(w) == bits(g), (op, a, b)
+        x32, y32 = x.astype(np.float32), y.astype(np.float32)
+        got = O.elementwise("div", "f32", x32, y32)
+        want = x32 / y32
+        assert np.array_equal(got[~np.isnan(want)].view(np.uint32), want[~np.isnan(want)].view(np.uint32))
+        assert np.array_equal(O.elementwise("lt", "f64", x, y), (x < y).astype(np.uint8))
+        assert O.elementwise("eq", "f64", [np.nan], [np.nan])[0] == 0
+        assert O.elementwise("ne", "f64", [np.nan], [np.nan])[0] == 1
+
+
+@pytest.mark.parametrize("t,w,signed", [("i64", 64, True), ("u64", 64, False), ("i32", 32, True), ("u8", 8, False)])
+def test_elementwise_integers_wrap(t, w, signed):
+    # Python integers reduced modulo 2^w (elementwise.rs:380-397, 449-470, 536-545, 612-737)
+    rng = np.random.default_rng(22 + w)
+    dt = O.NP[t]
+    info = np.iinfo(dt)
+    x = rng.integers(info.min, info.max, 400, dtype=dt, endpoint=True)
+    y = rng.integers(info.min, info.max, 400, dtype=dt, endpoint=True)
+    x[:4] = [info.min, info.max, 0, info.min]
+    y[:4] = [info.max, info.min, 1, -1 if signed else 1]
+    y[y == 0] = 3
+
+    def norm(v):
+        v %= 1 << w
+        return v - (1 << w) if signed and v >> (w - 1) else v
+
+    def trunc_div(a, b):
+        q = abs(a) // abs(b)
+        return norm(-q if (a < 0) != (b < 0) else q)
+
+    X, Y = x.tolist(), y.tolist()
+    for op, f in [("add", lambda a, b: norm(a + b)), ("sub", lambda a, b: norm(a - b)),
+                  ("mul", lambda a, b: norm(a * b)), ("div_const", trunc_div), ("min", min), ("max", max),
+                  ("and", lambda a, b: norm(a & b)), ("or", lambda a, b: norm(a | b)),
+                  ("xor", lambda a, b: norm(a ^ b))]:
+        assert O.elementwise(op, t, x, y).tolist() == [f(a, b) for a, b in zip(X, Y)], op
+    assert O.elementwise("not", t, x).tolist() == [norm(~a) for a in X]
+    if signed:
+        assert O.elementwise("neg", t, x).tolist() == [norm(-a) for a in X]
+        assert O.elementwise("abs", t, x).tolist() == [norm(abs(a)) for a in X]
+        assert O.elementwise("div_const", t, [info.min], [-1]).tolist() == [info.min]  # MIN / -1 wraps
+    with pytest.raises(O.OracleError, match="^div_const: division by zero$"):
+        O.elementwise("div_const", t, x, 0)
+    # tests/programs/arith.dd (div_sort): [5, 3, 9, 3, 7] / 2, negated dividend, negated divisor
+    if signed:
+        c = np.array([5, 3, 9, 3, 7], dt)
+        assert O.elementwise("div_const", t, c, 2).tolist() == [2, 1, 4, 1, 3]
+        assert O.elementwise("div_const", t, -c, 2).tolist() == [-2, -1, -4, -1, -3]
+        assert O.elementwise("div_const", t, c, -2).tolist() == [-2, -1, -4, -1, -3]
+
+
+def test_select():
+    c = np.array([1, 0, 1, 0], np.uint8)
+    assert O.select(c, "i64", [1, 2, 3, 4], [10, 20, 30, 40]).tolist() == [1, 20, 3, 40]
+    assert O.select(c, "f64", 1.5, [10.0, 20.0, 30.0, 40.0]).tolist() == [1.5, 20.0, 1.5, 40.0]
+
+
+def test_casts_follow_rust_as():
+    # float to int saturates and maps NaN to 0; int to int keeps the low bytes; to float rounds
+    # to nearest even (elementwise.rs:574-610)
+    f = np.array([np.nan, np.inf, -np.inf, 1e300, -1e300, 2.9, -2.9, -0.5, 255.5, 256.0, -1.0,
+                  2.0**63, -2.0**63, 2.0**64, 2.0**31 - 0.5, -2.0**31 - 1])
+    bounds = {"i64": (-2**63, 2**63 - 1), "u64": (0, 2**64 - 1), "i32": (-2**31, 2**31 - 1), "u8": (0, 255)}
+    for t, (lo, hi) in bounds.items():
+        want = [0 if math.isnan(v) else max(lo, min(hi, math.trunc(v))) if math.isfinite(v) else (hi if v > 0 else lo)
+                for v in f.tolist()]
+        assert O.cast(f, "f64", t).tolist() == want, t
+        assert O.cast(f.astype(np.float32), "f32", t).tolist() == [
+            0 if math.isnan(v) else max(lo, min(hi, math.trunc(v))) if math.isfinite(v) else (hi if v > 0 else lo)
+            for v in f.astype(np.float32).astype(np.float64).tolist()], t
+    i = np.array([-1, 2**40 + 7, -2**63, 2**63 - 1, 255, 256, -129], np.int64)
+    assert O.cast(i, "i64", "u8").tolist() == [v % 256 for v in i.tolist()]
+    assert O.cast(i, "i64", "i32").tolist() == [((v % 2**32) ^ 2**31) - 2**31 for v in i.tolist()]
+    assert O.cast(i, "i64", "u64").tolist() == [v % 2**64 for v in i.tolist()]
+    assert O.cast(np.array([-1, 7], np.int32), "i32", "u64").tolist() == [2**64 - 1, 7]  # sign-extends
+    assert O.cast(np.array([200], np.uint8), "u8", "i32").tolist() == [200]             # zero-extends
+    assert O.cast(np.array([1, 0], np.uint8), "bool", "f64").tolist() == [1.0, 0.0]
+    big = np.array([2**53 + 1, -(2**53 + 1), 2**63 - 1, 2**24 + 1, 2**62 + 2**38 + 1], np.int64)
+    assert [bits(v) for v in O.cast(big, "i64", "f64").tolist()] == [bits(float(v)) for v in big.tolist()]
+    assert [fbits(v) for v in O.cast(big, "i64", "f32").tolist()] == [fbits(_f32_round(v)) for v in big.tolist()]
+    u = np.array([2**64 - 1, 2**63 + 2**11 + 1], np.uint64)
+    assert [bits(v) for v in O.cast(u, "u64", "f64").tolist()] == [bits(float(v)) for v in u.tolist()]
+    assert [fbits(v) for v in O.cast(np.array([1e40, 1.0000001], np.float64), "f64", "f32").tolist()] == \
+        [fbits(np.inf), fbits(np.float32(1.0000001))]

@@ -26,12 +26,12 @@ struct In {
 };
 
 enum Impl {
-  R_CANON, R_SUM_INT, R_COUNT_CMP, R_COUNT_BOOL, R_EXTREME, R_ARG, R_BITS,
+  R_CANON, R_SUM_INT, R_PROD_INT, R_COUNT_CMP, R_COUNT_BOOL, R_EXTREME, R_ARG, R_BITS,
   S_COMPRESS, S_FILTER, S_WHERE,
   SC_SCAN, SC_FSCAN, SC_PREV,
   W_GATHER, W_ARGSORT, W_GROUP_ID, W_GROUP_ID2, W_HISTOGRAM, W_REDUCE_INTO, W_RAND_F64, W_RAND_I64,
   W_IOTA, W_LENGTH, W_FILL,
-  E_BINARY, E_UNARY
+  E_BINARY, E_UNARY, E_SELECT, E_DIVC
 };
 
 struct Def {
@@ -70,8 +70,10 @@ void build_table() {
   // reductions (src/kernels/scalar/reduce.rs:680-726)
   add({"reduce_add_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 0});
   add({"reduce_mul_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 4});
-  for (int32_t t : ints)
+  for (int32_t t : ints) {
     add({"reduce_add_" + tn(t), VK_REDUCTION, {A(t)}, false, sum_t(t), R_SUM_INT, 0, 0, t});
+    add({"reduce_mul_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_PROD_INT, 0, 0, t});
+  }
   for (int32_t t : nums) {
     add({"reduce_min_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_EXTREME, 0, 0, t});
     add({"reduce_max_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_EXTREME, 1, 0, t});
@@ -144,25 +146,43 @@ void build_table() {
     add({"length_" + tn(t), VK_WHOLE, {A(t)}, false, VK_I64, W_LENGTH, 0, 0, t});
     add({"fill_" + tn(t), VK_WHOLE, {S(VK_I64), S(t)}, true, t, W_FILL, 0, 0, t});
   }
-  // elementwise (elementwise.rs:740-912): the callers' glue of the hot path
+  // elementwise (elementwise.rs:740-912) over this backend's element types
   const char *arith[5] = {"add", "sub", "mul", "min", "max"};
-  const int arith_op[5] = {0, 1, 2, 14, 15};
-  for (int32_t t : {VK_F64, VK_I64}) {
+  const int arith_op[5] = {EW_ADD, EW_SUB, EW_MUL, EW_MIN, EW_MAX};
+  for (int32_t t : nums) {
     for (int k = 0; k < 5; ++k)
       add({std::string(arith[k]) + "_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, t, E_BINARY, arith_op[k], 0, t});
-    for (int c = 0; c < 6; ++c)
-      add({std::string(kCmp[c]) + "_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, VK_BOOL, E_BINARY, 3 + c, 0, t});
+    for (int c = 2; c < 6; ++c)  // lt le gt ge (register_num)
+      add({std::string(kCmp[c]) + "_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, VK_BOOL, E_BINARY, EW_EQ + c, 0, t});
   }
-  add({"and_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 9, 0, VK_BOOL});
-  add({"or_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 10, 0, VK_BOOL});
-  add({"xor_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 11, 0, VK_BOOL});
-  add({"eq_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 3, 0, VK_BOOL});
-  add({"ne_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, 4, 0, VK_BOOL});
-  add({"not_bool", VK_ELEMENTWISE, {E(VK_BOOL)}, true, VK_BOOL, E_UNARY, 12, 0, VK_BOOL});
-  add({"cast_bool_i64", VK_ELEMENTWISE, {E(VK_BOOL)}, true, VK_I64, E_UNARY, 13, 0, VK_BOOL});
-  add({"cast_i32_i64", VK_ELEMENTWISE, {E(VK_I32)}, true, VK_I64, E_UNARY, 13, 0, VK_I32});
-  add({"cast_u8_i64", VK_ELEMENTWISE, {E(VK_U8)}, true, VK_I64, E_UNARY, 13, 0, VK_U8});
-  add({"cast_i64_f64", VK_ELEMENTWISE, {E(VK_I64)}, true, VK_F64, E_UNARY, 16, 0, VK_I64});
+  for (int32_t t : all) {  // eq, ne and select exist for every type (register_any)
+    for (int c = 0; c < 2; ++c)
+      add({std::string(kCmp[c]) + "_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, VK_BOOL, E_BINARY, EW_EQ + c, 0, t});
+    add({"select_" + tn(t), VK_ELEMENTWISE, {E(VK_BOOL), E(t), E(t)}, true, t, E_SELECT, EW_SELECT, 0, t});
+  }
+  for (int32_t t : {VK_I64, VK_I32, VK_F64, VK_F32}) {  // register_signed
+    add({"neg_" + tn(t), VK_ELEMENTWISE, {E(t)}, true, t, E_UNARY, EW_NEG, 0, t});
+    add({"abs_" + tn(t), VK_ELEMENTWISE, {E(t)}, true, t, E_UNARY, EW_ABS, 0, t});
+  }
+  for (int32_t t : {VK_F64, VK_F32}) {  // register_float
+    add({"div_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, t, E_BINARY, EW_DIV, 0, t});
+    add({"sqrt_" + tn(t), VK_ELEMENTWISE, {E(t)}, true, t, E_UNARY, EW_SQRT, 0, t});
+  }
+  for (int32_t t : ints)  // register_div_const: the divisor is a scalar
+    add({"div_const_" + tn(t), VK_ELEMENTWISE, {E(t), S(t)}, true, t, E_DIVC, EW_DIVC, 0, t});
+  for (int32_t t : {VK_I32, VK_I64, VK_U8, VK_U64}) {  // register_bits
+    add({"and_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, t, E_BINARY, EW_BAND, 0, t});
+    add({"or_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, t, E_BINARY, EW_BOR, 0, t});
+    add({"xor_" + tn(t), VK_ELEMENTWISE, {E(t), E(t)}, true, t, E_BINARY, EW_BXOR, 0, t});
+    add({"not_" + tn(t), VK_ELEMENTWISE, {E(t)}, true, t, E_UNARY, EW_BNOT, 0, t});
+  }
+  add({"and_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, EW_AND, 0, VK_BOOL});
+  add({"or_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, EW_OR, 0, VK_BOOL});
+  add({"xor_bool", VK_ELEMENTWISE, {E(VK_BOOL), E(VK_BOOL)}, true, VK_BOOL, E_BINARY, EW_XOR, 0, VK_BOOL});
+  add({"not_bool", VK_ELEMENTWISE, {E(VK_BOOL)}, true, VK_BOOL, E_UNARY, EW_NOT, 0, VK_BOOL});
+  for (int32_t f : all)  // register_casts: every source to every numeric type
+    for (int32_t t : nums)
+      add({"cast_" + tn(f) + "_" + tn(t), VK_ELEMENTWISE, {E(f)}, true, t, E_UNARY, EW_CAST, 0, f});
 }
 
 const Def &def_of(const char *name) {
@@ -314,6 +334,7 @@ ReduceSpec reduce_spec(const Def &d, uint64_t base) {
   s.base_row = base;
   switch (d.impl) {
     case R_SUM_INT: s.kind = ReduceSpec::SUM_INT; break;
+    case R_PROD_INT: s.kind = ReduceSpec::PROD_INT; break;
     case R_COUNT_CMP: s.kind = ReduceSpec::COUNT_CMP; s.cmp = d.p0; break;
     case R_COUNT_BOOL: s.kind = ReduceSpec::COUNT_BOOL; s.type = VK_U8; break;
     case R_EXTREME: s.kind = ReduceSpec::EXTREME; s.want_max = d.p0 != 0; break;
@@ -396,6 +417,7 @@ vk_scalar finish_words(const Def &d, const uint64_t *w) {
       if (idx == INT64_MAX) runtime_error(std::string(d.p0 ? "argmax" : "argmin") + " of an empty array");
       return scalar(VK_I64, static_cast<uint64_t>(idx));
     }
+    case R_PROD_INT:
     case R_BITS: {
       uint64_t v = w[0];
       if (d.out_t == VK_BOOL) v = v != 0;
@@ -708,22 +730,22 @@ void run_elementwise(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, vk_
   EwSpec s{};
   s.op = d.p0;
   s.in_type = d.t0;
-  const void *a = nullptr, *b = nullptr;
-  if (args[0].kind == VK_SPLAT) {
-    s.a_splat = true;
-    s.a_bits = args[0].splat.bits;
-  } else {
-    a = dptr(args[0].rows);
-  }
-  if (nargs > 1) {
-    if (args[1].kind == VK_SPLAT) {
-      s.b_splat = true;
-      s.b_bits = args[1].splat.bits;
+  s.out_type = d.out_t;
+  const void *p[3] = {nullptr, nullptr, nullptr};
+  bool *splat[3] = {&s.a_splat, &s.b_splat, &s.c_splat};
+  uint64_t *bits[3] = {&s.a_bits, &s.b_bits, &s.c_bits};
+  for (size_t i = 0; i < nargs && i < 3; ++i) {
+    if (args[i].kind == VK_SPLAT) {
+      *splat[i] = true;
+      *bits[i] = args[i].splat.bits;
     } else {
-      b = dptr(args[1].rows);
+      p[i] = dptr(args[i].rows);
     }
   }
-  elementwise(c, s, a, b, out->len, dptr(out));
+  // DivConstBody::run (elementwise.rs:717-736): a zero divisor is the kernel's runtime error
+  if (d.impl == E_DIVC && (s.b_bits & (type_width(d.t0) == 8 ? ~uint64_t(0) : (uint64_t(1) << (8 * type_width(d.t0))) - 1)) == 0)
+    runtime_error("div_const: division by zero");
+  elementwise(c, s, p[0], p[1], p[2], out->len, dptr(out));
 }
 
 // Dispatch::call (dispatch.rs:56-141)

@@ -87,6 +87,17 @@ struct OpSum {
   __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
 };
 
+// IntProduct (reduce.rs:358-366): the wrapping product in the element type. The low bytes of
+// a product depend only on the low bytes of its factors, so the fold multiplies widened rows
+// modulo 2^64 and the caller keeps the element type's bytes.
+template <typename T>
+struct OpProd {
+  using Acc = SumAcc;
+  __device__ Acc identity() const { return {1}; }
+  __device__ void add(Acc &a, T v, uint64_t) const { a.s *= widen(v); }
+  __device__ void merge(Acc &a, const Acc &b) const { a.s *= b.s; }
+};
+
 template <typename T>
 struct OpCount {
   using Acc = SumAcc;
@@ -368,6 +379,7 @@ void reduce_state_init(Ctx &c, const ReduceSpec &s, void *state) {
       k_init_state<<<1, 64, 0, c.stream>>>(p, 1, s.bitop == 0 ? all : 0, 0, 0);
       break;
     }
+    case ReduceSpec::PROD_INT: k_init_state<<<1, 64, 0, c.stream>>>(p, 1, 1, 0, 0); break;
     case ReduceSpec::EXTREME: k_init_state<<<1, 64, 0, c.stream>>>(p, 2, 0, 0, 0); break;
     case ReduceSpec::ARG_EXTREME:
       k_init_state<<<1, 64, 0, c.stream>>>(p, 3, 0, uint64_t(kNone), uint64_t(kNone));
@@ -400,6 +412,15 @@ void with_op(const ReduceSpec &s, F &&f) {
         case VK_U64: f(make_count<uint64_t>(s.cmp, s.cbits), static_cast<uint64_t *>(nullptr)); break;
         case VK_I32: f(make_count<int32_t>(s.cmp, s.cbits), static_cast<int32_t *>(nullptr)); break;
         default: usage_error("unsupported count type");
+      }
+      break;
+    case ReduceSpec::PROD_INT:
+      switch (s.type) {
+        case VK_I64: f(OpProd<int64_t>{}, static_cast<int64_t *>(nullptr)); break;
+        case VK_U64: f(OpProd<uint64_t>{}, static_cast<uint64_t *>(nullptr)); break;
+        case VK_I32: f(OpProd<int32_t>{}, static_cast<int32_t *>(nullptr)); break;
+        case VK_U8: f(OpProd<uint8_t>{}, static_cast<uint8_t *>(nullptr)); break;
+        default: usage_error("unsupported integer product type");
       }
       break;
     case ReduceSpec::COUNT_BOOL: f(OpCountBool{}, static_cast<uint8_t *>(nullptr)); break;

@@ -163,7 +163,7 @@ enum Cmp { CMP_EQ = 0, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE };
 // Streaming reductions (k_reduce.hip). `state` is the reducer's device state; each
 // launch merges the reduction of its rows into it.
 struct ReduceSpec {
-  enum Kind { SUM_INT, COUNT_CMP, COUNT_BOOL, EXTREME, ARG_EXTREME, BITS } kind;
+  enum Kind { SUM_INT, COUNT_CMP, COUNT_BOOL, EXTREME, ARG_EXTREME, BITS, PROD_INT } kind;
   int32_t type;       // element type of x
   int cmp;            // COUNT_CMP
   uint64_t cbits;     // COUNT_CMP constant bits
@@ -345,14 +345,23 @@ void rand_f64(Ctx &c, uint64_t n, uint64_t seed, double *out);
 void rand_i64(Ctx &c, uint64_t n, uint64_t seed, int64_t *out);
 void iota(Ctx &c, uint64_t n, int64_t *out);
 void fill_bits(Ctx &c, uint32_t width, uint64_t n, uint64_t bits, void *out);
-// Elementwise binary op over rows/splats: op codes in k_misc.hip.
+// Elementwise ops over rows or splats (k_misc.hip). in_type is the operands' type (select: the
+// type of its two choices; cast: the source), out_type the target of a cast. Operand a is the
+// condition of select, b and the third operand e its choices; unary ops read a (and b's
+// scalar: the divisor of div_const).
+enum EwOp {
+  EW_ADD = 0, EW_SUB, EW_MUL, EW_EQ, EW_NE, EW_LT, EW_LE, EW_GT, EW_GE,
+  EW_AND, EW_OR, EW_XOR, EW_NOT, EW_CAST, EW_MIN, EW_MAX, EW_NEG, EW_ABS, EW_DIV, EW_SQRT,
+  EW_DIVC, EW_BAND, EW_BOR, EW_BXOR, EW_BNOT, EW_SELECT
+};
 struct EwSpec {
   int op;
   int32_t in_type;
-  bool a_splat, b_splat;
-  uint64_t a_bits, b_bits;
+  int32_t out_type;
+  bool a_splat, b_splat, c_splat;
+  uint64_t a_bits, b_bits, c_bits;
 };
-void elementwise(Ctx &c, const EwSpec &s, const void *a, const void *b, uint64_t n, void *out);
+void elementwise(Ctx &c, const EwSpec &s, const void *a, const void *b, const void *e, uint64_t n, void *out);
 void copy_d2d(Ctx &c, void *dst, const void *src, uint64_t bytes);
 
 }  // namespace vk
