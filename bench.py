@@ -104,30 +104,48 @@ def parse_trace(text: str):
     return out
 
 
-def cpu_baseline(sample_rows: int, budget_s: float = 12.0):
-    """The reference's best x86 CPU path, restated in C (oracle/cpu_best.c: x86-64-v4 auto
-    loops, AVX-512 compress stores and vpgatherqq), on one host core over the same suite at
-    `sample_rows` rows, repeated for about `budget_s` seconds. Every result is checked against
-    the oracle (the scalar restatement) before timing."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """The host cores this process may use, at most 16 (a one-GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_streaming_steps(B, n: int, shard: int):
+    """The streaming suite on host arrays of n rows (shard number `shard` picks the seeds), as
+    (label, call, kept) with every result checked against the oracle first."""
     import ctypes as C
     from oracle import oracle as O
-    B = O.best()
-    n = sample_rows
     p = lambda a: a.ctypes.data_as(C.c_void_p)
-    x1, x2 = O.rand_f64(n, 1), O.rand_f64(n, 2)
-    i1 = i64_uniform_40(n, 4)
-    tbl = O.rand_f64(min(1 << 26, n), 3)
-    idx = gather_indices(n, 5, tbl.size)
+    s0 = 16 * shard
+    x1, x2 = O.rand_f64(n, 1 + s0), O.rand_f64(n, 2 + s0)
+    i1 = i64_uniform_40(n, 4 + s0)
+    tbl = O.rand_f64(min(1 << 26, n), 3 + s0)
+    idx = gather_indices(n, 5 + s0, tbl.size)
     masks = {c: (x2 > c).astype(np.uint8) for c in SELECTIVITIES}
     fo = np.empty(n + 8, np.float64)
     io = np.empty(n, np.int64)
     bad = np.zeros(1, np.int64)
-    # correctness gate before timing
+    # correctness gate before timing (benches/tier1.rs:403-433)
     assert B.vkb_dot_f64(p(x1), p(x2), n) == O.dot_f64(x1, x2)
     assert B.vkb_sum_f64(p(x1), n) == O.canonical_sum_f64(x1)
     assert B.vkb_count_gt_f64(p(x1), n, 0.5) == O.count_cmp(x1, "gt", 0.5)
     k = B.vkb_filter_gt_f64(p(x1), n, 0.5, p(fo))
     assert np.array_equal(fo[:k], O.filter_cmp(x1, "gt", 0.5))
+    B.vkb_scan_add_i64(p(i1), n, p(io))
+    assert np.array_equal(io, O.scan_add_i64(i1))
     kept = {c: int((x1 > c).sum()) for c in SELECTIVITIES}
     kept_m = {c: int(masks[c].sum()) for c in SELECTIVITIES}
     steps = [
@@ -143,6 +161,94 @@ def cpu_baseline(sample_rows: int, budget_s: float = 12.0):
     for c in SELECTIVITIES:
         steps.append((f"compress_f64@{c}", lambda c=c: B.vkb_compress_f64(p(x1), p(masks[c]), n, p(fo)), kept_m[c]))
     steps.append(("gather_f64", lambda: B.vkb_gather_f64(p(tbl), tbl.size, p(idx), n, p(fo), p(bad)), 0))
+    keep = (x1, x2, i1, tbl, idx, masks, fo, io, bad)
+    return steps, keep
+
+
+def zipf_cdf(values: int = 1 << 20, s: float = 1.1) -> np.ndarray:
+    """The CDF of Zipf(s) over 1..values: P(k) proportional to k^-s."""
+    w = np.arange(1, values + 1, dtype=np.float64) ** -s
+    c = np.cumsum(w)
+    return c / c[-1]
+
+
+def zipf_from_uniform(u: np.ndarray, cdf: np.ndarray) -> np.ndarray:
+    """Inverse CDF: the least k with u < cdf[k - 1], as i64 in 1..len(cdf)."""
+    return (np.searchsorted(cdf, u, side="right") + 1).astype(np.int64)
+
+
+def cpu_sort_legs(B, n: int):
+    """One core: argsort (u64 uniform, i64 Zipf(1.1) over 2^20 values, f64 normal), group_id +
+    reduce_into_add_f64 over 1K, 1M and n/4 groups, and scan_by add, on n-row samples; each
+    checked against the oracle at 2^16 rows first. Algorithmic GB/s as for the device."""
+    import ctypes as C
+    from oracle import oracle as O
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    cdf = zipf_cdf()
+    rng = np.random.default_rng(3)
+
+    def sort_keys(m):
+        f = rng.normal(size=m)
+        f[rng.integers(0, m, max(1, m // 100000))] = 0.0
+        f[rng.integers(0, m, max(1, m // 100000))] = -0.0
+        return [("u64", 0, splitmix64_np(7, 0, m)), ("i64_zipf", 1, zipf_from_uniform(O.rand_f64(m, 9), cdf)),
+                ("f64_normal", 2, f)]
+
+    def group_keys(m, g):
+        pool = splitmix64_np(11, 0, g).view(np.int64)
+        return pool[(splitmix64_np(12, 0, m) % np.uint64(g)).astype(np.int64)]
+
+    small = 1 << 16
+    for label, kind, k in sort_keys(small):
+        out = np.empty(small, np.int64)
+        assert B.vkb_argsort_64(kind, p(np.ascontiguousarray(k)), small, p(out)) == 0
+        assert np.array_equal(out, O.argsort(k)), label
+    kk = group_keys(small, 1000)
+    ids = np.empty(small, np.int64)
+    assert B.vkb_group_id_64(p(kk), small, p(ids)) == 0 and np.array_equal(ids, O.group_id(kk))
+    res = {}
+    out = np.empty(n, np.int64)
+    for label, kind, k in sort_keys(n):
+        k = np.ascontiguousarray(k)
+        t0 = time.perf_counter()
+        assert B.vkb_argsort_64(kind, p(k), n, p(out)) == 0
+        dt = time.perf_counter() - t0
+        b = argsort_alg_bytes(k, kind)
+        res[f"argsort_{label}"] = {"rows": n, "ms": round(dt * 1e3, 1), "GBps": round(b / dt / 1e9, 3)}
+    vals = O.rand_f64(n, 1)
+    for g in (1 << 10, 1 << 20, n // 4):
+        kk = group_keys(n, g)
+        t0 = time.perf_counter()
+        assert B.vkb_group_id_64(p(kk), n, p(out)) == 0
+        t1 = time.perf_counter()
+        ng = int(out.max()) + 1
+        sums = np.empty(ng, np.float64)
+        assert B.vkb_reduce_into_add_f64(p(out), p(vals), n, ng, p(sums)) == 0
+        t2 = time.perf_counter()
+        res[f"group_sum_{g}"] = {"rows": n, "group_id_ms": round((t1 - t0) * 1e3, 1),
+                                 "reduce_into_ms": round((t2 - t1) * 1e3, 1),
+                                 "GBps_min_io": round((16 * n + 8 * ng) / (t2 - t0) / 1e9, 3)}
+    st = ((splitmix64_np(13, 0, n) % np.uint64(16)) == 0).astype(np.uint8)
+    iv = i64_uniform_40(n, 4)
+    t0 = time.perf_counter()
+    B.vkb_scan_by_add_i64(p(iv), p(st), n, p(out))
+    dt = time.perf_counter() - t0
+    res["scan_by_add_i64"] = {"rows": n, "ms": round(dt * 1e3, 1), "GBps": round(17 * n / dt / 1e9, 3)}
+    return res
+
+
+def cpu_baseline(sample_rows: int, budget_s: float = 8.0):
+    """The reference's best x86 CPU path, restated in C (oracle/cpu_best.c: x86-64-v4 auto
+    loops, AVX-512 compress stores, vpgatherqq, and the scalar radix argsort, hash group_id and
+    scan_by), on the host cores of this run over `sample_rows`-row samples of the same
+    workloads. One core (the reference's design), then one independent instance per core on
+    disjoint shards (the streaming suite). Every result is checked against the oracle (the
+    scalar restatement) before timing."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    B = O.best()
+    n = sample_rows
+    steps, keep = cpu_streaming_steps(B, n, 0)
     total_bytes = sum(alg_bytes(nm.split("@")[0], n, k) for nm, _, k in steps)
     t_end = time.perf_counter() + budget_s
     passes, spent = 0, 0.0
@@ -153,17 +259,45 @@ def cpu_baseline(sample_rows: int, budget_s: float = 12.0):
         spent += time.perf_counter() - t0
         passes += 1
     gbps = total_bytes * passes / spent / 1e9
-    cpu = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    del keep, steps
+    # all cores: ctypes releases the GIL during each foreign call, so threads run the C loops
+    # in parallel, one shard each
+    cores = host_cores()
+    shards = [cpu_streaming_steps(B, n, s + 1) for s in range(cores)]
+    all_passes = max(1, passes // 2)
+
+    def run(sh):
+        for _ in range(all_passes):
+            for _, f, _ in sh[0]:
+                f()
+
+    with ThreadPoolExecutor(cores) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(run, shards))
+        wall = time.perf_counter() - t0
+    all_bytes = sum(sum(alg_bytes(nm.split("@")[0], n, k) for nm, _, k in sh[0]) for sh in shards) * all_passes
+    del shards
+    sort = cpu_sort_legs(B, n)
+    cpu = cpu_model()
     return {"value": round(gbps, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"the same suite (one gather table) at {n} rows, {passes} passes in {spent:.1f} s, "
-                      f"C restatement of the reference's best x86-64-v4 path on one core of {cpu}"}
+            "sample": f"the streaming suite (one gather table) at {n} rows, {passes} passes in {spent:.1f} s, "
+                      f"C restatement of the reference's best x86-64-v4 path on one core of {cpu}",
+            "all_cores": {"value": round(all_bytes / wall / 1e9, 3), "unit": "GB/s", "cores": cores,
+                          "sample": f"{cores} independent instances of the same suite on disjoint {n}-row "
+                                    f"shards, {all_passes} passes each in {wall:.1f} s"},
+            "sort": sort}
+
+
+def argsort_alg_bytes(keys: np.ndarray, kind: int) -> int:
+    """2nw + 2(P-1)n(w+4) + 8n for 8-byte keys, P the non-constant byte digits of the images."""
+    n = keys.size
+    b = keys.view(np.uint64)
+    img = b ^ np.uint64(1 << 63) if kind == 1 else (np.where(b >> np.uint64(63), ~b, b ^ np.uint64(1 << 63)) if kind == 2 else b)
+    P = 0
+    for d in range(8):
+        dig = (img >> np.uint64(8 * d)) & np.uint64(255)
+        P += int(not np.all(dig == dig[0]))
+    return 16 * n + 2 * max(P - 1, 0) * n * 12 + 8 * n
 
 
 def batched_suite(D, count: int):
@@ -203,12 +337,69 @@ def batched_suite(D, count: int):
             "speedup": round(tp / tb, 2), "identical": same, "filter_sum": filter_sum}
 
 
-def sort_suite(D):
-    """Sort and grouping cases: argsort of 2^27 keys (u64 uniform splitmix64, i64 Zipf(1.1)
-    over 2^20 values, f64 normal(0,1) with +-0), group-by sum of rand_f64 values over i64 keys
-    with 1K, 1M and 64M distinct values at 2^28 rows (group_id, reduce_max + 1,
-    reduce_into_add_f64), and scan_by add over 2^28 rows with starts of probability 1/16.
-    Times are the median of three calls from HIP events; GB/s counts the algorithmic bytes."""
+def key_image(b: np.ndarray, kind: int) -> np.ndarray:
+    """The u64 sort images of 8-byte keys as bits: 0 u64, 1 i64 (sign flipped), 2 f64 (negative
+    values flip every bit, others the sign bit)."""
+    b = b.view(np.uint64)
+    if kind == 1:
+        return b ^ np.uint64(1 << 63)
+    if kind == 2:
+        return np.where(b >> np.uint64(63), ~b, b ^ np.uint64(1 << 63))
+    return b
+
+
+def check_argsort(keys: np.ndarray, kind: int, perm: np.ndarray) -> bool:
+    """Exact check of a stable ascending argsort: the positions are a permutation, the keys'
+    images are non-decreasing along it, and equal images keep their positions' order. The
+    stable ascending permutation is unique, so this equals a comparison with the oracle."""
+    n = keys.size
+    if perm.size != n or (n and (perm.min() < 0 or perm.max() >= n)):
+        return False
+    seen = np.zeros(n, np.bool_)
+    seen[perm] = True
+    if not seen.all():
+        return False
+    img = key_image(keys, kind)[perm]
+    if not np.all(img[1:] >= img[:-1]):
+        return False
+    eq = img[1:] == img[:-1]
+    return bool(np.all(perm[1:][eq] > perm[:-1][eq]))
+
+
+def check_group_ids(keys: np.ndarray, ids: np.ndarray) -> bool:
+    """Exact check of first-occurrence ids: rows with one id hold one key, distinct ids hold
+    distinct keys, and the running maximum of the ids grows by at most 1 from 0 (every new key
+    takes the next id). These fix the ids uniquely, as group.rs:1-5 defines them."""
+    n = keys.size
+    if ids.size != n:
+        return False
+    if n == 0:
+        return True
+    if ids[0] != 0 or ids.min() < 0:
+        return False
+    run = np.maximum.accumulate(ids)
+    if np.any(np.diff(run) > 1):
+        return False
+    g = int(run[-1]) + 1
+    key_of = np.empty(g, keys.dtype)
+    key_of[ids] = keys
+    if not np.array_equal(key_of[ids], keys):
+        return False
+    return np.unique(key_of).size == g
+
+
+def sort_suite(D, verify: bool = True):
+    """Sort and grouping (the configuration's third workload): argsort of 2^27 keys (u64 uniform
+    = splitmix64(7 + i), i64 Zipf(1.1) over 2^20 values by inverse CDF of rand_f64(2^27, 9),
+    f64 normal(0,1) with +0 and -0 injected), group-by sum of rand_f64(2^28, 1) values over i64
+    keys pool[splitmix64(12 + i) mod G], pool = splitmix64(11 + j), for G = 1K, 1M and 64M
+    (group_id, reduce_max + 1, reduce_into_add_f64), and scan_by add over 2^28 i64 rows with
+    starts where splitmix64(13 + i) mod 16 == 0. Inputs are generated on the device where the
+    table's kernels allow it. Times are medians of HIP-event times per call; GB/s counts the
+    algorithmic bytes (argsort 2nw + 2(P-1)n(w+4) + 8n, group-by 16n + 8G, scan_by 17n). Every
+    result is checked exactly: argsort and group_id ids by the properties that fix them,
+    reduce_into and scan_by against the oracle."""
+    from oracle import oracle as O
     from verkehrt_amd import Array, Scalar as S
     ctx = D.ctx
 
@@ -225,44 +416,126 @@ def sort_suite(D):
         ctx.set_trace(False)
         return float(np.median([us for _, us in tr]))
 
+    checks = {}
     out = {}
     n = 1 << 27
+    # i64 Zipf(1.1) over 1..2^20 on the device: binary search of the CDF for every uniform
+    cdf = zipf_cdf()
+    dcdf = Array.from_numpy(ctx, cdf)
+    u = D.call("rand_f64", [S("i64", n), S("i64", 9)])
+    lo = D.call("fill_i64", [S("i64", n), S("i64", 0)])
+    hi = D.call("fill_i64", [S("i64", n), S("i64", cdf.size - 1)])
+    for _ in range(21):  # invariant: the answer index is in [lo, hi]
+        mid = D.call("div_const_i64", [D.call("add_i64", [lo, hi]), S("i64", 2)])
+        go = D.call("le_f64", [D.call("gather_f64", [dcdf, mid]), u])  # cdf[mid] <= u: answer > mid
+        lo = D.call("select_i64", [go, D.call("add_i64", [mid, S("i64", 1)]), lo])
+        hi = D.call("select_i64", [go, hi, mid])
+    zipf = D.call("add_i64", [lo, S("i64", 1)])
+    del lo, hi, mid, go, u, dcdf
     rng = np.random.default_rng(3)
-    keys = {"u64": Array.from_numpy(ctx, splitmix64_np(7, 0, n), "u64"),
-            "i64_zipf": Array.from_numpy(ctx, np.minimum(rng.zipf(1.1, n), 1 << 20).astype(np.int64)),
-            "f64_normal": None}
     f = rng.normal(size=n)
     f[rng.integers(0, n, 1000)] = 0.0
     f[rng.integers(0, n, 1000)] = -0.0
-    keys["f64_normal"] = Array.from_numpy(ctx, f)
-    del f
-    for label, a in keys.items():
-        name = {"u64": "argsort_u64", "i64_zipf": "argsort_i64", "f64_normal": "argsort_f64"}[label]
+    cases = [("u64", "argsort_u64", 0, D.call("cast_i64_u64", [D.call("rand_i64", [S("i64", n), S("i64", 7)])])),
+             ("i64_zipf", "argsort_i64", 1, zipf), ("f64_normal", "argsort_f64", 2, Array.from_numpy(ctx, f))]
+    del f, zipf
+    for label, name, kind, a in cases:
         us = timed(name, [a])
-        out[f"argsort_{label}"] = {"rows": n, "us": round(us, 1)}
-    del keys
+        host = a.to_numpy()
+        b = argsort_alg_bytes(host, kind)
+        out[f"argsort_{label}"] = {"rows": n, "us": round(us, 1), "GBps": round(b / us / 1e3, 1),
+                                   "frac": round(b / us / 1e3 / HBM_PEAK_GBPS, 4), "bytes": b}
+        if verify:
+            checks[f"argsort_{label}"] = check_argsort(host, kind, D.call(name, [a]).to_numpy())
+        if label == "i64_zipf" and verify:
+            ref = zipf_from_uniform(O.rand_f64(1 << 16, 9), cdf)  # the device's inverse CDF
+            checks["zipf_inputs"] = bool(np.array_equal(host[:1 << 16], ref))
+        del host
+    del cases
     n = 1 << 28
     vals = D.call("rand_f64", [S("i64", n), S("i64", 1)])
+    vals_h = O.rand_f64(n, 1) if verify else None
     for g in (1 << 10, 1 << 20, 1 << 26):
-        pool = splitmix64_np(11, 0, g).view(np.int64)
-        sel = (splitmix64_np(12, 0, n) % np.uint64(g)).astype(np.int64)
-        k = Array.from_numpy(ctx, pool[sel])
-        del sel
+        pool = D.call("rand_i64", [S("i64", g), S("i64", 11)])
+        sel = D.call("and_i64", [D.call("rand_i64", [S("i64", n), S("i64", 12)]), S("i64", g - 1)])
+        k = D.call("gather_i64", [pool, sel])
+        del pool, sel
         t_id = timed("group_id_i64", [k], reps=2)
         ids = D.call("group_id_i64", [k])
         ng = D.call("reduce_max_i64", [ids]).value + 1
         t_ri = timed("reduce_into_add_f64", [ids, vals, S("i64", ng)], reps=2)
-        out[f"group_sum_{g}"] = {"rows": n, "group_id_us": round(t_id, 1), "reduce_into_us": round(t_ri, 1),
-                                 "GBps_min_io": round((16 * n + 8 * g) / (t_id + t_ri) / 1e3, 1)}
+        b = 16 * n + 8 * ng
+        out[f"group_sum_{g}"] = {"rows": n, "groups": ng, "group_id_us": round(t_id, 1), "reduce_into_us": round(t_ri, 1),
+                                 "GBps_min_io": round(b / (t_id + t_ri) / 1e3, 1),
+                                 "frac": round(b / (t_id + t_ri) / 1e3 / HBM_PEAK_GBPS, 4)}
+        if verify:
+            kh, ih = k.to_numpy(), ids.to_numpy()
+            checks[f"group_id_{g}"] = check_group_ids(kh, ih)
+            del kh
+            got = D.call("reduce_into_add_f64", [ids, vals, S("i64", ng)]).to_numpy()
+            checks[f"reduce_into_add_f64_{g}"] = bool(np.array_equal(got.view(np.uint64),
+                                                                     O.reduce_into(ih, vals_h, ng).view(np.uint64)))
+            del ih, got
         del k, ids
-    st = np.zeros(n, np.uint8)
-    st[(splitmix64_np(13, 0, n) % np.uint64(16)) == 0] = 1
-    sa = Array.from_numpy(ctx, st, "bool")
-    del st
-    iv = Array.from_numpy(ctx, i64_uniform_40(n, 4))
-    us = timed("scan_by_add_i64", [iv, sa])
-    out["scan_by_add_i64"] = {"rows": n, "us": round(us, 1), "GBps": round(17 * n / us / 1e3, 1)}
+    del vals, vals_h
+    st = D.call("eq_i64", [D.call("and_i64", [D.call("rand_i64", [S("i64", n), S("i64", 13)]), S("i64", 15)]),
+                           S("i64", 0)])
+    iv_h = i64_uniform_40(n, 4)
+    iv = Array.from_numpy(ctx, iv_h)
+    us = timed("scan_by_add_i64", [iv, st])
+    out["scan_by_add_i64"] = {"rows": n, "us": round(us, 1), "GBps": round(17 * n / us / 1e3, 1),
+                              "frac": round(17 * n / us / 1e3 / HBM_PEAK_GBPS, 4)}
+    if verify:
+        got = D.call("scan_by_add_i64", [iv, st]).to_numpy()
+        checks["scan_by_add_i64"] = bool(np.array_equal(got, O.scan_by_add_i64(iv_h, st.to_numpy())))
+    out["verified"] = bool(checks) and all(checks.values())
+    out["checks"] = checks
     return out
+
+
+def verify_streaming(D, calls, n, table_rows):
+    """Each streaming primitive's 2^28-row result, computed once more after the timed region,
+    against the oracle on the same inputs (as benches/tier1.rs:403-433 asserts before timing)."""
+    from oracle import oracle as O
+    x1, x2 = O.rand_f64(n, 1), O.rand_f64(n, 2)
+    inputs = {"x1": x1, "x2": x2}
+    want = {}
+    res = {}
+    for label, name, a in calls:
+        got = D.call(name, a)
+        if label == "dot_f64":
+            ok = got.value == O.dot_f64(x1, x2)
+        elif label == "reduce_add_f64":
+            ok = got.value == O.canonical_sum_f64(x1)
+        elif label == "count_gt_f64":
+            ok = got.value == O.count_cmp(x1, "gt", 0.5)
+        elif label == "reduce_min_f64":
+            ok = got.value == float(O.reduce_extreme(x1, False))
+        elif label == "argmax_f64":
+            ok = got.value == O.arg_extreme(x1, True)
+        elif label == "scan_add_i64":
+            ok = np.array_equal(got.to_numpy(), O.scan_add_i64(i64_uniform_40(n, 4)))
+        elif label.startswith("filter_gt_f64@"):
+            c = float(label.split("@")[1])
+            ok = np.array_equal(got.to_numpy().view(np.uint64), O.filter_cmp(x1, "gt", c).view(np.uint64))
+        elif label.startswith("compress_f64@"):
+            c = float(label.split("@")[1])
+            ok = np.array_equal(got.to_numpy().view(np.uint64), O.compress(x1, (x2 > c).astype(np.uint8)).view(np.uint64))
+        elif label == "gather_f64":
+            if "idx" not in want:
+                want["idx"] = gather_indices(n, 5, table_rows)
+            ok = np.array_equal(got.to_numpy().view(np.uint64),
+                                O.gather(O.rand_f64(table_rows, 3), want["idx"]).view(np.uint64))
+        elif label == "gather_i64":
+            if "idx" not in want:
+                want["idx"] = gather_indices(n, 5, table_rows)
+            ok = np.array_equal(got.to_numpy(), O.gather(i64_uniform_40(table_rows, 6), want["idx"]))
+        else:
+            continue
+        res[label] = bool(ok)
+        del got
+    del inputs
+    return res
 
 
 def serve_suite(requests: int, qps: float):
@@ -294,14 +567,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--cpu-rows", type=int, default=1 << 24)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-arrays batched comparison")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
-    ap.add_argument("--sort", action="store_true",
-                    help="also time the sort and grouping cases (argsort of 2^27 keys, group-by sum "
-                         "over 1K/1M/64M groups at 2^28 rows, scan_by add)")
+    ap.add_argument("--no-sort", action="store_true",
+                    help="skip the sort and grouping cases (argsort of 2^27 keys, group-by sum over "
+                         "1K/1M/64M groups at 2^28 rows, scan_by add)")
+    ap.add_argument("--no-verify", action="store_true",
+                    help="skip checking the results against the oracle after the timed region")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--only", default="",
@@ -391,10 +666,12 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": prims[dom]["GBps"], "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": measured_traffic(dom, args.log2n),
                 "bytes_per_launch": prims[dom]["bytes"], "launch_us": prims[dom]["us"]}
+    checks = None if (args.no_verify or args.only) else verify_streaming(D, calls, n, table_rows)
+    sort = None if (args.no_sort or args.only) else sort_suite(D, verify=not args.no_verify)
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
     serve = None if (args.no_serve or args.only) else serve_suite(args.serve_requests, args.serve_qps)
-    sort = sort_suite(D) if args.sort else None
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_rows, args.cpu_seconds)
+    cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.cpu_rows, args.cpu_seconds)
+    verified = checks is not None and all(checks.values()) and (sort is None or sort["verified"])
     line = {
         "metric": "HBM GB/s per primitive (% of peak); batched-vs-plain speedup; serve p99 ms",
         "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -406,6 +683,8 @@ def main():
                    "rows": n, "table_rows": table_rows},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "verified": verified,
+        "checks": checks,
         "primitives": prims,
         "batched": batched,
         "serve": serve,

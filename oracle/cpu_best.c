@@ -11,10 +11,14 @@
  *   "2 zmm for f64" tiling of the design's canonical order);
  * - filter / compress / where with AVX-512 compress stores (src/kernels/x86/compress/
  *   avx512.rs:48-279): a count pass sizes the output, then vcompresspd per 8 rows;
- * - gather with a vectorised bounds fold and vpgatherqq (src/kernels/x86/gather.rs:44-102).
+ * - gather with a vectorised bounds fold and vpgatherqq (src/kernels/x86/gather.rs:44-102);
+ * - the scalar entries the reference has no intrinsics for: segmented scan_by add
+ *   (scan.rs:282-326), the LSD byte radix argsort (index/radix.rs:123-259) and the hash
+ *   group_id (group.rs:164-257), plus reduce_into_add_f64 in this backend's defined order.
  */
 #include <immintrin.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 /* canonical sum / dot, reduce.rs:40-91 */
@@ -90,10 +94,13 @@ void vkb_scan_add_i64(const int64_t *x, uint64_t n, int64_t *out) {
   for (uint64_t i = 0; i < n; ++i) { acc += (uint64_t)x[i]; out[i] = (int64_t)acc; }
 }
 
-/* filter_gt_f64 with AVX-512 compress, select_loops.rs:105-204 + avx512.rs:144-165 */
+/* filter_gt_f64, select_loops.rs:105-204 + avx512.rs:144-165: a count pass sizes the output
+ * (append_chunks reserves exactly, select.rs:24-57), then AVX-512 compress stores. */
 uint64_t vkb_filter_gt_f64(const double *x, uint64_t n, double c, double *out) {
   const __m512d cv = _mm512_set1_pd(c);
   uint64_t k = 0, i = 0;
+  volatile int64_t sized = vkb_count_gt_f64(x, n, c);
+  (void)sized;
   for (; i + 8 <= n; i += 8) {
     __m512d v = _mm512_loadu_pd(x + i);
     __mmask8 m = _mm512_cmp_pd_mask(v, cv, _CMP_GT_OQ);
@@ -135,5 +142,156 @@ int vkb_gather_f64(const double *xs, uint64_t xlen, const int64_t *idx, uint64_t
   for (i = 0; i + 8 <= n; i += 8)
     _mm512_storeu_pd(out + i, _mm512_i64gather_pd(_mm512_loadu_si512(idx + i), xs, 8));
   for (; i < n; ++i) out[i] = xs[idx[i]];
+  return 0;
+}
+
+/* scan_by_add_i64, scan.rs:282-326: a nonzero start byte restarts the sum (row 0 always). */
+void vkb_scan_by_add_i64(const int64_t *x, const uint8_t *starts, uint64_t n, int64_t *out) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    acc = (i == 0 || starts[i]) ? (uint64_t)x[i] : acc + (uint64_t)x[i];
+    out[i] = (int64_t)acc;
+  }
+}
+
+/* ---------------------------------------------------------------- argsort */
+
+/* radix_into, radix.rs:187-259, with u32 positions (n < 2^32): one counting pass for every
+ * byte digit of the key images, constant digits skipped, (image, position) pairs moved through
+ * two buffers by stable scatters, the last pass writing i64 positions. `kind` picks the image:
+ * 0 u64 (itself), 1 i64 (sign bit flipped, radix.rs:51-101), 2 f64 (this backend's total
+ * order: negative values flip every bit, others the sign bit). */
+typedef struct { uint64_t k; uint32_t p; } vkb_pair;
+
+static inline uint64_t vkb_image(int kind, uint64_t b) {
+  if (kind == 1) return b ^ 0x8000000000000000ull;
+  if (kind == 2) return (b >> 63) ? ~b : b ^ 0x8000000000000000ull;
+  return b;
+}
+
+int vkb_argsort_64(int kind, const uint64_t *x, uint64_t n, int64_t *out) {
+  if (n == 0) return 0;
+  uint64_t (*counts)[256] = calloc(8, sizeof *counts);
+  if (!counts) return -1;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t k = vkb_image(kind, x[i]);
+    for (int d = 0; d < 8; ++d) counts[d][(k >> (8 * d)) & 255]++;
+  }
+  int active[8], passes = 0;
+  uint64_t first = vkb_image(kind, x[0]);
+  for (int d = 0; d < 8; ++d)
+    if (counts[d][(first >> (8 * d)) & 255] != n) active[passes++] = d;
+  if (passes == 0) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = (int64_t)i;
+    free(counts);
+    return 0;
+  }
+  vkb_pair *a = malloc(n * sizeof *a), *b = passes > 2 ? malloc(n * sizeof *b) : NULL;
+  if (!a || (passes > 2 && !b)) { free(a); free(b); free(counts); return -1; }
+  uint64_t off[256];
+  for (int q = 0; q < passes; ++q) {
+    const int d = active[q];
+    uint64_t sum = 0;
+    for (int v = 0; v < 256; ++v) { off[v] = sum; sum += counts[d][v]; }
+    if (q == 0 && passes == 1) {
+      for (uint64_t i = 0; i < n; ++i) out[off[(vkb_image(kind, x[i]) >> (8 * d)) & 255]++] = (int64_t)i;
+    } else if (q == 0) {
+      for (uint64_t i = 0; i < n; ++i) {
+        uint64_t k = vkb_image(kind, x[i]);
+        vkb_pair *dst = &a[off[(k >> (8 * d)) & 255]++];
+        dst->k = k;
+        dst->p = (uint32_t)i;
+      }
+    } else if (q == passes - 1) {
+      for (uint64_t i = 0; i < n; ++i) out[off[(a[i].k >> (8 * d)) & 255]++] = a[i].p;
+    } else {
+      for (uint64_t i = 0; i < n; ++i) b[off[(a[i].k >> (8 * d)) & 255]++] = a[i];
+      vkb_pair *t = a; a = b; b = t;
+    }
+  }
+  free(a); free(b); free(counts);
+  return 0;
+}
+
+/* --------------------------------------------------------------- group_id */
+
+/* probe_into, group.rs:199-257 with mix (group.rs:39-46) and grow (group.rs:164-197): slots
+ * of (key, tag = id + 1, u32), linear probing on the high bits of the hash, a first table of
+ * at most 256 slots that grows right after an insert leaves it more than half full, 2^3-fold
+ * when nearly every row so far opened a group, never past nextpow2(2n). The multiplier is a
+ * fixed odd constant (the reference draws one per call; the ids do not depend on it). */
+typedef struct { uint64_t k; uint32_t tag; } vkb_slot;
+
+static inline uint64_t vkb_mix(uint64_t x) { return (x ^ (x >> 32)) * 0x9E3779B97F4A7C15ull; }
+
+static uint64_t vkb_pow2_at_least(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+int vkb_group_id_64(const uint64_t *x, uint64_t n, int64_t *out) {
+  if (n == 0) return 0;
+  uint64_t max_slots = vkb_pow2_at_least(2 * n);
+  uint64_t slots = max_slots < 256 ? (max_slots < 2 ? 2 : max_slots) : 256;
+  int shift = 64 - __builtin_ctzll(slots);
+  vkb_slot *tab = calloc(slots, sizeof *tab);
+  if (!tab) return -1;
+  uint64_t groups = 0;
+  for (uint64_t row = 0; row < n; ++row) {
+    const uint64_t k = x[row], mask = slots - 1;
+    uint64_t i = vkb_mix(k) >> shift;
+    for (;;) {
+      if (tab[i].tag == 0) {
+        tab[i].k = k;
+        tab[i].tag = (uint32_t)(groups + 1);
+        out[row] = (int64_t)groups++;
+        break;
+      }
+      if (tab[i].k == k) { out[row] = (int64_t)tab[i].tag - 1; break; }
+      i = (i + 1) & mask;
+    }
+    if (2 * groups > slots) {
+      const int distinct = 8 * groups >= 7 * (row + 1);
+      uint64_t grown = slots << (distinct ? 3 : 1);
+      if (grown > max_slots) grown = max_slots;
+      const int new_shift = shift - __builtin_ctzll(grown / slots);
+      vkb_slot *nt = calloc(grown, sizeof *nt);
+      if (!nt) { free(tab); return -1; }
+      for (uint64_t s = 0; s < slots; ++s) {
+        if (tab[s].tag == 0) continue;
+        uint64_t j = vkb_mix(tab[s].k) >> new_shift;
+        while (nt[j].tag != 0) j = (j + 1) & (grown - 1);
+        nt[j] = tab[s];
+      }
+      free(tab);
+      tab = nt;
+      slots = grown;
+      shift = new_shift;
+    }
+  }
+  free(tab);
+  return 0;
+}
+
+/* reduce_into_add_f64 in this backend's defined order (the tier 2 candidate): row i below
+ * b = n - n % 16 folds into stripe i % 16 of its bucket in row order, the 16 stripes of a
+ * bucket combine pairwise, then rows from b add in row order. Keys are in 0..buckets. */
+int vkb_reduce_into_add_f64(const int64_t *keys, const double *vals, uint64_t n, uint64_t buckets, double *out) {
+  double *st = calloc((buckets ? buckets : 1) * 16, sizeof(double));
+  if (!st) return -1;
+  const uint64_t b = n - n % 16;
+  for (uint64_t i = 0; i < b; ++i) {
+    double *s = &st[(uint64_t)keys[i] * 16 + (i & 15)];
+    *s = *s + vals[i];
+  }
+  for (uint64_t k = 0; k < buckets; ++k) {
+    double *a = &st[k * 16];
+    for (int step = 8; step > 0; step /= 2)
+      for (int j = 0; j < step; ++j) a[j] = a[j] + a[j + step];
+    out[k] = a[0];
+  }
+  for (uint64_t i = b; i < n; ++i) out[keys[i]] = out[keys[i]] + vals[i];
+  free(st);
   return 0;
 }

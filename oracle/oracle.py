@@ -485,6 +485,10 @@ def best():
             ("vkb_argmax_f64", i64, [vp, u64]), ("vkb_scan_add_i64", None, [vp, u64, vp]),
             ("vkb_filter_gt_f64", u64, [vp, u64, dbl, vp]), ("vkb_compress_f64", u64, [vp, vp, u64, vp]),
             ("vkb_gather_f64", C.c_int, [vp, u64, vp, u64, vp, vp]),
+            ("vkb_scan_by_add_i64", None, [vp, vp, u64, vp]),
+            ("vkb_argsort_64", C.c_int, [C.c_int, vp, u64, vp]),
+            ("vkb_group_id_64", C.c_int, [vp, u64, vp]),
+            ("vkb_reduce_into_add_f64", C.c_int, [vp, vp, u64, u64, vp]),
         ]:
             f = getattr(L, name)
             f.restype = res

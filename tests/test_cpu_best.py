@@ -42,3 +42,31 @@ def test_cpu_best_matches_oracle(n):
     if n > 3:
         idx[3] = 64
         assert B.vkb_gather_f64(p(t), 64, p(idx), n, p(o), p(bad)) == -1 and bad[0] == 64
+
+
+@pytest.mark.parametrize("n", [0, 1, 17, 300, 5000, 70_000])
+def test_cpu_best_sort_and_group_match_oracle(n):
+    B = O.best()
+    rng = np.random.default_rng(n)
+    for kind, x in [(0, rng.integers(0, 2**64, n, dtype=np.uint64)),
+                    (1, rng.integers(-3, 3, n).astype(np.int64)),
+                    (1, (rng.zipf(1.1, n).clip(max=1 << 20)).astype(np.int64)),
+                    (2, np.where(rng.random(n) < 0.1, -0.0, rng.normal(size=n)))]:
+        out = np.empty(n, np.int64)
+        assert B.vkb_argsort_64(kind, p(np.ascontiguousarray(x)), n, p(out)) == 0
+        assert np.array_equal(out, O.argsort(x)), kind
+    for d in (1, 7, 1000, max(n, 1)):
+        k = rng.integers(-2**63, 2**63 - 1, d, dtype=np.int64)[rng.integers(0, d, n)]
+        ids = np.empty(n, np.int64)
+        assert B.vkb_group_id_64(p(k), n, p(ids)) == 0
+        assert np.array_equal(ids, O.group_id(k)), d
+        g = int(ids.max()) + 1 if n else 0
+        v = O.rand_f64(n, 3)
+        out = np.empty(max(g, 1), np.float64)
+        assert B.vkb_reduce_into_add_f64(p(ids), p(v), n, g, p(out)) == 0
+        assert np.array_equal(out[:g].view(np.uint64), O.reduce_into(ids, v, g).view(np.uint64))
+    x = O.rand_i64(n, 4)
+    st = (rng.random(n) < 1 / 16).astype(np.uint8)
+    out = np.empty(n, np.int64)
+    B.vkb_scan_by_add_i64(p(x), p(st), n, p(out))
+    assert np.array_equal(out, O.scan_by_add_i64(x, st))

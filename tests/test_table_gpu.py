@@ -208,11 +208,19 @@ def fused_lengths(rng):
     return sorted(set([0, 1, 15, 16, 17, 1000] + list(rng.integers(0, 5000, 6)) + [70_001, 1_048_593, 4_194_321]))
 
 
+def finite_mix(rng, n):
+    """Finite values whose products and sums stay finite: -1e3..1e3, and magnitudes from
+    subnormal to 2^200 (Mix::Finite of tests/fused_oracle.rs keeps float sums finite)."""
+    wide = np.ldexp(rng.uniform(-1, 1, n), rng.integers(-1074, 200, n))
+    pick = rng.random(n)
+    return np.where(pick < 0.8, rng.uniform(-1e3, 1e3, n), np.where(pick < 0.95, wide, 0.0))
+
+
 @pytest.mark.parametrize("special", [False, True])
 def test_fused_float_sums_equal_their_patterns(D, special):
     rng = np.random.default_rng(44 + special)
     for n in fused_lengths(rng):
-        x, y = floats(rng, n, special), floats(rng, n, special)
+        x, y = (floats(rng, n, True), floats(rng, n, True)) if special else (finite_mix(rng, n), finite_mix(rng, n))
         dx, dy = D.array(x), D.array(y)
         # dot_f64 = (reduce_add_f64 (mul_f64 ?x ?y)), fused.rs:341-352
         want = D.call("reduce_add_f64", [D.call("mul_f64", [dx, dy])]).value

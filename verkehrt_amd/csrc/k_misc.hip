@@ -126,7 +126,8 @@ __device__ __forceinline__ U cast_as(T x) {
     constexpr double hi = std::is_signed<U>::value ? static_cast<double>(uint64_t(1) << (8 * sizeof(U) - 1))
                                                    : 2.0 * static_cast<double>(uint64_t(1) << (8 * sizeof(U) - 1));
     if (v <= lo) return std::is_signed<U>::value ? static_cast<U>(uint64_t(1) << (8 * sizeof(U) - 1)) : U(0);
-    if (v >= hi) return static_cast<U>(~U(0) >> (std::is_signed<U>::value ? 1 : 0));
+    using UW = typename std::make_unsigned<U>::type;
+    if (v >= hi) return static_cast<U>(static_cast<UW>(~UW(0) >> (std::is_signed<U>::value ? 1 : 0)));
     if constexpr (std::is_signed<U>::value) return static_cast<U>(static_cast<int64_t>(v));
     else return static_cast<U>(static_cast<uint64_t>(v));
   } else if constexpr (std::is_integral<T>::value && std::is_integral<U>::value) {
