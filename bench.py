@@ -177,6 +177,24 @@ def zipf_from_uniform(u: np.ndarray, cdf: np.ndarray) -> np.ndarray:
     return (np.searchsorted(cdf, u, side="right") + 1).astype(np.int64)
 
 
+def zipf_device(D, n: int, seed: int, cdf: np.ndarray):
+    """i64 Zipf keys on the device: for u = rand_f64(n, seed), the least k with u < cdf[k - 1]
+    (zipf_from_uniform), by a binary search of the CDF built from the table's own gather,
+    comparison, select and integer kernels (setup only, never timed)."""
+    from verkehrt_amd import Array, Scalar as S
+    dcdf = Array.from_numpy(D.ctx, cdf)
+    u = D.call("rand_f64", [S("i64", n), S("i64", seed)])
+    lo = D.call("fill_i64", [S("i64", n), S("i64", 0)])
+    hi = D.call("fill_i64", [S("i64", n), S("i64", cdf.size - 1)])
+    steps = int(cdf.size).bit_length() + 1
+    for _ in range(steps):  # invariant: the answer index is in [lo, hi]
+        mid = D.call("div_const_i64", [D.call("add_i64", [lo, hi]), S("i64", 2)])
+        go = D.call("le_f64", [D.call("gather_f64", [dcdf, mid]), u])  # cdf[mid] <= u: answer > mid
+        lo = D.call("select_i64", [go, D.call("add_i64", [mid, S("i64", 1)]), lo])
+        hi = D.call("select_i64", [go, hi, mid])
+    return D.call("add_i64", [lo, S("i64", 1)])
+
+
 def cpu_sort_legs(B, n: int):
     """One core: argsort (u64 uniform, i64 Zipf(1.1) over 2^20 values, f64 normal), group_id +
     reduce_into_add_f64 over 1K, 1M and n/4 groups, and scan_by add, on n-row samples; each
@@ -419,19 +437,8 @@ def sort_suite(D, verify: bool = True):
     checks = {}
     out = {}
     n = 1 << 27
-    # i64 Zipf(1.1) over 1..2^20 on the device: binary search of the CDF for every uniform
     cdf = zipf_cdf()
-    dcdf = Array.from_numpy(ctx, cdf)
-    u = D.call("rand_f64", [S("i64", n), S("i64", 9)])
-    lo = D.call("fill_i64", [S("i64", n), S("i64", 0)])
-    hi = D.call("fill_i64", [S("i64", n), S("i64", cdf.size - 1)])
-    for _ in range(21):  # invariant: the answer index is in [lo, hi]
-        mid = D.call("div_const_i64", [D.call("add_i64", [lo, hi]), S("i64", 2)])
-        go = D.call("le_f64", [D.call("gather_f64", [dcdf, mid]), u])  # cdf[mid] <= u: answer > mid
-        lo = D.call("select_i64", [go, D.call("add_i64", [mid, S("i64", 1)]), lo])
-        hi = D.call("select_i64", [go, hi, mid])
-    zipf = D.call("add_i64", [lo, S("i64", 1)])
-    del lo, hi, mid, go, u, dcdf
+    zipf = zipf_device(D, n, 9, cdf)
     rng = np.random.default_rng(3)
     f = rng.normal(size=n)
     f[rng.integers(0, n, 1000)] = 0.0

@@ -600,24 +600,26 @@ void vko_argsort_f64(const double *x, uint64_t n, int64_t *out) {
  */
 static uint64_t mixh(uint64_t x) { return (x ^ (x >> 32)) * 0x9E3779B97F4A7C15ull; }
 
+typedef struct { uint64_t a, b; int64_t id; } vko_slot;
+
 static void group_ids(const uint64_t *a, const uint64_t *b, uint64_t n, int64_t *out) {
   if (n == 0) return;
   uint64_t slots = 2;
   while (slots < 2 * n) slots <<= 1;
-  uint64_t *ka = (uint64_t *)malloc(slots * 8), *kb = (uint64_t *)malloc(slots * 8);
-  int64_t *id = (int64_t *)malloc(slots * 8);
-  for (uint64_t s = 0; s < slots; ++s) id[s] = -1;
+  vko_slot *tab = (vko_slot *)malloc(slots * sizeof *tab);
+  for (uint64_t s = 0; s < slots; ++s) tab[s].id = -1;
   int64_t next = 0;
   for (uint64_t i = 0; i < n; ++i) {
     uint64_t h = b ? mixh(mixh(a[i]) ^ b[i]) : mixh(a[i]);
     uint64_t s = h & (slots - 1);
     for (;;) {
-      if (id[s] < 0) { ka[s] = a[i]; kb[s] = b ? b[i] : 0; id[s] = next++; out[i] = id[s]; break; }
-      if (ka[s] == a[i] && (!b || kb[s] == b[i])) { out[i] = id[s]; break; }
+      vko_slot *t = &tab[s];
+      if (t->id < 0) { t->a = a[i]; t->b = b ? b[i] : 0; t->id = next++; out[i] = t->id; break; }
+      if (t->a == a[i] && (!b || t->b == b[i])) { out[i] = t->id; break; }
       s = (s + 1) & (slots - 1);
     }
   }
-  free(ka); free(kb); free(id);
+  free(tab);
 }
 
 void vko_group_id_u64(const uint64_t *x, uint64_t n, int64_t *out) { group_ids(x, 0, n, out); }

@@ -203,18 +203,32 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   }
 }
 
+// Look-back scratch of one selection launch: status words (one per tile), their epoch, and a
+// tile ticket that is zero before the launch (the last tile returns it to zero).
+struct SelScratch {
+  uint64_t *status;
+  uint32_t epoch;
+  uint32_t *ticket;
+};
+
 template <typename T, int KIND>
 void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uint64_t cbits,
-            uint64_t base, void *out, uint64_t *d_count) {
+            uint64_t base, void *out, uint64_t *d_count, const SelScratch *given) {
   const uint64_t tiles = (n + kTile - 1) / kTile;
   if (tiles > 0xFFFFFFFFull) usage_error("selection too large");
-  uint64_t *status = status_words(c, ST_SELECT, tiles);
-  const uint32_t epoch = next_epoch(c);
+  SelScratch sc;
+  if (given) {
+    sc = *given;
+  } else {
+    sc.status = status_words(c, ST_SELECT, tiles);
+    sc.epoch = next_epoch(c);
+    sc.ticket = c.scratch.counters;
+  }
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
   k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
-      static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, status, epoch,
-      c.scratch.counters, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);
+      static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch,
+      sc.ticket, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);
   check_hip(hipGetLastError(), "select launch");
 }
 
@@ -249,33 +263,46 @@ void batch_by_width(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, 
 
 template <int KIND>
 void by_width(Ctx &c, int32_t type, const void *x, const uint8_t *mask, uint64_t n, int cmp,
-              uint64_t cbits, uint64_t base, void *out, uint64_t *d_count) {
+              uint64_t cbits, uint64_t base, void *out, uint64_t *d_count, const SelScratch *sc) {
   switch (type) {
-    case VK_F64: launch<double, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
-    case VK_F32: launch<float, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
-    case VK_I64: launch<int64_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
-    case VK_U64: launch<uint64_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
-    case VK_I32: launch<int32_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
-    case VK_U8: case VK_BOOL: launch<uint8_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count); break;
+    case VK_F64: launch<double, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count, sc); break;
+    case VK_F32: launch<float, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count, sc); break;
+    case VK_I64: launch<int64_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count, sc); break;
+    case VK_U64: launch<uint64_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count, sc); break;
+    case VK_I32: launch<int32_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count, sc); break;
+    case VK_U8: case VK_BOOL: launch<uint8_t, KIND>(c, x, mask, n, cmp, cbits, base, out, d_count, sc); break;
     default: usage_error("unsupported selection type");
   }
 }
 
 __global__ void k_zero_count(uint64_t *d) { *d = 0; }
 
-}  // namespace
-
-void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
-                 void *out, uint64_t *d_count) {
+void select_dispatch(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n, void *out,
+                     uint64_t *d_count, const SelScratch *sc) {
   if (n == 0) {
     k_zero_count<<<1, 1, 0, c.stream>>>(d_count);
     return;
   }
   switch (s.kind) {
-    case 0: by_width<0>(c, s.type, x, mask, n, 0, 0, 0, out, d_count); break;
-    case 1: by_width<1>(c, s.type, x, nullptr, n, s.cmp, s.cbits, 0, out, d_count); break;
-    default: launch<uint8_t, 2>(c, nullptr, mask, n, 0, 0, s.base, out, d_count); break;
+    case 0: by_width<0>(c, s.type, x, mask, n, 0, 0, 0, out, d_count, sc); break;
+    case 1: by_width<1>(c, s.type, x, nullptr, n, s.cmp, s.cbits, 0, out, d_count, sc); break;
+    default: launch<uint8_t, 2>(c, nullptr, mask, n, 0, 0, s.base, out, d_count, sc); break;
   }
+}
+
+}  // namespace
+
+void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
+                 void *out, uint64_t *d_count) {
+  select_dispatch(c, s, x, mask, n, out, d_count, nullptr);
+}
+
+uint64_t select_rows_per_tile() { return kTile; }
+
+void select_rows_scratch(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
+                         void *out, uint64_t *d_count, uint64_t *status, uint32_t epoch, uint32_t *ticket) {
+  const SelScratch sc{status, epoch, ticket};
+  select_dispatch(c, s, x, mask, n, out, d_count, &sc);
 }
 
 namespace {

@@ -204,6 +204,12 @@ struct SelectSpec {
 };
 void select_rows(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
                  void *out, uint64_t *d_count);
+// The same launch with the caller's look-back scratch: `status` holds one word per tile of
+// select_rows_per_tile() rows, written with `epoch` (words of other epochs read as not
+// published); `ticket` is zero before the launch and zero again after it.
+uint64_t select_rows_per_tile();
+void select_rows_scratch(Ctx &c, const SelectSpec &s, const void *x, const uint8_t *mask, uint64_t n,
+                         void *out, uint64_t *d_count, uint64_t *status, uint32_t epoch, uint32_t *ticket);
 
 // Batched launches: many independent arrays in one launch per kernel phase. Tiles are
 // numbered over all items; every tile belongs to one item, and look-backs stop at the
