@@ -3,10 +3,10 @@
 - argsort of 2^27 keys: u64 uniform (splitmix64(7 + i)), i64 Zipf(1.1) over 2^20 values (the
   bench's inverse-CDF keys), f64 normal(0,1) with +0.0 and -0.0 injected; the device result
   against the oracle's LSD radix restatement (radix.rs:187-259).
-- group-by sum over i64 keys pool[splitmix64(12 + i) mod G] with G = 1K, 1M and 64M at 2^27
-  rows (half the bench's 2^28, for the oracle's time on one host core): group_id against the
-  oracle's first-occurrence table, reduce_max + 1, then reduce_into_add_f64 of rand_f64 values
-  against the oracle's defined order.
+- group-by sum over i64 keys pool[splitmix64(12 + i) mod G] with G = 1K, 1M and 64M at 2^28
+  rows (the bench's size): group_id against the oracle's first-occurrence table,
+  reduce_max + 1, then reduce_into_add_f64 of rand_f64 values against the oracle's defined
+  order.
 
 Each case is one test, so a slow host shows which case it is.
 """
@@ -60,7 +60,7 @@ def test_argsort_full_size(D, case):
 
 @pytest.mark.parametrize("groups", [1 << 10, 1 << 20, 1 << 26])
 def test_group_sum_full_size(D, groups):
-    n = 1 << 27
+    n = 1 << 28
     pool = D.call("rand_i64", [S("i64", groups), S("i64", 11)])
     sel = D.call("and_i64", [D.call("rand_i64", [S("i64", n), S("i64", 12)]), S("i64", groups - 1)])
     keys = D.call("gather_i64", [pool, sel])
