@@ -196,6 +196,19 @@ int vk_select_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nar
 int vk_elementwise_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs,
                         vk_array *out, vk_status *st);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation): the selection
+ * `name` (filter_<cmp>_<t>, compress_<t>, where) over whole arguments, as vk_call runs it (base
+ * 0), writing the kept rows to the caller's array `out` (of the kernel's output type and at
+ * least as many rows as the input) and their count to *kept. Results equal vk_call's bit for
+ * bit. Its look-back status words, tile ticket and count live in a fixed device array of the
+ * library, one per GPU held until the process ends (about 1 MiB), so the call allocates
+ * nothing: inputs of more than VK_SELECT_INTO_MAX_ROWS rows are rejected. Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. It records no trace line. */
+#define VK_SELECT_INTO_MAX_ROWS (1ull << 31)
+int vk_select_into(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, vk_array *out,
+                   uint64_t *kept, vk_status *st);
+
 /* Releases the array a vk_value holds, if any. */
 void vk_value_release(vk_value *v);
 

@@ -1674,6 +1674,51 @@ int vk_select_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nar
   });
 }
 
+int vk_select_into(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, vk_array *out,
+                   uint64_t *kept, vk_status *st) {
+  return guarded(st, [&] {
+    const Def &d = def_of(name);
+    if (d.cls != VK_SELECTION) usage_error(d.name + " is not a selection");
+    const int64_t rows = check_args(d, args, nargs);
+    const uint64_t n = rows < 0 ? 0 : static_cast<uint64_t>(rows);
+    if (n > VK_SELECT_INTO_MAX_ROWS)
+      usage_error("vk_select_into takes at most " + std::to_string(VK_SELECT_INTO_MAX_ROWS) + " rows, got " +
+                  std::to_string(n));
+    if (!out || out->type != d.out_t)
+      usage_error(d.name + " produces " + tn(d.out_t) + " rows, got a " + (out ? tn(out->type) : std::string("null")) +
+                  " destination");
+    if (out->len < n)
+      usage_error("destination of " + std::to_string(out->len) + " rows is shorter than the input's " +
+                  std::to_string(n) + " rows");
+    if (!kept) usage_error("no place for the kept count");
+    Ctx &c = *ctx;
+    const IntoScratch sc = select_into_scratch(c, n);
+    SelectSpec s{};
+    const void *x = nullptr;
+    const uint8_t *mask = nullptr;
+    if (d.impl == S_COMPRESS) {
+      s.kind = 0;
+      s.type = d.t0;
+      x = dptr(args[0].rows);
+      mask = static_cast<const uint8_t *>(dptr(args[1].rows));
+    } else if (d.impl == S_FILTER) {
+      s.kind = 1;
+      s.type = d.t0;
+      s.cmp = d.p0;
+      s.cbits = args[1].splat.bits;
+      x = dptr(args[0].rows);
+    } else {
+      s.kind = 2;
+      s.type = VK_I64;
+      mask = static_cast<const uint8_t *>(dptr(args[0].rows));
+    }
+    select_rows_scratch(c, s, x, mask, n, dptr(out), sc.count, sc.status, sc.epoch, sc.ticket);
+    uint64_t got = 0;
+    read_back(c, sc.count, 8, &got);
+    *kept = got;
+  });
+}
+
 int vk_elementwise_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, vk_array *out,
                         vk_status *st) {
   return guarded(st, [&] {

@@ -300,6 +300,7 @@ Ctx *create_ctx(int device) {
   c->scratch.slots = static_cast<uint64_t *>(slots);
   fill_u64(*c, static_cast<uint64_t *>(counters), 256 / 8, 0);
   check_hip(hipStreamSynchronize(c->stream), "stream sync");
+  select_into_setup(*c);
   return c;
 }
 

@@ -303,6 +303,18 @@ void scan_add_f64(Ctx &c, const double *x, uint64_t n, double *out, double *stat
 void prev_rows(Ctx &c, uint32_t width, const void *x, uint64_t n, void *out, uint64_t *carry,
                uint64_t fill_bits);
 
+// The allocation-free selection's scratch (noalloc.hip): select_into_setup resolves this
+// device's copy of the fixed device array once (at context creation, with the device current);
+// select_into_scratch hands a call its ticket, count slot and status words with a new epoch.
+struct IntoScratch {
+  uint32_t *ticket;
+  uint64_t *count;
+  uint64_t *status;
+  uint32_t epoch;
+};
+void select_into_setup(Ctx &c);
+IntoScratch select_into_scratch(Ctx &c, uint64_t rows);
+
 // Gather (k_gather.hip): out[i] = xs[idx[i]]; the first bad row (or ~0) to *d_bad.
 void gather_rows(Ctx &c, uint32_t width, const void *xs, uint64_t xlen, const int64_t *idx,
                  uint64_t n, void *out, uint64_t *d_bad);

@@ -440,6 +440,16 @@ class Dispatch:
         _check(self._lib.vk_select_tile(self.ctx.handle, name.encode(), a, len(args), base,
                                         out.handle, C.byref(st)), st)
 
+    def select_into(self, name: str, args: Sequence[Value], out: Array) -> int:
+        """The selection written into the caller's array `out` with no device allocation
+        (vk_select_into); returns the kept row count."""
+        a = _args(args)
+        kept = C.c_uint64()
+        st = N.Status()
+        _check(self._lib.vk_select_into(self.ctx.handle, name.encode(), a, len(args), out.handle,
+                                        C.byref(kept), C.byref(st)), st)
+        return kept.value
+
     def elementwise_tile(self, name: str, args: Sequence[Value], out: Array):
         a = _args(args)
         st = N.Status()

@@ -85,6 +85,7 @@ SIGNATURES = {
     "vk_builder_free": (None, [_P]),
     "vk_select_tile": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, C.c_uint64, _P, _ST]),
     "vk_elementwise_tile": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, _P, _ST]),
+    "vk_select_into": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, _P, C.POINTER(C.c_uint64), _ST]),
     "vk_value_release": (None, [C.POINTER(CValue)]),
     "vk_batch_call": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, C.c_size_t,
                                 C.POINTER(CValue), _ST]),
