@@ -209,6 +209,29 @@ int vk_elementwise_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_
 int vk_select_into(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nargs, vk_array *out,
                    uint64_t *kept, vk_status *st);
 
+/* A fused region (src/exec/region.rs:1-60, src/exec/region/fused.rs:96-204): elementwise
+ * kernels in post-order, each argument a region input or an earlier kernel's result, feeding
+ * one reduction `root` (whose arguments are sources too). The result equals evaluating every
+ * kernel with vk_call and then the root, bit for bit, and so do the errors. When the kernels
+ * form one chain over at most two input arrays (each reads the previous result and a constant,
+ * an input array or that result again; y only under a canonical sum) and the root is
+ * reduce_add_f64, an integer sum, an extreme, an arg extreme, count_<cmp>_<t>, or count over a
+ * final comparison with a constant, the region runs as one device pass with the chain in
+ * registers and *fused is set to 1; otherwise it runs kernel by kernel and *fused is 0. */
+enum { VK_SRC_INPUT = 0, VK_SRC_KERNEL = 1 };
+typedef struct vk_src {
+  int32_t kind;
+  int32_t index;
+} vk_src;
+typedef struct vk_rkernel {
+  const char *name;
+  vk_src args[3];
+  int32_t nargs;
+} vk_rkernel;
+int vk_region_reduce(vk_ctx *ctx, const vk_rkernel *kernels, size_t nkernels, const char *root,
+                     const vk_src *root_args, size_t nroot, const vk_arg *inputs, size_t ninputs,
+                     vk_scalar *out, int32_t *fused, vk_status *st);
+
 /* Releases the array a vk_value holds, if any. */
 void vk_value_release(vk_value *v);
 

@@ -60,6 +60,9 @@ def _declare(L):
         "vko_count_cmp_f64": (i64, [vp, u64, i32, dbl]),
         "vko_count_cmp_i64": (i64, [vp, u64, i32, i64]),
         "vko_count_bool": (i64, [vp, u64]),
+        "vko_count_cmp_u64": (i64, [vp, u64, i32, u64]),
+        "vko_reduce_extreme_u64": (i32, [vp, u64, i32, vp]),
+        "vko_arg_extreme_u64": (i32, [vp, u64, i32, vp]),
         "vko_min2_f64": (dbl, [dbl, dbl]),
         "vko_max2_f64": (dbl, [dbl, dbl]),
         "vko_reduce_extreme_f64": (i32, [vp, u64, i32, vp]),
@@ -197,6 +200,8 @@ def count_cmp(x, cmp: str, c) -> int:
     x = np.ascontiguousarray(x)
     if x.dtype == np.float64:
         return lib().vko_count_cmp_f64(_p(x), x.size, CMPS[cmp], float(c))
+    if x.dtype == np.uint64:
+        return lib().vko_count_cmp_u64(_p(x), x.size, CMPS[cmp], int(c))
     x = _c(x, np.int64)
     return lib().vko_count_cmp_i64(_p(x), x.size, CMPS[cmp], int(c))
 
@@ -222,6 +227,9 @@ def reduce_extreme(x, want_max: bool):
     if x.dtype == np.float64:
         out = np.zeros(1, np.float64)
         r = lib().vko_reduce_extreme_f64(_p(x), x.size, int(want_max), _p(out))
+    elif x.dtype == np.uint64:
+        out = np.zeros(1, np.uint64)
+        r = lib().vko_reduce_extreme_u64(_p(x), x.size, int(want_max), _p(out))
     else:
         x = _c(x, np.int64)
         out = np.zeros(1, np.int64)
@@ -236,6 +244,8 @@ def arg_extreme(x, want_max: bool) -> int:
     out = np.zeros(1, np.int64)
     if x.dtype == np.float64:
         r = lib().vko_arg_extreme_f64(_p(x), x.size, int(want_max), _p(out))
+    elif x.dtype == np.uint64:
+        r = lib().vko_arg_extreme_u64(_p(x), x.size, int(want_max), _p(out))
     else:
         x = _c(x, np.int64)
         r = lib().vko_arg_extreme_i64(_p(x), x.size, int(want_max), _p(out))

@@ -244,6 +244,21 @@ int64_t vko_count_cmp_i64(const int64_t *x, uint64_t n, int cmp, int64_t c) {
   for (uint64_t i = 0; i < n; ++i) k += cmp_i64(cmp, x[i], c);
   return k;
 }
+static inline int cmp_u64(int cmp, uint64_t a, uint64_t b) {
+  switch (cmp) {
+    case 0: return a == b;
+    case 1: return a != b;
+    case 2: return a < b;
+    case 3: return a <= b;
+    case 4: return a > b;
+    default: return a >= b;
+  }
+}
+int64_t vko_count_cmp_u64(const uint64_t *x, uint64_t n, int cmp, uint64_t c) {
+  int64_t k = 0;
+  for (uint64_t i = 0; i < n; ++i) k += cmp_u64(cmp, x[i], c);
+  return k;
+}
 /* BoolCount, reduce.rs:527-529: bytes != 0 */
 int64_t vko_count_bool(const uint8_t *m, uint64_t n) {
   int64_t k = 0;
@@ -282,6 +297,15 @@ int vko_reduce_extreme_i64(const int64_t *x, uint64_t n, int want_max, int64_t *
   return 0;
 }
 
+int vko_reduce_extreme_u64(const uint64_t *x, uint64_t n, int want_max, uint64_t *out) {
+  if (n == 0) return -1;
+  uint64_t cur = x[0];
+  for (uint64_t i = 1; i < n; ++i)
+    cur = want_max ? (x[i] > cur ? x[i] : cur) : (x[i] < cur ? x[i] : cur);
+  *out = cur;
+  return 0;
+}
+
 /* Ordered::less / greater for floats, reduce.rs:183-199: -0.0 ranks below +0.0 */
 static inline int less_f64(double a, double b) {
   return a < b || (a == b && signbit(a) && !signbit(b));
@@ -306,6 +330,15 @@ int vko_arg_extreme_f64(const double *x, uint64_t n, int want_max, int64_t *out)
   return 0;
 }
 int vko_arg_extreme_i64(const int64_t *x, uint64_t n, int want_max, int64_t *out) {
+  if (n == 0) return -1;
+  int64_t best = 0;
+  for (uint64_t i = 1; i < n; ++i)
+    if (want_max ? x[i] > x[best] : x[i] < x[best]) best = (int64_t)i;
+  *out = best;
+  return 0;
+}
+
+int vko_arg_extreme_u64(const uint64_t *x, uint64_t n, int want_max, int64_t *out) {
   if (n == 0) return -1;
   int64_t best = 0;
   for (uint64_t i = 1; i < n; ++i)

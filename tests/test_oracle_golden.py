@@ -195,6 +195,23 @@ def test_count_cmp_i64_brute_force():
     assert O.count_cmp(np.array([], np.int64), "eq", 0) == 0
 
 
+def test_u64_extremes_and_counts_compare_unsigned():
+    # u64 rows order as unsigned integers (src/kernels/tests.rs:1322-1338: max 1 << 63)
+    x = np.array([3, 1 << 63, 5, 2**64 - 1, 3], np.uint64)
+    assert O.reduce_extreme(x, True) == 2**64 - 1 and O.reduce_extreme(x, False) == 3
+    assert O.arg_extreme(x, True) == 3 and O.arg_extreme(x, False) == 0
+    rng = np.random.default_rng(13)
+    y = np.concatenate([x, rng.integers(0, 2**64, 300, dtype=np.uint64)])
+    Y = [int(v) for v in y]
+    assert O.reduce_extreme(y, True) == max(Y) and O.reduce_extreme(y, False) == min(Y)
+    assert O.arg_extreme(y, True) == Y.index(max(Y)) and O.arg_extreme(y, False) == Y.index(min(Y))
+    ops = {"eq": lambda a, b: a == b, "ne": lambda a, b: a != b, "lt": lambda a, b: a < b,
+           "le": lambda a, b: a <= b, "gt": lambda a, b: a > b, "ge": lambda a, b: a >= b}
+    for cmp, f in ops.items():
+        for c in [0, 3, 1 << 63, 2**64 - 1]:
+            assert O.count_cmp(y, cmp, c) == sum(1 for v in Y if f(v, c)), (cmp, c)
+
+
 def test_integer_products_wrap():
     # src/kernels/tests.rs:847-850,1337: reduce_mul wraps in the element type, identity 1
     assert O.product_int(np.array([16, 17], np.uint8)) == 16

@@ -810,6 +810,223 @@ vk_value call(Ctx &c, const Def &d, const vk_arg *args, size_t nargs) {
   }
 }
 
+// ------------------------------------------------------------------ regions
+// A fused region (src/exec/region.rs:1-60, region/fused.rs:96-204): elementwise kernels in
+// post-order feeding one reduction. When the kernels form one chain over at most two input
+// arrays (each kernel reads the previous one's result and a constant, an input array or that
+// result again) and the root is a sum, extreme, arg extreme or count, the whole region is one
+// device pass with the chain in registers; otherwise it runs kernel by kernel. Both give the
+// results of the kernel-by-kernel evaluation bit for bit.
+
+struct RegionPlan {
+  ChainSpec ch{};
+  const vk_array *x = nullptr, *y = nullptr;
+  const Def *root = nullptr;
+  ReduceSpec rs{};
+  bool canon = false;
+};
+
+bool same_view(const vk_array *a, const vk_array *b) {
+  return a == b || (a->buf == b->buf && a->offset == b->offset && a->len == b->len && a->type == b->type);
+}
+
+int cmp_swapped(int cmp) {  // c <cmp> v as v <cmp'> c
+  switch (cmp) {
+    case CMP_LT: return CMP_GT;
+    case CMP_LE: return CMP_GE;
+    case CMP_GT: return CMP_LT;
+    case CMP_GE: return CMP_LE;
+    default: return cmp;
+  }
+}
+
+bool region_compile(const vk_rkernel *ks, size_t nk, const char *root, const vk_src *rargs, size_t nr,
+                    const vk_arg *in, size_t nin, RegionPlan &p) {
+  build_table();
+  auto rit = g_by_name.find(root ? root : "");
+  if (rit == g_by_name.end() || nk == 0 || nk > size_t(kChainMax) + 1) return false;
+  const Def &rd = g_defs[rit->second];
+  if (rd.cls != VK_REDUCTION) return false;
+  const bool root_ok = (rd.impl == R_CANON && rd.p0 == 0) || rd.impl == R_SUM_INT || rd.impl == R_EXTREME ||
+                       rd.impl == R_ARG || rd.impl == R_COUNT_CMP || rd.impl == R_COUNT_BOOL;
+  if (!root_ok || nr != rd.in.size()) return false;
+  if (rargs[0].kind != VK_SRC_KERNEL || rargs[0].index != int32_t(nk) - 1) return false;
+  for (size_t i = 1; i < nr; ++i) {  // count_<cmp>_<t>'s constant
+    if (rargs[i].kind != VK_SRC_INPUT || rargs[i].index < 0 || size_t(rargs[i].index) >= nin) return false;
+    if (!accepts(rd.in[i], in[rargs[i].index]) || in[rargs[i].index].kind != VK_SPLAT) return false;
+  }
+  int32_t T = -1;
+  int nops = 0;
+  bool count_cmp = false;
+  for (size_t j = 0; j < nk; ++j) {
+    auto it = g_by_name.find(ks[j].name ? ks[j].name : "");
+    if (it == g_by_name.end()) return false;
+    const Def &d = g_defs[it->second];
+    if (d.cls != VK_ELEMENTWISE || ks[j].nargs < 1 || ks[j].nargs > 3 || size_t(ks[j].nargs) != d.in.size()) return false;
+    const bool cmp = d.impl == E_BINARY && d.p0 >= EW_EQ && d.p0 <= EW_GE;
+    const bool arith = (d.impl == E_BINARY && (d.p0 == EW_ADD || d.p0 == EW_SUB || d.p0 == EW_MUL || d.p0 == EW_DIV ||
+                                               d.p0 == EW_MIN || d.p0 == EW_MAX || d.p0 == EW_BAND ||
+                                               d.p0 == EW_BOR || d.p0 == EW_BXOR)) ||
+                       (d.impl == E_UNARY && (d.p0 == EW_NEG || d.p0 == EW_ABS || d.p0 == EW_SQRT || d.p0 == EW_BNOT));
+    if (!(arith || (cmp && j + 1 == nk && rd.impl == R_COUNT_BOOL))) return false;
+    if (d.t0 == VK_BOOL) return false;
+    if (T < 0) T = d.t0;
+    if (d.t0 != T || (!cmp && d.out_t != T)) return false;
+    // classify the arguments: the chain value, a constant, or an input array
+    int chain_at = -1, other_at = -1;
+    bool self = false;
+    for (int a = 0; a < ks[j].nargs; ++a) {
+      const vk_src &sr = ks[j].args[a];
+      if (sr.kind == VK_SRC_KERNEL) {
+        if (j == 0 || sr.index != int32_t(j) - 1) return false;
+        if (chain_at >= 0) self = true;
+        else chain_at = a;
+        continue;
+      }
+      if (sr.kind != VK_SRC_INPUT || sr.index < 0 || size_t(sr.index) >= nin) return false;
+      if (!accepts(d.in[a], in[sr.index])) return false;
+    }
+    ChainOp op{};
+    if (j == 0) {
+      // the chain starts at the first array argument, which becomes x
+      for (int a = 0; a < ks[j].nargs && chain_at < 0; ++a)
+        if (in[ks[j].args[a].index].kind == VK_ROWS) chain_at = a;
+      if (chain_at < 0) return false;
+      p.x = in[ks[j].args[chain_at].index].rows;
+    } else if (chain_at < 0) {
+      return false;
+    }
+    if (d.in.size() == 2) other_at = 1 - chain_at;
+    if (self) {
+      op.src = 3;
+    } else if (other_at >= 0) {
+      const vk_arg &o = in[ks[j].args[other_at].index];
+      if (o.kind == VK_SPLAT) {
+        op.src = 0;
+        op.kbits = o.splat.bits;
+      } else if (same_view(o.rows, p.x)) {
+        op.src = 1;
+      } else if (!p.y || same_view(o.rows, p.y)) {
+        p.y = o.rows;
+        op.src = 2;
+      } else {
+        return false;
+      }
+    }
+    op.swap = chain_at == 1 ? 1 : 0;
+    if (cmp) {
+      // count(<cmp>(chain, c)): the root counts the chain's rows that pass the comparison
+      if (op.src != 0) return false;
+      p.rs = ReduceSpec{};
+      p.rs.kind = ReduceSpec::COUNT_CMP;
+      p.rs.type = T;
+      p.rs.cmp = op.swap ? cmp_swapped(d.p0 - EW_EQ) : d.p0 - EW_EQ;
+      p.rs.cbits = op.kbits;
+      count_cmp = true;
+      continue;
+    }
+    op.op = d.p0;
+    if (nops >= kChainMax) return false;
+    p.ch.ops[nops++] = op;
+  }
+  p.ch.n = nops;
+  p.ch.type = T;
+  if (p.y && p.y->len != p.x->len) return false;
+  p.root = &rd;
+  if (rd.impl == R_COUNT_BOOL) {
+    if (!count_cmp) return false;
+  } else {
+    if (rd.t0 != T && !(rd.impl == R_CANON && T == VK_F64)) return false;
+    if (rd.impl == R_CANON) {
+      p.canon = true;
+      return T == VK_F64;
+    }
+    p.rs = reduce_spec(rd, 0);
+    if (rd.impl == R_COUNT_CMP) p.rs.cbits = in[rargs[1].index].splat.bits;
+  }
+  return p.y == nullptr;  // the streaming reductions read one input
+}
+
+// The kernel-by-kernel evaluation of a region: every kernel as a whole call, then the root.
+vk_scalar region_eval(Ctx &c, const vk_rkernel *ks, size_t nk, const char *root, const vk_src *rargs, size_t nr,
+                      const vk_arg *in, size_t nin) {
+  std::vector<vk_value> vals;
+  auto release_all = [&] {
+    for (auto &v : vals)
+      if (v.kind == VK_VALUE_ARRAY && v.array) release(v.array);
+  };
+  auto arg_of = [&](const vk_src &sr, size_t upto) -> vk_arg {
+    if (sr.kind == VK_SRC_INPUT) {
+      if (sr.index < 0 || size_t(sr.index) >= nin) usage_error("region input " + std::to_string(sr.index) + " out of range");
+      return in[sr.index];
+    }
+    if (sr.kind != VK_SRC_KERNEL || sr.index < 0 || size_t(sr.index) >= upto)
+      usage_error("region kernel " + std::to_string(sr.index) + " is not an earlier kernel");
+    const vk_value &v = vals[sr.index];
+    vk_arg a{};
+    if (v.kind == VK_VALUE_ARRAY) {
+      a.kind = VK_ROWS;
+      a.rows = v.array;
+    } else {
+      a.kind = VK_SPLAT;
+      a.splat = v.scalar;
+    }
+    return a;
+  };
+  try {
+    for (size_t j = 0; j < nk; ++j) {
+      const Def &d = def_of(ks[j].name);
+      if (d.cls != VK_ELEMENTWISE) usage_error(d.name + " is not an elementwise kernel");
+      std::vector<vk_arg> a;
+      for (int k = 0; k < ks[j].nargs && k < 3; ++k) a.push_back(arg_of(ks[j].args[k], j));
+      vals.push_back(call(c, d, a.data(), a.size()));
+    }
+    const Def &rd = def_of(root);
+    if (rd.cls != VK_REDUCTION) usage_error(rd.name + " is not a reduction");
+    std::vector<vk_arg> a;
+    for (size_t k = 0; k < nr; ++k) a.push_back(arg_of(rargs[k], nk));
+    vk_value r = call(c, rd, a.data(), a.size());
+    release_all();
+    return r.scalar;
+  } catch (...) {
+    release_all();
+    throw;
+  }
+}
+
+vk_scalar region_run(Ctx &c, const RegionPlan &p) {
+  const uint64_t n = p.x->len;
+  const void *x = dptr(p.x);
+  const void *y = p.y ? dptr(p.y) : nullptr;
+  void *state = tmp_alloc(c, 256);
+  vk_scalar out{};
+  try {
+    if (p.canon) {
+      double *acc = static_cast<double *>(state);
+      canon_init(c, false, acc);
+      const uint64_t blocks = n / 16, rest = n % 16;
+      const double *xd = static_cast<const double *>(x), *yd = static_cast<const double *>(y);
+      canon_blocks_chain(c, p.ch, xd, yd, blocks, acc);
+      canon_rest_chain(c, p.ch, xd + blocks * 16, yd ? yd + blocks * 16 : nullptr, rest, acc + 16);
+      canon_finish(c, false, acc, acc + 16, static_cast<int>(rest), reinterpret_cast<double *>(c.scratch.slots));
+      uint64_t b;
+      read_back(c, c.scratch.slots, 8, &b);
+      out = scalar(VK_F64, b);
+    } else {
+      reduce_state_init(c, p.rs, state);
+      reduce_rows_chain(c, p.rs, p.ch, x, n, state);
+      uint64_t w[3];
+      read_back(c, state, 24, w);
+      out = finish_words(*p.root, w);
+    }
+  } catch (...) {
+    tmp_free(c, state);
+    throw;
+  }
+  tmp_free(c, state);
+  return out;
+}
+
 // ------------------------------------------------------------------ batching
 
 std::vector<vk_value> batch_call(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, size_t nitems) {
@@ -1671,6 +1888,27 @@ int vk_select_tile(vk_ctx *ctx, const char *name, const vk_arg *args, size_t nar
     const Def &d = def_of(name);
     if (d.cls != VK_SELECTION) usage_error(d.name + " is not a selection");
     select_tile(*ctx, d, args, nargs, base, out);
+  });
+}
+
+int vk_region_reduce(vk_ctx *ctx, const vk_rkernel *kernels, size_t nkernels, const char *root,
+                     const vk_src *root_args, size_t nroot, const vk_arg *inputs, size_t ninputs, vk_scalar *out,
+                     int32_t *fused, vk_status *st) {
+  return guarded(st, [&] {
+    RegionPlan p;
+    const bool one = region_compile(kernels, nkernels, root, root_args, nroot, inputs, ninputs, p);
+    if (fused) *fused = one ? 1 : 0;
+    // one trace line for a fused pass, named by its root; a kernel-by-kernel run traces each call
+    TraceScope ts(*ctx, root ? root : "region", nullptr, 0);
+    if (one) {
+      ts.n = p.x->len;
+      ts.read = abytes(p.x) + (p.y ? abytes(p.y) : 0);
+      *out = region_run(*ctx, p);
+      ts.done(0);
+    } else {
+      ts.on = false;
+      *out = region_eval(*ctx, kernels, nkernels, root, root_args, nroot, inputs, ninputs);
+    }
   });
 }
 

@@ -8,6 +8,7 @@
 // same 16 chains in parallel with binade run summaries (canon_binade.h) and returns the same
 // bits; it is HBM-bound on data whose running sums stay inside a binade for long stretches.
 #include "canon_binade.h"
+#include "chain.h"
 #include "vk_device.h"
 #include "vk_internal.h"
 
@@ -1350,6 +1351,36 @@ void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, ui
   if (rest == 0) return;
   with_row(s, [&](auto rv) { k_canon_rest<<<1, 64, 0, c.stream>>>(rv, x, y ? y : x, rest, tail); });
   check_hip(hipGetLastError(), "canon rest");
+}
+
+namespace {
+
+// The row value of a fused region: the chain over rows x (and y when TWO), in registers.
+template <bool TWO>
+struct RowChain {
+  static constexpr int kSummPasses = TWO ? 1 : 2;
+  static constexpr int kSummWaves = TWO ? 1 : 6;
+  ChainSpec ch;
+  __device__ __forceinline__ double operator()(const double *x, const double *y, uint64_t i) const {
+    return chain_eval<double>(ch, x[i], TWO ? y[i] : 0.0);
+  }
+};
+
+}  // namespace
+
+void canon_blocks_chain(Ctx &c, const ChainSpec &ch, const double *x, const double *y, uint64_t blocks,
+                        double *acc) {
+  if (blocks == 0) return;
+  if (y) canon_blocks_as(c, RowChain<true>{ch}, false, x, y, blocks, acc);
+  else canon_blocks_as(c, RowChain<false>{ch}, false, x, x, blocks, acc);
+}
+
+void canon_rest_chain(Ctx &c, const ChainSpec &ch, const double *x, const double *y, uint64_t rest,
+                      double *tail) {
+  if (rest == 0) return;
+  if (y) k_canon_rest<<<1, 64, 0, c.stream>>>(RowChain<true>{ch}, x, y, rest, tail);
+  else k_canon_rest<<<1, 64, 0, c.stream>>>(RowChain<false>{ch}, x, x, rest, tail);
+  check_hip(hipGetLastError(), "canon rest (chain)");
 }
 
 void canon_finish(Ctx &c, bool mul, const double *acc, const double *tail, int tail_len,

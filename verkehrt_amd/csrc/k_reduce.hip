@@ -8,6 +8,7 @@
 // per thread, per wave (shuffles) and per block (LDS) into one partial per block; a second,
 // one-block launch folds the partials into the reducer's device state.
 // HBM-bound: algorithmic bytes = n x width.
+#include "chain.h"
 #include "vk_device.h"
 #include "vk_internal.h"
 
@@ -207,6 +208,21 @@ template <class Op, class Acc, typename T>
 __device__ __forceinline__ void add_inorder(const Op &op, Acc &a, T v, uint64_t row, long) {
   op.add(a, v, row);
 }
+
+// A fused region's root: the inner operator over the chain's values of the rows (ChainSpec).
+// The chain runs in registers on each loaded row, so no intermediate array is written.
+template <class Inner, typename T>
+struct OpChain {
+  using Acc = typename Inner::Acc;
+  Inner in;
+  ChainSpec ch;
+  __device__ Acc identity() const { return in.identity(); }
+  __device__ void add(Acc &a, T v, uint64_t row) const { in.add(a, chain_eval<T>(ch, v, v), row); }
+  __device__ void add_inorder(Acc &a, T v, uint64_t row) const {
+    vk::add_inorder(in, a, chain_eval<T>(ch, v, v), row, 0);
+  }
+  __device__ void merge(Acc &a, const Acc &b) const { in.merge(a, b); }
+};
 
 // ------------------------------------------------------------------- the kernel
 
@@ -411,6 +427,7 @@ void with_op(const ReduceSpec &s, F &&f) {
         case VK_I64: f(make_count<int64_t>(s.cmp, s.cbits), static_cast<int64_t *>(nullptr)); break;
         case VK_U64: f(make_count<uint64_t>(s.cmp, s.cbits), static_cast<uint64_t *>(nullptr)); break;
         case VK_I32: f(make_count<int32_t>(s.cmp, s.cbits), static_cast<int32_t *>(nullptr)); break;
+        case VK_U8: f(make_count<uint8_t>(s.cmp, s.cbits), static_cast<uint8_t *>(nullptr)); break;
         default: usage_error("unsupported count type");
       }
       break;
@@ -499,6 +516,14 @@ void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *s
   with_op(s, [&](auto op, auto *tag) {
     using T = typename std::remove_pointer<decltype(tag)>::type;
     launch<T>(c, op, x, n, state);
+  });
+}
+
+void reduce_rows_chain(Ctx &c, const ReduceSpec &s, const ChainSpec &ch, const void *x, uint64_t n, void *state) {
+  if (n == 0) return;
+  with_op(s, [&](auto op, auto *tag) {
+    using T = typename std::remove_pointer<decltype(tag)>::type;
+    launch<T>(c, OpChain<decltype(op), T>{op, ch}, x, n, state);
   });
 }
 

@@ -380,6 +380,33 @@ struct EwSpec {
   uint64_t a_bits, b_bits, c_bits;
 };
 void elementwise(Ctx &c, const EwSpec &s, const void *a, const void *b, const void *e, uint64_t n, void *out);
+
+// A fused region's elementwise chain (one register, no intermediate arrays): v starts as row x,
+// then each op combines v with its operand, a constant (src 0), row x (1), row y (2) or v
+// itself (3); `swap` puts the operand on the left (k - v). Ops are EwOp arithmetic codes of
+// the chain's element type; every op rounds (or wraps) exactly as its elementwise kernel does.
+constexpr int kChainMax = 8;
+struct ChainOp {
+  int32_t op;
+  int32_t src;
+  int32_t swap;
+  int32_t pad;
+  uint64_t kbits;  // the constant's bits
+};
+struct ChainSpec {
+  int32_t n;
+  int32_t type;  // element type of x, y and every op
+  ChainOp ops[kChainMax];
+};
+// Canonical sum (reduce_add_f64 order) of the chain's f64 row values: accumulators and tail
+// exactly as canon_blocks / canon_rest update them; y may be null when no op reads it.
+void canon_blocks_chain(Ctx &c, const ChainSpec &ch, const double *x, const double *y, uint64_t blocks,
+                        double *acc);
+void canon_rest_chain(Ctx &c, const ChainSpec &ch, const double *x, const double *y, uint64_t rest,
+                      double *tail);
+// A streaming reduction (sum, count, extreme, arg) of the chain's values over one input x
+// (no op reads y), merged into the reducer state as reduce_rows does.
+void reduce_rows_chain(Ctx &c, const ReduceSpec &s, const ChainSpec &ch, const void *x, uint64_t n, void *state);
 void copy_d2d(Ctx &c, void *dst, const void *src, uint64_t bytes);
 
 }  // namespace vk

@@ -440,6 +440,31 @@ class Dispatch:
         _check(self._lib.vk_select_tile(self.ctx.handle, name.encode(), a, len(args), base,
                                         out.handle, C.byref(st)), st)
 
+    def region_reduce(self, kernels, root: str, root_args, inputs: Sequence[Value]):
+        """A fused region (vk_region_reduce): `kernels` is a list of (name, sources) in
+        post-order, `root` a reduction over `root_args`; a source is ("in", i) for inputs[i] or
+        ("k", j) for kernel j's result. Returns (result, fused) where fused tells whether the
+        region ran as one device pass."""
+        ks = (N.CRKernel * max(1, len(kernels)))()
+        names = []
+        def src(s):
+            kind, idx = s
+            return N.CSrc(N.SRC_INPUT if kind == "in" else N.SRC_KERNEL, idx)
+        for i, (name, srcs) in enumerate(kernels):
+            names.append(name.encode())
+            ks[i].name = names[-1]
+            ks[i].nargs = len(srcs)
+            for j, s_ in enumerate(srcs[:3]):
+                ks[i].args[j] = src(s_)
+        ra = (N.CSrc * max(1, len(root_args)))(*[src(s_) for s_ in root_args])
+        a = _args(inputs)
+        out = N.CScalar()
+        fused = C.c_int32()
+        st = N.Status()
+        _check(self._lib.vk_region_reduce(self.ctx.handle, ks, len(kernels), root.encode(), ra, len(root_args), a,
+                                          len(inputs), C.byref(out), C.byref(fused), C.byref(st)), st)
+        return Scalar.from_c(out), bool(fused.value)
+
     def select_into(self, name: str, args: Sequence[Value], out: Array) -> int:
         """The selection written into the caller's array `out` with no device allocation
         (vk_select_into); returns the kept row count."""

@@ -35,6 +35,15 @@ class CValue(C.Structure):
 
 
 BREF_INPUT, BREF_STMT, BREF_SCALAR = 0, 1, 2
+SRC_INPUT, SRC_KERNEL = 0, 1
+
+
+class CSrc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_int32)]
+
+
+class CRKernel(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("args", CSrc * 3), ("nargs", C.c_int32)]
 
 
 class CBref(C.Structure):
@@ -85,6 +94,8 @@ SIGNATURES = {
     "vk_builder_free": (None, [_P]),
     "vk_select_tile": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, C.c_uint64, _P, _ST]),
     "vk_elementwise_tile": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, _P, _ST]),
+    "vk_region_reduce": (C.c_int, [_P, C.POINTER(CRKernel), C.c_size_t, C.c_char_p, C.POINTER(CSrc), C.c_size_t,
+                                   C.POINTER(CArg), C.c_size_t, C.POINTER(CScalar), C.POINTER(C.c_int32), _ST]),
     "vk_select_into": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, _P, C.POINTER(C.c_uint64), _ST]),
     "vk_value_release": (None, [C.POINTER(CValue)]),
     "vk_batch_call": (C.c_int, [_P, C.c_char_p, C.POINTER(CArg), C.c_size_t, C.c_size_t,
