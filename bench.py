@@ -500,6 +500,55 @@ def sort_suite(D, verify: bool = True):
     return out
 
 
+def region_suite(D, x1, x2, n):
+    """Fused regions at n rows: sum((x * 2.0) + 1.0), sum((x - y) * (x - y)) and
+    count((x * 2.0) > 1.0), each as one device pass (vk_region_reduce) against the same plan run
+    kernel by kernel (elementwise results materialised in HBM). Results must be identical; times
+    are HIP-event medians of three calls; GB/s counts the fused pass's input bytes."""
+    from verkehrt_amd import Scalar as S
+    ctx = D.ctx
+    inputs = [x1, x2, S("f64", 2.0), S("f64", 1.0)]
+    shapes = {
+        "sum((x*2.0)+1.0)": ([("mul_f64", [("in", 0), ("in", 2)]), ("add_f64", [("k", 0), ("in", 3)])],
+                             "reduce_add_f64", [("k", 1)], 8),
+        "sum((x-y)*(x-y))": ([("sub_f64", [("in", 0), ("in", 1)]), ("mul_f64", [("k", 0), ("k", 0)])],
+                             "reduce_add_f64", [("k", 1)], 16),
+        "count((x*2.0)>1.0)": ([("mul_f64", [("in", 0), ("in", 2)]), ("gt_f64", [("k", 0), ("in", 3)])],
+                               "count", [("k", 1)], 8),
+    }
+    out = {}
+    for label, (kernels, root, rargs, bpr) in shapes.items():
+        def fused():
+            return D.region_reduce(kernels, root, rargs, inputs)
+
+        def unfused():
+            vals = []
+            get = lambda s_: vals[s_[1]] if s_[0] == "k" else inputs[s_[1]]
+            for name, srcs in kernels:
+                vals.append(D.call(name, [get(s_) for s_ in srcs]))
+            return D.call(root, [get(s_) for s_ in rargs])
+
+        times = {}
+        for mode, f in (("fused", fused), ("unfused", unfused)):
+            f()
+            ctx.sync()
+            ctx.set_trace(True)
+            ctx.take_trace()
+            per = []
+            for _ in range(3):
+                res = f()
+                ctx.sync()
+                per.append(sum(us for _, us in parse_trace(ctx.take_trace())))
+            ctx.set_trace(False)
+            times[mode] = (float(np.median(per)), res)
+        (rf, flag), ru = times["fused"][1], times["unfused"][1]
+        out[label] = {"fused": flag, "identical": rf.bits() == ru.bits(), "fused_us": round(times["fused"][0], 1),
+                      "unfused_us": round(times["unfused"][0], 1),
+                      "speedup": round(times["unfused"][0] / times["fused"][0], 2),
+                      "GBps": round(bpr * n / times["fused"][0] / 1e3, 1)}
+    return out
+
+
 def verify_streaming(D, calls, n, table_rows):
     """Each streaming primitive's 2^28-row result, computed once more after the timed region,
     against the oracle on the same inputs (as benches/tier1.rs:403-433 asserts before timing)."""
@@ -670,9 +719,18 @@ def main():
         prims[label] = {"us": round(us, 1), "GBps": round(b / us / 1e3, 1),
                         "frac": round(b / us / 1e3 / HBM_PEAK_GBPS, 4), "bytes": b}
     dom = max(prims, key=lambda k: prims[k]["us"])
+    traffic = measured_traffic(dom, args.log2n)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": prims[dom]["GBps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": measured_traffic(dom, args.log2n),
+                "unit": "GB/s", "frac": prims[dom]["frac"], "traffic": traffic,
                 "bytes_per_launch": prims[dom]["bytes"], "launch_us": prims[dom]["us"]}
+    if traffic:
+        # the HBM bytes the kernel actually moves per second (rocprofv3 FETCH_SIZE + WRITE_SIZE)
+        roofline["traffic_GBps"] = round(traffic / prims[dom]["us"] / 1e3, 1)
+        roofline["traffic_frac"] = round(traffic / prims[dom]["us"] / 1e3 / HBM_PEAK_GBPS, 4)
+    if dom.startswith("gather"):
+        # random 8-byte rows: each costs a 128-byte line, so the bound is the random-row rate
+        roofline["random_rows_G_per_s"] = round(n / prims[dom]["us"] / 1e3, 1)
+    regions = None if args.only else region_suite(D, x1, x2, n)
     checks = None if (args.no_verify or args.only) else verify_streaming(D, calls, n, table_rows)
     sort = None if (args.no_sort or args.only) else sort_suite(D, verify=not args.no_verify)
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
@@ -693,6 +751,7 @@ def main():
         "verified": verified,
         "checks": checks,
         "primitives": prims,
+        "regions": regions,
         "batched": batched,
         "serve": serve,
         "sort": sort,
