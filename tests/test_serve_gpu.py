@@ -75,13 +75,31 @@ def test_results_same_alone_and_in_the_mix():
         assert o.digest == oracle_digest(cols, r), (o.rid, o.program, o.rows)
 
 
+def test_coalesced_requests_match_alone_and_the_oracle():
+    # continuous batching: a saturated stream of small requests, each worker taking up to 16
+    # arrived requests of one program and running every statement once for the group
+    from verkehrt_amd import serve as SV
+    key_rows = 1 << 21
+    reqs = SV.make_requests(120, qps=1e9, seed=7, lo=1_000, hi=1 << 20, reuse=0.4)
+    srv = SV.Server(workers=4, budget=40 * 10**9, key_rows=key_rows, coalesce=16)
+    mix = srv.run(reqs, paced=False)
+    srv.close()
+    s = mix.summary()
+    assert s["requests"] == len(reqs) and s["mean_group"] > 2, s
+    alone = _alone(SV, reqs, key_rows)
+    cols = OracleColumns(SV, key_rows)
+    for o, r in zip(mix.outcomes, reqs):
+        assert o.digest == alone[o.rid], (o.rid, o.program, o.rows, o.group)
+        assert o.digest == oracle_digest(cols, r), (o.rid, o.program, o.rows, o.group)
+
+
 def test_configured_stream_matches_the_oracle():
     # the configured serving form: sizes log-uniform in [10K, 64M] rows, 40% reuse, a 200 GB
     # budget, four workers, a paced 1000 QPS stream; every result against the oracle
     from verkehrt_amd import serve as SV
     key_rows = 64 << 20
     reqs = SV.make_requests(24, qps=1000.0, seed=11, lo=10_000, hi=key_rows, reuse=0.4)
-    srv = SV.Server(workers=4, budget=200 * 10**9, key_rows=key_rows)
+    srv = SV.Server(workers=4, budget=200 * 10**9, key_rows=key_rows, coalesce=8)
     mix = srv.run(reqs)
     srv.close()
     s = mix.summary()

@@ -19,7 +19,10 @@ Scheduling: the scheduler admits a request as soon as a worker slot is free and 
 budget covers the request's columns, outputs and scratch (estimated per program), evicting
 idle columns least recently used first. Admitted requests run at once on their workers' own
 contexts (own HIP streams), so requests that arrive while others run overlap on the device;
-nothing waits for a batch to fill or drain. Latency is completion minus arrival time.
+nothing waits for a batch to fill or drain. With coalescing on, a worker that takes a request
+also takes the arrived requests of the same program behind it (up to `coalesce` in all, as
+the budget allows) and runs each statement once for all of them with vk_batch_call, so a
+backlog of small requests becomes few launches. Latency is completion minus arrival time.
 
 Every result is reduced to a digest on the device (wrapping sum and xor of its 64-bit words,
 or the scalar's bits), so a result can be compared between a run in the mix and a run alone.
@@ -80,11 +83,12 @@ def make_requests(count: int, qps: float, seed: int = 5, lo: int = 10_000, hi: i
 
 @dataclass
 class Column:
-    x: Array
+    x: Optional[Array]
     rows: int
     nbytes: int
     pins: int = 0
     last_use: float = 0.0
+    ready: bool = True  # False while the request that creates it runs
 
 
 class ColumnStore:
@@ -114,7 +118,7 @@ class ColumnStore:
         """Whether `need` more bytes fit beside the resident columns and the in-flight
         requests, after evicting idle columns other than `keep` (which this does)."""
         while self.base_bytes + self.resident + in_flight + need > self.budget:
-            idle = [(c.last_use, cid) for cid, c in self.cols.items() if c.pins == 0 and cid != keep]
+            idle = [(c.last_use, cid) for cid, c in self.cols.items() if c.pins == 0 and c.ready and cid != keep]
             if not idle:
                 return False
             _, cid = min(idle)
@@ -124,19 +128,33 @@ class ColumnStore:
             col.x.release()
         return True
 
-    def acquire(self, req: Request, now: float) -> Tuple[Array, Array, Array, bool]:
-        """The request's columns, pinned; creates the value column when it is not resident."""
+    def usable(self, req: Request) -> bool:
+        """Whether the request's value column is resident and complete, or not resident at all
+        (the request then creates it); a column that another request is creating is neither."""
+        col = self.cols.get(req.column)
+        return col is None or col.ready
+
+    def acquire(self, req: Request, now: float) -> Tuple[Optional[Array], Array, Array, bool]:
+        """The request's columns, pinned. A value column that is not resident is reserved and
+        returned as None: the worker that runs the request creates it on its own stream, ahead of
+        the request's kernels, and `created` hands it over when the request completes, so the
+        scheduler never waits for a column."""
         col = self.cols.get(req.column)
         created = col is None
         if created:
-            x = self.D.call("rand_f64", [Scalar("i64", req.rows), Scalar("i64", 1000 + req.column)])
-            self.D.sync()  # complete before another context reads it
-            col = Column(x, req.rows, req.rows * 8)
+            col = Column(None, req.rows, req.rows * 8, ready=False)
             self.cols[req.column] = col
             self.resident += col.nbytes
         col.pins += 1
         col.last_use = now
         return col.x, self.keys.slice(0, req.rows), self.x2.slice(0, req.rows), created
+
+    def created(self, req: Request, x: Optional[Array]):
+        """The column the request created, complete now that the request has finished."""
+        col = self.cols[req.column]
+        if x is not None:
+            col.x = x
+            col.ready = True
 
     def release(self, req: Request):
         self.cols[req.column].pins -= 1
@@ -174,6 +192,52 @@ def run_program(D: Dispatch, prog: str, x: Array, k: Array, x2: Array):
     raise ValueError(prog)
 
 
+def _digest_group(D: Dispatch, arrays: List[Array]) -> List[Tuple[int, int]]:
+    """_digest_words of every array, with one batched launch sequence per reduction."""
+    views = [Array.wrap(D.ctx, "i64", a.device_ptr(), len(a)) if a.type != "i64" else a for a in arrays]
+    live = [i for i, w in enumerate(views) if len(w)]
+    out = [(0, 0)] * len(arrays)
+    if live:
+        sums = D.batch_call("reduce_add_i64", [[views[i]] for i in live])
+        xors = D.batch_call("reduce_xor_i64", [[views[i]] for i in live])
+        for i, a, b in zip(live, sums, xors):
+            out[i] = (a.value & MASK64, b.value & MASK64)
+    return out
+
+
+def run_program_group(D: Dispatch, prog: str, cols: List[Tuple[Array, Array, Array]]):
+    """Runs one program for a group of requests with one batched call per statement
+    (vk_batch_call); returns (digest, algorithmic bytes) per request, each equal to
+    run_program's for that request alone."""
+    if len(cols) == 1:
+        return [run_program(D, prog, *cols[0])]
+    S = Scalar
+    xs = [c[0] for c in cols]
+    ns = [len(x) for x in xs]
+    if prog == "sum_filter":
+        fs = D.batch_call("filter_gt_f64", [[x, S("f64", 0.9)] for x in xs])
+        ss = D.batch_call("reduce_add_f64", [[f] for f in fs])
+        return [(("f64", s_.bits()), 8 * n + 16 * len(f)) for s_, f, n in zip(ss, fs, ns)]
+    if prog == "dot":
+        ds = D.batch_call("dot_f64", [[x, c[2]] for x, c in zip(xs, cols)])
+        return [(("f64", d.bits()), 16 * n) for d, n in zip(ds, ns)]
+    if prog == "count":
+        cs = D.batch_call("count_gt_f64", [[x, S("f64", 0.5)] for x in xs])
+        return [(("i64", c_.value), 8 * n) for c_, n in zip(cs, ns)]
+    if prog == "scan":
+        rs = D.batch_call("scan_add_i64", [[c[1]] for c in cols])
+        return [(("array", d), 16 * n) for d, n in zip(_digest_group(D, rs), ns)]
+    if prog == "argsort":
+        rs = D.batch_call("argsort_i64", [[c[1]] for c in cols])
+        return [(("array", d), 16 * n) for d, n in zip(_digest_group(D, rs), ns)]
+    if prog == "group_sum":
+        gs = D.batch_call("group_id_i64", [[c[1]] for c in cols])
+        ngs = [m.value + 1 for m in D.batch_call("reduce_max_i64", [[g] for g in gs])]
+        rs = D.batch_call("reduce_into_add_f64", [[g, x, S("i64", ng)] for g, x, ng in zip(gs, xs, ngs)])
+        return [(("array", d), 16 * n + 8 * ng) for d, n, ng in zip(_digest_group(D, rs), ns, ngs)]
+    raise ValueError(prog)
+
+
 @dataclass
 class Outcome:
     rid: int
@@ -186,6 +250,7 @@ class Outcome:
     digest: tuple
     nbytes: int
     concurrent: int  # requests running when this one was admitted (itself included)
+    group: int = 1   # requests run together with this one (coalesced), itself included
 
     @property
     def latency(self) -> float:
@@ -212,23 +277,31 @@ class Report:
                 "p50_ms": round(self.pct(50) * 1e3, 3), "p99_ms": round(self.pct(99) * 1e3, 3),
                 "GBps": round(nbytes / self.wall / 1e9, 1), "wall_s": round(self.wall, 3),
                 "columns_created": self.created, "evictions": self.evictions,
-                "max_concurrent": max((o.concurrent for o in self.outcomes), default=0)}
+                "max_concurrent": max((o.concurrent for o in self.outcomes), default=0),
+                "mean_group": round(sum(o.group for o in self.outcomes) / max(1, len(self.outcomes)), 2)}
 
 
 class Server:
     """The request loop: one scheduler (this thread) and `workers` worker threads, each with
     its own context."""
 
-    def __init__(self, workers: int = 4, budget: int = 200 * 10**9, key_rows: int = 64 << 20):
+    def __init__(self, workers: int = 4, budget: int = 200 * 10**9, key_rows: int = 64 << 20,
+                 coalesce: int = 1, window: int = 256):
         self.R = Registry()
         self.loader = self.R.resolve(0)
         self.store = ColumnStore(self.loader, budget, key_rows)
         self.nworkers = workers
         self.workers = [self.R.resolve(0) for _ in range(workers)]
+        # continuous batching: a worker that takes a request also takes up to coalesce - 1 more
+        # requests of the same program that have arrived and fit the budget (looking `window`
+        # pending requests ahead), and runs each statement once for all of them
+        self.coalesce = max(1, coalesce)
+        self.window = window
 
     def close(self):
         for c in list(self.store.cols.values()):
-            c.x.release()
+            if c.x is not None:
+                c.x.release()
         self.store.cols.clear()
 
     def run(self, requests: List[Request], paced: bool = True) -> Report:
@@ -244,15 +317,25 @@ class Server:
                 job = work_qs[i].get()
                 if job is None:
                     return
-                req, x, k, x2, t_admit, conc = job
+                group, t_admit, conc = job
+                made = [None] * len(group)
                 try:
-                    digest, nbytes = run_program(D, req.program, x, k, x2)
+                    # columns this group creates: on this worker's stream, ahead of its kernels
+                    cols = []
+                    for g, (req, x, k, x2) in enumerate(group):
+                        if x is None:
+                            x = D.call("rand_f64", [Scalar("i64", req.rows), Scalar("i64", 1000 + req.column)])
+                            made[g] = x
+                        cols.append((x, k, x2))
+                    res = run_program_group(D, group[0][0].program, cols)
                     D.sync()
                     err = None
                 except Exception as e:  # reported by the scheduler
-                    digest, nbytes, err = None, 0, e
-                done_q.put((i, req, Outcome(req.rid, req.program, req.rows, req.reuse, req.t_arrive,
-                                            t_admit, clock(), digest, nbytes, conc), err))
+                    res, err = [(None, 0)] * len(group), e
+                t_done = clock()
+                done_q.put((i, [(req, Outcome(req.rid, req.program, req.rows, req.reuse, req.t_arrive, t_admit,
+                                              t_done, dg, nb, conc, len(group)), m)
+                                for (req, _, _, _), (dg, nb), m in zip(group, res, made)], err))
 
         threads = [threading.Thread(target=worker, args=(i,), daemon=True) for i in range(self.nworkers)]
         for t in threads:
@@ -267,15 +350,17 @@ class Server:
         def reap(block: bool):
             nonlocal error
             try:
-                i, req, out, err = done_q.get(block=block, timeout=5.0 if block else None)
+                i, done, err = done_q.get(block=block, timeout=5.0 if block else None)
             except queue.Empty:
                 return False
             idle.append(i)
-            in_flight.pop(req.rid, None)
-            self.store.release(req)
+            for req, out, made in done:
+                in_flight.pop(req.rid, None)
+                self.store.created(req, made)
+                self.store.release(req)
+                outcomes.append(out)
             if err is not None and error is None:
                 error = err
-            outcomes.append(out)
             return True
 
         while pending or in_flight:
@@ -286,13 +371,18 @@ class Server:
             if not pending:
                 reap(True)
                 continue
-            req = pending[0]
+            # the first pending request whose column is not being created by another request
+            at = next((j for j in range(min(len(pending), self.window)) if self.store.usable(pending[j])), None)
+            if at is None or (paced and clock() < pending[at].t_arrive and at > 0):
+                reap(True)  # wait for a creating request to finish
+                continue
+            req = pending[at]
             if paced and clock() < req.t_arrive:
                 if in_flight:
                     # wait for a completion or the next arrival, whichever is first
                     try:
-                        i, r2, out, err = done_q.get(timeout=max(0.0, req.t_arrive - clock()))
-                        done_q.put((i, r2, out, err))
+                        got = done_q.get(timeout=max(0.0, req.t_arrive - clock()))
+                        done_q.put(got)
                     except queue.Empty:
                         pass
                 else:
@@ -305,12 +395,32 @@ class Server:
                     raise RuntimeError(f"request {req.rid} does not fit the HBM budget")
                 reap(True)
                 continue
-            pending.pop(0)
-            x, k, x2, made = self.store.acquire(req, clock())
+            pending.pop(at)
+            now = clock()
+            x, k, x2, made = self.store.acquire(req, now)
             created += made
-            i = idle.pop()
             in_flight[req.rid] = extra
-            work_qs[i].put((req, x, k, x2, clock(), len(in_flight)))
+            group = [(req, x, k, x2)]
+            # coalesce arrived requests of the same program that fit the budget
+            j = 0
+            while len(group) < self.coalesce and j < min(len(pending), self.window):
+                r2 = pending[j]
+                if paced and r2.t_arrive > now:
+                    break
+                if r2.program != req.program or not self.store.usable(r2):
+                    j += 1
+                    continue
+                extra2 = EXTRA_BYTES_PER_ROW[r2.program] * r2.rows
+                need2 = extra2 + (0 if r2.column in self.store.cols else 8 * r2.rows)
+                if not self.store.fits(need2, sum(in_flight.values()), r2.column):
+                    break
+                pending.pop(j)
+                x_, k_, x2_, made = self.store.acquire(r2, now)
+                created += made
+                in_flight[r2.rid] = extra2
+                group.append((r2, x_, k_, x2_))
+            i = idle.pop()
+            work_qs[i].put((group, clock(), len(in_flight)))
         for q in work_qs:
             q.put(None)
         for t in threads:
