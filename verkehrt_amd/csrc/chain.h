@@ -41,17 +41,28 @@ __device__ __forceinline__ T chain_apply(int op, T a, T b) {
   }
 }
 
+// The constant of an op from its low sizeof(T) bytes.
+template <typename T>
+__device__ __forceinline__ T chain_const(uint64_t bits) {
+  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, bits);
+  else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, static_cast<uint32_t>(bits));
+  else return __builtin_bit_cast(T, static_cast<uint8_t>(bits));
+}
+
+// The chain over one row: a loop over the ops, kept rolled so that a kernel evaluating it for
+// many rows stays small enough to keep its own row arrays unrolled in registers; ch.ops[i] is a
+// uniform load from the kernel argument.
 template <typename T>
 __device__ __forceinline__ T chain_eval(const ChainSpec &ch, T x, T y) {
   T v = x;
-#pragma unroll
-  for (int i = 0; i < kChainMax; ++i) {
-    if (i >= ch.n) break;
-    const ChainOp &o = ch.ops[i];
-    T k;
-    __builtin_memcpy(&k, &o.kbits, sizeof(T));
-    const T b = o.src == 0 ? k : o.src == 1 ? x : o.src == 2 ? y : v;
-    v = o.swap ? chain_apply<T>(o.op, b, v) : chain_apply<T>(o.op, v, b);
+  const int n = ch.n;
+#pragma unroll 1
+  for (int i = 0; i < n; ++i) {
+    const int op = ch.ops[i].op, src = ch.ops[i].src;
+    const bool swap = ch.ops[i].swap != 0;
+    const T k = chain_const<T>(ch.ops[i].kbits);
+    const T b = src == 0 ? k : src == 1 ? x : src == 2 ? y : v;
+    v = swap ? chain_apply<T>(op, b, v) : chain_apply<T>(op, v, b);
   }
   return v;
 }

@@ -160,6 +160,38 @@ constexpr int W_AAGG = 1, W_AINC = 17, W_F = 33, W_EXACT = 161, W_XF = 177;
 constexpr uint64_t F_AAGG = 1, F_AINC = 2, F_FN = 4, F_EXACT = 8;
 static_assert(kFastWaves == 16, "the tile summary tree joins 16 waves as 4 x 4");
 
+// Row functors with a `map` (fused regions) are loaded raw, all rows first, then mapped, so the
+// loads issue together ahead of the per-row work.
+template <class RV, class = void>
+struct Mapped : std::false_type {};
+template <class RV>
+struct Mapped<RV, decltype(void(RV::kMapped))> : std::true_type {};
+
+// Chain j's rows of blocks b .. b + min(cnt, 16) - 1 into v (zeros past cnt).
+template <class RV>
+__device__ __forceinline__ void load_run(const RV &rv, const double *x, const double *y, uint64_t b, int j,
+                                         int cnt, double (&v)[kRunBlocks]) {
+  if constexpr (Mapped<RV>::value) {
+    double u[kRunBlocks];
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) {
+      const uint64_t i = (b + k) * 16 + j;
+      v[k] = k < cnt ? x[i] : 0.0;
+      u[k] = RV::kTwo && k < cnt ? y[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv.map(v[k], u[k]) : 0.0;
+    return;
+  }
+  if (cnt >= kRunBlocks) {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b + k) * 16 + j);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b + k) * 16 + j) : 0.0;
+  }
+}
+
 __device__ __forceinline__ uint64_t flag_word(uint32_t epoch, uint64_t bits) {
   return (uint64_t(epoch) << 8) | bits;
 }
@@ -366,11 +398,17 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double
   // 1. load the run
   double v[kRunBlocks];
   double a = 0.0;
+  if constexpr (Mapped<RV>::value) {
+    load_run(rv, x, y, b0 + uint64_t(r) * kRunBlocks, j, nb - r * kRunBlocks, v);
 #pragma unroll
-  for (int k = 0; k < kRunBlocks; ++k) {
-    const int lb = r * kRunBlocks + k;
-    v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
-    a = a + v[k];
+    for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRunBlocks; ++k) {
+      const int lb = r * kRunBlocks + k;
+      v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
+      a = a + v[k];
+    }
   }
   // 2. approximate chain sums of the tile, published as an aggregate (or inclusive at the
   //    item's first tile)
@@ -647,18 +685,6 @@ __device__ __forceinline__ RunFn fn_of(double d0, double d1, double lo0, double 
   return f;
 }
 
-// Chain j's rows of blocks b .. b + min(cnt, 16) - 1 into v (zeros past cnt).
-template <class RV>
-__device__ __forceinline__ void load_run(const RV &rv, const double *x, const double *y, uint64_t b, int j,
-                                         int cnt, double (&v)[kRunBlocks]) {
-  if (cnt >= kRunBlocks) {
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k) v[k] = rv(x, y, (b + k) * 16 + j);
-  } else {
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k) v[k] = k < cnt ? rv(x, y, (b + k) * 16 + j) : 0.0;
-  }
-}
 
 // The tile summary of chain j from the 16 waves' joined runs: wave 0's lanes join four waves
 // each (lane group l / 16), then the four groups (xor 16, 32); every lane of chain j gets it.
@@ -1360,10 +1386,13 @@ template <bool TWO>
 struct RowChain {
   static constexpr int kSummPasses = TWO ? 1 : 2;
   static constexpr int kSummWaves = TWO ? 1 : 6;
+  static constexpr bool kTwo = TWO;
+  static constexpr bool kMapped = true;
   ChainSpec ch;
   __device__ __forceinline__ double operator()(const double *x, const double *y, uint64_t i) const {
     return chain_eval<double>(ch, x[i], TWO ? y[i] : 0.0);
   }
+  __device__ __forceinline__ double map(double x, double y) const { return chain_eval<double>(ch, x, y); }
 };
 
 }  // namespace
