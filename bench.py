@@ -406,6 +406,36 @@ def check_group_ids(keys: np.ndarray, ids: np.ndarray) -> bool:
     return np.unique(key_of).size == g
 
 
+def batched_literal_suite(D, count: int = 4096, lo: int = 1 << 10, hi: int = 1 << 24):
+    """The many-arrays form at the configuration's literal lengths, log-uniform in [1K, 16M]
+    rows (about 7 G rows, 113 GB of values and keys): run in chunks of consecutive arrays of at
+    most 2^29 rows, each chunk once as one call per array and statement and once as one batched
+    call per statement. Every sum and every 64th array's permutation and group sums are compared
+    bit for bit."""
+    from verkehrt_amd import batch as B
+    tp = tb = 0.0
+    rows = 0
+    same = True
+    for first, take in B.chunks(count, lo, hi):
+        wl = B.make_workload(D, count=count, lo=lo, hi=hi, first=first, take=take)
+        rows += wl.rows
+        t, rp = B.timed(B.plain, D, wl, repeat=1)
+        tp += t
+        t, rb = B.timed(B.batched, D, wl, repeat=1)
+        tb += t
+        step = 64
+        pick = [i for i in range(take) if (first + i) % step == 0]
+        same = same and B.same(B.Results(rp.sums, [rp.perms[i] for i in pick], [rp.groups[i] for i in pick]),
+                               B.Results(rb.sums, [rb.perms[i] for i in pick], [rb.groups[i] for i in pick]))
+        del wl, rp, rb
+        D.ctx.trim()
+    if not same:
+        sys.exit("batched results differ from plain dispatch at the literal lengths")
+    return {"arrays": count, "rows": rows, "lengths": "log-uniform in [2^10, 2^24] rows (chunks of <= 2^29 rows)",
+            "plain_ms": round(tp * 1e3, 1), "batched_ms": round(tb * 1e3, 1), "speedup": round(tp / tb, 2),
+            "identical": same}
+
+
 def sort_suite(D, verify: bool = True):
     """Sort and grouping (the configuration's third workload): argsort of 2^27 keys (u64 uniform
     = splitmix64(7 + i), i64 Zipf(1.1) over 2^20 values by inverse CDF of rand_f64(2^27, 9),
@@ -639,6 +669,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-arrays batched comparison")
     ap.add_argument("--batch-arrays", type=int, default=4096)
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the many-arrays form at the literal [1K, 16M] lengths (about 7 G rows)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-sort", action="store_true",
                     help="skip the sort and grouping cases (argsort of 2^27 keys, group-by sum over "
@@ -746,6 +778,8 @@ def main():
     checks = None if (args.no_verify or args.only) else verify_streaming(D, calls, n, table_rows)
     sort = None if (args.no_sort or args.only) else sort_suite(D, verify=not args.no_verify)
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
+    if batched is not None and not args.no_literal:
+        batched["literal_range"] = batched_literal_suite(D, args.batch_arrays)
     serve = None if (args.no_serve or args.only) else serve_suite(args.serve_requests, args.serve_qps)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     verified = checks is not None and all(checks.values()) and (sort is None or sort["verified"])

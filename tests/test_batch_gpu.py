@@ -272,3 +272,34 @@ def test_batched_workload_matches_plain(D):
         g = O.group_id(k)
         want = O.reduce_into(g, x, int(g.max()) + 1, "add")
         assert np.array_equal(b.groups[i].to_numpy().view(np.uint64), want.view(np.uint64))
+
+
+def test_batch_mixes_long_items(D):
+    # items of 2^22 rows or more run as single calls inside a batched argsort, group_id and
+    # reduce_into_add_f64; results, and the first failing item's error, stay the single calls'
+    from verkehrt_amd import KernelError
+    rng = np.random.default_rng(90)
+    sizes = [1000, (1 << 22) + 5, 37, 1 << 22, 0, 5000]
+    keys = [rng.integers(-150, 150, n).astype(np.int64) for n in sizes]
+    vals = [rng.normal(size=n) for n in sizes]
+    dk, dv = [D.array(k) for k in keys], [D.array(v) for v in vals]
+    for p, k in zip(D.batch_call("argsort_i64", [[k] for k in dk]), keys):
+        assert np.array_equal(p.to_numpy(), O.argsort(k))
+    gs = D.batch_call("group_id_i64", [[k] for k in dk])
+    ids = [g.to_numpy() for g in gs]
+    for g, k in zip(ids, keys):
+        assert np.array_equal(g, O.group_id(k))
+    ng = [int(g.max()) + 1 if g.size else 0 for g in ids]
+    rs = D.batch_call("reduce_into_add_f64", [[g, v, S("i64", n)] for g, v, n in zip(gs, dv, ng)])
+    for r, g, v, n in zip(rs, ids, vals, ng):
+        assert np.array_equal(r.to_numpy().view(np.uint64), O.reduce_into(g, v, n).view(np.uint64))
+    # a bad key in the long item 1 and in the short item 2: item 1's error, as in order
+    bad = [k.copy() for k in ids]
+    bad[1][123] = 10**6
+    bad[2][3] = -1
+    items = [[D.array(b), v, S("i64", n)] for b, v, n in zip(bad, dv, ng)]
+    with pytest.raises(KernelError) as e1:
+        D.batch_call("reduce_into_add_f64", items)
+    with pytest.raises(KernelError) as e2:
+        D.call("reduce_into_add_f64", items[1])
+    assert str(e1.value) == str(e2.value) and "bucket key 1000000" in str(e1.value)

@@ -75,22 +75,40 @@ class Workload:
 
 
 def make_workload(D: Dispatch, count: int = 4096, lo: int = 1 << 10, hi: int = 1 << 20,
-                  keys: int = 256, seed: int = 41) -> Workload:
-    ns = lengths(count, seed, lo, hi)
+                  keys: int = 256, seed: int = 41, first: int = 0, take: int | None = None) -> Workload:
+    """Arrays first .. first + take - 1 of the `count`-array workload (all by default). Row r of
+    the arrays' concatenation holds x = rand_f64 element splitmix64(seed + 1 + r) and
+    k = pool[splitmix64(seed + 3 + r) mod keys] with pool[j] = splitmix64(seed + 2 + j), so any
+    run of consecutive arrays is generated alone with the values it has in the whole workload.
+    Generated on the device (keys: a gather from the pool by masked rand_i64 words)."""
+    all_ns = lengths(count, seed, lo, hi)
+    take = count - first if take is None else take
+    ns = all_ns[first:first + take]
+    base = int(sum(all_ns[:first]))
     offs = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64).tolist()
     total = int(sum(ns))
-    x = D.call("rand_f64", [Scalar("i64", total), Scalar("i64", seed + 1)])
-    pool = splitmix64_np(seed + 2, 0, keys).view(np.int64)
-    k = Array.empty(D.ctx, "i64", total)
-    step = 1 << 24
-    for s0 in range(0, total, step):
-        c = min(step, total - s0)
-        sel = (splitmix64_np(seed + 3, s0, c) & np.uint64(keys - 1)).astype(np.int64)
-        chunk = Array.from_numpy(D.ctx, pool[sel])
-        D.elementwise_tile("add_i64", [chunk, Scalar("i64", 0)], k.slice(s0, c))
-        del chunk
+    S = Scalar
+    x = D.call("rand_f64", [S("i64", total), S("i64", seed + 1 + base)])
+    pool = D.call("rand_i64", [S("i64", keys), S("i64", seed + 2)])
+    sel = D.call("and_i64", [D.call("rand_i64", [S("i64", total), S("i64", seed + 3 + base)]), S("i64", keys - 1)])
+    k = D.call("gather_i64", [pool, sel])
+    del sel
     D.sync()
     return Workload(x, k, offs, ns)
+
+
+def chunks(count: int, lo: int, hi: int, seed: int = 41, max_rows: int = 1 << 29) -> List[tuple]:
+    """(first, take) runs of consecutive arrays of at most max_rows rows each (an array longer
+    than max_rows is a run of its own)."""
+    ns = lengths(count, seed, lo, hi)
+    out, first, rows = [], 0, 0
+    for i, n in enumerate(ns):
+        if i > first and rows + n > max_rows:
+            out.append((first, i - first))
+            first, rows = i, 0
+        rows += n
+    out.append((first, count - first))
+    return out
 
 
 @dataclass

@@ -1029,7 +1029,59 @@ vk_scalar region_run(Ctx &c, const RegionPlan &p) {
 
 // ------------------------------------------------------------------ batching
 
+std::vector<vk_value> batch_call_all(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, size_t nitems);
+
+// Items of at least this many rows run as single calls in a batched argsort, group_id or
+// reduce_into_add_f64: the batched forms pay for their per-item tables and the single-call
+// kernels are faster on long inputs (4096 arrays of 1K to 16M rows ran 1.2x slower batched).
+constexpr uint64_t kBatchLargeRows = uint64_t(1) << 22;
+
+void release_values(std::vector<vk_value> &vs) {
+  for (auto &v : vs)
+    if (v.kind == VK_VALUE_ARRAY && v.array) {
+      release(v.array);
+      v.array = nullptr;
+    }
+}
+
 std::vector<vk_value> batch_call(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, size_t nitems) {
+  const bool split = d.cls == VK_WHOLE && nargs >= 1 &&
+                     (d.impl == W_ARGSORT || d.impl == W_GROUP_ID || (d.impl == W_REDUCE_INTO && d.p0 == 4));
+  std::vector<size_t> small, large;
+  if (split)
+    for (size_t i = 0; i < nitems; ++i) {
+      const vk_arg &a = args[i * nargs];
+      const uint64_t n = a.kind == VK_ROWS && a.rows ? a.rows->len : 0;
+      (n >= kBatchLargeRows ? large : small).push_back(i);
+    }
+  if (large.empty()) return batch_call_all(c, d, args, nargs, nitems);
+  std::vector<vk_value> out(nitems, vk_value{VK_VALUE_ARRAY, nullptr, {0, 0}});
+  try {
+    for (size_t i : large) out[i] = call(c, d, args + i * nargs, nargs);
+    if (!small.empty()) {
+      std::vector<vk_arg> sub;
+      sub.reserve(small.size() * nargs);
+      for (size_t i : small)
+        for (size_t k = 0; k < nargs; ++k) sub.push_back(args[i * nargs + k]);
+      std::vector<vk_value> r = batch_call_all(c, d, sub.data(), nargs, small.size());
+      for (size_t j = 0; j < small.size(); ++j) out[small[j]] = r[j];
+    }
+  } catch (...) {
+    // the first failing item's error, as one call per item in order reports it
+    release_values(out);
+    for (size_t i = 0; i < nitems; ++i) {
+      try {
+        out[i] = call(c, d, args + i * nargs, nargs);
+      } catch (...) {
+        release_values(out);
+        throw;
+      }
+    }
+  }
+  return out;
+}
+
+std::vector<vk_value> batch_call_all(Ctx &c, const Def &d, const vk_arg *args, size_t nargs, size_t nitems) {
   std::vector<vk_value> out(nitems, vk_value{VK_VALUE_ARRAY, nullptr, {0, 0}});
   // every item's arguments pass the checks of a single call first
   std::vector<int64_t> rows(nitems, -1);
