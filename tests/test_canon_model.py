@@ -25,6 +25,9 @@ def model():
     L.vkm_canonical_fold_streamed.restype = C.c_int
     L.vkm_canonical_fold_streamed.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                               C.c_void_p]
+    L.vkm_wide_table_apply.restype = C.c_int
+    L.vkm_wide_table_apply.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_void_p, C.c_uint64, C.c_void_p,
+                                       C.c_void_p, C.POINTER(C.c_int32)]
     return L
 
 
@@ -39,10 +42,10 @@ def chains_oracle(vals, acc):
     return [float(np.cumsum(np.concatenate([[acc[j]], vals[j::16]]))[-1]) for j in range(16)]
 
 
-def run(model, vals, acc, lag, streamed=False, records=True):
+def run(model, vals, acc, lag, streamed=False, records=1):
     vals = np.ascontiguousarray(vals, np.float64)
     a = np.array(acc, np.float64)
-    stats = np.zeros(6, np.uint64)
+    stats = np.zeros(8, np.uint64)
     if streamed:
         model.vkm_canonical_fold_streamed(vals.ctypes.data, vals.size // 16, a.ctypes.data, lag, int(records),
                                           stats.ctypes.data)
@@ -121,13 +124,14 @@ def check_bits(name, got, want):
             assert bits(got[j]) == bits(want[j]), (name, j, got[j], want[j])
 
 
-@pytest.mark.parametrize("records", [True, False])
+@pytest.mark.parametrize("records", [1, 0, 3, 2])
 @pytest.mark.parametrize("group_tiles", [1, 4, 64])
 def test_streamed_model_matches_sequential_chains(model, group_tiles, records):
     """The streamed form (prefix hypotheses, clean groups, the walk) on the same data; small
     groups make joined spans, failed spans and segments all occur on 41 tiles. Tiles whose
-    summary fails go through run records or, without them, the fallback wave."""
-    st = np.zeros(6, np.uint64)
+    summary fails go through their wide table (records & 2), then run records (records & 1) or
+    the fallback wave."""
+    st = np.zeros(8, np.uint64)
     for name, vals, acc in datasets():
         got, stats = run(model, vals, acc, group_tiles, streamed=True, records=records)
         check_bits(name, got, chains_oracle(vals, acc))
@@ -135,8 +139,10 @@ def test_streamed_model_matches_sequential_chains(model, group_tiles, records):
     assert st[0] > 0 and st[3] > 0  # joined spans applied, tiles evaluated exactly
     if 1 < group_tiles < 64:
         assert st[2] > 0  # segments of non-clean groups applied at once
-    if records:
+    if records == 1:
         assert st[4] > 0 and st[5] > 0  # run segments applied, runs folded
+    if records & 2:
+        assert st[6] > 0 and st[7] > 0  # wide tables applied, and some that could not apply
 
 
 def test_streamed_model_long_uniform(model):
@@ -148,3 +154,78 @@ def test_streamed_model_long_uniform(model):
     check_bits("uniform-300", got, chains_oracle(x, np.zeros(16)))
     assert 0 < stats[3] <= 16 * 11
     assert stats[0] > 0
+
+
+def wide(model, chain, approx, starts):
+    chain = np.ascontiguousarray(chain, np.float64)
+    starts = np.ascontiguousarray(starts, np.float64)
+    out = np.zeros(starts.size)
+    applied = np.zeros(starts.size, np.uint8)
+    nres = C.c_int32(0)
+    model.vkm_wide_table_apply(chain.ctypes.data, chain.size, approx, starts.ctypes.data, starts.size,
+                               out.ctypes.data, applied.ctypes.data, C.byref(nres))
+    return out, applied.astype(bool), nres.value
+
+
+def seq_fold(start, chain):
+    return float(np.cumsum(np.concatenate([[start], chain]))[-1])
+
+
+def wide_cases():
+    rng = np.random.default_rng(21)
+    # from 40 down through zero (tiny binades, a change of sign) and back up
+    drift = np.concatenate([np.full(512, -40.0 / 500), np.full(512, 40.0 / 500)])
+    yield "through-zero", rng.normal(size=1024) * 0.3 + drift, 40.0, True
+    # zero-mean steps from 0.37: the running value rises about six binades, beyond the table
+    yield "normal-from-small", rng.normal(size=1024), 0.37, False
+    # a start just under 2^40 with steps of 1/2 + multiples of 2^-14: ties on the start binade's
+    # grid (u0 = 2^-13) and a rise into the next binade, so the end depends on the start's residue
+    yield "tie-crossing-up", 0.5 + rng.integers(0, 4, 1024) * 2.0 ** -14, 2.0 ** 40 - 300.0, True
+    # a walk that falls two binades and climbs back
+    yield "down-and-up", np.concatenate([np.full(512, -3.0), np.full(512, 3.0)]), 2000.3, True
+    # the same from 2000: the running value lands on 512 = 2^9 exactly, where the margin rule
+    # refuses every start below (conservative there, still exact where it applies)
+    yield "lands-on-power-of-two", np.concatenate([np.full(512, -3.0), np.full(512, 3.0)]), 2000.0, None
+    # late zero-mean steps with a large running value
+    yield "normal-large", rng.normal(size=1024), 4100.3, True
+    # a rise of six binades (beyond the 16-entry table)
+    yield "steep-rise", np.full(1024, 1.0), 20.0, False
+    # a NaN step, an infinite step, and a fold that lands on zero exactly
+    c = rng.random(1024)
+    c[500] = np.nan
+    yield "nan", c, 3.0, False
+    c = rng.random(1024)
+    c[10] = np.inf
+    yield "inf", c, 3.0, False
+    yield "hits-zero", np.concatenate([[-1.0] * 4, rng.random(1020)]), 4.0, False
+
+
+@pytest.mark.parametrize("case", [c[0] for c in wide_cases()])
+def test_wide_table_matches_sequential_folds(model, case):
+    """The wide table of a chain (built from an approximate start, as the walk's dirty tiles are)
+    applied to exact starts around it: wherever it applies, the bits of the sequential fold.
+    The starts include every residue modulo 16 ulps and offsets of up to 400 ulps, small beside
+    every case's steps (as an approximate start's error is beside real data's steps), which the
+    table must nearly all cover, and offsets of up to 10^6 ulps, which can move a crossing to
+    another step and must then be refused."""
+    name, chain, approx, expect_apply = next(c for c in wide_cases() if c[0] == case)
+    e = int(np.frexp(approx)[1]) - 1
+    u0 = 2.0 ** (e - 52)
+    rng = np.random.default_rng(5)
+    near = np.concatenate([np.arange(-40, 40), rng.integers(-400, 400, 200)])
+    offs = np.concatenate([near, rng.integers(-10 ** 6, 10 ** 6, 300)])
+    starts = approx + offs * u0
+    got, applied, nres = wide(model, chain, approx, starts)
+    for s, g, a in zip(starts, got, applied):
+        if a:
+            assert bits(g) == bits(seq_fold(s, chain)), (name, s)
+    if expect_apply:
+        assert nres > 0 and applied[:near.size].mean() > 0.9, (name, nres, applied[:near.size].mean())
+    elif expect_apply is None:
+        assert applied.any() and not applied.all(), name
+    else:
+        assert not applied.any(), name
+    if name == "tie-crossing-up":
+        # the residue matters: the folds from these starts do not all move by one offset
+        ends = np.array([seq_fold(s, chain) for s in starts[:64]]) - starts[:64]
+        assert len(set(ends.tolist())) > 1 and nres >= 4

@@ -665,6 +665,25 @@ def test_canonical_streamed_long(D):
         del dx
 
 
+def test_canonical_zero_mean_large(D):
+    # zero-mean rows: every chain's running sum returns near zero again and again, so thousands
+    # of tiles per chain cross binades; the walk takes them through their wide tables
+    rng = np.random.default_rng(26)
+    n = (1 << 26) + 5
+    x = rng.normal(size=n)
+    y = rng.normal(size=n)
+    dx, dy = D.array(x), D.array(y)
+    assert same_float(O.canonical_sum_f64(x), D.call("reduce_add_f64", [dx]).value)
+    assert same_float(O.dot_f64(x, y), D.call("dot_f64", [dx, dy]).value)
+    want = O.condsum_f64(x, "lt", 0.5, -0.25)
+    assert same_float(want, D.call("condsum_lt_f64", [dx, S("f64", 0.5), S("f64", -0.25)]).value)
+    # the same fold over x * 3 - y as one fused region
+    z = x * 3.0 - y
+    fused, one_pass = D.region_reduce([("mul_f64", [("in", 0), ("in", 2)]), ("sub_f64", [("k", 0), ("in", 1)])],
+                                      "reduce_add_f64", [("k", 1)], [dx, dy, S("f64", 3.0)])
+    assert one_pass and same_float(O.canonical_sum_f64(z), fused.value)
+
+
 def test_canonical_streamed_slices(D):
     # the streamed form continuing a running state, between slices of the single-pass form
     rng = np.random.default_rng(78)

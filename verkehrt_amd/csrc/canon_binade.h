@@ -239,4 +239,161 @@ __host__ __device__ __forceinline__ bool run_apply(const RunFn &f, const Binade 
   return true;
 }
 
+// ------------------------------------------------------------------ wide summaries
+// A stretch whose running value crosses binades has no run summary. It still has a small exact
+// form. Let U be the largest ulp of any running value R_i of the fold. For delta a multiple of
+// 2U, fl(x + delta) = fl(x) + delta whenever x and x + delta lie in one binade: both round on
+// the same grid of that binade's ulp, and an even number of ulps keeps ties to even. So a fold
+// started at S = Z + delta gives R_i + delta at every step, as long as each R_i + delta stays at
+// least one ulp inside the binade of R_i (then the exact sum before the rounding stays inside it
+// too). Its end value is then R_end + delta, exactly representable (a multiple of the last ulp,
+// inside its binade), so one add gives it. R_end depends on the residue of S modulo P = 2U, so
+// a table of P / u0 entries (u0 the ulp of the start binade), each from one exact simulation of
+// the stretch from Z_r = B + r u0, covers every start near B. Each entry keeps R_end and the
+// window [lo, hi] of the delta = S - Z_r that pass every step.
+constexpr int kWideRes = 16;  // residues simulated: periods up to 16 u0 (three binades above the start)
+
+struct WideHead {
+  double b;      // B, a multiple of 16 u0 next to the approximate start (toward zero)
+  int32_t e0;    // exponent of the start binade, u0 = 2^(e0 - 52)
+  int32_t nres;  // table entries kept, 2^(M + 1) for M binades above the start; 0: no table
+};
+
+struct WideEnt {
+  double end; // the simulation's end value R_end
+  double lo;  // least admissible delta (+inf when the entry never applies)
+  double hi;  // greatest admissible delta (-inf then)
+};
+
+// One residue's simulation: the running value, the segment of consecutive running values with
+// one sign and exponent (key = the top 12 bits, 0xFFFFFFFF before the first step) with its least
+// and greatest value, the delta window so far, and the greatest exponent seen.
+struct WideAcc {
+  double z, cur, mn, mx, lo, hi;
+  uint32_t key;
+  int emax;
+  bool bad;
+};
+
+__host__ __device__ __forceinline__ uint32_t sign_exp_key(double v) { return static_cast<uint32_t>(dbits_(v) >> 52); }
+
+__host__ __device__ __forceinline__ WideAcc wide_start(double z) {
+  const double inf = __builtin_inf();
+  return WideAcc{z, z, 0.0, 0.0, -inf, inf, 0xFFFFFFFFu, -2000, false};
+}
+
+// Folds the finished segment's constraint into the window: every running value v with this key
+// must keep v + delta within [2^e + u, 2^(e+1) - u] (mirrored for negative values). The bounds
+// are exact: both sides are multiples of u below 2^(e+1).
+__host__ __device__ __forceinline__ void wide_flush(WideAcc &a) {
+  if (a.key == 0xFFFFFFFFu) return;
+  const int ex = static_cast<int>(a.key & 0x7FF);
+  if (ex < 1023 - 960 || ex > 1023 + 960) {  // zero, subnormal, extreme, infinite or NaN
+    a.bad = true;
+    return;
+  }
+  const int e = ex - 1023;
+  a.emax = a.emax > e ? a.emax : e;
+  const double u = pow2(e - 52);
+  const double bot = pow2(e) + u, top = pow2(e + 1) - u;
+  double lo, hi;
+  if (a.key & 0x800) {
+    lo = -top - a.mn;
+    hi = -bot - a.mx;
+  } else {
+    lo = bot - a.mn;
+    hi = top - a.mx;
+  }
+  a.lo = a.lo > lo ? a.lo : lo;
+  a.hi = a.hi < hi ? a.hi : hi;
+}
+
+// Records the new running value r (a.cur is set by the caller).
+__host__ __device__ __forceinline__ void wide_visit(WideAcc &a, double r) {
+  const uint32_t k = sign_exp_key(r);
+  if (__builtin_expect(k != a.key, 0)) {
+    wide_flush(a);
+    a.key = k;
+    a.mn = r;
+    a.mx = r;
+  } else {
+    a.mn = __builtin_fmin(a.mn, r);
+    a.mx = __builtin_fmax(a.mx, r);
+  }
+}
+
+__host__ __device__ __forceinline__ void wide_push(WideAcc &a, double v) {
+  a.cur = a.cur + v;
+  wide_visit(a, a.cur);
+}
+
+// N elements at once, with the same result as N wide_push calls: the running values first (the
+// only dependent chain), then, when all of them keep the current segment's key, one min and max
+// over them; otherwise they are visited one by one.
+template <int N>
+__host__ __device__ __forceinline__ void wide_push_n(WideAcc &a, const double (&v)[N]) {
+  double r[N];
+  r[0] = a.cur + v[0];
+#pragma unroll
+  for (int m = 1; m < N; ++m) r[m] = r[m - 1] + v[m];
+  a.cur = r[N - 1];
+  bool same = true;
+  double mn = r[0], mx = r[0];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    same = same && sign_exp_key(r[m]) == a.key;
+    mn = __builtin_fmin(mn, r[m]);
+    mx = __builtin_fmax(mx, r[m]);
+  }
+  if (__builtin_expect(same, 1)) {
+    a.mn = __builtin_fmin(a.mn, mn);
+    a.mx = __builtin_fmax(a.mx, mx);
+  } else {
+#pragma unroll
+    for (int m = 0; m < N; ++m) wide_visit(a, r[m]);
+  }
+}
+
+// The finished entry; an entry whose simulation met a value with no binade never applies.
+__host__ __device__ __forceinline__ WideEnt wide_end(WideAcc &a) {
+  wide_flush(a);
+  const double inf = __builtin_inf();
+  if (a.bad || !(a.lo <= a.hi)) return WideEnt{0.0, inf, -inf};
+  return WideEnt{a.cur, a.lo, a.hi};
+}
+
+// B for the approximate start `approx` under hypothesis h (h.ok): approx less its residue
+// modulo 16 u0, exact.
+__host__ __device__ __forceinline__ double wide_base(double approx, const Binade &h) {
+  return approx - __builtin_fmod(approx, 16.0 * pow2(h.e - 52));
+}
+
+// The number of entries to keep from the 16 simulated ones, given M, the greatest number of
+// binades any valid entry's path rose above the start (at least 0); 0 when M > 3, or when no
+// entry is valid (m < -1000, wide_start's value).
+__host__ __device__ __forceinline__ int wide_nres(int m) { return m < -1000 || m > 3 ? 0 : 2 << (m < 0 ? 0 : m); }
+
+// Applies the table (head, entries) to the exact start s: returns true and the exact end value
+// when s falls in the entry's window. s and B must have one sign and lie within a factor of two
+// of each other (so s - B is exact), and s must be a multiple of u0.
+__host__ __device__ __forceinline__ bool wide_apply(const WideHead &hd, const WideEnt *ents, double s,
+                                                    double *out) {
+  if (hd.nres <= 0) return false;
+  const double b = hd.b;
+  const bool near = b > 0 ? (s >= 0.5 * b && s <= 2.0 * b) : (s <= 0.5 * b && s >= 2.0 * b);
+  if (!near) return false;
+  const double u0 = pow2(hd.e0 - 52), inv_u0 = pow2(52 - hd.e0);
+  // t = s - B is exact (Sterbenz) and below 2^(e0 + 1) in magnitude, so t / u0 is an integer
+  // below 2^53 when s is a multiple of u0; its low bits give the residue (nres is a power of 2)
+  const double t = s - b;
+  const double k = t * inv_u0;
+  if (k != __builtin_floor(k)) return false;
+  const int r = static_cast<int>(static_cast<int64_t>(k) & (hd.nres - 1));
+  const WideEnt e = ents[r];
+  const double delta = t - static_cast<double>(r) * u0;
+  if (!(delta >= e.lo && delta <= e.hi)) return false;
+  *out = e.end + delta;
+  return true;
+}
+
 }  // namespace vk

@@ -252,9 +252,9 @@ __device__ __forceinline__ Binade unpack_binade(uint64_t w) {
 //    the current exact value and scanned in order; the runs before the first one whose check
 //    fails are applied at once, that run is folded element by element, and the next round
 //    starts after it. Each round passes at least one run.
-//  - many changes (a running value near zero): each run is summarised under the binade of its
-//    own approximate start and the runs are applied one after another; a run whose check
-//    fails is folded element by element. One pass whatever the data.
+//  - many changes (a running value near zero): the sequential fold itself, one element after
+//    another, each read from its run's lane (about a thousand dependent adds, fewer cycles
+//    than a summary check per run where most runs would fail theirs).
 // A check fails only where the value leaves its binade or meets a huge, infinite or NaN
 // element, and a failed check is always followed by the element-by-element fold, so the
 // result is the sequential one either way.
@@ -311,27 +311,17 @@ __device__ double eval_chain(const RV &rv, const double *x, const double *y, uin
     }
     return cur;
   }
-  RunFn f = run_identity();
-  if (ha.ok) f = run_of(v, nb - l * kRunBlocks, binade_ref(ha));
+  // many changes (a running value near zero): the sequential fold itself, element by element,
+  // each element read from its run's lane, stopping at block nb
   for (int r = 0; r < nruns; ++r) {
-    RunFn fr;
-    fr.d[0] = __shfl(f.d[0], r);
-    fr.d[1] = __shfl(f.d[1], r);
-    fr.lo[0] = __shfl(f.lo[0], r);
-    fr.lo[1] = __shfl(f.lo[1], r);
-    fr.hi[0] = __shfl(f.hi[0], r);
-    fr.hi[1] = __shfl(f.hi[1], r);
-    const Binade hr{__shfl(ha.e, r), __shfl(ha.neg, r), __shfl(ha.ok, r), 0};
-    double out;
-    if (run_apply(fr, hr, cur, &out)) {
-      cur = out;
-    } else {
-      const int nk = nb - r * kRunBlocks < kRunBlocks ? nb - r * kRunBlocks : kRunBlocks;
+    const int nk = nb - r * kRunBlocks < kRunBlocks ? nb - r * kRunBlocks : kRunBlocks;
+    if (nk == kRunBlocks) {
 #pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k) {
-        const double u = __shfl(v[k], r);
-        if (k < nk) cur = cur + u;
-      }
+      for (int k = 0; k < kRunBlocks; ++k) cur = cur + bcast_pod(v[k], r);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRunBlocks; ++k)
+        if (k < nk) cur = cur + bcast_pod(v[k], r);
     }
   }
   return cur;
@@ -638,22 +628,23 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fast(RV rv, const double
 //  3. k_canon_prefix: the exclusive prefix of those sums from the start state, the approximate
 //     start of every tile (any order of these approximate sums serves: the applies below prove
 //     every result);
-//  4. k_canon_check: every chain's summary is checked at its approximate start; tiles with a
-//     chain whose prediction missed the binade of its approximate start, or whose summary is not
-//     usable there, are listed;
-//  5. k_canon_fix: the listed tiles are re-read: missed chains are summarised again, and every
-//     chain still not usable gets run records (each run summarised under the binade of its own
-//     approximate start);
+//  4. k_canon_check: every chain's summary is checked at its approximate start; a chain whose
+//     summary is not usable there takes a slot for a wide table; tiles with a chain whose
+//     prediction missed the binade of its approximate start are listed;
+//  5. k_canon_fix: the listed tiles are re-read and their missed chains summarised again (a
+//     chain still not usable takes a slot); k_canon_wide: the wide table (canon_binade.h) of
+//     every slot's chain, simulated from its approximate start;
 //  6. k_canon_groups: the ordered join of each group of 64 tiles whose summaries are all usable
 //     under one hypothesis ("clean" groups);
 //  7. k_canon_walk: one wave per chain walks the groups in order from the exact start value:
 //     consecutive clean groups with one hypothesis join, and their joined summary is applied
 //     once; other groups are split into segments of usable same-hypothesis tiles, each applied
-//     once; a summary that fails is retried tile by tile; a tile that fails goes the same way
-//     through its run records (segments of runs, then runs, then a run's elements one by one),
-//     or, without records, is evaluated exactly by the wave (eval_chain).
+//     once; a summary that fails is retried tile by tile; a tile that fails applies its wide
+//     table, or, where that does not apply or it has none, is evaluated exactly by the wave
+//     (eval_chain).
 // The applies prove each result equal to the sequential fold, as in the single-pass form. A
-// wrong prediction costs only the re-read of its tile in step 5.
+// wrong prediction costs only the re-read of its tile in step 5. On zero-mean data, where the
+// running sums keep returning near zero, most tiles that cross a binade take one table apply.
 constexpr int kGroupTiles = 64;
 constexpr uint64_t kUsableBit = uint64_t(1) << 16;
 
@@ -668,11 +659,6 @@ struct GroupSum {
   uint64_t clean;  // every tile usable under that hypothesis
 };
 
-struct RunRec {
-  double d0, d1, lo0, lo1, hi0, hi1;
-  uint64_t hword;  // pack_binade(h) | kUsableBit when the run applies at its approximate start
-  uint64_t pad;
-};
 
 __device__ __forceinline__ RunFn fn_of(double d0, double d1, double lo0, double lo1, double hi0, double hi1) {
   RunFn f;
@@ -869,18 +855,38 @@ __global__ __launch_bounds__(summ_threads<RV>()) __attribute__((amdgpu_waves_per
   }
 }
 
-// 4. One thread per tile and chain: a chain whose stored hypothesis is the binade of its approximate
-// start only has its usability re-checked there; a tile with a chain whose hypothesis differs,
-// or with a chain whose summary is not usable, is listed for step 5.
+// Slot k + 1 for each lane of the wave where `need` holds (0 where it does not, or where the
+// rcap slots have run out): one atomic per wave on the slot counter; the slot's (tile, chain)
+// key goes to keys[k].
+__device__ __forceinline__ uint32_t take_slot(bool need, uint32_t key, uint32_t *counter, uint32_t rcap,
+                                              uint32_t *keys) {
+  const uint64_t m = __ballot(need);
+  if (!m) return 0;
+  const int leader = __builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(m)));
+  base = __shfl(base, leader);
+  const uint32_t k = base + static_cast<uint32_t>(__builtin_amdgcn_mbcnt_hi(
+                                static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0)));
+  if (!need || k >= rcap) return 0;
+  keys[k] = key;
+  return k + 1;
+}
+
+// 4. One thread per tile and chain: a chain whose stored hypothesis is the binade of its
+// approximate start has its usability re-checked there, and takes a wide-table slot when it is
+// not usable (its TileSum.pad is the slot + 1); a tile with a chain whose hypothesis differs
+// is listed for step 5.
 __global__ __launch_bounds__(kBlock) void k_canon_check(const double *pre, uint64_t tiles, TileSum *sums,
-                                                        uint32_t *list, uint32_t *counts) {
+                                                        uint32_t *list, uint32_t *counts, uint32_t rcap,
+                                                        uint32_t *keys) {
   // thread (tile, chain j): j = t % 16, so a tile's chains are 16 lanes of one wave
   const uint64_t g = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
   const uint64_t tile = g / 16;
   const int j = static_cast<int>(g % 16), l = lane_id();
-  bool listed = false;
+  bool listed = false, need = false;
+  TileSum *ts = sums + uint64_t(j) * tiles + tile;
   if (tile < tiles) {
-    TileSum *ts = sums + uint64_t(j) * tiles + tile;
     const double start = pre[tile * 16 + j];
     const Binade h = binade_of(start);
     const uint64_t hw = ts->hword;
@@ -889,33 +895,30 @@ __global__ __launch_bounds__(kBlock) void k_canon_check(const double *pre, uint6
       const bool usable = run_apply(fn_of(ts->d0, ts->d1, ts->lo0, ts->lo1, ts->hi0, ts->hi1), h, start, &probe);
       const uint64_t nw = pack_binade(h) | (usable ? kUsableBit : 0);
       if (nw != hw) ts->hword = nw;
-      listed = !usable;
+      need = !usable;
     } else {
       listed = true;
     }
   }
+  const uint32_t slot = take_slot(need, static_cast<uint32_t>(tile * 16 + j), counts + 1, rcap, keys);
+  if (slot) ts->pad = slot;
   const uint64_t any = __ballot(listed);
   if (tile < tiles && j == 0 && ((any >> (l & ~15)) & 0xFFFF)) list[atomicAdd(counts, 1u)] = static_cast<uint32_t>(tile);
 }
 
 // 5. The listed tiles, one block each (a grid-stride loop over the list): chains whose
-// hypothesis differs are summarised again under the binade of their approximate start; every
-// chain whose summary is then not usable gets a slot of run records, while slots last (its
-// TileSum.pad is the slot + 1): each run's summary under the binade of its own approximate
-// start (the tile's plus the approximate sums of the runs before it), marked usable when it
-// applies there.
+// hypothesis differs are summarised again under the binade of their approximate start, and
+// those whose summary is then not usable take a wide-table slot.
 template <class RV>
 __global__ __launch_bounds__(kFastThreads) void k_canon_fix(RV rv, const double *__restrict__ x,
                                                             const double *__restrict__ y, uint64_t blocks,
                                                             const double *pre, uint64_t tiles, TileSum *sums,
-                                                            const uint32_t *list, uint32_t *counts,
-                                                            RunRec *recs, uint32_t rcap) {
+                                                            const uint32_t *list, uint32_t *counts, uint32_t rcap,
+                                                            uint32_t *keys) {
   __shared__ RunFn s_f[kFastWaves][16];
-  __shared__ double s_a[kFastWaves][16];
   __shared__ int s_mis[16];
-  __shared__ uint32_t s_slot[16];
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  const int j = t & 15, r = t >> 4, q = l >> 4;
+  const int j = t & 15, r = t >> 4;
   const uint32_t n = counts[0];
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
     const uint64_t tile = list[i];
@@ -924,25 +927,15 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fix(RV rv, const double 
     const double start = pre[tile * 16 + j];
     const Binade h = binade_of(start);
     if (t < 16) s_mis[t] = (sums[uint64_t(t) * tiles + tile].hword & ~kUsableBit) != pack_binade(h);
-    double v[kRunBlocks];
-    const int cnt = nb - r * kRunBlocks;
-    load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
-    // run sums, their inclusive prefix over the wave's four runs of the chain, wave totals
-    double a = 0.0;
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k) a = a + v[k];
-    double inc = a;
-    {
-      const double o = __shfl_up(inc, 16);
-      if (q >= 1) inc = o + inc;
-      const double o2 = __shfl_up(inc, 32);
-      if (q >= 2) inc = o2 + inc;
-    }
-    if (q == 3) s_a[w][j] = inc;
     __syncthreads();
     const bool mis = s_mis[j] != 0;
     RunFn f = run_identity();
-    if (mis && h.ok) f = run_of(v, cnt, binade_ref(h));
+    if (mis && h.ok) {
+      double v[kRunBlocks];
+      const int cnt = nb - r * kRunBlocks;
+      load_run(rv, x, y, b0 + r * kRunBlocks, j, cnt, v);
+      f = run_of(v, cnt, binade_ref(h));
+    }
     {
       RunFn o = shfl_xor_pod(f, 16);
       f = (l & 16) ? run_join(o, f) : run_join(f, o);
@@ -953,41 +946,92 @@ __global__ __launch_bounds__(kFastThreads) void k_canon_fix(RV rv, const double 
     __syncthreads();
     if (w == 0) {
       const RunFn ft = tile_tree(s_f, j, l);
-      if (l < 16) {
-        TileSum *ts = sums + uint64_t(l) * tiles + tile;
-        bool usable;
-        if (s_mis[l]) {
-          double probe;
-          usable = run_apply(ft, h, start, &probe);
-          *ts = TileSum{ft.d[0], ft.d[1], ft.lo[0], ft.lo[1], ft.hi[0], ft.hi[1],
-                        pack_binade(h) | (usable ? kUsableBit : 0), 0};
-        } else {
-          usable = (ts->hword & kUsableBit) != 0;
-        }
-        uint32_t slot = 0;
-        if (!usable) {
-          const uint32_t k = atomicAdd(counts + 1, 1u);
-          if (k < rcap) slot = k + 1;
-        }
-        s_slot[l] = slot;
-        ts->pad = slot;
+      TileSum *ts = sums + uint64_t(j) * tiles + tile;
+      bool need = false;
+      if (l < 16 && s_mis[l]) {
+        double probe;
+        const bool usable = run_apply(ft, h, start, &probe);
+        *ts = TileSum{ft.d[0], ft.d[1], ft.lo[0], ft.lo[1], ft.hi[0], ft.hi[1],
+                      pack_binade(h) | (usable ? kUsableBit : 0), 0};
+        need = !usable;
       }
+      const uint32_t slot = take_slot(need, static_cast<uint32_t>(tile * 16 + j), counts + 1, rcap, keys);
+      if (slot) ts->pad = slot;
     }
     __syncthreads();
-    const uint32_t slot = s_slot[j];
-    if (slot) {
-      double rs = start;
-      for (int ww = 0; ww < w; ++ww) rs = rs + s_a[ww][j];
-      rs = rs + (inc - a);
-      const Binade hr = binade_of(rs);
-      RunFn fr = run_identity();
-      if (hr.ok) fr = run_of(v, cnt, binade_ref(hr));
-      double probe;
-      const bool usable = r * kRunBlocks < nb && run_apply(fr, hr, rs, &probe);
-      recs[uint64_t(slot - 1) * kWave + r] = RunRec{fr.d[0], fr.d[1], fr.lo[0], fr.lo[1], fr.hi[0], fr.hi[1],
-                                                    pack_binade(hr) | (usable ? kUsableBit : 0), 0};
+  }
+}
+
+// 5b. The wide tables (canon_binade.h) of the chains that took a slot, four chains per wave
+// (one wave per block): the four chains' elements are first staged in LDS, every lane loading
+// 64 of them in one round trip, then lane 16 q + r simulates residue r of slot s0 + q over the
+// tile's blocks from Z_r = B + r u0, reading the elements from LDS (past the tile's end they
+// are zeros, which leave a nonzero running value as it is, so the loop bound is uniform). The
+// slot's 16 lanes then agree on M and keep 2^(M + 1) entries; the other entries never apply.
+constexpr int kWideSlots = 4;
+template <class RV>
+__global__ __launch_bounds__(kWave) void k_canon_wide(RV rv, const double *__restrict__ x,
+                                                      const double *__restrict__ y, uint64_t blocks,
+                                                      const double *pre, const uint32_t *keys,
+                                                      const uint32_t *counts, uint32_t rcap, WideHead *heads,
+                                                      WideEnt *ents) {
+  __shared__ double s_v[kWideSlots][kTileBlocks];
+  const uint32_t nslots = counts[1] < rcap ? counts[1] : rcap;
+  const int l = lane_id(), q = l >> 4, r = l & 15;
+  for (uint32_t s0 = blockIdx.x * kWideSlots; s0 < nslots; s0 += gridDim.x * kWideSlots) {
+    const uint32_t s = s0 + q;
+    const bool live = s < nslots;
+    const uint32_t key = live ? keys[s] : 0;
+    const uint64_t tile = key >> 4;
+    const int j = static_cast<int>(key & 15);
+    const double a = live ? pre[tile * 16 + j] : 0.0;
+    const Binade h = binade_of(a);
+    const bool go = live && h.ok;
+    const uint64_t b0 = tile * kTileBlocks;
+    const int nb = go ? static_cast<int>(blocks - b0 < uint64_t(kTileBlocks) ? blocks - b0 : kTileBlocks) : 0;
+    // stage: lane l loads elements l + 64 m of each of the four chains
+    {
+      double st[kWideSlots][kTileBlocks / kWave];
+#pragma unroll
+      for (int qq = 0; qq < kWideSlots; ++qq) {
+        const uint64_t bq = __shfl(b0, 16 * qq);
+        const int jq = __shfl(j, 16 * qq), nq = __shfl(nb, 16 * qq);
+#pragma unroll
+        for (int m = 0; m < kTileBlocks / kWave; ++m) {
+          const int k = l + kWave * m;
+          st[qq][m] = k < nq ? rv(x, y, (bq + k) * 16 + jq) : 0.0;
+        }
+      }
+#pragma unroll
+      for (int qq = 0; qq < kWideSlots; ++qq)
+#pragma unroll
+        for (int m = 0; m < kTileBlocks / kWave; ++m) s_v[qq][l + kWave * m] = st[qq][m];
     }
     __syncthreads();
+    const double base = go ? wide_base(a, h) : 1.0;
+    WideAcc acc = wide_start(base + r * pow2(go ? h.e - 52 : -52));
+#pragma unroll 2
+    for (int k0 = 0; k0 < kTileBlocks; k0 += 16) {
+      double v[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = s_v[q][k0 + m];
+      wide_push_n(acc, v);
+    }
+    __syncthreads();
+    const WideEnt e = wide_end(acc);
+    int mm = e.lo <= e.hi ? acc.emax - h.e : -2000;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int o = __shfl_xor(mm, d);
+      mm = o > mm ? o : mm;
+    }
+    const int nres = go ? wide_nres(mm) : 0;
+    if (live) {
+      // every entry is written: the walk stages all 16
+      const double inf = __builtin_inf();
+      if (r == 0) heads[s] = WideHead{base, h.e, nres};
+      ents[uint64_t(s) * kWideRes + r] = r < nres ? e : WideEnt{0.0, inf, -inf};
+    }
   }
 }
 
@@ -1077,70 +1121,44 @@ __device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, d
   return cur;
 }
 
-// Chain j's blocks b0 .. b0 + nb - 1 (nb >= 1) from the exact value s with the tile's run records
-// (lane = run); a run whose record does not apply is folded element by element, its values
-// loaded ahead when its record was not usable at its approximate start.
-template <class RV>
-__device__ double walk_runs(const RV &rv, const double *x, const double *y, const RunRec *rec, uint64_t b0,
-                            int nb, int j, double s) {
-  const int l = lane_id();
-  const int nruns = (nb + kRunBlocks - 1) / kRunBlocks;
-  RunFn f = run_identity();
-  uint64_t hw = 0;
-  if (l < nruns) {
-    const RunRec rr = rec[l];
-    f = fn_of(rr.d0, rr.d1, rr.lo0, rr.lo1, rr.hi0, rr.hi1);
-    hw = rr.hword;
-  }
-  const int cnt = nb - l * kRunBlocks;
-  double v[kRunBlocks];
-  const bool ahead = l < nruns && !(hw & kUsableBit);
-#pragma unroll
-  for (int k = 0; k < kRunBlocks; ++k) v[k] = 0.0;
-  if (ahead) load_run(rv, x, y, b0 + l * kRunBlocks, j, cnt, v);
-  const uint64_t loaded = __ballot(ahead);
-  return walk_lanes(f, hw, nruns, false, s, [&](int i, double cur) {
-    const int ci = nb - i * kRunBlocks < kRunBlocks ? nb - i * kRunBlocks : kRunBlocks;
-    double u[kRunBlocks];
-    if ((loaded >> i) & 1) {
-#pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k) u[k] = bcast_pod(v[k], i);
-    } else {
-      // lane k < ci loads element k of run i
-      const double e = l < ci ? rv(x, y, (b0 + uint64_t(i) * kRunBlocks + l) * 16 + j) : 0.0;
-#pragma unroll
-      for (int k = 0; k < kRunBlocks; ++k) u[k] = bcast_pod(e, k);
-    }
-#pragma unroll
-    for (int k = 0; k < kRunBlocks; ++k)
-      if (k < ci) cur = cur + u[k];
-    return cur;
-  });
-}
-
 // The tiles of group g from the exact value cur: lane = tile, through walk_lanes; a tile whose
-// summary does not apply goes by its run records, or, without a slot, through eval_chain.
+// summary does not apply goes by its wide table, staged in LDS (s_ent[lane][r]) when the group is
+// loaded, or, where that does not apply or it has none, through eval_chain.
 template <class RV>
 __device__ double walk_group(const RV &rv, const double *x, const double *y, uint64_t blocks,
-                             const TileSum *sums, uint64_t tiles, const RunRec *recs, int j, uint64_t g,
-                             double cur, bool singles) {
+                             const TileSum *sums, uint64_t tiles, const WideHead *heads, const WideEnt *ents,
+                             WideEnt (*s_ent)[kWideRes], int j, uint64_t g, double cur, bool singles) {
   const int l = lane_id();
   const uint64_t tile = g * kGroupTiles + l;
   const int n = tiles - g * kGroupTiles < uint64_t(kWave) ? static_cast<int>(tiles - g * kGroupTiles) : kWave;
   RunFn f = run_identity();
   uint64_t hw = 0, slot = 0;
+  WideHead wh{0.0, 0, 0};
   if (l < n) {
     const TileSum ts = sums[uint64_t(j) * tiles + tile];
     f = fn_of(ts.d0, ts.d1, ts.lo0, ts.lo1, ts.hi0, ts.hi1);
     hw = ts.hword;
     slot = ts.pad;
   }
-  return walk_lanes(f, hw, n, singles, cur, [&](int i, double c) {
+  if (slot) {
+    wh = heads[slot - 1];
+    WideEnt e[kWideRes];
+#pragma unroll
+    for (int r = 0; r < kWideRes; ++r) e[r] = ents[(slot - 1) * kWideRes + r];
+#pragma unroll
+    for (int r = 0; r < kWideRes; ++r) s_ent[l][r] = e[r];
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const double out = walk_lanes(f, hw, n, singles, cur, [&](int i, double c) {
     const uint64_t tb = (g * kGroupTiles + i) * kTileBlocks;
     const int nb = static_cast<int>(blocks - tb < uint64_t(kTileBlocks) ? blocks - tb : kTileBlocks);
-    const uint64_t si = bcast_pod(slot, i);
-    return si ? walk_runs(rv, x, y, recs + (si - 1) * kWave, tb, nb, j, c) : eval_chain(rv, x, y, tb, nb, j, c);
+    double o;
+    if (bcast_pod(slot, i) && wide_apply(bcast_pod(wh, i), s_ent[i], c, &o)) return o;
+    return eval_chain(rv, x, y, tb, nb, j, c);
   });
+  __builtin_amdgcn_wave_barrier();
+  return out;
 }
 
 // 7. One wave per chain: groups 64 at a time through walk_lanes (lane = group; a clean group's
@@ -1149,8 +1167,10 @@ __device__ double walk_group(const RV &rv, const double *x, const double *y, uin
 template <class RV>
 __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__restrict__ x,
                                                       const double *__restrict__ y, uint64_t blocks,
-                                                      const TileSum *sums, uint64_t tiles, const RunRec *recs,
-                                                      const GroupSum *gsum, uint64_t groups, double *acc) {
+                                                      const TileSum *sums, uint64_t tiles, const WideHead *heads,
+                                                      const WideEnt *ents, const GroupSum *gsum, uint64_t groups,
+                                                      double *acc) {
+  __shared__ WideEnt s_ent[kWave][kWideRes];
   const int j = blockIdx.x, l = lane_id();
   double cur = acc[j];
   for (uint64_t g0 = 0; g0 < groups; g0 += kWave) {
@@ -1163,7 +1183,7 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
       hw = gs.hword | (gs.clean ? kUsableBit : 0);
     }
     cur = walk_lanes(f, hw, n, false, cur, [&](int i, double c) {
-      return walk_group(rv, x, y, blocks, sums, tiles, recs, j, g0 + i, c, false);
+      return walk_group(rv, x, y, blocks, sums, tiles, heads, ents, s_ent, j, g0 + i, c, false);
     });
   }
   if (l == 0) acc[j] = cur;
@@ -1248,12 +1268,14 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
   if (tiles >= kStreamMinTiles) {
     const uint64_t groups = (tiles + kGroupTiles - 1) / kGroupTiles;
     const uint64_t pairs = tiles * 16;
-    const uint32_t rcap = static_cast<uint32_t>(pairs < 8192 + pairs / 8 ? pairs : 8192 + pairs / 8);
+    const uint32_t rcap = static_cast<uint32_t>(pairs < 8192 + pairs / 4 ? pairs : 8192 + pairs / 4);
     double *asum = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
     double *pre = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
     auto *sums = static_cast<TileSum *>(tmp_alloc(c, tiles * 16 * sizeof(TileSum)));
     auto *gsum = static_cast<GroupSum *>(tmp_alloc(c, groups * 16 * sizeof(GroupSum)));
-    auto *recs = static_cast<RunRec *>(tmp_alloc(c, uint64_t(rcap) * kWave * sizeof(RunRec)));
+    auto *keys = static_cast<uint32_t *>(tmp_alloc(c, uint64_t(rcap) * 4));
+    auto *heads = static_cast<WideHead *>(tmp_alloc(c, uint64_t(rcap) * sizeof(WideHead)));
+    auto *ents = static_cast<WideEnt *>(tmp_alloc(c, uint64_t(rcap) * kWideRes * sizeof(WideEnt)));
     auto *list = static_cast<uint32_t *>(tmp_alloc(c, tiles * 4 + 8));
     uint32_t *counts = list + tiles;  // [0] listed tiles, [1] record slots taken
     const uint32_t fix_grid = static_cast<uint32_t>(tiles < 512 ? tiles : 512);
@@ -1263,18 +1285,24 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
                                                                     counts);
     k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
     k_canon_check<<<uint32_t((tiles * 16 + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(pre, tiles, sums, list,
-                                                                                       counts);
-    k_canon_fix<RV><<<fix_grid, kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, list, counts, recs,
-                                                            rcap);
+                                                                                       counts, rcap, keys);
+    k_canon_fix<RV><<<fix_grid, kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, list, counts, rcap,
+                                                            keys);
+    const uint32_t wide_grid = static_cast<uint32_t>((rcap + kWideSlots - 1) / kWideSlots < 4096
+                                                          ? (rcap + kWideSlots - 1) / kWideSlots
+                                                          : 4096);
+    k_canon_wide<RV><<<wide_grid, kWave, 0, c.stream>>>(rv, x, y, blocks, pre, keys, counts, rcap, heads, ents);
     k_canon_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(sums, tiles, groups,
                                                                                               gsum);
-    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, recs, gsum, groups, acc);
+    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, heads, ents, gsum, groups, acc);
     check_hip(hipGetLastError(), "canon streamed");
     tmp_free(c, asum);
     tmp_free(c, pre);
     tmp_free(c, sums);
     tmp_free(c, gsum);
-    tmp_free(c, recs);
+    tmp_free(c, keys);
+    tmp_free(c, heads);
+    tmp_free(c, ents);
     tmp_free(c, list);
     return;
   }

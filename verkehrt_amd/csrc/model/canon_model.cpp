@@ -38,8 +38,7 @@ RunFn run_at(const double *vals, uint64_t b, uint64_t b1, int j, const BinadeRef
 // most two changes of the per-run binade hypotheses (each run's approximate start is s plus
 // the approximate sums of the runs before it, summed in the device's scan order), rounds:
 // summaries under the binade of the current value, applied up to the first run whose check
-// fails, that run folded element by element. Otherwise each run is summarised under its own
-// hypothesis and applied in order, a run whose check fails folded element by element.
+// fails, that run folded element by element. Otherwise the elements are folded one by one.
 double eval_chain(const double *vals, uint64_t b0, uint64_t b1, int j, double s, uint64_t *rounds) {
   const uint64_t nruns = (b1 - b0 + kRun - 1) / kRun;
   auto run_of = [&](uint64_t r, const Binade &h) {
@@ -90,16 +89,30 @@ double eval_chain(const double *vals, uint64_t b0, uint64_t b1, int j, double s,
     }
     return cur;
   }
-  for (uint64_t r = 0; r < nruns; ++r) {
-    double out;
-    if (run_apply(run_of(r, ha[r]), ha[r], cur, &out)) {
-      cur = out;
-    } else {
-      ++*rounds;
-      cur = fold_run(r, cur);
-    }
-  }
+  for (uint64_t r = 0; r < nruns; ++r) cur = fold_run(r, cur);
   return cur;
+}
+
+// The wide table (canon_binade.h) of chain j over blocks [b0, b1): 16 residues simulated from
+// B = approx less its residue modulo 16 u0, then the entries the greatest rise M needs.
+void wide_table(const double *vals, uint64_t b0, uint64_t b1, int j, double approx, WideHead *hd, WideEnt *ents) {
+  const Binade h = binade_of(approx);
+  *hd = WideHead{0.0, 0, 0};
+  if (!h.ok) return;
+  const double b = wide_base(approx, h), u0 = pow2(h.e - 52);
+  int m = -2000;
+  for (int r = 0; r < kWideRes; ++r) {
+    // as the device: 16 elements at a time (zeros past b1)
+    WideAcc a = wide_start(b + r * u0);
+    for (uint64_t k0 = b0; k0 < b1; k0 += 16) {
+      double v[16];
+      for (uint64_t m = 0; m < 16; ++m) v[m] = k0 + m < b1 ? vals[(k0 + m) * 16 + j] : 0.0;
+      wide_push_n(a, v);
+    }
+    ents[r] = wide_end(a);
+    if (ents[r].lo <= ents[r].hi) m = std::max(m, a.emax - h.e);
+  }
+  *hd = WideHead{b, h.e, wide_nres(m)};
 }
 
 }  // namespace
@@ -185,17 +198,18 @@ extern "C" int vkm_canonical_fold(const double *vals, uint64_t blocks, double *a
 // under one hypothesis; the walk takes 64 groups at a time, joins consecutive clean groups of
 // one hypothesis and applies the joined summary to the exact value (group by group where that
 // fails), and splits other groups into segments of usable same-hypothesis tiles, each applied
-// once or tile by tile, a tile that fails evaluated through its run records (with `records`) or by the
-// fallback wave.
+// once or tile by tile, a tile that fails evaluated through its wide table (`records` bit 1),
+// then its run records (bit 0) or by the fallback wave.
 // stats: [0] joined clean spans applied, [1] spans that failed, [2] segments applied, [3] tiles
-// whose summary did not apply, [4] run segments applied, [5] runs folded element by element.
+// whose summary did not apply, [4] run segments applied, [5] runs folded element by element,
+// [6] tiles applied from their wide table, [7] wide tables that did not apply.
 extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, double *acc,
                                            int group_tiles, int records, uint64_t *stats) {
   const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
   if (group_tiles < 1) group_tiles = 1;
   const uint64_t gt = static_cast<uint64_t>(group_tiles);
   const uint64_t groups = (tiles + gt - 1) / gt;
-  uint64_t st[6] = {0, 0, 0, 0, 0, 0}, rounds = 0;
+  uint64_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0}, rounds = 0;
   for (int j = 0; j < 16; ++j) {
     // 1. approximate tile sums as the first summary pass forms them: each run's elements in
     //    order (from 0.0, missing elements as 0.0), runs in fours, fours in fours, and so on
@@ -305,11 +319,26 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
       }
       return cur;
     };
+    // the tile's wide table (canon_binade.h) built from its approximate start
+    auto by_wide = [&](uint64_t t, double cur, double *out) {
+      const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
+      WideHead hd;
+      WideEnt ents[kWideRes];
+      wide_table(vals, b0, b1, j, pre[t], &hd, ents);
+      return wide_apply(hd, ents, cur, out);
+    };
     auto one = [&](uint64_t t, double cur) {
       double out;
       if (usable[t] && run_apply(fn[t], hyp[t], cur, &out)) return out;
       ++st[3];
-      if (records) return by_runs(t, cur);
+      if (records & 2) {
+        if (by_wide(t, cur, &out)) {
+          ++st[6];
+          return out;
+        }
+        ++st[7];
+      }
+      if (records & 1) return by_runs(t, cur);
       const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
       return eval_chain(vals, b0, b1, j, cur, &rounds);
     };
@@ -372,6 +401,23 @@ extern "C" int vkm_canonical_fold_streamed(const double *vals, uint64_t blocks, 
     acc[j] = cur;
   }
   if (stats)
-    for (int k = 0; k < 6; ++k) stats[k] = st[k];
+    for (int k = 0; k < 8; ++k) stats[k] = st[k];
+  return 0;
+}
+
+// The wide table of chain j over blocks [b0, b1) from `approx`, as k_canon_wide builds it.
+extern "C" int vkm_wide_table_apply(const double *chain, uint64_t n, double approx, const double *starts,
+                                    uint64_t nstarts, double *out, uint8_t *applied, int32_t *nres) {
+  WideHead hd;
+  WideEnt ents[kWideRes];
+  std::vector<double> vals(n * 16, 0.0);
+  for (uint64_t i = 0; i < n; ++i) vals[i * 16] = chain[i];
+  wide_table(vals.data(), 0, n, 0, approx, &hd, ents);
+  *nres = hd.nres;
+  for (uint64_t i = 0; i < nstarts; ++i) {
+    double o = 0.0;
+    applied[i] = wide_apply(hd, ents, starts[i], &o) ? 1 : 0;
+    out[i] = o;
+  }
   return 0;
 }
