@@ -579,6 +579,58 @@ def region_suite(D, x1, x2, n):
     return out
 
 
+def _timed_calls(D, name, args, reps=3):
+    """One warm-up call, then `reps` traced calls: (results, mean microseconds per call)."""
+    D.call(name, args)
+    D.ctx.sync()
+    D.ctx.set_trace(True)
+    D.ctx.take_trace()
+    res = [D.call(name, args) for _ in range(reps)]
+    D.ctx.sync()
+    D.ctx.set_trace(False)
+    return res, float(np.mean([t for _, t in parse_trace(D.ctx.take_trace())]))
+
+
+def zero_mean_suite(D, x1, n: int, verify: bool = True):
+    """reduce_add_f64 and dot_f64 over normal(0,1) rows (numpy's generator, seeds 5 and 6, made on
+    the host and copied in): every chain's running sum keeps returning near zero, so thousands
+    of tiles per chain cross binades, the canonical form's hardest input. Also scan_add_f64
+    (out[i] = the canonical sum of x[0..=i]) over the same rows and over the uniform rows x1.
+    Timed per call over 3 calls after one warm-up, then checked bit for bit against the oracle."""
+    from verkehrt_amd import Array
+    rng5, rng6 = np.random.default_rng(5), np.random.default_rng(6)
+    h, g = rng5.normal(size=n), rng6.normal(size=n)
+    x, y = Array.from_numpy(D.ctx, h), Array.from_numpy(D.ctx, g)
+    out = {"data": "normal(0,1) rows, numpy default_rng seeds 5 and 6", "rows": n}
+
+    def entry(us, nbytes):
+        return {"us": round(us, 1), "GBps": round(nbytes / us / 1e3, 1),
+                "frac": round(nbytes / us / 1e3 / HBM_PEAK_GBPS, 4)}
+
+    for name, args, nbytes in (("reduce_add_f64", [x], 8 * n), ("dot_f64", [x, y], 16 * n)):
+        res, us = _timed_calls(D, name, args)
+        out[name] = entry(us, nbytes)
+        if verify:
+            from oracle import oracle as O
+            want = O.canonical_sum_f64(h) if name == "reduce_add_f64" else O.dot_f64(h, g)
+            out[name]["verified"] = all(r.value == want for r in res)
+    del y
+    scans = {}
+    for label, arr in (("normal", x), ("uniform", x1)):
+        res, us = _timed_calls(D, "scan_add_f64", [arr], reps=2)
+        scans[label] = entry(us, 16 * n)
+        if verify:
+            from oracle import oracle as O
+            host = h if label == "normal" else O.rand_f64(n, 1)
+            want = O.scan_add_f64(host).view(np.uint64)
+            scans[label]["verified"] = all(np.array_equal(r.to_numpy().view(np.uint64), want) for r in res)
+            del host, want
+        del res
+    out["scan_add_f64"] = scans
+    del x
+    return out
+
+
 def verify_streaming(D, calls, n, table_rows):
     """Each streaming primitive's 2^28-row result, computed once more after the timed region,
     against the oracle on the same inputs (as benches/tier1.rs:403-433 asserts before timing)."""
@@ -775,6 +827,7 @@ def main():
         # random 8-byte rows: each costs a 128-byte line, so the bound is the random-row rate
         roofline["random_rows_G_per_s"] = round(n / prims[dom]["us"] / 1e3, 1)
     regions = None if args.only else region_suite(D, x1, x2, n)
+    zero_mean = None if args.only else zero_mean_suite(D, x1, n, verify=not args.no_verify)
     checks = None if (args.no_verify or args.only) else verify_streaming(D, calls, n, table_rows)
     sort = None if (args.no_sort or args.only) else sort_suite(D, verify=not args.no_verify)
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
@@ -782,7 +835,9 @@ def main():
         batched["literal_range"] = batched_literal_suite(D, args.batch_arrays)
     serve = None if (args.no_serve or args.only) else serve_suite(args.serve_requests, args.serve_qps)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.cpu_rows, args.cpu_seconds)
-    verified = checks is not None and all(checks.values()) and (sort is None or sort["verified"])
+    verified = (checks is not None and all(checks.values()) and (sort is None or sort["verified"])
+                and (zero_mean is None or (all(zero_mean[k]["verified"] for k in ("reduce_add_f64", "dot_f64"))
+                                           and all(v["verified"] for v in zero_mean["scan_add_f64"].values()))))
     line = {
         "metric": "HBM GB/s per primitive (% of peak); batched-vs-plain speedup; serve p99 ms",
         "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -798,6 +853,7 @@ def main():
         "checks": checks,
         "primitives": prims,
         "regions": regions,
+        "zero_mean": zero_mean,
         "batched": batched,
         "serve": serve,
         "sort": sort,

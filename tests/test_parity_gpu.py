@@ -366,6 +366,33 @@ def test_scan_add_f64_defined_order(D):
         assert same_array(O.scan_add_f64(x), D.call("scan_add_f64", [view(D, x, 1)]).to_numpy()), n
 
 
+@pytest.mark.parametrize("n", [16384, 16384 * 3 + 5, 1_000_003, 4_000_037])
+def test_scan_add_f64_tiles(D, n):
+    # from 1024 whole blocks the exact tile starts come from the canonical fold's walk and the
+    # tiles run in parallel: uniform, zero-mean, specials (NaN, inf, subnormals, +-0), parity
+    # ties, and a sum drifting through zero
+    rng = np.random.default_rng(n)
+    for name, x in fast_path_sets(rng, n):
+        got = D.call("scan_add_f64", [view(D, x, 1)]).to_numpy()
+        assert same_array(O.scan_add_f64(x), got), (name, n)
+
+
+def test_scan_add_f64_tiles_scanner(D):
+    # a scanner fed in updates of whole blocks (the last one partial): each update continues
+    # from the accumulators the previous left, above and below the tiled form's threshold
+    from verkehrt_amd import Array
+    rng = np.random.default_rng(160)
+    n = 3_000_007
+    x = rng.normal(size=n) * 100.0 + 0.125
+    dx = D.array(x)
+    out = Array.empty(D.ctx, "f64", n)
+    sc = D.scanner("scan_add_f64")
+    cuts = [0, 16 * 50, 16 * 70_000, 16 * 71_000, 16 * 150_000, n]
+    for a, b in zip(cuts, cuts[1:]):
+        sc.update([dx.slice(a, b - a)], out.slice(a, b - a))
+    assert same_array(O.scan_add_f64(x), out.to_numpy())
+
+
 # ---------------------------------------------------------------------- gather
 
 @pytest.mark.parametrize("t,dt", [("f64", np.float64), ("i64", np.int64), ("i32", np.int32), ("u8", np.uint8)])

@@ -7,6 +7,8 @@
 // is the dependent-add latency, n/16 adds in a row. The fast form (sums only) evaluates the
 // same 16 chains in parallel with binade run summaries (canon_binade.h) and returns the same
 // bits; it is HBM-bound on data whose running sums stay inside a binade for long stretches.
+#include <type_traits>
+
 #include "canon_binade.h"
 #include "chain.h"
 #include "vk_device.h"
@@ -1071,13 +1073,23 @@ __global__ __launch_bounds__(kBlock) void k_canon_groups(const TileSum *sums, ui
   }
 }
 
+// No record of the exact values before each lane.
+struct NoRec {
+  __device__ void operator()(int, int, double) const {}
+};
+
 // Lanes 0 .. n - 1 of the wave in order from the exact value cur: lane i holds a summary f
 // under the hypothesis word hw (kUsableBit: it applied at its approximate start). Segments of
 // consecutive usable lanes with one hypothesis are applied at once (with `singles`, every lane
 // is its own segment); a segment that fails goes lane by lane, and a lane whose summary does
 // not apply goes through one(i, cur), which returns the exact value after lane i.
-template <class One>
-__device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, double cur, One &&one) {
+// rec(a, b, v) receives the exact value before every lane: lanes a .. b each hold their own v.
+// Inside a segment applied at once, lane i's value is the segment's prefix before it (the
+// exclusive scan) applied to the segment's start; that apply always holds, because the
+// prefix's running range lies inside the whole segment's, which held.
+template <class One, class Rec = NoRec>
+__device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, double cur, One &&one,
+                             Rec &&rec = Rec{}) {
   const int l = lane_id();
   const bool valid = l < n;
   const bool usable = valid && (hw & kUsableBit);
@@ -1098,6 +1110,10 @@ __device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, d
     }
   }
   const uint64_t starts = __ballot(start && valid);
+  // the exclusive prefix within the segment (for the records): the previous lane's scan
+  constexpr bool kRec = !std::is_same<std::decay_t<Rec>, NoRec>::value;
+  RunFn scx = run_identity();
+  if (kRec) scx = shfl_up_pod(sc, 1);
   // segment ends: the lane before each start, and lane n - 1
   uint64_t ends = ((starts >> 1) | (uint64_t(1) << (n - 1))) & (n == kWave ? ~uint64_t(0) : (uint64_t(1) << n) - 1);
   int first = 0;
@@ -1107,9 +1123,15 @@ __device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, d
     const uint64_t hw_e = bcast_pod(hw, e);
     double out;
     if ((hw_e & kUsableBit) && run_apply(bcast_pod(sc, e), unpack_binade(hw_e & ~kUsableBit), cur, &out)) {
+      if (kRec) {
+        double v = cur;
+        if (l > first && l <= e) run_apply(scx, unpack_binade(hw_e & ~kUsableBit), cur, &v);
+        rec(first, e, v);
+      }
       cur = out;
     } else {
       for (int i = first; i <= e; ++i) {
+        rec(i, i, cur);
         const uint64_t hw_i = bcast_pod(hw, i);
         if (!((hw_i & kUsableBit) && run_apply(bcast_pod(f, i), unpack_binade(hw_i & ~kUsableBit), cur, &out)))
           out = one(i, cur);
@@ -1124,10 +1146,11 @@ __device__ double walk_lanes(const RunFn &f, uint64_t hw, int n, bool singles, d
 // The tiles of group g from the exact value cur: lane = tile, through walk_lanes; a tile whose
 // summary does not apply goes by its wide table, staged in LDS (s_ent[lane][r]) when the group is
 // loaded, or, where that does not apply or it has none, through eval_chain.
-template <class RV>
+template <class RV, class Rec = NoRec>
 __device__ double walk_group(const RV &rv, const double *x, const double *y, uint64_t blocks,
                              const TileSum *sums, uint64_t tiles, const WideHead *heads, const WideEnt *ents,
-                             WideEnt (*s_ent)[kWideRes], int j, uint64_t g, double cur, bool singles) {
+                             WideEnt (*s_ent)[kWideRes], int j, uint64_t g, double cur, bool singles,
+                             Rec &&rec = Rec{}) {
   const int l = lane_id();
   const uint64_t tile = g * kGroupTiles + l;
   const int n = tiles - g * kGroupTiles < uint64_t(kWave) ? static_cast<int>(tiles - g * kGroupTiles) : kWave;
@@ -1156,7 +1179,7 @@ __device__ double walk_group(const RV &rv, const double *x, const double *y, uin
     double o;
     if (bcast_pod(slot, i) && wide_apply(bcast_pod(wh, i), s_ent[i], c, &o)) return o;
     return eval_chain(rv, x, y, tb, nb, j, c);
-  });
+  }, rec);
   __builtin_amdgcn_wave_barrier();
   return out;
 }
@@ -1169,7 +1192,7 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
                                                       const double *__restrict__ y, uint64_t blocks,
                                                       const TileSum *sums, uint64_t tiles, const WideHead *heads,
                                                       const WideEnt *ents, const GroupSum *gsum, uint64_t groups,
-                                                      double *acc) {
+                                                      double *acc, double *gstart) {
   __shared__ WideEnt s_ent[kWave][kWideRes];
   const int j = blockIdx.x, l = lane_id();
   double cur = acc[j];
@@ -1182,11 +1205,36 @@ __global__ __launch_bounds__(kWave) void k_canon_walk(RV rv, const double *__res
       f = fn_of(gs.d0, gs.d1, gs.lo0, gs.lo1, gs.hi0, gs.hi1);
       hw = gs.hword | (gs.clean ? kUsableBit : 0);
     }
-    cur = walk_lanes(f, hw, n, false, cur, [&](int i, double c) {
+    auto one = [&](int i, double c) {
       return walk_group(rv, x, y, blocks, sums, tiles, heads, ents, s_ent, j, g0 + i, c, false);
-    });
+    };
+    if (gstart) {
+      cur = walk_lanes(f, hw, n, false, cur, one, [&](int a, int b, double v) {
+        if (l >= a && l <= b) gstart[(g0 + l) * 16 + j] = v;
+      });
+    } else {
+      cur = walk_lanes(f, hw, n, false, cur, one);
+    }
   }
   if (l == 0) acc[j] = cur;
+}
+
+// For scans: one wave per (group, chain) from the group's exact start (recorded by the walk)
+// through walk_group, recording the exact start of every tile, tstart[tile * 16 + j].
+template <class RV>
+__global__ __launch_bounds__(kWave) void k_canon_expand(RV rv, const double *__restrict__ x,
+                                                        const double *__restrict__ y, uint64_t blocks,
+                                                        const TileSum *sums, uint64_t tiles, const WideHead *heads,
+                                                        const WideEnt *ents, uint64_t groups, const double *gstart,
+                                                        double *tstart) {
+  __shared__ WideEnt s_ent[kWave][kWideRes];
+  const uint64_t g = blockIdx.x / 16;
+  const int j = static_cast<int>(blockIdx.x % 16), l = lane_id();
+  if (g >= groups) return;
+  walk_group(rv, x, y, blocks, sums, tiles, heads, ents, s_ent, j, g, gstart[g * 16 + j], false,
+             [&](int a, int b, double v) {
+               if (l >= a && l <= b) tstart[(g * kGroupTiles + l) * 16 + j] = v;
+             });
 }
 
 template <class RV>
@@ -1255,6 +1303,12 @@ void with_row(const CanonSpec &s, F &&f) {
 
 namespace {
 
+// The streamed form (below kStreamMinTiles tiles only when tstart is asked for). With tstart,
+// the exact start of every tile and chain is written there (tstart[tile * 16 + j]).
+template <class RV>
+void canon_streamed(Ctx &c, RV rv, const double *x, const double *y, uint64_t blocks, double *acc,
+                    double *tstart);
+
 template <class RV>
 void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, uint64_t blocks,
                      double *acc) {
@@ -1266,6 +1320,24 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
   }
   const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
   if (tiles >= kStreamMinTiles) {
+    canon_streamed(c, rv, x, y, blocks, acc, nullptr);
+    return;
+  }
+  uint64_t *status = status_words(c, ST_CANON, tiles * kStride);
+  const uint32_t epoch = next_epoch(c);
+  double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
+  copy_d2d(c, acc_in, acc, 16 * 8);
+  k_canon_fast<RV, false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, acc_in, acc,
+                                                               status, epoch, c.scratch.counters,
+                                                               uint32_t(tiles), nullptr, nullptr);
+  check_hip(hipGetLastError(), "canon fast");
+}
+
+template <class RV>
+void canon_streamed(Ctx &c, RV rv, const double *x, const double *y, uint64_t blocks, double *acc,
+                    double *tstart) {
+  const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
+  {
     const uint64_t groups = (tiles + kGroupTiles - 1) / kGroupTiles;
     const uint64_t pairs = tiles * 16;
     const uint32_t rcap = static_cast<uint32_t>(pairs < 8192 + pairs / 4 ? pairs : 8192 + pairs / 4);
@@ -1294,8 +1366,14 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
     k_canon_wide<RV><<<wide_grid, kWave, 0, c.stream>>>(rv, x, y, blocks, pre, keys, counts, rcap, heads, ents);
     k_canon_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(sums, tiles, groups,
                                                                                               gsum);
-    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, heads, ents, gsum, groups, acc);
+    double *gstart = tstart ? static_cast<double *>(tmp_alloc(c, groups * 16 * 8)) : nullptr;
+    k_canon_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, heads, ents, gsum, groups, acc,
+                                                 gstart);
+    if (tstart)
+      k_canon_expand<RV><<<uint32_t(groups * 16), kWave, 0, c.stream>>>(rv, x, y, blocks, sums, tiles, heads, ents,
+                                                                       groups, gstart, tstart);
     check_hip(hipGetLastError(), "canon streamed");
+    if (gstart) tmp_free(c, gstart);
     tmp_free(c, asum);
     tmp_free(c, pre);
     tmp_free(c, sums);
@@ -1304,16 +1382,7 @@ void canon_blocks_as(Ctx &c, RV rv, bool mul, const double *x, const double *y, 
     tmp_free(c, heads);
     tmp_free(c, ents);
     tmp_free(c, list);
-    return;
   }
-  uint64_t *status = status_words(c, ST_CANON, tiles * kStride);
-  const uint32_t epoch = next_epoch(c);
-  double *acc_in = reinterpret_cast<double *>(c.scratch.slots + 32);  // 16 words: the start state
-  copy_d2d(c, acc_in, acc, 16 * 8);
-  k_canon_fast<RV, false><<<uint32_t(tiles), kFastThreads, 0, c.stream>>>(rv, x, y, blocks, acc_in, acc,
-                                                               status, epoch, c.scratch.counters,
-                                                               uint32_t(tiles), nullptr, nullptr);
-  check_hip(hipGetLastError(), "canon fast");
 }
 
 }  // namespace
@@ -1372,6 +1441,13 @@ void canon_batch(Ctx &c, const CanonSpec &s, std::vector<BatchItem> &items, doub
   batch_plan_free(c, q);
   tmp_free(c, acc);
   tmp_free(c, zero);
+}
+
+// The exact start of every 1024-block tile of chain j (tstart[tile * 16 + j]) for the scan's
+// fold of x's blocks from acc, which ends holding the accumulators after the last block.
+void canon_tile_starts(Ctx &c, const double *x, uint64_t blocks, double *acc, double *tstart) {
+  if (blocks == 0) return;
+  canon_streamed(c, Row<0>{0.0, 0.0, 0}, x, x, blocks, acc, tstart);
 }
 
 uint64_t canon_tile_blocks() { return kTileBlocks; }

@@ -355,10 +355,105 @@ void launch(Ctx &c, bool by, const void *x, const uint8_t *starts, uint64_t n, v
 
 // ---------------------------------------------------------------- f64 scan add
 // Build-defined order (the reference has no f64 scan add, builtins.dd:50-54; tier 2 design
-// :172 candidate): out[i] is the canonical reduction of x[0..=i]. Pass 1 (one wave) folds
-// the 16 accumulators block by block and records them before every block; pass 2 finishes
-// every row of a block in parallel: pairwise combine of the recorded accumulators, then the
-// block's rows up to the row in index order.
+// :172 candidate): out[i] is the canonical reduction of x[0..=i]: the 16 accumulators after
+// the whole blocks of x[0..=i], combined pairwise (+8, +4, +2, +1), then the rows of the
+// partial block in index order. Every row needs the exact accumulators before its block.
+//
+// From 1024 whole blocks on: the exact canonical fold (k_canon.hip) gives the exact
+// accumulators at the start of every 1024-block tile, then k_fscan_tiles runs each tile's 16
+// chains through its blocks and finishes every row. Below that, one wave folds the
+// accumulators block by block (k_fscan_acc) and k_fscan_rows finishes the rows.
+constexpr int kFscanChunk = 64;  // blocks per step of k_fscan_tiles
+
+// One wave per tile of 1024 blocks (the last may be shorter), 64 blocks at a time: the rows
+// are staged in LDS; lanes 0..15 fold chain j from the tile's exact start, recording the
+// accumulators before every block; then lane b finishes block b's 16 rows (the rows before the
+// block's end from the pairwise combine of the accumulators before it, the last row from
+// those after it), staged in LDS and written back coalesced.
+__global__ __launch_bounds__(64) void k_fscan_tiles(const double *__restrict__ x, uint64_t full,
+                                                    uint64_t tile_blocks, const double *tstart,
+                                                    double *__restrict__ out) {
+  __shared__ double s_x[kFscanChunk * 16];
+  __shared__ double s_acc[kFscanChunk + 1][16];
+  __shared__ double s_out[kFscanChunk * 16];
+  const uint64_t tile = blockIdx.x;
+  const int l = lane_id();
+  const uint64_t b0 = tile * tile_blocks;
+  const uint64_t nbt = full - b0 < tile_blocks ? full - b0 : tile_blocks;
+  double a = l < 16 ? tstart[tile * 16 + l] : 0.0;
+  for (uint64_t c0 = 0; c0 < nbt; c0 += kFscanChunk) {
+    const int cb = static_cast<int>(nbt - c0 < uint64_t(kFscanChunk) ? nbt - c0 : kFscanChunk);
+    const uint64_t r0 = (b0 + c0) * 16;
+    const int rows = cb * 16;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int k = l + 64 * m;
+      s_x[k] = k < rows ? x[r0 + k] : 0.0;
+    }
+    __syncthreads();
+    if (l < 16) {
+      if (cb == kFscanChunk) {
+#pragma unroll
+        for (int b = 0; b < kFscanChunk; ++b) {
+          s_acc[b][l] = a;
+          a = a + s_x[b * 16 + l];
+        }
+      } else {
+        for (int b = 0; b < cb; ++b) {
+          s_acc[b][l] = a;
+          a = a + s_x[b * 16 + l];
+        }
+      }
+      s_acc[cb][l] = a;
+    }
+    __syncthreads();
+    if (l < cb) {
+      double t[16], u[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        t[j] = s_acc[l][j];
+        u[j] = s_acc[l + 1][j];
+      }
+#pragma unroll
+      for (int step = 8; step > 0; step /= 2)
+#pragma unroll
+        for (int j = 0; j < step; ++j) {
+          t[j] = t[j] + t[j + step];
+          u[j] = u[j] + u[j + step];
+        }
+      double v = t[0];
+#pragma unroll
+      for (int k = 0; k < 15; ++k) {
+        v = v + s_x[l * 16 + k];
+        s_out[l * 16 + k] = v;
+      }
+      s_out[l * 16 + 15] = u[0];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int k = l + 64 * m;
+      if (k < rows) out[r0 + k] = s_out[k];
+    }
+    __syncthreads();
+  }
+}
+
+// The rows of a trailing partial block: the pairwise combine of the accumulators after every
+// whole block, then the rows in index order.
+__global__ void k_fscan_tail(const double *__restrict__ x, uint64_t r0, uint64_t n, const double *acc,
+                             double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double t[16];
+  for (int j = 0; j < 16; ++j) t[j] = acc[j];
+  for (int step = 8; step > 0; step /= 2)
+    for (int j = 0; j < step; ++j) t[j] = t[j] + t[j + step];
+  double v = t[0];
+  for (uint64_t r = r0; r < n; ++r) {
+    v = v + x[r];
+    out[r] = v;
+  }
+}
 __global__ __launch_bounds__(64) void k_fscan_acc(const double *__restrict__ x, uint64_t blocks,
                                                   double *acc_state, double *acc_before) {
   const int l = lane_id();
@@ -480,6 +575,17 @@ void scan_rows(Ctx &c, const ScanSpec &s, const void *x, const uint8_t *starts, 
 void scan_add_f64(Ctx &c, const double *x, uint64_t n, double *out, double *state) {
   if (n == 0) return;
   const uint64_t full = n / 16, nb = (n + 15) / 16;
+  const uint64_t tb = canon_tile_blocks();
+  if (full >= tb) {
+    const uint64_t tiles = (full + tb - 1) / tb;
+    double *tstart = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
+    canon_tile_starts(c, x, full, state, tstart);
+    k_fscan_tiles<<<uint32_t(tiles), 64, 0, c.stream>>>(x, full, tb, tstart, out);
+    if (nb > full) k_fscan_tail<<<1, 64, 0, c.stream>>>(x, full * 16, n, state, out);
+    check_hip(hipGetLastError(), "f64 scan (tiles)");
+    tmp_free(c, tstart);
+    return;
+  }
   double *before = static_cast<double *>(tmp_alloc(c, (nb + 1) * 16 * 8));
   k_fscan_acc<<<1, 64, 0, c.stream>>>(x, full, state, before);
   // rows of a trailing partial block use the accumulators after all full blocks

@@ -299,6 +299,7 @@ struct ScanSpec {
 void scan_rows(Ctx &c, const ScanSpec &s, const void *x, const uint8_t *starts, uint64_t n,
                void *out, uint64_t *carry);
 void scan_add_f64(Ctx &c, const double *x, uint64_t n, double *out, double *state);
+void canon_tile_starts(Ctx &c, const double *x, uint64_t blocks, double *acc, double *tstart);
 // prev: row i is row i - 1, row 0 the carried last row or `fill_bits`.
 void prev_rows(Ctx &c, uint32_t width, const void *x, uint64_t n, void *out, uint64_t *carry,
                uint64_t fill_bits);
