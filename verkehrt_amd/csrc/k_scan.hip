@@ -366,54 +366,70 @@ void launch(Ctx &c, bool by, const void *x, const uint8_t *starts, uint64_t n, v
 constexpr int kFscanChunk = 64;  // blocks per step of k_fscan_tiles
 
 // One wave per tile of 1024 blocks (the last may be shorter), 64 blocks at a time: the rows
-// are staged in LDS; lanes 0..15 fold chain j from the tile's exact start, recording the
-// accumulators before every block; then lane b finishes block b's 16 rows (the rows before the
-// block's end from the pairwise combine of the accumulators before it, the last row from
-// those after it), staged in LDS and written back coalesced.
+// are staged in LDS (the next 64 blocks' rows are loaded into registers meanwhile); lanes
+// 0..15 fold chain j from the tile's exact start, recording the accumulators before every
+// block; then lane b finishes block b's 16 rows in place (the rows before the block's end from
+// the pairwise combine of the accumulators before it, the last row from those after it), and
+// the rows are written back coalesced. Rows of 16 doubles are padded to 17 in LDS, so that a
+// lane reading its own block (lanes 128 bytes apart) does not hit one bank with every lane.
 __global__ __launch_bounds__(64) void k_fscan_tiles(const double *__restrict__ x, uint64_t full,
                                                     uint64_t tile_blocks, const double *tstart,
                                                     double *__restrict__ out) {
-  __shared__ double s_x[kFscanChunk * 16];
-  __shared__ double s_acc[kFscanChunk + 1][16];
-  __shared__ double s_out[kFscanChunk * 16];
+  __shared__ double s_x[kFscanChunk * 17];
+  __shared__ double s_acc[kFscanChunk + 1][17];
+  auto px = [](int i) { return i + (i >> 4); };  // row i of the chunk in the padded layout
   const uint64_t tile = blockIdx.x;
   const int l = lane_id();
   const uint64_t b0 = tile * tile_blocks;
   const uint64_t nbt = full - b0 < tile_blocks ? full - b0 : tile_blocks;
   double a = l < 16 ? tstart[tile * 16 + l] : 0.0;
+  double nx[16];
+  auto load = [&](uint64_t c0) {
+    const uint64_t r0 = (b0 + c0) * 16;
+    const int rows = c0 < nbt ? static_cast<int>((nbt - c0 < uint64_t(kFscanChunk) ? nbt - c0 : kFscanChunk) * 16) : 0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int k = l + 64 * m;
+      nx[m] = k < rows ? x[r0 + k] : 0.0;
+    }
+  };
+  load(0);
   for (uint64_t c0 = 0; c0 < nbt; c0 += kFscanChunk) {
     const int cb = static_cast<int>(nbt - c0 < uint64_t(kFscanChunk) ? nbt - c0 : kFscanChunk);
     const uint64_t r0 = (b0 + c0) * 16;
     const int rows = cb * 16;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int k = l + 64 * m;
-      s_x[k] = k < rows ? x[r0 + k] : 0.0;
-    }
+    for (int m = 0; m < 16; ++m) s_x[px(l + 64 * m)] = nx[m];
     __syncthreads();
+    load(c0 + kFscanChunk);
     if (l < 16) {
       if (cb == kFscanChunk) {
+        double v[kFscanChunk];
+#pragma unroll
+        for (int b = 0; b < kFscanChunk; ++b) v[b] = s_x[b * 17 + l];
 #pragma unroll
         for (int b = 0; b < kFscanChunk; ++b) {
           s_acc[b][l] = a;
-          a = a + s_x[b * 16 + l];
+          a = a + v[b];
         }
       } else {
         for (int b = 0; b < cb; ++b) {
           s_acc[b][l] = a;
-          a = a + s_x[b * 16 + l];
+          a = a + s_x[b * 17 + l];
         }
       }
       s_acc[cb][l] = a;
     }
     __syncthreads();
     if (l < cb) {
-      double t[16], u[16];
+      double t[16], u[16], r[15];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         t[j] = s_acc[l][j];
         u[j] = s_acc[l + 1][j];
       }
+#pragma unroll
+      for (int k = 0; k < 15; ++k) r[k] = s_x[l * 17 + k];
 #pragma unroll
       for (int step = 8; step > 0; step /= 2)
 #pragma unroll
@@ -424,16 +440,16 @@ __global__ __launch_bounds__(64) void k_fscan_tiles(const double *__restrict__ x
       double v = t[0];
 #pragma unroll
       for (int k = 0; k < 15; ++k) {
-        v = v + s_x[l * 16 + k];
-        s_out[l * 16 + k] = v;
+        v = v + r[k];
+        s_x[l * 17 + k] = v;
       }
-      s_out[l * 16 + 15] = u[0];
+      s_x[l * 17 + 15] = u[0];
     }
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       const int k = l + 64 * m;
-      if (k < rows) out[r0 + k] = s_out[k];
+      if (k < rows) out[r0 + k] = s_x[px(k)];
     }
     __syncthreads();
   }
