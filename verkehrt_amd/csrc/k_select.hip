@@ -203,6 +203,181 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   }
 }
 
+// Persistent form for long inputs: one 1024-thread block per compute unit takes 16384-row tiles
+// in ticket order and loads the next tile into registers before it resolves the current one,
+// so the loads of one tile are in flight while the block waits on the look-back and writes the
+// survivors of the other: HBM stays busy through every look-back. A tile's count is published
+// as soon as its rows are in registers, and the ticket after the next tile is taken while the
+// current one resolves. Ticket: a block takes no more after one finds no tile, so a launch
+// takes ntiles + gridDim.x tickets and the last one returns the ticket to zero. A block only
+// waits on tiles below its current one, all held by running blocks, so no wait is circular.
+template <typename T, int KIND>
+__global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_select_p(
+    const T *__restrict__ x, const uint8_t *__restrict__ mask, uint64_t n, uint32_t keepset, T c, uint64_t base,
+    void *out_v, uint64_t *status, uint32_t epoch, uint32_t *ticket, uint32_t ntiles, uint64_t *d_count) {
+  using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
+  struct alignas(16) Q {
+    uint32_t a[4];
+  };
+  Out *out = static_cast<Out *>(out_v);
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  const uint32_t takes = ntiles + gridDim.x;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_cnt[kItems][kWaves];
+  __shared__ uint32_t s_excl[kItems][kWaves];
+  __shared__ uint32_t s_total;
+  __shared__ uint64_t s_prefix;
+  __shared__ uint8_t s_mask[KIND == 1 ? 1 : kWaves][KIND == 1 ? 1 : kItems][kWave];
+
+  auto take = [&]() -> uint32_t {
+    __syncthreads();
+    if (t == 0) {
+      const uint32_t k = atomicAdd(ticket, 1u);
+      if (k == takes - 1) atomicExch(ticket, 0u);
+      s_tile = k;
+    }
+    __syncthreads();
+    return s_tile;
+  };
+  auto rows_of = [&](uint32_t tile) -> uint32_t {
+    const uint64_t rem = n - uint64_t(tile) * kTile;
+    return rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);
+  };
+  // the tile's rows into registers (rows past the end load the last row), and the mask bytes
+  // of an aligned full tile (16 per lane); other tiles' mask bytes are loaded when counted
+  auto load = [&](uint32_t tile, Out (&val)[kItems], Q &mq, bool &wide) {
+    const uint64_t t0 = uint64_t(tile) * kTile;
+    const uint32_t lim = rows_of(tile);
+    const T *xt = KIND == 2 ? nullptr : x + t0;
+    wide = false;
+    if constexpr (KIND != 1) {
+      const uint8_t *mt = mask + t0;
+      wide = lim == uint32_t(kTile) && (reinterpret_cast<uintptr_t>(mt) & 15) == 0;
+      if (wide && (l >> 2) < kItems)
+        mq = *reinterpret_cast<const Q *>(mt + uint32_t(l >> 2) * kSelBlock + uint32_t(w) * kWave + (l & 3) * 16);
+    }
+    if constexpr (KIND != 2) {
+#pragma unroll
+      for (int i = 0; i < kItems; ++i) {
+        const uint32_t lr = uint32_t(i) * kSelBlock + t;
+        val[i] = static_cast<Out>(xt[lr < lim ? lr : lim - 1]);
+      }
+    }
+  };
+  // ballots, the block's exclusive counts (s_excl, s_total) and the published aggregate
+  auto count = [&](uint32_t tile, const Out (&val)[kItems], const Q &mq, bool wide, uint64_t (&keepm)[kItems]) {
+    const uint32_t lim = rows_of(tile);
+    uint8_t mv[kItems];
+    if constexpr (KIND != 1) {
+      if (wide) {
+        if ((l >> 2) < kItems) *reinterpret_cast<Q *>(&s_mask[w][l >> 2][(l & 3) * 16]) = mq;
+        // the wave reads back what it wrote (LDS operations of one wave complete in order)
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) mv[i] = s_mask[w][i][l];
+      } else {
+        const uint8_t *mt = mask + uint64_t(tile) * kTile;
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+          const uint32_t lr = uint32_t(i) * kSelBlock + t;
+          mv[i] = mt[lr < lim ? lr : lim - 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const uint32_t lr = uint32_t(i) * kSelBlock + t;
+      const bool keep = KIND == 1 ? (keepset & outcome(static_cast<T>(val[i]), c)) != 0 : mv[i] != 0;
+      keepm[i] = __ballot(keep && lr < lim);
+      if (l == 0) s_cnt[i][w] = __popcll(keepm[i]);
+    }
+    __syncthreads();
+    constexpr int PER = kItems * kWaves / kWave;
+    if (w == 0) {
+      uint32_t c4[PER], v = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int k = PER * l + q;
+        c4[q] = s_cnt[k / kWaves][k % kWaves];
+        v += c4[q];
+      }
+      uint32_t inc = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d);
+        if (l >= d) inc += o;
+      }
+      uint32_t e = inc - v;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int k = PER * l + q;
+        s_excl[k / kWaves][k % kWaves] = e;
+        e += c4[q];
+      }
+      const uint32_t total = __shfl(inc, 63);
+      if (l == 0) {
+        s_total = total;
+        st_rlx(&status[tile], pack(epoch, tile == 0 ? 2 : 1, total));
+      }
+    }
+  };
+  // the look-back (wave 0), the inclusive count, and the survivors written
+  auto finish = [&](uint32_t tile, const Out (&val)[kItems], const uint64_t (&keepm)[kItems]) {
+    if (w == 0) {
+      const uint32_t total = s_total;
+      uint64_t prefix = 0;
+      if (tile > 0) {
+        prefix = look_back(status, epoch, tile, 0);
+        if (l == 0) st_rlx(&status[tile], pack(epoch, 2, prefix + total));
+      }
+      if (l == 0) {
+        s_prefix = prefix;
+        if (tile == ntiles - 1) *d_count = prefix + total;
+      }
+    }
+    __syncthreads();
+    Out *ot = out + s_prefix;
+    const uint64_t t0 = uint64_t(tile) * kTile;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if ((keepm[i] >> l) & 1) {
+        const uint32_t at = s_excl[i][w] + rank_below(keepm[i]);
+        if (KIND == 2) ot[at] = static_cast<Out>(base + t0 + uint32_t(i) * kSelBlock + t);
+        else ot[at] = val[i];
+      }
+    }
+  };
+
+  uint32_t tile = take();
+  if (tile >= ntiles) return;
+  uint32_t next = take();
+  Out va[kItems], vb[kItems];
+  uint64_t ka[kItems];
+  Q qa{}, qb{};
+  bool wa = false, wb = false;
+  load(tile, va, qa, wa);
+  count(tile, va, qa, wa, ka);
+  while (true) {
+    if (next < ntiles) load(next, vb, qb, wb);
+    // the ticket after `next`, taken while this tile resolves (no take once one finds no tile)
+    uint32_t after = 0;
+    if (t == 0 && next < ntiles) {
+      after = atomicAdd(ticket, 1u);
+      if (after == takes - 1) atomicExch(ticket, 0u);
+    }
+    finish(tile, va, ka);
+    if (next >= ntiles) break;
+    if (t == 0) s_tile = after;
+    __syncthreads();  // the survivors of `tile` are written before s_excl is reused; s_tile is set
+    tile = next;
+    next = s_tile;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) va[i] = vb[i];
+    qa = qb;
+    wa = wb;
+    count(tile, va, qa, wa, ka);
+  }
+}
+
 // Look-back scratch of one selection launch: status words (one per tile), their epoch, and a
 // tile ticket that is zero before the launch (the last tile returns it to zero).
 struct SelScratch {
@@ -226,6 +401,17 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
   }
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
+  if constexpr (KIND == 1) {
+    // long filters: one persistent block per compute unit (compress and where, which also hold
+    // mask bytes, keep one tile per block: their persistent form spilled registers)
+    if (tiles >= 2 * uint64_t(c.cus)) {
+      k_select_p<T, KIND><<<static_cast<uint32_t>(c.cus), kSelBlock, 0, c.stream>>>(
+          static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch, sc.ticket,
+          static_cast<uint32_t>(tiles), d_count);
+      check_hip(hipGetLastError(), "select launch");
+      return;
+    }
+  }
   k_select<T, KIND, false><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
       static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch,
       sc.ticket, static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);

@@ -284,6 +284,9 @@ Ctx *create_ctx(int device) {
   check_hip(hipSetDevice(device), "hipSetDevice");
   auto *c = new Ctx();
   c->device = device;
+  c->cus = 0;
+  check_hip(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+  if (c->cus < 1) c->cus = 1;
   check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream create");
   c->pool = new Pool(device);
   c->epoch = 0;

@@ -107,6 +107,7 @@ struct Scratch {
 
 struct vk_ctx {
   int device;
+  int cus;  // compute units of the device (the grid of persistent kernels)
   hipStream_t stream;
   vk::Pool *pool;
   vk::Scratch scratch;
