@@ -1,0 +1,126 @@
+"""argsort by top-digit buckets (two HBM passes over the two top non-constant digits, then one
+LDS sort per 16-bit bucket) against the oracle's LSD radix restatement (radix.rs:187-259).
+
+The bucket path runs for positions-only sorts of at least 2^22 rows with four or more
+non-constant digits whose two top digits would give buckets of at most 0.7 x 3072 (or 0.6 x
+8192) rows if they were independent. The cases below reach both local capacities, buckets that still
+overflow them (correlated top digits: the batched passes over those row ranges), many equal
+keys inside a bucket (stability), constant middle digits, i32 keys (4 digits) and f64 bit
+patterns, at row counts that are not multiples of any tile.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    from verkehrt_amd import Registry
+    return Registry().resolve(0)
+
+
+def _check(D, keys, t):
+    got = D.call(f"argsort_{t}", [D.array(keys, t)]).to_numpy()
+    want = O.argsort(keys)
+    assert np.array_equal(got, want), (t, keys.size)
+
+
+def _rng_u64(rng, n):
+    return rng.integers(0, 2**64, n, dtype=np.uint64)
+
+
+@pytest.mark.parametrize("n", [1 << 22, (1 << 23) + 12345])
+def test_uniform_u64(D, n):
+    _check(D, _rng_u64(np.random.default_rng(n), n), "u64")
+
+
+def test_uniform_i64_and_f64_bits(D):
+    rng = np.random.default_rng(7)
+    n = (1 << 22) + 777
+    _check(D, _rng_u64(rng, n).view(np.int64), "i64")
+    # every bit pattern, NaNs and infinities included, in the image's total order
+    _check(D, _rng_u64(rng, n).view(np.float64), "f64")
+
+
+def test_uniform_i32(D):
+    rng = np.random.default_rng(8)
+    n = (1 << 22) + 3
+    _check(D, rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32), "i32")
+
+
+def test_equal_keys_inside_buckets(D):
+    # uniform top 16 bits, lower bits from only 4 values: long runs of equal keys per bucket
+    rng = np.random.default_rng(9)
+    n = (1 << 22) + 99
+    top = rng.integers(0, 1 << 16, n, dtype=np.uint64) << np.uint64(48)
+    low = rng.integers(0, 4, n, dtype=np.uint64) * np.uint64(0x0101010101)
+    _check(D, top | low, "u64")
+
+
+def test_constant_middle_digits(D):
+    # digits 2..5 constant: active digits 0, 1, 6, 7 (two local digits, both in the low bytes)
+    rng = np.random.default_rng(10)
+    n = 1 << 22
+    k = (rng.integers(0, 1 << 16, n, dtype=np.uint64) << np.uint64(48)) | np.uint64(0x0000ABCDEF120000)
+    k |= rng.integers(0, 1 << 16, n, dtype=np.uint64)
+    _check(D, k, "u64")
+
+
+def test_large_capacity(D):
+    # 32 x 32 top values: about 4096 rows per bucket, the 8192-row local sort
+    rng = np.random.default_rng(11)
+    n = (1 << 22) + 5
+    a = rng.integers(0, 32, n, dtype=np.uint64)
+    b = rng.integers(0, 32, n, dtype=np.uint64)
+    k = (a << np.uint64(56)) | (b << np.uint64(48)) | rng.integers(0, 1 << 48, n, dtype=np.uint64)
+    _check(D, k, "u64")
+
+
+def test_overflowing_buckets(D):
+    # the two top digits equal: each looks uniform alone, but every bucket (j, j) holds n/256
+    # rows, above either capacity, so all of them take the batched passes
+    rng = np.random.default_rng(12)
+    n = (1 << 22) + 17
+    j = rng.integers(0, 256, n, dtype=np.uint64)
+    k = (j << np.uint64(56)) | (j << np.uint64(48)) | rng.integers(0, 1 << 48, n, dtype=np.uint64)
+    _check(D, k, "u64")
+
+
+def test_some_buckets_overflow(D):
+    # a quarter of the rows in the correlated buckets, the rest uniform: both paths in one call
+    rng = np.random.default_rng(13)
+    n = 1 << 23
+    k = _rng_u64(rng, n)
+    hot = rng.random(n) < 0.25
+    j = rng.integers(0, 64, hot.sum(), dtype=np.uint64)
+    k[hot] = (j << np.uint64(56)) | (j << np.uint64(48)) | (k[hot] & np.uint64((1 << 48) - 1))
+    _check(D, k, "u64")
+
+
+def test_i32_overflow_and_three_digits(D):
+    # i32 keys with correlated top digits: every bucket overflows and the batched passes sort
+    # the two lower digits; with a constant low byte only three digits are active, so the
+    # ordinary passes run
+    rng = np.random.default_rng(14)
+    n = (1 << 22) + 1
+    j = rng.integers(0, 128, n, dtype=np.int64)
+    k = ((j << 24) | (j << 16) | rng.integers(0, 1 << 16, n, dtype=np.int64)).astype(np.uint32).view(np.int32)
+    _check(D, k, "i32")
+    k3 = (rng.integers(-2**31, 2**31, n, dtype=np.int64) & ~0xFF).astype(np.int32)
+    _check(D, k3, "i32")
+
+
+def test_large_sub_buckets(D):
+    # about 1024 rows per bucket whose next digit takes two values: sub-buckets of about 512
+    # rows, too large to count, so those blocks sort by every local digit in LDS
+    rng = np.random.default_rng(15)
+    n = (1 << 22) + 31
+    hi = rng.integers(0, 256, n, dtype=np.uint64)
+    lo = rng.integers(0, 16, n, dtype=np.uint64)
+    mid = rng.integers(0, 2, n, dtype=np.uint64)
+    k = (hi << np.uint64(56)) | (lo << np.uint64(48)) | (mid << np.uint64(40))
+    k |= rng.integers(0, 1 << 40, n, dtype=np.uint64)
+    _check(D, k, "u64")

@@ -393,6 +393,217 @@ __global__ void k_copy_images(int32_t type, const void *keys, uint64_t n, U *out
     out[i] = image<U>(type, keys, i);
 }
 
+// ------------------------------------------------- top-digit buckets, local sorts
+// Roughly uniform keys need not go through every digit in HBM. Two passes by the two most
+// significant non-constant digits order the (image, position) pairs by a 16-bit bucket; each
+// bucket, a few thousand rows, is then sorted by its remaining digits inside one block's LDS
+// and its positions written once. The result is the same stable permutation.
+
+constexpr uint32_t kBuckets = 65536;
+
+template <typename U>
+__device__ __forceinline__ uint32_t bucket16(U k, int sh_hi, int sh_lo) {
+  return ((static_cast<uint32_t>(k >> sh_hi) & 255) << 8) | (static_cast<uint32_t>(k >> sh_lo) & 255);
+}
+
+// start[b] = the first row of bucket b in keys sorted by bucket (start[] holds n beforehand,
+// so buckets after the last row keep n): the row where the bucket changes writes the start of
+// every bucket from the one after its predecessor's up to its own.
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_bucket_starts(const U *__restrict__ k, uint64_t n, int sh_hi,
+                                                          int sh_lo, uint32_t *__restrict__ start) {
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+    const int64_t id = bucket16<U>(k[i], sh_hi, sh_lo);
+    const int64_t prev = i ? int64_t(bucket16<U>(k[i - 1], sh_hi, sh_lo)) : -1;
+    for (int64_t b = prev + 1; b <= id; ++b) start[b] = static_cast<uint32_t>(i);
+  }
+}
+
+struct LocalDigits {
+  int n;
+  int shift[8];
+};
+
+// One block per bucket: rows [start[b], start[b + 1]) of (keys, pos) sorted by bucket and, in
+// each bucket, by row. A bucket of at most THREADS * ITEMS rows is loaded into registers and
+// ordered by its local digits `dg` (ascending) in LDS:
+// - one stable counting pass by the most significant local digit (wave64 match ranks,
+//   per-wave digit counters, rows written to LDS in digit order) splits it into sub-buckets,
+//   each in row order; when none holds more than kSubMax rows, every row's place in its
+//   sub-bucket is the number of rows there with a smaller key, or an equal key and an
+//   earlier place, and its position goes straight to out;
+// - otherwise the block runs a stable pass per local digit, least significant first, from the
+//   rows as loaded.
+// A larger bucket is listed in overflow[1..] (count in overflow[0]) for the batched passes.
+constexpr uint32_t kSubMax = 96;
+
+template <typename U, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ keys, const uint32_t *__restrict__ pos,
+                                                        const uint32_t *__restrict__ start, const LocalDigits dg,
+                                                        int64_t *__restrict__ out, uint32_t *__restrict__ overflow) {
+  constexpr int C = THREADS * ITEMS;
+  constexpr int W = THREADS / kWave;
+  static_assert(THREADS >= kRadix, "one thread per digit in the digit scan");
+  __shared__ U s_key[C];
+  __shared__ uint32_t s_pos[C];
+  __shared__ uint32_t s_wcnt[W][kRadix];
+  __shared__ uint32_t s_excl[kRadix + 1];
+  __shared__ uint32_t s_wsum[kRadix / kWave];
+  __shared__ uint32_t s_max;
+  const uint32_t b = blockIdx.x;
+  const uint32_t s = start[b];
+  const uint32_t m = start[b + 1] - s;
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  if (m == 0) return;
+  if (m > static_cast<uint32_t>(C)) {
+    if (t == 0) overflow[1 + atomicAdd(&overflow[0], 1u)] = b;
+    return;
+  }
+  if (dg.n == 0) {  // the bucket's rows are in order already
+    for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(pos[s + j]);
+    return;
+  }
+  // rows are dealt to the waves in equal contiguous runs of ipw * 64
+  const uint32_t ipw = (m + W * kWave - 1) / (W * kWave);
+  const uint32_t wbase = uint32_t(w) * ipw * kWave;
+  const uint64_t below = (uint64_t(1) << l) - 1;
+  U key[ITEMS];
+  uint32_t p[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t r = wbase + uint32_t(i) * kWave + l;
+    const bool ok = uint32_t(i) < ipw && r < m;
+    key[i] = ok ? keys[s + r] : U(0);
+    p[i] = ok ? pos[s + r] : 0u;
+  }
+  // One stable pass by the digit at `shift` over the rows in registers: they land in LDS in
+  // digit order, s_excl holds the digit starts (s_excl[256] = m) and s_max the largest count.
+  auto lds_pass = [&](int shift) {
+    uint32_t rnk[ITEMS];
+    for (int q = t; q < W * kRadix; q += THREADS) (&s_wcnt[0][0])[q] = 0;
+    if (t == 0) s_max = 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      rnk[i] = 0;
+      if (uint32_t(i) < ipw) {  // uniform across the block
+        const uint32_t r = wbase + uint32_t(i) * kWave + l;
+        const bool valid = r < m;
+        const uint32_t d = static_cast<uint32_t>(key[i] >> shift) & 255;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+          const uint64_t bal = __ballot((d >> bit) & 1);
+          peers &= ((d >> bit) & 1) ? bal : ~bal;
+        }
+        if (valid) {
+          rnk[i] = s_wcnt[w][d] + __popcll(peers & below);
+          if (l == __builtin_ctzll(peers)) s_wcnt[w][d] += __popcll(peers);
+        }
+      }
+    }
+    __syncthreads();
+    uint32_t cnt = 0, inc = 0;
+    if (t < kRadix) {
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        const uint32_t v = s_wcnt[q][t];
+        s_wcnt[q][t] = cnt;
+        cnt += v;
+      }
+      inc = cnt;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o);
+        if (l >= o) inc += u;
+      }
+      if (l == kWave - 1) s_wsum[w] = inc;
+      uint32_t mx = cnt;
+#pragma unroll
+      for (int o = kWave / 2; o >= 1; o >>= 1) {
+        const uint32_t u = __shfl_xor(mx, o);
+        mx = u > mx ? u : mx;
+      }
+      if (l == 0) atomicMax(&s_max, mx);
+    }
+    __syncthreads();
+    if (t < kRadix) {
+      uint32_t wpre = 0;
+      for (int q = 0; q < w; ++q) wpre += s_wsum[q];
+      s_excl[t] = wpre + inc - cnt;
+      if (t == 0) s_excl[kRadix] = m;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t r = wbase + uint32_t(i) * kWave + l;
+      if (uint32_t(i) < ipw && r < m) {
+        const uint32_t d = static_cast<uint32_t>(key[i] >> shift) & 255;
+        const uint32_t at = s_excl[d] + s_wcnt[w][d] + rnk[i];
+        s_key[at] = key[i];
+        s_pos[at] = p[i];
+      }
+    }
+    __syncthreads();
+  };
+  const int top = dg.shift[dg.n - 1];
+  lds_pass(top);
+  const bool counted = s_max <= kSubMax;
+  __syncthreads();  // every wave has read s_max before a further pass clears it
+  if (counted) {
+    // each row's place, kept in registers until every row has been read, then the positions
+    // placed in LDS and written out in order
+    uint32_t dst[ITEMS], pv[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t j = uint32_t(t) + uint32_t(i) * THREADS;
+      dst[i] = 0;
+      pv[i] = 0;
+      if (j < m) {
+        const U kj = s_key[j];
+        const uint32_t d = static_cast<uint32_t>(kj >> top) & 255;
+        const uint32_t lo = s_excl[d], hi = s_excl[d + 1];
+        // four reads in flight per step (the sub-bucket holds j, so hi - 1 is in it)
+        uint32_t r = 0;
+        for (uint32_t q = lo; q < hi; q += 4) {
+          U kq[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) kq[u] = s_key[q + u < hi ? q + u : hi - 1];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t qq = q + u;
+            r += (qq < hi && (kq[u] < kj || (kq[u] == kj && qq < j))) ? 1u : 0u;
+          }
+        }
+        dst[i] = lo + r;
+        pv[i] = s_pos[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (uint32_t(t) + uint32_t(i) * THREADS < m) s_pos[dst[i]] = pv[i];
+    __syncthreads();
+    for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(s_pos[j]);
+    return;
+  }
+  // a sub-bucket too large to count: every local digit, from the rows as loaded
+  for (int k = 0; k < dg.n; ++k) {
+    lds_pass(dg.shift[k]);
+    if (k + 1 < dg.n) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t r = wbase + uint32_t(i) * kWave + l;
+        if (uint32_t(i) < ipw && r < m) {
+          key[i] = s_key[r];
+          p[i] = s_pos[r];
+        }
+      }
+    }
+  }
+  for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(s_pos[j]);
+}
+
 template <typename U, typename V, bool FIRST, bool LAST>
 void run_pass(Ctx &c, int32_t type, const void *kin, const V *pin, const uint64_t *vals, uint64_t n,
               int shift, const uint32_t *dbase, U *kout, V *pout, int64_t *out_pos) {
@@ -577,6 +788,99 @@ void radix_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, U *sorted_
   batch_plan_free(c, p);
 }
 
+// The local sort's capacity in rows for a bucket-size estimate: 3072 (256 threads x 12 rows,
+// four blocks per compute unit) or 8192 (512 threads x 16 rows, one block), 0 when buckets
+// would not fit.
+inline int local_capacity(double est_max) {
+  if (est_max <= 0.7 * 3072) return 3072;
+  if (est_max <= 0.6 * 8192) return 8192;
+  return 0;
+}
+
+// Argsort by top-digit buckets (see k_local_sort). `digits` are the active digits, ascending,
+// at least four; hist holds their exclusive digit bases. The two top digits go through HBM
+// passes, the others through the local sorts; buckets larger than the local capacity are
+// sorted by those digits with the batched passes over their row ranges.
+template <typename U>
+void argsort_buckets(Ctx &c, int32_t type, const void *keys, uint64_t n, const std::vector<int> &digits,
+                     const uint32_t *hist, int cap, int64_t *out_pos) {
+  using V = uint32_t;
+  constexpr int P = sizeof(U);
+  const int np = static_cast<int>(digits.size());
+  const int dlo = digits[np - 2], dhi = digits[np - 1];
+  U *ka = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
+  V *pa = static_cast<V *>(tmp_alloc(c, n * sizeof(V)));
+  U *kb = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
+  V *pb = static_cast<V *>(tmp_alloc(c, n * sizeof(V)));
+  run_pass<U, V, true, false>(c, type, keys, nullptr, nullptr, n, 8 * dlo, hist + dlo * kRadix, ka, pa, nullptr);
+  run_pass<U, V, false, false>(c, type, ka, pa, nullptr, n, 8 * dhi, hist + dhi * kRadix, kb, pb, nullptr);
+  // bucket starts (kBuckets + 1 words) and the overflow list (count + kBuckets ids)
+  uint32_t *start = static_cast<uint32_t *>(tmp_alloc(c, (2 * kBuckets + 4) * 4));
+  uint32_t *overflow = start + kBuckets + 2;
+  fill_u64(c, reinterpret_cast<uint64_t *>(start), (kBuckets + 2) / 2, n | (n << 32));
+  fill_u64(c, reinterpret_cast<uint64_t *>(overflow), 1, 0);
+  k_bucket_starts<U><<<stream_grid(n, kBlock, 8192), kBlock, 0, c.stream>>>(kb, n, 8 * dhi, 8 * dlo, start);
+  check_hip(hipGetLastError(), "bucket starts");
+  LocalDigits dg{};
+  dg.n = np - 2;
+  for (int k = 0; k < np - 2; ++k) dg.shift[k] = 8 * digits[k];
+  if (cap == 3072) k_local_sort<U, 256, 12><<<kBuckets, 256, 0, c.stream>>>(kb, pb, start, dg, out_pos, overflow);
+  else k_local_sort<U, 512, 16><<<kBuckets, 512, 0, c.stream>>>(kb, pb, start, dg, out_pos, overflow);
+  check_hip(hipGetLastError(), "local sort");
+  uint32_t nover = 0;
+  read_back(c, overflow, 4, &nover);
+  if (nover) {
+    std::vector<uint32_t> st(kBuckets + 1), ids(nover);
+    check_hip(hipMemcpyAsync(st.data(), start, (kBuckets + 1) * 4, hipMemcpyDeviceToHost, c.stream), "starts");
+    check_hip(hipMemcpyAsync(ids.data(), overflow + 1, nover * 4, hipMemcpyDeviceToHost, c.stream), "overflow");
+    check_hip(hipStreamSynchronize(c.stream), "sync");
+    std::vector<BatchItem> items;
+    for (uint32_t id : ids) {
+      BatchItem it{};
+      it.a = kb + st[id];
+      it.out = out_pos + st[id];
+      it.n = st[id + 1] - st[id];
+      it.aux = st[id];
+      items.push_back(it);
+    }
+    // every item's rows are images already: the identity image of U
+    const int32_t img = sizeof(U) == 8 ? int32_t(VK_U64) : kRawU32;
+    BatchPlan bp = batch_plan(c, items, kTile);
+    const uint64_t hwords = items.size() * P * kRadix;
+    uint32_t *bh = static_cast<uint32_t *>(tmp_alloc(c, hwords * 4 + 64));
+    uint32_t *active = bh + hwords;
+    fill_u64(c, reinterpret_cast<uint64_t *>(bh), (hwords * 4 + 64) / 8, 0);
+    k_hist_batch<U><<<bp.ntiles, kBlock, 0, c.stream>>>(img, bp.items, bp.tile_item, bh);
+    check_hip(hipGetLastError(), "overflow histogram");
+    k_base_batch<<<stream_grid(items.size() * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(bp.items, items.size(),
+                                                                                         P, bh, active);
+    check_hip(hipGetLastError(), "overflow bases");
+    // the bucket's rows are in row order; a stable pass over each lower digit (constant ones
+    // are identity passes) finishes them, the last writing positions
+    const U *kin = kb;
+    const V *pin = pb;
+    for (int k = 0; k < np - 2; ++k) {
+      const int d = digits[k];
+      if (k == np - 3) {
+        run_pass_batch<U, V, false, true>(c, img, bp, P, d, kin, pin, bh, nullptr, nullptr);
+      } else {
+        U *ko = (k % 2 == 0) ? ka : kb;
+        V *po = (k % 2 == 0) ? pa : pb;
+        run_pass_batch<U, V, false, false>(c, img, bp, P, d, kin, pin, bh, ko, po);
+        kin = ko;
+        pin = po;
+      }
+    }
+    tmp_free(c, bh);
+    batch_plan_free(c, bp);
+  }
+  tmp_free(c, start);
+  tmp_free(c, ka);
+  tmp_free(c, pa);
+  tmp_free(c, kb);
+  tmp_free(c, pb);
+}
+
 // Sorts rows by image. Positions mode (V = u32): positions of the sorted rows to out_pos,
 // sorted images to sorted_keys when not null. Values mode (V = u64): the sorted images and
 // the value bits of `vals` that go with them to sorted_keys / sorted_vals.
@@ -636,6 +940,24 @@ void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_
     return;
   }
   const int np = static_cast<int>(active.size());
+  if constexpr (POS && sizeof(U) >= 4) {
+    // positions only, at least four digits and buckets of the two top digits that fit a local
+    // sort if the two digits were independent: bucket passes
+    if (out_pos && !sorted_keys && np >= 4 && n >= (uint64_t(1) << 22)) {
+      const int dlo = active[np - 2], dhi = active[np - 1];
+      uint32_t mlo = 0, mhi = 0;
+      for (int v = 0; v < kRadix; ++v) {
+        mlo = h[dlo * kRadix + v] > mlo ? h[dlo * kRadix + v] : mlo;
+        mhi = h[dhi * kRadix + v] > mhi ? h[dhi * kRadix + v] : mhi;
+      }
+      const int cap = local_capacity(double(mlo) * double(mhi) / double(n));
+      if (cap) {
+        argsort_buckets<U>(c, type, keys, n, active, hist, cap, out_pos);
+        tmp_free(c, hist);
+        return;
+      }
+    }
+  }
   U *ka = nullptr, *kb = nullptr;
   V *pa = nullptr, *pb = nullptr;
   if (np > 1) {
