@@ -711,69 +711,78 @@ __global__ __launch_bounds__(kBlock) void k_canon_sample(RV rv, const double *__
   if (t < 16) asum[t * tiles + tile] = 16.0 * ((s_a[0][t] + s_a[1][t]) + (s_a[2][t] + s_a[3][t]));
 }
 
-// Prefix: one block per chain: thread p sums part p of the tiles (ceil(tiles / 1024) each), the
-//    part sums are scanned, then each thread writes its tiles' approximate starts,
-//    pre[tile * 16 + j].
-constexpr int kPrefixParts = 1024;
-__global__ __launch_bounds__(kPrefixParts) void k_canon_prefix(const double *asum, uint64_t tiles,
-                                                               const double *acc_in, double *pre) {
-  __shared__ double s_part[kPrefixParts];
-  const int j = blockIdx.x, p = threadIdx.x;
-  const double *a = asum + uint64_t(j) * tiles;
-  const uint64_t per = (tiles + kPrefixParts - 1) / kPrefixParts;
-  const uint64_t t0 = p * per < tiles ? p * per : tiles;
+// Prefix: pre[tile * 16 + j] = the start state of chain j plus the approximate sums of chain j
+//    over the tiles before `tile` (any order of these sums serves: the applies prove every
+//    result). Blocks take equal tile ranges in ticket order, one wave per chain: a block sums its
+//    range per chain, publishes the 16 sums (the canonical family's status layout: aggregates at
+//    W_AAGG, flag F_AAGG in word 0), adds the published sums of every block before it, then scans
+//    its range 64 tiles at a time and writes each round of 64 x 16 starts as consecutive words
+//    through LDS. A block waits only on blocks that took earlier tickets, which publish before
+//    they wait on anyone.
+constexpr int kPrefixThreads = 1024;
+constexpr uint32_t kPrefixMaxBlocks = 64;
+__global__ __launch_bounds__(kPrefixThreads) void k_canon_prefix(const double *asum, uint64_t tiles,
+                                                                 const double *acc_in, double *pre,
+                                                                 uint64_t *status, uint32_t epoch, uint32_t *ticket,
+                                                                 uint32_t nblk) {
+  static_assert(kPrefixThreads == 16 * kWave, "one wave per chain");
+  __shared__ double s_agg[16];
+  __shared__ double s_lb[kPrefixMaxBlocks][17];
+  __shared__ double s_out[kWave * 16];
+  const uint32_t p = take_ticket(ticket, nblk);
+  const uint64_t per = (tiles + nblk - 1) / nblk;
+  const uint64_t t0 = uint64_t(p) * per < tiles ? uint64_t(p) * per : tiles;
   const uint64_t t1 = t0 + per < tiles ? t0 + per : tiles;
-  double sum = 0.0;
-  uint64_t k = t0;
-  for (; k + 8 <= t1; k += 8) {
-    double u[8];
+  const int t = threadIdx.x, j = t / kWave, l = lane_id();
+  const double *a = asum + uint64_t(j) * tiles;
+  double part = 0.0;
+  for (uint64_t r0 = t0; r0 < t1; r0 += kWave) part = part + (r0 + l < t1 ? a[r0 + l] : 0.0);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) u[m] = a[k + m];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) sum = sum + u[m];
-  }
-  for (; k < t1; ++k) sum = sum + a[k];
-  // inclusive scan of the part sums: within each wave (Hillis-Steele over lanes), then the
-  // wave totals (Hillis-Steele over the 16 waves), each part adding the totals before its wave
-  const int l = lane_id(), w = p / kWave;
-  double inc = sum;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const double o = __shfl_up(inc, d);
-    if (l >= d) inc = o + inc;
-  }
-  __shared__ double s_wave[kPrefixParts / kWave];
-  if (l == kWave - 1) s_wave[w] = inc;
+  for (int d = kWave / 2; d >= 1; d >>= 1) part = part + __shfl_xor(part, d);
+  if (l == 0) s_agg[j] = part;
   __syncthreads();
-  if (p < kWave) {
-    double t = p < kPrefixParts / kWave ? s_wave[p] : 0.0;
+  uint64_t *st = status + uint64_t(p) * kStride;
+  if (t < 16) st_d(st + W_AAGG + t, s_agg[t]);
+  if (t < kWave) {
+    publish_fence();
+    if (t == 0) st_rlx(st, flag_word(epoch, F_AAGG));
+    // lane q reads block q's sums (q < p)
+    if (static_cast<uint32_t>(t) < p) {
+      const uint64_t *sq = status + uint64_t(t) * kStride;
+      wait_flag(sq, epoch, F_AAGG);
 #pragma unroll
-    for (int d = 1; d < kPrefixParts / kWave; d <<= 1) {
-      const double o = __shfl_up(t, d);
-      if (p >= d) t = o + t;
-    }
-    if (p < kPrefixParts / kWave) s_wave[p] = t;
-  }
-  __syncthreads();
-  if (w > 0) inc = s_wave[w - 1] + inc;
-  s_part[p] = inc;
-  __syncthreads();
-  double run = acc_in[j] + (p > 0 ? s_part[p - 1] : 0.0);
-  k = t0;
-  for (; k + 8 <= t1; k += 8) {
-    double u[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) u[m] = a[k + m];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      pre[(k + m) * 16 + j] = run;
-      run = run + u[m];
+      for (int k = 0; k < 16; ++k) s_lb[t][k] = ld_d(sq + W_AAGG + k);
     }
   }
-  for (; k < t1; ++k) {
-    pre[k * 16 + j] = run;
-    run = run + a[k];
+  __syncthreads();
+  double run = acc_in[j];
+  for (uint32_t q = 0; q < p; ++q) run = run + s_lb[q][j];
+  // rounds of 64 tiles: lane l's exclusive prefix within the round, then the round's starts
+  // written as 64 x 16 consecutive words
+  for (uint64_t r0 = t0; r0 < t1; r0 += kWave) {
+    const double v = r0 + l < t1 ? a[r0 + l] : 0.0;
+    double inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const double o = __shfl_up(inc, d);
+      if (l >= d) inc = o + inc;
+    }
+    s_out[l * 16 + j] = run + (inc - v);
+    run = run + __shfl(inc, kWave - 1);
+    __syncthreads();
+    const uint64_t rows = (t1 - r0 < uint64_t(kWave) ? t1 - r0 : uint64_t(kWave)) * 16;
+    if (static_cast<uint64_t>(t) < rows) pre[r0 * 16 + t] = s_out[t];
+    __syncthreads();
   }
+}
+
+// Launches the prefix over `tiles` tiles: up to kPrefixMaxBlocks blocks of at least 256 tiles.
+void canon_prefix(Ctx &c, const double *asum, uint64_t tiles, const double *acc_in, double *pre) {
+  uint64_t nb = tiles / 256;
+  nb = nb < 1 ? 1 : nb > kPrefixMaxBlocks ? kPrefixMaxBlocks : nb;
+  uint64_t *status = status_words(c, ST_CANON, nb * kStride);
+  k_canon_prefix<<<uint32_t(nb), kPrefixThreads, 0, c.stream>>>(asum, tiles, acc_in, pre, status, next_epoch(c),
+                                                                c.scratch.counters, uint32_t(nb));
 }
 
 // 2. Like steps 1, 3 and 4 of the single-pass form, with the hypothesis known at the start;
@@ -1352,10 +1361,10 @@ void canon_streamed(Ctx &c, RV rv, const double *x, const double *y, uint64_t bl
     uint32_t *counts = list + tiles;  // [0] listed tiles, [1] record slots taken
     const uint32_t fix_grid = static_cast<uint32_t>(tiles < 512 ? tiles : 512);
     k_canon_sample<RV><<<uint32_t(tiles), kBlock, 0, c.stream>>>(rv, x, y, blocks, tiles, asum);
-    k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
+    canon_prefix(c, asum, tiles, acc, pre);
     k_canon_summ<RV><<<uint32_t(tiles), summ_threads<RV>(), 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, asum,
                                                                     counts);
-    k_canon_prefix<<<16, kPrefixParts, 0, c.stream>>>(asum, tiles, acc, pre);
+    canon_prefix(c, asum, tiles, acc, pre);
     k_canon_check<<<uint32_t((tiles * 16 + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(pre, tiles, sums, list,
                                                                                        counts, rcap, keys);
     k_canon_fix<RV><<<fix_grid, kFastThreads, 0, c.stream>>>(rv, x, y, blocks, pre, tiles, sums, list, counts, rcap,
