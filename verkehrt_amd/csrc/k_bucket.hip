@@ -509,37 +509,102 @@ __global__ __launch_bounds__(kBlock) void k_check_keys_batch(const BatchItem *it
   if (lane_id() == 0 && my_bad != ~uint64_t(0)) atomicMin(reinterpret_cast<unsigned long long *>(&bad[it]), my_bad);
 }
 
-// One wave per 64 sorted rows of an item: the wave folds, one after another, the chains that
-// start in its rows, reading each chain 64 rows at a time (lanes load, lane order folds).
+// One wave per range of kFoldAhead * 64 sorted rows of an item: the wave loads the range in
+// one round (every load issued before the first use), then walks it in row order: each chain
+// that starts in the range folds its values, read from the loaded lanes (readlane), from +0.0 in
+// row order (the sort is stable). Rows before the range's first start belong to a chain that
+// started earlier, which its own range folds; the range's last chain goes on into the rows after
+// the range, loaded kFoldAhead * 64 at a time, until its key changes.
+constexpr int kFoldAhead = 8;
+
+__device__ __forceinline__ uint32_t read_lane_u32(uint32_t v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ double read_lane_d(double v, int j) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+  const uint64_t lo = read_lane_u32(static_cast<uint32_t>(b), j);
+  const uint64_t hi = read_lane_u32(static_cast<uint32_t>(b >> 32), j);
+  return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
+}
+template <typename K>
+__device__ __forceinline__ K read_lane_k(K v, int j) {
+  if constexpr (sizeof(K) == 8) {
+    const uint64_t lo = read_lane_u32(static_cast<uint32_t>(v), j);
+    const uint64_t hi = read_lane_u32(static_cast<uint32_t>(uint64_t(v) >> 32), j);
+    return static_cast<K>((hi << 32) | lo);
+  } else {
+    return static_cast<K>(read_lane_u32(static_cast<uint32_t>(v), j));
+  }
+}
+
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_chain_fold_batch(const BatchItem *items, const uint32_t *tile_item,
                                                              const K *sk, const double *sv,
                                                              double *stripes) {
+  constexpr uint64_t kSpan = uint64_t(kFoldAhead) * kWave;
+  static_assert(kBTile % ((kBlock / kWave) * kSpan) == 0, "ranges tile the block's rows");
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile;
+  const uint64_t n = I.n;
   const K *k_ = sk + I.aux;
   const double *v_ = sv + I.aux;
   double *st = stripes + I.aux3;
-  const int l = lane_id();
-  for (uint64_t r0 = t0 + (threadIdx.x / kWave) * kWave; r0 < t0 + kBTile && r0 < I.n;
-       r0 += kBlock) {
-    const uint64_t r = r0 + l;
-    const bool start = r < I.n && (r == 0 || k_[r - 1] != k_[r]);
-    uint64_t starts = __ballot(start);
-    while (starts) {
-      const int s = __builtin_ctzll(starts);
-      starts &= starts - 1;
-      const uint64_t p = r0 + s;
-      const K k = k_[p];
-      double a = 0.0;
-      for (uint64_t q = p;; q += kWave) {
-        const uint64_t rr = q + l;
-        const bool in = rr < I.n && k_[rr] == k;
-        const double v = in ? v_[rr] : 0.0;
-        const uint64_t out = __ballot(!in);
-        const int cnt = out ? __builtin_ctzll(out) : kWave;
-        for (int j = 0; j < cnt; ++j) a = a + __shfl(v, j);
-        if (cnt < kWave) break;
+  const int l = lane_id(), w = threadIdx.x / kWave;
+  K kk[kFoldAhead];
+  double vv[kFoldAhead];
+  auto load = [&](uint64_t base) {
+#pragma unroll
+    for (int u = 0; u < kFoldAhead; ++u) {
+      const uint64_t q = base + uint64_t(u) * kWave + l;
+      kk[u] = q < n ? k_[q] : K(0);
+      vv[u] = q < n ? v_[q] : 0.0;
+    }
+  };
+  for (uint64_t r0 = t0 + uint64_t(w) * kSpan; r0 < t0 + kBTile && r0 < n; r0 += (kBlock / kWave) * kSpan) {
+    load(r0);
+    K prev = r0 > 0 ? k_[r0 - 1] : K(0);  // the key of the row before the range
+    bool open = false;                     // a chain that started in the range is being folded
+    K k = 0;
+    double a = 0.0;
+#pragma unroll
+    for (int u = 0; u < kFoldAhead; ++u) {
+      const uint64_t c0 = r0 + uint64_t(u) * kWave;
+      if (c0 < n) {
+        const int lim = n - c0 < uint64_t(kWave) ? static_cast<int>(n - c0) : kWave;
+        const K up = static_cast<K>(__shfl_up(static_cast<unsigned long long>(kk[u]), 1));
+        const K pk = l == 0 ? prev : up;
+        const uint64_t starts = __ballot(l < lim && (c0 + l == 0 || pk != kk[u]));
+        prev = read_lane_k(kk[u], kWave - 1);
+        int j = 0;
+        while (j < lim) {
+          if ((starts >> j) & 1) {
+            if (open && l == 0) st[k] = a;
+            k = read_lane_k(kk[u], j);
+            a = 0.0;
+            open = true;
+          }
+          const uint64_t after = j < kWave - 1 ? starts & (~uint64_t(0) << (j + 1)) : 0;
+          const int end = after ? __builtin_ctzll(after) : lim;
+          if (open)
+            for (int q = j; q < end; ++q) a = a + read_lane_d(vv[u], q);
+          j = end;
+        }
+      }
+    }
+    if (open) {
+      for (uint64_t at = r0 + kSpan; at < n;) {
+        load(at);
+        bool done = false;
+#pragma unroll
+        for (int u = 0; u < kFoldAhead; ++u) {
+          const uint64_t c0 = at + uint64_t(u) * kWave;
+          if (!done && c0 < n) {
+            const uint64_t other = __ballot(!(c0 + l < n && kk[u] == k));
+            const int end = other ? __builtin_ctzll(other) : kWave;
+            for (int q = 0; q < end; ++q) a = a + read_lane_d(vv[u], q);
+            done = end < kWave;
+          }
+        }
+        if (done) break;
+        at += kSpan;
       }
       if (l == 0) st[k] = a;
     }
