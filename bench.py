@@ -734,7 +734,8 @@ def main():
     ap.add_argument("--only", default="",
                     help="comma-separated primitive labels to run (for profiling); 'copy_f64' adds a "
                          "plain copy (add_f64 x + 0.0) used to calibrate HBM counters; 'zero_mean' "
-                         "alone runs only the zero-mean canonical and f64 scan leg")
+                         "alone runs only the zero-mean canonical and f64 scan leg, 'sort' alone only the "
+                         "sort and grouping suite")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this backend has no multi-GPU path (asked for {args.gpus})",
@@ -748,6 +749,9 @@ def main():
     table_rows = min(1 << 26, n)
     S = Scalar
 
+    if args.only == "sort":
+        print(json.dumps({"sort": sort_suite(D, verify=not args.no_verify)}))
+        return
     x1 = D.call("rand_f64", [S("i64", n), S("i64", 1)])
     if args.only == "zero_mean":
         print(json.dumps({"zero_mean": zero_mean_suite(D, x1, n, verify=not args.no_verify)}))

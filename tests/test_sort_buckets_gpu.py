@@ -124,3 +124,18 @@ def test_large_sub_buckets(D):
     k = (hi << np.uint64(56)) | (lo << np.uint64(48)) | (mid << np.uint64(40))
     k |= rng.integers(0, 1 << 40, n, dtype=np.uint64)
     _check(D, k, "u64")
+
+
+@pytest.mark.parametrize("pool", [1024, 4096])
+def test_few_distinct_keys(D, pool):
+    # keys drawn from a pool of random values: each top-16 bucket holds one key, repeated about
+    # n / pool times; 1024 keys overflow every local sort (the positions are copied as they
+    # stand, no lower digit varying in any bucket), 4096 keys fit (one-key buckets)
+    rng = np.random.default_rng(16 + pool)
+    n = (1 << 22) + 3
+    vals = _rng_u64(rng, pool)
+    _check(D, vals[rng.integers(0, pool, n)], "u64")
+    # the same with a second key value in some buckets: lower digits vary there
+    k = vals[rng.integers(0, pool, n)]
+    k[rng.random(n) < 0.01] ^= np.uint64(0x5A5A)
+    _check(D, k, "u64")

@@ -434,7 +434,8 @@ struct LocalDigits {
 //   earlier place, and its position goes straight to out;
 // - otherwise the block runs a stable pass per local digit, least significant first, from the
 //   rows as loaded.
-// A larger bucket is listed in overflow[1..] (count in overflow[0]) for the batched passes.
+// A bucket whose rows all hold one key is in order as it stands. A larger bucket is listed in
+// overflow[1..] (count in overflow[0]) for the batched passes.
 constexpr uint32_t kSubMax = 96;
 
 template <typename U, int THREADS, int ITEMS>
@@ -469,12 +470,20 @@ __global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ ke
   const uint64_t below = (uint64_t(1) << l) - 1;
   U key[ITEMS];
   uint32_t p[ITEMS];
+  const U k0 = keys[s];
+  bool differ = false;
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t r = wbase + uint32_t(i) * kWave + l;
     const bool ok = uint32_t(i) < ipw && r < m;
     key[i] = ok ? keys[s + r] : U(0);
     p[i] = ok ? pos[s + r] : 0u;
+    differ = differ || (ok && key[i] != k0);
+  }
+  // a bucket of one key is in order already
+  if (!__syncthreads_or(differ)) {
+    for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(pos[s + j]);
+    return;
   }
   // One stable pass by the digit at `shift` over the rows in registers: they land in LDS in
   // digit order, s_excl holds the digit starts (s_excl[256] = m) and s_max the largest count.
@@ -693,6 +702,16 @@ __global__ void k_iota_batch(const BatchItem *items, const uint32_t *tile_item) 
   for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) out[r] = static_cast<int64_t>(r);
 }
 
+// Every item's rows are positions in order already: out[r] = pos[aux + r] as i64.
+__global__ void k_pos_batch(const BatchItem *items, const uint32_t *tile_item, const uint32_t *pos) {
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
+  const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
+  int64_t *out = static_cast<int64_t *>(I.out);
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) out[r] = static_cast<int64_t>(pos[I.aux + r]);
+}
+
 template <typename U, typename V, bool FIRST, bool LAST>
 void run_pass_batch(Ctx &c, int32_t type, const BatchPlan &p, int P, int d, const void *kin,
                     const V *pin, const uint32_t *hist, U *kout, V *pout) {
@@ -855,13 +874,23 @@ void argsort_buckets(Ctx &c, int32_t type, const void *keys, uint64_t n, const s
     k_base_batch<<<stream_grid(items.size() * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(bp.items, items.size(),
                                                                                          P, bh, active);
     check_hip(hipGetLastError(), "overflow bases");
-    // the bucket's rows are in row order; a stable pass over each lower digit (constant ones
-    // are identity passes) finishes them, the last writing positions
+    uint32_t act[8] = {0};
+    read_back(c, active, P * 4, act);
+    // the buckets' rows are in row order; a stable pass over each lower digit that varies in some
+    // bucket (an identity pass where it does not) finishes them, the last writing positions; with
+    // none, the positions are copied as they are (few keys, each repeated: one key per bucket)
+    std::vector<int> lower;
+    for (int k = 0; k < np - 2; ++k)
+      if (act[digits[k]]) lower.push_back(digits[k]);
     const U *kin = kb;
     const V *pin = pb;
-    for (int k = 0; k < np - 2; ++k) {
-      const int d = digits[k];
-      if (k == np - 3) {
+    if (lower.empty()) {
+      k_pos_batch<<<bp.ntiles, kBlock, 0, c.stream>>>(bp.items, bp.tile_item, pb);
+      check_hip(hipGetLastError(), "overflow positions");
+    }
+    for (size_t k = 0; k < lower.size(); ++k) {
+      const int d = lower[k];
+      if (k + 1 == lower.size()) {
         run_pass_batch<U, V, false, true>(c, img, bp, P, d, kin, pin, bh, nullptr, nullptr);
       } else {
         U *ko = (k % 2 == 0) ? ka : kb;
