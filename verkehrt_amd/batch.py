@@ -64,14 +64,18 @@ class Workload:
     k: Array          # all arrays' keys, concatenated
     offsets: List[int]
     lengths: List[int]
+    views: List[tuple] = None  # array i's (x, k) views, made once: the resident inputs
+
+    def __post_init__(self):
+        if self.views is None:
+            self.views = [(self.x.slice(o, n), self.k.slice(o, n)) for o, n in zip(self.offsets, self.lengths)]
 
     @property
     def rows(self) -> int:
         return sum(self.lengths)
 
     def item(self, i: int):
-        o, n = self.offsets[i], self.lengths[i]
-        return self.x.slice(o, n), self.k.slice(o, n)
+        return self.views[i]
 
 
 def make_workload(D: Dispatch, count: int = 4096, lo: int = 1 << 10, hi: int = 1 << 20,

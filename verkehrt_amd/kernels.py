@@ -90,8 +90,11 @@ class Scalar:
 
     @staticmethod
     def from_c(cs: N.CScalar) -> "Scalar":
-        t = TYPES[cs.type]
-        b = cs.bits
+        return Scalar.from_bits(TYPES[cs.type], cs.bits)
+
+    @staticmethod
+    def from_bits(t: str, b: int) -> "Scalar":
+        """The scalar of type t whose 64-bit word is b (the layout of vk_scalar.bits)."""
         if t == "f64":
             v = struct.unpack("<d", struct.pack("<Q", b))[0]
         elif t == "f32":
@@ -242,19 +245,50 @@ class Array:
 Value = Union[Array, Scalar]
 
 
+# vk_arg and vk_value as numpy records (the layouts of N.CArg and N.CValue: 32 bytes each), so
+# long argument lists of batched calls are filled column by column instead of field by field.
+_ARG_DT = np.dtype({"names": ["kind", "rows", "stype", "bits"], "formats": ["<i4", "<u8", "<i4", "<u8"],
+                    "offsets": [0, 8, 16, 24], "itemsize": 32})
+_VALUE_DT = np.dtype({"names": ["kind", "array", "stype", "bits"], "formats": ["<i4", "<u8", "<i4", "<u8"],
+                      "offsets": [0, 8, 16, 24], "itemsize": 32})
+
+
 def _args(values: Sequence[Value]):
-    arr = (N.CArg * max(1, len(values)))()
+    n = len(values)
+    if n < 64:  # short lists: field by field
+        arr = (N.CArg * max(1, n))()
+        for i, v in enumerate(values):
+            if isinstance(v, Array):
+                arr[i].kind = N.ROWS
+                arr[i].rows = v.handle
+            elif isinstance(v, Scalar):
+                arr[i].kind = N.SPLAT
+                arr[i].splat.type = TYPE_CODE[v.type]
+                arr[i].splat.bits = v.bits()
+            else:
+                raise TypeError(f"argument {i} is neither Array nor Scalar: {v!r}")
+        return arr
+    kinds, rows, stypes, bits = [], [], [], []
     for i, v in enumerate(values):
         if isinstance(v, Array):
-            arr[i].kind = N.ROWS
-            arr[i].rows = v.handle
+            kinds.append(N.ROWS)
+            rows.append(v.handle.value or 0)
+            stypes.append(0)
+            bits.append(0)
         elif isinstance(v, Scalar):
-            arr[i].kind = N.SPLAT
-            arr[i].splat.type = TYPE_CODE[v.type]
-            arr[i].splat.bits = v.bits()
+            kinds.append(N.SPLAT)
+            rows.append(0)
+            stypes.append(TYPE_CODE[v.type])
+            bits.append(v.bits())
         else:
             raise TypeError(f"argument {i} is neither Array nor Scalar: {v!r}")
-    return arr
+    rec = np.zeros(max(1, n), _ARG_DT)
+    if n:
+        rec["kind"][:n] = kinds
+        rec["rows"][:n] = rows
+        rec["stype"][:n] = stypes
+        rec["bits"][:n] = np.array(bits, dtype=np.uint64)
+    return (N.CArg * max(1, n)).from_buffer(rec)  # keeps rec alive
 
 
 class Registry:
@@ -408,16 +442,18 @@ class Dispatch:
             raise ValueError("every item needs the same number of arguments")
         flat = [v for it in items for v in it]
         a = _args(flat)
-        out = (N.CValue * len(items))()
+        rec = np.zeros(len(items), _VALUE_DT)
+        out = (N.CValue * len(items)).from_buffer(rec)
         st = N.Status()
         _check(self._lib.vk_batch_call(self.ctx.handle, name.encode(), a, nargs, len(items), out,
                                        C.byref(st)), st)
         res = []
-        for o in out:
-            if o.kind == N.VALUE_SCALAR:
-                res.append(Scalar.from_c(o.scalar))
+        for kind, arr, stype, b in zip(rec["kind"].tolist(), rec["array"].tolist(), rec["stype"].tolist(),
+                                       rec["bits"].tolist()):
+            if kind == N.VALUE_SCALAR:
+                res.append(Scalar.from_bits(TYPES[stype], b))
             else:
-                res.append(Array(self.ctx, o.array))
+                res.append(Array(self.ctx, arr))
         return res
 
     def batch_plan(self, inputs: Sequence[Sequence[Array]]) -> "BatchPlan":
