@@ -1413,7 +1413,7 @@ struct PlanStmt {
   SelectSpec ss{};
   CanonSpec cs{};
   ReduceSpec rs{};
-  int items_per_thread = 16;    // selection rows per thread (autotuned: 8 or 16)
+  int items_per_thread = 16;    // selection rows per thread (autotuned: 8, 16 or 0, the persistent filter)
   uint64_t tile_rows = 8192;    // reduction rows per tile (autotuned: 4096, 8192 or 16384)
   std::vector<uint64_t> host;   // results copied back after a run
 };
@@ -1531,16 +1531,21 @@ void plan_tune(vk_bplan &p) {
     PlanStmt &s = p.st[k];
     std::string line = "statement " + std::to_string(k) + " " + s.d->name + ":";
     if (s.kind == 0) {
+      // one tile per block with 8 or 16 rows per thread, or (filters) the persistent form: one
+      // block per compute unit loading its next tile during each look-back
       int best = 16;
       float bt = 1e30f;
-      for (int ipt : {8, 16}) {
+      std::vector<int> cands = {8, 16};
+      if (s.ss.kind == 1) cands.push_back(0);
+      for (int ipt : cands) {
         s.items_per_thread = ipt;
         const float t = time_stmt(s);
-        line += " " + std::to_string(ipt) + " rows/thread " + std::to_string(t * 1000.0f) + " us;";
+        line += ipt ? " " + std::to_string(ipt) + " rows/thread " : std::string(" persistent ");
+        line += std::to_string(t * 1000.0f) + " us;";
         if (t < bt) { bt = t; best = ipt; }
       }
       s.items_per_thread = best;
-      line += " chose " + std::to_string(best);
+      line += best ? " chose " + std::to_string(best) : std::string(" chose persistent");
     } else if (s.kind == 2) {
       uint64_t best = 8192;
       float bt = 1e30f;

@@ -211,15 +211,21 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
 // current one resolves. Ticket: a block takes no more after one finds no tile, so a launch
 // takes ntiles + gridDim.x tickets and the last one returns the ticket to zero. A block only
 // waits on tiles below its current one, all held by running blocks, so no wait is circular.
-template <typename T, int KIND>
+// BATCH: a tile's item (rows, row count, output, first tile, count slot) comes from the batch
+// tables, and its look-back stops at the item's first tile; a launch whose tiles were planned
+// on the device passes dev_ntiles (every block reads the same planned count) and dev_epoch.
+template <typename T, int KIND, bool BATCH = false>
 __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_select_p(
     const T *__restrict__ x, const uint8_t *__restrict__ mask, uint64_t n, uint32_t keepset, T c, uint64_t base,
-    void *out_v, uint64_t *status, uint32_t epoch, uint32_t *ticket, uint32_t ntiles, uint64_t *d_count) {
+    void *out_v, uint64_t *status, uint32_t epoch, uint32_t *ticket, uint32_t ntiles, uint64_t *d_count,
+    const BatchItem *items = nullptr, const uint32_t *tile_item = nullptr, uint64_t *counts = nullptr,
+    const uint32_t *dev_ntiles = nullptr, const uint32_t *dev_epoch = nullptr) {
   using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
   struct alignas(16) Q {
     uint32_t a[4];
   };
-  Out *out = static_cast<Out *>(out_v);
+  if (dev_ntiles) ntiles = *dev_ntiles;
+  if (dev_epoch) epoch = *dev_epoch;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint32_t takes = ntiles + gridDim.x;
   __shared__ uint32_t s_tile;
@@ -239,19 +245,55 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(4, 4)
     __syncthreads();
     return s_tile;
   };
-  auto rows_of = [&](uint32_t tile) -> uint32_t {
-    const uint64_t rem = n - uint64_t(tile) * kTile;
-    return rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);
+  // a tile's rows: its array (values x, mask m), first row t0 in the array, row count, the
+  // first tile of its array, whether it is the array's last tile, output, count slot, base
+  struct Ref {
+    const T *x;
+    const uint8_t *m;
+    uint64_t t0;
+    uint32_t lim, first;
+    bool last;
+    Out *out;
+    uint64_t *cnt;
+    uint64_t base;
+  };
+  auto ref = [&](uint32_t tile) -> Ref {
+    Ref r;
+    uint64_t rows;
+    if constexpr (BATCH) {
+      const uint32_t it = tile_item[tile];
+      const BatchItem &I = items[it];
+      r.x = static_cast<const T *>(I.a);
+      r.m = KIND == 2 ? static_cast<const uint8_t *>(I.a) : static_cast<const uint8_t *>(I.b);
+      r.first = static_cast<uint32_t>(I.tile0);
+      r.out = static_cast<Out *>(I.out);
+      r.cnt = counts + it;
+      r.base = I.aux;
+      rows = I.n;
+    } else {
+      r.x = x;
+      r.m = mask;
+      r.first = 0;
+      r.out = static_cast<Out *>(out_v);
+      r.cnt = d_count;
+      r.base = base;
+      rows = n;
+    }
+    r.t0 = uint64_t(tile - r.first) * kTile;
+    const uint64_t rem = rows - r.t0;
+    r.lim = rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);
+    r.last = rem <= uint64_t(kTile);
+    return r;
   };
   // the tile's rows into registers (rows past the end load the last row), and the mask bytes
   // of an aligned full tile (16 per lane); other tiles' mask bytes are loaded when counted
   auto load = [&](uint32_t tile, Out (&val)[kItems], Q &mq, bool &wide) {
-    const uint64_t t0 = uint64_t(tile) * kTile;
-    const uint32_t lim = rows_of(tile);
-    const T *xt = KIND == 2 ? nullptr : x + t0;
+    const Ref r = ref(tile);
+    const uint32_t lim = r.lim;
+    const T *xt = KIND == 2 ? nullptr : r.x + r.t0;
     wide = false;
     if constexpr (KIND != 1) {
-      const uint8_t *mt = mask + t0;
+      const uint8_t *mt = r.m + r.t0;
       wide = lim == uint32_t(kTile) && (reinterpret_cast<uintptr_t>(mt) & 15) == 0;
       if (wide && (l >> 2) < kItems)
         mq = *reinterpret_cast<const Q *>(mt + uint32_t(l >> 2) * kSelBlock + uint32_t(w) * kWave + (l & 3) * 16);
@@ -266,7 +308,8 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(4, 4)
   };
   // ballots, the block's exclusive counts (s_excl, s_total) and the published aggregate
   auto count = [&](uint32_t tile, const Out (&val)[kItems], const Q &mq, bool wide, uint64_t (&keepm)[kItems]) {
-    const uint32_t lim = rows_of(tile);
+    const Ref r = ref(tile);
+    const uint32_t lim = r.lim;
     uint8_t mv[kItems];
     if constexpr (KIND != 1) {
       if (wide) {
@@ -275,7 +318,7 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(4, 4)
 #pragma unroll
         for (int i = 0; i < kItems; ++i) mv[i] = s_mask[w][i][l];
       } else {
-        const uint8_t *mt = mask + uint64_t(tile) * kTile;
+        const uint8_t *mt = r.m + r.t0;
 #pragma unroll
         for (int i = 0; i < kItems; ++i) {
           const uint32_t lr = uint32_t(i) * kSelBlock + t;
@@ -316,27 +359,29 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(4, 4)
       const uint32_t total = __shfl(inc, 63);
       if (l == 0) {
         s_total = total;
-        st_rlx(&status[tile], pack(epoch, tile == 0 ? 2 : 1, total));
+        st_rlx(&status[tile], pack(epoch, tile == r.first ? 2 : 1, total));
       }
     }
   };
   // the look-back (wave 0), the inclusive count, and the survivors written
   auto finish = [&](uint32_t tile, const Out (&val)[kItems], const uint64_t (&keepm)[kItems]) {
+    const Ref r = ref(tile);
     if (w == 0) {
       const uint32_t total = s_total;
       uint64_t prefix = 0;
-      if (tile > 0) {
-        prefix = look_back(status, epoch, tile, 0);
+      if (tile > r.first) {
+        prefix = look_back(status, epoch, tile, r.first);
         if (l == 0) st_rlx(&status[tile], pack(epoch, 2, prefix + total));
       }
       if (l == 0) {
         s_prefix = prefix;
-        if (tile == ntiles - 1) *d_count = prefix + total;
+        if (r.last) *r.cnt = prefix + total;
       }
     }
     __syncthreads();
-    Out *ot = out + s_prefix;
-    const uint64_t t0 = uint64_t(tile) * kTile;
+    Out *ot = r.out + s_prefix;
+    const uint64_t t0 = r.t0;
+    const uint64_t base = r.base;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       if ((keepm[i] >> l) & 1) {
@@ -426,6 +471,17 @@ void launch_batch(Ctx &c, const SelectSpec &s, std::vector<BatchItem> &items, ui
     const uint32_t epoch = next_epoch(c);
     T cv;
     __builtin_memcpy(&cv, &s.cbits, sizeof(T));
+    if constexpr (KIND == 1) {
+      // many tiles: the persistent form, every item's tiles in one ticket order
+      if (p.ntiles >= 2 * uint32_t(c.cus)) {
+        k_select_p<T, KIND, true><<<static_cast<uint32_t>(c.cus), kSelBlock, 0, c.stream>>>(
+            nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
+            nullptr, p.items, p.tile_item, d_counts);
+        check_hip(hipGetLastError(), "select batch launch");
+        batch_plan_free(c, p);
+        return;
+      }
+    }
     k_select<T, KIND, true><<<p.ntiles, kSelBlock, 0, c.stream>>>(
         nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, status, epoch, c.scratch.counters, p.ntiles,
         nullptr, p.items, p.tile_item, d_counts, nullptr, nullptr);
@@ -493,13 +549,21 @@ void select_rows_scratch(Ctx &c, const SelectSpec &s, const void *x, const uint8
 
 namespace {
 
+// ITEMS 0: the persistent form (filters only), one block per compute unit over 16K-row tiles.
 template <typename T, int KIND, int ITEMS>
 void launch_dev(Ctx &c, const SelectSpec &s, const DevPlan &p, uint64_t *d_counts) {
   T cv;
   __builtin_memcpy(&cv, &s.cbits, sizeof(T));
-  k_select<T, KIND, true, ITEMS><<<p.max_tiles, kSelBlock, 0, c.stream>>>(
-      nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, p.status, 0, p.ticket, p.max_tiles, nullptr,
-      p.items, p.tile_item, d_counts, p.ntiles, p.epoch);
+  if constexpr (ITEMS == 0) {
+    static_assert(KIND == 1, "the persistent form is a filter's");
+    k_select_p<T, KIND, true><<<static_cast<uint32_t>(c.cus), kSelBlock, 0, c.stream>>>(
+        nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, p.status, 0, p.ticket, p.max_tiles, nullptr,
+        p.items, p.tile_item, d_counts, p.ntiles, p.epoch);
+  } else {
+    k_select<T, KIND, true, ITEMS><<<p.max_tiles, kSelBlock, 0, c.stream>>>(
+        nullptr, nullptr, 0, keepset_of(s.cmp), cv, 0, nullptr, p.status, 0, p.ticket, p.max_tiles, nullptr,
+        p.items, p.tile_item, d_counts, p.ntiles, p.epoch);
+  }
 }
 
 template <int KIND, int ITEMS>
@@ -517,10 +581,15 @@ void dev_by_width(Ctx &c, const SelectSpec &s, const DevPlan &p, uint64_t *d_cou
 
 template <int ITEMS>
 void dev_by_kind(Ctx &c, const SelectSpec &s, const DevPlan &p, uint64_t *d_counts) {
-  switch (s.kind) {
-    case 0: dev_by_width<0, ITEMS>(c, s, p, d_counts); break;
-    case 1: dev_by_width<1, ITEMS>(c, s, p, d_counts); break;
-    default: launch_dev<uint8_t, 2, ITEMS>(c, s, p, d_counts); break;
+  if constexpr (ITEMS == 0) {
+    if (s.kind != 1) usage_error("the persistent selection form is a filter's");
+    dev_by_width<1, 0>(c, s, p, d_counts);
+  } else {
+    switch (s.kind) {
+      case 0: dev_by_width<0, ITEMS>(c, s, p, d_counts); break;
+      case 1: dev_by_width<1, ITEMS>(c, s, p, d_counts); break;
+      default: launch_dev<uint8_t, 2, ITEMS>(c, s, p, d_counts); break;
+    }
   }
 }
 
@@ -533,6 +602,7 @@ void select_batch_dev(Ctx &c, const SelectSpec &s, int items_per_thread, const D
   fill_u64(c, d_counts, nitems, 0);  // items without rows keep a count of 0
   if (p.max_tiles == 0) return;
   if (items_per_thread == 8) dev_by_kind<8>(c, s, p, d_counts);
+  else if (items_per_thread == 0) dev_by_kind<0>(c, s, p, d_counts);
   else dev_by_kind<16>(c, s, p, d_counts);
   check_hip(hipGetLastError(), "select batch (device plan)");
 }

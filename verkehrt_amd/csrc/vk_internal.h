@@ -267,7 +267,8 @@ struct DevPlan {
 // the map and the tile count.
 void plan_tiles_dev(Ctx &c, const DevPlan &p, uint32_t nitems, const uint64_t *dev_n, uint32_t unit,
                     uint64_t per_tile, uint64_t min_rows = 0);
-// Rows per tile of the batched selection with 8 or 16 rows per thread.
+// Rows per tile of the batched selection with 8 or 16 rows per thread (0: the persistent filter
+// form, one block per compute unit over 16K-row tiles).
 uint64_t select_tile_rows(int items_per_thread);
 void select_batch_dev(Ctx &c, const SelectSpec &s, int items_per_thread, const DevPlan &p, uint32_t nitems,
                       uint64_t *d_counts);
