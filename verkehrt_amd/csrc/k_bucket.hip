@@ -509,106 +509,41 @@ __global__ __launch_bounds__(kBlock) void k_check_keys_batch(const BatchItem *it
   if (lane_id() == 0 && my_bad != ~uint64_t(0)) atomicMin(reinterpret_cast<unsigned long long *>(&bad[it]), my_bad);
 }
 
-// One wave per range of kFoldAhead * 64 sorted rows of an item: the wave loads the range in
-// one round (every load issued before the first use), then walks it in row order: each chain
-// that starts in the range folds its values, read from the loaded lanes (readlane), from +0.0 in
-// row order (the sort is stable). Rows before the range's first start belong to a chain that
-// started earlier, which its own range folds; the range's last chain goes on into the rows after
-// the range, loaded kFoldAhead * 64 at a time, until its key changes.
-constexpr int kFoldAhead = 8;
-
-__device__ __forceinline__ uint32_t read_lane_u32(uint32_t v, int j) { return __builtin_amdgcn_readlane(v, j); }
-__device__ __forceinline__ double read_lane_d(double v, int j) {
-  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
-  const uint64_t lo = read_lane_u32(static_cast<uint32_t>(b), j);
-  const uint64_t hi = read_lane_u32(static_cast<uint32_t>(b >> 32), j);
-  return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
-}
+// One lane per chain key of an item (items here: n = the item's chains, aux = its first row in
+// the sorted buffers, aux2 = its rows, aux3 = its first stripe): the lane finds its chain's rows
+// by binary search of the sorted chain keys and folds them from +0.0 in row order (the sort is
+// stable), eight loads in flight; an empty chain leaves its stripe at 0.0.
 template <typename K>
-__device__ __forceinline__ K read_lane_k(K v, int j) {
-  if constexpr (sizeof(K) == 8) {
-    const uint64_t lo = read_lane_u32(static_cast<uint32_t>(v), j);
-    const uint64_t hi = read_lane_u32(static_cast<uint32_t>(uint64_t(v) >> 32), j);
-    return static_cast<K>((hi << 32) | lo);
-  } else {
-    return static_cast<K>(read_lane_u32(static_cast<uint32_t>(v), j));
-  }
-}
-
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_chain_fold_batch(const BatchItem *items, const uint32_t *tile_item,
-                                                             const K *sk, const double *sv,
-                                                             double *stripes) {
-  constexpr uint64_t kSpan = uint64_t(kFoldAhead) * kWave;
-  static_assert(kBTile % ((kBlock / kWave) * kSpan) == 0, "ranges tile the block's rows");
+__global__ __launch_bounds__(kBlock) void k_chain_fold_keys_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                                  const K *sk, const double *sv, double *stripes) {
   const BatchItem &I = items[tile_item[blockIdx.x]];
-  const uint64_t t0 = (blockIdx.x - I.tile0) * kBTile;
-  const uint64_t n = I.n;
+  const uint64_t c = (blockIdx.x - I.tile0) * kBlock + threadIdx.x;  // the chain key
+  if (c >= I.n) return;
   const K *k_ = sk + I.aux;
   const double *v_ = sv + I.aux;
-  double *st = stripes + I.aux3;
-  const int l = lane_id(), w = threadIdx.x / kWave;
-  K kk[kFoldAhead];
-  double vv[kFoldAhead];
-  auto load = [&](uint64_t base) {
-#pragma unroll
-    for (int u = 0; u < kFoldAhead; ++u) {
-      const uint64_t q = base + uint64_t(u) * kWave + l;
-      kk[u] = q < n ? k_[q] : K(0);
-      vv[u] = q < n ? v_[q] : 0.0;
+  const uint64_t rows = I.aux2;
+  auto lower = [&](uint64_t key) {
+    uint64_t lo = 0, hi = rows;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (static_cast<uint64_t>(k_[mid]) < key) lo = mid + 1;
+      else hi = mid;
     }
+    return lo;
   };
-  for (uint64_t r0 = t0 + uint64_t(w) * kSpan; r0 < t0 + kBTile && r0 < n; r0 += (kBlock / kWave) * kSpan) {
-    load(r0);
-    K prev = r0 > 0 ? k_[r0 - 1] : K(0);  // the key of the row before the range
-    bool open = false;                     // a chain that started in the range is being folded
-    K k = 0;
-    double a = 0.0;
+  const uint64_t s = lower(c), e = lower(c + 1);
+  if (s == e) return;
+  double a = 0.0;
+  uint64_t q = s;
+  for (; q + 8 <= e; q += 8) {
+    double u[8];
 #pragma unroll
-    for (int u = 0; u < kFoldAhead; ++u) {
-      const uint64_t c0 = r0 + uint64_t(u) * kWave;
-      if (c0 < n) {
-        const int lim = n - c0 < uint64_t(kWave) ? static_cast<int>(n - c0) : kWave;
-        const K up = static_cast<K>(__shfl_up(static_cast<unsigned long long>(kk[u]), 1));
-        const K pk = l == 0 ? prev : up;
-        const uint64_t starts = __ballot(l < lim && (c0 + l == 0 || pk != kk[u]));
-        prev = read_lane_k(kk[u], kWave - 1);
-        int j = 0;
-        while (j < lim) {
-          if ((starts >> j) & 1) {
-            if (open && l == 0) st[k] = a;
-            k = read_lane_k(kk[u], j);
-            a = 0.0;
-            open = true;
-          }
-          const uint64_t after = j < kWave - 1 ? starts & (~uint64_t(0) << (j + 1)) : 0;
-          const int end = after ? __builtin_ctzll(after) : lim;
-          if (open)
-            for (int q = j; q < end; ++q) a = a + read_lane_d(vv[u], q);
-          j = end;
-        }
-      }
-    }
-    if (open) {
-      for (uint64_t at = r0 + kSpan; at < n;) {
-        load(at);
-        bool done = false;
+    for (int j = 0; j < 8; ++j) u[j] = v_[q + j];
 #pragma unroll
-        for (int u = 0; u < kFoldAhead; ++u) {
-          const uint64_t c0 = at + uint64_t(u) * kWave;
-          if (!done && c0 < n) {
-            const uint64_t other = __ballot(!(c0 + l < n && kk[u] == k));
-            const int end = other ? __builtin_ctzll(other) : kWave;
-            for (int q = 0; q < end; ++q) a = a + read_lane_d(vv[u], q);
-            done = end < kWave;
-          }
-        }
-        if (done) break;
-        at += kSpan;
-      }
-      if (l == 0) st[k] = a;
-    }
+    for (int j = 0; j < 8; ++j) a = a + u[j];
   }
+  for (; q < e; ++q) a = a + v_[q];
+  stripes[I.aux3 + c] = a;
 }
 
 // One block per item: the pairwise combine of every bucket's 16 stripes (finish_lanes,
@@ -668,12 +603,27 @@ void reduce_into_add_f64_batch(Ctx &c, std::vector<BatchItem> &items) {
     const size_t kw = wide ? 8 : 4;
     void *sk = tmp_alloc(c, total * kw);
     double *sv = static_cast<double *>(tmp_alloc(c, total * 8));
-    BatchPlan p = batch_plan(c, whole, kBTile);
     // the sort reads each item's bucket keys at a (chain keys computed as it reads them) and
     // its values at b
     sort_chain_values_batch(c, wide, whole, sk, reinterpret_cast<uint64_t *>(sv));
-    if (wide) k_chain_fold_batch<uint64_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<const uint64_t *>(sk), sv, stripes);
-    else k_chain_fold_batch<uint32_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, static_cast<const uint32_t *>(sk), sv, stripes);
+    // one lane per chain key of every item
+    std::vector<BatchItem> chains(whole.size());
+    for (size_t i = 0; i < whole.size(); ++i) {
+      chains[i] = BatchItem{};
+      chains[i].n = whole[i].n ? items[i].aux2 * 16 : 0;
+      chains[i].aux = whole[i].aux;
+      chains[i].aux2 = whole[i].n;
+      chains[i].aux3 = items[i].aux3;
+    }
+    BatchPlan p = batch_plan(c, chains, kBlock);
+    if (p.ntiles) {
+      if (wide)
+        k_chain_fold_keys_batch<uint64_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item,
+                                                                             static_cast<const uint64_t *>(sk), sv, stripes);
+      else
+        k_chain_fold_keys_batch<uint32_t><<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item,
+                                                                             static_cast<const uint32_t *>(sk), sv, stripes);
+    }
     check_hip(hipGetLastError(), "chain fold batch");
     batch_plan_free(c, p);
     tmp_free(c, sk);
