@@ -223,6 +223,23 @@ def test_group_id_batch(D, t):
         assert np.array_equal(g.to_numpy(), O.group_id(x)), (t, x.size)
 
 
+def test_group_id_batch_small_tables_overflow(D):
+    # items whose distinct keys overflow the first small tables (over 8192 slots' worth) beside
+    # items that fit them, and items with every key distinct
+    rng = np.random.default_rng(23)
+    xs = []
+    for i, n in enumerate([200000, 3000, 150000, 70000, 1, 100000, 40000]):
+        g = [20000, 7, 100000, 256, 1, None, 9000][i]
+        if g is None:
+            xs.append(rng.permutation(n).astype(np.int64) * 7919 - 3)
+        else:
+            pool = rng.integers(-2**62, 2**62, g)
+            xs.append(pool[rng.integers(0, g, n)])
+    got = D.batch_call("group_id_i64", [[D.array(x)] for x in xs])
+    for x, g in zip(xs, got):
+        assert np.array_equal(g.to_numpy(), O.group_id(x)), x.size
+
+
 def test_reduce_into_add_f64_batch(D):
     rng = np.random.default_rng(18)
     sizes = uneven(rng, 25, hi=80000)

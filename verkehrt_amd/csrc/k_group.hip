@@ -448,9 +448,11 @@ __global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const Sl
 }
 
 // ------------------------------------------------------------------ batched form
-// The same four passes for many arrays at once: each item has its own table (words aux2 ..
-// aux2 + 2^aux3 of one allocation) and its own rows in the concatenated slot and first-flag
-// buffers (from row aux); one launch per pass covers the tiles of every item.
+// The same four passes for many arrays at once: each item has its own table (2^aux3 words at
+// b) and its own rows in the concatenated slot and first-flag buffers (from row aux); one launch
+// per pass covers the tiles of every item. Tables start small (at most 2^kBSmallLg words): an
+// insert whose probe run there passes kMaxRun marks its item's overflow word, and only those
+// items insert again into tables of 2n words before the other passes run.
 constexpr uint64_t kGTile = 4096;
 
 struct ItemRows {
@@ -463,22 +465,25 @@ struct ItemRows {
 
 // The batched tables hold u64 words (first row + 1); a key is compared through the column.
 // Inserts (key k, row r): claims an empty slot with r + 1 or lowers the word of k's slot.
-__device__ void word_insert(const KeyCol &a, unsigned long long *table, uint64_t mask, int shift, uint64_t k,
-                            uint64_t r) {
+// Returns false when the probe run passes max_run slots (the key is not inserted).
+__device__ bool word_insert(const KeyCol &a, unsigned long long *table, uint64_t mask, int shift, uint64_t k,
+                            uint64_t r, uint64_t max_run) {
   uint64_t h = mix(k) >> shift;
   for (uint64_t p = 0; p <= mask; ++p) {
+    if (p > max_run) return false;
     unsigned long long s = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (s == 0) {
       const unsigned long long old = atomicCAS(&table[h], 0ull, r + 1);
-      if (old == 0) return;
+      if (old == 0) return true;
       s = old;
     }
     if (a.bits(s - 1) == k) {
       if (r + 1 < s) atomicMin(&table[h], r + 1);
-      return;
+      return true;
     }
     h = (h + 1) & mask;
   }
+  return false;
 }
 
 // The slot of key k (every key is in the table).
@@ -500,19 +505,23 @@ constexpr int kBLBits = 11;
 constexpr int kBLSlots = 1 << kBLBits;
 constexpr int kBRows = static_cast<int>(kGTile / kBlock);
 
+// aux2: the item's longest probe run (kNoLimit in a table of 2n words); overflow[item] is set
+// when a run passes it.
 __global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const BatchItem *items,
-                                                         const uint32_t *tile_item,
-                                                         unsigned long long *tables) {
+                                                         const uint32_t *tile_item, uint32_t *overflow) {
   __shared__ unsigned long long s_key[kBLSlots];
   __shared__ uint32_t s_row[kBLSlots];
   __shared__ uint32_t s_cnt;
-  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
   const ItemRows R(I, blockIdx.x);
   const KeyCol a{type, I.a};
-  unsigned long long *table = tables + I.aux2;
+  unsigned long long *table = static_cast<unsigned long long *>(const_cast<void *>(I.b));
   const int lg = static_cast<int>(I.aux3);
   const uint64_t mask = (uint64_t(1) << lg) - 1;
+  const uint64_t max_run = I.aux2;
   const int t = threadIdx.x;
+  bool over = false;
   for (int i = t; i < kBLSlots; i += kBlock) {
     s_key[i] = kLEmpty;
     s_row[i] = 0xFFFFFFFFu;
@@ -537,21 +546,21 @@ __global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const Bat
       if (__hip_atomic_load(&s_row[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > r)
         atomicMin(&s_row[h], static_cast<uint32_t>(r));
     } else {
-      word_insert(a, table, mask, 64 - lg, k[j], r);
+      if (!word_insert(a, table, mask, 64 - lg, k[j], r, max_run)) over = true;
     }
   }
   __syncthreads();
   for (int i = t; i < kBLSlots; i += kBlock) {
     const uint64_t kk = s_key[i];
-    if (kk != kLEmpty) word_insert(a, table, mask, 64 - lg, kk, s_row[i]);
+    if (kk != kLEmpty && !word_insert(a, table, mask, 64 - lg, kk, s_row[i], max_run)) over = true;
   }
+  if (over) atomicOr(&overflow[it], 1u);
 }
 
 // Phase 1 finds (or claims) every row's LDS entry, the claiming lane resolving the key's slot
 // and first row; after the barrier, phase 2 writes each row's slot and first-row flag.
 __global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const BatchItem *items,
                                                         const uint32_t *tile_item,
-                                                        const unsigned long long *tables,
                                                         uint32_t *slot, uint8_t *first) {
   __shared__ unsigned long long s_key[kBLSlots];
   __shared__ uint32_t s_slot[kBLSlots];
@@ -560,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const Batc
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const ItemRows R(I, blockIdx.x);
   const KeyCol a{type, I.a};
-  const unsigned long long *table = tables + I.aux2;
+  const unsigned long long *table = static_cast<const unsigned long long *>(I.b);
   const int lg = static_cast<int>(I.aux3);
   const uint64_t mask = (uint64_t(1) << lg) - 1;
   const int t = threadIdx.x;
@@ -610,22 +619,21 @@ __global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const Batc
 // Row j < groups of an item is the item's j-th first row: its slot gets id j.
 __global__ __launch_bounds__(kBlock) void k_number_batch(const BatchItem *items, const uint32_t *tile_item,
                                                          const int64_t *firsts, const uint64_t *groups,
-                                                         const uint32_t *slot, unsigned long long *tables) {
+                                                         const uint32_t *slot) {
   const uint32_t it = tile_item[blockIdx.x];
   const BatchItem &I = items[it];
   const ItemRows R(I, blockIdx.x);
   const uint64_t g = groups[it];
-  uint32_t *id_of_slot = reinterpret_cast<uint32_t *>(tables + I.aux2);
+  uint32_t *id_of_slot = static_cast<uint32_t *>(const_cast<void *>(I.b));
   for (uint64_t j = R.t0 + threadIdx.x; j < R.t1 && j < g; j += kBlock)
     id_of_slot[slot[I.aux + firsts[I.aux + j]]] = static_cast<uint32_t>(j);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, const uint32_t *tile_item,
-                                                      const uint32_t *slot,
-                                                      const unsigned long long *tables) {
+                                                      const uint32_t *slot) {
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const ItemRows R(I, blockIdx.x);
-  const uint32_t *id_of_slot = reinterpret_cast<const uint32_t *>(tables + I.aux2);
+  const uint32_t *id_of_slot = static_cast<const uint32_t *>(I.b);
   int64_t *out = static_cast<int64_t *>(I.out);
   for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) out[r] = id_of_slot[slot[I.aux + r]];
 }
@@ -633,48 +641,91 @@ __global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, co
 }  // namespace
 
 void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_t *d_groups) {
-  uint64_t total = 0, words = 0;
-  for (auto &it : items) {
+  constexpr int kBSmallLg = 13;  // first tables: at most 8192 words (64 KB) per item
+  uint64_t total = 0, small_words = 0;
+  std::vector<int> full_lg(items.size());
+  for (size_t i = 0; i < items.size(); ++i) {
+    BatchItem &it = items[i];
     if (it.n > 0x7FFFFFFFull) usage_error("group_id supports at most 2^31 - 1 rows");
-    uint64_t slots = 2;
     int lg = 1;
-    while (slots < 2 * it.n) {
-      slots <<= 1;
-      ++lg;
-    }
+    while ((uint64_t(1) << lg) < 2 * it.n) ++lg;
+    full_lg[i] = lg;
     it.aux = total;
-    it.aux2 = words;
-    it.aux3 = static_cast<uint64_t>(lg);
+    it.aux3 = static_cast<uint64_t>(lg < kBSmallLg ? lg : kBSmallLg);
     total += it.n;
-    words += slots;
+    small_words += uint64_t(1) << it.aux3;
   }
   const uint64_t nitems = items.size();
   fill_u64(c, d_groups, nitems, 0);
   if (total == 0) return;
-  auto *tables = static_cast<unsigned long long *>(tmp_alloc(c, words * 8));
-  fill_u64(c, reinterpret_cast<uint64_t *>(tables), words, 0);
+  auto *small = static_cast<unsigned long long *>(tmp_alloc(c, small_words * 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(small), small_words, 0);
+  auto *over = static_cast<uint32_t *>(tmp_alloc(c, nitems * 4 + 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(over), (nitems * 4 + 8) / 8, 0);
+  uint64_t w = 0;
+  for (size_t i = 0; i < nitems; ++i) {
+    items[i].b = small + w;
+    items[i].aux2 = int(items[i].aux3) < full_lg[i] ? uint64_t(kMaxRun) : uint64_t(kNoLimit);
+    w += uint64_t(1) << items[i].aux3;
+  }
+  {
+    BatchPlan p1 = batch_plan(c, items, kGTile);
+    k_insert_batch<<<p1.ntiles, kBlock, 0, c.stream>>>(type, p1.items, p1.tile_item, over);
+    check_hip(hipGetLastError(), "group_id batch insert");
+    batch_plan_free(c, p1);
+  }
+  // items whose small table overflowed insert again into tables of 2n words
+  std::vector<uint32_t> ov(nitems);
+  check_hip(hipMemcpyAsync(ov.data(), over, nitems * 4, hipMemcpyDeviceToHost, c.stream), "overflow");
+  check_hip(hipStreamSynchronize(c.stream), "sync");
+  std::vector<BatchItem> again;
+  std::vector<size_t> again_idx;
+  uint64_t full_words = 0;
+  for (size_t i = 0; i < nitems; ++i)
+    if (ov[i]) {
+      again_idx.push_back(i);
+      full_words += uint64_t(1) << full_lg[i];
+    }
+  unsigned long long *full = nullptr;
+  if (!again_idx.empty()) {
+    full = static_cast<unsigned long long *>(tmp_alloc(c, full_words * 8));
+    fill_u64(c, reinterpret_cast<uint64_t *>(full), full_words, 0);
+    uint64_t f = 0;
+    for (size_t i : again_idx) {
+      items[i].b = full + f;
+      items[i].aux3 = static_cast<uint64_t>(full_lg[i]);
+      items[i].aux2 = uint64_t(kNoLimit);
+      f += uint64_t(1) << full_lg[i];
+      again.push_back(items[i]);
+    }
+    BatchPlan p2 = batch_plan(c, again, kGTile);
+    k_insert_batch<<<p2.ntiles, kBlock, 0, c.stream>>>(type, p2.items, p2.tile_item, over);
+    check_hip(hipGetLastError(), "group_id batch insert (full tables)");
+    batch_plan_free(c, p2);
+  }
   auto *slot = static_cast<uint32_t *>(tmp_alloc(c, total * 4));
   auto *first = static_cast<uint8_t *>(tmp_alloc(c, total));
   auto *firsts = static_cast<int64_t *>(tmp_alloc(c, total * 8));
   BatchPlan p = batch_plan(c, items, kGTile);
-  k_insert_batch<<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, tables);
-  k_probe_batch<<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, tables, slot, first);
+  k_probe_batch<<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, slot, first);
   check_hip(hipGetLastError(), "group_id batch table");
   // each item's first rows in row order
-  std::vector<BatchItem> w(nitems);
+  std::vector<BatchItem> wl(nitems);
   for (uint64_t i = 0; i < nitems; ++i) {
-    w[i] = BatchItem{};
-    w[i].a = first + items[i].aux;
-    w[i].out = firsts + items[i].aux;
-    w[i].n = items[i].n;
+    wl[i] = BatchItem{};
+    wl[i].a = first + items[i].aux;
+    wl[i].out = firsts + items[i].aux;
+    wl[i].n = items[i].n;
   }
   SelectSpec ss{2, VK_I64, 0, 0, 0};
-  select_batch(c, ss, w, d_groups);
-  k_number_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, firsts, d_groups, slot, tables);
-  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, slot, tables);
+  select_batch(c, ss, wl, d_groups);
+  k_number_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, firsts, d_groups, slot);
+  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, slot);
   check_hip(hipGetLastError(), "group_id batch ids");
   batch_plan_free(c, p);
-  tmp_free(c, tables);
+  tmp_free(c, small);
+  if (full) tmp_free(c, full);
+  tmp_free(c, over);
   tmp_free(c, slot);
   tmp_free(c, first);
   tmp_free(c, firsts);
