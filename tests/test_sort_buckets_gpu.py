@@ -139,3 +139,40 @@ def test_few_distinct_keys(D, pool):
     k = vals[rng.integers(0, pool, n)]
     k[rng.random(n) < 0.01] ^= np.uint64(0x5A5A)
     _check(D, k, "u64")
+
+
+@pytest.mark.parametrize("t", ["u64", "i64", "f64", "i32"])
+def test_batched_buckets(D, t):
+    # batched argsort: items whose keys come from small pools (at least 64 rows per pair of the
+    # two top digits) sort by buckets, the others by LSD passes, in one call; pools of 256 keys
+    # give one-key segments above the local capacity, pools where pairs share several keys give
+    # larger segments (local sorts and range passes)
+    rng = np.random.default_rng(31)
+    dt = {"u64": np.uint64, "i64": np.int64, "f64": np.float64, "i32": np.int32}[t]
+
+    def pool(g):
+        if t == "i32":
+            return rng.integers(-2**31, 2**31, g, dtype=np.int64).astype(np.int32)
+        return rng.integers(0, 2**64, g, dtype=np.uint64).view(dt)
+
+    xs = []
+    specs = [(300000, 256), (5000, 40), (200000, None), (4095, 3), (100000, 64), (1, 1), (250000, "shared"),
+             (70000, 1000), (0, 1), (120000, None)]
+    for n, g in specs:
+        if g is None:
+            xs.append(pool(n) if n else np.zeros(0, dt))
+        elif g == "shared":
+            # pairs of the two top digits each shared by many keys that differ in lower digits
+            base = pool(64)
+            k = base[rng.integers(0, 64, n)]
+            low = rng.integers(0, 1 << 12, n, dtype=np.int64)
+            if t == "i32":
+                k = (k.astype(np.int64) & ~0xFFF | low).astype(np.int32)
+            else:
+                k = (k.view(np.uint64) & ~np.uint64(0xFFFFF) | low.astype(np.uint64)).view(dt)
+            xs.append(k)
+        else:
+            xs.append(pool(g)[rng.integers(0, g, n)] if n else np.zeros(0, dt))
+    got = D.batch_call(f"argsort_{t}", [[D.array(x, t)] for x in xs])
+    for x, p in zip(xs, got):
+        assert np.array_equal(p.to_numpy(), O.argsort(x)), (t, x.size)

@@ -424,24 +424,24 @@ struct LocalDigits {
   int shift[8];
 };
 
-// One block per bucket: rows [start[b], start[b + 1]) of (keys, pos) sorted by bucket and, in
-// each bucket, by row. A bucket of at most THREADS * ITEMS rows is loaded into registers and
-// ordered by its local digits `dg` (ascending) in LDS:
-// - one stable counting pass by the most significant local digit (wave64 match ranks,
-//   per-wave digit counters, rows written to LDS in digit order) splits it into sub-buckets,
-//   each in row order; when none holds more than kSubMax rows, every row's place in its
-//   sub-bucket is the number of rows there with a smaller key, or an equal key and an
-//   earlier place, and its position goes straight to out;
-// - otherwise the block runs a stable pass per local digit, least significant first, from the
-//   rows as loaded.
-// A bucket whose rows all hold one key is in order as it stands. A larger bucket is listed in
-// overflow[1..] (count in overflow[0]) for the batched passes.
+// The rows [s, s + m) of (keys, pos), ordered by their top digits (a bucket or a segment) and,
+// inside, by row, sorted by one block and their positions written to out[0 .. m) as i64:
+// - rows that all hold one key are in order as they stand;
+// - otherwise at most THREADS * ITEMS rows are loaded into registers and ordered by the local
+//   digits `dg` (ascending) in LDS: one stable counting pass by the most significant local
+//   digit (wave64 match ranks, per-wave digit counters, rows written to LDS in digit order)
+//   splits them into sub-buckets, each in row order; when none holds more than kSubMax rows,
+//   every row's place in its sub-bucket is the number of rows there with a smaller key, or an
+//   equal key and an earlier place; else the block runs a stable pass per local digit, least
+//   significant first, from the rows as loaded.
+// Returns false, writing nothing, for more rows than that with more than one key: the caller
+// lists them for the batched passes.
 constexpr uint32_t kSubMax = 96;
 
 template <typename U, int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ keys, const uint32_t *__restrict__ pos,
-                                                        const uint32_t *__restrict__ start, const LocalDigits dg,
-                                                        int64_t *__restrict__ out, uint32_t *__restrict__ overflow) {
+__device__ __forceinline__ bool local_sort_rows(const U *__restrict__ keys, const uint32_t *__restrict__ pos,
+                                                uint64_t s, uint32_t m, const LocalDigits &dg,
+                                                int64_t *__restrict__ out) {
   constexpr int C = THREADS * ITEMS;
   constexpr int W = THREADS / kWave;
   static_assert(THREADS >= kRadix, "one thread per digit in the digit scan");
@@ -451,18 +451,20 @@ __global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ ke
   __shared__ uint32_t s_excl[kRadix + 1];
   __shared__ uint32_t s_wsum[kRadix / kWave];
   __shared__ uint32_t s_max;
-  const uint32_t b = blockIdx.x;
-  const uint32_t s = start[b];
-  const uint32_t m = start[b + 1] - s;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  if (m == 0) return;
+  if (m == 0) return true;
   if (m > static_cast<uint32_t>(C)) {
-    if (t == 0) overflow[1 + atomicAdd(&overflow[0], 1u)] = b;
-    return;
+    // too many rows for LDS: written as they stand if they all hold one key
+    const U k0 = keys[s];
+    bool differ = false;
+    for (uint32_t j = t; j < m && !differ; j += THREADS) differ = keys[s + j] != k0;
+    if (__syncthreads_or(differ)) return false;
+    for (uint32_t j = t; j < m; j += THREADS) out[j] = static_cast<int64_t>(pos[s + j]);
+    return true;
   }
-  if (dg.n == 0) {  // the bucket's rows are in order already
-    for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(pos[s + j]);
-    return;
+  if (dg.n == 0) {  // the rows are in order already
+    for (uint32_t j = t; j < m; j += THREADS) out[j] = static_cast<int64_t>(pos[s + j]);
+    return true;
   }
   // rows are dealt to the waves in equal contiguous runs of ipw * 64
   const uint32_t ipw = (m + W * kWave - 1) / (W * kWave);
@@ -480,10 +482,10 @@ __global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ ke
     p[i] = ok ? pos[s + r] : 0u;
     differ = differ || (ok && key[i] != k0);
   }
-  // a bucket of one key is in order already
+  // rows of one key are in order already
   if (!__syncthreads_or(differ)) {
-    for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(pos[s + j]);
-    return;
+    for (uint32_t j = t; j < m; j += THREADS) out[j] = static_cast<int64_t>(pos[s + j]);
+    return true;
   }
   // One stable pass by the digit at `shift` over the rows in registers: they land in LDS in
   // digit order, s_excl holds the digit starts (s_excl[256] = m) and s_max the largest count.
@@ -593,8 +595,8 @@ __global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ ke
     for (int i = 0; i < ITEMS; ++i)
       if (uint32_t(t) + uint32_t(i) * THREADS < m) s_pos[dst[i]] = pv[i];
     __syncthreads();
-    for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(s_pos[j]);
-    return;
+    for (uint32_t j = t; j < m; j += THREADS) out[j] = static_cast<int64_t>(s_pos[j]);
+    return true;
   }
   // a sub-bucket too large to count: every local digit, from the rows as loaded
   for (int k = 0; k < dg.n; ++k) {
@@ -610,7 +612,43 @@ __global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ ke
       }
     }
   }
-  for (uint32_t j = t; j < m; j += THREADS) out[s + j] = static_cast<int64_t>(s_pos[j]);
+  for (uint32_t j = t; j < m; j += THREADS) out[j] = static_cast<int64_t>(s_pos[j]);
+  return true;
+}
+
+// One block per bucket b: rows [start[b], start[b + 1]); a bucket the block cannot sort is
+// listed in overflow[1..] (count in overflow[0]).
+template <typename U, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ keys, const uint32_t *__restrict__ pos,
+                                                        const uint32_t *__restrict__ start, const LocalDigits dg,
+                                                        int64_t *__restrict__ out, uint32_t *__restrict__ overflow) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t s = start[b];
+  if (!local_sort_rows<U, THREADS, ITEMS>(keys, pos, s, start[b + 1] - s, dg, out + s) && threadIdx.x == 0)
+    overflow[1 + atomicAdd(&overflow[0], 1u)] = b;
+}
+
+// One block per segment j of a batched sort: rows [seg[j], seg[j + 1]) (the last to `total`) of
+// the concatenated rows, inside one item (an item's first row starts a segment); the item is
+// found by binary search of the items' first rows (aux) and the positions go to its output. A
+// segment the block cannot sort is listed in overflow[1..].
+template <typename U, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_local_sort_seg(const U *__restrict__ keys, const uint32_t *__restrict__ pos,
+                                                            const int64_t *__restrict__ seg, uint64_t nseg,
+                                                            uint64_t total, const BatchItem *items, uint32_t nitems,
+                                                            const LocalDigits dg, uint32_t *__restrict__ overflow) {
+  const uint64_t j = blockIdx.x;
+  const uint64_t s = static_cast<uint64_t>(seg[j]);
+  const uint64_t e = j + 1 < nseg ? static_cast<uint64_t>(seg[j + 1]) : total;
+  uint32_t lo = 0, hi = nitems;  // the last item whose first row is at most s
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (items[mid].aux <= s) lo = mid;
+    else hi = mid;
+  }
+  int64_t *out = static_cast<int64_t *>(items[lo].out) + (s - items[lo].aux);
+  if (!local_sort_rows<U, THREADS, ITEMS>(keys, pos, s, static_cast<uint32_t>(e - s), dg, out) && threadIdx.x == 0)
+    overflow[1 + atomicAdd(&overflow[0], 1u)] = static_cast<uint32_t>(j);
 }
 
 template <typename U, typename V, bool FIRST, bool LAST>
@@ -807,6 +845,79 @@ void radix_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, U *sorted_
   batch_plan_free(c, p);
 }
 
+// ------------------------------------------------ batched sorts by top-digit buckets
+
+constexpr uint32_t kPairWords = kBuckets / 32;  // one bit per (top digit, second digit) pair
+constexpr uint32_t kDenseTile = 4096;           // a tile with more pairs marks its item dense
+
+// Per item, the distinct pairs of the digits at sh_hi and sh_lo: a tile sets the bits of its
+// rows' pairs in an LDS bitmap, then, unless it holds more than kDenseTile pairs (which marks
+// the item dense), ORs its nonzero words into the item's bitmap (kPairWords words).
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_pairs_batch(int32_t type, const BatchItem *items, const uint32_t *tile_item,
+                                                        int sh_hi, int sh_lo, uint32_t *bitmaps, uint32_t *dense) {
+  __shared__ uint32_t s_bm[kPairWords];
+  __shared__ uint32_t s_cnt;
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
+  const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
+  for (uint32_t w = threadIdx.x; w < kPairWords; w += kBlock) s_bm[w] = 0;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  with_image<U>(type, I.a, [&](auto img) {
+    for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) {
+      const uint32_t b = bucket16<U>(img(r), sh_hi, sh_lo);
+      atomicOr(&s_bm[b >> 5], 1u << (b & 31));
+    }
+  });
+  __syncthreads();
+  uint32_t cnt = 0;
+  for (uint32_t w = threadIdx.x; w < kPairWords; w += kBlock) cnt += __popc(s_bm[w]);
+  atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (s_cnt > kDenseTile) {
+    if (threadIdx.x == 0) atomicOr(&dense[it], 1u);
+    return;
+  }
+  uint32_t *bm = bitmaps + uint64_t(it) * kPairWords;
+  for (uint32_t w = threadIdx.x; w < kPairWords; w += kBlock)
+    if (s_bm[w]) atomicOr(&bm[w], s_bm[w]);
+}
+
+// count[item] = the item's distinct pairs, or ~0 when it is dense.
+__global__ __launch_bounds__(kBlock) void k_pairs_count(const uint32_t *bitmaps, const uint32_t *dense, uint32_t *count) {
+  __shared__ uint32_t s_cnt;
+  const uint32_t it = blockIdx.x;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  uint32_t cnt = 0;
+  for (uint32_t w = threadIdx.x; w < kPairWords; w += kBlock) cnt += __popc(bitmaps[uint64_t(it) * kPairWords + w]);
+  atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) count[it] = dense[it] ? ~0u : s_cnt;
+}
+
+// dst row i = src row idx[i] (rows of `width` words): the digit bases of a subset of items.
+__global__ void k_gather_rows(const uint32_t *src, const uint32_t *idx, uint64_t rows, uint32_t width, uint32_t *dst) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < rows * width;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[uint64_t(idx[i / width]) * width + i % width];
+}
+
+// flags[aux + r] = 1 where row r of an item starts a segment: the item's first row, or a row
+// whose pair of the digits at sh_hi and sh_lo differs from the row before (keys sorted by them).
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_seg_flags_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                            const U *keys, int sh_hi, int sh_lo, uint8_t *flags) {
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
+  const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
+  const U *k = keys + I.aux;
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock)
+    flags[I.aux + r] = r == 0 || bucket16<U>(k[r], sh_hi, sh_lo) != bucket16<U>(k[r - 1], sh_hi, sh_lo);
+}
+
 // The local sort's capacity in rows for a bucket-size estimate: 3072 (256 threads x 12 rows,
 // four blocks per compute unit) or 8192 (512 threads x 16 rows, one block), 0 when buckets
 // would not fit.
@@ -816,15 +927,63 @@ inline int local_capacity(double est_max) {
   return 0;
 }
 
-// Argsort by top-digit buckets (see k_local_sort). `digits` are the active digits, ascending,
+// Row ranges of sorted (image, position) buffers kb / pb left by the local sorts (each range in
+// row order, by the digits above `lower` equal): sorted by the batched passes over the digits of
+// `lower` (ascending) that vary in some range, an identity pass where one does not, the last
+// writing positions to each range's output (items: a = kb + first row, out, n, aux = first row);
+// with none varying, the positions are copied as they stand. ka / pa are free buffers of the
+// same rows.
+template <typename U>
+void finish_ranges(Ctx &c, std::vector<BatchItem> &items, const std::vector<int> &lower, U *ka, uint32_t *pa, U *kb,
+                   uint32_t *pb) {
+  using V = uint32_t;
+  constexpr int P = sizeof(U);
+  const int32_t img = sizeof(U) == 8 ? int32_t(VK_U64) : kRawU32;  // the rows are images already
+  BatchPlan bp = batch_plan(c, items, kTile);
+  const uint64_t hwords = items.size() * P * kRadix;
+  uint32_t *bh = static_cast<uint32_t *>(tmp_alloc(c, hwords * 4 + 64));
+  uint32_t *active = bh + hwords;
+  fill_u64(c, reinterpret_cast<uint64_t *>(bh), (hwords * 4 + 64) / 8, 0);
+  k_hist_batch<U><<<bp.ntiles, kBlock, 0, c.stream>>>(img, bp.items, bp.tile_item, bh);
+  check_hip(hipGetLastError(), "range histogram");
+  k_base_batch<<<stream_grid(items.size() * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(bp.items, items.size(), P,
+                                                                                       bh, active);
+  check_hip(hipGetLastError(), "range bases");
+  uint32_t act[8] = {0};
+  read_back(c, active, P * 4, act);
+  std::vector<int> vary;
+  for (int d : lower)
+    if (act[d]) vary.push_back(d);
+  const U *kin = kb;
+  const V *pin = pb;
+  if (vary.empty()) {
+    k_pos_batch<<<bp.ntiles, kBlock, 0, c.stream>>>(bp.items, bp.tile_item, pb);
+    check_hip(hipGetLastError(), "range positions");
+  }
+  for (size_t k = 0; k < vary.size(); ++k) {
+    const int d = vary[k];
+    if (k + 1 == vary.size()) {
+      run_pass_batch<U, V, false, true>(c, img, bp, P, d, kin, pin, bh, nullptr, nullptr);
+    } else {
+      U *ko = (k % 2 == 0) ? ka : kb;
+      V *po = (k % 2 == 0) ? pa : pb;
+      run_pass_batch<U, V, false, false>(c, img, bp, P, d, kin, pin, bh, ko, po);
+      kin = ko;
+      pin = po;
+    }
+  }
+  tmp_free(c, bh);
+  batch_plan_free(c, bp);
+}
+
+// Argsort by top-digit buckets (see local_sort_rows). `digits` are the active digits, ascending,
 // at least four; hist holds their exclusive digit bases. The two top digits go through HBM
-// passes, the others through the local sorts; buckets larger than the local capacity are
-// sorted by those digits with the batched passes over their row ranges.
+// passes, the others through the local sorts; buckets the local sorts cannot take are finished
+// by the batched passes over their row ranges.
 template <typename U>
 void argsort_buckets(Ctx &c, int32_t type, const void *keys, uint64_t n, const std::vector<int> &digits,
                      const uint32_t *hist, int cap, int64_t *out_pos) {
   using V = uint32_t;
-  constexpr int P = sizeof(U);
   const int np = static_cast<int>(digits.size());
   const int dlo = digits[np - 2], dhi = digits[np - 1];
   U *ka = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
@@ -853,61 +1012,223 @@ void argsort_buckets(Ctx &c, int32_t type, const void *keys, uint64_t n, const s
     check_hip(hipMemcpyAsync(st.data(), start, (kBuckets + 1) * 4, hipMemcpyDeviceToHost, c.stream), "starts");
     check_hip(hipMemcpyAsync(ids.data(), overflow + 1, nover * 4, hipMemcpyDeviceToHost, c.stream), "overflow");
     check_hip(hipStreamSynchronize(c.stream), "sync");
-    std::vector<BatchItem> items;
+    std::vector<BatchItem> ranges;
     for (uint32_t id : ids) {
       BatchItem it{};
       it.a = kb + st[id];
       it.out = out_pos + st[id];
       it.n = st[id + 1] - st[id];
       it.aux = st[id];
-      items.push_back(it);
+      ranges.push_back(it);
     }
-    // every item's rows are images already: the identity image of U
-    const int32_t img = sizeof(U) == 8 ? int32_t(VK_U64) : kRawU32;
-    BatchPlan bp = batch_plan(c, items, kTile);
-    const uint64_t hwords = items.size() * P * kRadix;
-    uint32_t *bh = static_cast<uint32_t *>(tmp_alloc(c, hwords * 4 + 64));
-    uint32_t *active = bh + hwords;
-    fill_u64(c, reinterpret_cast<uint64_t *>(bh), (hwords * 4 + 64) / 8, 0);
-    k_hist_batch<U><<<bp.ntiles, kBlock, 0, c.stream>>>(img, bp.items, bp.tile_item, bh);
-    check_hip(hipGetLastError(), "overflow histogram");
-    k_base_batch<<<stream_grid(items.size() * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(bp.items, items.size(),
-                                                                                         P, bh, active);
-    check_hip(hipGetLastError(), "overflow bases");
-    uint32_t act[8] = {0};
-    read_back(c, active, P * 4, act);
-    // the buckets' rows are in row order; a stable pass over each lower digit that varies in some
-    // bucket (an identity pass where it does not) finishes them, the last writing positions; with
-    // none, the positions are copied as they are (few keys, each repeated: one key per bucket)
-    std::vector<int> lower;
-    for (int k = 0; k < np - 2; ++k)
-      if (act[digits[k]]) lower.push_back(digits[k]);
-    const U *kin = kb;
-    const V *pin = pb;
-    if (lower.empty()) {
-      k_pos_batch<<<bp.ntiles, kBlock, 0, c.stream>>>(bp.items, bp.tile_item, pb);
-      check_hip(hipGetLastError(), "overflow positions");
-    }
-    for (size_t k = 0; k < lower.size(); ++k) {
-      const int d = lower[k];
-      if (k + 1 == lower.size()) {
-        run_pass_batch<U, V, false, true>(c, img, bp, P, d, kin, pin, bh, nullptr, nullptr);
-      } else {
-        U *ko = (k % 2 == 0) ? ka : kb;
-        V *po = (k % 2 == 0) ? pa : pb;
-        run_pass_batch<U, V, false, false>(c, img, bp, P, d, kin, pin, bh, ko, po);
-        kin = ko;
-        pin = po;
-      }
-    }
-    tmp_free(c, bh);
-    batch_plan_free(c, bp);
+    finish_ranges<U>(c, ranges, std::vector<int>(digits.begin(), digits.end() - 2), ka, pa, kb, pb);
   }
   tmp_free(c, start);
   tmp_free(c, ka);
   tmp_free(c, pa);
   tmp_free(c, kb);
   tmp_free(c, pb);
+}
+
+// Batched argsort (positions): one histogram of every item gives the digit bases and the active
+// digits. With four or more active digits, the distinct pairs of the two top ones are counted
+// per item (k_pairs_batch); items of at least kBucketItemRows rows with at least 64 rows per
+// pair sort by buckets: two batched passes by the two top digits, the segments of one pair found
+// by flags and a selection, one block per segment (local_sort_rows), segments it cannot take
+// finished by finish_ranges. The other items take one batched pass per active digit.
+constexpr uint64_t kBucketItemRows = 4096;
+
+template <typename U>
+void argsort_batch_pos(Ctx &c, int32_t type, std::vector<BatchItem> &items) {
+  using V = uint32_t;
+  constexpr int P = sizeof(U);
+  constexpr uint32_t kRow = P * kRadix;  // histogram words per item
+  uint64_t total = 0;
+  for (auto &it : items) {
+    if (it.n > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
+    it.aux = total;
+    total += it.n;
+  }
+  if (total == 0) return;
+  const uint64_t nitems = items.size();
+  BatchPlan p = batch_plan(c, items, kTile);
+  const uint64_t hwords = nitems * kRow;
+  uint32_t *hist = static_cast<uint32_t *>(tmp_alloc(c, hwords * 4 + 64));
+  uint32_t *active = hist + hwords;
+  fill_u64(c, reinterpret_cast<uint64_t *>(hist), (hwords * 4 + 64) / 8, 0);
+  k_hist_batch<U><<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, hist);
+  check_hip(hipGetLastError(), "radix batch histogram");
+  k_base_batch<<<stream_grid(nitems * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(p.items, nitems, P, hist, active);
+  check_hip(hipGetLastError(), "radix batch bases");
+  uint32_t act[8] = {0};
+  read_back(c, active, P * 4, act);
+  std::vector<int> digits;
+  for (int d = 0; d < P; ++d)
+    if (act[d]) digits.push_back(d);
+  const int np = static_cast<int>(digits.size());
+  if (np == 0) {
+    k_iota_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item);
+    check_hip(hipGetLastError(), "radix batch iota");
+    tmp_free(c, hist);
+    batch_plan_free(c, p);
+    return;
+  }
+  // which items sort by buckets
+  std::vector<uint32_t> pairs(nitems, ~0u);
+  if (np >= 4) {
+    const int dlo = digits[np - 2], dhi = digits[np - 1];
+    uint32_t *bm = static_cast<uint32_t *>(tmp_alloc(c, (nitems * kPairWords + 2 * nitems) * 4 + 8));
+    uint32_t *dense = bm + nitems * kPairWords;
+    uint32_t *cnt = dense + nitems;
+    fill_u64(c, reinterpret_cast<uint64_t *>(bm), ((nitems * kPairWords + 2 * nitems) * 4 + 8) / 8, 0);
+    k_pairs_batch<U><<<p.ntiles, kBlock, 0, c.stream>>>(type, p.items, p.tile_item, 8 * dhi, 8 * dlo, bm, dense);
+    k_pairs_count<<<uint32_t(nitems), kBlock, 0, c.stream>>>(bm, dense, cnt);
+    check_hip(hipGetLastError(), "pair counts");
+    check_hip(hipMemcpyAsync(pairs.data(), cnt, nitems * 4, hipMemcpyDeviceToHost, c.stream), "pair counts");
+    check_hip(hipStreamSynchronize(c.stream), "sync");
+    tmp_free(c, bm);
+  }
+  std::vector<uint32_t> bidx, lidx;
+  for (uint32_t i = 0; i < nitems; ++i) {
+    const BatchItem &it = items[i];
+    const bool buckets = it.n >= kBucketItemRows && pairs[i] != ~0u && it.n >= 64 * uint64_t(pairs[i]);
+    (buckets ? bidx : lidx).push_back(i);
+  }
+  // a subset of the items: its own rows (aux), plan and gathered digit bases
+  struct Sub {
+    std::vector<BatchItem> items;
+    uint64_t rows = 0;
+    BatchPlan plan;
+    uint32_t *hist = nullptr;
+  };
+  auto subset = [&](const std::vector<uint32_t> &idx) {
+    Sub s;
+    for (uint32_t i : idx) {
+      BatchItem it = items[i];
+      it.aux = s.rows;
+      s.rows += it.n;
+      s.items.push_back(it);
+    }
+    s.plan = batch_plan(c, s.items, kTile);
+    s.hist = static_cast<uint32_t *>(tmp_alloc(c, idx.size() * kRow * 4 + 8));
+    uint32_t *d_idx = static_cast<uint32_t *>(tmp_alloc(c, idx.size() * 4 + 8));
+    check_hip(hipMemcpyAsync(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, c.stream), "subset");
+    k_gather_rows<<<stream_grid(idx.size() * kRow, kBlock, 8192), kBlock, 0, c.stream>>>(hist, d_idx, idx.size(), kRow,
+                                                                                      s.hist);
+    check_hip(hipGetLastError(), "subset bases");
+    check_hip(hipStreamSynchronize(c.stream), "subset");
+    tmp_free(c, d_idx);
+    return s;
+  };
+  // one batched pass per active digit, the last writing positions
+  auto lsd = [&](const BatchPlan &bp, const uint32_t *bh, uint64_t rows) {
+    U *ka = nullptr, *kb = nullptr;
+    V *pa = nullptr, *pb = nullptr;
+    if (np > 1) {
+      ka = static_cast<U *>(tmp_alloc(c, rows * sizeof(U)));
+      pa = static_cast<V *>(tmp_alloc(c, rows * sizeof(V)));
+      if (np > 2) {
+        kb = static_cast<U *>(tmp_alloc(c, rows * sizeof(U)));
+        pb = static_cast<V *>(tmp_alloc(c, rows * sizeof(V)));
+      }
+    }
+    const U *kin = nullptr;
+    const V *pin = nullptr;
+    for (int k = 0; k < np; ++k) {
+      const int d = digits[k];
+      const bool first = k == 0, last = k == np - 1;
+      U *ko = last ? nullptr : (k % 2 == 0) ? ka : kb;
+      V *po = last ? nullptr : (k % 2 == 0) ? pa : pb;
+      if (first && last) run_pass_batch<U, V, true, true>(c, type, bp, P, d, nullptr, nullptr, bh, ko, po);
+      else if (first) run_pass_batch<U, V, true, false>(c, type, bp, P, d, nullptr, nullptr, bh, ko, po);
+      else if (last) run_pass_batch<U, V, false, true>(c, type, bp, P, d, kin, pin, bh, ko, po);
+      else run_pass_batch<U, V, false, false>(c, type, bp, P, d, kin, pin, bh, ko, po);
+      kin = ko;
+      pin = po;
+    }
+    if (ka) tmp_free(c, ka);
+    if (pa) tmp_free(c, pa);
+    if (kb) tmp_free(c, kb);
+    if (pb) tmp_free(c, pb);
+  };
+  if (bidx.empty()) {
+    lsd(p, hist, total);
+  } else {
+    if (!lidx.empty()) {
+      Sub L = subset(lidx);
+      lsd(L.plan, L.hist, L.rows);
+      tmp_free(c, L.hist);
+      batch_plan_free(c, L.plan);
+    }
+    Sub B = subset(bidx);
+    const int dlo = digits[np - 2], dhi = digits[np - 1];
+    const uint64_t rows = B.rows;
+    U *ka = static_cast<U *>(tmp_alloc(c, rows * sizeof(U)));
+    V *pa = static_cast<V *>(tmp_alloc(c, rows * sizeof(V)));
+    U *kb = static_cast<U *>(tmp_alloc(c, rows * sizeof(U)));
+    V *pb = static_cast<V *>(tmp_alloc(c, rows * sizeof(V)));
+    run_pass_batch<U, V, true, false>(c, type, B.plan, P, dlo, nullptr, nullptr, B.hist, ka, pa);
+    run_pass_batch<U, V, false, false>(c, type, B.plan, P, dhi, ka, pa, B.hist, kb, pb);
+    // segments: an item's first row, and rows whose pair of the two top digits changes
+    uint8_t *flags = static_cast<uint8_t *>(tmp_alloc(c, rows + 8));
+    int64_t *seg = static_cast<int64_t *>(tmp_alloc(c, rows * 8 + 8));
+    k_seg_flags_batch<U><<<B.plan.ntiles, kBlock, 0, c.stream>>>(B.plan.items, B.plan.tile_item, kb, 8 * dhi, 8 * dlo,
+                                                                  flags);
+    check_hip(hipGetLastError(), "segment flags");
+    SelectSpec ss{2, VK_I64, 0, 0, 0};
+    uint64_t *d_cnt = c.scratch.slots + 21;
+    select_rows(c, ss, nullptr, flags, rows, seg, d_cnt);
+    uint64_t nseg = 0;
+    read_back(c, d_cnt, 8, &nseg);
+    uint32_t *overflow = static_cast<uint32_t *>(tmp_alloc(c, nseg * 4 + 8));
+    fill_u64(c, reinterpret_cast<uint64_t *>(overflow), 1, 0);
+    LocalDigits dg{};
+    dg.n = np - 2;
+    for (int k = 0; k < np - 2; ++k) dg.shift[k] = 8 * digits[k];
+    if (nseg > 0x7FFFFFFFull) usage_error("batched argsort has too many segments");
+    k_local_sort_seg<U, 256, 12><<<uint32_t(nseg), 256, 0, c.stream>>>(kb, pb, seg, nseg, rows, B.plan.items,
+                                                                     uint32_t(B.items.size()), dg, overflow);
+    check_hip(hipGetLastError(), "segment sort");
+    uint32_t nover = 0;
+    read_back(c, overflow, 4, &nover);
+    if (nover) {
+      std::vector<uint32_t> ids(nover);
+      std::vector<int64_t> starts(nseg);
+      check_hip(hipMemcpyAsync(ids.data(), overflow + 1, nover * 4, hipMemcpyDeviceToHost, c.stream), "overflow");
+      check_hip(hipMemcpyAsync(starts.data(), seg, nseg * 8, hipMemcpyDeviceToHost, c.stream), "segments");
+      check_hip(hipStreamSynchronize(c.stream), "sync");
+      std::vector<BatchItem> ranges;
+      for (uint32_t j : ids) {
+        const uint64_t s = static_cast<uint64_t>(starts[j]);
+        const uint64_t e = j + 1 < nseg ? static_cast<uint64_t>(starts[j + 1]) : rows;
+        // the item holding the segment: the last whose first row is at most s
+        size_t lo = 0, hi = B.items.size();
+        while (hi - lo > 1) {
+          const size_t mid = (lo + hi) / 2;
+          if (B.items[mid].aux <= s) lo = mid;
+          else hi = mid;
+        }
+        BatchItem it{};
+        it.a = kb + s;
+        it.out = static_cast<int64_t *>(B.items[lo].out) + (s - B.items[lo].aux);
+        it.n = e - s;
+        it.aux = s;
+        ranges.push_back(it);
+      }
+      finish_ranges<U>(c, ranges, std::vector<int>(digits.begin(), digits.end() - 2), ka, pa, kb, pb);
+    }
+    tmp_free(c, overflow);
+    tmp_free(c, flags);
+    tmp_free(c, seg);
+    tmp_free(c, ka);
+    tmp_free(c, pa);
+    tmp_free(c, kb);
+    tmp_free(c, pb);
+    tmp_free(c, B.hist);
+    batch_plan_free(c, B.plan);
+  }
+  tmp_free(c, hist);
+  batch_plan_free(c, p);
 }
 
 // Sorts rows by image. Positions mode (V = u32): positions of the sorted rows to out_pos,
@@ -1035,8 +1356,8 @@ void argsort(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out) {
 
 void argsort_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items) {
   switch (type) {
-    case VK_U64: case VK_I64: case VK_F64: radix_batch<uint64_t, uint32_t>(c, type, items); break;
-    case VK_I32: radix_batch<uint32_t, uint32_t>(c, type, items); break;
+    case VK_U64: case VK_I64: case VK_F64: argsort_batch_pos<uint64_t>(c, type, items); break;
+    case VK_I32: argsort_batch_pos<uint32_t>(c, type, items); break;
     case VK_U8: radix_batch<uint8_t, uint32_t>(c, type, items); break;
     default: usage_error("unsupported argsort type");
   }
