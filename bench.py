@@ -735,7 +735,7 @@ def main():
                     help="comma-separated primitive labels to run (for profiling); 'copy_f64' adds a "
                          "plain copy (add_f64 x + 0.0) used to calibrate HBM counters; 'zero_mean' "
                          "alone runs only the zero-mean canonical and f64 scan leg, 'sort' alone only the "
-                         "sort and grouping suite")
+                         "sort and grouping suite, 'batched' alone only the many-arrays comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this backend has no multi-GPU path (asked for {args.gpus})",
@@ -751,6 +751,9 @@ def main():
 
     if args.only == "sort":
         print(json.dumps({"sort": sort_suite(D, verify=not args.no_verify)}))
+        return
+    if args.only == "batched":
+        print(json.dumps({"batched": batched_suite(D, args.batch_arrays)}))
         return
     x1 = D.call("rand_f64", [S("i64", n), S("i64", 1)])
     if args.only == "zero_mean":
@@ -841,6 +844,13 @@ def main():
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
     if batched is not None and not args.no_literal:
         batched["literal_range"] = batched_literal_suite(D, args.batch_arrays)
+    if not (args.no_serve or args.only):
+        # the serving loop's workers allocate from pools of their own: give this context's
+        # cached blocks (the sort and batched legs' buffers) back to the device first
+        import gc
+        gc.collect()
+        ctx.sync()
+        ctx.trim()
     serve = None if (args.no_serve or args.only) else serve_suite(args.serve_requests, args.serve_qps)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     verified = (checks is not None and all(checks.values()) and (sort is None or sort["verified"])
