@@ -64,9 +64,10 @@ __device__ __forceinline__ void with_image(int32_t type, const void *keys, F &&f
 
 // ------------------------------------------------------------- counting pass
 
-// Adds one row's digits to the block's LDS histograms. A digit that every active lane of the
-// wave shares (the constant high digits of small keys, runs of equal keys) is added once, by
-// the first active lane, instead of by 64 atomics on one LDS word.
+// Adds one row's digits to the block's LDS histograms. The lanes that share the first active
+// lane's digit (all of them for the constant high digits of small keys and runs of equal keys,
+// many for skewed keys) are added once, by that lane, instead of by one atomic each on one LDS
+// word; the other lanes add themselves.
 template <typename U>
 __device__ __forceinline__ void hist_add(uint32_t (&s_h)[sizeof(U)][kRadix], U k) {
   const uint64_t act = __ballot(1);
@@ -75,11 +76,9 @@ __device__ __forceinline__ void hist_add(uint32_t (&s_h)[sizeof(U)][kRadix], U k
   for (int d = 0; d < static_cast<int>(sizeof(U)); ++d) {
     const uint32_t dig = static_cast<uint32_t>(k >> (8 * d)) & 255;
     const uint32_t first = __builtin_amdgcn_readfirstlane(dig);
-    if (__ballot(dig == first) == act) {
-      if (static_cast<int>(lane_id()) == lead) atomicAdd(&s_h[d][first], static_cast<uint32_t>(__popcll(act)));
-    } else {
-      atomicAdd(&s_h[d][dig], 1u);
-    }
+    const uint64_t same = __ballot(dig == first);
+    if (static_cast<int>(lane_id()) == lead) atomicAdd(&s_h[d][first], static_cast<uint32_t>(__popcll(same)));
+    if (dig != first) atomicAdd(&s_h[d][dig], 1u);
   }
 }
 
