@@ -335,6 +335,147 @@ __global__ __launch_bounds__(kWave) void k_chain_fold_lanes(const uint64_t *__re
   }
 }
 
+// Class folds. When the sort leaves the chain keys' top non-constant digit unsorted, the rows
+// of one class (the key bits below `shift`) are contiguous and in row order, and the chains of
+// a class differ only in the digit at `shift` (the bits above it are equal in every row).
+// k_class_bounds finds every class's first row by binary search of the masked sorted keys
+// (starts[nclasses] = b). k_class_fold takes one class per block and streams its rows in
+// chunks of THREADS * kClassItems: a chunk is partitioned stably by the digit in LDS (each wave
+// ranks its 64 rows per step among equal digits with one ballot per digit bit, per-wave
+// counters give row order across the wave's steps and across waves), then thread d adds the
+// chunk's rows of digit d to its chain's running value, in row order, and keeps the value for
+// the next chunk. At the end each thread that saw rows writes its chain's stripe. Every
+// digit is below 2^nbits.
+constexpr int kClassItems = 8;
+constexpr int kClassRadix = 256;
+
+__global__ void k_class_bounds(const uint32_t *__restrict__ sk, uint64_t b, uint64_t nclasses, uint32_t mask,
+                               uint64_t *starts) {
+  for (uint64_t c = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; c <= nclasses;
+       c += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t lo = 0, hi = b;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if ((sk[mid] & mask) < c) lo = mid + 1;
+      else hi = mid;
+    }
+    starts[c] = lo;
+  }
+}
+
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_class_fold(const uint64_t *__restrict__ starts, int shift, int nbits,
+                                                        const uint32_t *__restrict__ sk,
+                                                        const double *__restrict__ sv, double *stripes) {
+  constexpr int W = THREADS / kWave;
+  constexpr int C = THREADS * kClassItems;
+  constexpr int kWaveRows = kClassItems * kWave;
+  static_assert(THREADS >= kClassRadix, "one thread per digit");
+  __shared__ double s_v[C];
+  __shared__ uint32_t s_wcnt[W][kClassRadix];
+  __shared__ uint32_t s_excl[kClassRadix + 1];
+  __shared__ uint32_t s_wsum[kClassRadix / kWave];
+  __shared__ uint32_t s_ck[kClassRadix];
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  const uint64_t p0 = starts[blockIdx.x], p1 = starts[blockIdx.x + 1];
+  if (p0 == p1) return;
+  const uint64_t below = (uint64_t(1) << l) - 1;
+  uint32_t key[kClassItems];
+  double val[kClassItems];
+  auto load = [&](uint64_t q) {
+#pragma unroll
+    for (int i = 0; i < kClassItems; ++i) {
+      const uint64_t r = q + uint64_t(w) * kWaveRows + uint64_t(i) * kWave + l;
+      const bool ok = r < p1;
+      key[i] = ok ? sk[r] : 0u;
+      val[i] = ok ? sv[r] : 0.0;
+    }
+  };
+  double acc = 0.0;
+  bool seen = false;
+  uint32_t chain = 0;
+  load(p0);
+  for (uint64_t q = p0; q < p1; q += C) {
+    const uint32_t m = p1 - q < uint64_t(C) ? static_cast<uint32_t>(p1 - q) : uint32_t(C);
+    for (int z = t; z < W * kClassRadix; z += THREADS) (&s_wcnt[0][0])[z] = 0;
+    __syncthreads();
+    uint32_t rnk[kClassItems];
+#pragma unroll
+    for (int i = 0; i < kClassItems; ++i) {
+      const uint32_t r = uint32_t(w) * kWaveRows + uint32_t(i) * kWave + l;
+      const bool valid = r < m;
+      const uint32_t d = (key[i] >> shift) & (kClassRadix - 1);
+      uint64_t peers = __ballot(valid);
+      for (int bit = 0; bit < nbits; ++bit) {
+        const uint64_t bal = __ballot((d >> bit) & 1);
+        peers &= ((d >> bit) & 1) ? bal : ~bal;
+      }
+      rnk[i] = 0;
+      if (valid) {
+        rnk[i] = s_wcnt[w][d] + __popcll(peers & below);
+        if (l == __builtin_ctzll(peers)) {
+          s_wcnt[w][d] += __popcll(peers);
+          s_ck[d] = key[i];
+        }
+      }
+    }
+    __syncthreads();
+    uint32_t cnt = 0, inc = 0;
+    if (t < kClassRadix) {
+#pragma unroll
+      for (int z = 0; z < W; ++z) {
+        const uint32_t v = s_wcnt[z][t];
+        s_wcnt[z][t] = cnt;
+        cnt += v;
+      }
+      inc = cnt;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o);
+        if (l >= o) inc += u;
+      }
+      if (l == kWave - 1) s_wsum[w] = inc;
+    }
+    __syncthreads();
+    if (t < kClassRadix) {
+      uint32_t wpre = 0;
+      for (int z = 0; z < w; ++z) wpre += s_wsum[z];
+      s_excl[t] = wpre + inc - cnt;
+      if (t == 0) s_excl[kClassRadix] = m;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kClassItems; ++i) {
+      const uint32_t r = uint32_t(w) * kWaveRows + uint32_t(i) * kWave + l;
+      if (r < m) {
+        const uint32_t d = (key[i] >> shift) & (kClassRadix - 1);
+        s_v[s_excl[d] + s_wcnt[w][d] + rnk[i]] = val[i];
+      }
+    }
+    // the next chunk's loads go out before this chunk is folded
+    if (q + C < p1) load(q + C);
+    __syncthreads();
+    if (t < kClassRadix) {
+      const uint32_t lo = s_excl[t], hi = s_excl[t + 1];
+      if (hi > lo) {
+        seen = true;
+        chain = s_ck[t];
+      }
+      uint32_t j = lo;
+      for (; j + 4 <= hi; j += 4) {
+        double u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) u[k] = s_v[j + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc = acc + u[k];
+      }
+      for (; j < hi; ++j) acc = acc + s_v[j];
+    }
+    __syncthreads();
+  }
+  if (t < kClassRadix && seen) stripes[chain] = acc;
+}
+
 __global__ void k_combine_stripes(const double *__restrict__ stripes, uint64_t buckets, double *out) {
   for (uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; k < buckets;
        k += uint64_t(gridDim.x) * blockDim.x) {
@@ -426,8 +567,14 @@ namespace {
 // kBucketFoldMin buckets on, the buckets alone give the parallelism: one lane per bucket
 // folds straight into out (k_bucket_fold, no stripe buffer of 16 x buckets values).
 // Returns true when out holds the combined buckets.
+// From kClassFoldMin to kClassFoldMax buckets (chain keys of two or three bytes) the sort
+// leaves the top digit out and the classes fold (k_class_fold) into the stripes, with a
+// stripe buffer of its own from kBucketFoldMin buckets on; the forms above serve the rest.
 constexpr uint64_t kLaneFoldChains = 4096;
 constexpr uint64_t kBucketFoldMin = uint64_t(1) << 16;
+constexpr uint64_t kClassFoldMin = 17;
+constexpr uint64_t kClassFoldMax = uint64_t(1) << 20;
+constexpr uint64_t kClassWideRows = 16384;  // rows per class from which a class takes 1024 threads
 
 template <typename K>
 bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, uint64_t buckets,
@@ -435,14 +582,46 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
   constexpr bool wide = sizeof(K) == 8;
   K *sk = static_cast<K *>(tmp_alloc(c, b * sizeof(K)));
   double *sv = static_cast<double *>(tmp_alloc(c, b * 8));
+  const bool class_form = !wide && buckets >= kClassFoldMin && buckets <= kClassFoldMax;
+  int shift = 0;
   // the sort computes each row's chain key from its bucket key as it reads it, and checks the
   // bucket keys in its counting pass
   sort_chain_values(c, wide, keys, reinterpret_cast<const uint64_t *>(vals), b, sk,
-                    reinterpret_cast<uint64_t *>(sv), buckets, bad_row);
+                    reinterpret_cast<uint64_t *>(sv), buckets, bad_row, class_form ? &shift : nullptr);
   if (*bad_row != ~uint64_t(0)) {
     tmp_free(c, sk);
     tmp_free(c, sv);
     return false;
+  }
+  if constexpr (!wide) {
+    if (class_form) {
+      const uint64_t nclasses = uint64_t(1) << shift;  // shift is 8 or 16 below 2^20 buckets
+      const uint64_t nchains = buckets * 16;
+      uint64_t top = (nchains - 1) >> shift;
+      if (top > kClassRadix - 1) top = kClassRadix - 1;
+      int nbits = 0;
+      while ((uint64_t(1) << nbits) <= top) ++nbits;
+      uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, (nclasses + 1) * 8));
+      k_class_bounds<<<stream_grid(nclasses + 1, kBlock, 8192), kBlock, 0, c.stream>>>(sk, b, nclasses,
+                                                                                      uint32_t(nclasses - 1), starts);
+      double *st = stripes;
+      if (!st) {
+        st = static_cast<double *>(tmp_alloc(c, nchains * 8));
+        k_fill<double><<<stream_grid(nchains, kBlock, 8192), kBlock, 0, c.stream>>>(st, nchains, 0.0);
+      }
+      if ((b >> shift) >= kClassWideRows)
+        k_class_fold<1024><<<uint32_t(nclasses), 1024, 0, c.stream>>>(starts, shift, nbits, sk, sv, st);
+      else
+        k_class_fold<256><<<uint32_t(nclasses), 256, 0, c.stream>>>(starts, shift, nbits, sk, sv, st);
+      if (!stripes)
+        k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(st, buckets, out);
+      check_hip(hipGetLastError(), "class fold");
+      if (!stripes) tmp_free(c, st);
+      tmp_free(c, starts);
+      tmp_free(c, sk);
+      tmp_free(c, sv);
+      return !stripes;
+    }
   }
   const bool by_bucket = buckets >= kBucketFoldMin;
   if (by_bucket) {

@@ -1233,9 +1233,13 @@ void argsort_batch_pos(Ctx &c, int32_t type, std::vector<BatchItem> &items) {
 // Sorts rows by image. Positions mode (V = u32): positions of the sorted rows to out_pos,
 // sorted images to sorted_keys when not null. Values mode (V = u64): the sorted images and
 // the value bits of `vals` that go with them to sorted_keys / sorted_vals.
+// With class_shift, the sort leaves out the top non-constant digit T when there are two or
+// more: the rows come out in stable order of the key bits below 8 T (their class), and
+// *class_shift = 8 T; otherwise every digit is sorted and *class_shift is 8 above the top one.
 template <typename U, typename V>
 void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_t n, int64_t *out_pos,
-           U *sorted_keys, V *sorted_vals, uint64_t limit = 0, uint64_t *bad_row = nullptr) {
+           U *sorted_keys, V *sorted_vals, uint64_t limit = 0, uint64_t *bad_row = nullptr,
+           int *class_shift = nullptr) {
   constexpr bool POS = std::is_same<V, uint32_t>::value;
   if (n == 0) return;
   if (n > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
@@ -1282,6 +1286,10 @@ void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_
   check_hip(hipStreamSynchronize(c.stream), "sync");
   const uint32_t g = stream_grid(n, kBlock * 4, 8192);
   if (active.empty() && !POS) active.push_back(0);  // one stable pass moves the pairs
+  if (class_shift) {
+    *class_shift = 8 * (active.back() + (active.size() >= 2 ? 0 : 1));
+    if (active.size() >= 2) active.pop_back();
+  }
   if (active.empty()) {
     k_iota64<<<g, kBlock, 0, c.stream>>>(n, out_pos);
     if (sorted_keys) k_copy_images<U><<<g, kBlock, 0, c.stream>>>(type, keys, n, sorted_keys);
@@ -1375,13 +1383,14 @@ void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, 
 }
 
 void sort_chain_values(Ctx &c, bool wide, const int64_t *keys, const uint64_t *vals, uint64_t n,
-                       void *sorted_keys, uint64_t *sorted_vals, uint64_t buckets, uint64_t *bad_row) {
+                       void *sorted_keys, uint64_t *sorted_vals, uint64_t buckets, uint64_t *bad_row,
+                       int *class_shift) {
   if (wide)
     radix<uint64_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint64_t *>(sorted_keys), sorted_vals,
-                              buckets, bad_row);
+                              buckets, bad_row, class_shift);
   else
     radix<uint32_t, uint64_t>(c, kChainKey, keys, vals, n, nullptr, static_cast<uint32_t *>(sorted_keys), sorted_vals,
-                              buckets, bad_row);
+                              buckets, bad_row, class_shift);
 }
 
 void sort_chain_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,

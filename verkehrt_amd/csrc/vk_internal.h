@@ -345,9 +345,12 @@ void sort_key_values(Ctx &c, bool wide, const void *keys, const uint64_t *vals, 
 // The same for chain keys computed on the fly: row r of bucket keys[r] has chain key
 // keys[r] * 16 + r % 16 (u32 unless wide). The counting pass also checks the bucket keys:
 // *bad_row is set to the first row whose key is outside 0..buckets (or ~0), and nothing is
-// sorted when there is one.
+// sorted when there is one. With class_shift, the top non-constant digit of the chain keys is
+// left unsorted when there are two or more (the rows are in stable order of the key bits below
+// *class_shift, their class); otherwise *class_shift is 8 above the top non-constant digit.
 void sort_chain_values(Ctx &c, bool wide, const int64_t *keys, const uint64_t *vals, uint64_t n,
-                       void *sorted_keys, uint64_t *sorted_vals, uint64_t buckets, uint64_t *bad_row);
+                       void *sorted_keys, uint64_t *sorted_vals, uint64_t buckets, uint64_t *bad_row,
+                       int *class_shift = nullptr);
 // Batched: item i sorts its keys a with the values b (n rows) into sorted_keys / sorted_vals
 // from row aux (set here) of the concatenated rows.
 void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
