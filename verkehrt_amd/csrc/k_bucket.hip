@@ -340,12 +340,19 @@ __global__ __launch_bounds__(kWave) void k_chain_fold_lanes(const uint64_t *__re
 // a class differ only in the digit at `shift` (the bits above it are equal in every row).
 // k_class_bounds finds every class's first row by binary search of the masked sorted keys
 // (starts[nclasses] = b). k_class_fold takes one class per block and streams its rows in
-// chunks of THREADS * kClassItems: a chunk is partitioned stably by the digit in LDS (each wave
-// ranks its 64 rows per step among equal digits with one ballot per digit bit, per-wave
-// counters give row order across the wave's steps and across waves), then thread d adds the
-// chunk's rows of digit d to its chain's running value, in row order, and keeps the value for
-// the next chunk. At the end each thread that saw rows writes its chain's stripe. Every
-// digit is below 2^nbits.
+// chunks of THREADS * kClassItems: a chunk is partitioned stably by the digit in LDS, then
+// thread d adds the chunk's rows of digit d to its chain's running value, in row order, and
+// keeps the value for the next chunk. At the end each thread that saw rows writes its chain's
+// stripe. The fold of a chunk runs while the waves rank the next one (ranking uses registers
+// and per-wave entries only), and the loads of the chunk after that go out once the next
+// chunk is placed.
+//
+// Ranking: a wave takes 64 rows of the chunk per step. Each lane ORs its lane bit into its
+// digit's mask in the wave's LDS entries (an OR does not depend on order), then reads the mask
+// back: the lanes of its step with its digit. Its rank is the digit's count from the wave's
+// earlier steps plus the lanes of that mask below it; the lowest lane of the mask clears the
+// mask and advances the count. A wave's LDS operations complete in issue order, so the read
+// sees every lane's bit of its step, and the next step's ORs follow the clear.
 constexpr int kClassItems = 8;
 constexpr int kClassRadix = 256;
 
@@ -364,24 +371,26 @@ __global__ void k_class_bounds(const uint32_t *__restrict__ sk, uint64_t b, uint
 }
 
 template <int THREADS>
-__global__ __launch_bounds__(THREADS) void k_class_fold(const uint64_t *__restrict__ starts, int shift, int nbits,
+__global__ __launch_bounds__(THREADS) void k_class_fold(const uint64_t *__restrict__ starts, int shift,
                                                         const uint32_t *__restrict__ sk,
                                                         const double *__restrict__ sv, double *stripes) {
   constexpr int W = THREADS / kWave;
   constexpr int C = THREADS * kClassItems;
   constexpr int kWaveRows = kClassItems * kWave;
+  constexpr int kFoldAhead = 8;
   static_assert(THREADS >= kClassRadix, "one thread per digit");
   __shared__ double s_v[C];
-  __shared__ uint32_t s_wcnt[W][kClassRadix];
+  __shared__ unsigned long long s_mask[W][kClassRadix];
+  __shared__ uint32_t s_cnt[W][kClassRadix];
   __shared__ uint32_t s_excl[kClassRadix + 1];
   __shared__ uint32_t s_wsum[kClassRadix / kWave];
-  __shared__ uint32_t s_ck[kClassRadix];
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint64_t p0 = starts[blockIdx.x], p1 = starts[blockIdx.x + 1];
   if (p0 == p1) return;
   const uint64_t below = (uint64_t(1) << l) - 1;
   uint32_t key[kClassItems];
   double val[kClassItems];
+  uint32_t rnk[kClassItems];
   auto load = [&](uint64_t q) {
 #pragma unroll
     for (int i = 0; i < kClassItems; ++i) {
@@ -391,42 +400,39 @@ __global__ __launch_bounds__(THREADS) void k_class_fold(const uint64_t *__restri
       val[i] = ok ? sv[r] : 0.0;
     }
   };
-  double acc = 0.0;
-  bool seen = false;
-  uint32_t chain = 0;
-  load(p0);
-  for (uint64_t q = p0; q < p1; q += C) {
-    const uint32_t m = p1 - q < uint64_t(C) ? static_cast<uint32_t>(p1 - q) : uint32_t(C);
-    for (int z = t; z < W * kClassRadix; z += THREADS) (&s_wcnt[0][0])[z] = 0;
-    __syncthreads();
-    uint32_t rnk[kClassItems];
+  auto rows_at = [&](uint64_t q) {
+    return p1 - q < uint64_t(C) ? static_cast<uint32_t>(p1 - q) : uint32_t(C);
+  };
+  // the ranks of the m rows in registers, from the wave's own entries
+  auto rank = [&](uint32_t m) {
 #pragma unroll
     for (int i = 0; i < kClassItems; ++i) {
       const uint32_t r = uint32_t(w) * kWaveRows + uint32_t(i) * kWave + l;
-      const bool valid = r < m;
       const uint32_t d = (key[i] >> shift) & (kClassRadix - 1);
-      uint64_t peers = __ballot(valid);
-      for (int bit = 0; bit < nbits; ++bit) {
-        const uint64_t bal = __ballot((d >> bit) & 1);
-        peers &= ((d >> bit) & 1) ? bal : ~bal;
-      }
       rnk[i] = 0;
-      if (valid) {
-        rnk[i] = s_wcnt[w][d] + __popcll(peers & below);
-        if (l == __builtin_ctzll(peers)) {
-          s_wcnt[w][d] += __popcll(peers);
-          s_ck[d] = key[i];
+      if (r < m) {
+        atomicOr(&s_mask[w][d], 1ull << l);
+        const uint64_t peers = s_mask[w][d];
+        const uint32_t cnt = s_cnt[w][d];
+        const uint32_t ahead = static_cast<uint32_t>(__popcll(peers & below));
+        rnk[i] = cnt + ahead;
+        if (ahead == 0) {
+          s_mask[w][d] = 0;
+          s_cnt[w][d] = cnt + static_cast<uint32_t>(__popcll(peers));
         }
       }
     }
-    __syncthreads();
+  };
+  // after the ranks: the digit starts (s_excl), and each wave's count of digit d becomes the
+  // place of the wave's first row of digit d
+  auto scan = [&](uint32_t m) {
     uint32_t cnt = 0, inc = 0;
+    uint32_t col[W];
     if (t < kClassRadix) {
 #pragma unroll
       for (int z = 0; z < W; ++z) {
-        const uint32_t v = s_wcnt[z][t];
-        s_wcnt[z][t] = cnt;
-        cnt += v;
+        col[z] = cnt;
+        cnt += s_cnt[z][t];
       }
       inc = cnt;
 #pragma unroll
@@ -440,40 +446,85 @@ __global__ __launch_bounds__(THREADS) void k_class_fold(const uint64_t *__restri
     if (t < kClassRadix) {
       uint32_t wpre = 0;
       for (int z = 0; z < w; ++z) wpre += s_wsum[z];
-      s_excl[t] = wpre + inc - cnt;
+      const uint32_t ex = wpre + inc - cnt;
+      s_excl[t] = ex;
       if (t == 0) s_excl[kClassRadix] = m;
+#pragma unroll
+      for (int z = 0; z < W; ++z) s_cnt[z][t] = ex + col[z];
     }
-    __syncthreads();
+  };
+  auto place = [&](uint32_t m) {
 #pragma unroll
     for (int i = 0; i < kClassItems; ++i) {
       const uint32_t r = uint32_t(w) * kWaveRows + uint32_t(i) * kWave + l;
       if (r < m) {
         const uint32_t d = (key[i] >> shift) & (kClassRadix - 1);
-        s_v[s_excl[d] + s_wcnt[w][d] + rnk[i]] = val[i];
+        s_v[s_cnt[w][d] + rnk[i]] = val[i];
       }
     }
-    // the next chunk's loads go out before this chunk is folded
-    if (q + C < p1) load(q + C);
-    __syncthreads();
-    if (t < kClassRadix) {
-      const uint32_t lo = s_excl[t], hi = s_excl[t + 1];
-      if (hi > lo) {
-        seen = true;
-        chain = s_ck[t];
-      }
-      uint32_t j = lo;
-      for (; j + 4 <= hi; j += 4) {
-        double u[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) u[k] = s_v[j + k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) acc = acc + u[k];
-      }
-      for (; j < hi; ++j) acc = acc + s_v[j];
-    }
-    __syncthreads();
+    // the wave's counts start again from zero for its next ranks (its reads above come first)
+    for (int z = l; z < kClassRadix; z += kWave) s_cnt[w][z] = 0;
+  };
+  for (int z = t; z < W * kClassRadix; z += THREADS) {
+    (&s_mask[0][0])[z] = 0;
+    (&s_cnt[0][0])[z] = 0;
   }
-  if (t < kClassRadix && seen) stripes[chain] = acc;
+  __syncthreads();
+  uint64_t q = p0;
+  load(q);
+  rank(rows_at(q));
+  __syncthreads();
+  scan(rows_at(q));
+  __syncthreads();
+  place(rows_at(q));
+  if (q + C < p1) load(q + C);
+  __syncthreads();
+  double acc = 0.0;
+  bool seen = false;
+  while (true) {
+    // chunk q's rows of digit t are s_v[lo, hi)
+    uint32_t lo = 0, hi = 0;
+    if (t < kClassRadix) {
+      lo = s_excl[t];
+      hi = s_excl[t + 1];
+    }
+    const uint64_t qn = q + C;
+    const bool more = qn < p1;
+    if (t < kClassRadix) {
+      seen = seen || hi > lo;
+      uint32_t j = lo;
+      for (; j + kFoldAhead <= hi; j += kFoldAhead) {
+        double u[kFoldAhead];
+#pragma unroll
+        for (int k = 0; k < kFoldAhead; ++k) u[k] = s_v[j + k];
+#pragma unroll
+        for (int k = 0; k < kFoldAhead; ++k) acc = acc + u[k];
+      }
+      if (j < hi) {
+        // past hi a read repeats the last row and its add is skipped
+        double u[kFoldAhead];
+#pragma unroll
+        for (int k = 0; k < kFoldAhead; ++k) u[k] = s_v[j + k < hi ? j + k : hi - 1];
+#pragma unroll
+        for (int k = 0; k < kFoldAhead; ++k)
+          if (j + k < hi) acc = acc + u[k];
+      }
+    }
+    if (more) rank(rows_at(qn));
+    __syncthreads();  // chunk q folded, chunk qn ranked
+    if (!more) break;
+    scan(rows_at(qn));
+    __syncthreads();
+    place(rows_at(qn));
+    if (qn + C < p1) load(qn + C);
+    __syncthreads();
+    q = qn;
+  }
+  if (t < kClassRadix && seen) {
+    // the chain of digit t: the bits above the digit as in every row, then t, then the class
+    const uint32_t above = sk[p0] >> shift >> 8;
+    stripes[(((above << 8) | uint32_t(t)) << shift) | uint32_t(blockIdx.x)] = acc;
+  }
 }
 
 __global__ void k_combine_stripes(const double *__restrict__ stripes, uint64_t buckets, double *out) {
@@ -597,10 +648,6 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
     if (class_form) {
       const uint64_t nclasses = uint64_t(1) << shift;  // shift is 8 or 16 below 2^20 buckets
       const uint64_t nchains = buckets * 16;
-      uint64_t top = (nchains - 1) >> shift;
-      if (top > kClassRadix - 1) top = kClassRadix - 1;
-      int nbits = 0;
-      while ((uint64_t(1) << nbits) <= top) ++nbits;
       uint64_t *starts = static_cast<uint64_t *>(tmp_alloc(c, (nclasses + 1) * 8));
       k_class_bounds<<<stream_grid(nclasses + 1, kBlock, 8192), kBlock, 0, c.stream>>>(sk, b, nclasses,
                                                                                       uint32_t(nclasses - 1), starts);
@@ -610,9 +657,9 @@ bool fold_chains(Ctx &c, const int64_t *keys, const double *vals, uint64_t b, ui
         k_fill<double><<<stream_grid(nchains, kBlock, 8192), kBlock, 0, c.stream>>>(st, nchains, 0.0);
       }
       if ((b >> shift) >= kClassWideRows)
-        k_class_fold<1024><<<uint32_t(nclasses), 1024, 0, c.stream>>>(starts, shift, nbits, sk, sv, st);
+        k_class_fold<1024><<<uint32_t(nclasses), 1024, 0, c.stream>>>(starts, shift, sk, sv, st);
       else
-        k_class_fold<256><<<uint32_t(nclasses), 256, 0, c.stream>>>(starts, shift, nbits, sk, sv, st);
+        k_class_fold<256><<<uint32_t(nclasses), 256, 0, c.stream>>>(starts, shift, sk, sv, st);
       if (!stripes)
         k_combine_stripes<<<stream_grid(buckets, kBlock, 8192), kBlock, 0, c.stream>>>(st, buckets, out);
       check_hip(hipGetLastError(), "class fold");
