@@ -7,6 +7,12 @@ non-constant digits whose two top digits would give buckets of at most 0.7 x 307
 overflow them (correlated top digits: the batched passes over those row ranges), many equal
 keys inside a bucket (stability), constant middle digits, i32 keys (4 digits) and f64 bit
 patterns, at row counts that are not multiples of any tile.
+
+When the two top digits are too skewed for 16-bit buckets but the three top digits would give
+segments that fit (normal floats: sign and exponent in the top bits), the sort takes three HBM
+passes, finds the segments where the three top digits change and sorts each segment locally;
+the cases below take both local capacities, segments of one key past them, and segments with
+several keys past them (the batched passes over their row ranges).
 """
 import numpy as np
 import pytest
@@ -176,3 +182,55 @@ def test_batched_buckets(D, t):
     got = D.batch_call(f"argsort_{t}", [[D.array(x, t)] for x in xs])
     for x, p in zip(xs, got):
         assert np.array_equal(p.to_numpy(), O.argsort(x)), (t, x.size)
+
+
+def _normal_with_specials(rng, n):
+    f = rng.standard_normal(n)
+    for v in (0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 5e-324, -5e-324):
+        f[rng.integers(0, n, 50)] = v
+    return f
+
+
+@pytest.mark.parametrize("n", [(1 << 22) + 7, (1 << 24) + 3])
+def test_segments_normal_f64(D, n):
+    # three-digit segments: about 140 rows each at 2^22 (3072-row local sorts), more at 2^24
+    _check(D, _normal_with_specials(np.random.default_rng(n), n), "f64")
+
+
+def test_segments_overflow_f64(D):
+    # 1% of the rows one key (a segment of one key past the local capacity, written as it
+    # stands) and 1% in one segment with random lower bits (past the capacity with many keys:
+    # the batched passes over its rows)
+    rng = np.random.default_rng(41)
+    n = (1 << 22) + 1001
+    f = _normal_with_specials(rng, n)
+    f[rng.random(n) < 0.01] = 1.0
+    hot = rng.random(n) < 0.01
+    bits = np.float64(1.5).view(np.uint64) | rng.integers(0, 1 << 40, hot.sum(), dtype=np.uint64)
+    f[hot] = bits.view(np.float64)
+    _check(D, f, "f64")
+
+
+def test_segments_skewed_top_byte_u64(D):
+    # the top byte one value for 90% of the rows: the two-digit estimate fails, three fit
+    rng = np.random.default_rng(42)
+    n = (1 << 22) + 77
+    top = np.where(rng.random(n) < 0.9, 1, rng.integers(0, 256, n)).astype(np.uint64)
+    k = (top << np.uint64(56)) | rng.integers(0, 1 << 56, n, dtype=np.uint64)
+    _check(D, k, "u64")
+    _check(D, k.view(np.int64), "i64")
+
+
+def test_segments_three_levels_u64(D):
+    # 16 x 16 x 20 values of the three top bytes with weights 0.5 / 1 / 1.6 / 2.4 / 4: segments of
+    # about 1.2K and 2.4K rows (3072-row local sorts), 3.8K (5120-row), 5.7K (8192-row) and 9.5K
+    # rows (past every capacity: the batched passes over their rows)
+    rng = np.random.default_rng(43)
+    n = (1 << 24) + 5
+    a, b, c = np.meshgrid(np.arange(16), np.arange(16), np.arange(20), indexing="ij")
+    combos = ((a.ravel() << 16) | (b.ravel() << 8) | c.ravel()).astype(np.uint64)
+    w = rng.choice(np.array([0.5, 1.0, 1.6, 2.4]), combos.size)
+    w[rng.integers(0, combos.size, 40)] = 4.0
+    top = combos[rng.choice(combos.size, n, p=w / w.sum())]
+    k = (top << np.uint64(40)) | rng.integers(0, 1 << 40, n, dtype=np.uint64)
+    _check(D, k, "u64")

@@ -630,13 +630,15 @@ __global__ __launch_bounds__(THREADS) void k_local_sort(const U *__restrict__ ke
 // One block per segment j of a batched sort: rows [seg[j], seg[j + 1]) (the last to `total`) of
 // the concatenated rows, inside one item (an item's first row starts a segment); the item is
 // found by binary search of the items' first rows (aux) and the positions go to its output. A
-// segment the block cannot sort is listed in overflow[1..].
+// segment the block cannot sort is listed in overflow[1..]. With ids, block i takes segment
+// ids[i].
 template <typename U, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_local_sort_seg(const U *__restrict__ keys, const uint32_t *__restrict__ pos,
                                                             const int64_t *__restrict__ seg, uint64_t nseg,
                                                             uint64_t total, const BatchItem *items, uint32_t nitems,
-                                                            const LocalDigits dg, uint32_t *__restrict__ overflow) {
-  const uint64_t j = blockIdx.x;
+                                                            const LocalDigits dg, uint32_t *__restrict__ overflow,
+                                                            const uint32_t *__restrict__ ids = nullptr) {
+  const uint64_t j = ids ? ids[blockIdx.x] : blockIdx.x;
   const uint64_t s = static_cast<uint64_t>(seg[j]);
   const uint64_t e = j + 1 < nseg ? static_cast<uint64_t>(seg[j + 1]) : total;
   uint32_t lo = 0, hi = nitems;  // the last item whose first row is at most s
@@ -917,6 +919,15 @@ __global__ __launch_bounds__(kBlock) void k_seg_flags_batch(const BatchItem *ite
     flags[I.aux + r] = r == 0 || bucket16<U>(k[r], sh_hi, sh_lo) != bucket16<U>(k[r - 1], sh_hi, sh_lo);
 }
 
+// flags[r] = 1 where row r of keys sorted by the bits of `mask` starts a segment: the first row,
+// or a row whose masked bits differ from the row before.
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_seg_flags(const U *__restrict__ keys, uint64_t n, U mask,
+                                                      uint8_t *__restrict__ flags) {
+  for (uint64_t r = blockIdx.x * uint64_t(kBlock) + threadIdx.x; r < n; r += uint64_t(gridDim.x) * kBlock)
+    flags[r] = r == 0 || ((keys[r] ^ keys[r - 1]) & mask) != 0;
+}
+
 // The local sort's capacity in rows for a bucket-size estimate: 3072 (256 threads x 12 rows,
 // four blocks per compute unit) or 8192 (512 threads x 16 rows, one block), 0 when buckets
 // would not fit.
@@ -1023,6 +1034,103 @@ void argsort_buckets(Ctx &c, int32_t type, const void *keys, uint64_t n, const s
     finish_ranges<U>(c, ranges, std::vector<int>(digits.begin(), digits.end() - 2), ka, pa, kb, pb);
   }
   tmp_free(c, start);
+  tmp_free(c, ka);
+  tmp_free(c, pa);
+  tmp_free(c, kb);
+  tmp_free(c, pb);
+}
+
+// Argsort by segments of the three top digits: for keys whose two top digits are too skewed for
+// 16-bit buckets (normal floats: sign and exponent bits), three HBM passes by the three top
+// digits, then the segments where those digits change (flags and the ordered selection), one
+// local sort per segment over the other digits, and the batched passes for segments the
+// local sorts cannot take. `digits` are the active digits, ascending, at least five; hist
+// holds their exclusive digit bases.
+template <typename U>
+void argsort_segments(Ctx &c, int32_t type, const void *keys, uint64_t n, const std::vector<int> &digits,
+                      const uint32_t *hist, int cap, int64_t *out_pos) {
+  using V = uint32_t;
+  const int np = static_cast<int>(digits.size());
+  const int d0 = digits[np - 3], d1 = digits[np - 2], d2 = digits[np - 1];
+  U *ka = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
+  V *pa = static_cast<V *>(tmp_alloc(c, n * sizeof(V)));
+  U *kb = static_cast<U *>(tmp_alloc(c, n * sizeof(U)));
+  V *pb = static_cast<V *>(tmp_alloc(c, n * sizeof(V)));
+  run_pass<U, V, true, false>(c, type, keys, nullptr, nullptr, n, 8 * d0, hist + d0 * kRadix, ka, pa, nullptr);
+  run_pass<U, V, false, false>(c, type, ka, pa, nullptr, n, 8 * d1, hist + d1 * kRadix, kb, pb, nullptr);
+  run_pass<U, V, false, false>(c, type, kb, pb, nullptr, n, 8 * d2, hist + d2 * kRadix, ka, pa, nullptr);
+  const U mask = (U(255) << (8 * d0)) | (U(255) << (8 * d1)) | (U(255) << (8 * d2));
+  uint8_t *flags = static_cast<uint8_t *>(tmp_alloc(c, n + 8));
+  int64_t *seg = static_cast<int64_t *>(tmp_alloc(c, n * 8 + 8));
+  k_seg_flags<U><<<stream_grid(n, kBlock, 8192), kBlock, 0, c.stream>>>(ka, n, mask, flags);
+  check_hip(hipGetLastError(), "segment flags");
+  SelectSpec ss{2, VK_I64, 0, 0, 0};
+  uint64_t *d_cnt = c.scratch.slots + 21;
+  select_rows(c, ss, nullptr, flags, n, seg, d_cnt);
+  uint64_t nseg = 0;
+  read_back(c, d_cnt, 8, &nseg);
+  if (nseg > 0x7FFFFFFFull) usage_error("argsort has too many segments");
+  // one item: the whole output from row 0
+  BatchItem whole{};
+  whole.out = out_pos;
+  whole.n = n;
+  auto *d_items = static_cast<BatchItem *>(tmp_alloc(c, sizeof(BatchItem)));
+  check_hip(hipMemcpyAsync(d_items, &whole, sizeof(BatchItem), hipMemcpyHostToDevice, c.stream), "segment item");
+  // every segment through the 3072-row local sort (four blocks per compute unit); with the
+  // larger capacity, the segments it lists go through the 5120-row one (two blocks per unit),
+  // and those that one lists through the 8192-row one
+  uint32_t *over = static_cast<uint32_t *>(tmp_alloc(c, 3 * (nseg * 4 + 8)));
+  uint32_t *overflow = over;
+  fill_u64(c, reinterpret_cast<uint64_t *>(overflow), 1, 0);
+  LocalDigits dg{};
+  dg.n = np - 3;
+  for (int k = 0; k < np - 3; ++k) dg.shift[k] = 8 * digits[k];
+  k_local_sort_seg<U, 256, 12><<<uint32_t(nseg), 256, 0, c.stream>>>(ka, pa, seg, nseg, n, d_items, 1, dg, overflow);
+  check_hip(hipGetLastError(), "segment sort");
+  uint32_t nover = 0;
+  read_back(c, overflow, 4, &nover);
+  if (nover && cap == 8192) {
+    uint32_t *mid = over + nseg + 2;
+    fill_u64(c, reinterpret_cast<uint64_t *>(mid), 1, 0);
+    k_local_sort_seg<U, 512, 10><<<nover, 512, 0, c.stream>>>(ka, pa, seg, nseg, n, d_items, 1, dg, mid,
+                                                             overflow + 1);
+    check_hip(hipGetLastError(), "segment sort");
+    overflow = mid;
+    read_back(c, overflow, 4, &nover);
+  }
+  if (nover && cap == 8192) {
+    uint32_t *wide = over + 2 * (nseg + 2);
+    fill_u64(c, reinterpret_cast<uint64_t *>(wide), 1, 0);
+    k_local_sort_seg<U, 512, 16><<<nover, 512, 0, c.stream>>>(ka, pa, seg, nseg, n, d_items, 1, dg, wide,
+                                                             overflow + 1);
+    check_hip(hipGetLastError(), "wide segment sort");
+    overflow = wide;
+    read_back(c, overflow, 4, &nover);
+  }
+  if (nover) {
+    std::vector<uint32_t> ids(nover);
+    std::vector<int64_t> starts(nseg);
+    check_hip(hipMemcpyAsync(ids.data(), overflow + 1, nover * 4, hipMemcpyDeviceToHost, c.stream), "overflow");
+    check_hip(hipMemcpyAsync(starts.data(), seg, nseg * 8, hipMemcpyDeviceToHost, c.stream), "segments");
+    check_hip(hipStreamSynchronize(c.stream), "sync");
+    std::vector<BatchItem> ranges;
+    for (uint32_t j : ids) {
+      const uint64_t s0 = static_cast<uint64_t>(starts[j]);
+      const uint64_t e = j + 1 < nseg ? static_cast<uint64_t>(starts[j + 1]) : n;
+      BatchItem it{};
+      it.a = ka + s0;
+      it.out = out_pos + s0;
+      it.n = e - s0;
+      it.aux = s0;
+      ranges.push_back(it);
+    }
+    // the sorted rows are in ka / pa; kb / pb are free
+    finish_ranges<U>(c, ranges, std::vector<int>(digits.begin(), digits.end() - 3), kb, pb, ka, pa);
+  }
+  tmp_free(c, over);
+  tmp_free(c, d_items);
+  tmp_free(c, flags);
+  tmp_free(c, seg);
   tmp_free(c, ka);
   tmp_free(c, pa);
   tmp_free(c, kb);
@@ -1312,6 +1420,19 @@ void radix(Ctx &c, int32_t type, const void *keys, const uint64_t *vals, uint64_
         argsort_buckets<U>(c, type, keys, n, active, hist, cap, out_pos);
         tmp_free(c, hist);
         return;
+      }
+      // two top digits too skewed: segments of three, when there are two or more digits left
+      // for the local sorts and the three top digits would give segments that fit
+      if (np >= 5) {
+        const int d3 = active[np - 3];
+        uint32_t m3 = 0;
+        for (int v = 0; v < kRadix; ++v) m3 = h[d3 * kRadix + v] > m3 ? h[d3 * kRadix + v] : m3;
+        const int cap3 = local_capacity(double(mlo) * double(mhi) * double(m3) / double(n) / double(n));
+        if (cap3) {
+          argsort_segments<U>(c, type, keys, n, active, hist, cap3, out_pos);
+          tmp_free(c, hist);
+          return;
+        }
       }
     }
   }
