@@ -444,9 +444,12 @@ __device__ __forceinline__ bool local_sort_rows(const U *__restrict__ keys, cons
   constexpr int C = THREADS * ITEMS;
   constexpr int W = THREADS / kWave;
   static_assert(THREADS >= kRadix, "one thread per digit in the digit scan");
+  // per-wave digit counts stay below 2^16 (at most ITEMS * 64 rows per wave): 16-bit counters
+  // keep the LDS of the 3072-row form small enough for three blocks per compute unit
+  using Cnt = typename std::conditional<(ITEMS * kWave < 65536), uint16_t, uint32_t>::type;
   __shared__ U s_key[C];
   __shared__ uint32_t s_pos[C];
-  __shared__ uint32_t s_wcnt[W][kRadix];
+  __shared__ Cnt s_wcnt[W][kRadix];
   __shared__ uint32_t s_excl[kRadix + 1];
   __shared__ uint32_t s_wsum[kRadix / kWave];
   __shared__ uint32_t s_max;
@@ -564,29 +567,80 @@ __device__ __forceinline__ bool local_sort_rows(const U *__restrict__ keys, cons
     // each row's place, kept in registers until every row has been read, then the positions
     // placed in LDS and written out in order
     uint32_t dst[ITEMS], pv[ITEMS];
+    constexpr int kPosBits = 24;  // LDS places (C <= 2^24)
+    static_assert(C <= (1 << kPosBits), "places fit the composite");
+    if (top + kPosBits <= 8 * static_cast<int>(sizeof(U))) {
+      // The rows of a sub-bucket share every bit from `top` up, so inside it (key, place)
+      // orders as (the key's bits below top, place): each key in LDS becomes that one word,
+      // unique, and a row's place in its sub-bucket is the number of smaller words there.
+      const U lowmask = (U(1) << top) - 1;
+      uint32_t lo[ITEMS], hi[ITEMS];
+      U cj[ITEMS];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t j = uint32_t(t) + uint32_t(i) * THREADS;
-      dst[i] = 0;
-      pv[i] = 0;
-      if (j < m) {
-        const U kj = s_key[j];
-        const uint32_t d = static_cast<uint32_t>(kj >> top) & 255;
-        const uint32_t lo = s_excl[d], hi = s_excl[d + 1];
-        // four reads in flight per step (the sub-bucket holds j, so hi - 1 is in it)
-        uint32_t r = 0;
-        for (uint32_t q = lo; q < hi; q += 4) {
-          U kq[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) kq[u] = s_key[q + u < hi ? q + u : hi - 1];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t qq = q + u;
-            r += (qq < hi && (kq[u] < kj || (kq[u] == kj && qq < j))) ? 1u : 0u;
-          }
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(t) + uint32_t(i) * THREADS;
+        lo[i] = 0;
+        hi[i] = 0;
+        pv[i] = 0;
+        cj[i] = 0;
+        if (j < m) {
+          const U kj = s_key[j];
+          const uint32_t d = static_cast<uint32_t>(kj >> top) & 255;
+          lo[i] = s_excl[d];
+          hi[i] = s_excl[d + 1];
+          pv[i] = s_pos[j];
+          cj[i] = ((kj & lowmask) << kPosBits) | U(j);
         }
-        dst[i] = lo + r;
-        pv[i] = s_pos[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(t) + uint32_t(i) * THREADS;
+        if (j < m) s_key[j] = cj[i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(t) + uint32_t(i) * THREADS;
+        dst[i] = 0;
+        if (j < m) {
+          // four reads in flight per step; past hi a read stands for no row
+          uint32_t r = 0;
+          for (uint32_t q = lo[i]; q < hi[i]; q += 4) {
+            U kq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) kq[u] = q + u < hi[i] ? s_key[q + u] : ~U(0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r += kq[u] < cj[i] ? 1u : 0u;
+          }
+          dst[i] = lo[i] + r;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(t) + uint32_t(i) * THREADS;
+        dst[i] = 0;
+        pv[i] = 0;
+        if (j < m) {
+          const U kj = s_key[j];
+          const uint32_t d = static_cast<uint32_t>(kj >> top) & 255;
+          const uint32_t lo = s_excl[d], hi = s_excl[d + 1];
+          // four reads in flight per step (the sub-bucket holds j, so hi - 1 is in it)
+          uint32_t r = 0;
+          for (uint32_t q = lo; q < hi; q += 4) {
+            U kq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) kq[u] = s_key[q + u < hi ? q + u : hi - 1];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t qq = q + u;
+              r += (qq < hi && (kq[u] < kj || (kq[u] == kj && qq < j))) ? 1u : 0u;
+            }
+          }
+          dst[i] = lo + r;
+          pv[i] = s_pos[j];
+        }
       }
     }
     __syncthreads();
@@ -928,8 +982,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_flags(const U *__restrict__ keys
     flags[r] = r == 0 || ((keys[r] ^ keys[r - 1]) & mask) != 0;
 }
 
-// The local sort's capacity in rows for a bucket-size estimate: 3072 (256 threads x 12 rows,
-// four blocks per compute unit) or 8192 (512 threads x 16 rows, one block), 0 when buckets
+// The local sort's capacity in rows for a bucket-size estimate: 3072 (512 threads x 6 rows,
+// three blocks per compute unit) or 8192 (512 threads x 16 rows, one block), 0 when buckets
 // would not fit.
 inline int local_capacity(double est_max) {
   if (est_max <= 0.7 * 3072) return 3072;
@@ -1012,7 +1066,7 @@ void argsort_buckets(Ctx &c, int32_t type, const void *keys, uint64_t n, const s
   LocalDigits dg{};
   dg.n = np - 2;
   for (int k = 0; k < np - 2; ++k) dg.shift[k] = 8 * digits[k];
-  if (cap == 3072) k_local_sort<U, 256, 12><<<kBuckets, 256, 0, c.stream>>>(kb, pb, start, dg, out_pos, overflow);
+  if (cap == 3072) k_local_sort<U, 512, 6><<<kBuckets, 512, 0, c.stream>>>(kb, pb, start, dg, out_pos, overflow);
   else k_local_sort<U, 512, 16><<<kBuckets, 512, 0, c.stream>>>(kb, pb, start, dg, out_pos, overflow);
   check_hip(hipGetLastError(), "local sort");
   uint32_t nover = 0;
@@ -1076,7 +1130,7 @@ void argsort_segments(Ctx &c, int32_t type, const void *keys, uint64_t n, const 
   whole.n = n;
   auto *d_items = static_cast<BatchItem *>(tmp_alloc(c, sizeof(BatchItem)));
   check_hip(hipMemcpyAsync(d_items, &whole, sizeof(BatchItem), hipMemcpyHostToDevice, c.stream), "segment item");
-  // every segment through the 3072-row local sort (four blocks per compute unit); with the
+  // every segment through the 3072-row local sort (three blocks per compute unit); with the
   // larger capacity, the segments it lists go through the 5120-row one (two blocks per unit),
   // and those that one lists through the 8192-row one
   uint32_t *over = static_cast<uint32_t *>(tmp_alloc(c, 3 * (nseg * 4 + 8)));
@@ -1085,7 +1139,7 @@ void argsort_segments(Ctx &c, int32_t type, const void *keys, uint64_t n, const 
   LocalDigits dg{};
   dg.n = np - 3;
   for (int k = 0; k < np - 3; ++k) dg.shift[k] = 8 * digits[k];
-  k_local_sort_seg<U, 256, 12><<<uint32_t(nseg), 256, 0, c.stream>>>(ka, pa, seg, nseg, n, d_items, 1, dg, overflow);
+  k_local_sort_seg<U, 512, 6><<<uint32_t(nseg), 512, 0, c.stream>>>(ka, pa, seg, nseg, n, d_items, 1, dg, overflow);
   check_hip(hipGetLastError(), "segment sort");
   uint32_t nover = 0;
   read_back(c, overflow, 4, &nover);
@@ -1293,7 +1347,7 @@ void argsort_batch_pos(Ctx &c, int32_t type, std::vector<BatchItem> &items) {
     dg.n = np - 2;
     for (int k = 0; k < np - 2; ++k) dg.shift[k] = 8 * digits[k];
     if (nseg > 0x7FFFFFFFull) usage_error("batched argsort has too many segments");
-    k_local_sort_seg<U, 256, 12><<<uint32_t(nseg), 256, 0, c.stream>>>(kb, pb, seg, nseg, rows, B.plan.items,
+    k_local_sort_seg<U, 512, 6><<<uint32_t(nseg), 512, 0, c.stream>>>(kb, pb, seg, nseg, rows, B.plan.items,
                                                                      uint32_t(B.items.size()), dg, overflow);
     check_hip(hipGetLastError(), "segment sort");
     uint32_t nover = 0;
