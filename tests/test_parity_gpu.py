@@ -569,6 +569,21 @@ def test_group_id_large(D):
     assert np.array_equal(D.call("group_id_u8", [D.array(xu)]).to_numpy(), O.group_id(xu))
 
 
+def test_group_id_sketch_sized_table(D):
+    """More keys than the small table takes but far fewer than rows: the full table is sized from
+    the sketched count of distinct keys (here 2^23 slots for about 3M keys, against 2^24 for 2n),
+    with the empty marker's key among them, and i32 keys of the same shape."""
+    rng = np.random.default_rng(44)
+    n = 8_000_003
+    marker = np.array([0xFFF8DEADBEEF1357], np.uint64).view(np.int64)[0]
+    base = rng.integers(-2**63, 2**63 - 1, 3_000_000, dtype=np.int64)
+    base[1234] = marker
+    x = base[rng.integers(0, base.size, n)]
+    assert np.array_equal(D.call("group_id_i64", [view(D, x, 3)]).to_numpy(), O.group_id(x))
+    xi = rng.integers(-2**31, 2**31 - 1, 2_500_000, dtype=np.int64).astype(np.int32)[rng.integers(0, 2_500_000, n)]
+    assert np.array_equal(D.call("group_id_i32", [D.array(xi)]).to_numpy(), O.group_id(xi))
+
+
 # ----------------------------------------------------------------- elementwise
 
 def test_elementwise(D):

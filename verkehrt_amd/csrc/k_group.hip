@@ -13,6 +13,8 @@
 //     first-occurrence order: the slot of the j-th first row gets id j;
 //  4. every row reads the id of its slot.
 // Keys compare by their zero-extended bit patterns (Column::bits, group.rs:84-116).
+#include <cmath>
+
 #include "vk_device.h"
 #include "vk_internal.h"
 
@@ -287,6 +289,49 @@ __global__ __launch_bounds__(kGBlock) void k_insert1(KeyCol a, uint64_t n, Slot 
     const uint64_t k = s_key[i];
     if (k != kLEmpty && !global_insert(table, mask, shift, k, s_row[i], max_run)) atomicOr(overflow, 1u);
   }
+}
+
+// A HyperLogLog sketch of the distinct keys (4096 registers, standard error about 1.6%): each
+// row's key is hashed (a 64-bit finalizer), its top 12 bits pick a register and the register
+// keeps the most leading zeros, plus one, seen in the remaining bits; per-block LDS registers,
+// merged into reg[] with one atomicMax per nonzero register.
+constexpr int kHllBits = 12;
+constexpr int kHllRegs = 1 << kHllBits;
+
+__device__ __forceinline__ uint64_t hll_hash(uint64_t x) {
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hll(KeyCol a, uint64_t n, uint32_t *reg) {
+  __shared__ uint32_t s_reg[kHllRegs];
+  for (int i = threadIdx.x; i < kHllRegs; i += kBlock) s_reg[i] = 0;
+  __syncthreads();
+  a.with_bits([&](auto bits) {
+    for (uint64_t r0 = blockIdx.x * uint64_t(kBlock) * 4; r0 < n; r0 += uint64_t(gridDim.x) * kBlock * 4) {
+      uint64_t k[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t r = r0 + uint64_t(u) * kBlock + threadIdx.x;
+        k[u] = r < n ? bits(r) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t r = r0 + uint64_t(u) * kBlock + threadIdx.x;
+        if (r >= n) continue;
+        const uint64_t h = hll_hash(k[u]);
+        const uint32_t idx = static_cast<uint32_t>(h >> (64 - kHllBits));
+        const uint64_t rest = h << kHllBits;
+        const uint32_t rho = rest ? static_cast<uint32_t>(__clzll(static_cast<long long>(rest))) + 1u
+                                  : static_cast<uint32_t>(64 - kHllBits + 1);
+        if (s_reg[idx] < rho) atomicMax(&s_reg[idx], rho);
+      }
+    }
+  });
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHllRegs; i += kBlock)
+    if (s_reg[i]) atomicMax(&reg[i], s_reg[i]);
 }
 
 // Slots 0 .. slots (the last one the empty marker's own): a used slot flags its least row.
@@ -779,6 +824,35 @@ void group_id_pair(Ctx &c, KeyCol ka, KeyCol kb, uint64_t n, int64_t *out) {
 
 }  // namespace
 
+namespace {
+
+constexpr int kSizedRun = 1024;  // longest probe run allowed in a table sized by the sketch
+
+// The sketched count of distinct keys of a (HyperLogLog: the registers' harmonic mean, with
+// linear counting while the estimate is small and registers are still zero).
+uint64_t distinct_estimate(Ctx &c, const KeyCol &a, uint64_t n) {
+  auto *reg = static_cast<uint32_t *>(tmp_alloc(c, kHllRegs * 4));
+  fill_u64(c, reinterpret_cast<uint64_t *>(reg), kHllRegs / 2, 0);
+  k_hll<<<stream_grid(n, kBlock * 4, 2048), kBlock, 0, c.stream>>>(a, n, reg);
+  check_hip(hipGetLastError(), "group_id sketch");
+  std::vector<uint32_t> h(kHllRegs);
+  check_hip(hipMemcpyAsync(h.data(), reg, kHllRegs * 4, hipMemcpyDeviceToHost, c.stream), "sketch");
+  check_hip(hipStreamSynchronize(c.stream), "sync");
+  tmp_free(c, reg);
+  const double m = kHllRegs;
+  double z = 0.0;
+  int zeros = 0;
+  for (uint32_t v : h) {
+    z += std::ldexp(1.0, -static_cast<int>(v));
+    zeros += v == 0;
+  }
+  double e = 0.7213 / (1.0 + 1.079 / m) * m * m / z;
+  if (e <= 2.5 * m && zeros) e = m * std::log(m / zeros);
+  return static_cast<uint64_t>(e) + 1;
+}
+
+}  // namespace
+
 void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint64_t n,
               int64_t *out) {
   if (n == 0) return;
@@ -818,6 +892,33 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
     } else {
       slots = kSmallSlots;
       lg = kSmallLg;
+    }
+  }
+  if (!table && full > kSmallSlots) {
+    // more keys than the small table takes: a table of the next power of two above twice the
+    // sketched count of distinct keys, when that is below 2n slots; a probe run past
+    // kSizedRun there (an estimate far too low) sends the insert to 2n slots
+    const uint64_t est = distinct_estimate(c, ka, n);
+    uint64_t sized = kSmallSlots;
+    int lg_sized = kSmallLg;
+    while (sized < 2 * est) {
+      sized <<= 1;
+      ++lg_sized;
+    }
+    if (sized < full) {
+      table = new_table(sized);
+      fill_u64(c, c.scratch.slots + 22, 1, 0);
+      k_insert1<<<g, kGBlock, 0, c.stream>>>(ka, n, table, sized - 1, 64 - lg_sized, kSizedRun, d_over);
+      check_hip(hipGetLastError(), "group_id insert");
+      uint32_t over = 0;
+      read_back(c, d_over, 4, &over);
+      if (over) {
+        tmp_free(c, table);
+        table = nullptr;
+      } else {
+        slots = sized;
+        lg = lg_sized;
+      }
     }
   }
   if (!table) {
