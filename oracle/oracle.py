@@ -48,6 +48,8 @@ def _declare(L):
         "vko_canonical_sum_f64": (dbl, [vp, u64]),
         "vko_canonical_product_f64": (dbl, [vp, u64]),
         "vko_canonical_sum_f32": (C.c_float, [vp, u64]),
+        "vko_canonical_fold_f32": (C.c_uint32, [i32, vp, vp, u64, i32, C.c_float, C.c_float]),
+        "vko_fused_sum_int": (u64, [i32, i32, vp, vp, u64, i32, u64, u64]),
         "vko_canonical_fold_f64": (i32, [i32, vp, vp, u64, i32, dbl, dbl, vp, i32, i32, vp]),
         "vko_dot_f64": (dbl, [vp, vp, u64]),
         "vko_sumsq_f64": (dbl, [vp, u64]),
@@ -144,6 +146,36 @@ def canonical_product_f64(x) -> float:
 def canonical_sum_f32(x) -> np.float32:
     x = _c(x, np.float32)
     return np.float32(lib().vko_canonical_sum_f32(_p(x), x.size))
+
+
+def canonical_fold_f32(kind: str, x, y=None, cmp="gt", c=0.0, e=0.0) -> np.float32:
+    """The canonical f32 fold (accumulation in f32) of row values of ``kind``: sum, dot, sumsq,
+    condsum or mul (the product)."""
+    kinds = {"sum": 0, "dot": 1, "sumsq": 2, "condsum": 3, "mul": 4}
+    x = _c(x, np.float32)
+    y = _c(y if y is not None else x, np.float32)
+    if x.size != y.size:
+        raise OracleError(f"length mismatch: {x.size} vs {y.size}")
+    bits = lib().vko_canonical_fold_f32(kinds[kind], _p(x), _p(y), x.size, CMPS[cmp], float(np.float32(c)),
+                                        float(np.float32(e)))
+    return np.array([bits], np.uint32).view(np.float32)[0]
+
+
+def fused_sum_int(kind: str, x, y=None, cmp="gt", c=0, e=0) -> int:
+    """dot, sumsq or condsum over an i64, u64 or i32 array: wrapping row values in the element
+    type, widened, summed with wrapping; the result as a Python int of the sum type (i64 for
+    i64 and i32, u64 for u64)."""
+    kinds = {"dot": 1, "sumsq": 2, "condsum": 3}
+    x = np.ascontiguousarray(x)
+    t = {np.dtype(np.int64): 2, np.dtype(np.uint64): 3, np.dtype(np.int32): 4}[x.dtype]
+    y = np.ascontiguousarray(y if y is not None else x)
+    if y.dtype != x.dtype or y.size != x.size:
+        raise OracleError(f"length mismatch: {x.size} vs {y.size}")
+    mask = (1 << 64) - 1
+    bits = lib().vko_fused_sum_int(kinds[kind], t, _p(x), _p(y), x.size, CMPS[cmp], int(c) & mask, int(e) & mask)
+    if x.dtype == np.uint64:
+        return bits
+    return bits - (1 << 64) if bits >> 63 else bits
 
 
 def canonical_fold_f64(kind: str, x, y=None, cmp="gt", c=0.0, e=0.0, cuts=(), mul=False) -> float:

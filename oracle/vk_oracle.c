@@ -197,6 +197,52 @@ double vko_condsum_f64(const double *x, uint64_t n, int cmp, double c, double e)
   return r;
 }
 
+/* The canonical fold at T = f32 (update_lanes_with / finish_lanes reduce.rs:40-91 and
+ * CanonicalAcc reduce.rs:249-323 instantiated at f32, Reducible reduce.rs:567-575), with the
+ * fused row values of fused.rs:72-210 at f32: kind 0 x[i] (reduce_add_f32), 1 x[i]*y[i]
+ * (dot_f32), 2 x[i]*x[i] (sumsq_f32), 3 condsum pick (condsum_<cmp>_f32), 4 the product of
+ * x[i] (reduce_mul_f32, identity 1). Every operation is an f32 operation (no FMA: built with
+ * -ffp-contract=off). Returns the f32 result's bits. */
+static inline int cmp_f32(int cmp, float a, float b) {
+  switch (cmp) {
+    case 0: return a == b;
+    case 1: return a != b;
+    case 2: return a < b;
+    case 3: return a <= b;
+    case 4: return a > b;
+    default: return a >= b;
+  }
+}
+uint32_t vko_canonical_fold_f32(int kind, const float *x, const float *y, uint64_t n, int cmp, float c, float e) {
+  const int mul = kind == 4;
+  float acc[LANES];
+  for (int j = 0; j < LANES; ++j) acc[j] = mul ? 1.0f : 0.0f;
+  uint64_t b = n - n % LANES;
+  for (uint64_t i = 0; i < n; ++i) {
+    float v;
+    switch (kind) {
+      case 1: v = x[i] * y[i]; break;
+      case 2: v = x[i] * x[i]; break;
+      case 3: v = cmp_f32(cmp, x[i], c) ? x[i] : e; break;
+      default: v = x[i]; break;
+    }
+    if (i < b) {
+      acc[i % LANES] = mul ? acc[i % LANES] * v : acc[i % LANES] + v;
+    } else {
+      if (i == b)
+        for (int step = LANES / 2; step > 0; step /= 2)
+          for (int j = 0; j < step; ++j) acc[j] = mul ? acc[j] * acc[j + step] : acc[j] + acc[j + step];
+      acc[0] = mul ? acc[0] * v : acc[0] + v;
+    }
+  }
+  if (b == n)
+    for (int step = LANES / 2; step > 0; step /= 2)
+      for (int j = 0; j < step; ++j) acc[j] = mul ? acc[j] * acc[j + step] : acc[j] + acc[j + step];
+  uint32_t bits;
+  memcpy(&bits, &acc[0], 4);
+  return bits;
+}
+
 /* ----------------------------------------------------- integer reductions */
 
 /* IntSum wrapping in the widened type, reduce.rs:325-377 */
@@ -231,6 +277,48 @@ uint64_t vko_product_int(const void *x, uint32_t width, uint64_t n) {
     p *= v; /* the low `width` bytes of a product depend only on the low bytes of its factors */
   }
   return width == 8 ? p : p & ((1ull << (8 * width)) - 1);
+}
+
+/* dot, sumsq and condsum over integers (Dot, SumSq, CondSum fused.rs:72-210 with the IntSum
+ * accumulator reduce.rs:325-377): the row value in the element type (Num::mul is a wrapping
+ * product, elementwise.rs:380-397), widened (i32 sign-extends to i64) and added with wrapping.
+ * kind 1 x*y, 2 x*x, 3 (x <cmp> c ? x : e). type: 2 i64, 3 u64, 4 i32 (the element type codes
+ * below); c and e are the constants' bits. Returns the sum's 64 bits. */
+uint64_t vko_fused_sum_int(int kind, int type, const void *x, const void *y, uint64_t n, int cmp, uint64_t c,
+                           uint64_t e) {
+  uint64_t s = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t w;
+    if (type == 4) {
+      const int32_t a = ((const int32_t *)x)[i], bb = kind == 1 ? ((const int32_t *)y)[i] : a;
+      int32_t v;
+      if (kind == 3) v = cmp_i64(cmp, a, (int32_t)(uint32_t)c) ? a : (int32_t)(uint32_t)e;
+      else v = (int32_t)((uint32_t)a * (uint32_t)bb);
+      w = (uint64_t)(int64_t)v;
+    } else if (type == 3) {
+      const uint64_t a = ((const uint64_t *)x)[i], bb = kind == 1 ? ((const uint64_t *)y)[i] : a;
+      if (kind == 3) {
+        int k;
+        switch (cmp) {
+          case 0: k = a == c; break;
+          case 1: k = a != c; break;
+          case 2: k = a < c; break;
+          case 3: k = a <= c; break;
+          case 4: k = a > c; break;
+          default: k = a >= c; break;
+        }
+        w = k ? a : e;
+      } else {
+        w = a * bb;
+      }
+    } else {
+      const int64_t a = ((const int64_t *)x)[i], bb = kind == 1 ? ((const int64_t *)y)[i] : a;
+      if (kind == 3) w = cmp_i64(cmp, a, (int64_t)c) ? (uint64_t)a : e;
+      else w = (uint64_t)a * (uint64_t)bb;
+    }
+    s += w;
+  }
+  return s;
 }
 
 /* count_cmp, fused.rs:142-147 (CountCmp fused.rs:149-179) */

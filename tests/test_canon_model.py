@@ -25,6 +25,8 @@ def model():
     L.vkm_canonical_fold_streamed.restype = C.c_int
     L.vkm_canonical_fold_streamed.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_int,
                                               C.c_void_p]
+    L.vkm_c32_fold.restype = C.c_int
+    L.vkm_c32_fold.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
     L.vkm_wide_table_apply.restype = C.c_int
     L.vkm_wide_table_apply.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_void_p, C.c_uint64, C.c_void_p,
                                        C.c_void_p, C.POINTER(C.c_int32)]
@@ -229,3 +231,57 @@ def test_wide_table_matches_sequential_folds(model, case):
         # the residue matters: the folds from these starts do not all move by one offset
         ends = np.array([seq_fold(s, chain) for s in starts[:64]]) - starts[:64]
         assert len(set(ends.tolist())) > 1 and nres >= 4
+
+
+# ------------------------------------------------------------------ f32 chains
+
+def chains_oracle_f32(vals, acc):
+    """Sequential f32 chain folds: numpy's float32 cumulative sum adds left to right in f32."""
+    vals = np.asarray(vals, np.float32)
+    return [np.cumsum(np.concatenate([[np.float32(acc[j])], vals[j::16]]).astype(np.float32),
+                      dtype=np.float32)[-1] for j in range(16)]
+
+
+def f32_datasets():
+    rng = np.random.default_rng(3)
+    n = 16 * 1024 * 12 + 16 * 300  # 12 tiles and a partial one
+    f = np.float32
+    yield "uniform", rng.random(n).astype(f), np.zeros(16)
+    yield "uniform-from-large-start", rng.random(n).astype(f), np.full(16, 3.0e5)
+    yield "negative", (-rng.random(n)).astype(f), np.zeros(16)
+    yield "zero-mean", rng.standard_normal(n).astype(f), np.zeros(16)
+    v = rng.standard_normal(n).astype(f) * f(1e-3)
+    v[:: 997] = f(3.0e4)
+    yield "spikes", v, np.zeros(16)
+    # ties: halves and quarters against a running sum whose ulp is 1 or 2
+    t = (rng.integers(0, 8, n) * 0.25).astype(f)
+    yield "ties", t, np.full(16, 8388608.0)
+    w = rng.random(n).astype(f)
+    w[5000] = np.float32(np.inf)
+    w[90001] = np.float32(np.nan)
+    yield "inf-nan", w, np.zeros(16)
+    s = (rng.random(n) * 1e-38).astype(f)  # subnormal f32 values
+    yield "subnormal", s, np.zeros(16)
+    z = np.zeros(n, f)
+    z[1::2] = f(-0.0)
+    yield "signed-zeros", z, np.zeros(16)
+    big = rng.random(n).astype(f) * f(3e38)
+    yield "overflow", big, np.zeros(16)
+
+
+@pytest.mark.parametrize("name,vals,acc", list(f32_datasets()), ids=lambda p: p if isinstance(p, str) else "")
+def test_f32_streamed_model_is_exact(model, name, vals, acc):
+    a = np.array(acc, np.float64)
+    st = np.zeros(4, np.uint64)
+    v = np.ascontiguousarray(vals.astype(np.float64))
+    with np.errstate(all="ignore"):
+        want = chains_oracle_f32(vals, acc)
+    model.vkm_c32_fold(v.ctypes.data, v.size // 16, a.ctypes.data, st.ctypes.data)
+    got = a.astype(np.float32)
+    for j in range(16):
+        assert np.float32(got[j]).tobytes() == np.float32(want[j]).tobytes(), (name, j, got[j], want[j])
+        assert float(np.float32(a[j])) == a[j] or np.isnan(a[j])  # the state holds f32 values
+    if name in ("uniform", "uniform-from-large-start", "negative"):
+        assert st[0] > st[1], st  # most tiles apply their summaries
+    if name == "zero-mean":
+        assert st[1] > 0 and st[3] > 0, st  # evaluations, runs folded element by element

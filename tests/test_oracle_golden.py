@@ -136,6 +136,89 @@ def test_dot_sumsq_condsum_are_sums_of_row_values():
     assert bits(O.condsum_f64(x, "gt", 0.25, -7.0)) == bits(reference_sum(picked))
 
 
+def reference_sum_f32(vals, mul=False):
+    """The canonical order at f32 written bullet by bullet with numpy float32 scalars:
+    accumulator j takes rows j, j + 16, ... of the whole blocks, the accumulators combine
+    pairwise (+8, +4, +2, +1), the tail rows fold into accumulator 0 (reduce.rs:40-91)."""
+    f = np.float32
+    op = (lambda a, b: f(a * b)) if mul else (lambda a, b: f(a + b))
+    v = [f(x) for x in vals]
+    b = len(v) - len(v) % 16
+    acc = [f(1.0) if mul else f(0.0)] * 16
+    for i in range(b):
+        acc[i % 16] = op(acc[i % 16], v[i])
+    step = 8
+    while step:
+        for j in range(step):
+            acc[j] = op(acc[j], acc[j + step])
+        step //= 2
+    a = acc[0]
+    for x in v[b:]:
+        a = op(a, x)
+    return a
+
+
+def test_canonical_fold_f32_kinds():
+    # reduce.rs:40-91 and fused.rs:72-210 instantiated at f32 (Reducible, reduce.rs:567-575)
+    ys = [16777216.0, 0.5, 1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5, 8.5, 9.5, 10.5, 11.5, 12.5, 13.5, 14.5,
+          1.0, 0.25, 0.75, 1.25, 1.75, 2.25, 2.75, 3.25, 3.75, 4.25, 4.75, 5.25, 5.75, 6.25, 6.75, 7.25,
+          3.0, 3.0, 3.0]
+    assert fbits(O.canonical_fold_f32("sum", ys)) == 0x4B80005A  # tests.rs:761-791
+    rng = np.random.default_rng(5)
+    with np.errstate(all="ignore"):
+        for n in [0, 1, 15, 16, 17, 35, 100, 333]:
+            x = (rng.standard_normal(n) * 10.0 ** rng.integers(-3, 4, n)).astype(np.float32)
+            y = rng.random(n).astype(np.float32)
+            if n > 20:
+                x[3], x[7], x[19] = np.float32(1e30), np.float32(-0.0), np.float32(1e-40)
+            f = np.float32
+            assert fbits(O.canonical_fold_f32("sum", x)) == fbits(reference_sum_f32(x)), n
+            assert fbits(O.canonical_fold_f32("dot", x, y)) == fbits(
+                reference_sum_f32([f(a * b) for a, b in zip(x, y)])), n
+            assert fbits(O.canonical_fold_f32("sumsq", x)) == fbits(reference_sum_f32([f(a * a) for a in x])), n
+            picked = [a if a <= f(0.25) else f(-7.0) for a in x]
+            assert fbits(O.canonical_fold_f32("condsum", x, cmp="le", c=0.25, e=-7.0)) == fbits(
+                reference_sum_f32(picked)), n
+            z = (1.0 + rng.random(n) * 0.01).astype(np.float32)
+            assert fbits(O.canonical_fold_f32("mul", z)) == fbits(reference_sum_f32(z, mul=True)), n
+    # identities and a NaN
+    assert fbits(O.canonical_fold_f32("sum", [])) == 0
+    assert fbits(O.canonical_fold_f32("mul", [])) == fbits(1.0)
+    assert math.isnan(O.canonical_fold_f32("sum", [1.0, float("nan"), 2.0]))
+
+
+@pytest.mark.parametrize("dt", [np.int64, np.uint64, np.int32])
+def test_fused_integer_sums_wrap(dt):
+    # Dot, SumSq, CondSum (fused.rs:72-210) with IntSum (reduce.rs:325-377): wrapping products
+    # in the element type (elementwise.rs:380-397), widened, summed with wrapping; Python ints
+    w = np.dtype(dt).itemsize * 8
+    signed = np.dtype(dt).kind == "i"
+    rng = np.random.default_rng(9)
+    info = np.iinfo(dt)
+    x = rng.integers(info.min, info.max, 500, dtype=dt, endpoint=True)
+    y = rng.integers(info.min, info.max, 500, dtype=dt, endpoint=True)
+    x[:4] = [info.min, info.max, 0, 1 if not signed else -1]
+    sum_bits = 64
+
+    def elem(v):  # wrap to the element type
+        v &= (1 << w) - 1
+        return v - (1 << w) if signed and v >> (w - 1) else v
+
+    def total(vals):
+        s = sum(vals) & ((1 << sum_bits) - 1)
+        return s - (1 << 64) if signed and s >> 63 else s
+
+    xs, ys = [int(v) for v in x], [int(v) for v in y]
+    assert O.fused_sum_int("dot", x, y) == total(elem(a * b) for a, b in zip(xs, ys))
+    assert O.fused_sum_int("sumsq", x) == total(elem(a * a) for a in xs)
+    c = int(x[10])
+    e = int(info.min) if signed else int(info.max)
+    for cmp, fn in [("eq", lambda a: a == c), ("ne", lambda a: a != c), ("lt", lambda a: a < c),
+                    ("le", lambda a: a <= c), ("gt", lambda a: a > c), ("ge", lambda a: a >= c)]:
+        assert O.fused_sum_int("condsum", x, cmp=cmp, c=c, e=e) == total(a if fn(a) else e for a in xs), cmp
+    assert O.fused_sum_int("dot", x[:0], y[:0]) == 0
+
+
 # --------------------------------------------------- 1e6 / 1e7 goldens
 
 def test_readme_example_goldens():

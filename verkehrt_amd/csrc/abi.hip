@@ -26,7 +26,7 @@ struct In {
 };
 
 enum Impl {
-  R_CANON, R_SUM_INT, R_PROD_INT, R_COUNT_CMP, R_COUNT_BOOL, R_EXTREME, R_ARG, R_BITS,
+  R_CANON, R_SUM_INT, R_PROD_INT, R_COUNT_CMP, R_COUNT_BOOL, R_EXTREME, R_ARG, R_BITS, R_FSUM_INT,
   S_COMPRESS, S_FILTER, S_WHERE,
   SC_SCAN, SC_FSCAN, SC_PREV,
   W_GATHER, W_ARGSORT, W_GROUP_ID, W_GROUP_ID2, W_HISTOGRAM, W_REDUCE_INTO, W_RAND_F64, W_RAND_I64,
@@ -68,8 +68,10 @@ void build_table() {
   auto sum_t = [](int32_t t) { return t == VK_I32 ? VK_I64 : t == VK_U8 ? VK_U64 : t; };
 
   // reductions (src/kernels/scalar/reduce.rs:680-726)
-  add({"reduce_add_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 0});
-  add({"reduce_mul_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 4});
+  add({"reduce_add_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 0, 0, VK_F64});
+  add({"reduce_mul_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 4, 0, VK_F64});
+  add({"reduce_add_f32", VK_REDUCTION, {A(VK_F32)}, false, VK_F32, R_CANON, 0, 0, VK_F32});
+  add({"reduce_mul_f32", VK_REDUCTION, {A(VK_F32)}, false, VK_F32, R_CANON, 4, 0, VK_F32});
   for (int32_t t : ints) {
     add({"reduce_add_" + tn(t), VK_REDUCTION, {A(t)}, false, sum_t(t), R_SUM_INT, 0, 0, t});
     add({"reduce_mul_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_PROD_INT, 0, 0, t});
@@ -89,12 +91,23 @@ void build_table() {
   add({"count", VK_REDUCTION, {A(VK_BOOL)}, false, VK_I64, R_COUNT_BOOL});
   add({"any", VK_REDUCTION, {A(VK_BOOL)}, false, VK_BOOL, R_BITS, 1, 0, VK_BOOL});
   add({"all", VK_REDUCTION, {A(VK_BOOL)}, false, VK_BOOL, R_BITS, 0, 0, VK_BOOL});
-  // fused kernels (src/kernels/scalar/fused.rs:248-340)
-  add({"dot_f64", VK_REDUCTION, {A(VK_F64), A(VK_F64)}, false, VK_F64, R_CANON, 1});
-  add({"sumsq_f64", VK_REDUCTION, {A(VK_F64)}, false, VK_F64, R_CANON, 2});
+  // fused kernels (src/kernels/scalar/fused.rs:248-379): canonical float sums for f64 and f32,
+  // wrapping widened sums for i64, u64 and i32
+  for (int32_t t : {VK_F64, VK_F32}) {
+    add({"dot_" + tn(t), VK_REDUCTION, {A(t), A(t)}, false, t, R_CANON, 1, 0, t});
+    add({"sumsq_" + tn(t), VK_REDUCTION, {A(t)}, false, t, R_CANON, 2, 0, t});
+    for (int c = 0; c < 6; ++c)
+      add({std::string("condsum_") + kCmp[c] + "_" + tn(t), VK_REDUCTION, {A(t), S(t), S(t)}, false, t, R_CANON,
+           3, c, t});
+  }
+  for (int32_t t : {VK_I64, VK_U64, VK_I32}) {
+    add({"dot_" + tn(t), VK_REDUCTION, {A(t), A(t)}, false, sum_t(t), R_FSUM_INT, 1, 0, t});
+    add({"sumsq_" + tn(t), VK_REDUCTION, {A(t)}, false, sum_t(t), R_FSUM_INT, 2, 0, t});
+    for (int c = 0; c < 6; ++c)
+      add({std::string("condsum_") + kCmp[c] + "_" + tn(t), VK_REDUCTION, {A(t), S(t), S(t)}, false, sum_t(t),
+           R_FSUM_INT, 3, c, t});
+  }
   for (int c = 0; c < 6; ++c) {
-    add({std::string("condsum_") + kCmp[c] + "_f64", VK_REDUCTION, {A(VK_F64), S(VK_F64), S(VK_F64)},
-         false, VK_F64, R_CANON, 3, c});
     for (int32_t t : {VK_F64, VK_F32, VK_I64, VK_U64, VK_I32}) {
       add({std::string("count_") + kCmp[c] + "_" + tn(t), VK_REDUCTION, {A(t), S(t)}, false, VK_I64,
            R_COUNT_CMP, c, 0, t});
@@ -339,6 +352,7 @@ ReduceSpec reduce_spec(const Def &d, uint64_t base) {
     case R_COUNT_BOOL: s.kind = ReduceSpec::COUNT_BOOL; s.type = VK_U8; break;
     case R_EXTREME: s.kind = ReduceSpec::EXTREME; s.want_max = d.p0 != 0; break;
     case R_ARG: s.kind = ReduceSpec::ARG_EXTREME; s.want_max = d.p0 != 0; break;
+    case R_FSUM_INT: s.kind = ReduceSpec::FSUM_INT; s.fkind = d.p0; s.cmp = d.p1; break;
     default: s.kind = ReduceSpec::BITS; s.bitop = d.p0; break;
   }
   return s;
@@ -360,32 +374,51 @@ void reducer_update(vk_reducer *r, const vk_arg *args, size_t nargs) {
   const Def &d = g_defs[r->def];
   if (d.impl == R_CANON) {
     const int kind = d.p0;
+    const int32_t ft = d.t0;  // VK_F64 or VK_F32
     const vk_array *x = nullptr, *y = nullptr;
     CanonSpec cs{kind, d.p1, 0, 0};
+    auto splat_val = [&](const vk_arg &a) {
+      const uint64_t b = splat_arg(a, ft);
+      if (ft == VK_F32) {
+        float f;
+        const uint32_t w = static_cast<uint32_t>(b);
+        std::memcpy(&f, &w, 4);
+        return static_cast<double>(f);
+      }
+      double v;
+      std::memcpy(&v, &b, 8);
+      return v;
+    };
     if (kind == 1) {
       if (nargs != 2) arity(2, nargs);
-      x = rows_arg(args[0], VK_F64);
-      y = rows_arg(args[1], VK_F64);
+      x = rows_arg(args[0], ft);
+      y = rows_arg(args[1], ft);
       if (x->len != y->len)
         usage_error("length mismatch: " + std::to_string(x->len) + " vs " + std::to_string(y->len));
     } else if (kind == 3) {
       if (nargs != 3) arity(3, nargs);
-      x = rows_arg(args[0], VK_F64);
-      uint64_t cb = splat_arg(args[1], VK_F64), eb = splat_arg(args[2], VK_F64);
-      std::memcpy(&cs.c, &cb, 8);
-      std::memcpy(&cs.e, &eb, 8);
+      x = rows_arg(args[0], ft);
+      cs.c = splat_val(args[1]);
+      cs.e = splat_val(args[2]);
     } else {
       if (nargs != 1) arity(1, nargs);
-      x = rows_arg(args[0], VK_F64);
+      x = rows_arg(args[0], ft);
     }
     // CanonicalAcc::feed_blocks (reduce.rs:284-308)
     if (r->tail_len != 0) runtime_error("canonical update after a partial block");
     const uint64_t n = x->len, blocks = n / 16, rest = n % 16;
-    const double *xp = static_cast<const double *>(dptr(x));
-    const double *yp = y ? static_cast<const double *>(dptr(y)) : nullptr;
     double *acc = static_cast<double *>(r->state);
-    canon_blocks(c, cs, xp, yp, blocks, acc);
-    canon_rest(c, cs, xp + blocks * 16, yp ? yp + blocks * 16 : nullptr, rest, acc + 16);
+    if (ft == VK_F32) {
+      const float *xp = static_cast<const float *>(dptr(x));
+      const float *yp = y ? static_cast<const float *>(dptr(y)) : nullptr;
+      canon32_blocks(c, cs, xp, yp, blocks, acc);
+      canon32_rest(c, cs, xp + blocks * 16, yp ? yp + blocks * 16 : nullptr, rest, acc + 16);
+    } else {
+      const double *xp = static_cast<const double *>(dptr(x));
+      const double *yp = y ? static_cast<const double *>(dptr(y)) : nullptr;
+      canon_blocks(c, cs, xp, yp, blocks, acc);
+      canon_rest(c, cs, xp + blocks * 16, yp ? yp + blocks * 16 : nullptr, rest, acc + 16);
+    }
     r->tail_len = static_cast<int>(rest);
     r->consumed += n;
     return;
@@ -397,6 +430,26 @@ void reducer_update(vk_reducer *r, const vk_arg *args, size_t nargs) {
     ReduceSpec s = reduce_spec(d, r->consumed);
     s.cbits = splat_arg(args[1], d.t0);
     reduce_rows(c, s, dptr(x), x->len, r->state);
+  } else if (d.impl == R_FSUM_INT) {
+    ReduceSpec s = reduce_spec(d, r->consumed);
+    if (d.p0 == 1) {  // Dot::update (fused.rs:77-98)
+      if (nargs != 2) arity(2, nargs);
+      x = rows_arg(args[0], d.t0);
+      const vk_array *y = rows_arg(args[1], d.t0);
+      if (x->len != y->len)
+        usage_error("length mismatch: " + std::to_string(x->len) + " vs " + std::to_string(y->len));
+      reduce_rows2(c, s, dptr(x), dptr(y), x->len, r->state);
+    } else if (d.p0 == 3) {  // CondSum::update (fused.rs:186-203)
+      if (nargs != 3) arity(3, nargs);
+      x = rows_arg(args[0], d.t0);
+      s.cbits = splat_arg(args[1], d.t0);
+      s.ebits = splat_arg(args[2], d.t0);
+      reduce_rows(c, s, dptr(x), x->len, r->state);
+    } else {  // SumSq::update (fused.rs:113-131)
+      if (nargs != 1) arity(1, nargs);
+      x = rows_arg(args[0], d.t0);
+      reduce_rows(c, s, dptr(x), x->len, r->state);
+    }
   } else {
     if (nargs != 1) arity(1, nargs);
     x = rows_arg(args[0], d.in[0].t);
@@ -433,10 +486,13 @@ vk_scalar reducer_finish(vk_reducer *r) {
   const Def &d = g_defs[r->def];
   if (d.impl == R_CANON) {
     double *acc = static_cast<double *>(r->state);
-    canon_finish(c, d.p0 == 4, acc, acc + 16, r->tail_len, reinterpret_cast<double *>(c.scratch.slots));
+    if (d.t0 == VK_F32)
+      canon32_finish(c, d.p0 == 4, acc, acc + 16, r->tail_len, reinterpret_cast<uint64_t *>(c.scratch.slots));
+    else
+      canon_finish(c, d.p0 == 4, acc, acc + 16, r->tail_len, reinterpret_cast<double *>(c.scratch.slots));
     uint64_t b;
     read_back(c, c.scratch.slots, 8, &b);
-    return scalar(VK_F64, b);
+    return scalar(d.t0, b);
   }
   uint64_t w[3];
   read_back(c, r->state, 24, w);
@@ -936,7 +992,7 @@ bool region_compile(const vk_rkernel *ks, size_t nk, const char *root, const vk_
   if (rd.impl == R_COUNT_BOOL) {
     if (!count_cmp) return false;
   } else {
-    if (rd.t0 != T && !(rd.impl == R_CANON && T == VK_F64)) return false;
+    if (rd.t0 != T) return false;
     if (rd.impl == R_CANON) {
       p.canon = true;
       return T == VK_F64;
@@ -1162,6 +1218,11 @@ std::vector<vk_value> batch_call_all(Ctx &c, const Def &d, const vk_arg *args, s
       out[i].array = new vk_array{1, buf->buf, buf->offset + off[i] * w, got[i], d.out_t};
     }
     release(buf);
+    return out;
+  }
+  if (d.impl == R_FSUM_INT || (d.impl == R_CANON && (d.p0 == 4 || d.t0 != VK_F64))) {
+    // no batched form: integer dot/sumsq/condsum and the f32 canonical folds run item by item
+    each();
     return out;
   }
   if (d.impl == R_CANON && d.p0 != 4) {
@@ -1603,6 +1664,8 @@ void plan_add(vk_bplan &p, const char *name, const vk_bref *args, size_t nargs) 
       s.ss.kind = 0;
       s.ss.type = d.t0;
     }
+  } else if (d.cls == VK_REDUCTION && (d.impl == R_FSUM_INT || (d.impl == R_CANON && d.t0 != VK_F64))) {
+    usage_error(d.name + " cannot be planned (f64 canonical sums, selections and the other reductions can)");
   } else if (d.cls == VK_REDUCTION && d.impl == R_CANON && d.p0 != 4) {
     s.kind = 1;
     const RowsRef r0 = rows_type(0);

@@ -25,6 +25,22 @@
 
 namespace vk {
 
+// The precision of the chain being evaluated. Values are held in doubles either way; an f32
+// chain's values are f32 values, and its adds round to f32 (rnd). For a sum of two f32 values
+// the double sum rounded to f32 equals the f32 sum: a double has 53 >= 2 * 24 + 2 bits, which
+// makes the double rounding innocuous. u = 2^(e - kMant) is the grid of binade e; binades with
+// e outside [kEmin, kEmax] get no hypothesis (their chains are folded sequentially).
+struct PrecF64 {
+  static constexpr int kMant = 52, kEmin = -960, kEmax = 960;
+  __host__ __device__ static __forceinline__ double rnd(double v) { return v; }
+};
+struct PrecF32 {
+  static constexpr int kMant = 23, kEmin = -100, kEmax = 100;
+  __host__ __device__ static __forceinline__ double rnd(double v) {
+    return static_cast<double>(static_cast<float>(v));
+  }
+};
+
 // A binade hypothesis: exponent e of the running value and its sign. `ok` false means no
 // hypothesis (the run is folded sequentially).
 struct Binade {
@@ -59,13 +75,14 @@ __host__ __device__ __forceinline__ uint64_t dbits_(double v) {
 
 // The hypothesis of a running value near `approx`; none for zero, subnormal, non-finite or
 // extreme magnitudes (where u or 1/u would leave the normal range).
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ Binade binade_of(double approx) {
   Binade h{0, 0, 0, 0};
   const uint64_t b = dbits_(approx);
   const int ex = static_cast<int>((b >> 52) & 0x7FF);
   if (ex == 0 || ex == 0x7FF) return h;
   const int e = ex - 1023;
-  if (e < -960 || e > 960) return h;
+  if (e < P::kEmin || e > P::kEmax) return h;
   h.e = e;
   h.neg = static_cast<int>(b >> 63);
   h.ok = 1;
@@ -95,11 +112,12 @@ struct BinadeRef {
   double half_u;   // u / 2
 };
 
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ BinadeRef binade_ref(const Binade &h) {
   const double two_e = pow2(h.e);
-  const double u = pow2(h.e - 52);
+  const double u = pow2(h.e - P::kMant);
   const double z0 = (h.neg ? -1.5 : 1.5) * two_e;
-  return BinadeRef{z0, z0 + u, pow2(52 - h.e), 0.5 * two_e, pow2(h.e - 53)};
+  return BinadeRef{z0, z0 + u, pow2(P::kMant - h.e), 0.5 * two_e, pow2(h.e - P::kMant - 1)};
 }
 
 // A run summary being built: the summary so far, the current parity for each start parity
@@ -115,9 +133,10 @@ struct RunAcc {
 
 __host__ __device__ __forceinline__ RunAcc run_start() { return RunAcc{run_identity(), {false, true}, false}; }
 
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ void run_push(RunAcc &a, const BinadeRef &r, double v) {
-  const double a0 = ((r.z0 + v) - r.z0) * r.inv_u;  // rounding from an even value
-  const double a1 = ((r.z1 + v) - r.z1) * r.inv_u;  // from an odd one
+  const double a0 = (P::rnd(r.z0 + v) - r.z0) * r.inv_u;  // rounding from an even value
+  const double a1 = (P::rnd(r.z1 + v) - r.z1) * r.inv_u;  // from an odd one
   const bool p0 = parity(a0) != 0, p1 = parity(a1) != 0;
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
@@ -150,14 +169,14 @@ __host__ __device__ __forceinline__ RunFn run_end(const RunAcc &a) {
 // rounds to a multiple of u and fl(Z0 + v) - Z0 - v is exact by Fast2Sum, |Z0| > |v|) makes
 // the result false: the run needs the two-track form (run_of_two). NaN and infinite elements
 // give no tie but make the run bad either way.
-template <int N>
+template <int N, class P = PrecF64>
 __host__ __device__ __forceinline__ bool run_of_one(const double (&v)[N], int cnt, const BinadeRef &r, RunFn *f,
                                                     double *asum) {
   const double inf = __builtin_inf();
   double d = 0.0, lo = inf, hi = -inf, a = 0.0;
   bool tie = false, bad = false;
   auto one = [&](double x) {
-    const double w = (r.z0 + x) - r.z0;
+    const double w = P::rnd(r.z0 + x) - r.z0;
     const double e = x - w;
     tie = tie || (e < 0 ? -e : e) == r.half_u;
     d = d + w * r.inv_u;
@@ -182,22 +201,22 @@ __host__ __device__ __forceinline__ bool run_of_one(const double (&v)[N], int cn
 }
 
 // The two-track summary (start parities 0 and 1 followed separately) of the first cnt elements.
-template <int N>
+template <int N, class P = PrecF64>
 __host__ __device__ __forceinline__ RunFn run_of_two(const double (&v)[N], int cnt, const BinadeRef &r) {
   RunAcc a = run_start();
 #pragma unroll
   for (int k = 0; k < N; ++k)
-    if (k < cnt) run_push(a, r, v[k]);
+    if (k < cnt) run_push<P>(a, r, v[k]);
   return run_end(a);
 }
 
 // The summary of the first cnt (at most N) of the elements v.
-template <int N>
+template <int N, class P = PrecF64>
 __host__ __device__ __forceinline__ RunFn run_of(const double (&v)[N], int cnt, const BinadeRef &r) {
   RunFn f;
   double a;
-  if (__builtin_expect(run_of_one(v, cnt, r, &f, &a), 1)) return f;
-  return run_of_two(v, cnt, r);
+  if (__builtin_expect((run_of_one<N, P>(v, cnt, r, &f, &a)), 1)) return f;
+  return run_of_two<N, P>(v, cnt, r);
 }
 
 // The summary of run a followed by run b (both under the same hypothesis).
@@ -216,26 +235,27 @@ __host__ __device__ __forceinline__ RunFn run_join(const RunFn &a, const RunFn &
 
 // Applies run f (under hypothesis h) to the exact start value s. Returns true and the exact
 // end value in *out when the summary proves the sequential fold equals it.
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ bool run_apply(const RunFn &f, const Binade &h, double s,
                                                    double *out) {
   if (!h.ok) return false;
-  const Binade hs = binade_of(s);
+  const Binade hs = binade_of<P>(s);
   if (!same_binade(hs, h)) return false;
-  const double inv_u = pow2(52 - h.e);
-  const double k = s * inv_u;  // exact integer, 2^52 <= |k| < 2^53
+  const double inv_u = pow2(P::kMant - h.e);
+  const double k = s * inv_u;  // exact integer, 2^kMant <= |k| < 2^(kMant + 1)
   const int p = parity(k);
-  const double two52 = 4503599627370496.0, two53 = 9007199254740992.0;
+  const double two_m = pow2(P::kMant), two_m1 = pow2(P::kMant + 1);
   const double lo = k + f.lo[p], hi = k + f.hi[p];
   bool ok;
-  if (!h.neg) ok = lo >= two52 + 1.0 && hi <= two53 - 1.0;
-  else ok = hi <= -(two52 + 1.0) && lo >= -(two53 - 1.0);
+  if (!h.neg) ok = lo >= two_m + 1.0 && hi <= two_m1 - 1.0;
+  else ok = hi <= -(two_m + 1.0) && lo >= -(two_m1 - 1.0);
   // an empty run (lo = +inf, hi = -inf) leaves s unchanged
   if (f.lo[p] > f.hi[p]) {
     *out = s;
     return true;
   }
   if (!ok) return false;
-  *out = (k + f.d[p]) * pow2(h.e - 52);
+  *out = (k + f.d[p]) * pow2(h.e - P::kMant);
   return true;
 }
 

@@ -1,0 +1,399 @@
+// Canonical-order f32 reductions: reduce_add_f32 / reduce_mul_f32 (CanonicalAcc instantiated
+// at f32, src/kernels/scalar/reduce.rs:249-323 and 567-575), dot_f32, sumsq_f32 and
+// condsum_<cmp>_f32 (fused.rs:72-210 with T = f32). Accumulator j folds rows j, j + 16, ... in
+// index order; every add (or multiply) rounds to f32, and there is no FMA (built with
+// -ffp-contract=off).
+//
+// Values are carried in doubles that hold f32 values exactly; a chain step rounds through
+// PrecF32::rnd, which equals the f32 operation (canon_binade.h). Products of two f32 values are
+// exact in a double, so rounding them once to f32 is the f32 product.
+//
+// Two forms:
+//  - sequential: one wave, lane j < 16 owning accumulator j (products, and inputs of fewer
+//    than kC32MinBlocks blocks of 16 rows);
+//  - streamed (sums): tiles of 16384 rows (1024 blocks). (1) the approximate sum of every tile
+//    chain, (2) their prefix per chain: the approximate start of every tile chain, (3) every
+//    tile chain summarised under the binade of its approximate start (run summaries of
+//    canon_binade.h at f32 precision, joined in order), (4) one wave per chain walks the tiles
+//    from the exact start, 64 tiles at a time: the summaries of the tiles whose hypothesis is the
+//    running value's binade are joined by a lane scan and applied; a tile whose summary does not
+//    apply is evaluated from its rows (rounds of run summaries, or the plain sequential fold
+//    when the running value keeps changing binade). Every applied summary proves its result, so
+//    the result is the sequential one.
+// HBM traffic: two reads of the rows (4 bytes each, 8 for dot) plus 64 bytes per tile chain.
+#include "canon_binade.h"
+#include "vk_device.h"
+#include "vk_internal.h"
+
+namespace vk {
+namespace {
+
+// Row i's value in a canonical f32 reduction of kind K: x (0: sums, and the factors of
+// products), x * y (1: dot), x * x (2: sumsq), x where its comparison with c holds, else e
+// (3: condsum); the f32 result held in a double.
+template <int K>
+struct Row32 {
+  float c, e;
+  uint32_t keep;  // condsum: the comparison outcomes (outcome()) that keep x
+  __device__ __forceinline__ double operator()(const float *x, const float *y, uint64_t i) const {
+    if constexpr (K == 1) {
+      const float p = x[i] * y[i];
+      return p;
+    } else if constexpr (K == 2) {
+      const float v = x[i];
+      const float p = v * v;
+      return p;
+    } else if constexpr (K == 3) {
+      const float v = x[i];
+      return (keep & outcome(v, c)) ? v : e;
+    } else {
+      return x[i];
+    }
+  }
+};
+
+__device__ __forceinline__ double r32(double v) { return PrecF32::rnd(v); }
+
+constexpr int kC32Threads = 1024;
+constexpr int kC32Run = 16;                                   // blocks per run
+constexpr uint64_t kC32TileBlocks = kC32Threads / 16 * kC32Run;  // 1024 blocks, 16384 rows
+constexpr uint64_t kC32MinBlocks = 2 * kC32TileBlocks;
+
+struct TileFn32 {
+  double d0, d1, lo0, lo1, hi0, hi1;
+  uint64_t hword;  // the hypothesis (pack below)
+  uint64_t pad;
+};
+
+__device__ __forceinline__ uint64_t pack_h(const Binade &h) {
+  return (static_cast<uint64_t>(static_cast<uint32_t>(h.e + 2048)) << 2) | (uint64_t(h.neg) << 1) | uint64_t(h.ok);
+}
+__device__ __forceinline__ Binade unpack_h(uint64_t w) {
+  return Binade{static_cast<int>(w >> 2) - 2048, static_cast<int>((w >> 1) & 1), static_cast<int>(w & 1), 0};
+}
+__device__ __forceinline__ RunFn run_bad() {
+  const double inf = __builtin_inf();
+  return RunFn{{0.0, 0.0}, {-inf, -inf}, {inf, inf}};
+}
+
+// acc[16] <- fold of rows [0, 16 * blocks) in the canonical order, one wave (as k_canon_chain).
+constexpr int kGroup32 = 8;
+template <class RV>
+__global__ __launch_bounds__(64) void k_c32_chain(RV rv, bool mul, const float *__restrict__ x,
+                                                   const float *__restrict__ y, uint64_t blocks, double *acc) {
+  const int l = lane_id();
+  const uint64_t rows = blocks * 16;
+  double a = l < 16 ? acc[l] : 0.0;
+  const uint64_t chunks = (rows + 63) / 64;
+  const uint64_t groups = (chunks + kGroup32 - 1) / kGroup32;
+  if (groups == 0) return;
+  double cur[kGroup32], nxt[kGroup32];
+  auto load = [&](double *v, uint64_t g) {
+#pragma unroll
+    for (int k = 0; k < kGroup32; ++k) {
+      const uint64_t r = (g * kGroup32 + k) * 64 + l;
+      v[k] = r < rows ? rv(x, y, r) : 0.0;
+    }
+  };
+  load(cur, 0);
+  for (uint64_t g = 0; g < groups; ++g) {
+    if (g + 1 < groups) load(nxt, g + 1);
+#pragma unroll
+    for (int k = 0; k < kGroup32; ++k) {
+      const uint64_t base = (g * kGroup32 + k) * 64;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double v = __shfl(cur[k], q * 16 + (l & 15));
+        if (l < 16 && base + q * 16 < rows) a = mul ? r32(a * v) : r32(a + v);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kGroup32; ++k) cur[k] = nxt[k];
+  }
+  if (l < 16) acc[l] = a;
+}
+
+// Thread t of a tile: chain j = t % 16, run r = t / 16 (blocks 16 r .. 16 r + 15 of the tile).
+template <class RV>
+__device__ __forceinline__ void load_run32(const RV &rv, const float *x, const float *y, uint64_t b0, int nb,
+                                           int r, int j, double (&v)[kC32Run]) {
+#pragma unroll
+  for (int k = 0; k < kC32Run; ++k) {
+    const int lb = r * kC32Run + k;
+    v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
+  }
+}
+
+// (1) asum[tile * 16 + j]: the approximate sum of chain j's rows in the tile.
+template <class RV>
+__global__ __launch_bounds__(kC32Threads) void k_c32_asum(RV rv, const float *__restrict__ x,
+                                                           const float *__restrict__ y, uint64_t blocks,
+                                                           double *asum) {
+  const uint64_t tile = blockIdx.x;
+  const uint64_t b0 = tile * kC32TileBlocks;
+  const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
+  const int t = threadIdx.x, j = t & 15, r = t >> 4, w = t / kWave, l = lane_id();
+  double v[kC32Run];
+  load_run32(rv, x, y, b0, nb, r, j, v);
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kC32Run; ++k) s = s + v[k];
+  s = s + __shfl_xor(s, 16);
+  s = s + __shfl_xor(s, 32);
+  __shared__ double part[kC32Threads / kWave][16];
+  if (l < 16) part[w][j] = s;
+  __syncthreads();
+  if (t < 16) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < kC32Threads / kWave; ++q) a = a + part[q][t];
+    asum[tile * 16 + t] = a;
+  }
+}
+
+// (2) pre[tile * 16 + j] = acc[j] + the approximate sums of chain j's earlier tiles; one block
+// per chain, each thread a contiguous range of tiles.
+__global__ __launch_bounds__(kC32Threads) void k_c32_prefix(const double *asum, uint64_t tiles, const double *acc,
+                                                             double *pre) {
+  const int j = blockIdx.x, t = threadIdx.x;
+  const uint64_t per = (tiles + kC32Threads - 1) / kC32Threads;
+  const uint64_t a = t * per < tiles ? t * per : tiles, b = a + per < tiles ? a + per : tiles;
+  double s = 0.0;
+  for (uint64_t q = a; q < b; ++q) s = s + asum[q * 16 + j];
+  __shared__ double sc[kC32Threads];
+  sc[t] = s;
+  __syncthreads();
+  for (int d = 1; d < kC32Threads; d <<= 1) {
+    const double o = t >= d ? sc[t - d] : 0.0;
+    __syncthreads();
+    sc[t] = sc[t] + o;
+    __syncthreads();
+  }
+  double run = acc[j] + (sc[t] - s);  // exclusive
+  for (uint64_t q = a; q < b; ++q) {
+    pre[q * 16 + j] = run;
+    run = run + asum[q * 16 + j];
+  }
+}
+
+// (3) fns[tile * 16 + j]: chain j's tile summary under the binade of its approximate start.
+template <class RV>
+__global__ __launch_bounds__(kC32Threads) void k_c32_summ(RV rv, const float *__restrict__ x,
+                                                           const float *__restrict__ y, uint64_t blocks,
+                                                           const double *pre, TileFn32 *fns) {
+  const uint64_t tile = blockIdx.x;
+  const uint64_t b0 = tile * kC32TileBlocks;
+  const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
+  const int t = threadIdx.x, j = t & 15, r = t >> 4, w = t / kWave, l = lane_id();
+  double v[kC32Run];
+  load_run32(rv, x, y, b0, nb, r, j, v);
+  const Binade h = binade_of<PrecF32>(pre[tile * 16 + j]);
+  RunFn f = run_bad();
+  if (h.ok) {
+    const int cnt = nb - r * kC32Run;
+    f = run_of<kC32Run, PrecF32>(v, cnt < 0 ? 0 : cnt, binade_ref<PrecF32>(h));
+  }
+  // lanes j, 16 + j, 32 + j, 48 + j hold runs 4w .. 4w + 3 of chain j: join them in order
+  RunFn o = shfl_xor_pod(f, 16);
+  f = (l & 16) ? run_join(o, f) : run_join(f, o);
+  o = shfl_xor_pod(f, 32);
+  f = (l & 32) ? run_join(o, f) : run_join(f, o);
+  __shared__ RunFn s_f[kC32Threads / kWave][16];
+  if (l < 16) s_f[w][j] = f;
+  __syncthreads();
+  if (t < 16) {
+    RunFn g = s_f[0][t];
+    for (int q = 1; q < kC32Threads / kWave; ++q) g = run_join(g, s_f[q][t]);
+    TileFn32 out{g.d[0], g.d[1], g.lo[0], g.lo[1], g.hi[0], g.hi[1], pack_h(h), 0};
+    fns[tile * 16 + t] = out;
+  }
+}
+
+// The exact fold of chain j's rows in blocks b0 .. b0 + nb - 1 (nb <= 1024) from the exact value
+// s, by one wave: lane r holds run r. Few binade changes among the approximate run starts: rounds
+// (runs summarised under the current value's binade and scanned; the runs before the first
+// failing one applied at once, that run folded element by element). Many changes: the
+// sequential fold itself, each element read from its run's lane.
+template <class RV>
+__device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint64_t b0, int nb, int j,
+                              double s) {
+  const int l = lane_id();
+  double v[kC32Run];
+  double asum = 0.0;
+#pragma unroll
+  for (int k = 0; k < kC32Run; ++k) {
+    const int lb = l * kC32Run + k;
+    v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
+    asum = asum + v[k];
+  }
+  const int nruns = (nb + kC32Run - 1) / kC32Run;
+  double inc = asum;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double o = __shfl_up(inc, d);
+    if (l >= d) inc = inc + o;
+  }
+  const Binade ha = binade_of<PrecF32>(s + (inc - asum));
+  const Binade hp{__shfl_up(ha.e, 1), __shfl_up(ha.neg, 1), __shfl_up(ha.ok, 1), 0};
+  const uint64_t changes = __ballot(l > 0 && l < nruns && !same_binade(ha, hp));
+  double cur = s;
+  if (__popcll(changes) <= 2) {
+    for (int r0 = 0; r0 < nruns;) {
+      const Binade h = binade_of<PrecF32>(cur);
+      RunFn f = run_identity();
+      if (h.ok && l >= r0) {
+        const int cnt = nb - l * kC32Run;
+        f = run_of<kC32Run, PrecF32>(v, cnt < 0 ? 0 : cnt, binade_ref<PrecF32>(h));
+      }
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const RunFn o = shfl_up_pod(f, d);
+        if (l >= d) f = run_join(o, f);
+      }
+      double val = cur;
+      const bool ok = l < r0 || run_apply<PrecF32>(f, h, cur, &val);
+      const uint64_t fails = __ballot(!ok);
+      if (fails == 0) {
+        cur = __shfl(val, kWave - 1);
+        break;
+      }
+      const int m = __builtin_ctzll(fails);
+      double t = m > r0 ? __shfl(val, m - 1) : cur;
+      if (l == m) {
+#pragma unroll
+        for (int k = 0; k < kC32Run; ++k)
+          if (m * kC32Run + k < nb) t = r32(t + v[k]);
+      }
+      cur = __shfl(t, m);
+      r0 = m + 1;
+    }
+    return cur;
+  }
+  for (int r = 0; r < nruns; ++r) {
+    const int nk = nb - r * kC32Run < kC32Run ? nb - r * kC32Run : kC32Run;
+#pragma unroll
+    for (int k = 0; k < kC32Run; ++k)
+      if (k < nk) cur = r32(cur + bcast_pod(v[k], r));
+  }
+  return cur;
+}
+
+// (4) One wave per chain: acc[j] <- the exact fold of chain j over every tile.
+template <class RV>
+__global__ __launch_bounds__(kWave) void k_c32_walk(RV rv, const float *__restrict__ x,
+                                                     const float *__restrict__ y, uint64_t blocks,
+                                                     const TileFn32 *fns, uint64_t tiles, double *acc) {
+  const int j = blockIdx.x, l = lane_id();
+  double s = acc[j];
+  uint64_t t0 = 0;
+  while (t0 < tiles) {
+    const uint64_t tt = t0 + l;
+    const bool valid = tt < tiles;
+    RunFn f = run_bad();
+    Binade h{0, 0, 0, 0};
+    if (valid) {
+      const TileFn32 q = fns[tt * 16 + j];
+      f = RunFn{{q.d0, q.d1}, {q.lo0, q.lo1}, {q.hi0, q.hi1}};
+      h = unpack_h(q.hword);
+    }
+    const Binade hs = binade_of<PrecF32>(s);
+    const bool same = valid && same_binade(h, hs);
+    if (!same) f = run_bad();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const RunFn o = shfl_up_pod(f, d);
+      if (l >= d) f = run_join(o, f);
+    }
+    double val = s;
+    const bool ok = same && run_apply<PrecF32>(f, hs, s, &val);
+    const uint64_t fails = __ballot(!ok);
+    const int m = fails ? __builtin_ctzll(fails) : kWave;
+    if (m > 0) {
+      s = __shfl(val, m - 1);
+      t0 += uint64_t(m);
+      continue;
+    }
+    // tile t0's summary does not apply from s: evaluate the tile from its rows
+    const uint64_t b0 = t0 * kC32TileBlocks;
+    const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
+    s = eval_tile32(rv, x, y, b0, nb, j, s);
+    t0 += 1;
+  }
+  if (l == 0) acc[j] = s;
+}
+
+template <class RV>
+__global__ void k_c32_rest(RV rv, const float *x, const float *y, uint64_t rest, double *tail) {
+  const uint64_t i = threadIdx.x;
+  if (i < rest) tail[i] = rv(x, y, i);
+}
+
+// finish_lanes (reduce.rs:75-91) at f32: pairwise +8, +4, +2, +1, then the tail into acc 0;
+// out receives the f32 bits.
+__global__ void k_c32_finish(bool mul, const double *acc_in, const double *tail, int tail_len, uint64_t *out) {
+  if (threadIdx.x != 0) return;
+  double acc[16];
+  for (int j = 0; j < 16; ++j) acc[j] = acc_in[j];
+  for (int step = 8; step > 0; step /= 2)
+    for (int j = 0; j < step; ++j) acc[j] = mul ? r32(acc[j] * acc[j + step]) : r32(acc[j] + acc[j + step]);
+  double a = acc[0];
+  for (int j = 0; j < tail_len; ++j) a = mul ? r32(a * tail[j]) : r32(a + tail[j]);
+  const float f = static_cast<float>(a);
+  uint32_t b;
+  __builtin_memcpy(&b, &f, 4);
+  *out = b;
+}
+
+template <class F>
+void with_row32(const CanonSpec &s, F &&f) {
+  const float c = static_cast<float>(s.c), e = static_cast<float>(s.e);
+  switch (s.kind) {
+    case 1: f(Row32<1>{c, e, 0}); break;
+    case 2: f(Row32<2>{c, e, 0}); break;
+    case 3: f(Row32<3>{c, e, keepset_of(s.cmp)}); break;
+    default: f(Row32<0>{c, e, 0}); break;
+  }
+}
+
+template <class RV>
+void c32_blocks_as(Ctx &c, RV rv, bool mul, const float *x, const float *y, uint64_t blocks, double *acc) {
+  if (mul || blocks < kC32MinBlocks) {
+    k_c32_chain<<<1, 64, 0, c.stream>>>(rv, mul, x, y, blocks, acc);
+    check_hip(hipGetLastError(), "canon f32 chain");
+    return;
+  }
+  const uint64_t tiles = (blocks + kC32TileBlocks - 1) / kC32TileBlocks;
+  if (tiles > 0xFFFFFFFFull) usage_error("reduction too large");
+  double *asum = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
+  double *pre = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
+  auto *fns = static_cast<TileFn32 *>(tmp_alloc(c, tiles * 16 * sizeof(TileFn32)));
+  const uint32_t grid = static_cast<uint32_t>(tiles);
+  k_c32_asum<RV><<<grid, kC32Threads, 0, c.stream>>>(rv, x, y, blocks, asum);
+  k_c32_prefix<<<16, kC32Threads, 0, c.stream>>>(asum, tiles, acc, pre);
+  k_c32_summ<RV><<<grid, kC32Threads, 0, c.stream>>>(rv, x, y, blocks, pre, fns);
+  k_c32_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, fns, tiles, acc);
+  check_hip(hipGetLastError(), "canon f32 streamed");
+  tmp_free(c, asum);
+  tmp_free(c, pre);
+  tmp_free(c, fns);
+}
+
+}  // namespace
+
+void canon32_blocks(Ctx &c, const CanonSpec &s, const float *x, const float *y, uint64_t blocks, double *acc) {
+  if (blocks == 0) return;
+  with_row32(s, [&](auto rv) { c32_blocks_as(c, rv, s.kind == 4, x, y ? y : x, blocks, acc); });
+}
+
+void canon32_rest(Ctx &c, const CanonSpec &s, const float *x, const float *y, uint64_t rest, double *tail) {
+  if (rest == 0) return;
+  with_row32(s, [&](auto rv) { k_c32_rest<<<1, 64, 0, c.stream>>>(rv, x, y ? y : x, rest, tail); });
+  check_hip(hipGetLastError(), "canon f32 rest");
+}
+
+void canon32_finish(Ctx &c, bool mul, const double *acc, const double *tail, int tail_len, uint64_t *out) {
+  k_c32_finish<<<1, 64, 0, c.stream>>>(mul, acc, tail, tail_len, out);
+  check_hip(hipGetLastError(), "canon f32 finish");
+}
+
+}  // namespace vk

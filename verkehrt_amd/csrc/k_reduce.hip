@@ -99,6 +99,33 @@ struct OpProd {
   __device__ void merge(Acc &a, const Acc &b) const { a.s *= b.s; }
 };
 
+// Num::mul over integers (elementwise.rs:380-397): the wrapping product in the element type.
+template <typename T>
+__device__ __forceinline__ T wrap_mul(T a, T b) {
+  using U = typename std::make_unsigned<T>::type;
+  return static_cast<T>(static_cast<U>(static_cast<U>(a) * static_cast<U>(b)));
+}
+
+// sumsq and condsum over integers (SumSq, CondSum fused.rs:108-140, 181-210, with IntSum): the
+// row value in the element type, widened, added with wrapping.
+template <typename T>
+struct OpSumSq {
+  using Acc = SumAcc;
+  __device__ Acc identity() const { return {0}; }
+  __device__ void add(Acc &a, T v, uint64_t) const { a.s += widen(wrap_mul(v, v)); }
+  __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
+};
+
+template <typename T>
+struct OpCondSum {
+  using Acc = SumAcc;
+  uint32_t keepset;  // the comparison outcomes (outcome()) that keep the row
+  T c, e;
+  __device__ Acc identity() const { return {0}; }
+  __device__ void add(Acc &a, T v, uint64_t) const { a.s += widen((keepset & outcome(v, c)) != 0 ? v : e); }
+  __device__ void merge(Acc &a, const Acc &b) const { a.s += b.s; }
+};
+
 template <typename T>
 struct OpCount {
   using Acc = SumAcc;
@@ -368,6 +395,77 @@ void launch(Ctx &c, const Op &op, const void *x, uint64_t n, void *state) {
   check_hip(hipGetLastError(), "reduce launch");
 }
 
+// Dot over integers (Dot fused.rs:72-106 with IntSum): two inputs, read like k_reduce (16-byte
+// loads, one batch per block, block 0 folding the rows outside whole 16-byte vectors) when both
+// start at the same offset within 16 bytes, one row per load otherwise.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(kRedThreads) void k_dot_int(const T *__restrict__ x, const T *__restrict__ y,
+                                                         uint64_t n, uint64_t head, SumAcc *partials) {
+  constexpr int V = VEC ? 16 / sizeof(T) : 1;
+  constexpr int kL = red_loads<T>() / 2 > 0 ? red_loads<T>() / 2 : 1;
+  const int t = threadIdx.x;
+  const uint64_t h = VEC ? head : 0;
+  const uint64_t nv = (n - h) / V;
+  const uint64_t base = blockIdx.x * uint64_t(kRedThreads) * kL;
+  uint64_t s = 0;
+  if constexpr (VEC) {
+    const Vec16<T> *xv = reinterpret_cast<const Vec16<T> *>(x + h);
+    const Vec16<T> *yv = reinterpret_cast<const Vec16<T> *>(y + h);
+    Vec16<T> a[kL], b[kL];
+#pragma unroll
+    for (int u = 0; u < kL; ++u) {
+      const uint64_t i = base + uint64_t(u) * kRedThreads + t;
+      if (i < nv) {
+        a[u] = xv[i];
+        b[u] = yv[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kL; ++u) {
+      const uint64_t i = base + uint64_t(u) * kRedThreads + t;
+      if (i < nv) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) s += widen(wrap_mul(a[u].e[k], b[u].e[k]));
+      }
+    }
+  } else {
+    T a[kL], b[kL];
+#pragma unroll
+    for (int u = 0; u < kL; ++u) {
+      const uint64_t i = base + uint64_t(u) * kRedThreads + t;
+      a[u] = i < nv ? x[i] : T(0);
+      b[u] = i < nv ? y[i] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kL; ++u) s += widen(wrap_mul(a[u], b[u]));
+  }
+  if (VEC && blockIdx.x == 0) {
+    if (uint64_t(t) < h) s += widen(wrap_mul(x[t], y[t]));
+    for (uint64_t r = h + nv * V + t; r < n; r += kRedThreads) s += widen(wrap_mul(x[r], y[r]));
+  }
+  SumAcc acc{s};
+  acc = block_merge_big(OpSum<T>{}, acc);
+  if (t == 0) partials[blockIdx.x] = acc;
+}
+
+template <typename T>
+void launch_dot(Ctx &c, const T *x, const T *y, uint64_t n, void *state) {
+  const uintptr_t ax = reinterpret_cast<uintptr_t>(x) & 15, ay = reinterpret_cast<uintptr_t>(y) & 15;
+  const bool vec = ax == ay;
+  uint64_t head = ((16 - ax) & 15) / sizeof(T);
+  if (head > n) head = n;
+  const uint64_t per = uint64_t(kRedThreads) * (red_loads<T>() / 2 > 0 ? red_loads<T>() / 2 : 1);
+  const uint64_t units = vec ? (n - head) / (16 / sizeof(T)) : n;
+  const uint64_t blocks = units == 0 ? 1 : (units + per - 1) / per;
+  if (blocks > 0xFFFFFFFFull) usage_error("reduction too large");
+  const uint32_t grid = static_cast<uint32_t>(blocks);
+  SumAcc *parts = static_cast<SumAcc *>(partials(c, sizeof(SumAcc) * grid));
+  if (vec) k_dot_int<T, true><<<grid, kRedThreads, 0, c.stream>>>(x, y, n, head, parts);
+  else k_dot_int<T, false><<<grid, kRedThreads, 0, c.stream>>>(x, y, n, 0, parts);
+  k_reduce_merge<OpSum<T>><<<1, kRedThreads, 0, c.stream>>>(OpSum<T>{}, parts, grid, static_cast<SumAcc *>(state));
+  check_hip(hipGetLastError(), "dot launch");
+}
+
 template <typename T>
 OpCount<T> make_count(int cmp, uint64_t cbits) {
   T c;
@@ -440,6 +538,28 @@ void with_op(const ReduceSpec &s, F &&f) {
         default: usage_error("unsupported integer product type");
       }
       break;
+    case ReduceSpec::FSUM_INT: {
+      auto go = [&](auto *tag) {
+        using T = typename std::remove_pointer<decltype(tag)>::type;
+        if (s.fkind == 2) {
+          f(OpSumSq<T>{}, tag);
+        } else if (s.fkind == 3) {
+          T cv, ev;
+          __builtin_memcpy(&cv, &s.cbits, sizeof(T));
+          __builtin_memcpy(&ev, &s.ebits, sizeof(T));
+          f(OpCondSum<T>{keepset_of(s.cmp), cv, ev}, tag);
+        } else {
+          usage_error("an integer dot reads two inputs");
+        }
+      };
+      switch (s.type) {
+        case VK_I64: go(static_cast<int64_t *>(nullptr)); break;
+        case VK_U64: go(static_cast<uint64_t *>(nullptr)); break;
+        case VK_I32: go(static_cast<int32_t *>(nullptr)); break;
+        default: usage_error("unsupported fused integer sum type");
+      }
+      break;
+    }
     case ReduceSpec::COUNT_BOOL: f(OpCountBool{}, static_cast<uint8_t *>(nullptr)); break;
     case ReduceSpec::EXTREME:
     case ReduceSpec::ARG_EXTREME: {
@@ -517,6 +637,16 @@ void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *s
     using T = typename std::remove_pointer<decltype(tag)>::type;
     launch<T>(c, op, x, n, state);
   });
+}
+
+void reduce_rows2(Ctx &c, const ReduceSpec &s, const void *x, const void *y, uint64_t n, void *state) {
+  if (n == 0) return;
+  switch (s.type) {
+    case VK_I64: launch_dot(c, static_cast<const int64_t *>(x), static_cast<const int64_t *>(y), n, state); break;
+    case VK_U64: launch_dot(c, static_cast<const uint64_t *>(x), static_cast<const uint64_t *>(y), n, state); break;
+    case VK_I32: launch_dot(c, static_cast<const int32_t *>(x), static_cast<const int32_t *>(y), n, state); break;
+    default: usage_error("unsupported integer dot type");
+  }
 }
 
 void reduce_rows_chain(Ctx &c, const ReduceSpec &s, const ChainSpec &ch, const void *x, uint64_t n, void *state) {

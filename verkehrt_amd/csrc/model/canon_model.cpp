@@ -421,3 +421,120 @@ extern "C" int vkm_wide_table_apply(const double *chain, uint64_t n, double appr
   }
   return 0;
 }
+
+// ------------------------------------------------------------------ f32 chains
+// The streamed form of k_canon32.hip, chain by chain: tiles of 1024 blocks; the approximate
+// start of every tile (the start plus the approximate sums of the tiles before) picks the
+// tile's hypothesis at f32 precision; the tile summary joins its runs' summaries in order; the
+// walk applies a tile's summary when its hypothesis is the running value's binade and the check
+// holds, and otherwise evaluates the tile as the kernel's wave does (rounds of run summaries, or
+// the sequential fold when the approximate run starts change binade more than twice). vals are
+// f32 values held in doubles; every fold step rounds to f32. stats: [0] tiles applied, [1] tiles
+// evaluated, [2] evaluations by rounds, [3] runs folded element by element.
+namespace {
+
+double eval_tile32_model(const double *vals, uint64_t b0, uint64_t b1, int j, double s, uint64_t *st) {
+  const uint64_t nb = b1 - b0;
+  const uint64_t nruns = (nb + kRun - 1) / kRun;
+  auto run32 = [&](uint64_t r, const Binade &h) {
+    double v[kRun];
+    const uint64_t rb = b0 + r * kRun;
+    const int cnt = static_cast<int>(std::min<uint64_t>(kRun, b1 - rb));
+    for (int k = 0; k < static_cast<int>(kRun); ++k) v[k] = k < cnt ? vals[(rb + k) * 16 + j] : 0.0;
+    return run_of<kRun, PrecF32>(v, cnt, binade_ref<PrecF32>(h));
+  };
+  auto fold_run = [&](uint64_t r, double t) {
+    for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b)
+      t = PrecF32::rnd(t + vals[b * 16 + j]);
+    return t;
+  };
+  std::vector<double> asum(64, 0.0);
+  for (uint64_t r = 0; r < nruns; ++r)
+    for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b) asum[r] += vals[b * 16 + j];
+  std::vector<double> inc(asum);
+  for (int d = 1; d < 64; d <<= 1) {
+    std::vector<double> nxt(inc);
+    for (int l = d; l < 64; ++l) nxt[l] = inc[l] + inc[l - d];
+    inc = nxt;
+  }
+  int changes = 0;
+  Binade prev = binade_of<PrecF32>(s + (inc[0] - asum[0]));
+  for (uint64_t r = 1; r < nruns; ++r) {
+    const Binade h = binade_of<PrecF32>(s + (inc[r] - asum[r]));
+    if (!same_binade(h, prev)) ++changes;
+    prev = h;
+  }
+  double cur = s;
+  if (changes <= 2) {
+    st[2]++;
+    uint64_t r0 = 0;
+    while (r0 < nruns) {
+      const Binade h = binade_of<PrecF32>(cur);
+      RunFn p = run_identity();
+      uint64_t m = nruns;
+      double last = cur;
+      for (uint64_t r = r0; r < 64; ++r) {
+        if (h.ok) p = run_join(p, r < nruns ? run32(r, h) : run_identity());
+        double val = cur;
+        if (!run_apply<PrecF32>(p, h, cur, &val)) {
+          m = r;
+          break;
+        }
+        last = val;
+      }
+      if (m >= nruns) {
+        cur = last;
+        break;
+      }
+      st[3]++;
+      cur = fold_run(m, m > r0 ? last : cur);
+      r0 = m + 1;
+    }
+    return cur;
+  }
+  for (uint64_t b = b0; b < b1; ++b) cur = PrecF32::rnd(cur + vals[b * 16 + j]);
+  return cur;
+}
+
+}  // namespace
+
+extern "C" int vkm_c32_fold(const double *vals, uint64_t blocks, double *acc, uint64_t *stats) {
+  uint64_t st[4] = {0, 0, 0, 0};
+  const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
+  for (int j = 0; j < 16; ++j) {
+    std::vector<double> pre(tiles);
+    double run = acc[j];
+    for (uint64_t t = 0; t < tiles; ++t) {
+      pre[t] = run;
+      double a = 0.0;
+      for (uint64_t b = t * kTileBlocks; b < std::min(blocks, (t + 1) * kTileBlocks); ++b) a += vals[b * 16 + j];
+      run += a;
+    }
+    double s = acc[j];
+    for (uint64_t t = 0; t < tiles; ++t) {
+      const uint64_t b0 = t * kTileBlocks, b1 = std::min(blocks, b0 + kTileBlocks);
+      const Binade h = binade_of<PrecF32>(pre[t]);
+      RunFn f = run_identity();
+      if (h.ok)
+        for (uint64_t b = b0; b < b1; b += kRun) {
+          double v[kRun];
+          const int cnt = static_cast<int>(std::min<uint64_t>(kRun, b1 - b));
+          for (int k = 0; k < static_cast<int>(kRun); ++k) v[k] = k < cnt ? vals[(b + k) * 16 + j] : 0.0;
+          f = run_join(f, run_of<kRun, PrecF32>(v, cnt, binade_ref<PrecF32>(h)));
+        }
+      const Binade hs = binade_of<PrecF32>(s);
+      double out = s;
+      if (h.ok && same_binade(h, hs) && run_apply<PrecF32>(f, hs, s, &out)) {
+        s = out;
+        st[0]++;
+      } else {
+        s = eval_tile32_model(vals, b0, b1, j, s, st);
+        st[1]++;
+      }
+    }
+    acc[j] = s;
+  }
+  if (stats)
+    for (int k = 0; k < 4; ++k) stats[k] = st[k];
+  return 0;
+}

@@ -164,17 +164,21 @@ enum Cmp { CMP_EQ = 0, CMP_NE, CMP_LT, CMP_LE, CMP_GT, CMP_GE };
 // Streaming reductions (k_reduce.hip). `state` is the reducer's device state; each
 // launch merges the reduction of its rows into it.
 struct ReduceSpec {
-  enum Kind { SUM_INT, COUNT_CMP, COUNT_BOOL, EXTREME, ARG_EXTREME, BITS, PROD_INT } kind;
+  enum Kind { SUM_INT, COUNT_CMP, COUNT_BOOL, EXTREME, ARG_EXTREME, BITS, PROD_INT, FSUM_INT } kind;
   int32_t type;       // element type of x
-  int cmp;            // COUNT_CMP
-  uint64_t cbits;     // COUNT_CMP constant bits
+  int cmp;            // COUNT_CMP, FSUM_INT condsum
+  uint64_t cbits;     // COUNT_CMP / condsum constant bits
   bool want_max;      // EXTREME / ARG_EXTREME
   int bitop;          // BITS: 0 and, 1 or, 2 xor
   uint64_t base_row;  // ARG_EXTREME: rows consumed before this update
+  int fkind;          // FSUM_INT: 1 dot (two inputs, reduce_rows2), 2 sumsq, 3 condsum
+  uint64_t ebits;     // FSUM_INT condsum: the value of rows that fail the comparison
 };
 // Reduce state layouts (device): SUM_INT/COUNT/BITS: u64; EXTREME: {u64 bits, u64 has};
 // ARG_EXTREME: {u64 best_bits, i64 best_idx, i64 nan_idx}.
 void reduce_rows(Ctx &c, const ReduceSpec &s, const void *x, uint64_t n, void *state);
+// Integer dot (FSUM_INT, fkind 1): the wrapping sum of the widened wrapping products x[i] * y[i].
+void reduce_rows2(Ctx &c, const ReduceSpec &s, const void *x, const void *y, uint64_t n, void *state);
 void reduce_state_init(Ctx &c, const ReduceSpec &s, void *state);
 
 // Canonical float reductions (k_canon.hip). kind: 0 sum, 1 dot, 2 sumsq, 3 condsum, 4 mul.
@@ -186,6 +190,11 @@ struct CanonSpec {
 };
 void canon_blocks(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t blocks,
                   double *acc);
+// The same at f32 (k_canon32.hip): acc holds 16 f32 values as doubles; every step rounds to
+// f32. canon32_finish writes the result's f32 bits (zero-extended) to out.
+void canon32_blocks(Ctx &c, const CanonSpec &s, const float *x, const float *y, uint64_t blocks, double *acc);
+void canon32_rest(Ctx &c, const CanonSpec &s, const float *x, const float *y, uint64_t rest, double *tail);
+void canon32_finish(Ctx &c, bool mul, const double *acc, const double *tail, int tail_len, uint64_t *out);
 // Writes the row values of rows [0, rest) (rest < 16) to `tail`.
 void canon_rest(Ctx &c, const CanonSpec &s, const double *x, const double *y, uint64_t rest,
                 double *tail);
