@@ -138,7 +138,9 @@ def test_plan_errors(D):
     with pytest.raises(KernelError, match="has no array result"):
         plan.add("reduce_add_f64", [Stmt(s)])
     with pytest.raises(KernelError, match="cannot be planned"):
-        plan.add("argsort_f64", [Input(0)])
+        plan.add("scan_add_f64", [Input(0)])
+    with pytest.raises(KernelError, match="group count of a group_id statement"):
+        plan.add("reduce_into_add_f64", [Input(0), Input(0), S("i64", 4)])
     with pytest.raises(KernelError, match="not compiled"):
         plan.run()
     plan.compile()
@@ -146,4 +148,69 @@ def test_plan_errors(D):
     assert plan.result(s, 0).value == 45.0
     with pytest.raises(KernelError, match="already compiled"):
         plan.add("reduce_add_f64", [Input(1)])
+    plan.close()
+
+
+def test_plan_sort_and_group(D):
+    """argsort, group_id and reduce_into_add_f64 over the group ids and group counts inside one
+    captured plan, beside filter+sum: every replay against the single calls and the oracle."""
+    from verkehrt_amd import Input, Stmt
+    rng = np.random.default_rng(23)
+    sizes = [0, 1, 15, 16, 17, 300, 4095, 4096, 4097, 8193, 16385, 70000, 200003, 1048577]
+    sizes += [int(v) for v in rng.integers(0, 120000, 12)]
+    items, host = [], []
+    pool = rng.integers(-(1 << 40), 1 << 40, 256)
+    for j, n in enumerate(sizes):
+        if j % 3 == 0:  # distinct keys: the small group tables overflow past 4096 rows
+            k = rng.integers(-(1 << 62), 1 << 62, n)
+        elif j % 3 == 1:
+            k = pool[rng.integers(0, 256, n)]
+        else:
+            k = rng.integers(0, 3, n) * (1 << 33)  # few keys, constant low digits
+        x = rng.random(n)
+        f = rng.standard_normal(n)
+        if n > 4:
+            f[:4] = [0.0, -0.0, -0.0, 0.0]
+        i32 = rng.integers(-(1 << 31), (1 << 31) - 1, n).astype(np.int32)
+        u8 = rng.integers(0, 256, n).astype(np.uint8)
+        host.append((k, x, f, i32, u8))
+        items.append([D.array(k), D.array(x), D.array(f), D.array(i32), D.array(u8, "u8")])
+    plan = D.batch_plan(items)
+    st_sort = plan.add("argsort_i64", [Input(0)])
+    st_gid = plan.add("group_id_i64", [Input(0)])
+    st_fold = plan.add("reduce_into_add_f64", [Stmt(st_gid), Input(1), Stmt(st_gid)])
+    st_filt = plan.add("filter_gt_f64", [Input(1), S("f64", 0.9)])
+    st_sum = plan.add("reduce_add_f64", [Stmt(st_filt)])
+    st_fsum = plan.add("reduce_add_f64", [Stmt(st_fold)])
+    st_f64 = plan.add("argsort_f64", [Input(2)])
+    st_i32 = plan.add("argsort_i32", [Input(3)])
+    st_u8 = plan.add("argsort_u8", [Input(4)])
+    plan.compile(tune=True)
+    want = []
+    for it, (k, x, f, i32, u8) in zip(items, host):
+        ids = D.call("group_id_i64", [it[0]])
+        g = int(ids.to_numpy().max()) + 1 if len(k) else 0
+        fold = D.call("reduce_into_add_f64", [ids, it[1], S("i64", g)])
+        kept = D.call("filter_gt_f64", [it[1], S("f64", 0.9)])
+        want.append({st_sort: D.call("argsort_i64", [it[0]]), st_gid: ids, st_fold: fold, st_filt: kept,
+                     st_sum: D.call("reduce_add_f64", [kept]), st_fsum: D.call("reduce_add_f64", [fold]),
+                     st_f64: D.call("argsort_f64", [it[2]]), st_i32: D.call("argsort_i32", [it[3]]),
+                     st_u8: D.call("argsort_u8", [it[4]]), "groups": g})
+    for rep in range(3):
+        plan.run()
+        for i, (k, x, f, i32, u8) in enumerate(host):
+            for st in [st_sort, st_gid, st_fold, st_filt, st_sum, st_fsum, st_f64, st_i32, st_u8]:
+                assert same_value(plan.result(st, i), want[i][st]), (rep, i, sizes[i], st)
+            if i % 5 == 0:  # the oracle on some items
+                assert np.array_equal(plan.result(st_sort, i).to_numpy(), O.argsort(k)), (i, sizes[i])
+                ids = O.group_id(k)
+                assert np.array_equal(plan.result(st_gid, i).to_numpy(), ids)
+                g = want[i]["groups"]
+                assert np.array_equal(plan.result(st_fold, i).to_numpy().view(np.uint64),
+                                      O.reduce_into(ids, x, g, "add").view(np.uint64))
+        assert [int(v) for v in plan.bits(st_gid)] == [w["groups"] for w in want]
+        assert [int(v) for v in plan.bits(st_fold)] == [w["groups"] for w in want]
+        junk = D.call("argsort_i64", [items[-1][0]])
+        D.call("group_id_i64", [junk])
+        D.sync()
     plan.close()

@@ -1458,7 +1458,8 @@ constexpr bool kCapturePlans = true;
 
 struct PlanStmt {
   const Def *d = nullptr;
-  int kind = 0;  // 0 selection, 1 canonical sum, 2 other reduction
+  int kind = 0;  // 0 selection, 1 canonical sum, 2 other reduction, 3 argsort, 4 group_id,
+                 // 5 reduce_into_add_f64 over a group_id statement's ids and group counts
   std::vector<vk_bref> args;
   std::vector<uint64_t> bound;  // per item: an upper bound of its rows
   int src = -1;                 // statement whose arrays are the rows (or -1: plan inputs)
@@ -1477,6 +1478,11 @@ struct PlanStmt {
   int items_per_thread = 16;    // selection rows per thread (autotuned: 8, 16 or 0, the persistent filter)
   uint64_t tile_rows = 8192;    // reduction rows per tile (autotuned: 4096, 8192 or 16384)
   std::vector<uint64_t> host;   // results copied back after a run
+  int gsrc = -1;                // reduce_into_add_f64: the group_id statement
+  PlanSort *sort = nullptr;     // argsort
+  PlanGroup *group = nullptr;   // group_id (its group counts at group->groups)
+  PlanFold *fold = nullptr;     // reduce_into_add_f64
+  std::vector<std::pair<uint64_t *, uint64_t>> clear;  // look-back status words (cleared on epoch wrap)
 };
 
 }  // namespace
@@ -1523,11 +1529,14 @@ RowsRef rows_ref(const vk_bplan &p, const vk_bref &a, size_t i) {
   if (a.kind == VK_BREF_STMT) {
     if (a.index < 0 || size_t(a.index) >= p.st.size()) usage_error("plan statement out of range");
     const PlanStmt &s = p.st[size_t(a.index)];
-    if (s.kind != 0) usage_error("statement " + std::to_string(a.index) + " has no array result");
+    if (s.kind == 1 || s.kind == 2) usage_error("statement " + std::to_string(a.index) + " has no array result");
     // the statement's buffer exists once it is compiled (compiling goes in statement order)
     const uint32_t w = type_width(s.d->out_t);
     const void *ptr = s.outbuf ? static_cast<const char *>(dptr(s.outbuf)) + s.off[i] * w : nullptr;
-    return RowsRef{ptr, s.d->out_t, s.bound[i], a.index};
+    // argsort and group_id results have the rows of their input; selections and bucket sums
+    // have device row counts
+    const int src = (s.kind == 3 || s.kind == 4) ? -1 : a.index;
+    return RowsRef{ptr, s.d->out_t, s.bound[i], src};
   }
   usage_error("a plan argument of " + std::string("kind ") + std::to_string(a.kind) + " is not rows");
 }
@@ -1545,9 +1554,15 @@ void plan_launch(vk_bplan &p, PlanStmt &s) {
   } else if (s.kind == 1) {
     plan_tiles_dev(c, s.dp, ni, dev_n, 16, canon_tile_blocks(), canon_tile_min_rows());
     canon_batch_dev(c, s.cs, s.dp, ni, s.acc, s.zero, reinterpret_cast<double *>(s.res));
-  } else {
+  } else if (s.kind == 2) {
     plan_tiles_dev(c, s.dp, ni, dev_n, 1, s.tile_rows);
     reduce_batch_dev(c, s.rs, s.tile_rows, s.dp, ni, s.partials, s.res);
+  } else if (s.kind == 3) {
+    plan_sort_run(c, *s.sort, p.epoch);
+  } else if (s.kind == 4) {
+    plan_group_run(c, *s.group);
+  } else {
+    plan_fold_run(c, *s.fold, p.epoch);
   }
 }
 
@@ -1555,9 +1570,9 @@ void plan_prologue(vk_bplan &p) {
   Ctx &c = *p.c;
   k_plan_bump<<<1, 1, 0, c.stream>>>(p.epoch, p.epoch + 1);
   for (auto &s : p.st)
-    if (s.status_words)
-      k_plan_clear<<<stream_grid(s.status_words, kBlock, 256), kBlock, 0, c.stream>>>(s.dp.status, s.status_words,
-                                                                                     p.epoch + 1);
+    for (auto &r : s.clear)
+      if (r.second)
+        k_plan_clear<<<stream_grid(r.second, kBlock, 256), kBlock, 0, c.stream>>>(r.first, r.second, p.epoch + 1);
   check_hip(hipGetLastError(), "plan prologue");
 }
 
@@ -1687,15 +1702,33 @@ void plan_add(vk_bplan &p, const char *name, const vk_bref *args, size_t nargs) 
     s.rs = reduce_spec(d, 0);
     if (d.impl == R_COUNT_CMP) s.rs.cbits = scalar_of(1);
     s.words = reduce_state_words(s.rs);
+  } else if (d.cls == VK_WHOLE && (d.impl == W_ARGSORT || d.impl == W_GROUP_ID)) {
+    s.kind = d.impl == W_ARGSORT ? 3 : 4;
+    const RowsRef r0 = rows_type(0);
+    if (r0.src >= 0) usage_error("a planned " + d.name + " takes a plan input or an argsort or group_id result");
+  } else if (d.cls == VK_WHOLE && d.impl == W_REDUCE_INTO && d.p0 == 4) {
+    // reduce_into_add_f64(ids, values, buckets) with ids and buckets from one group_id statement
+    s.kind = 5;
+    auto group_stmt = [&](size_t k) {
+      if (args[k].kind != VK_BREF_STMT || args[k].index < 0 || size_t(args[k].index) >= p.st.size() ||
+          p.st[size_t(args[k].index)].kind != 4)
+        usage_error("a planned reduce_into_add_f64 takes the ids and the group count of a group_id statement");
+      return args[k].index;
+    };
+    s.gsrc = group_stmt(0);
+    if (group_stmt(2) != s.gsrc) usage_error("the ids and the bucket count come from different statements");
+    const RowsRef r1 = rows_type(1);
+    if (r1.src >= 0) usage_error("a planned reduce_into_add_f64 takes plan input values");
   } else {
-    usage_error(d.name + " cannot be planned (selections, canonical sums and reductions can)");
+    usage_error(d.name + " cannot be planned (selections, canonical sums, reductions, argsort, group_id and "
+                "reduce_into_add_f64 over a group_id can)");
   }
   // per-item bounds and equal lengths of two-array statements
   s.bound.resize(p.nitems);
   for (size_t i = 0; i < p.nitems; ++i) {
     const RowsRef r0 = rows_ref(p, s.args[0], i);
     s.bound[i] = r0.bound;
-    if ((s.kind == 0 && d.impl == S_COMPRESS) || (s.kind == 1 && d.p0 == 1)) {
+    if ((s.kind == 0 && d.impl == S_COMPRESS) || (s.kind == 1 && d.p0 == 1) || s.kind == 5) {
       const RowsRef r1 = rows_ref(p, s.args[1], i);
       if (r1.bound != r0.bound)
         usage_error("length mismatch: " + std::to_string(r0.bound) + " vs " + std::to_string(r1.bound));
@@ -1712,6 +1745,37 @@ void plan_compile(vk_bplan &p, bool tune) {
   fill_u64(c, reinterpret_cast<uint64_t *>(p.epoch), 8, 0);
   for (size_t k = 0; k < p.st.size(); ++k) {
     PlanStmt &s = p.st[k];
+    if (s.kind >= 3) {
+      // argsort, group_id, reduce_into_add_f64: results at each item's bound (n rows)
+      uint64_t total = 0;
+      s.off.resize(ni);
+      for (size_t i = 0; i < ni; ++i) {
+        s.off[i] = total;
+        total += s.bound[i];
+      }
+      s.outbuf = new_array(c, s.d->out_t, total ? total : 1);
+      std::vector<BatchItem> items(ni);
+      for (size_t i = 0; i < ni; ++i) {
+        BatchItem &it = items[i];
+        it.a = rows_ref(p, s.args[0], i).ptr;
+        it.n = s.bound[i];
+        it.out = static_cast<char *>(dptr(s.outbuf)) + s.off[i] * 8;
+        if (s.kind == 5) it.b = rows_ref(p, s.args[1], i).ptr;
+      }
+      if (s.kind == 3) {
+        s.sort = plan_sort_new(c, s.d->t0, true, true, items, nullptr, nullptr);
+        s.clear.emplace_back(s.sort->status, s.sort->status_words);
+      } else if (s.kind == 4) {
+        s.group = plan_group_new(c, s.d->t0, items, p.epoch);
+        s.counts = s.group->groups;
+        s.clear.emplace_back(s.group->sel.status, s.group->status_words);
+      } else {
+        s.counts = p.st[size_t(s.gsrc)].group->groups;
+        s.fold = plan_fold_new(c, items, s.counts);
+        s.clear.emplace_back(s.fold->sort->status, s.fold->sort->status_words);
+      }
+      continue;
+    }
     // tile bounds for the smallest tile any candidate uses
     uint64_t unit = 1, per = 1;
     if (s.kind == 0) per = select_tile_rows(8);
@@ -1755,6 +1819,7 @@ void plan_compile(vk_bplan &p, bool tune) {
     if (s.status_words) {
       s.dp.status = static_cast<uint64_t *>(plan_alloc(p, s.status_words * 8));
       fill_u64(c, s.dp.status, s.status_words, 0);
+      s.clear.emplace_back(s.dp.status, s.status_words);
     }
     if (s.kind == 1) {
       s.res = static_cast<uint64_t *>(plan_alloc(p, ni * 8));
@@ -1787,7 +1852,9 @@ void plan_fetch(vk_bplan &p) {
   if (p.fresh) return;
   Ctx &c = *p.c;
   for (auto &s : p.st) {
-    if (s.kind == 0) {
+    if (s.kind == 3) {
+      s.host.assign(s.bound.begin(), s.bound.end());
+    } else if (s.kind == 0 || s.kind == 4 || s.kind == 5) {
       s.host.resize(p.nitems);
       check_hip(hipMemcpyAsync(s.host.data(), s.counts, p.nitems * 8, hipMemcpyDeviceToHost, c.stream), "counts");
     } else {
@@ -1806,11 +1873,12 @@ vk_value plan_result(vk_bplan &p, size_t k, size_t i) {
   plan_fetch(p);
   PlanStmt &s = p.st[k];
   vk_value v{VK_VALUE_SCALAR, nullptr, {0, 0}};
-  if (s.kind == 0) {
+  if (s.kind == 0 || s.kind >= 3) {
+    // selections and bucket sums: their device row counts; argsort and group_id: n rows
+    const uint64_t len = (s.kind == 3 || s.kind == 4) ? s.bound[i] : s.host[i];
     v.kind = VK_VALUE_ARRAY;
     s.outbuf->buf->refs++;
-    v.array = new vk_array{1, s.outbuf->buf, s.outbuf->offset + s.off[i] * type_width(s.d->out_t), s.host[i],
-                           s.d->out_t};
+    v.array = new vk_array{1, s.outbuf->buf, s.outbuf->offset + s.off[i] * type_width(s.d->out_t), len, s.d->out_t};
   } else if (s.kind == 1) {
     v.scalar = scalar(VK_F64, s.host[i]);
   } else {
@@ -1827,8 +1895,12 @@ void plan_free(vk_bplan *p) {
   (void)hipStreamSynchronize(c.stream);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
-  for (auto &s : p->st)
+  for (auto &s : p->st) {
     if (s.outbuf) release(s.outbuf);
+    plan_sort_free(c, s.sort);
+    plan_group_free(c, s.group);
+    plan_fold_free(c, s.fold);
+  }
   for (void *q : p->allocs) tmp_free(c, q);
   for (vk_array *a : p->inputs) release(a);
   delete p;
@@ -2165,7 +2237,7 @@ int vk_bplan_counts(vk_bplan *plan, size_t stmt, uint64_t *out, vk_status *st) {
     if (stmt >= plan->st.size()) usage_error("plan result out of range");
     plan_fetch(*plan);
     const PlanStmt &s = plan->st[stmt];
-    if (s.kind == 0) {
+    if (s.kind == 0 || s.kind >= 3) {  // kept rows, rows, group counts, bucket counts
       std::memcpy(out, s.host.data(), plan->nitems * 8);
     } else if (s.kind == 1) {
       std::memcpy(out, s.host.data(), plan->nitems * 8);

@@ -905,4 +905,139 @@ uint64_t reduce_into_add_f64(Ctx &c, const int64_t *keys, const double *vals, ui
   return bad;
 }
 
+
+// ------------------------------------------- reduce_into_add_f64 inside batch plans
+// Keys are the group ids of a group_id statement of the same plan, and the bucket count of item
+// i is that statement's group count, on the device. The (chain key, value) sort of each item's
+// whole blocks runs as a plan sort (no host read); then one lane per chain key of every item
+// folds its chain in row order from +0.0, the 16 lanes of a bucket combine pairwise
+// (finish_lanes, reduce.rs:75-91), and the bucket's first lane adds the item's tail rows of that
+// bucket in row order: the order of reduce_into_add_f64 (and of the stripes in
+// reduce_into_add_f64_batch). Tiles of 16 buckets are planned on the device from the group
+// counts.
+
+namespace {
+
+// Fold items: n = the bucket count (written by the device planner), a keys (i64), b values,
+// out the output, aux the item's first row in the sorted buffers, aux2 its whole rows, aux3 its
+// rows.
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_fold_combine_plan(const BatchItem *items, const uint32_t *tile_item,
+                                                              const uint32_t *ntiles, const K *sk,
+                                                              const double *sv) {
+  if (blockIdx.x >= *ntiles) return;
+  const BatchItem &I = items[tile_item[blockIdx.x]];
+  const uint64_t ck = (blockIdx.x - I.tile0) * kBlock + threadIdx.x;  // the lane's chain key
+  const uint64_t bucket = ck / 16;
+  const int j = static_cast<int>(ck & 15);
+  const bool valid = bucket < I.n;
+  double a = 0.0;
+  if (valid) {
+    const K *k_ = sk + I.aux;
+    const double *v_ = sv + I.aux;
+    const uint64_t rows = I.aux2;
+    auto lower = [&](uint64_t key) {
+      uint64_t lo = 0, hi = rows;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (static_cast<uint64_t>(k_[mid]) < key) lo = mid + 1;
+        else hi = mid;
+      }
+      return lo;
+    };
+    const uint64_t s0 = lower(ck), e = lower(ck + 1);
+    uint64_t q = s0;
+    for (; q + 8 <= e; q += 8) {
+      double u[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) u[m] = v_[q + m];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) a = a + u[m];
+    }
+    for (; q < e; ++q) a = a + v_[q];
+  }
+  // finish_lanes over the bucket's 16 lanes: acc[j] += acc[j + step] for j < step
+#pragma unroll
+  for (int step = 8; step > 0; step /= 2) {
+    const double o = __shfl_down(a, step, 16);
+    if (j < step) a = a + o;
+  }
+  if (valid && j == 0) {
+    const int64_t *keys = static_cast<const int64_t *>(I.a);
+    const double *vals = static_cast<const double *>(I.b);
+    for (uint64_t r = I.aux2; r < I.aux3; ++r)
+      if (static_cast<uint64_t>(keys[r]) == bucket) a = a + vals[r];
+    static_cast<double *>(I.out)[bucket] = a;
+  }
+}
+
+}  // namespace
+
+PlanFold *plan_fold_new(Ctx &c, std::vector<BatchItem> &items, const uint64_t *groups) {
+  auto *f = new PlanFold{};
+  const uint64_t ni = items.size();
+  f->nitems = ni;
+  f->groups = groups;
+  uint64_t whole_total = 0, maxn = 0, max_tiles = 0;
+  std::vector<BatchItem> sort_items(ni), fold(ni);
+  for (size_t i = 0; i < ni; ++i) {
+    const BatchItem &it = items[i];
+    maxn = it.n > maxn ? it.n : maxn;
+    sort_items[i] = BatchItem{};
+    sort_items[i].a = it.a;
+    sort_items[i].b = it.b;
+    sort_items[i].n = it.n - it.n % 16;
+    fold[i] = BatchItem{};
+    fold[i].a = it.a;
+    fold[i].b = it.b;
+    fold[i].out = it.out;
+    fold[i].n = it.n;  // the bucket count bound; the device planner writes the count
+    fold[i].aux = whole_total;
+    fold[i].aux2 = it.n - it.n % 16;
+    fold[i].aux3 = it.n;
+    whole_total += it.n - it.n % 16;
+    max_tiles += (it.n + 15) / 16;
+  }
+  if (max_tiles > 0x7FFFFFFFull) usage_error("plan statement too large");
+  f->wide = maxn * 16 > 0xFFFFFFFFull;
+  f->sk = tmp_alloc(c, (whole_total ? whole_total : 1) * (f->wide ? 8 : 4));
+  f->sv = static_cast<double *>(tmp_alloc(c, (whole_total ? whole_total : 1) * 8));
+  f->sort = plan_sort_new(c, chain_key_type(), false, f->wide, sort_items, f->sk, reinterpret_cast<uint64_t *>(f->sv));
+  DevPlan &dp = f->dp;
+  dp.items = static_cast<BatchItem *>(tmp_alloc(c, (ni ? ni : 1) * sizeof(BatchItem)));
+  if (ni)
+    check_hip(hipMemcpyAsync(dp.items, fold.data(), ni * sizeof(BatchItem), hipMemcpyHostToDevice, c.stream),
+              "fold items");
+  check_hip(hipStreamSynchronize(c.stream), "fold items");
+  dp.max_tiles = static_cast<uint32_t>(max_tiles);
+  dp.tile_item = static_cast<uint32_t *>(tmp_alloc(c, (max_tiles ? max_tiles : 1) * 4));
+  dp.ntiles = static_cast<uint32_t *>(tmp_alloc(c, 8));
+  return f;
+}
+
+void plan_fold_run(Ctx &c, PlanFold &f, const uint32_t *epoch) {
+  plan_sort_run(c, *f.sort, epoch);
+  plan_tiles_dev(c, f.dp, static_cast<uint32_t>(f.nitems), f.groups, 1, kBlock / 16);
+  if (f.dp.max_tiles) {
+    if (f.wide)
+      k_fold_combine_plan<uint64_t><<<f.dp.max_tiles, kBlock, 0, c.stream>>>(
+          f.dp.items, f.dp.tile_item, f.dp.ntiles, static_cast<const uint64_t *>(f.sk), f.sv);
+    else
+      k_fold_combine_plan<uint32_t><<<f.dp.max_tiles, kBlock, 0, c.stream>>>(
+          f.dp.items, f.dp.tile_item, f.dp.ntiles, static_cast<const uint32_t *>(f.sk), f.sv);
+  }
+  check_hip(hipGetLastError(), "plan reduce_into_add_f64");
+}
+
+void plan_fold_free(Ctx &c, PlanFold *f) {
+  if (!f) return;
+  plan_sort_free(c, f->sort);
+  tmp_free(c, f->sk);
+  tmp_free(c, f->sv);
+  tmp_free(c, f->dp.items);
+  tmp_free(c, f->dp.tile_item);
+  tmp_free(c, f->dp.ntiles);
+  delete f;
+}
+
 }  // namespace vk

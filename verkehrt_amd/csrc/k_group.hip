@@ -551,13 +551,16 @@ constexpr int kBLSlots = 1 << kBLBits;
 constexpr int kBRows = static_cast<int>(kGTile / kBlock);
 
 // aux2: the item's longest probe run (kNoLimit in a table of 2n words); overflow[item] is set
-// when a run passes it.
+// when a run passes it. With `only`, items whose word there is zero are skipped (a plan's
+// second insert, into full tables, of the items whose small table overflowed).
 __global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const BatchItem *items,
-                                                         const uint32_t *tile_item, uint32_t *overflow) {
+                                                         const uint32_t *tile_item, uint32_t *overflow,
+                                                         const uint32_t *only) {
   __shared__ unsigned long long s_key[kBLSlots];
   __shared__ uint32_t s_row[kBLSlots];
   __shared__ uint32_t s_cnt;
   const uint32_t it = tile_item[blockIdx.x];
+  if (only && only[it] == 0) return;  // uniform over the block
   const BatchItem &I = items[it];
   const ItemRows R(I, blockIdx.x);
   const KeyCol a{type, I.a};
@@ -715,7 +718,7 @@ void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_
   }
   {
     BatchPlan p1 = batch_plan(c, items, kGTile);
-    k_insert_batch<<<p1.ntiles, kBlock, 0, c.stream>>>(type, p1.items, p1.tile_item, over);
+    k_insert_batch<<<p1.ntiles, kBlock, 0, c.stream>>>(type, p1.items, p1.tile_item, over, nullptr);
     check_hip(hipGetLastError(), "group_id batch insert");
     batch_plan_free(c, p1);
   }
@@ -744,7 +747,7 @@ void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_
       again.push_back(items[i]);
     }
     BatchPlan p2 = batch_plan(c, again, kGTile);
-    k_insert_batch<<<p2.ntiles, kBlock, 0, c.stream>>>(type, p2.items, p2.tile_item, over);
+    k_insert_batch<<<p2.ntiles, kBlock, 0, c.stream>>>(type, p2.items, p2.tile_item, over, nullptr);
     check_hip(hipGetLastError(), "group_id batch insert (full tables)");
     batch_plan_free(c, p2);
   }
@@ -967,6 +970,170 @@ void group_id(Ctx &c, int32_t ta, const void *a, int32_t tb, const void *b, uint
     k_ids2<<<stream_grid(n, kBlock * 4, 8192), kBlock, 0, c.stream>>>(ka, n, table, mask, shift, out);
   check_hip(hipGetLastError(), "group_id ids");
   tmp_free(c, table);
+}
+
+
+// ------------------------------------------------------------ group_id inside batch plans
+// group_id_batch with its one host read (which items overflowed their small table) moved to
+// the device: every item with more than 4096 rows has a full table of 2n words allocated at
+// compile time; a run zero-fills and inserts into it only for items whose small table
+// overflowed (their blocks check the item's overflow word), then the table each item ended in
+// is picked on the device, and probe, first rows (a device-planned `where` selection), numbering
+// and ids run over it. The group count of item i is left in groups[i].
+
+namespace {
+
+constexpr uint64_t kFillTile = 65536;  // table words per block of the conditional fill
+
+// Zero item it's full table (aux: the item's index) when its small table overflowed.
+__global__ __launch_bounds__(kBlock) void k_fill_full_plan(const BatchItem *fitems, const uint32_t *tile_item,
+                                                           const uint32_t *over) {
+  const BatchItem &F = fitems[tile_item[blockIdx.x]];
+  if (over[F.aux] == 0) return;
+  unsigned long long *t = static_cast<unsigned long long *>(const_cast<void *>(F.a));
+  const uint64_t w0 = (blockIdx.x - F.tile0) * kFillTile;
+  const uint64_t w1 = w0 + kFillTile < F.n ? w0 + kFillTile : F.n;
+  for (uint64_t w = w0 + threadIdx.x; w < w1; w += kBlock) t[w] = 0;
+}
+
+__global__ void k_pick_items(const BatchItem *small, const BatchItem *full, const uint32_t *over, uint64_t nitems,
+                             BatchItem *fin) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < nitems) fin[i] = over[i] ? full[i] : small[i];
+}
+
+BatchItem *upload_items(Ctx &c, const std::vector<BatchItem> &v) {
+  auto *d = static_cast<BatchItem *>(tmp_alloc(c, (v.size() ? v.size() : 1) * sizeof(BatchItem)));
+  if (!v.empty())
+    check_hip(hipMemcpyAsync(d, v.data(), v.size() * sizeof(BatchItem), hipMemcpyHostToDevice, c.stream), "items");
+  check_hip(hipStreamSynchronize(c.stream), "items");
+  return d;
+}
+
+}  // namespace
+
+PlanGroup *plan_group_new(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint32_t *epoch) {
+  constexpr int kBSmallLg = 13;
+  auto *g = new PlanGroup{};
+  g->type = type;
+  g->nitems = items.size();
+  const uint64_t ni = g->nitems;
+  uint64_t total = 0, small_words = 0, full_words = 0;
+  std::vector<int> full_lg(ni);
+  for (size_t i = 0; i < ni; ++i) {
+    BatchItem &it = items[i];
+    if (it.n > 0x7FFFFFFFull) usage_error("group_id supports at most 2^31 - 1 rows");
+    int lg = 1;
+    while ((uint64_t(1) << lg) < 2 * it.n) ++lg;
+    full_lg[i] = lg;
+    it.aux = total;
+    it.aux3 = static_cast<uint64_t>(lg < kBSmallLg ? lg : kBSmallLg);
+    total += it.n;
+    small_words += uint64_t(1) << it.aux3;
+    if (lg > kBSmallLg) full_words += uint64_t(1) << lg;
+  }
+  g->total = total;
+  g->small_words = small_words;
+  g->small = static_cast<unsigned long long *>(tmp_alloc(c, (small_words ? small_words : 1) * 8));
+  g->full = static_cast<unsigned long long *>(tmp_alloc(c, (full_words ? full_words : 1) * 8));
+  g->over = static_cast<uint32_t *>(tmp_alloc(c, ni * 4 + 8));
+  std::vector<BatchItem> full_items(ni), fill;
+  uint64_t w = 0, f = 0;
+  for (size_t i = 0; i < ni; ++i) {
+    items[i].b = g->small + w;
+    items[i].aux2 = int(items[i].aux3) < full_lg[i] ? uint64_t(kMaxRun) : uint64_t(kNoLimit);
+    w += uint64_t(1) << items[i].aux3;
+    full_items[i] = items[i];
+    if (full_lg[i] > kBSmallLg) {
+      full_items[i].b = g->full + f;
+      full_items[i].aux3 = static_cast<uint64_t>(full_lg[i]);
+      full_items[i].aux2 = uint64_t(kNoLimit);
+      BatchItem fi{};
+      fi.a = g->full + f;
+      fi.n = uint64_t(1) << full_lg[i];
+      fi.aux = i;
+      fill.push_back(fi);
+      f += uint64_t(1) << full_lg[i];
+    }
+  }
+  g->small_p = batch_plan(c, items, kGTile);  // sets tile0 (the same in full_items below)
+  for (size_t i = 0; i < ni; ++i) full_items[i].tile0 = items[i].tile0;
+  g->items_full = upload_items(c, full_items);
+  g->items_final = static_cast<BatchItem *>(tmp_alloc(c, (ni ? ni : 1) * sizeof(BatchItem)));
+  g->fill_p = batch_plan(c, fill, kFillTile);
+  g->slot = static_cast<uint32_t *>(tmp_alloc(c, (total ? total : 1) * 4));
+  g->first = static_cast<uint8_t *>(tmp_alloc(c, total + 8));
+  g->firsts = static_cast<int64_t *>(tmp_alloc(c, (total ? total : 1) * 8));
+  g->groups = static_cast<uint64_t *>(tmp_alloc(c, ni * 8 + 8));
+  // the first-row selection: item i's flags at first + aux, its first rows to firsts + aux
+  std::vector<BatchItem> wl(ni);
+  uint64_t max_tiles = 0;
+  const uint64_t per = select_tile_rows(16);
+  for (size_t i = 0; i < ni; ++i) {
+    wl[i] = BatchItem{};
+    wl[i].a = g->first + items[i].aux;
+    wl[i].out = g->firsts + items[i].aux;
+    wl[i].n = items[i].n;
+    max_tiles += (items[i].n + per - 1) / per;
+  }
+  if (max_tiles > 0x7FFFFFFFull) usage_error("plan statement too large");
+  DevPlan &dp = g->sel;
+  dp.items = upload_items(c, wl);
+  dp.max_tiles = static_cast<uint32_t>(max_tiles);
+  dp.tile_item = static_cast<uint32_t *>(tmp_alloc(c, (max_tiles ? max_tiles : 1) * 4));
+  dp.ntiles = static_cast<uint32_t *>(tmp_alloc(c, 8));
+  dp.ticket = static_cast<uint32_t *>(tmp_alloc(c, 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(dp.ticket), 1, 0);
+  dp.epoch = epoch;
+  g->status_words = max_tiles;
+  dp.status = static_cast<uint64_t *>(tmp_alloc(c, (max_tiles ? max_tiles : 1) * 8));
+  fill_u64(c, dp.status, max_tiles ? max_tiles : 1, 0);
+  return g;
+}
+
+void plan_group_run(Ctx &c, PlanGroup &g) {
+  const uint64_t ni = g.nitems;
+  fill_u64(c, g.groups, ni, 0);
+  if (g.total == 0) return;
+  fill_u64(c, reinterpret_cast<uint64_t *>(g.small), g.small_words, 0);
+  fill_u64(c, reinterpret_cast<uint64_t *>(g.over), (ni * 4 + 8) / 8, 0);
+  const BatchPlan &p = g.small_p;
+  k_insert_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.type, p.items, p.tile_item, g.over, nullptr);
+  if (g.fill_p.ntiles) {
+    k_fill_full_plan<<<g.fill_p.ntiles, kBlock, 0, c.stream>>>(g.fill_p.items, g.fill_p.tile_item, g.over);
+    k_insert_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.type, g.items_full, p.tile_item, g.over, g.over);
+  }
+  k_pick_items<<<stream_grid(ni, kBlock, 1u << 20), kBlock, 0, c.stream>>>(p.items, g.items_full, g.over, ni,
+                                                                          g.items_final);
+  k_probe_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.type, g.items_final, p.tile_item, g.slot, g.first);
+  check_hip(hipGetLastError(), "plan group_id table");
+  plan_tiles_dev(c, g.sel, static_cast<uint32_t>(ni), nullptr, 1, select_tile_rows(16));
+  SelectSpec ss{2, VK_I64, 0, 0, 0};
+  select_batch_dev(c, ss, 16, g.sel, static_cast<uint32_t>(ni), g.groups);
+  k_number_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.items_final, p.tile_item, g.firsts, g.groups, g.slot);
+  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.items_final, p.tile_item, g.slot);
+  check_hip(hipGetLastError(), "plan group_id ids");
+}
+
+void plan_group_free(Ctx &c, PlanGroup *g) {
+  if (!g) return;
+  batch_plan_free(c, g->small_p);
+  batch_plan_free(c, g->fill_p);
+  tmp_free(c, g->small);
+  tmp_free(c, g->full);
+  tmp_free(c, g->over);
+  tmp_free(c, g->items_full);
+  tmp_free(c, g->items_final);
+  tmp_free(c, g->slot);
+  tmp_free(c, g->first);
+  tmp_free(c, g->firsts);
+  tmp_free(c, g->groups);
+  tmp_free(c, g->sel.items);
+  tmp_free(c, g->sel.tile_item);
+  tmp_free(c, g->sel.ntiles);
+  tmp_free(c, g->sel.ticket);
+  tmp_free(c, g->sel.status);
+  delete g;
 }
 
 }  // namespace vk

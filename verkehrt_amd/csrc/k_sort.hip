@@ -170,6 +170,12 @@ struct PassArgs {
   const BatchItem *items;
   const uint32_t *tile_item;
   int P, d;
+  // batch plans: the pass is slot `slot` of the digits listed on the device (slots[0] = their
+  // count, slots[1 + k] = the k-th), and the epoch is read from epoch_ptr; a launch whose slot
+  // is past the list, or is (or is not) the last one against LAST, exits before taking a ticket
+  const int32_t *slots;
+  int slot;
+  const uint32_t *epoch_ptr;
 };
 
 // FIRST: keys are the typed input (imaged here); the payload starts as described above.
@@ -185,19 +191,26 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
   __shared__ uint32_t s_wsum[kWaves];
 
   const int32_t type = a.type;
+  int pd = a.d, shift = a.shift;
+  uint32_t epoch = a.epoch;
+  if (a.slots) {
+    const int m = a.slots[0];
+    if (a.slot >= m || (a.slot == m - 1) != LAST) return;  // uniform over the grid
+    pd = a.slots[1 + a.slot];
+    shift = 8 * pd;
+    epoch = *a.epoch_ptr;
+  }
   const uint32_t tile = take_ticket(a.ticket, a.ntiles);
   uint32_t first = 0;
   const void *keys_in = a.keys_in;
   const V *__restrict__ pos_in = a.pay_in;
   const uint64_t *__restrict__ vals = a.vals;
   uint64_t n = a.n;
-  const int shift = a.shift;
   const uint32_t *__restrict__ digit_base = a.digit_base;
   U *__restrict__ keys_out = a.keys_out;
   V *__restrict__ pos_out = a.pay_out;
   int64_t *__restrict__ out_pos = a.out_pos;
   uint64_t *status = a.status;
-  const uint32_t epoch = a.epoch;
   if (BATCH) {
     const uint32_t it = a.tile_item[tile];
     const BatchItem &I = a.items[it];
@@ -207,7 +220,7 @@ __global__ __launch_bounds__(kSortBlock) void k_pass(const PassArgs<U, V> a) {
     keys_in = FIRST ? I.a : static_cast<const void *>(static_cast<const U *>(a.keys_in) + row0);
     if (FIRST) vals = static_cast<const uint64_t *>(I.b);
     else pos_in += row0;
-    digit_base += (uint64_t(it) * a.P + a.d) * kRadix;
+    digit_base += (uint64_t(it) * a.P + pd) * kRadix;
     if (LAST && POS) {
       out_pos = static_cast<int64_t *>(I.out);
       keys_out = nullptr;
@@ -1582,6 +1595,153 @@ void sort_key_values_batch(Ctx &c, bool wide, std::vector<BatchItem> &items, voi
     radix_batch<uint64_t, uint64_t>(c, VK_U64, items, static_cast<uint64_t *>(sorted_keys), sorted_vals);
   else
     radix_batch<uint32_t, uint64_t>(c, kRawU32, items, static_cast<uint32_t *>(sorted_keys), sorted_vals);
+}
+
+
+// ------------------------------------------------------------ sorts inside batch plans
+// A batch plan's argsort, and the (chain key, value) sort of its reduce_into_add_f64, run with
+// no host read, so that a plan captures them into its graph. The digits that vary in some item
+// are listed on the device (k_plan_slots), and each of the P pass slots is launched twice, as a
+// middle pass and as the last pass; k_pass exits at once, in every block, when its slot is not
+// its role. Slot k reads what slot k - 1 wrote (buffers by parity), so the passes are the
+// batched sort's over the active digits. Each slot has its own look-back status words, all
+// stamped with the plan's device epoch.
+
+namespace {
+
+__global__ void k_plan_slots(const uint32_t *active, int P, int pos, int32_t *slots) {
+  if (threadIdx.x != 0) return;
+  int m = 0;
+  for (int d = 0; d < P; ++d)
+    if (active[d]) slots[1 + m++] = d;
+  // with no digit to sort by, one stable pass on digit 0 still moves (key, value) rows as they are
+  if (m == 0 && !pos) slots[1 + m++] = 0;
+  slots[0] = m;
+}
+
+// Positions of items with no varying digit in any item: the identity.
+__global__ void k_iota_plan(const BatchItem *items, const uint32_t *tile_item, const int32_t *slots) {
+  if (slots[0] != 0) return;
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const uint64_t t0 = (blockIdx.x - I.tile0) * uint64_t(kTile);
+  const uint64_t t1 = t0 + kTile < I.n ? t0 + kTile : I.n;
+  int64_t *out = static_cast<int64_t *>(I.out);
+  for (uint64_t r = t0 + threadIdx.x; r < t1; r += kBlock) out[r] = static_cast<int64_t>(r);
+}
+
+template <typename U, typename V>
+void plan_sort_run_as(Ctx &c, PlanSort &s, const uint32_t *epoch) {
+  constexpr int P = sizeof(U);
+  constexpr bool POS = std::is_same<V, uint32_t>::value;
+  if (s.bp.ntiles == 0) return;
+  uint32_t *active = s.hist + s.nitems * P * kRadix;
+  fill_u64(c, reinterpret_cast<uint64_t *>(s.hist), (s.nitems * P * kRadix + 16) / 2, 0);
+  k_hist_batch<U><<<s.bp.ntiles, kBlock, 0, c.stream>>>(s.type, s.bp.items, s.bp.tile_item, s.hist);
+  k_base_batch<<<stream_grid(s.nitems * P, kBlock, 1u << 20), kBlock, 0, c.stream>>>(s.bp.items, s.nitems, P,
+                                                                                   s.hist, active);
+  k_plan_slots<<<1, 64, 0, c.stream>>>(active, P, POS ? 1 : 0, s.slots);
+  if (POS) k_iota_plan<<<s.bp.ntiles, kBlock, 0, c.stream>>>(s.bp.items, s.bp.tile_item, s.slots);
+  for (int k = 0; k < P; ++k) {
+    for (int last = 0; last < 2; ++last) {
+      PassArgs<U, V> a{};
+      a.type = s.type;
+      a.keys_in = k ? s.kbuf[(k - 1) & 1] : nullptr;
+      a.pay_in = k ? static_cast<const V *>(s.pbuf[(k - 1) & 1]) : nullptr;
+      a.digit_base = s.hist;
+      a.keys_out = static_cast<U *>(last ? s.sorted_keys : s.kbuf[k & 1]);
+      a.pay_out = static_cast<V *>(last ? static_cast<void *>(s.sorted_vals) : s.pbuf[k & 1]);
+      a.status = s.status + uint64_t(k) * s.bp.ntiles * kRadix;
+      a.epoch_ptr = epoch;
+      a.ticket = s.ticket;
+      a.ntiles = s.bp.ntiles;
+      a.items = s.bp.items;
+      a.tile_item = s.bp.tile_item;
+      a.P = P;
+      a.slots = s.slots;
+      a.slot = k;
+      if (k == 0 && last) k_pass<U, V, true, true, true><<<s.bp.ntiles, kSortBlock, 0, c.stream>>>(a);
+      else if (k == 0) k_pass<U, V, true, false, true><<<s.bp.ntiles, kSortBlock, 0, c.stream>>>(a);
+      else if (last) k_pass<U, V, false, true, true><<<s.bp.ntiles, kSortBlock, 0, c.stream>>>(a);
+      else k_pass<U, V, false, false, true><<<s.bp.ntiles, kSortBlock, 0, c.stream>>>(a);
+    }
+  }
+  check_hip(hipGetLastError(), "plan sort");
+}
+
+int key_bytes(int32_t type, bool wide) {
+  switch (type) {
+    case VK_I32: return 4;
+    case VK_U8: return 1;
+    case kChainKey: return wide ? 8 : 4;
+    default: return 8;
+  }
+}
+
+}  // namespace
+
+int32_t chain_key_type() { return kChainKey; }
+
+PlanSort *plan_sort_new(Ctx &c, int32_t type, bool pos, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
+                        uint64_t *sorted_vals) {
+  auto *s = new PlanSort{};
+  s->type = type;
+  s->pos = pos;
+  s->wide = wide;
+  s->nitems = items.size();
+  uint64_t total = 0;
+  for (auto &it : items) {
+    if (it.n > 0xFFFFFFFFull) usage_error("argsort supports at most 2^32 - 1 rows");
+    it.aux = total;
+    total += it.n;
+  }
+  s->total = total;
+  const int P = key_bytes(type, wide);
+  s->P = P;
+  s->bp = batch_plan(c, items, kTile);
+  s->hist = static_cast<uint32_t *>(tmp_alloc(c, (s->nitems * P * kRadix + 16) * 4));
+  s->slots = static_cast<int32_t *>(tmp_alloc(c, 64));
+  const uint64_t vb = pos ? 4 : 8;
+  for (int k = 0; k < 2; ++k) {
+    s->kbuf[k] = tmp_alloc(c, (total ? total : 1) * uint64_t(P));
+    s->pbuf[k] = tmp_alloc(c, (total ? total : 1) * vb);
+  }
+  s->sorted_keys = sorted_keys;
+  s->sorted_vals = sorted_vals;
+  s->status_words = uint64_t(P) * s->bp.ntiles * kRadix;
+  s->status = static_cast<uint64_t *>(tmp_alloc(c, (s->status_words ? s->status_words : 1) * 8));
+  fill_u64(c, s->status, s->status_words, 0);
+  s->ticket = static_cast<uint32_t *>(tmp_alloc(c, 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(s->ticket), 1, 0);
+  return s;
+}
+
+void plan_sort_run(Ctx &c, PlanSort &s, const uint32_t *epoch) {
+  if (s.pos) {
+    switch (s.P) {
+      case 8: plan_sort_run_as<uint64_t, uint32_t>(c, s, epoch); break;
+      case 4: plan_sort_run_as<uint32_t, uint32_t>(c, s, epoch); break;
+      default: plan_sort_run_as<uint8_t, uint32_t>(c, s, epoch); break;
+    }
+  } else if (s.P == 8) {
+    plan_sort_run_as<uint64_t, uint64_t>(c, s, epoch);
+  } else {
+    plan_sort_run_as<uint32_t, uint64_t>(c, s, epoch);
+  }
+}
+
+void plan_sort_free(Ctx &c, PlanSort *s) {
+  if (!s) return;
+  batch_plan_free(c, s->bp);
+  tmp_free(c, s->hist);
+  tmp_free(c, s->slots);
+  for (int k = 0; k < 2; ++k) {
+    tmp_free(c, s->kbuf[k]);
+    tmp_free(c, s->pbuf[k]);
+  }
+  tmp_free(c, s->status);
+  tmp_free(c, s->ticket);
+  delete s;
 }
 
 }  // namespace vk

@@ -346,6 +346,64 @@ void check_keys(Ctx &c, const int64_t *keys, uint64_t n, uint64_t buckets, uint6
 void argsort(Ctx &c, int32_t type, const void *keys, uint64_t n, int64_t *out);
 // Batched argsort: item i sorts keys a (n rows) into the i64 positions at out; aux is set.
 void argsort_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items);
+// Sorts inside batch plans (k_sort.hip): every launch reads its decisions from the device, so a
+// plan captures them. Positions mode (pos) writes each item's i64 positions to item.out; (key,
+// value) mode sorts the keys' images (type kChainKey: i64 bucket k of row r as k * 16 + r % 16,
+// u64 images when wide, else u32) with the u64 values of item.b into sorted_keys / sorted_vals
+// at each item's first row of the concatenated rows (item.aux, set here).
+struct PlanSort {
+  int32_t type = 0;
+  bool pos = true, wide = true;
+  int P = 8;
+  uint64_t nitems = 0, total = 0;
+  BatchPlan bp;
+  uint32_t *hist = nullptr;     // per item and digit: counts, then bases; then the active digits
+  int32_t *slots = nullptr;     // [0] active digits, [1 + k] the k-th
+  void *kbuf[2] = {nullptr, nullptr}, *pbuf[2] = {nullptr, nullptr};
+  void *sorted_keys = nullptr;
+  uint64_t *sorted_vals = nullptr;
+  uint64_t *status = nullptr;   // P slots x tiles x 256 look-back words
+  uint64_t status_words = 0;
+  uint32_t *ticket = nullptr;
+};
+int32_t chain_key_type();
+// group_id inside batch plans (k_group.hip): item.a the keys, item.out the ids (n rows);
+// groups[i] receives item i's group count. Status words of its selection: sel.status.
+struct PlanGroup {
+  int32_t type = 0;
+  uint64_t nitems = 0, total = 0, small_words = 0;
+  BatchPlan small_p, fill_p;
+  BatchItem *items_full = nullptr, *items_final = nullptr;
+  unsigned long long *small = nullptr, *full = nullptr;
+  uint32_t *over = nullptr, *slot = nullptr;
+  uint8_t *first = nullptr;
+  int64_t *firsts = nullptr;
+  uint64_t *groups = nullptr;
+  DevPlan sel;
+  uint64_t status_words = 0;
+};
+PlanGroup *plan_group_new(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint32_t *epoch);
+void plan_group_run(Ctx &c, PlanGroup &g);
+void plan_group_free(Ctx &c, PlanGroup *g);
+// reduce_into_add_f64 inside batch plans (k_bucket.hip): item.a the keys (group ids of a
+// group_id statement), item.b the f64 values, item.n the rows, item.out the output (room for
+// n rows); groups: the group_id statement's device group counts, the bucket counts.
+struct PlanFold {
+  uint64_t nitems = 0;
+  const uint64_t *groups = nullptr;
+  bool wide = false;
+  void *sk = nullptr;
+  double *sv = nullptr;
+  PlanSort *sort = nullptr;
+  DevPlan dp;
+};
+PlanFold *plan_fold_new(Ctx &c, std::vector<BatchItem> &items, const uint64_t *groups);
+void plan_fold_run(Ctx &c, PlanFold &f, const uint32_t *epoch);
+void plan_fold_free(Ctx &c, PlanFold *f);
+PlanSort *plan_sort_new(Ctx &c, int32_t type, bool pos, bool wide, std::vector<BatchItem> &items, void *sorted_keys,
+                        uint64_t *sorted_vals);
+void plan_sort_run(Ctx &c, PlanSort &s, const uint32_t *epoch);
+void plan_sort_free(Ctx &c, PlanSort *s);
 // Stable sort of u64 keys: sorted keys and the positions they came from.
 void sort_u64_pairs(Ctx &c, const uint64_t *keys, uint64_t n, uint64_t *sorted_keys, int64_t *perm);
 // Stable sort of (key, 8-byte value) pairs by key: keys are u64 (wide) or u32.
