@@ -156,7 +156,8 @@ def test_plan_sort_and_group(D):
     captured plan, beside filter+sum: every replay against the single calls and the oracle."""
     from verkehrt_amd import Input, Stmt
     rng = np.random.default_rng(23)
-    sizes = [0, 1, 15, 16, 17, 300, 4095, 4096, 4097, 8193, 16385, 70000, 200003, 1048577]
+    # item 12 has 1.2M distinct keys: more bucket tiles (of 16 buckets) than the fold's grid
+    sizes = [0, 1, 15, 16, 17, 300, 4095, 4096, 4097, 8193, 16385, 70000, 1_200_000, 1048577]
     sizes += [int(v) for v in rng.integers(0, 120000, 12)]
     items, host = [], []
     pool = rng.integers(-(1 << 40), 1 << 40, 256)

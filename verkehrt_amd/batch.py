@@ -182,6 +182,38 @@ def filter_sum_plan(D: Dispatch, wl: Workload, tune: bool = True):
     return plan
 
 
+def program_plan(D: Dispatch, wl: Workload, tune: bool = True):
+    """The whole program as one batch plan, replayed as one hipGraph: filter and sum, argsort,
+    group_id, and reduce_into_add_f64 whose bucket count is the group_id statement's group count
+    on the device (reduce_max_i64(g) + 1 by definition of first-occurrence ids)."""
+    from .kernels import Input, Stmt
+    plan = D.batch_plan([[x, k] for x, k in (wl.item(i) for i in range(len(wl.lengths)))])
+    plan.add("filter_gt_f64", [Input(0), Scalar("f64", 0.9)])
+    plan.add("reduce_add_f64", [Stmt(0)])
+    plan.add("argsort_i64", [Input(1)])
+    plan.add("group_id_i64", [Input(1)])
+    plan.add("reduce_into_add_f64", [Stmt(3), Input(0), Stmt(3)])
+    plan.compile(tune=tune)
+    return plan
+
+
+def program_plan_run(D: Dispatch, plan) -> np.ndarray:
+    """One replay of the program plan; waits for it by reading every item's sum bits."""
+    plan.run()
+    return plan.bits(1)
+
+
+def program_plan_results(plan, every: int = 1) -> Results:
+    """The program plan's results after a replay: all sums, every `every`-th item's
+    permutation and group sums."""
+    bits = plan.bits(1)
+    n = len(bits)
+    sums = [Scalar.from_bits("f64", int(b)) for b in bits]
+    perms = [plan.result(2, i) for i in range(0, n, every)]
+    groups = [plan.result(4, i) for i in range(0, n, every)]
+    return Results(sums, perms, groups)
+
+
 def plan_sums(D: Dispatch, plan) -> np.ndarray:
     """One replay of a filter-then-sum plan; its sums as f64 bits (one read-back for all)."""
     plan.run()

@@ -920,14 +920,11 @@ namespace {
 
 // Fold items: n = the bucket count (written by the device planner), a keys (i64), b values,
 // out the output, aux the item's first row in the sorted buffers, aux2 its whole rows, aux3 its
-// rows.
+// rows. A block takes tiles tile, tile + gridDim.x, ... (the grid is capped: a launch may not
+// exceed 2^32 work-items).
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_fold_combine_plan(const BatchItem *items, const uint32_t *tile_item,
-                                                              const uint32_t *ntiles, const K *sk,
-                                                              const double *sv) {
-  if (blockIdx.x >= *ntiles) return;
-  const BatchItem &I = items[tile_item[blockIdx.x]];
-  const uint64_t ck = (blockIdx.x - I.tile0) * kBlock + threadIdx.x;  // the lane's chain key
+__device__ __forceinline__ void fold_combine_tile(const BatchItem &I, uint32_t tile, const K *sk, const double *sv) {
+  const uint64_t ck = (tile - I.tile0) * kBlock + threadIdx.x;  // the lane's chain key
   const uint64_t bucket = ck / 16;
   const int j = static_cast<int>(ck & 15);
   const bool valid = bucket < I.n;
@@ -969,6 +966,17 @@ __global__ __launch_bounds__(kBlock) void k_fold_combine_plan(const BatchItem *i
       if (static_cast<uint64_t>(keys[r]) == bucket) a = a + vals[r];
     static_cast<double *>(I.out)[bucket] = a;
   }
+}
+
+constexpr uint32_t kFoldGrid = 65536;
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_fold_combine_plan(const BatchItem *items, const uint32_t *tile_item,
+                                                              const uint32_t *ntiles, const K *sk,
+                                                              const double *sv) {
+  const uint32_t nt = *ntiles;
+  for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x)  // uniform over the block
+    fold_combine_tile(items[tile_item[tile]], tile, sk, sv);
 }
 
 }  // namespace
@@ -1019,12 +1027,13 @@ void plan_fold_run(Ctx &c, PlanFold &f, const uint32_t *epoch) {
   plan_sort_run(c, *f.sort, epoch);
   plan_tiles_dev(c, f.dp, static_cast<uint32_t>(f.nitems), f.groups, 1, kBlock / 16);
   if (f.dp.max_tiles) {
+    const uint32_t grid = f.dp.max_tiles < kFoldGrid ? f.dp.max_tiles : kFoldGrid;
     if (f.wide)
-      k_fold_combine_plan<uint64_t><<<f.dp.max_tiles, kBlock, 0, c.stream>>>(
-          f.dp.items, f.dp.tile_item, f.dp.ntiles, static_cast<const uint64_t *>(f.sk), f.sv);
+      k_fold_combine_plan<uint64_t><<<grid, kBlock, 0, c.stream>>>(f.dp.items, f.dp.tile_item, f.dp.ntiles,
+                                                                  static_cast<const uint64_t *>(f.sk), f.sv);
     else
-      k_fold_combine_plan<uint32_t><<<f.dp.max_tiles, kBlock, 0, c.stream>>>(
-          f.dp.items, f.dp.tile_item, f.dp.ntiles, static_cast<const uint32_t *>(f.sk), f.sv);
+      k_fold_combine_plan<uint32_t><<<grid, kBlock, 0, c.stream>>>(f.dp.items, f.dp.tile_item, f.dp.ntiles,
+                                                                  static_cast<const uint32_t *>(f.sk), f.sv);
   }
   check_hip(hipGetLastError(), "plan reduce_into_add_f64");
 }
