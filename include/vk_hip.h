@@ -265,8 +265,12 @@ int vk_batch_call(vk_ctx *ctx, const char *name, const vk_arg *args, size_t narg
  * each item alone, bit for bit.
  * Plannable kernels: filter_<cmp>_<t> (over a plan input or an earlier selection),
  * compress_<t> (plan inputs), reduce_add_f64, sumsq_f64 and condsum_<cmp>_f64 (over a plan
- * input or a selection), dot_f64 (plan inputs), and the other reductions (over a plan input or
- * a selection). A plan retains its inputs; they must not change while it is in use. */
+ * input or a selection), dot_f64 (plan inputs), the other reductions (over a plan input or a
+ * selection), argsort_<t> and group_id_<t> (plan inputs), and reduce_into_add_f64(ids, x, g)
+ * whose ids and bucket count g both name one group_id statement (its group count per item, on
+ * the device) and whose x is a plan input. Argsort, group_id and that bucket sum decide their
+ * radix digits, table sizes and bucket counts on the device, so they are captured too.
+ * A plan retains its inputs; they must not change while it is in use. */
 typedef struct vk_bplan vk_bplan;
 
 enum { VK_BREF_INPUT = 0, VK_BREF_STMT = 1, VK_BREF_SCALAR = 2 };
@@ -295,7 +299,8 @@ int vk_bplan_run(vk_bplan *plan, vk_status *st);
  * of the plan's buffer, one reference owned by the caller) or a scalar. */
 int vk_bplan_result(vk_bplan *plan, size_t stmt, size_t item, vk_value *out, vk_status *st);
 /* Statement `stmt`'s results for every item at once after the last run (waits for it): a
- * selection's kept row counts, or every item's scalar result bits (out has nitems words). */
+ * selection's kept row counts, an argsort's rows, a group_id's group counts, a bucket sum's
+ * bucket counts, or every item's scalar result bits (out has nitems words). */
 int vk_bplan_counts(vk_bplan *plan, size_t stmt, uint64_t *out, vk_status *st);
 /* The autotuner's measurements and choices, one line per tuned statement; returns the full
  * length (copies at most cap - 1 bytes and a terminating zero). */
