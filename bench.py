@@ -647,6 +647,47 @@ def zero_mean_suite(D, x1, n: int, verify: bool = True):
     return out
 
 
+def f32_int_suite(D, n: int, verify: bool = True):
+    """The f32 canonical sums (reduce_add_f32 over uniform and normal(0,1) rows, dot_f32) and an
+    integer dot (dot_i64 over uniform i64 rows in [-2^40, 2^40)) at n rows, made on the host
+    (numpy default_rng seed 7) and copied in; timed per call over 3 calls after one warm-up,
+    then checked bit for bit against the oracle."""
+    from verkehrt_amd import Array
+    rng = np.random.default_rng(7)
+    u = rng.random(n, dtype=np.float32)
+    z = rng.standard_normal(n, dtype=np.float32)
+    xu, xz = Array.from_numpy(D.ctx, u), Array.from_numpy(D.ctx, z)
+    out = {"data": "f32 uniform [0, 1) and normal(0,1) rows, i64 uniform [-2^40, 2^40), numpy default_rng seed 7",
+           "rows": n}
+
+    def entry(us, nbytes):
+        return {"us": round(us, 1), "GBps": round(nbytes / us / 1e3, 1),
+                "frac": round(nbytes / us / 1e3 / HBM_PEAK_GBPS, 4)}
+
+    for label, name, args, nbytes, want in (
+            ("reduce_add_f32", "reduce_add_f32", [xu], 4 * n, lambda O: O.canonical_fold_f32("sum", u)),
+            ("reduce_add_f32_normal", "reduce_add_f32", [xz], 4 * n, lambda O: O.canonical_fold_f32("sum", z)),
+            ("dot_f32", "dot_f32", [xu, xz], 8 * n, lambda O: O.canonical_fold_f32("dot", u, z))):
+        res, us = _timed_calls(D, name, args)
+        out[label] = entry(us, nbytes)
+        if verify:
+            from oracle import oracle as O
+            w = want(O)
+            out[label]["verified"] = all(np.float32(r.value).tobytes() == np.float32(w).tobytes() for r in res)
+    del xu, xz, u, z
+    a = rng.integers(-(1 << 40), 1 << 40, n)
+    b = rng.integers(-(1 << 40), 1 << 40, n)
+    xa, xb = Array.from_numpy(D.ctx, a), Array.from_numpy(D.ctx, b)
+    res, us = _timed_calls(D, "dot_i64", [xa, xb])
+    out["dot_i64"] = entry(us, 16 * n)
+    if verify:
+        from oracle import oracle as O
+        w = O.fused_sum_int("dot", a, b)
+        out["dot_i64"]["verified"] = all(r.value == w for r in res)
+    del xa, xb
+    return out
+
+
 def verify_streaming(D, calls, n, table_rows):
     """Each streaming primitive's 2^28-row result, computed once more after the timed region,
     against the oracle on the same inputs (as benches/tier1.rs:403-433 asserts before timing)."""
@@ -775,6 +816,9 @@ def main():
     if args.only == "zero_mean":
         print(json.dumps({"zero_mean": zero_mean_suite(D, x1, n, verify=not args.no_verify)}))
         return
+    if args.only == "f32_int":
+        print(json.dumps({"f32_int": f32_int_suite(D, n, verify=not args.no_verify)}))
+        return
     x2 = D.call("rand_f64", [S("i64", n), S("i64", 2)])
     i1 = Array.from_numpy(ctx, i64_uniform_40(n, 4))
     tbl = D.call("rand_f64", [S("i64", table_rows), S("i64", 3)])
@@ -855,6 +899,7 @@ def main():
         roofline["random_rows_G_per_s"] = round(n / prims[dom]["us"] / 1e3, 1)
     regions = None if args.only else region_suite(D, x1, x2, n)
     zero_mean = None if args.only else zero_mean_suite(D, x1, n, verify=not args.no_verify)
+    f32_int = None if args.only else f32_int_suite(D, n, verify=not args.no_verify)
     checks = None if (args.no_verify or args.only) else verify_streaming(D, calls, n, table_rows)
     sort = None if (args.no_sort or args.only) else sort_suite(D, verify=not args.no_verify)
     batched = None if (args.no_batched or args.only) else batched_suite(D, args.batch_arrays)
@@ -871,7 +916,8 @@ def main():
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.cpu_rows, args.cpu_seconds)
     verified = (checks is not None and all(checks.values()) and (sort is None or sort["verified"])
                 and (zero_mean is None or (all(zero_mean[k]["verified"] for k in ("reduce_add_f64", "dot_f64"))
-                                           and all(v["verified"] for v in zero_mean["scan_add_f64"].values()))))
+                                           and all(v["verified"] for v in zero_mean["scan_add_f64"].values())))
+                and (f32_int is None or all(v["verified"] for k, v in f32_int.items() if isinstance(v, dict))))
     line = {
         "metric": "HBM GB/s per primitive (% of peak); batched-vs-plain speedup; serve p99 ms",
         "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -888,6 +934,7 @@ def main():
         "primitives": prims,
         "regions": regions,
         "zero_mean": zero_mean,
+        "f32_int": f32_int,
         "batched": batched,
         "serve": serve,
         "sort": sort,

@@ -305,16 +305,17 @@ __host__ __device__ __forceinline__ WideAcc wide_start(double z) {
 // Folds the finished segment's constraint into the window: every running value v with this key
 // must keep v + delta within [2^e + u, 2^(e+1) - u] (mirrored for negative values). The bounds
 // are exact: both sides are multiples of u below 2^(e+1).
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ void wide_flush(WideAcc &a) {
   if (a.key == 0xFFFFFFFFu) return;
   const int ex = static_cast<int>(a.key & 0x7FF);
-  if (ex < 1023 - 960 || ex > 1023 + 960) {  // zero, subnormal, extreme, infinite or NaN
+  if (ex < 1023 + P::kEmin || ex > 1023 + P::kEmax) {  // zero, subnormal, extreme, infinite or NaN
     a.bad = true;
     return;
   }
   const int e = ex - 1023;
   a.emax = a.emax > e ? a.emax : e;
-  const double u = pow2(e - 52);
+  const double u = pow2(e - P::kMant);
   const double bot = pow2(e) + u, top = pow2(e + 1) - u;
   double lo, hi;
   if (a.key & 0x800) {
@@ -329,10 +330,11 @@ __host__ __device__ __forceinline__ void wide_flush(WideAcc &a) {
 }
 
 // Records the new running value r (a.cur is set by the caller).
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ void wide_visit(WideAcc &a, double r) {
   const uint32_t k = sign_exp_key(r);
   if (__builtin_expect(k != a.key, 0)) {
-    wide_flush(a);
+    wide_flush<P>(a);
     a.key = k;
     a.mn = r;
     a.mx = r;
@@ -342,20 +344,21 @@ __host__ __device__ __forceinline__ void wide_visit(WideAcc &a, double r) {
   }
 }
 
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ void wide_push(WideAcc &a, double v) {
-  a.cur = a.cur + v;
-  wide_visit(a, a.cur);
+  a.cur = P::rnd(a.cur + v);
+  wide_visit<P>(a, a.cur);
 }
 
 // N elements at once, with the same result as N wide_push calls: the running values first (the
 // only dependent chain), then, when all of them keep the current segment's key, one min and max
 // over them; otherwise they are visited one by one.
-template <int N>
+template <int N, class P = PrecF64>
 __host__ __device__ __forceinline__ void wide_push_n(WideAcc &a, const double (&v)[N]) {
   double r[N];
-  r[0] = a.cur + v[0];
+  r[0] = P::rnd(a.cur + v[0]);
 #pragma unroll
-  for (int m = 1; m < N; ++m) r[m] = r[m - 1] + v[m];
+  for (int m = 1; m < N; ++m) r[m] = P::rnd(r[m - 1] + v[m]);
   a.cur = r[N - 1];
   bool same = true;
   double mn = r[0], mx = r[0];
@@ -370,13 +373,14 @@ __host__ __device__ __forceinline__ void wide_push_n(WideAcc &a, const double (&
     a.mx = __builtin_fmax(a.mx, mx);
   } else {
 #pragma unroll
-    for (int m = 0; m < N; ++m) wide_visit(a, r[m]);
+    for (int m = 0; m < N; ++m) wide_visit<P>(a, r[m]);
   }
 }
 
 // The finished entry; an entry whose simulation met a value with no binade never applies.
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ WideEnt wide_end(WideAcc &a) {
-  wide_flush(a);
+  wide_flush<P>(a);
   const double inf = __builtin_inf();
   if (a.bad || !(a.lo <= a.hi)) return WideEnt{0.0, inf, -inf};
   return WideEnt{a.cur, a.lo, a.hi};
@@ -384,8 +388,9 @@ __host__ __device__ __forceinline__ WideEnt wide_end(WideAcc &a) {
 
 // B for the approximate start `approx` under hypothesis h (h.ok): approx less its residue
 // modulo 16 u0, exact.
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ double wide_base(double approx, const Binade &h) {
-  return approx - __builtin_fmod(approx, 16.0 * pow2(h.e - 52));
+  return approx - __builtin_fmod(approx, 16.0 * pow2(h.e - P::kMant));
 }
 
 // The number of entries to keep from the 16 simulated ones, given M, the greatest number of
@@ -396,15 +401,16 @@ __host__ __device__ __forceinline__ int wide_nres(int m) { return m < -1000 || m
 // Applies the table (head, entries) to the exact start s: returns true and the exact end value
 // when s falls in the entry's window. s and B must have one sign and lie within a factor of two
 // of each other (so s - B is exact), and s must be a multiple of u0.
+template <class P = PrecF64>
 __host__ __device__ __forceinline__ bool wide_apply(const WideHead &hd, const WideEnt *ents, double s,
                                                     double *out) {
   if (hd.nres <= 0) return false;
   const double b = hd.b;
   const bool near = b > 0 ? (s >= 0.5 * b && s <= 2.0 * b) : (s <= 0.5 * b && s >= 2.0 * b);
   if (!near) return false;
-  const double u0 = pow2(hd.e0 - 52), inv_u0 = pow2(52 - hd.e0);
+  const double u0 = pow2(hd.e0 - P::kMant), inv_u0 = pow2(P::kMant - hd.e0);
   // t = s - B is exact (Sterbenz) and below 2^(e0 + 1) in magnitude, so t / u0 is an integer
-  // below 2^53 when s is a multiple of u0; its low bits give the residue (nres is a power of 2)
+  // below 2^(kMant + 1) when s is a multiple of u0; its low bits give the residue (nres is a power of 2)
   const double t = s - b;
   const double k = t * inv_u0;
   if (k != __builtin_floor(k)) return false;

@@ -17,9 +17,11 @@
 //    canon_binade.h at f32 precision, joined in order), (4) one wave per chain walks the tiles
 //    from the exact start, 64 tiles at a time: the summaries of the tiles whose hypothesis is the
 //    running value's binade are joined by a lane scan and applied; a tile whose summary does not
-//    apply is evaluated from its rows (rounds of run summaries, or the plain sequential fold
-//    when the running value keeps changing binade). Every applied summary proves its result, so
-//    the result is the sequential one.
+//    apply goes by its wide table (canon_binade.h, built in parallel for every tile chain whose
+//    summary fails at its approximate start: the tiles where the running value crosses a binade),
+//    or, where that does not apply either, is evaluated from its rows (rounds of run summaries,
+//    or the plain sequential fold when the running value keeps changing binade). Every applied
+//    summary or table proves its result, so the result is the sequential one.
 // HBM traffic: two reads of the rows (4 bytes each, 8 for dot) plus 64 bytes per tile chain.
 #include "canon_binade.h"
 #include "vk_device.h"
@@ -58,11 +60,13 @@ constexpr int kC32Threads = 1024;
 constexpr int kC32Run = 16;                                   // blocks per run
 constexpr uint64_t kC32TileBlocks = kC32Threads / 16 * kC32Run;  // 1024 blocks, 16384 rows
 constexpr uint64_t kC32MinBlocks = 2 * kC32TileBlocks;
+constexpr int kC32Rounds = 3;          // summary rounds of a tile evaluated from its rows
+constexpr uint64_t kC32GroupTiles = 64;  // tiles per group of the walk
 
 struct TileFn32 {
   double d0, d1, lo0, lo1, hi0, hi1;
   uint64_t hword;  // the hypothesis (pack below)
-  uint64_t pad;
+  uint64_t slot;   // wide-table slot + 1, or 0
 };
 
 __device__ __forceinline__ uint64_t pack_h(const Binade &h) {
@@ -176,11 +180,14 @@ __global__ __launch_bounds__(kC32Threads) void k_c32_prefix(const double *asum, 
   }
 }
 
-// (3) fns[tile * 16 + j]: chain j's tile summary under the binade of its approximate start.
+// (3) fns[tile * 16 + j]: chain j's tile summary under the binade of its approximate start; a
+// summary that does not apply at that start takes a wide-table slot (counter, keys: tile * 16 + j)
+// while fewer than rcap are taken.
 template <class RV>
 __global__ __launch_bounds__(kC32Threads) void k_c32_summ(RV rv, const float *__restrict__ x,
                                                            const float *__restrict__ y, uint64_t blocks,
-                                                           const double *pre, TileFn32 *fns) {
+                                                           const double *pre, TileFn32 *fns, uint32_t *counter,
+                                                           uint32_t rcap, uint32_t *keys) {
   const uint64_t tile = blockIdx.x;
   const uint64_t b0 = tile * kC32TileBlocks;
   const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
@@ -204,8 +211,85 @@ __global__ __launch_bounds__(kC32Threads) void k_c32_summ(RV rv, const float *__
   if (t < 16) {
     RunFn g = s_f[0][t];
     for (int q = 1; q < kC32Threads / kWave; ++q) g = run_join(g, s_f[q][t]);
-    TileFn32 out{g.d[0], g.d[1], g.lo[0], g.lo[1], g.hi[0], g.hi[1], pack_h(h), 0};
+    const double start = pre[tile * 16 + t];
+    double probe;
+    uint64_t slot = 0;
+    if (h.ok && !run_apply<PrecF32>(g, h, start, &probe)) {
+      const uint32_t q = atomicAdd(counter, 1u);
+      if (q < rcap) {
+        keys[q] = static_cast<uint32_t>(tile * 16 + t);
+        slot = uint64_t(q) + 1;
+      }
+    }
+    TileFn32 out{g.d[0], g.d[1], g.lo[0], g.lo[1], g.hi[0], g.hi[1], pack_h(h), slot};
     fns[tile * 16 + t] = out;
+  }
+}
+
+// (3b) The wide tables of the tile chains that took a slot, four per wave: the four chains'
+// rows staged in LDS, then lane 16 q + r simulates residue r of chain q from Z_r = B + r u0 at
+// f32 precision (as k_canon_wide does at f64).
+constexpr int kC32WideSlots = 4;
+template <class RV>
+__global__ __launch_bounds__(kWave) void k_c32_wide(RV rv, const float *__restrict__ x, const float *__restrict__ y,
+                                                     uint64_t blocks, const double *pre, const uint32_t *keys,
+                                                     const uint32_t *counter, uint32_t rcap, WideHead *heads,
+                                                     WideEnt *ents) {
+  __shared__ double s_v[kC32WideSlots][kC32TileBlocks];
+  const uint32_t nslots = *counter < rcap ? *counter : rcap;
+  const int l = lane_id(), q = l >> 4, r = l & 15;
+  for (uint32_t s0 = blockIdx.x * kC32WideSlots; s0 < nslots; s0 += gridDim.x * kC32WideSlots) {
+    const uint32_t sl = s0 + q;
+    const bool live = sl < nslots;
+    const uint32_t key = live ? keys[sl] : 0;
+    const uint64_t tile = key >> 4;
+    const int j = static_cast<int>(key & 15);
+    const double a = live ? pre[tile * 16 + j] : 0.0;
+    const Binade h = binade_of<PrecF32>(a);
+    const bool go = live && h.ok;
+    const uint64_t b0 = tile * kC32TileBlocks;
+    const int nb = go ? static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks) : 0;
+    {
+      double st[kC32WideSlots][kC32TileBlocks / kWave];
+#pragma unroll
+      for (int qq = 0; qq < kC32WideSlots; ++qq) {
+        const uint64_t bq = __shfl(b0, 16 * qq);
+        const int jq = __shfl(j, 16 * qq), nq = __shfl(nb, 16 * qq);
+#pragma unroll
+        for (int m = 0; m < static_cast<int>(kC32TileBlocks / kWave); ++m) {
+          const int k = l + kWave * m;
+          st[qq][m] = k < nq ? rv(x, y, (bq + k) * 16 + jq) : 0.0;
+        }
+      }
+#pragma unroll
+      for (int qq = 0; qq < kC32WideSlots; ++qq)
+#pragma unroll
+        for (int m = 0; m < static_cast<int>(kC32TileBlocks / kWave); ++m) s_v[qq][l + kWave * m] = st[qq][m];
+    }
+    __syncthreads();
+    const double base = go ? wide_base<PrecF32>(a, h) : 1.0;
+    WideAcc acc = wide_start(base + r * pow2(go ? h.e - PrecF32::kMant : -PrecF32::kMant));
+#pragma unroll 2
+    for (int k0 = 0; k0 < static_cast<int>(kC32TileBlocks); k0 += 16) {
+      double v[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = s_v[q][k0 + m];
+      wide_push_n<16, PrecF32>(acc, v);
+    }
+    __syncthreads();
+    const WideEnt e = wide_end<PrecF32>(acc);
+    int mm = e.lo <= e.hi ? acc.emax - h.e : -2000;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int o = __shfl_xor(mm, d);
+      mm = o > mm ? o : mm;
+    }
+    const int nres = go ? wide_nres(mm) : 0;
+    if (live) {
+      const double inf = __builtin_inf();
+      if (r == 0) heads[sl] = WideHead{base, h.e, nres};
+      ents[uint64_t(sl) * kWideRes + r] = r < nres ? e : WideEnt{0.0, inf, -inf};
+    }
   }
 }
 
@@ -236,9 +320,12 @@ __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint
   const Binade ha = binade_of<PrecF32>(s + (inc - asum));
   const Binade hp{__shfl_up(ha.e, 1), __shfl_up(ha.neg, 1), __shfl_up(ha.ok, 1), 0};
   const uint64_t changes = __ballot(l > 0 && l < nruns && !same_binade(ha, hp));
+  // few changes: rounds (at most kC32Rounds; a running value that keeps failing its run
+  // summaries is close to a binade edge, where the plain fold is cheaper than more rounds)
   double cur = s;
+  int r0 = 0;
   if (__popcll(changes) <= 2) {
-    for (int r0 = 0; r0 < nruns;) {
+    for (int round = 0; round < kC32Rounds && r0 < nruns; ++round) {
       const Binade h = binade_of<PrecF32>(cur);
       RunFn f = run_identity();
       if (h.ok && l >= r0) {
@@ -253,10 +340,7 @@ __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint
       double val = cur;
       const bool ok = l < r0 || run_apply<PrecF32>(f, h, cur, &val);
       const uint64_t fails = __ballot(!ok);
-      if (fails == 0) {
-        cur = __shfl(val, kWave - 1);
-        break;
-      }
+      if (fails == 0) return __shfl(val, kWave - 1);
       const int m = __builtin_ctzll(fails);
       double t = m > r0 ? __shfl(val, m - 1) : cur;
       if (l == m) {
@@ -267,57 +351,153 @@ __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint
       cur = __shfl(t, m);
       r0 = m + 1;
     }
-    return cur;
   }
-  for (int r = 0; r < nruns; ++r) {
-    const int nk = nb - r * kC32Run < kC32Run ? nb - r * kC32Run : kC32Run;
+  // the sequential fold of runs r0 .. in f32 arithmetic (the same operations: every value is an
+  // f32 value), each element read from its run's lane
+  float vf[kC32Run];
 #pragma unroll
-    for (int k = 0; k < kC32Run; ++k)
-      if (k < nk) cur = r32(cur + bcast_pod(v[k], r));
+  for (int k = 0; k < kC32Run; ++k) vf[k] = static_cast<float>(v[k]);
+  float cf = static_cast<float>(cur);
+  for (int r = r0; r < nruns; ++r) {
+    const int nk = nb - r * kC32Run < kC32Run ? nb - r * kC32Run : kC32Run;
+    if (nk == kC32Run) {
+#pragma unroll
+      for (int k = 0; k < kC32Run; ++k) cf = cf + bcast_pod(vf[k], r);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kC32Run; ++k)
+        if (k < nk) cf = cf + bcast_pod(vf[k], r);
+    }
   }
-  return cur;
+  return cf;
 }
 
-// (4) One wave per chain: acc[j] <- the exact fold of chain j over every tile.
+// Segmented walk of up to 64 lanes from the exact value s (lane i: summary f of unit i under
+// the hypothesis word hw, bit 0 set when there is one): a segmented scan joins each run of
+// consecutive lanes with one hypothesis word; segment by segment the joined summary is applied
+// at once, and a segment whose summary does not apply goes lane by lane, a lane whose own
+// summary does not apply going through one(i, s).
+template <class One>
+__device__ double walk_lanes32(const RunFn &fl, uint64_t hw, int n, double s, One &&one) {
+  const int l = lane_id();
+  const bool valid = l < n;
+  const bool usable = valid && (hw & 1);
+  const uint64_t hprev = __shfl_up(hw, 1);
+  const bool uprev = __shfl_up(static_cast<int>(usable), 1) != 0;
+  const bool start = l == 0 || !usable || !uprev || hw != hprev;
+  RunFn sc = fl;
+  int ss = start ? 1 : 0;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const RunFn o = shfl_up_pod(sc, d);
+    const int os = __shfl_up(ss, d);
+    if (l >= d && !ss) {
+      sc = run_join(o, sc);
+      ss = os;
+    }
+  }
+  const uint64_t starts = __ballot(start && valid);
+  uint64_t ends = ((starts >> 1) | (uint64_t(1) << (n - 1))) & (n == kWave ? ~uint64_t(0) : (uint64_t(1) << n) - 1);
+  int first = 0;
+  while (ends) {
+    const int e = __builtin_ctzll(ends);
+    ends &= ends - 1;
+    const uint64_t he = bcast_pod(hw, e);
+    double out;
+    if ((he & 1) && run_apply<PrecF32>(bcast_pod(sc, e), unpack_h(he), s, &out)) {
+      s = out;
+    } else {
+      for (int i = first; i <= e; ++i) {
+        const uint64_t hi = bcast_pod(hw, i);
+        if ((hi & 1) && run_apply<PrecF32>(bcast_pod(fl, i), unpack_h(hi), s, &out)) s = out;
+        else s = one(i, s);
+      }
+    }
+    first = e + 1;
+  }
+  return s;
+}
+
+// The tiles t0 .. t0 + n - 1 of chain j from the exact value s; a tile whose summary does not
+// apply takes its wide table or, failing that, its rows.
+template <class RV>
+__device__ double walk_tiles32(const RV &rv, const float *x, const float *y, uint64_t blocks, const TileFn32 *fns,
+                               const WideHead *heads, const WideEnt *ents, WideEnt *s_e, int j, uint64_t t0, int n,
+                               double s) {
+  const int l = lane_id();
+  TileFn32 q{};
+  if (l < n) q = fns[(t0 + l) * 16 + j];
+  const RunFn fl{{q.d0, q.d1}, {q.lo0, q.lo1}, {q.hi0, q.hi1}};
+  const uint64_t hw = l < n ? q.hword : 0;
+  const uint64_t slot = l < n ? q.slot : 0;
+  return walk_lanes32(fl, hw, n, s, [&](int i, double c) {
+    const uint64_t sl = bcast_pod(slot, i);
+    double out;
+    if (sl) {
+      const WideHead hd = heads[sl - 1];
+      const WideEnt we = ents[(sl - 1) * kWideRes + (l & (kWideRes - 1))];
+      if (l < kWideRes) s_e[l] = we;
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      const bool done = wide_apply<PrecF32>(hd, s_e, c, &out);
+      __builtin_amdgcn_wave_barrier();
+      if (done) return out;
+    }
+    const uint64_t b0 = (t0 + i) * kC32TileBlocks;
+    const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
+    return eval_tile32(rv, x, y, b0, nb, j, c);
+  });
+}
+
+// (3c) One wave per (chain j, group of 64 tiles): the group's tiles joined in order into
+// gfn[j * groups + g], with the group's hypothesis word when every tile has that one (else 0).
+__global__ __launch_bounds__(kBlock) void k_c32_groups(const TileFn32 *fns, uint64_t tiles, uint64_t groups,
+                                                        TileFn32 *gfn) {
+  const uint64_t wid = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
+  if (wid >= 16 * groups) return;
+  const int l = lane_id();
+  const uint64_t j = wid / groups, g = wid % groups;
+  const uint64_t tile = g * kC32GroupTiles + l;
+  const bool valid = tile < tiles;
+  RunFn f = run_identity();
+  uint64_t hw = 0;
+  if (valid) {
+    const TileFn32 q = fns[tile * 16 + j];
+    f = RunFn{{q.d0, q.d1}, {q.lo0, q.lo1}, {q.hi0, q.hi1}};
+    hw = q.hword;
+  }
+  const uint64_t h0 = __shfl(hw, 0);
+  const bool clean = __ballot(valid && (hw != h0 || !(hw & 1))) == 0;
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    const RunFn o = shfl_xor_pod(f, m);
+    f = (l & m) ? run_join(o, f) : run_join(f, o);
+  }
+  if (l == 0) gfn[j * groups + g] = TileFn32{f.d[0], f.d[1], f.lo[0], f.lo[1], f.hi[0], f.hi[1], clean ? h0 : 0, 0};
+}
+
+// (4) One wave per chain: acc[j] <- the exact fold of chain j over every tile, 64 groups of 64
+// tiles at a time; a group whose joined summary does not apply goes tile by tile.
 template <class RV>
 __global__ __launch_bounds__(kWave) void k_c32_walk(RV rv, const float *__restrict__ x,
                                                      const float *__restrict__ y, uint64_t blocks,
-                                                     const TileFn32 *fns, uint64_t tiles, double *acc) {
+                                                     const TileFn32 *fns, uint64_t tiles, const TileFn32 *gfn,
+                                                     uint64_t groups, const WideHead *heads, const WideEnt *ents,
+                                                     double *acc) {
+  __shared__ WideEnt s_e[kWideRes];
   const int j = blockIdx.x, l = lane_id();
   double s = acc[j];
-  uint64_t t0 = 0;
-  while (t0 < tiles) {
-    const uint64_t tt = t0 + l;
-    const bool valid = tt < tiles;
-    RunFn f = run_bad();
-    Binade h{0, 0, 0, 0};
-    if (valid) {
-      const TileFn32 q = fns[tt * 16 + j];
-      f = RunFn{{q.d0, q.d1}, {q.lo0, q.lo1}, {q.hi0, q.hi1}};
-      h = unpack_h(q.hword);
-    }
-    const Binade hs = binade_of<PrecF32>(s);
-    const bool same = valid && same_binade(h, hs);
-    if (!same) f = run_bad();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const RunFn o = shfl_up_pod(f, d);
-      if (l >= d) f = run_join(o, f);
-    }
-    double val = s;
-    const bool ok = same && run_apply<PrecF32>(f, hs, s, &val);
-    const uint64_t fails = __ballot(!ok);
-    const int m = fails ? __builtin_ctzll(fails) : kWave;
-    if (m > 0) {
-      s = __shfl(val, m - 1);
-      t0 += uint64_t(m);
-      continue;
-    }
-    // tile t0's summary does not apply from s: evaluate the tile from its rows
-    const uint64_t b0 = t0 * kC32TileBlocks;
-    const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
-    s = eval_tile32(rv, x, y, b0, nb, j, s);
-    t0 += 1;
+  for (uint64_t g0 = 0; g0 < groups; g0 += kWave) {
+    const int n = groups - g0 < uint64_t(kWave) ? static_cast<int>(groups - g0) : kWave;
+    TileFn32 q{};
+    if (l < n) q = gfn[uint64_t(j) * groups + g0 + l];
+    const RunFn fl{{q.d0, q.d1}, {q.lo0, q.lo1}, {q.hi0, q.hi1}};
+    const uint64_t hw = l < n ? q.hword : 0;
+    s = walk_lanes32(fl, hw, n, s, [&](int i, double c) {
+      const uint64_t t0 = (g0 + i) * kC32GroupTiles;
+      const int nt = tiles - t0 < kC32GroupTiles ? static_cast<int>(tiles - t0) : static_cast<int>(kC32GroupTiles);
+      return walk_tiles32(rv, x, y, blocks, fns, heads, ents, s_e, j, t0, nt, c);
+    });
   }
   if (l == 0) acc[j] = s;
 }
@@ -364,18 +544,38 @@ void c32_blocks_as(Ctx &c, RV rv, bool mul, const float *x, const float *y, uint
   }
   const uint64_t tiles = (blocks + kC32TileBlocks - 1) / kC32TileBlocks;
   if (tiles > 0xFFFFFFFFull) usage_error("reduction too large");
+  // wide-table slots: every tile chain when there are few, else 8192 and a quarter of them
+  const uint64_t pairs = tiles * 16;
+  const uint32_t rcap = static_cast<uint32_t>(pairs < 8192 + pairs / 4 ? pairs : 8192 + pairs / 4);
   double *asum = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
   double *pre = static_cast<double *>(tmp_alloc(c, tiles * 16 * 8));
   auto *fns = static_cast<TileFn32 *>(tmp_alloc(c, tiles * 16 * sizeof(TileFn32)));
+  auto *keys = static_cast<uint32_t *>(tmp_alloc(c, uint64_t(rcap) * 4 + 8));
+  auto *heads = static_cast<WideHead *>(tmp_alloc(c, uint64_t(rcap) * sizeof(WideHead)));
+  auto *ents = static_cast<WideEnt *>(tmp_alloc(c, uint64_t(rcap) * kWideRes * sizeof(WideEnt)));
+  uint32_t *counter = static_cast<uint32_t *>(tmp_alloc(c, 8));
+  fill_u64(c, reinterpret_cast<uint64_t *>(counter), 1, 0);
   const uint32_t grid = static_cast<uint32_t>(tiles);
   k_c32_asum<RV><<<grid, kC32Threads, 0, c.stream>>>(rv, x, y, blocks, asum);
   k_c32_prefix<<<16, kC32Threads, 0, c.stream>>>(asum, tiles, acc, pre);
-  k_c32_summ<RV><<<grid, kC32Threads, 0, c.stream>>>(rv, x, y, blocks, pre, fns);
-  k_c32_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, fns, tiles, acc);
+  k_c32_summ<RV><<<grid, kC32Threads, 0, c.stream>>>(rv, x, y, blocks, pre, fns, counter, rcap, keys);
+  const uint64_t wide_blocks = (uint64_t(rcap) + kC32WideSlots - 1) / kC32WideSlots;
+  k_c32_wide<RV><<<uint32_t(wide_blocks < 4096 ? wide_blocks : 4096), kWave, 0, c.stream>>>(
+      rv, x, y, blocks, pre, keys, counter, rcap, heads, ents);
+  const uint64_t groups = (tiles + kC32GroupTiles - 1) / kC32GroupTiles;
+  auto *gfn = static_cast<TileFn32 *>(tmp_alloc(c, groups * 16 * sizeof(TileFn32)));
+  k_c32_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(fns, tiles, groups,
+                                                                                             gfn);
+  k_c32_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, fns, tiles, gfn, groups, heads, ents, acc);
   check_hip(hipGetLastError(), "canon f32 streamed");
   tmp_free(c, asum);
   tmp_free(c, pre);
   tmp_free(c, fns);
+  tmp_free(c, keys);
+  tmp_free(c, heads);
+  tmp_free(c, ents);
+  tmp_free(c, counter);
+  tmp_free(c, gfn);
 }
 
 }  // namespace

@@ -428,9 +428,10 @@ extern "C" int vkm_wide_table_apply(const double *chain, uint64_t n, double appr
 // tile's hypothesis at f32 precision; the tile summary joins its runs' summaries in order; the
 // walk applies a tile's summary when its hypothesis is the running value's binade and the check
 // holds, and otherwise evaluates the tile as the kernel's wave does (rounds of run summaries, or
-// the sequential fold when the approximate run starts change binade more than twice). vals are
-// f32 values held in doubles; every fold step rounds to f32. stats: [0] tiles applied, [1] tiles
-// evaluated, [2] evaluations by rounds, [3] runs folded element by element.
+// the sequential fold when the approximate run starts change binade more than twice), after the
+// tile's wide table when its summary failed at its approximate start. vals are f32 values held
+// in doubles; every fold step rounds to f32. stats: [0] tiles applied, [1] tiles evaluated, [2]
+// evaluations by rounds, [3] runs folded element by element, [4] wide tables applied.
 namespace {
 
 double eval_tile32_model(const double *vals, uint64_t b0, uint64_t b1, int j, double s, uint64_t *st) {
@@ -465,10 +466,10 @@ double eval_tile32_model(const double *vals, uint64_t b0, uint64_t b1, int j, do
     prev = h;
   }
   double cur = s;
+  uint64_t r0 = 0;
   if (changes <= 2) {
     st[2]++;
-    uint64_t r0 = 0;
-    while (r0 < nruns) {
+    for (int round = 0; round < 3 && r0 < nruns; ++round) {  // kC32Rounds
       const Binade h = binade_of<PrecF32>(cur);
       RunFn p = run_identity();
       uint64_t m = nruns;
@@ -482,24 +483,40 @@ double eval_tile32_model(const double *vals, uint64_t b0, uint64_t b1, int j, do
         }
         last = val;
       }
-      if (m >= nruns) {
-        cur = last;
-        break;
-      }
+      if (m >= nruns) return last;
       st[3]++;
       cur = fold_run(m, m > r0 ? last : cur);
       r0 = m + 1;
     }
-    return cur;
   }
-  for (uint64_t b = b0; b < b1; ++b) cur = PrecF32::rnd(cur + vals[b * 16 + j]);
+  for (uint64_t b = b0 + r0 * kRun; b < b1; ++b) cur = PrecF32::rnd(cur + vals[b * 16 + j]);
   return cur;
+}
+
+// The f32 wide table of chain j over blocks [b0, b1) from `approx`, as k_c32_wide builds it.
+void wide_table32(const double *vals, uint64_t b0, uint64_t b1, int j, double approx, WideHead *hd, WideEnt *ents) {
+  const Binade h = binade_of<PrecF32>(approx);
+  *hd = WideHead{0.0, 0, 0};
+  if (!h.ok) return;
+  const double b = wide_base<PrecF32>(approx, h), u0 = pow2(h.e - PrecF32::kMant);
+  int m = -2000;
+  for (int r = 0; r < kWideRes; ++r) {
+    WideAcc a = wide_start(b + r * u0);
+    for (uint64_t k0 = b0; k0 < b1; k0 += 16) {
+      double v[16];
+      for (uint64_t q = 0; q < 16; ++q) v[q] = k0 + q < b1 ? vals[(k0 + q) * 16 + j] : 0.0;
+      wide_push_n<16, PrecF32>(a, v);
+    }
+    ents[r] = wide_end<PrecF32>(a);
+    if (ents[r].lo <= ents[r].hi) m = std::max(m, a.emax - h.e);
+  }
+  *hd = WideHead{b, h.e, wide_nres(m)};
 }
 
 }  // namespace
 
 extern "C" int vkm_c32_fold(const double *vals, uint64_t blocks, double *acc, uint64_t *stats) {
-  uint64_t st[4] = {0, 0, 0, 0};
+  uint64_t st[5] = {0, 0, 0, 0, 0};
   const uint64_t tiles = (blocks + kTileBlocks - 1) / kTileBlocks;
   for (int j = 0; j < 16; ++j) {
     std::vector<double> pre(tiles);
@@ -523,18 +540,30 @@ extern "C" int vkm_c32_fold(const double *vals, uint64_t blocks, double *acc, ui
           f = run_join(f, run_of<kRun, PrecF32>(v, cnt, binade_ref<PrecF32>(h)));
         }
       const Binade hs = binade_of<PrecF32>(s);
-      double out = s;
+      double out = s, probe;
       if (h.ok && same_binade(h, hs) && run_apply<PrecF32>(f, hs, s, &out)) {
         s = out;
         st[0]++;
-      } else {
-        s = eval_tile32_model(vals, b0, b1, j, s, st);
-        st[1]++;
+        continue;
       }
+      // the summary does not apply: the tile's wide table when its summary failed at its
+      // approximate start (the device builds one then), else the rows
+      if (h.ok && !run_apply<PrecF32>(f, h, pre[t], &probe)) {
+        WideHead hd;
+        WideEnt ents[kWideRes];
+        wide_table32(vals, b0, b1, j, pre[t], &hd, ents);
+        if (wide_apply<PrecF32>(hd, ents, s, &out)) {
+          s = out;
+          st[4]++;
+          continue;
+        }
+      }
+      s = eval_tile32_model(vals, b0, b1, j, s, st);
+      st[1]++;
     }
     acc[j] = s;
   }
   if (stats)
-    for (int k = 0; k < 4; ++k) stats[k] = st[k];
+    for (int k = 0; k < 5; ++k) stats[k] = st[k];
   return 0;
 }
