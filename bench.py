@@ -371,6 +371,21 @@ def batched_suite(D, count: int):
             "filter_sum": filter_sum}
 
 
+def program_plan_suite(D, count: int):
+    """The many-arrays program as one batch plan alone (for profiling its graph replay): compile
+    with autotuning, one warm-up replay, then three timed replays; results of every 16th item
+    checked against the batched calls."""
+    from verkehrt_amd import batch as B
+    wl = B.make_workload(D, count=count)
+    plan = B.program_plan(D, wl, tune=True)
+    B.program_plan_run(D, plan)
+    t, _ = B.timed(B.program_plan_run, D, plan, repeat=3)
+    rb = B.batched(D, wl)
+    same = B.same(B.program_plan_results(plan, every=16), B.Results(rb.sums, rb.perms[::16], rb.groups[::16]))
+    plan.close()
+    return {"arrays": count, "rows": wl.rows, "plan_ms": round(t * 1e3, 1), "identical": same}
+
+
 def key_image(b: np.ndarray, kind: int) -> np.ndarray:
     """The u64 sort images of 8-byte keys as bits: 0 u64, 1 i64 (sign flipped), 2 f64 (negative
     values flip every bit, others the sign bit)."""
@@ -792,7 +807,9 @@ def main():
                     help="comma-separated primitive labels to run (for profiling); 'copy_f64' adds a "
                          "plain copy (add_f64 x + 0.0) used to calibrate HBM counters; 'zero_mean' "
                          "alone runs only the zero-mean canonical and f64 scan leg, 'sort' alone only the "
-                         "sort and grouping suite, 'batched' alone only the many-arrays comparison")
+                         "sort and grouping suite, 'batched' alone only the many-arrays comparison, "
+                         "'program_plan' alone only the many-arrays program as one batch plan, 'f32_int' "
+                         "alone only the f32 canonical and integer dot leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this backend has no multi-GPU path (asked for {args.gpus})",
@@ -811,6 +828,9 @@ def main():
         return
     if args.only == "batched":
         print(json.dumps({"batched": batched_suite(D, args.batch_arrays)}))
+        return
+    if args.only == "program_plan":
+        print(json.dumps({"program_plan": program_plan_suite(D, args.batch_arrays)}))
         return
     x1 = D.call("rand_f64", [S("i64", n), S("i64", 1)])
     if args.only == "zero_mean":
