@@ -498,7 +498,10 @@ __global__ __launch_bounds__(kGBlock) void k_ids1(KeyCol a, uint64_t n, const Sl
 // per pass covers the tiles of every item. Tables start small (at most 2^kBSmallLg words): an
 // insert whose probe run there passes kMaxRun marks its item's overflow word, and only those
 // items insert again into tables of 2n words before the other passes run.
-constexpr uint64_t kGTile = 4096;
+// A block takes a tile of kGTile rows in sub-tiles of kGSub (16 rows per thread): the keys a
+// tile repeats reach the item's table once per tile, through the block's LDS table.
+constexpr uint64_t kGSub = 4096;
+constexpr uint64_t kGTile = 4 * kGSub;
 
 struct ItemRows {
   uint64_t t0, t1;
@@ -548,7 +551,7 @@ __device__ uint64_t word_slot(const KeyCol &a, const unsigned long long *table, 
 // the tile's keys reaches the item's table once.
 constexpr int kBLBits = 11;
 constexpr int kBLSlots = 1 << kBLBits;
-constexpr int kBRows = static_cast<int>(kGTile / kBlock);
+constexpr int kBRows = static_cast<int>(kGSub / kBlock);
 
 // aux2: the item's longest probe run (kNoLimit in a table of 2n words); overflow[item] is set
 // when a run passes it. With `only`, items whose word there is zero are skipped (a plan's
@@ -576,25 +579,27 @@ __global__ __launch_bounds__(kBlock) void k_insert_batch(int32_t type, const Bat
   }
   if (t == 0) s_cnt = 0;
   __syncthreads();
-  uint64_t k[kBRows];
-  a.with_bits([&](auto bits) {
+  for (uint64_t s0 = R.t0; s0 < R.t1; s0 += kGSub) {
+    uint64_t k[kBRows];
+    a.with_bits([&](auto bits) {
+#pragma unroll
+      for (int j = 0; j < kBRows; ++j) {
+        const uint64_t r = s0 + uint64_t(j) * kBlock + t;
+        k[j] = r < R.t1 ? bits(r) : 0;
+      }
+    });
 #pragma unroll
     for (int j = 0; j < kBRows; ++j) {
-      const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-      k[j] = r < R.t1 ? bits(r) : 0;
-    }
-  });
-#pragma unroll
-  for (int j = 0; j < kBRows; ++j) {
-    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-    if (r >= R.t1) continue;
-    bool claimed;
-    const int h = lds_find<kBLBits>(s_key, &s_cnt, k[j], &claimed);
-    if (h >= 0) {
-      if (__hip_atomic_load(&s_row[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > r)
-        atomicMin(&s_row[h], static_cast<uint32_t>(r));
-    } else {
-      if (!word_insert(a, table, mask, 64 - lg, k[j], r, max_run)) over = true;
+      const uint64_t r = s0 + uint64_t(j) * kBlock + t;
+      if (r >= R.t1) continue;
+      bool claimed;
+      const int h = lds_find<kBLBits>(s_key, &s_cnt, k[j], &claimed);
+      if (h >= 0) {
+        if (__hip_atomic_load(&s_row[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > r)
+          atomicMin(&s_row[h], static_cast<uint32_t>(r));
+      } else {
+        if (!word_insert(a, table, mask, 64 - lg, k[j], r, max_run)) over = true;
+      }
     }
   }
   __syncthreads();
@@ -624,43 +629,47 @@ __global__ __launch_bounds__(kBlock) void k_probe_batch(int32_t type, const Batc
   for (int i = t; i < kBLSlots; i += kBlock) s_key[i] = kLEmpty;
   if (t == 0) s_cnt = 0;
   __syncthreads();
-  uint64_t k[kBRows];
-  int h[kBRows];
-  a.with_bits([&](auto bits) {
+  // per sub-tile: phase 1, barrier, phase 2 (an entry claimed in an earlier sub-tile was
+  // resolved before that sub-tile's barrier; phase 1 writes only entries it claims)
+  for (uint64_t s0 = R.t0; s0 < R.t1; s0 += kGSub) {
+    uint64_t k[kBRows];
+    int h[kBRows];
+    a.with_bits([&](auto bits) {
+#pragma unroll
+      for (int j = 0; j < kBRows; ++j) {
+        const uint64_t r = s0 + uint64_t(j) * kBlock + t;
+        k[j] = r < R.t1 ? bits(r) : 0;
+      }
+    });
 #pragma unroll
     for (int j = 0; j < kBRows; ++j) {
-      const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-      k[j] = r < R.t1 ? bits(r) : 0;
+      const uint64_t r = s0 + uint64_t(j) * kBlock + t;
+      h[j] = -1;
+      if (r >= R.t1) continue;
+      bool claimed;
+      h[j] = lds_find<kBLBits>(s_key, &s_cnt, k[j], &claimed);
+      if (claimed) {
+        const uint64_t g = word_slot(a, table, mask, 64 - lg, k[j]);
+        s_slot[h[j]] = static_cast<uint32_t>(g);
+        s_rep[h[j]] = static_cast<uint32_t>(table[g] - 1);
+      }
     }
-  });
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kBRows; ++j) {
-    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-    h[j] = -1;
-    if (r >= R.t1) continue;
-    bool claimed;
-    h[j] = lds_find<kBLBits>(s_key, &s_cnt, k[j], &claimed);
-    if (claimed) {
-      const uint64_t g = word_slot(a, table, mask, 64 - lg, k[j]);
-      s_slot[h[j]] = static_cast<uint32_t>(g);
-      s_rep[h[j]] = static_cast<uint32_t>(table[g] - 1);
+    for (int j = 0; j < kBRows; ++j) {
+      const uint64_t r = s0 + uint64_t(j) * kBlock + t;
+      if (r >= R.t1) continue;
+      uint64_t g, rep;
+      if (h[j] >= 0) {
+        g = s_slot[h[j]];
+        rep = s_rep[h[j]];
+      } else {
+        g = word_slot(a, table, mask, 64 - lg, k[j]);
+        rep = table[g] - 1;
+      }
+      slot[I.aux + r] = static_cast<uint32_t>(g);
+      first[I.aux + r] = rep == r;
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kBRows; ++j) {
-    const uint64_t r = R.t0 + uint64_t(j) * kBlock + t;
-    if (r >= R.t1) continue;
-    uint64_t g, rep;
-    if (h[j] >= 0) {
-      g = s_slot[h[j]];
-      rep = s_rep[h[j]];
-    } else {
-      g = word_slot(a, table, mask, 64 - lg, k[j]);
-      rep = table[g] - 1;
-    }
-    slot[I.aux + r] = static_cast<uint32_t>(g);
-    first[I.aux + r] = rep == r;
   }
 }
 
@@ -677,13 +686,34 @@ __global__ __launch_bounds__(kBlock) void k_number_batch(const BatchItem *items,
     id_of_slot[slot[I.aux + firsts[I.aux + j]]] = static_cast<uint32_t>(j);
 }
 
+// A thread's kBRows rows: every slot load issued first, then every id load, then the stores
+// (one load round trip per step instead of one per row).
 __global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, const uint32_t *tile_item,
                                                       const uint32_t *slot) {
   const BatchItem &I = items[tile_item[blockIdx.x]];
   const ItemRows R(I, blockIdx.x);
   const uint32_t *id_of_slot = static_cast<const uint32_t *>(I.b);
+  const uint32_t *sl = slot + I.aux;
   int64_t *out = static_cast<int64_t *>(I.out);
-  for (uint64_t r = R.t0 + threadIdx.x; r < R.t1; r += kBlock) out[r] = id_of_slot[slot[I.aux + r]];
+  const int t = threadIdx.x;
+  for (uint64_t s0 = R.t0; s0 < R.t1; s0 += kGSub) {
+    uint32_t s[kBRows], id[kBRows];
+#pragma unroll
+    for (int u = 0; u < kBRows; ++u) {
+      const uint64_t r = s0 + uint64_t(u) * kBlock + t;
+      s[u] = r < R.t1 ? sl[r] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBRows; ++u) {
+      const uint64_t r = s0 + uint64_t(u) * kBlock + t;
+      id[u] = r < R.t1 ? id_of_slot[s[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBRows; ++u) {
+      const uint64_t r = s0 + uint64_t(u) * kBlock + t;
+      if (r < R.t1) out[r] = id[u];
+    }
+  }
 }
 
 }  // namespace
