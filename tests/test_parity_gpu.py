@@ -692,6 +692,28 @@ def test_many_tiles(D):
     assert D.call("argmax_f64", [dx]).value == int(np.argmax(x))
 
 
+def test_mask_selections_long(D):
+    # compress and where over at least 512 tiles take a count pass over the mask bytes and a scan
+    # of the tile counts instead of the look-back: aligned and unaligned masks, a partial last
+    # tile, nonzero bytes other than 1, densities from none to all
+    rng = np.random.default_rng(15)
+    n = 9_000_013
+    x = rng.random(n)
+    for density in (0.0, 0.01, 0.5, 1.0):
+        m = (rng.random(n) < density).astype(np.uint8)
+        if density == 0.5:
+            m[m != 0] = rng.integers(1, 256, int(m.sum())).astype(np.uint8)
+        for off in (0, 3):
+            dx, dm = view(D, x, off), view(D, m, (off * 5) % 7, "bool")
+            got = D.call("compress_f64", [dx, dm]).to_numpy()
+            assert np.array_equal(got.view(np.uint64), x[m != 0].view(np.uint64)), (density, off)
+            w = D.call("where", [dm]).to_numpy()
+            assert np.array_equal(w, np.flatnonzero(m)), (density, off)
+    xi = rng.integers(-5, 5, n).astype(np.int32)
+    m = (rng.random(n) < 0.3).astype(np.uint8)
+    assert np.array_equal(D.call("compress_i32", [D.array(xi), D.array(m, "bool")]).to_numpy(), xi[m != 0])
+
+
 def test_canonical_streamed_long(D):
     # above 64 tiles of 16K rows the streamed form runs; 80M rows make 4883 tiles, so the
     # walk over groups of 64 tiles goes past one wave's 64 groups

@@ -67,7 +67,10 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
 // the other waits on its look-back.
 // ITEMS rows per thread (a tile is 1024 x ITEMS rows). A launch whose tiles were planned on
 // the device passes dev_ntiles (blocks past it exit) and dev_epoch (the epoch to publish with).
-template <typename T, int KIND, bool BATCH, int ITEMS = kItems>
+// PRE (compress and where over long inputs): a count pass over the mask and a scan have left
+// the tile's exclusive prefix of kept rows in status[tile]; tiles go in block order with no
+// ticket and no look-back.
+template <typename T, int KIND, bool BATCH, int ITEMS = kItems, bool PRE = false>
 __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
                                                    uint32_t keepset, T c, uint64_t base, void *out_v,
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
   constexpr int kTile = kSelBlock * ITEMS;
   constexpr int kItems = ITEMS;
-  const uint32_t tile = take_ticket(ticket, ntiles);
+  const uint32_t tile = PRE ? blockIdx.x : take_ticket(ticket, ntiles);
   if (dev_ntiles && tile >= *dev_ntiles) return;
   if (dev_epoch) epoch = *dev_epoch;
   uint32_t first = 0, last = ntiles - 1;
@@ -179,7 +182,9 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
     const uint32_t total = __shfl(inc, 63);
     // publish and look back; the status word carries its payload, so relaxed stores suffice
     uint64_t prefix = 0;
-    if (tile == first) {
+    if constexpr (PRE) {
+      prefix = status[tile];
+    } else if (tile == first) {
       if (l == 0) st_rlx(&status[tile], pack(epoch, 2, total));
     } else {
       if (l == 0) st_rlx(&status[tile], pack(epoch, 1, total));
@@ -431,6 +436,70 @@ struct SelScratch {
   uint32_t *ticket;
 };
 
+// The count pass of mask selections over long inputs: cnt[tile] = the nonzero mask bytes of the
+// tile's 16384 rows, 16 bytes per thread (byte by byte in a partial or unaligned tile). The counts
+// go into the look-back status words, whose epoch field they leave 0, which no launch uses.
+__global__ __launch_bounds__(kSelBlock) void k_mask_count(const uint8_t *__restrict__ mask, uint64_t n,
+                                                          uint64_t *cnt) {
+  const uint64_t t0 = uint64_t(blockIdx.x) * kTile;
+  const uint64_t rem = n - t0;
+  const uint32_t lim = rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);
+  const uint8_t *mt = mask + t0;
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  uint32_t c = 0;
+  if (lim == uint32_t(kTile) && (reinterpret_cast<uintptr_t>(mt) & 15) == 0) {
+    struct alignas(16) Q {
+      uint32_t a[4];
+    };
+    const Q q = reinterpret_cast<const Q *>(mt)[t];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      // the high bit of every nonzero byte
+      const uint32_t v = q.a[k];
+      c += __popc((((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u);
+    }
+  } else {
+    for (uint32_t r = uint32_t(t); r < lim; r += kSelBlock) c += mt[r] != 0;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
+  __shared__ uint32_t s_w[kWaves];
+  if (l == 0) s_w[w] = c;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t a = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) a += s_w[k];
+    cnt[blockIdx.x] = a;
+  }
+}
+
+// In place: cnt[tile] becomes the exclusive prefix of the counts; *d_count the total. One block,
+// each thread a contiguous range of tiles.
+__global__ __launch_bounds__(kSelBlock) void k_tile_offsets(uint64_t *cnt, uint32_t tiles, uint64_t *d_count) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (tiles + kSelBlock - 1) / kSelBlock;
+  const uint32_t a = t * per < tiles ? t * per : tiles, b = a + per < tiles ? a + per : tiles;
+  uint64_t s = 0;
+  for (uint32_t q = a; q < b; ++q) s += cnt[q];
+  __shared__ uint64_t sc[kSelBlock];
+  sc[t] = s;
+  __syncthreads();
+  for (uint32_t d = 1; d < kSelBlock; d <<= 1) {
+    const uint64_t o = t >= d ? sc[t - d] : 0;
+    __syncthreads();
+    sc[t] += o;
+    __syncthreads();
+  }
+  uint64_t run = sc[t] - s;
+  for (uint32_t q = a; q < b; ++q) {
+    const uint64_t v = cnt[q];
+    cnt[q] = run;
+    run += v;
+  }
+  if (t == kSelBlock - 1) *d_count = sc[t];
+}
+
 template <typename T, int KIND>
 void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uint64_t cbits,
             uint64_t base, void *out, uint64_t *d_count, const SelScratch *given) {
@@ -446,6 +515,19 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
   }
   T cv;
   __builtin_memcpy(&cv, &cbits, sizeof(T));
+  if constexpr (KIND != 1) {
+    // long compress and where: the kept rows per tile are counted from the mask bytes alone (a
+    // read of n bytes), scanned, and the selection pass takes each tile's offset from there
+    if (tiles >= 2 * uint64_t(c.cus)) {
+      k_mask_count<<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(mask, n, sc.status);
+      k_tile_offsets<<<1, kSelBlock, 0, c.stream>>>(sc.status, static_cast<uint32_t>(tiles), d_count);
+      k_select<T, KIND, false, kItems, true><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
+          static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch, sc.ticket,
+          static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);
+      check_hip(hipGetLastError(), "select launch");
+      return;
+    }
+  }
   if constexpr (KIND == 1) {
     // long filters: one persistent block per compute unit (compress and where, which also hold
     // mask bytes, keep one tile per block: their persistent form spilled registers)
