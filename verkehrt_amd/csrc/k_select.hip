@@ -437,10 +437,12 @@ struct SelScratch {
 };
 
 // The count pass of mask selections over long inputs: cnt[tile] = the nonzero mask bytes of the
-// tile's 16384 rows, 16 bytes per thread (byte by byte in a partial or unaligned tile). The counts
-// go into the look-back status words, whose epoch field they leave 0, which no launch uses.
-__global__ __launch_bounds__(kSelBlock) void k_mask_count(const uint8_t *__restrict__ mask, uint64_t n,
-                                                          uint64_t *cnt) {
+// tile's 16384 rows, by a block of 256 threads, 64 bytes per thread as four 16-byte loads (byte by
+// byte in a partial or unaligned tile). The counts go into the look-back status words, whose epoch
+// field they leave 0, which no launch uses.
+constexpr int kCountThreads = 256;
+__global__ __launch_bounds__(kCountThreads) void k_mask_count(const uint8_t *__restrict__ mask, uint64_t n,
+                                                              uint64_t *cnt) {
   const uint64_t t0 = uint64_t(blockIdx.x) * kTile;
   const uint64_t rem = n - t0;
   const uint32_t lim = rem >= uint64_t(kTile) ? uint32_t(kTile) : uint32_t(rem);
@@ -451,53 +453,75 @@ __global__ __launch_bounds__(kSelBlock) void k_mask_count(const uint8_t *__restr
     struct alignas(16) Q {
       uint32_t a[4];
     };
-    const Q q = reinterpret_cast<const Q *>(mt)[t];
+    constexpr int kLoads = kTile / 16 / kCountThreads;
+    Q q[kLoads];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      // the high bit of every nonzero byte
-      const uint32_t v = q.a[k];
-      c += __popc((((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u);
-    }
+    for (int u = 0; u < kLoads; ++u) q[u] = reinterpret_cast<const Q *>(mt)[u * kCountThreads + t];
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        // the high bit of every nonzero byte
+        const uint32_t v = q[u].a[k];
+        c += __popc((((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u);
+      }
   } else {
-    for (uint32_t r = uint32_t(t); r < lim; r += kSelBlock) c += mt[r] != 0;
+    for (uint32_t r = uint32_t(t); r < lim; r += kCountThreads) c += mt[r] != 0;
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
-  __shared__ uint32_t s_w[kWaves];
+  __shared__ uint32_t s_w[kCountThreads / kWave];
   if (l == 0) s_w[w] = c;
   __syncthreads();
   if (t == 0) {
     uint32_t a = 0;
 #pragma unroll
-    for (int k = 0; k < kWaves; ++k) a += s_w[k];
+    for (int k = 0; k < kCountThreads / kWave; ++k) a += s_w[k];
     cnt[blockIdx.x] = a;
   }
 }
 
-// In place: cnt[tile] becomes the exclusive prefix of the counts; *d_count the total. One block,
-// each thread a contiguous range of tiles.
+// In place: cnt[tile] becomes the exclusive prefix of the counts; *d_count the total. One block:
+// thread t holds tiles t * per .. t * per + per - 1 (per = ceil(tiles / 1024)), all loaded
+// before the sums; the thread sums are scanned by wave shuffles, then across the 16 waves.
 __global__ __launch_bounds__(kSelBlock) void k_tile_offsets(uint64_t *cnt, uint32_t tiles, uint64_t *d_count) {
-  const uint32_t t = threadIdx.x;
+  constexpr int kPer = 16;  // tiles per thread loaded at once
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint32_t per = (tiles + kSelBlock - 1) / kSelBlock;
-  const uint32_t a = t * per < tiles ? t * per : tiles, b = a + per < tiles ? a + per : tiles;
+  const uint32_t a = uint32_t(t) * per < tiles ? uint32_t(t) * per : tiles;
+  const uint32_t b = a + per < tiles ? a + per : tiles;
   uint64_t s = 0;
-  for (uint32_t q = a; q < b; ++q) s += cnt[q];
-  __shared__ uint64_t sc[kSelBlock];
-  sc[t] = s;
+  for (uint32_t q0 = a; q0 < b; q0 += kPer) {
+    uint64_t v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) v[u] = q0 + u < b ? cnt[q0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) s += v[u];
+  }
+  uint64_t inc = s;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t o = __shfl_up(inc, d);
+    if (l >= d) inc += o;
+  }
+  __shared__ uint64_t s_w[kWaves];
+  if (l == kWave - 1) s_w[w] = inc;
   __syncthreads();
-  for (uint32_t d = 1; d < kSelBlock; d <<= 1) {
-    const uint64_t o = t >= d ? sc[t - d] : 0;
-    __syncthreads();
-    sc[t] += o;
-    __syncthreads();
+  uint64_t before = 0;
+  for (int k = 0; k < w; ++k) before += s_w[k];
+  uint64_t run = before + inc - s;
+  if (t == kSelBlock - 1) *d_count = before + inc;
+  for (uint32_t q0 = a; q0 < b; q0 += kPer) {
+    uint64_t v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) v[u] = q0 + u < b ? cnt[q0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (q0 + u < b) {
+        cnt[q0 + u] = run;
+        run += v[u];
+      }
   }
-  uint64_t run = sc[t] - s;
-  for (uint32_t q = a; q < b; ++q) {
-    const uint64_t v = cnt[q];
-    cnt[q] = run;
-    run += v;
-  }
-  if (t == kSelBlock - 1) *d_count = sc[t];
 }
 
 template <typename T, int KIND>
@@ -519,7 +543,7 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
     // long compress and where: the kept rows per tile are counted from the mask bytes alone (a
     // read of n bytes), scanned, and the selection pass takes each tile's offset from there
     if (tiles >= 2 * uint64_t(c.cus)) {
-      k_mask_count<<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(mask, n, sc.status);
+      k_mask_count<<<static_cast<uint32_t>(tiles), kCountThreads, 0, c.stream>>>(mask, n, sc.status);
       k_tile_offsets<<<1, kSelBlock, 0, c.stream>>>(sc.status, static_cast<uint32_t>(tiles), d_count);
       k_select<T, KIND, false, kItems, true><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
           static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch, sc.ticket,
