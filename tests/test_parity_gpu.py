@@ -714,6 +714,26 @@ def test_mask_selections_long(D):
     assert np.array_equal(D.call("compress_i32", [D.array(xi), D.array(m, "bool")]).to_numpy(), xi[m != 0])
 
 
+def test_filters_staged_long(D):
+    # filters over 2^24 rows or more that keep few rows (a sample of every 1024th row decides)
+    # write each tile's survivors to a staging slot, scan the tile counts and move them: none
+    # kept, a few in the last partial tile only, 1% and 5% kept, and 30% (the look-back form)
+    rng = np.random.default_rng(16)
+    n = (1 << 24) + 4099
+    x = rng.random(n)
+    dx = D.array(x)
+    for c in (2.0, 0.99, 0.95, 0.7):
+        got = D.call("filter_gt_f64", [dx, S("f64", c)]).to_numpy()
+        assert np.array_equal(got.view(np.uint64), x[x > c].view(np.uint64)), c
+    y = np.zeros(n)
+    y[-3:] = [5.0, -1.0, 7.0]
+    got = D.call("filter_gt_f64", [view(D, y, 1), S("f64", 1.0)]).to_numpy()
+    assert list(got) == [5.0, 7.0]
+    xi = rng.integers(0, 1000, n).astype(np.int32)
+    got = D.call("filter_lt_i32", [D.array(xi), S("i32", 7)]).to_numpy()
+    assert np.array_equal(got, xi[xi < 7])
+
+
 def test_canonical_streamed_long(D):
     # above 64 tiles of 16K rows the streamed form runs; 80M rows make 4883 tiles, so the
     # walk over groups of 64 tiles goes past one wave's 64 groups

@@ -67,10 +67,13 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t epoch, uint32_t tile, u
 // the other waits on its look-back.
 // ITEMS rows per thread (a tile is 1024 x ITEMS rows). A launch whose tiles were planned on
 // the device passes dev_ntiles (blocks past it exit) and dev_epoch (the epoch to publish with).
-// PRE (compress and where over long inputs): a count pass over the mask and a scan have left
-// the tile's exclusive prefix of kept rows in status[tile]; tiles go in block order with no
+// PM (how a tile learns where its survivors go), for single calls: 0 the look-back; 1 (compress
+// and where over long inputs) a count pass over the mask and a scan have left the tile's
+// exclusive prefix in status[tile]; 2 (filters over long inputs with few survivors) the tile
+// writes its survivors to its own 16384-row slot of out (a staging buffer) and its count to
+// status[tile], for a scan and a move pass. Modes 1 and 2 take tiles in block order, with no
 // ticket and no look-back.
-template <typename T, int KIND, bool BATCH, int ITEMS = kItems, bool PRE = false>
+template <typename T, int KIND, bool BATCH, int ITEMS = kItems, int PM = 0>
 __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_select(const T *__restrict__ x,
                                                    const uint8_t *__restrict__ mask, uint64_t n,
                                                    uint32_t keepset, T c, uint64_t base, void *out_v,
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
   using Out = typename std::conditional<KIND == 2, int64_t, T>::type;
   constexpr int kTile = kSelBlock * ITEMS;
   constexpr int kItems = ITEMS;
-  const uint32_t tile = PRE ? blockIdx.x : take_ticket(ticket, ntiles);
+  const uint32_t tile = PM ? blockIdx.x : take_ticket(ticket, ntiles);
   if (dev_ntiles && tile >= *dev_ntiles) return;
   if (dev_epoch) epoch = *dev_epoch;
   uint32_t first = 0, last = ntiles - 1;
@@ -182,8 +185,11 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
     const uint32_t total = __shfl(inc, 63);
     // publish and look back; the status word carries its payload, so relaxed stores suffice
     uint64_t prefix = 0;
-    if constexpr (PRE) {
+    if constexpr (PM == 1) {
       prefix = status[tile];
+    } else if constexpr (PM == 2) {
+      prefix = t0;
+      if (l == 0) status[tile] = total;  // a raw count: epoch field 0, which no launch uses
     } else if (tile == first) {
       if (l == 0) st_rlx(&status[tile], pack(epoch, 2, total));
     } else {
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(kSelBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
     }
     if (l == 0) {
       s_prefix = prefix;
-      if (tile == last) *d_count = prefix + total;
+      if (PM != 2 && tile == last) *d_count = prefix + total;
     }
   }
   __syncthreads();
@@ -481,47 +487,106 @@ __global__ __launch_bounds__(kCountThreads) void k_mask_count(const uint8_t *__r
   }
 }
 
-// In place: cnt[tile] becomes the exclusive prefix of the counts; *d_count the total. One block:
-// thread t holds tiles t * per .. t * per + per - 1 (per = ceil(tiles / 1024)), all loaded
-// before the sums; the thread sums are scanned by wave shuffles, then across the 16 waves.
-__global__ __launch_bounds__(kSelBlock) void k_tile_offsets(uint64_t *cnt, uint32_t tiles, uint64_t *d_count) {
-  constexpr int kPer = 16;  // tiles per thread loaded at once
-  const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  const uint32_t per = (tiles + kSelBlock - 1) / kSelBlock;
-  const uint32_t a = uint32_t(t) * per < tiles ? uint32_t(t) * per : tiles;
-  const uint32_t b = a + per < tiles ? a + per : tiles;
-  uint64_t s = 0;
-  for (uint32_t q0 = a; q0 < b; q0 += kPer) {
-    uint64_t v[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) v[u] = q0 + u < b ? cnt[q0 + u] : 0;
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) s += v[u];
-  }
-  uint64_t inc = s;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint64_t o = __shfl_up(inc, d);
-    if (l >= d) inc += o;
-  }
+// In place: cnt[tile] becomes the exclusive prefix of the counts; *d_count the total. One block
+// over chunks of 4096 tiles: a chunk is loaded into LDS with coalesced loads, thread t scans its
+// four entries, the thread sums are scanned by wave shuffles and across the 16 waves, and the
+// chunk goes back with coalesced stores. With `gate`, a launch whose gate word is 0 does nothing.
+constexpr uint32_t kOffChunk = 4 * kSelBlock;
+__global__ __launch_bounds__(kSelBlock) void k_tile_offsets(uint64_t *cnt, uint32_t tiles, uint64_t *d_count,
+                                                            const uint32_t *gate = nullptr) {
+  if (gate && *gate == 0) return;
+  __shared__ uint64_t s_v[kOffChunk];
   __shared__ uint64_t s_w[kWaves];
-  if (l == kWave - 1) s_w[w] = inc;
-  __syncthreads();
-  uint64_t before = 0;
-  for (int k = 0; k < w; ++k) before += s_w[k];
-  uint64_t run = before + inc - s;
-  if (t == kSelBlock - 1) *d_count = before + inc;
-  for (uint32_t q0 = a; q0 < b; q0 += kPer) {
-    uint64_t v[kPer];
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  uint64_t carry = 0;
+  for (uint32_t c0 = 0; c0 < tiles; c0 += kOffChunk) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) v[u] = q0 + u < b ? cnt[q0 + u] : 0;
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = c0 + uint32_t(u) * kSelBlock + t;
+      s_v[u * kSelBlock + t] = i < tiles ? cnt[i] : 0;
+    }
+    __syncthreads();
+    uint64_t a[4], sum = 0;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u)
-      if (q0 + u < b) {
-        cnt[q0 + u] = run;
-        run += v[u];
-      }
+    for (int u = 0; u < 4; ++u) {
+      a[u] = s_v[4 * t + u];
+      sum += a[u];
+    }
+    uint64_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint64_t o = __shfl_up(inc, d);
+      if (l >= d) inc += o;
+    }
+    if (l == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      before += k < w ? s_w[k] : 0;
+      all += s_w[k];
+    }
+    uint64_t e = carry + before + inc - sum;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s_v[4 * t + u] = e;
+      e += a[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = c0 + uint32_t(u) * kSelBlock + t;
+      if (i < tiles) cnt[i] = s_v[u * kSelBlock + t];
+    }
+    carry += all;
+    __syncthreads();
   }
+  if (t == 0) *d_count = carry;
+}
+
+// The kept rows among every `step`-th row of a filter's input (the choice of the staged form).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_filter_sample(const T *__restrict__ x, uint64_t n, uint32_t keepset, T c,
+                                                          uint64_t step, uint32_t *kept) {
+  uint32_t k = 0;
+  const uint64_t samples = (n + step - 1) / step;
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < samples; i += uint64_t(gridDim.x) * kBlock)
+    k += (keepset & outcome(x[i * step], c)) != 0;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) k += __shfl_xor(k, m);
+  if (lane_id() == 0 && k) atomicAdd(kept, k);
+}
+
+// The staged survivors of every tile (its slot of stage; status[tile] is its offset after the
+// scan turned the counts into offsets) moved to their place in out, blocks taking tiles
+// blockIdx.x, + gridDim.x, ...
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stage_move(const T *__restrict__ stage, const uint64_t *off, uint32_t tiles,
+                                                       const uint64_t *d_count, T *__restrict__ out,
+                                                       const uint32_t *gate) {
+  if (*gate == 0) return;
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const uint64_t o = off[tile];
+    const uint64_t e = tile + 1 < tiles ? off[tile + 1] : *d_count;
+    const T *src = stage + uint64_t(tile) * kTile;
+    for (uint64_t r = threadIdx.x; r < e - o; r += kBlock) out[o + r] = src[r];
+  }
+}
+
+// Filters over at least this many rows sample their selectivity first and, below 1 in
+// kStageInv kept rows, take the staged form (no look-back; the survivors cross HBM twice more).
+// The choice is made on the device: k_stage_choose writes the tile counts the two forms run
+// with (the other gets 0 and its launches return at once), so no read-back sits between them.
+constexpr uint64_t kStageMinRows = uint64_t(1) << 24;
+constexpr uint64_t kSampleStep = 4096;
+constexpr uint32_t kStageInv = 16;
+
+// gate[0] = tiles for the staged form and gate[1] = 0, or the other way round.
+__global__ void k_stage_choose(const uint32_t *kept, uint64_t samples, uint32_t tiles, uint32_t *gate) {
+  if (threadIdx.x != 0) return;
+  const bool staged = uint64_t(*kept) * kStageInv < samples;
+  gate[0] = staged ? tiles : 0;
+  gate[1] = staged ? 0 : tiles;
 }
 
 template <typename T, int KIND>
@@ -545,7 +610,7 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
     if (tiles >= 2 * uint64_t(c.cus)) {
       k_mask_count<<<static_cast<uint32_t>(tiles), kCountThreads, 0, c.stream>>>(mask, n, sc.status);
       k_tile_offsets<<<1, kSelBlock, 0, c.stream>>>(sc.status, static_cast<uint32_t>(tiles), d_count);
-      k_select<T, KIND, false, kItems, true><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
+      k_select<T, KIND, false, kItems, 1><<<static_cast<uint32_t>(tiles), kSelBlock, 0, c.stream>>>(
           static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch, sc.ticket,
           static_cast<uint32_t>(tiles), d_count, nullptr, nullptr, nullptr, nullptr, nullptr);
       check_hip(hipGetLastError(), "select launch");
@@ -553,6 +618,31 @@ void launch(Ctx &c, const void *x, const uint8_t *mask, uint64_t n, int cmp, uin
     }
   }
   if constexpr (KIND == 1) {
+    // long filters: the staged form when a sample finds few kept rows, else the persistent
+    // look-back form; both are launched, gated on the device
+    if (!given && n >= kStageMinRows && tiles >= 2 * uint64_t(c.cus)) {
+      uint32_t *kept = static_cast<uint32_t *>(tmp_alloc(c, 16));
+      uint32_t *gate = kept + 2;
+      fill_u64(c, reinterpret_cast<uint64_t *>(kept), 2, 0);
+      const uint32_t nt = static_cast<uint32_t>(tiles);
+      k_filter_sample<T><<<64, kBlock, 0, c.stream>>>(static_cast<const T *>(x), n, keepset_of(cmp), cv, kSampleStep,
+                                                    kept);
+      k_stage_choose<<<1, 64, 0, c.stream>>>(kept, (n + kSampleStep - 1) / kSampleStep, nt, gate);
+      T *stage = static_cast<T *>(tmp_alloc(c, tiles * kTile * sizeof(T)));
+      k_select<T, 1, false, kItems, 2><<<nt, kSelBlock, 0, c.stream>>>(
+          static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, stage, sc.status, sc.epoch, sc.ticket, nt,
+          d_count, nullptr, nullptr, nullptr, gate, nullptr);
+      k_tile_offsets<<<1, kSelBlock, 0, c.stream>>>(sc.status, nt, d_count, gate);
+      const uint32_t mg = nt < 1024 ? nt : 1024;
+      k_stage_move<T><<<mg, kBlock, 0, c.stream>>>(stage, sc.status, nt, d_count, static_cast<T *>(out), gate);
+      k_select_p<T, KIND><<<static_cast<uint32_t>(c.cus), kSelBlock, 0, c.stream>>>(
+          static_cast<const T *>(x), mask, n, keepset_of(cmp), cv, base, out, sc.status, sc.epoch, sc.ticket, nt,
+          d_count, nullptr, nullptr, nullptr, gate + 1, nullptr);
+      check_hip(hipGetLastError(), "select launch");
+      tmp_free(c, stage);
+      tmp_free(c, kept);
+      return;
+    }
     // long filters: one persistent block per compute unit (compress and where, which also hold
     // mask bytes, keep one tile per block: their persistent form spilled registers)
     if (tiles >= 2 * uint64_t(c.cus)) {

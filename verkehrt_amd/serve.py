@@ -46,7 +46,7 @@ PROGRAMS = ("sum_filter", "dot", "count", "scan", "argsort", "group_sum")
 
 # Device bytes per row a program needs beyond its input columns (outputs and scratch), an
 # upper estimate used for admission.
-EXTRA_BYTES_PER_ROW = {"sum_filter": 16, "dot": 0, "count": 0, "scan": 16, "argsort": 64,
+EXTRA_BYTES_PER_ROW = {"sum_filter": 24, "dot": 0, "count": 0, "scan": 16, "argsort": 64,
                        "group_sum": 112}
 
 
