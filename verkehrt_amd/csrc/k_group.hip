@@ -686,16 +686,20 @@ __global__ __launch_bounds__(kBlock) void k_number_batch(const BatchItem *items,
     id_of_slot[slot[I.aux + firsts[I.aux + j]]] = static_cast<uint32_t>(j);
 }
 
-// A thread's kBRows rows: every slot load issued first, then every id load, then the stores
-// (one load round trip per step instead of one per row).
-__global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, const uint32_t *tile_item,
-                                                      const uint32_t *slot) {
-  const BatchItem &I = items[tile_item[blockIdx.x]];
-  const ItemRows R(I, blockIdx.x);
-  const uint32_t *id_of_slot = static_cast<const uint32_t *>(I.b);
+// A thread's kBRows rows: every slot load issued first, then every id lookup, then the stores
+// (one load round trip per step instead of one per row). Lookups go through a per-block LDS
+// cache of 2^kIdsLg entries in front of the item's id table: the entry of slot s sits at
+// s mod 2^kIdsLg and holds ((s >> kIdsLg) + 1) << idbits | id in one 32-bit word, so a read
+// racing a write from another wave sees either whole entry, and 0 is empty. An item whose
+// group count does not fit in idbits bits reads the table directly.
+constexpr int kIdsLg = 13;
+template <bool CACHE>
+__device__ __forceinline__ void ids_rows(const BatchItem &I, const ItemRows &R, const uint32_t *slot,
+                                         const uint32_t *id_of_slot, uint32_t *cache, int idbits) {
   const uint32_t *sl = slot + I.aux;
   int64_t *out = static_cast<int64_t *>(I.out);
   const int t = threadIdx.x;
+  const uint32_t idmask = (1u << idbits) - 1;
   for (uint64_t s0 = R.t0; s0 < R.t1; s0 += kGSub) {
     uint32_t s[kBRows], id[kBRows];
 #pragma unroll
@@ -703,16 +707,52 @@ __global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, co
       const uint64_t r = s0 + uint64_t(u) * kBlock + t;
       s[u] = r < R.t1 ? sl[r] : 0;
     }
+    if (CACHE) {
+      uint32_t miss = 0;
 #pragma unroll
-    for (int u = 0; u < kBRows; ++u) {
-      const uint64_t r = s0 + uint64_t(u) * kBlock + t;
-      id[u] = r < R.t1 ? id_of_slot[s[u]] : 0;
+      for (int u = 0; u < kBRows; ++u) {
+        const uint32_t e = cache[s[u] & ((1u << kIdsLg) - 1)];
+        const bool hit = (e >> idbits) == (s[u] >> kIdsLg) + 1;
+        id[u] = e & idmask;
+        const uint64_t r = s0 + uint64_t(u) * kBlock + t;
+        if (!hit && r < R.t1) miss |= 1u << u;
+      }
+#pragma unroll
+      for (int u = 0; u < kBRows; ++u)
+        if (miss >> u & 1) id[u] = id_of_slot[s[u]];
+#pragma unroll
+      for (int u = 0; u < kBRows; ++u)
+        if (miss >> u & 1) cache[s[u] & ((1u << kIdsLg) - 1)] = (((s[u] >> kIdsLg) + 1) << idbits) | id[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < kBRows; ++u) {
+        const uint64_t r = s0 + uint64_t(u) * kBlock + t;
+        id[u] = r < R.t1 ? id_of_slot[s[u]] : 0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kBRows; ++u) {
       const uint64_t r = s0 + uint64_t(u) * kBlock + t;
       if (r < R.t1) out[r] = id[u];
     }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ids_batch(const BatchItem *items, const uint32_t *tile_item,
+                                                      const uint32_t *slot, const uint64_t *groups) {
+  __shared__ uint32_t s_cache[1 << kIdsLg];
+  const uint32_t it = tile_item[blockIdx.x];
+  const BatchItem &I = items[it];
+  const ItemRows R(I, blockIdx.x);
+  const uint32_t *id_of_slot = static_cast<const uint32_t *>(I.b);
+  const int lg = static_cast<int>(I.aux3);
+  const int idbits = 31 - (lg > kIdsLg ? lg - kIdsLg : 0);
+  if (idbits > 0 && groups[it] <= (uint64_t(1) << idbits)) {
+    for (int i = threadIdx.x; i < (1 << kIdsLg); i += kBlock) s_cache[i] = 0;
+    __syncthreads();
+    ids_rows<true>(I, R, slot, id_of_slot, s_cache, idbits);
+  } else {
+    ids_rows<false>(I, R, slot, id_of_slot, nullptr, 31);
   }
 }
 
@@ -798,7 +838,7 @@ void group_id_batch(Ctx &c, int32_t type, std::vector<BatchItem> &items, uint64_
   SelectSpec ss{2, VK_I64, 0, 0, 0};
   select_batch(c, ss, wl, d_groups);
   k_number_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, firsts, d_groups, slot);
-  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, slot);
+  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(p.items, p.tile_item, slot, d_groups);
   check_hip(hipGetLastError(), "group_id batch ids");
   batch_plan_free(c, p);
   tmp_free(c, small);
@@ -1141,7 +1181,7 @@ void plan_group_run(Ctx &c, PlanGroup &g) {
   SelectSpec ss{2, VK_I64, 0, 0, 0};
   select_batch_dev(c, ss, 16, g.sel, static_cast<uint32_t>(ni), g.groups);
   k_number_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.items_final, p.tile_item, g.firsts, g.groups, g.slot);
-  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.items_final, p.tile_item, g.slot);
+  k_ids_batch<<<p.ntiles, kBlock, 0, c.stream>>>(g.items_final, p.tile_item, g.slot, g.groups);
   check_hip(hipGetLastError(), "plan group_id ids");
 }
 
