@@ -104,6 +104,31 @@ def parse_trace(text: str):
     return out
 
 
+def gather_by_table(D, n: int, reps: int = 3) -> dict:
+    """gather_f64 of the same n random indices into tables of 2^19 rows (4 MB: one L2), 2^24
+    rows (128 MB: inside the 256 MB Infinity Cache) and 2^26 rows (512 MB: HBM), in us per
+    launch. Equal times at 2^24 and 2^26 rows mean the bound is the rate of L2 misses (one
+    128-byte line each), not the HBM bandwidth."""
+    from verkehrt_amd import Array, Scalar as S
+    out = {}
+    for lg in (19, 24, 26):
+        rows = 1 << lg
+        tbl = D.call("rand_f64", [S("i64", rows), S("i64", 3)])
+        idx = Array.from_numpy(D.ctx, gather_indices(n, 5, rows))
+        D.call("gather_f64", [tbl, idx])  # warm-up
+        D.ctx.sync()
+        D.ctx.set_trace(True)
+        D.ctx.take_trace()
+        for _ in range(reps):
+            D.call("gather_f64", [tbl, idx])
+        D.ctx.sync()
+        D.ctx.set_trace(False)
+        t = [us for name, us in parse_trace(D.ctx.take_trace()) if name == "gather_f64"]
+        out[f"2^{lg}"] = round(float(np.mean(t)), 1)
+        del tbl, idx
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -917,6 +942,8 @@ def main():
     if dom.startswith("gather"):
         # random 8-byte rows: each costs a 128-byte line, so the bound is the random-row rate
         roofline["random_rows_G_per_s"] = round(n / prims[dom]["us"] / 1e3, 1)
+        if not args.only:
+            roofline["us_by_table_rows"] = gather_by_table(D, n)
     regions = None if args.only else region_suite(D, x1, x2, n)
     zero_mean = None if args.only else zero_mean_suite(D, x1, n, verify=not args.no_verify)
     f32_int = None if args.only else f32_int_suite(D, n, verify=not args.no_verify)
