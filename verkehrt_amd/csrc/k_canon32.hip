@@ -20,8 +20,10 @@
 //    apply goes by its wide table (canon_binade.h, built in parallel for every tile chain whose
 //    summary fails at its approximate start: the tiles where the running value crosses a binade),
 //    or, where that does not apply either, is evaluated from its rows (rounds of run summaries,
-//    or the plain sequential fold when the running value keeps changing binade). Every applied
-//    summary or table proves its result, so the result is the sequential one.
+//    or the plain sequential fold when the running value keeps changing binade; the rows of the
+//    first kC32RowSlots slots are kept chain-major by the table kernel, so such a tile chain is
+//    read with 16-byte loads instead of one 64-byte line per row). Every applied summary or
+//    table proves its result, so the result is the sequential one.
 // HBM traffic: two reads of the rows (4 bytes each, 8 for dot) plus 64 bytes per tile chain.
 #include "canon_binade.h"
 #include "vk_device.h"
@@ -60,6 +62,7 @@ constexpr int kC32Threads = 1024;
 constexpr int kC32Run = 16;                                   // blocks per run
 constexpr uint64_t kC32TileBlocks = kC32Threads / 16 * kC32Run;  // 1024 blocks, 16384 rows
 constexpr uint64_t kC32MinBlocks = 2 * kC32TileBlocks;
+constexpr uint32_t kC32RowSlots = 32768;  // slots whose rows k_c32_wide also keeps, chain-major (128 MB)
 constexpr int kC32Rounds = 3;          // summary rounds of a tile evaluated from its rows
 constexpr uint64_t kC32GroupTiles = 64;  // tiles per group of the walk
 
@@ -234,7 +237,7 @@ template <class RV>
 __global__ __launch_bounds__(kWave) void k_c32_wide(RV rv, const float *__restrict__ x, const float *__restrict__ y,
                                                      uint64_t blocks, const double *pre, const uint32_t *keys,
                                                      const uint32_t *counter, uint32_t rcap, WideHead *heads,
-                                                     WideEnt *ents) {
+                                                     WideEnt *ents, float *rows) {
   __shared__ double s_v[kC32WideSlots][kC32TileBlocks];
   const uint32_t nslots = *counter < rcap ? *counter : rcap;
   const int l = lane_id(), q = l >> 4, r = l & 15;
@@ -248,7 +251,7 @@ __global__ __launch_bounds__(kWave) void k_c32_wide(RV rv, const float *__restri
     const Binade h = binade_of<PrecF32>(a);
     const bool go = live && h.ok;
     const uint64_t b0 = tile * kC32TileBlocks;
-    const int nb = go ? static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks) : 0;
+    const int nb = live ? static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks) : 0;
     {
       double st[kC32WideSlots][kC32TileBlocks / kWave];
 #pragma unroll
@@ -265,6 +268,17 @@ __global__ __launch_bounds__(kWave) void k_c32_wide(RV rv, const float *__restri
       for (int qq = 0; qq < kC32WideSlots; ++qq)
 #pragma unroll
         for (int m = 0; m < static_cast<int>(kC32TileBlocks / kWave); ++m) s_v[qq][l + kWave * m] = st[qq][m];
+      // the first kC32RowSlots slots keep their chain's rows (zero past the tile's end) for the
+      // walk, which evaluates such a tile chain from them when its table does not apply
+#pragma unroll
+      for (int qq = 0; qq < kC32WideSlots; ++qq) {
+        const uint32_t sq = s0 + qq;
+        if (sq < nslots && sq < kC32RowSlots) {
+#pragma unroll
+          for (int m = 0; m < static_cast<int>(kC32TileBlocks / kWave); ++m)
+            rows[uint64_t(sq) * kC32TileBlocks + l + kWave * m] = static_cast<float>(st[qq][m]);
+        }
+      }
     }
     __syncthreads();
     const double base = go ? wide_base<PrecF32>(a, h) : 1.0;
@@ -294,22 +308,36 @@ __global__ __launch_bounds__(kWave) void k_c32_wide(RV rv, const float *__restri
 }
 
 // The exact fold of chain j's rows in blocks b0 .. b0 + nb - 1 (nb <= 1024) from the exact value
-// s, by one wave: lane r holds run r. Few binade changes among the approximate run starts: rounds
-// (runs summarised under the current value's binade and scanned; the runs before the first
-// failing one applied at once, that run folded element by element). Many changes: the
-// sequential fold itself, each element read from its run's lane.
+// s, by one wave: lane r holds run r, read from the rows k_c32_wide kept for the tile chain's slot
+// (kept) or else from x and y. Few binade changes among the approximate run starts: rounds (runs
+// summarised under the current value's binade and scanned; the runs before the first failing one
+// applied at once, that run folded element by element). Many changes: the sequential fold itself,
+// its elements read from LDS (s_f, one tile chain).
 template <class RV>
 __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint64_t b0, int nb, int j,
-                              double s) {
+                              double s, float *s_f, const float *kept) {
   const int l = lane_id();
   double v[kC32Run];
   double asum = 0.0;
+  if (kept) {
+    // the chain's rows as k_c32_wide kept them: 16 consecutive values per lane
 #pragma unroll
-  for (int k = 0; k < kC32Run; ++k) {
-    const int lb = l * kC32Run + k;
-    v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
-    asum = asum + v[k];
+    for (int k = 0; k < kC32Run; k += 4) {
+      const float4 q = *reinterpret_cast<const float4 *>(kept + l * kC32Run + k);
+      v[k] = q.x;
+      v[k + 1] = q.y;
+      v[k + 2] = q.z;
+      v[k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kC32Run; ++k) {
+      const int lb = l * kC32Run + k;
+      v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < kC32Run; ++k) asum = asum + v[k];
   const int nruns = (nb + kC32Run - 1) / kC32Run;
   double inc = asum;
 #pragma unroll
@@ -353,22 +381,26 @@ __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint
     }
   }
   // the sequential fold of runs r0 .. in f32 arithmetic (the same operations: every value is an
-  // f32 value), each element read from its run's lane
-  float vf[kC32Run];
+  // f32 value), the elements staged in LDS in block order so that the chain of dependent adds
+  // reads them four at a time at one address for the whole wave
 #pragma unroll
-  for (int k = 0; k < kC32Run; ++k) vf[k] = static_cast<float>(v[k]);
+  for (int k = 0; k < kC32Run; k += 4)
+    *reinterpret_cast<float4 *>(s_f + l * kC32Run + k) =
+        make_float4(static_cast<float>(v[k]), static_cast<float>(v[k + 1]), static_cast<float>(v[k + 2]),
+                    static_cast<float>(v[k + 3]));
+  __syncthreads();
   float cf = static_cast<float>(cur);
-  for (int r = r0; r < nruns; ++r) {
-    const int nk = nb - r * kC32Run < kC32Run ? nb - r * kC32Run : kC32Run;
-    if (nk == kC32Run) {
-#pragma unroll
-      for (int k = 0; k < kC32Run; ++k) cf = cf + bcast_pod(vf[k], r);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kC32Run; ++k)
-        if (k < nk) cf = cf + bcast_pod(vf[k], r);
-    }
+  int i = r0 * kC32Run;
+#pragma unroll 4
+  for (; i + 4 <= nb; i += 4) {
+    const float4 q = *reinterpret_cast<const float4 *>(s_f + i);
+    cf = cf + q.x;
+    cf = cf + q.y;
+    cf = cf + q.z;
+    cf = cf + q.w;
   }
+  for (; i < nb; ++i) cf = cf + s_f[i];
+  __syncthreads();
   return cf;
 }
 
@@ -422,8 +454,8 @@ __device__ double walk_lanes32(const RunFn &fl, uint64_t hw, int n, double s, On
 // apply takes its wide table or, failing that, its rows.
 template <class RV>
 __device__ double walk_tiles32(const RV &rv, const float *x, const float *y, uint64_t blocks, const TileFn32 *fns,
-                               const WideHead *heads, const WideEnt *ents, WideEnt *s_e, int j, uint64_t t0, int n,
-                               double s) {
+                               const WideHead *heads, const WideEnt *ents, const float *rows, WideEnt *s_e,
+                               float *s_f, int j, uint64_t t0, int n, double s) {
   const int l = lane_id();
   TileFn32 q{};
   if (l < n) q = fns[(t0 + l) * 16 + j];
@@ -445,7 +477,8 @@ __device__ double walk_tiles32(const RV &rv, const float *x, const float *y, uin
     }
     const uint64_t b0 = (t0 + i) * kC32TileBlocks;
     const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
-    return eval_tile32(rv, x, y, b0, nb, j, c);
+    const float *kept = sl && sl - 1 < kC32RowSlots ? rows + (sl - 1) * kC32TileBlocks : nullptr;
+    return eval_tile32(rv, x, y, b0, nb, j, c, s_f, kept);
   });
 }
 
@@ -483,8 +516,9 @@ __global__ __launch_bounds__(kWave) void k_c32_walk(RV rv, const float *__restri
                                                      const float *__restrict__ y, uint64_t blocks,
                                                      const TileFn32 *fns, uint64_t tiles, const TileFn32 *gfn,
                                                      uint64_t groups, const WideHead *heads, const WideEnt *ents,
-                                                     double *acc) {
+                                                     const float *rows, double *acc) {
   __shared__ WideEnt s_e[kWideRes];
+  __shared__ __align__(16) float s_f[kC32TileBlocks];
   const int j = blockIdx.x, l = lane_id();
   double s = acc[j];
   for (uint64_t g0 = 0; g0 < groups; g0 += kWave) {
@@ -496,7 +530,7 @@ __global__ __launch_bounds__(kWave) void k_c32_walk(RV rv, const float *__restri
     s = walk_lanes32(fl, hw, n, s, [&](int i, double c) {
       const uint64_t t0 = (g0 + i) * kC32GroupTiles;
       const int nt = tiles - t0 < kC32GroupTiles ? static_cast<int>(tiles - t0) : static_cast<int>(kC32GroupTiles);
-      return walk_tiles32(rv, x, y, blocks, fns, heads, ents, s_e, j, t0, nt, c);
+      return walk_tiles32(rv, x, y, blocks, fns, heads, ents, rows, s_e, s_f, j, t0, nt, c);
     });
   }
   if (l == 0) acc[j] = s;
@@ -553,6 +587,8 @@ void c32_blocks_as(Ctx &c, RV rv, bool mul, const float *x, const float *y, uint
   auto *keys = static_cast<uint32_t *>(tmp_alloc(c, uint64_t(rcap) * 4 + 8));
   auto *heads = static_cast<WideHead *>(tmp_alloc(c, uint64_t(rcap) * sizeof(WideHead)));
   auto *ents = static_cast<WideEnt *>(tmp_alloc(c, uint64_t(rcap) * kWideRes * sizeof(WideEnt)));
+  const uint64_t row_slots = rcap < kC32RowSlots ? rcap : kC32RowSlots;
+  auto *rows = static_cast<float *>(tmp_alloc(c, row_slots * kC32TileBlocks * 4));
   uint32_t *counter = static_cast<uint32_t *>(tmp_alloc(c, 8));
   fill_u64(c, reinterpret_cast<uint64_t *>(counter), 1, 0);
   const uint32_t grid = static_cast<uint32_t>(tiles);
@@ -561,12 +597,13 @@ void c32_blocks_as(Ctx &c, RV rv, bool mul, const float *x, const float *y, uint
   k_c32_summ<RV><<<grid, kC32Threads, 0, c.stream>>>(rv, x, y, blocks, pre, fns, counter, rcap, keys);
   const uint64_t wide_blocks = (uint64_t(rcap) + kC32WideSlots - 1) / kC32WideSlots;
   k_c32_wide<RV><<<uint32_t(wide_blocks < 4096 ? wide_blocks : 4096), kWave, 0, c.stream>>>(
-      rv, x, y, blocks, pre, keys, counter, rcap, heads, ents);
+      rv, x, y, blocks, pre, keys, counter, rcap, heads, ents, rows);
   const uint64_t groups = (tiles + kC32GroupTiles - 1) / kC32GroupTiles;
   auto *gfn = static_cast<TileFn32 *>(tmp_alloc(c, groups * 16 * sizeof(TileFn32)));
   k_c32_groups<<<uint32_t((16 * groups * kWave + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(fns, tiles, groups,
                                                                                              gfn);
-  k_c32_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, fns, tiles, gfn, groups, heads, ents, acc);
+  k_c32_walk<RV><<<16, kWave, 0, c.stream>>>(rv, x, y, blocks, fns, tiles, gfn, groups, heads, ents, rows,
+                                                    acc);
   check_hip(hipGetLastError(), "canon f32 streamed");
   tmp_free(c, asum);
   tmp_free(c, pre);
@@ -574,6 +611,7 @@ void c32_blocks_as(Ctx &c, RV rv, bool mul, const float *x, const float *y, uint
   tmp_free(c, keys);
   tmp_free(c, heads);
   tmp_free(c, ents);
+  tmp_free(c, rows);
   tmp_free(c, counter);
   tmp_free(c, gfn);
 }
