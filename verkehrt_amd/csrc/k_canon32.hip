@@ -391,13 +391,34 @@ __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint
   __syncthreads();
   float cf = static_cast<float>(cur);
   int i = r0 * kC32Run;
-#pragma unroll 4
-  for (; i + 4 <= nb; i += 4) {
-    const float4 q = *reinterpret_cast<const float4 *>(s_f + i);
-    cf = cf + q.x;
-    cf = cf + q.y;
-    cf = cf + q.z;
-    cf = cf + q.w;
+  const int full = i + (nb - i) / kC32Run * kC32Run;
+  if (i < full) {
+    // whole runs of 16, the next run's four loads in flight while this run is added
+    const float4 *p4 = reinterpret_cast<const float4 *>(s_f);
+    float4 a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = p4[i / 4 + u];
+    for (; i + kC32Run < full; i += kC32Run) {
+      float4 b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) b[u] = p4[(i + kC32Run) / 4 + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cf = cf + a[u].x;
+        cf = cf + a[u].y;
+        cf = cf + a[u].z;
+        cf = cf + a[u].w;
+        a[u] = b[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cf = cf + a[u].x;
+      cf = cf + a[u].y;
+      cf = cf + a[u].z;
+      cf = cf + a[u].w;
+    }
+    i += kC32Run;
   }
   for (; i < nb; ++i) cf = cf + s_f[i];
   __syncthreads();
@@ -451,7 +472,8 @@ __device__ double walk_lanes32(const RunFn &fl, uint64_t hw, int n, double s, On
 }
 
 // The tiles t0 .. t0 + n - 1 of chain j from the exact value s; a tile whose summary does not
-// apply takes its wide table or, failing that, its rows.
+// apply takes its wide table (s_e: the group's tables, kWideRes entries per tile) or, failing
+// that, its rows.
 template <class RV>
 __device__ double walk_tiles32(const RV &rv, const float *x, const float *y, uint64_t blocks, const TileFn32 *fns,
                                const WideHead *heads, const WideEnt *ents, const float *rows, WideEnt *s_e,
@@ -462,24 +484,30 @@ __device__ double walk_tiles32(const RV &rv, const float *x, const float *y, uin
   const RunFn fl{{q.d0, q.d1}, {q.lo0, q.lo1}, {q.hi0, q.hi1}};
   const uint64_t hw = l < n ? q.hword : 0;
   const uint64_t slot = l < n ? q.slot : 0;
-  return walk_lanes32(fl, hw, n, s, [&](int i, double c) {
+  // every table of the group's tiles staged in LDS at once (lane = tile), so that applying one
+  // costs no memory round trip
+  WideHead wh{0.0, 0, 0};
+  if (slot) {
+    wh = heads[slot - 1];
+    WideEnt e[kWideRes];
+#pragma unroll
+    for (int r = 0; r < kWideRes; ++r) e[r] = ents[(slot - 1) * kWideRes + r];
+#pragma unroll
+    for (int r = 0; r < kWideRes; ++r) s_e[l * kWideRes + r] = e[r];
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const double res = walk_lanes32(fl, hw, n, s, [&](int i, double c) {
     const uint64_t sl = bcast_pod(slot, i);
     double out;
-    if (sl) {
-      const WideHead hd = heads[sl - 1];
-      const WideEnt we = ents[(sl - 1) * kWideRes + (l & (kWideRes - 1))];
-      if (l < kWideRes) s_e[l] = we;
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
-      const bool done = wide_apply<PrecF32>(hd, s_e, c, &out);
-      __builtin_amdgcn_wave_barrier();
-      if (done) return out;
-    }
+    if (sl && wide_apply<PrecF32>(bcast_pod(wh, i), s_e + i * kWideRes, c, &out)) return out;
     const uint64_t b0 = (t0 + i) * kC32TileBlocks;
     const int nb = static_cast<int>(blocks - b0 < kC32TileBlocks ? blocks - b0 : kC32TileBlocks);
     const float *kept = sl && sl - 1 < kC32RowSlots ? rows + (sl - 1) * kC32TileBlocks : nullptr;
     return eval_tile32(rv, x, y, b0, nb, j, c, s_f, kept);
   });
+  __builtin_amdgcn_wave_barrier();
+  return res;
 }
 
 // (3c) One wave per (chain j, group of 64 tiles): the group's tiles joined in order into
@@ -517,7 +545,7 @@ __global__ __launch_bounds__(kWave) void k_c32_walk(RV rv, const float *__restri
                                                      const TileFn32 *fns, uint64_t tiles, const TileFn32 *gfn,
                                                      uint64_t groups, const WideHead *heads, const WideEnt *ents,
                                                      const float *rows, double *acc) {
-  __shared__ WideEnt s_e[kWideRes];
+  __shared__ WideEnt s_e[kWave * kWideRes];
   __shared__ __align__(16) float s_f[kC32TileBlocks];
   const int j = blockIdx.x, l = lane_id();
   double s = acc[j];
