@@ -103,6 +103,17 @@ def test_f32_large(D, kind, start):
     assert same_f32(O.canonical_fold_f32("sum", x), D.call("reduce_add_f32", [D.array(x)]).value)
 
 
+def test_f32_normal_2p28(D):
+    # 2^28 normal rows: tens of thousands of tile chains cross binades, more than the 32768 slots
+    # whose rows the table kernel keeps, so evaluated tile chains read both kept rows and x
+    rng = np.random.default_rng(13)
+    x = rng.standard_normal(1 << 28, dtype=np.float32)
+    dx = D.array(x)
+    assert same_f32(O.canonical_fold_f32("sum", x), D.call("reduce_add_f32", [dx]).value)
+    y = x[::-1].copy()
+    assert same_f32(O.canonical_fold_f32("dot", x, y), D.call("dot_f32", [dx, D.array(y)]).value)
+
+
 def test_f32_reducer_tiles(D):
     # CanonicalAcc::feed_blocks (reduce.rs:284-308): updates in multiples of 16 give the
     # one-shot bits, across the sequential (short) and streamed (long) forms
