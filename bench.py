@@ -780,8 +780,10 @@ def serve_suite(requests: int, qps: float):
     arriving at once."""
     from verkehrt_amd import serve as SV
     reqs = SV.make_requests(requests, qps=qps, seed=11)
+    # the warm-ups cover the whole size range, so the paced runs start with the workers' pools
+    # holding blocks of every size class (steady state)
     srv = SV.Server(workers=4, budget=200 * 10**9, key_rows=64 << 20)
-    srv.run(SV.make_requests(50, qps=1e9, seed=99, hi=1 << 20), paced=False)  # warm-up
+    srv.run(SV.make_requests(50, qps=1e9, seed=99), paced=False)  # warm-up
     srv.close()
     paced = srv.run(reqs).summary()
     srv.close()
@@ -790,7 +792,7 @@ def serve_suite(requests: int, qps: float):
     # continuous batching: the same stream with arrived same-program requests coalesced into
     # batched calls (up to 32 per group)
     srv_c = SV.Server(workers=4, budget=200 * 10**9, key_rows=64 << 20, coalesce=32)
-    srv_c.run(SV.make_requests(50, qps=1e9, seed=99, hi=1 << 20), paced=False)  # warm-up
+    srv_c.run(SV.make_requests(50, qps=1e9, seed=99), paced=False)  # warm-up
     srv_c.close()
     paced_c = srv_c.run(reqs).summary()
     srv_c.close()
