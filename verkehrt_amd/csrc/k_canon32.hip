@@ -19,10 +19,9 @@
 //    running value's binade are joined by a lane scan and applied; a tile whose summary does not
 //    apply goes by its wide table (canon_binade.h, built in parallel for every tile chain whose
 //    summary fails at its approximate start: the tiles where the running value crosses a binade),
-//    or, where that does not apply either, is evaluated from its rows (rounds of run summaries,
-//    or the plain sequential fold when the running value keeps changing binade; the rows of the
-//    first kC32RowSlots slots are kept chain-major by the table kernel, so such a tile chain is
-//    read with 16-byte loads instead of one 64-byte line per row). Every applied summary or
+//    or, where that does not apply either, is folded from its rows in f32 arithmetic (the rows of
+//    the first kC32RowSlots slots are kept chain-major by the table kernel, so such a tile chain
+//    is read with 16-byte loads instead of one 64-byte line per row). Every applied summary or
 //    table proves its result, so the result is the sequential one.
 // HBM traffic: two reads of the rows (4 bytes each, 8 for dot) plus 64 bytes per tile chain.
 #include "canon_binade.h"
@@ -63,7 +62,6 @@ constexpr int kC32Run = 16;                                   // blocks per run
 constexpr uint64_t kC32TileBlocks = kC32Threads / 16 * kC32Run;  // 1024 blocks, 16384 rows
 constexpr uint64_t kC32MinBlocks = 2 * kC32TileBlocks;
 constexpr uint32_t kC32RowSlots = 32768;  // slots whose rows k_c32_wide also keeps, chain-major (128 MB)
-constexpr int kC32Rounds = 3;          // summary rounds of a tile evaluated from its rows
 constexpr uint64_t kC32GroupTiles = 64;  // tiles per group of the walk
 
 struct TileFn32 {
@@ -308,96 +306,45 @@ __global__ __launch_bounds__(kWave) void k_c32_wide(RV rv, const float *__restri
 }
 
 // The exact fold of chain j's rows in blocks b0 .. b0 + nb - 1 (nb <= 1024) from the exact value
-// s, by one wave: lane r holds run r, read from the rows k_c32_wide kept for the tile chain's slot
-// (kept) or else from x and y. Few binade changes among the approximate run starts: rounds (runs
-// summarised under the current value's binade and scanned; the runs before the first failing one
-// applied at once, that run folded element by element). Many changes: the sequential fold itself,
-// its elements read from LDS (s_f, one tile chain).
+// s, by one wave: the rows (kept: as k_c32_wide kept them for the tile chain's slot; else read
+// from x and y) staged in LDS in block order, then the sequential fold in f32 arithmetic (the same
+// operations: every value is an f32 value), the next 16 rows' LDS loads in flight while 16 are
+// added. Rounds of run summaries from the exact value (the first failing run folded element by
+// element, the rest summarised again) measured slower than this fold at 12 cycles a row: on
+// normal(0,1) rows at 2^28, 36.5 ms with up to three rounds, 29.0 ms without.
 template <class RV>
 __device__ double eval_tile32(const RV &rv, const float *x, const float *y, uint64_t b0, int nb, int j,
                               double s, float *s_f, const float *kept) {
   const int l = lane_id();
-  double v[kC32Run];
-  double asum = 0.0;
+  float vf[kC32Run];
   if (kept) {
-    // the chain's rows as k_c32_wide kept them: 16 consecutive values per lane
 #pragma unroll
     for (int k = 0; k < kC32Run; k += 4) {
       const float4 q = *reinterpret_cast<const float4 *>(kept + l * kC32Run + k);
-      v[k] = q.x;
-      v[k + 1] = q.y;
-      v[k + 2] = q.z;
-      v[k + 3] = q.w;
+      vf[k] = q.x;
+      vf[k + 1] = q.y;
+      vf[k + 2] = q.z;
+      vf[k + 3] = q.w;
     }
   } else {
 #pragma unroll
     for (int k = 0; k < kC32Run; ++k) {
       const int lb = l * kC32Run + k;
-      v[k] = lb < nb ? rv(x, y, (b0 + lb) * 16 + j) : 0.0;
+      vf[k] = lb < nb ? static_cast<float>(rv(x, y, (b0 + lb) * 16 + j)) : 0.0f;
     }
   }
-#pragma unroll
-  for (int k = 0; k < kC32Run; ++k) asum = asum + v[k];
-  const int nruns = (nb + kC32Run - 1) / kC32Run;
-  double inc = asum;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const double o = __shfl_up(inc, d);
-    if (l >= d) inc = inc + o;
-  }
-  const Binade ha = binade_of<PrecF32>(s + (inc - asum));
-  const Binade hp{__shfl_up(ha.e, 1), __shfl_up(ha.neg, 1), __shfl_up(ha.ok, 1), 0};
-  const uint64_t changes = __ballot(l > 0 && l < nruns && !same_binade(ha, hp));
-  // few changes: rounds (at most kC32Rounds; a running value that keeps failing its run
-  // summaries is close to a binade edge, where the plain fold is cheaper than more rounds)
-  double cur = s;
-  int r0 = 0;
-  if (__popcll(changes) <= 2) {
-    for (int round = 0; round < kC32Rounds && r0 < nruns; ++round) {
-      const Binade h = binade_of<PrecF32>(cur);
-      RunFn f = run_identity();
-      if (h.ok && l >= r0) {
-        const int cnt = nb - l * kC32Run;
-        f = run_of<kC32Run, PrecF32>(v, cnt < 0 ? 0 : cnt, binade_ref<PrecF32>(h));
-      }
-#pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) {
-        const RunFn o = shfl_up_pod(f, d);
-        if (l >= d) f = run_join(o, f);
-      }
-      double val = cur;
-      const bool ok = l < r0 || run_apply<PrecF32>(f, h, cur, &val);
-      const uint64_t fails = __ballot(!ok);
-      if (fails == 0) return __shfl(val, kWave - 1);
-      const int m = __builtin_ctzll(fails);
-      double t = m > r0 ? __shfl(val, m - 1) : cur;
-      if (l == m) {
-#pragma unroll
-        for (int k = 0; k < kC32Run; ++k)
-          if (m * kC32Run + k < nb) t = r32(t + v[k]);
-      }
-      cur = __shfl(t, m);
-      r0 = m + 1;
-    }
-  }
-  // the sequential fold of runs r0 .. in f32 arithmetic (the same operations: every value is an
-  // f32 value), the elements staged in LDS in block order so that the chain of dependent adds
-  // reads them four at a time at one address for the whole wave
 #pragma unroll
   for (int k = 0; k < kC32Run; k += 4)
-    *reinterpret_cast<float4 *>(s_f + l * kC32Run + k) =
-        make_float4(static_cast<float>(v[k]), static_cast<float>(v[k + 1]), static_cast<float>(v[k + 2]),
-                    static_cast<float>(v[k + 3]));
+    *reinterpret_cast<float4 *>(s_f + l * kC32Run + k) = make_float4(vf[k], vf[k + 1], vf[k + 2], vf[k + 3]);
   __syncthreads();
-  float cf = static_cast<float>(cur);
-  int i = r0 * kC32Run;
-  const int full = i + (nb - i) / kC32Run * kC32Run;
+  float cf = static_cast<float>(s);
+  int i = 0;
+  const int full = nb / kC32Run * kC32Run;
   if (i < full) {
-    // whole runs of 16, the next run's four loads in flight while this run is added
     const float4 *p4 = reinterpret_cast<const float4 *>(s_f);
     float4 a[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = p4[i / 4 + u];
+    for (int u = 0; u < 4; ++u) a[u] = p4[u];
     for (; i + kC32Run < full; i += kC32Run) {
       float4 b[4];
 #pragma unroll

@@ -427,70 +427,16 @@ extern "C" int vkm_wide_table_apply(const double *chain, uint64_t n, double appr
 // start of every tile (the start plus the approximate sums of the tiles before) picks the
 // tile's hypothesis at f32 precision; the tile summary joins its runs' summaries in order; the
 // walk applies a tile's summary when its hypothesis is the running value's binade and the check
-// holds, and otherwise evaluates the tile as the kernel's wave does (rounds of run summaries, or
-// the sequential fold when the approximate run starts change binade more than twice), after the
-// tile's wide table when its summary failed at its approximate start. vals are f32 values held
-// in doubles; every fold step rounds to f32. stats: [0] tiles applied, [1] tiles evaluated, [2]
-// evaluations by rounds, [3] runs folded element by element, [4] wide tables applied.
+// holds, and otherwise folds the tile's rows in f32 arithmetic as the kernel's wave does, after
+// the tile's wide table when its summary failed at its approximate start. vals are f32 values
+// held in doubles; every fold step rounds to f32. stats: [0] tiles applied, [1] tiles evaluated,
+// [2] unused (0), [3] runs of 16 rows folded element by element, [4] wide tables applied.
 namespace {
 
 double eval_tile32_model(const double *vals, uint64_t b0, uint64_t b1, int j, double s, uint64_t *st) {
-  const uint64_t nb = b1 - b0;
-  const uint64_t nruns = (nb + kRun - 1) / kRun;
-  auto run32 = [&](uint64_t r, const Binade &h) {
-    double v[kRun];
-    const uint64_t rb = b0 + r * kRun;
-    const int cnt = static_cast<int>(std::min<uint64_t>(kRun, b1 - rb));
-    for (int k = 0; k < static_cast<int>(kRun); ++k) v[k] = k < cnt ? vals[(rb + k) * 16 + j] : 0.0;
-    return run_of<kRun, PrecF32>(v, cnt, binade_ref<PrecF32>(h));
-  };
-  auto fold_run = [&](uint64_t r, double t) {
-    for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b)
-      t = PrecF32::rnd(t + vals[b * 16 + j]);
-    return t;
-  };
-  std::vector<double> asum(64, 0.0);
-  for (uint64_t r = 0; r < nruns; ++r)
-    for (uint64_t b = b0 + r * kRun; b < std::min(b1, b0 + (r + 1) * kRun); ++b) asum[r] += vals[b * 16 + j];
-  std::vector<double> inc(asum);
-  for (int d = 1; d < 64; d <<= 1) {
-    std::vector<double> nxt(inc);
-    for (int l = d; l < 64; ++l) nxt[l] = inc[l] + inc[l - d];
-    inc = nxt;
-  }
-  int changes = 0;
-  Binade prev = binade_of<PrecF32>(s + (inc[0] - asum[0]));
-  for (uint64_t r = 1; r < nruns; ++r) {
-    const Binade h = binade_of<PrecF32>(s + (inc[r] - asum[r]));
-    if (!same_binade(h, prev)) ++changes;
-    prev = h;
-  }
-  double cur = s;
-  uint64_t r0 = 0;
-  if (changes <= 2) {
-    st[2]++;
-    for (int round = 0; round < 3 && r0 < nruns; ++round) {  // kC32Rounds
-      const Binade h = binade_of<PrecF32>(cur);
-      RunFn p = run_identity();
-      uint64_t m = nruns;
-      double last = cur;
-      for (uint64_t r = r0; r < 64; ++r) {
-        if (h.ok) p = run_join(p, r < nruns ? run32(r, h) : run_identity());
-        double val = cur;
-        if (!run_apply<PrecF32>(p, h, cur, &val)) {
-          m = r;
-          break;
-        }
-        last = val;
-      }
-      if (m >= nruns) return last;
-      st[3]++;
-      cur = fold_run(m, m > r0 ? last : cur);
-      r0 = m + 1;
-    }
-  }
-  for (uint64_t b = b0 + r0 * kRun; b < b1; ++b) cur = PrecF32::rnd(cur + vals[b * 16 + j]);
-  return cur;
+  st[3] += (b1 - b0 + kRun - 1) / kRun;
+  for (uint64_t b = b0; b < b1; ++b) s = PrecF32::rnd(s + vals[b * 16 + j]);
+  return s;
 }
 
 // The f32 wide table of chain j over blocks [b0, b1) from `approx`, as k_c32_wide builds it.
